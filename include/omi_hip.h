@@ -1,0 +1,199 @@
+/* omi_hip.h — C ABI of the MI355X contact pipeline (libomi_hip.so).
+ *
+ * Plain C: pointers, sizes and int status codes (0 = ok, < 0 = error, message
+ * in omi_last_error()).  No device types cross this boundary; device memory is
+ * owned by the library.  Host buffers are caller-owned, C-contiguous,
+ * row-major ([n][3] vectors, [n][4] quaternions (x, y, z, w), [n][3][3]
+ * matrices), exactly the numpy column layout of the reference's
+ * PhysicsWorld (world.py:75-82).
+ *
+ * Two families of entry points:
+ *   1. Stage-level calls that replace one reference function on identical
+ *      inputs (synchronous; they copy in, run on the GPU, copy out).  Each
+ *      declaration cites the reference interface it replaces.
+ *   2. The batch: many independent worlds resident in HBM, stepped together
+ *      (one world is the plain single-world form).
+ */
+#ifndef OMI_HIP_H
+#define OMI_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OMI_OK 0
+#define OMI_ERR_ARG (-1)
+#define OMI_ERR_HIP (-2)
+#define OMI_ERR_CAPACITY (-3)
+#define OMI_ERR_STATE (-4)
+
+#define OMI_FP64 64
+#define OMI_FP32 32
+
+/* shape kinds (world._refit_kind codes, world.py:440-441; 3 = convex hull) */
+#define OMI_KIND_NONE 0
+#define OMI_KIND_SPHERE 1
+#define OMI_KIND_BOX 2
+#define OMI_KIND_CONVEX 3
+
+/* Message of the last failed call on this thread. */
+const char *omi_last_error(void);
+/* ABI version (bumped on incompatible change). */
+int32_t omi_abi_version(void);
+/* Number of visible HIP devices (0 on a host without a GPU). */
+int32_t omi_device_count(void);
+
+/* ---- 1. stage-level entry points ------------------------------------- */
+
+/* Replaces PhysicsWorld.refit_aabbs (world.py:471-508) on one world's columns:
+ * box |R|·half ± margin, sphere radius ± margin, no shape a point, convex the
+ * hull's world bounds ± margin.  FP64, bit-exact with the reference's NumPy. */
+int32_t omi_refit(int32_t n, const double *pos, const double *quat, const int32_t *kind,
+                  const double *half, const int32_t *hull_off, const int32_t *hull_cnt,
+                  const double *hull_pts, int64_t n_hull_pts, double margin,
+                  double *aabb_min, double *aabb_max);
+
+/* Replaces BroadPhase.pairs (broadphase.py:243-294): stable sweep on lo.x,
+ * inclusive y/z overlap, >= 1 dynamic body, collision-filter table lookup;
+ * pairs (min, max) in the reference's (rank_i, rank_j) order.  Writes up to
+ * `cap` pairs to pairs[2*cap] and the full count to *count (count > cap means
+ * the output was truncated). */
+int32_t omi_sap_pairs(int32_t n, const double *aabb_min, const double *aabb_max,
+                      const int32_t *motion_type, const int32_t *filt_slot, int32_t n_filt,
+                      const uint8_t *filter_table, int32_t *pairs, int64_t cap, int64_t *count);
+
+/* Replaces _collide_native.box_box (_collide_native.pyx:266-282), batched over
+ * npairs independent pairs.  R* are [npairs][3][3] with the box axes in the
+ * columns.  Outputs: count[npairs] in 0..4, normal[npairs][3] (a -> b),
+ * points[npairs][4][3], depths[npairs][4].  precision is OMI_FP64 or OMI_FP32. */
+int32_t omi_box_box(int32_t npairs, int32_t precision,
+                    const double *ca, const double *Ra, const double *ha,
+                    const double *cb, const double *Rb, const double *hb,
+                    int32_t *count, double *normal, double *points, double *depths);
+
+/* Replaces _collide_native.sphere_sphere (_collide_native.pyx:285-299), batched. */
+int32_t omi_sphere_sphere(int32_t npairs, int32_t precision, const double *ca, const double *ra,
+                          const double *cb, const double *rb, int32_t *count, double *normal,
+                          double *point, double *depth);
+
+/* Replaces _collide_native.sphere_box (_collide_native.pyx:302-344), batched;
+ * the normal points sphere -> box like the reference. */
+int32_t omi_sphere_box(int32_t npairs, int32_t precision, const double *cs, const double *rs,
+                       const double *cb, const double *Rb, const double *hb, int32_t *count,
+                       double *normal, double *point, double *depth);
+
+/* Replaces gjk.collide_convex (gjk.py:249-264) for one batch of pairs.
+ * kind_* is OMI_KIND_SPHERE/BOX/CONVEX; hulls are [*_cnt] points starting at
+ * *_off in hull_pts (local frame); R* [npairs][3][3]; half for boxes, half[0]
+ * the radius for spheres.  count is 0 or 1. */
+int32_t omi_collide_convex(int32_t npairs, int32_t precision,
+                           const int32_t *kind_a, const double *ca, const double *Ra,
+                           const double *ha, const int32_t *off_a, const int32_t *cnt_a,
+                           const int32_t *kind_b, const double *cb, const double *Rb,
+                           const double *hb, const int32_t *off_b, const int32_t *cnt_b,
+                           const double *hull_pts, int64_t n_hull_pts,
+                           int32_t *count, double *normal, double *point, double *depth);
+
+/* Replaces _solver_native.prepare_and_solve (_solver_native.pyx:179-312) with
+ * the same arguments: K contacts over N bodies; pos/lv/av [N][3], nImp [K] and
+ * tImp [K][2] updated in place.  The GPU keeps the reference's Gauss-Seidel
+ * contact order: contacts run concurrently only when no earlier contact in
+ * the sequence touches the same dynamic body, so FP64 results are
+ * bit-identical to the sequential loop. */
+int32_t omi_prepare_and_solve(int64_t K, int32_t N, const int64_t *a_idx, const int64_t *b_idx,
+                              const double *point, const double *normal, const double *depth,
+                              double *pos, double *lv, double *av, const double *inv_mass,
+                              const double *invIw, const double *restitution,
+                              const double *friction, double *nImp, double *tImp,
+                              double restitution_threshold, int32_t iters, int32_t warm_start,
+                              double correction, double slop, int32_t precision);
+
+/* ---- 2. batches of device-resident worlds ------------------------------ */
+
+/* One world as flat host arrays (omi_physics_amd.world.export_world_arrays). */
+typedef struct {
+    int32_t n;
+    const double *position, *orientation, *linear_velocity, *angular_velocity;
+    const double *prev_position, *prev_orientation;
+    const double *inv_mass, *inv_inertia;
+    const double *gravity_factor, *linear_damping, *angular_damping, *quadratic_drag;
+    const double *sleep_timer;
+    const int32_t *motion_type, *collider_shape, *trigger_shape;
+    const uint8_t *awake;
+    const int32_t *kind;
+    const double *half;
+    const int32_t *hull_off, *hull_cnt;
+    const double *hull_pts;
+    int64_t n_hull_pts;
+    const int32_t *mat_slot;
+    int32_t n_mat;
+    const double *friction_table, *restitution_table;
+    const int32_t *filt_slot;
+    int32_t n_filt;
+    const uint8_t *filter_table;
+    double gravity[3];
+    double default_linear_damping, default_angular_damping;
+    int32_t sleep_enabled, velocity_iterations, has_collision;
+} omi_world_desc;
+
+typedef struct {
+    int32_t max_worlds;
+    int32_t precision;          /* OMI_FP64 or OMI_FP32 (narrowphase + solver arithmetic) */
+    int64_t max_bodies;
+    int64_t max_pairs;          /* solid pairs per step, all worlds */
+    int64_t max_contacts;       /* contact points per step, all worlds */
+    int64_t max_hull_pts;
+    int32_t record_broadphase;  /* also keep the unfiltered broadphase pair list */
+    int32_t solver_block;       /* threads per world in the solver launch (64..1024, 0 = auto) */
+} omi_batch_config;
+
+typedef struct omi_batch omi_batch;
+
+int32_t omi_batch_create(const omi_batch_config *cfg, omi_batch **out);
+int32_t omi_batch_destroy(omi_batch *b);
+/* Place (or replace) a world in `slot`; resets its warm-start cache. */
+int32_t omi_batch_load_world(omi_batch *b, int32_t slot, const omi_world_desc *d);
+/* Free `slot`; its bodies stop stepping and their memory is reusable. */
+int32_t omi_batch_remove_world(omi_batch *b, int32_t slot);
+/* Overwrite the mutable state of a loaded world (same body count). */
+int32_t omi_batch_set_state(omi_batch *b, int32_t slot, const double *pos, const double *quat,
+                            const double *lv, const double *av, const uint8_t *awake,
+                            const double *sleep_timer);
+/* Enqueue nsteps fixed steps of dt for every loaded world (asynchronous). */
+int32_t omi_batch_step(omi_batch *b, double dt, int32_t nsteps);
+/* Recompute AABBs only (asynchronous). */
+int32_t omi_batch_refit(omi_batch *b, double margin);
+/* Wait for the batch's queued work; reports capacity overflow of any step. */
+int32_t omi_batch_sync(omi_batch *b);
+/* Copy one world's state out (any pointer may be NULL). */
+int32_t omi_batch_read_world(omi_batch *b, int32_t slot, double *pos, double *quat, double *lv,
+                             double *av, uint8_t *awake, double *sleep_timer, double *aabb_min,
+                             double *aabb_max, double *prev_pos, double *prev_quat);
+/* Contacts of one world's last step in solver order: count first (pass NULLs),
+ * then copy up to `cap` of them. Body indices are world-local. */
+int32_t omi_batch_read_contacts(omi_batch *b, int32_t slot, int64_t cap, int64_t *count,
+                                int32_t *a, int32_t *bb, double *point, double *normal,
+                                double *depth, double *normal_impulse, double *tangent_impulse);
+/* Pairs of one world's last step (solid = 1: the narrowphase input; 0: the
+ * broadphase output, needs record_broadphase). World-local indices. */
+int32_t omi_batch_read_pairs(omi_batch *b, int32_t slot, int32_t solid, int64_t cap,
+                             int64_t *count, int32_t *pairs);
+/* Totals of the last step over all worlds:
+ * [bodies, x_candidates, broadphase pairs, solid pairs, manifolds, contacts,
+ *  max levels per world, overflow flags]. */
+int32_t omi_batch_counters(omi_batch *b, int64_t *out8);
+/* Per-kernel device time of the last timed step, in ms, for `n` kernels in
+ * the order of omi_batch_kernel_names (needs omi_batch_set_timing(b, 1)). */
+int32_t omi_batch_set_timing(omi_batch *b, int32_t on);
+int32_t omi_batch_kernel_times(omi_batch *b, int32_t n, double *ms);
+const char *omi_batch_kernel_names(void);
+/* Capture the step in a HIP graph (1) or launch kernels eagerly (0). */
+int32_t omi_batch_use_graph(omi_batch *b, int32_t on);
+/* Device bytes held by the batch. */
+int64_t omi_batch_device_bytes(const omi_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,933 @@
+// capi.hip — host side of libomi_hip.so: batch memory, world placement, the
+// per-step launch sequence (optionally captured in a HIP graph) and the C ABI
+// declared in include/omi_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/omi_hip.h"
+#include "omi_batch.hpp"
+#include "omi_math.hpp"
+
+namespace omi {
+void launch_begin(const Dev& d, hipStream_t s);
+void launch_pre(const Dev& d, int precision, int64_t nhw, double dt, double margin, hipStream_t s);
+void launch_refit(const Dev& d, int64_t nhw, double margin, hipStream_t s);
+void launch_sap(const Dev& d, const int32_t* wl_small, int32_t n_small, int32_t np_small, const int32_t* wl_big,
+                int32_t n_big, int32_t record, hipStream_t s);
+void launch_narrow(const Dev& d, int precision, hipStream_t s);
+void launch_compact(const Dev& d, int precision, int32_t record, hipStream_t s);
+void launch_wake(const Dev& d, hipStream_t s);
+void launch_cache(const Dev& d, int precision, hipStream_t s);
+void launch_post(const Dev& d, int64_t nhw, double dt, hipStream_t s);
+void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s);
+void launch_toggle(const Dev& d, hipStream_t s);
+void launch_stage_boxbox(int precision, int32_t np, const double* ca, const double* Ra, const double* ha,
+                         const double* cb, const double* Rb, const double* hb, int32_t* cnt, double* nrm,
+                         double* pts, double* dep, hipStream_t s);
+void launch_stage_spheres(int precision, int32_t np, int32_t mode, const double* ca, const double* ra,
+                          const double* cb, const double* Rb, const double* hb, const double* rb, int32_t* cnt,
+                          double* nrm, double* pt, double* dep, hipStream_t s);
+void launch_solve(const Dev& d, int precision, int block, SolveIn in, int32_t lds_bodies, bool stage,
+                  hipStream_t s);
+}  // namespace omi
+
+using namespace omi;
+
+static thread_local std::string g_err;
+
+static int32_t fail(int32_t code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return fail(OMI_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// First-fit allocator over [0, cap) used for body ranges and table pools.
+struct Ranges {
+    int64_t cap = 0;
+    std::vector<std::pair<int64_t, int64_t>> free_;   // (offset, length), sorted
+    void init(int64_t c) { cap = c; free_.clear(); if (c > 0) free_.push_back({0, c}); }
+    int64_t alloc(int64_t n) {
+        if (n == 0) return 0;
+        for (size_t i = 0; i < free_.size(); i++)
+            if (free_[i].second >= n) {
+                int64_t o = free_[i].first;
+                free_[i].first += n; free_[i].second -= n;
+                if (free_[i].second == 0) free_.erase(free_.begin() + i);
+                return o;
+            }
+        return -1;
+    }
+    void release(int64_t o, int64_t n) {
+        if (n == 0) return;
+        free_.push_back({o, n});
+        std::sort(free_.begin(), free_.end());
+        std::vector<std::pair<int64_t, int64_t>> m;
+        for (auto& r : free_) {
+            if (!m.empty() && m.back().first + m.back().second == r.first) m.back().second += r.second;
+            else m.push_back(r);
+        }
+        free_.swap(m);
+    }
+    int64_t high_water() const {
+        // end of the last allocated byte: cap minus a free tail
+        if (!free_.empty() && free_.back().first + free_.back().second == cap) return free_.back().first;
+        return cap;
+    }
+};
+
+struct WorldSlot {
+    bool live = false;
+    int64_t off = 0, n = 0;
+    int64_t mat_off = 0, mat_n = 0, filt_off = 0, filt_n = 0, hull_off = 0, hull_n = 0;
+    int32_t flags = 0;
+};
+
+static const char* kKernelNames =
+    "begin,pre,sap,narrow,compact,wake,solve,cache,post,sleep,toggle";
+static const int kNumKernels = 11;
+
+struct omi_batch {
+    omi_batch_config cfg{};
+    Dev d{};
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    int64_t bytes = 0;
+    Ranges bodies, mats, filts, hulls;
+    std::vector<WorldSlot> slots;
+    int32_t* d_wl_small = nullptr;
+    int32_t* d_wl_big = nullptr;
+    int32_t n_small = 0, n_big = 0, np_small = 0, lds_bodies = 0, np_lds_max = 4096;
+    bool lists_dirty = true;
+    int64_t nhw = 0;
+    bool use_graph = true;
+    hipGraphExec_t graph = nullptr;
+    double graph_dt = 0.0;
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<double> kms;
+    int solver_block = 256;
+
+    template <typename T>
+    int32_t alloc(T** p, int64_t count) {
+        size_t sz = sizeof(T) * (size_t)(count > 0 ? count : 1);
+        void* q = nullptr;
+        HIPCHK(hipMalloc(&q, sz));
+        HIPCHK(hipMemset(q, 0, sz));
+        allocs.push_back(q);
+        bytes += (int64_t)sz;
+        *p = (T*)q;
+        return OMI_OK;
+    }
+};
+
+extern "C" const char* omi_last_error(void) { return g_err.c_str(); }
+extern "C" int32_t omi_abi_version(void) { return 1; }
+extern "C" int32_t omi_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+static int32_t check_device() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return fail(OMI_ERR_HIP, "no HIP device is visible: the contact pipeline needs an MI355X (gfx950)");
+    return OMI_OK;
+}
+
+static void free_all(omi_batch* b) {
+    if (b->graph) hipGraphExecDestroy(b->graph);
+    for (auto e : b->ev) hipEventDestroy(e);
+    for (void* p : b->allocs) hipFree(p);
+    if (b->stream) hipStreamDestroy(b->stream);
+    b->allocs.clear();
+}
+
+extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out) {
+    if (!cfg || !out) return fail(OMI_ERR_ARG, "null argument");
+    if (cfg->precision != OMI_FP64 && cfg->precision != OMI_FP32)
+        return fail(OMI_ERR_ARG, "precision must be OMI_FP64 (64) or OMI_FP32 (32)");
+    if (cfg->max_worlds <= 0 || cfg->max_bodies <= 0 || cfg->max_pairs <= 0 || cfg->max_contacts <= 0)
+        return fail(OMI_ERR_ARG, "capacities must be positive");
+    if (cfg->max_pairs > (1ll << 30) || cfg->max_contacts > (1ll << 29) || cfg->max_bodies > (1ll << 30))
+        return fail(OMI_ERR_ARG, "capacity exceeds the 32-bit index range");
+    int32_t rc = check_device();
+    if (rc) return rc;
+    omi_batch* b = new omi_batch();
+    b->cfg = *cfg;
+    Dev& d = b->d;
+    const int64_t N = cfg->max_bodies, P = cfg->max_pairs, K = cfg->max_contacts;
+    const int32_t W = cfg->max_worlds;
+    int64_t H = 16;
+    while (H < 2 * K) H <<= 1;
+    d.ncap = N; d.pcap = P; d.kcap = K; d.hcap = H; d.wcap = W;
+    const size_t rs = cfg->precision == OMI_FP64 ? 8 : 4;
+    int dev = 0;
+    hipGetDevice(&dev);
+    int smem = 65536;
+    hipDeviceGetAttribute(&smem, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    b->np_lds_max = smem >= 4096 * 16 + 1024 ? 4096 : 2048;
+    b->lds_bodies = 0;
+    b->solver_block = cfg->solver_block > 0 ? cfg->solver_block : 256;
+#define A(p, n) do { if ((rc = b->alloc(&(p), (n))) != OMI_OK) { free_all(b); delete b; return rc; } } while (0)
+    A(d.pos, 3 * N); A(d.quat, 4 * N); A(d.ppos, 3 * N); A(d.pquat, 4 * N);
+    A(d.lv, 6 * N); d.av = d.lv + 3 * N;
+    A(d.lo, 3 * N); A(d.hi, 3 * N); A(d.half, 3 * N); A(d.inv_mass, N); A(d.inv_inertia, 3 * N);
+    A(d.gf, N); A(d.ld, N); A(d.ad, N); A(d.qd, N); A(d.timer, N);
+    unsigned char *rot, *iiw, *svel;
+    A(rot, 9 * N * (int64_t)rs); A(iiw, 9 * N * (int64_t)rs);
+    A(svel, cfg->precision == OMI_FP32 ? 6 * N * 4 : 1);
+    d.rot = rot; d.iiw = iiw; d.svel = svel;
+    A(d.world, N); A(d.mat, N); A(d.filt, N); A(d.hoff, N); A(d.hcnt, N);
+    A(d.mt, N); A(d.kind, N); A(d.bflags, N); A(d.awake, N); A(d.awake0, N); A(d.contacted, N); A(d.notready, N);
+    A(d.parent, N); A(d.last, N);
+    A(d.skey, 2 * N); A(d.sidx, 2 * N); A(d.soff, 2 * N);
+    A(d.hull, 3 * std::max<int64_t>(cfg->max_hull_pts, 1));
+    A(d.w_off, W); A(d.w_cnt, W); A(d.w_flags, W); A(d.w_iters, W);
+    A(d.w_mat_off, W); A(d.w_nmat, W); A(d.w_filt_off, W); A(d.w_nfilt, W);
+    A(d.w_grav, 3 * W); A(d.w_damp, 2 * W);
+    A(d.w_ps, W); A(d.w_pn, W); A(d.w_bps, W); A(d.w_bpn, W); A(d.w_cs, W); A(d.w_cn, W); A(d.w_levels, W);
+    A(d.w_cache_s, 2 * W); A(d.w_cache_n, 2 * W);
+    const int64_t TAB = std::max<int64_t>(64 * (int64_t)W, 1024);
+    A(d.mat_fric, TAB); A(d.mat_rest, TAB); A(d.filt_tab, TAB);
+    A(d.pa, P); A(d.pb, P);
+    A(d.bpa, cfg->record_broadphase ? P : 1); A(d.bpb, cfg->record_broadphase ? P : 1);
+    A(d.mcnt, P); A(d.mpts, 3 * 4 * P);
+    unsigned char *mn, *md, *cn, *cd, *ci, *cs;
+    A(mn, 3 * P * (int64_t)rs); A(md, 4 * P * (int64_t)rs);
+    d.mnrm = mn; d.mdep = md;
+    A(d.ca, K); A(d.cb, K); A(d.cpt, 3 * K);
+    A(cn, 3 * K * (int64_t)rs); A(cd, K * (int64_t)rs); A(ci, 3 * K * (int64_t)rs); A(cs, 19 * K * (int64_t)rs);
+    d.cnrm = cn; d.cdep = cd; d.cimp = ci; d.csolv = cs;
+    A(d.clevel, K); A(d.corder, K); A(d.lcount, K);
+    for (int q = 0; q < 2; q++) {
+        unsigned char* ki;
+        A(d.ka[q], K); A(d.kb[q], K); A(d.kpt[q], 3 * K); A(ki, 3 * K * (int64_t)rs); d.kimp[q] = ki;
+        A(d.hkey[q], H); A(d.hval[q], H);
+        HIPCHK(hipMemset(d.hkey[q], 0xff, sizeof(uint64_t) * H));
+    }
+    A(d.sel, 1); A(d.cnt, C_COUNT);
+    A(b->d_wl_small, W); A(b->d_wl_big, W);
+#undef A
+    // every body slot starts free (world = -1)
+    {
+        std::vector<int32_t> m1((size_t)N, -1);
+        HIPCHK(hipMemcpy(d.world, m1.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    b->bodies.init(N);
+    b->mats.init(TAB);
+    b->filts.init(TAB);
+    b->hulls.init(std::max<int64_t>(cfg->max_hull_pts, 1));
+    b->slots.resize(W);
+    *out = b;
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_destroy(omi_batch* b) {
+    if (!b) return OMI_OK;
+    hipStreamSynchronize(b->stream);
+    free_all(b);
+    delete b;
+    return OMI_OK;
+}
+
+extern "C" int64_t omi_batch_device_bytes(const omi_batch* b) { return b ? b->bytes : 0; }
+
+// host [n][k] rows -> device component-major rows at body offset `off`
+template <typename T>
+static int32_t upload_soa(omi_batch* b, T* dst, int64_t ldd, int64_t off, const T* src, int64_t n, int k) {
+    std::vector<T> tmp((size_t)n);
+    for (int c = 0; c < k; c++) {
+        for (int64_t i = 0; i < n; i++) tmp[(size_t)i] = src[i * k + c];
+        HIPCHK(hipMemcpyAsync(dst + (size_t)c * ldd + off, tmp.data(), sizeof(T) * n, hipMemcpyHostToDevice, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
+    return OMI_OK;
+}
+
+template <typename T>
+static int32_t download_soa(omi_batch* b, T* dst, const T* src, int64_t lds, int64_t off, int64_t n, int k) {
+    std::vector<T> tmp((size_t)n);
+    for (int c = 0; c < k; c++) {
+        HIPCHK(hipMemcpyAsync(tmp.data(), src + (size_t)c * lds + off, sizeof(T) * n, hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        for (int64_t i = 0; i < n; i++) dst[i * k + c] = tmp[(size_t)i];
+    }
+    return OMI_OK;
+}
+
+template <typename T>
+static int32_t put(omi_batch* b, T* dst, const T* src, int64_t n) {
+    if (n <= 0) return OMI_OK;
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return OMI_OK;
+}
+
+template <typename T>
+static int32_t get(omi_batch* b, T* dst, const T* src, int64_t n) {
+    if (n <= 0) return OMI_OK;
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return OMI_OK;
+}
+
+static void invalidate_graph(omi_batch* b) {
+    if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+}
+
+extern "C" int32_t omi_batch_remove_world(omi_batch* b, int32_t slot) {
+    if (!b || slot < 0 || slot >= b->cfg.max_worlds) return fail(OMI_ERR_ARG, "bad world slot");
+    WorldSlot& s = b->slots[slot];
+    if (!s.live) return OMI_OK;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    std::vector<int32_t> m1((size_t)s.n, -1);
+    int32_t rc;
+    if ((rc = put(b, b->d.world + s.off, m1.data(), s.n))) return rc;
+    int32_t zero = 0;
+    if ((rc = put(b, b->d.w_flags + slot, &zero, 1))) return rc;
+    if ((rc = put(b, b->d.w_cache_n + slot, &zero, 1))) return rc;
+    if ((rc = put(b, b->d.w_cache_n + b->cfg.max_worlds + slot, &zero, 1))) return rc;
+    b->bodies.release(s.off, s.n);
+    b->mats.release(s.mat_off, s.mat_n);
+    b->filts.release(s.filt_off, s.filt_n);
+    b->hulls.release(s.hull_off, s.hull_n);
+    s = WorldSlot();
+    b->lists_dirty = true;
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_world_desc* w) {
+    if (!b || !w) return fail(OMI_ERR_ARG, "null argument");
+    if (slot < 0 || slot >= b->cfg.max_worlds) return fail(OMI_ERR_ARG, "world slot out of range");
+    if (w->n < 0) return fail(OMI_ERR_ARG, "negative body count");
+    if (w->n_mat < 1 || w->n_filt < 1) return fail(OMI_ERR_ARG, "material and filter tables need >= 1 slot");
+    const int64_t n = w->n;
+    for (int64_t i = 0; i < n; i++) {
+        int k = w->kind[i], m = w->motion_type[i];
+        if (k < 0 || k > 3 || m < 0 || m > 2) return fail(OMI_ERR_ARG, "bad kind or motion type");
+        if (w->mat_slot[i] < 0 || w->mat_slot[i] >= w->n_mat || w->filt_slot[i] < 0 || w->filt_slot[i] >= w->n_filt)
+            return fail(OMI_ERR_ARG, "material/filter slot out of range");
+        if (k == 3 && (w->hull_off[i] < 0 || w->hull_cnt[i] < 1 || w->hull_off[i] + w->hull_cnt[i] > w->n_hull_pts))
+            return fail(OMI_ERR_ARG, "convex body without valid hull points");
+    }
+    int32_t rc = omi_batch_remove_world(b, slot);
+    if (rc) return rc;
+    WorldSlot s;
+    s.off = b->bodies.alloc(n);
+    if (s.off < 0) return fail(OMI_ERR_CAPACITY, "not enough free body capacity in the batch");
+    s.n = n;
+    s.mat_n = (int64_t)w->n_mat * w->n_mat;
+    s.filt_n = (int64_t)w->n_filt * w->n_filt;
+    s.hull_n = w->n_hull_pts;
+    s.mat_off = b->mats.alloc(s.mat_n);
+    s.filt_off = b->filts.alloc(s.filt_n);
+    s.hull_off = b->hulls.alloc(s.hull_n);
+    if (s.mat_off < 0 || s.filt_off < 0 || s.hull_off < 0) {
+        b->bodies.release(s.off, s.n);
+        if (s.mat_off >= 0) b->mats.release(s.mat_off, s.mat_n);
+        if (s.filt_off >= 0) b->filts.release(s.filt_off, s.filt_n);
+        if (s.hull_off >= 0) b->hulls.release(s.hull_off, s.hull_n);
+        return fail(OMI_ERR_CAPACITY, "table or hull pool of the batch is full");
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    Dev& d = b->d;
+    const int64_t N = d.ncap;
+    if ((rc = upload_soa(b, d.pos, N, s.off, w->position, n, 3))) return rc;
+    if ((rc = upload_soa(b, d.quat, N, s.off, w->orientation, n, 4))) return rc;
+    if ((rc = upload_soa(b, d.ppos, N, s.off, w->prev_position ? w->prev_position : w->position, n, 3))) return rc;
+    if ((rc = upload_soa(b, d.pquat, N, s.off, w->prev_orientation ? w->prev_orientation : w->orientation, n, 4))) return rc;
+    if ((rc = upload_soa(b, d.lv, N, s.off, w->linear_velocity, n, 3))) return rc;
+    if ((rc = upload_soa(b, d.av, N, s.off, w->angular_velocity, n, 3))) return rc;
+    if ((rc = upload_soa(b, d.half, N, s.off, w->half, n, 3))) return rc;
+    if ((rc = upload_soa(b, d.inv_inertia, N, s.off, w->inv_inertia, n, 3))) return rc;
+    if ((rc = put(b, d.inv_mass + s.off, w->inv_mass, n))) return rc;
+    if ((rc = put(b, d.gf + s.off, w->gravity_factor, n))) return rc;
+    if ((rc = put(b, d.ld + s.off, w->linear_damping, n))) return rc;
+    if ((rc = put(b, d.ad + s.off, w->angular_damping, n))) return rc;
+    if ((rc = put(b, d.qd + s.off, w->quadratic_drag, n))) return rc;
+    if ((rc = put(b, d.timer + s.off, w->sleep_timer, n))) return rc;
+    if ((rc = put(b, d.awake + s.off, w->awake, n))) return rc;
+    std::vector<uint8_t> mt((size_t)n), kind((size_t)n), bf((size_t)n);
+    std::vector<int32_t> wid((size_t)n, slot), hoff((size_t)n), hcnt((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        mt[i] = (uint8_t)w->motion_type[i];
+        kind[i] = (uint8_t)w->kind[i];
+        bf[i] = (uint8_t)((w->collider_shape[i] >= 0 ? BF_COLLIDER : 0) | (w->trigger_shape[i] >= 0 ? BF_TRIGGER : 0));
+        hoff[i] = w->kind[i] == 3 ? (int32_t)(s.hull_off + w->hull_off[i]) : 0;
+        hcnt[i] = w->kind[i] == 3 ? w->hull_cnt[i] : 0;
+    }
+    if ((rc = put(b, d.mt + s.off, mt.data(), n))) return rc;
+    if ((rc = put(b, d.kind + s.off, kind.data(), n))) return rc;
+    if ((rc = put(b, d.bflags + s.off, bf.data(), n))) return rc;
+    if ((rc = put(b, d.mat + s.off, w->mat_slot, n))) return rc;
+    if ((rc = put(b, d.filt + s.off, w->filt_slot, n))) return rc;
+    if ((rc = put(b, d.hoff + s.off, hoff.data(), n))) return rc;
+    if ((rc = put(b, d.hcnt + s.off, hcnt.data(), n))) return rc;
+    if ((rc = put(b, d.mat_fric + s.mat_off, w->friction_table, s.mat_n))) return rc;
+    if ((rc = put(b, d.mat_rest + s.mat_off, w->restitution_table, s.mat_n))) return rc;
+    if ((rc = put(b, d.filt_tab + s.filt_off, w->filter_table, s.filt_n))) return rc;
+    if (s.hull_n && (rc = put(b, d.hull + 3 * s.hull_off, w->hull_pts, 3 * s.hull_n))) return rc;
+    const int32_t W = b->cfg.max_worlds;
+    int32_t i32;
+    s.flags = WF_ACTIVE | (w->has_collision ? WF_COLLISION : 0) | (w->sleep_enabled ? WF_SLEEP : 0);
+    i32 = (int32_t)s.off; if ((rc = put(b, d.w_off + slot, &i32, 1))) return rc;
+    i32 = (int32_t)n; if ((rc = put(b, d.w_cnt + slot, &i32, 1))) return rc;
+    i32 = w->velocity_iterations; if ((rc = put(b, d.w_iters + slot, &i32, 1))) return rc;
+    i32 = (int32_t)s.mat_off; if ((rc = put(b, d.w_mat_off + slot, &i32, 1))) return rc;
+    i32 = w->n_mat; if ((rc = put(b, d.w_nmat + slot, &i32, 1))) return rc;
+    i32 = (int32_t)s.filt_off; if ((rc = put(b, d.w_filt_off + slot, &i32, 1))) return rc;
+    i32 = w->n_filt; if ((rc = put(b, d.w_nfilt + slot, &i32, 1))) return rc;
+    i32 = 0;
+    if ((rc = put(b, d.w_cache_n + slot, &i32, 1))) return rc;
+    if ((rc = put(b, d.w_cache_n + W + slot, &i32, 1))) return rc;
+    if ((rc = put(b, d.w_pn + slot, &i32, 1))) return rc;
+    if ((rc = put(b, d.w_bpn + slot, &i32, 1))) return rc;
+    if ((rc = put(b, d.w_cn + slot, &i32, 1))) return rc;
+    for (int r = 0; r < 3; r++)
+        if ((rc = put(b, d.w_grav + (size_t)r * W + slot, &w->gravity[r], 1))) return rc;
+    if ((rc = put(b, d.w_damp + slot, &w->default_linear_damping, 1))) return rc;
+    if ((rc = put(b, d.w_damp + W + slot, &w->default_angular_damping, 1))) return rc;
+    if ((rc = put(b, d.w_flags + slot, &s.flags, 1))) return rc;
+    if ((rc = put(b, d.world + s.off, wid.data(), n))) return rc;
+    s.live = true;
+    b->slots[slot] = s;
+    b->lists_dirty = true;
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_set_state(omi_batch* b, int32_t slot, const double* pos, const double* quat,
+                                       const double* lv, const double* av, const uint8_t* awake,
+                                       const double* timer) {
+    if (!b || slot < 0 || slot >= b->cfg.max_worlds || !b->slots[slot].live) return fail(OMI_ERR_ARG, "bad world slot");
+    const WorldSlot& s = b->slots[slot];
+    Dev& d = b->d;
+    int32_t rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (pos && (rc = upload_soa(b, d.pos, d.ncap, s.off, pos, s.n, 3))) return rc;
+    if (quat && (rc = upload_soa(b, d.quat, d.ncap, s.off, quat, s.n, 4))) return rc;
+    if (lv && (rc = upload_soa(b, d.lv, d.ncap, s.off, lv, s.n, 3))) return rc;
+    if (av && (rc = upload_soa(b, d.av, d.ncap, s.off, av, s.n, 3))) return rc;
+    if (awake && (rc = put(b, d.awake + s.off, awake, s.n))) return rc;
+    if (timer && (rc = put(b, d.timer + s.off, timer, s.n))) return rc;
+    return OMI_OK;
+}
+
+static int32_t rebuild_lists(omi_batch* b) {
+    std::vector<int32_t> small, big;
+    int32_t np_small = 2, maxn = 0;
+    for (int32_t w = 0; w < b->cfg.max_worlds; w++) {
+        const WorldSlot& s = b->slots[w];
+        if (!s.live) continue;
+        int32_t NP = 1;
+        while (NP < s.n) NP <<= 1;
+        if (NP <= b->np_lds_max) { small.push_back(w); np_small = std::max(np_small, NP); }
+        else big.push_back(w);
+        maxn = std::max<int32_t>(maxn, (int32_t)s.n);
+    }
+    int32_t rc;
+    if ((rc = put(b, b->d_wl_small, small.data(), (int64_t)small.size()))) return rc;
+    if ((rc = put(b, b->d_wl_big, big.data(), (int64_t)big.size()))) return rc;
+    b->n_small = (int32_t)small.size();
+    b->n_big = (int32_t)big.size();
+    b->np_small = np_small;
+    b->lds_bodies = std::min<int32_t>(maxn, 12288);
+    b->nhw = b->bodies.high_water();
+    b->lists_dirty = false;
+    invalidate_graph(b);
+    return OMI_OK;
+}
+
+static void enqueue_step(omi_batch* b, double dt) {
+    const Dev& d = b->d;
+    hipStream_t s = b->stream;
+    const int prec = b->cfg.precision;
+    const bool tm = b->timing && !b->ev.empty();
+    int e = 0;
+    auto mark = [&]() { if (tm) hipEventRecord(b->ev[e++], s); };
+    mark();
+    launch_begin(d, s); mark();
+    launch_pre(d, prec, b->nhw, dt, 0.05, s); mark();
+    launch_sap(d, b->d_wl_small, b->n_small, b->np_small, b->d_wl_big, b->n_big, b->cfg.record_broadphase, s); mark();
+    launch_narrow(d, prec, s); mark();
+    launch_compact(d, prec, b->cfg.record_broadphase, s); mark();
+    launch_wake(d, s); mark();
+    SolveIn in{nullptr, nullptr, 0.5, 0.8, 0.005, 1};
+    launch_solve(d, prec, b->solver_block, in, b->lds_bodies, false, s); mark();
+    launch_cache(d, prec, s); mark();
+    launch_post(d, b->nhw, dt, s); mark();
+    launch_sleep(d, b->nhw, dt, s); mark();
+    launch_toggle(d, s); mark();
+}
+
+extern "C" int32_t omi_batch_set_timing(omi_batch* b, int32_t on) {
+    if (!b) return fail(OMI_ERR_ARG, "null batch");
+    b->timing = on != 0;
+    if (b->timing && b->ev.empty()) {
+        b->ev.resize(kNumKernels + 1);
+        for (auto& e : b->ev) HIPCHK(hipEventCreate(&e));
+    }
+    return OMI_OK;
+}
+
+extern "C" const char* omi_batch_kernel_names(void) { return kKernelNames; }
+
+extern "C" int32_t omi_batch_kernel_times(omi_batch* b, int32_t n, double* ms) {
+    if (!b || !ms) return fail(OMI_ERR_ARG, "null argument");
+    if (b->ev.empty()) return fail(OMI_ERR_STATE, "timing is off");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (int i = 0; i < n && i < kNumKernels; i++) {
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, b->ev[i], b->ev[i + 1]));
+        ms[i] = t;
+    }
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_use_graph(omi_batch* b, int32_t on) {
+    if (!b) return fail(OMI_ERR_ARG, "null batch");
+    b->use_graph = on != 0;
+    if (!b->use_graph) invalidate_graph(b);
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_step(omi_batch* b, double dt, int32_t nsteps) {
+    if (!b) return fail(OMI_ERR_ARG, "null batch");
+    if (!(dt > 0.0)) return fail(OMI_ERR_ARG, "dt must be positive");
+    int32_t rc;
+    if (b->lists_dirty && (rc = rebuild_lists(b))) return rc;
+    for (int32_t k = 0; k < nsteps; k++) {
+        if (b->use_graph && !b->timing) {
+            if (!b->graph || b->graph_dt != dt) {
+                invalidate_graph(b);
+                hipGraph_t g;
+                HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                enqueue_step(b, dt);
+                HIPCHK(hipStreamEndCapture(b->stream, &g));
+                HIPCHK(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+                hipGraphDestroy(g);
+                b->graph_dt = dt;
+            }
+            HIPCHK(hipGraphLaunch(b->graph, b->stream));
+        } else {
+            enqueue_step(b, dt);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_refit(omi_batch* b, double margin) {
+    if (!b) return fail(OMI_ERR_ARG, "null batch");
+    int32_t rc;
+    if (b->lists_dirty && (rc = rebuild_lists(b))) return rc;
+    launch_refit(b->d, b->nhw, margin, b->stream);
+    HIPCHK(hipGetLastError());
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_sync(omi_batch* b) {
+    if (!b) return fail(OMI_ERR_ARG, "null batch");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    uint64_t ov = 0;
+    HIPCHK(hipMemcpy(&ov, b->d.cnt + C_OVERFLOW, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (ov) {
+        char msg[256];
+        snprintf(msg, sizeof msg,
+                 "batch capacity exceeded during the last step (flags 0x%llx: 1 pairs, 2 contacts, 4 cache, "
+                 "8 broadphase record); results of that step are invalid — recreate the batch with larger "
+                 "max_pairs/max_contacts",
+                 (unsigned long long)ov);
+        return fail(OMI_ERR_CAPACITY, msg);
+    }
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_counters(omi_batch* b, int64_t* out8) {
+    if (!b || !out8) return fail(OMI_ERR_ARG, "null argument");
+    uint64_t c[C_COUNT];
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(c, b->d.cnt, sizeof c, hipMemcpyDeviceToHost));
+    int64_t bodies = 0;
+    for (auto& s : b->slots) if (s.live) bodies += s.n;
+    out8[0] = bodies; out8[1] = (int64_t)c[C_XCAND]; out8[2] = (int64_t)c[C_BPTOTAL]; out8[3] = (int64_t)c[C_PAIRS];
+    out8[4] = (int64_t)c[C_MANIFOLDS]; out8[5] = (int64_t)c[C_CONTACTS]; out8[6] = (int64_t)c[C_MAXLEVEL];
+    out8[7] = (int64_t)c[C_OVERFLOW];
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_read_world(omi_batch* b, int32_t slot, double* pos, double* quat, double* lv,
+                                        double* av, uint8_t* awake, double* timer, double* lo, double* hi,
+                                        double* ppos, double* pquat) {
+    if (!b || slot < 0 || slot >= b->cfg.max_worlds || !b->slots[slot].live) return fail(OMI_ERR_ARG, "bad world slot");
+    const WorldSlot& s = b->slots[slot];
+    const Dev& d = b->d;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    int32_t rc;
+    if (pos && (rc = download_soa(b, pos, d.pos, d.ncap, s.off, s.n, 3))) return rc;
+    if (quat && (rc = download_soa(b, quat, d.quat, d.ncap, s.off, s.n, 4))) return rc;
+    if (lv && (rc = download_soa(b, lv, d.lv, d.ncap, s.off, s.n, 3))) return rc;
+    if (av && (rc = download_soa(b, av, d.av, d.ncap, s.off, s.n, 3))) return rc;
+    if (lo && (rc = download_soa(b, lo, d.lo, d.ncap, s.off, s.n, 3))) return rc;
+    if (hi && (rc = download_soa(b, hi, d.hi, d.ncap, s.off, s.n, 3))) return rc;
+    if (ppos && (rc = download_soa(b, ppos, d.ppos, d.ncap, s.off, s.n, 3))) return rc;
+    if (pquat && (rc = download_soa(b, pquat, d.pquat, d.ncap, s.off, s.n, 4))) return rc;
+    if (awake && (rc = get(b, awake, d.awake + s.off, s.n))) return rc;
+    if (timer && (rc = get(b, timer, d.timer + s.off, s.n))) return rc;
+    return OMI_OK;
+}
+
+template <typename R>
+static void to_double(const std::vector<unsigned char>& raw, double* out, int64_t n) {
+    const R* p = (const R*)raw.data();
+    for (int64_t i = 0; i < n; i++) out[i] = (double)p[i];
+}
+
+extern "C" int32_t omi_batch_read_contacts(omi_batch* b, int32_t slot, int64_t cap, int64_t* count, int32_t* a,
+                                           int32_t* bb, double* point, double* normal, double* depth,
+                                           double* nimp, double* timp) {
+    if (!b || slot < 0 || slot >= b->cfg.max_worlds || !b->slots[slot].live) return fail(OMI_ERR_ARG, "bad world slot");
+    const WorldSlot& s = b->slots[slot];
+    const Dev& d = b->d;
+    int32_t rc, cs = 0, cn = 0;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if ((rc = get(b, &cs, d.w_cs + slot, 1))) return rc;
+    if ((rc = get(b, &cn, d.w_cn + slot, 1))) return rc;
+    if (count) *count = cn;
+    const int64_t m = std::min<int64_t>(cn, cap);
+    if (m <= 0) return OMI_OK;
+    const size_t rs = b->cfg.precision == OMI_FP64 ? 8 : 4;
+    std::vector<int32_t> ia((size_t)m), ib((size_t)m);
+    if ((rc = get(b, ia.data(), d.ca + cs, m))) return rc;
+    if ((rc = get(b, ib.data(), d.cb + cs, m))) return rc;
+    for (int64_t k = 0; k < m; k++) {
+        if (a) a[k] = (int32_t)(ia[k] - s.off);
+        if (bb) bb[k] = (int32_t)(ib[k] - s.off);
+    }
+    std::vector<double> tmp((size_t)m);
+    std::vector<unsigned char> raw((size_t)m * rs);
+    for (int c = 0; c < 3; c++) {
+        if (point) {
+            if ((rc = get(b, tmp.data(), d.cpt + (size_t)c * d.kcap + cs, m))) return rc;
+            for (int64_t k = 0; k < m; k++) point[3 * k + c] = tmp[k];
+        }
+        if (normal) {
+            HIPCHK(hipMemcpy(raw.data(), (const unsigned char*)d.cnrm + ((size_t)c * d.kcap + cs) * rs, m * rs, hipMemcpyDeviceToHost));
+            if (rs == 8) to_double<double>(raw, tmp.data(), m); else to_double<float>(raw, tmp.data(), m);
+            for (int64_t k = 0; k < m; k++) normal[3 * k + c] = tmp[k];
+        }
+        if (nimp || timp) {
+            HIPCHK(hipMemcpy(raw.data(), (const unsigned char*)d.cimp + ((size_t)c * d.kcap + cs) * rs, m * rs, hipMemcpyDeviceToHost));
+            if (rs == 8) to_double<double>(raw, tmp.data(), m); else to_double<float>(raw, tmp.data(), m);
+            for (int64_t k = 0; k < m; k++) {
+                if (c == 0 && nimp) nimp[k] = tmp[k];
+                if (c > 0 && timp) timp[2 * k + c - 1] = tmp[k];
+            }
+        }
+    }
+    if (depth) {
+        HIPCHK(hipMemcpy(raw.data(), (const unsigned char*)d.cdep + (size_t)cs * rs, m * rs, hipMemcpyDeviceToHost));
+        if (rs == 8) to_double<double>(raw, depth, m); else to_double<float>(raw, depth, m);
+    }
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_read_pairs(omi_batch* b, int32_t slot, int32_t solid, int64_t cap, int64_t* count,
+                                        int32_t* pairs) {
+    if (!b || slot < 0 || slot >= b->cfg.max_worlds || !b->slots[slot].live) return fail(OMI_ERR_ARG, "bad world slot");
+    if (!solid && !b->cfg.record_broadphase) return fail(OMI_ERR_STATE, "broadphase pairs are recorded only with record_broadphase");
+    const WorldSlot& s = b->slots[slot];
+    const Dev& d = b->d;
+    int32_t ps = 0, pn = 0, rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if ((rc = get(b, &ps, (solid ? d.w_ps : d.w_bps) + slot, 1))) return rc;
+    if ((rc = get(b, &pn, (solid ? d.w_pn : d.w_bpn) + slot, 1))) return rc;
+    if (count) *count = pn;
+    const int64_t m = std::min<int64_t>(pn, cap);
+    if (m <= 0 || !pairs) return OMI_OK;
+    std::vector<int32_t> pa((size_t)m), pb((size_t)m);
+    if ((rc = get(b, pa.data(), (solid ? d.pa : d.bpa) + ps, m))) return rc;
+    if ((rc = get(b, pb.data(), (solid ? d.pb : d.bpb) + ps, m))) return rc;
+    for (int64_t k = 0; k < m; k++) { pairs[2 * k] = (int32_t)(pa[k] - s.off); pairs[2 * k + 1] = (int32_t)(pb[k] - s.off); }
+    return OMI_OK;
+}
+
+// ---------------------------------------------------------------- stages --
+
+struct DevBuf {
+    std::vector<void*> p;
+    ~DevBuf() { for (void* q : p) hipFree(q); }
+    template <typename T>
+    T* up(const T* src, int64_t n) {
+        void* q = nullptr;
+        if (hipMalloc(&q, sizeof(T) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess) return nullptr;
+        p.push_back(q);
+        if (src && n > 0) hipMemcpy(q, src, sizeof(T) * n, hipMemcpyHostToDevice);
+        return (T*)q;
+    }
+};
+
+extern "C" int32_t omi_box_box(int32_t np, int32_t precision, const double* ca, const double* Ra, const double* ha,
+                               const double* cb, const double* Rb, const double* hb, int32_t* count, double* normal,
+                               double* points, double* depths) {
+    if (np < 0 || (precision != 64 && precision != 32)) return fail(OMI_ERR_ARG, "bad arguments");
+    int32_t rc = check_device();
+    if (rc) return rc;
+    if (np == 0) return OMI_OK;
+    DevBuf m;
+    double *dca = m.up(ca, 3 * np), *dRa = m.up(Ra, 9 * np), *dha = m.up(ha, 3 * np);
+    double *dcb = m.up(cb, 3 * np), *dRb = m.up(Rb, 9 * np), *dhb = m.up(hb, 3 * np);
+    int32_t* dc = m.up<int32_t>(nullptr, np);
+    double *dn = m.up<double>(nullptr, 3 * np), *dp = m.up<double>(nullptr, 12 * np), *dd = m.up<double>(nullptr, 4 * np);
+    if (!dd) return fail(OMI_ERR_HIP, "device allocation failed");
+    launch_stage_boxbox(precision, np, dca, dRa, dha, dcb, dRb, dhb, dc, dn, dp, dd, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(count, dc, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(normal, dn, sizeof(double) * 3 * np, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(points, dp, sizeof(double) * 12 * np, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(depths, dd, sizeof(double) * 4 * np, hipMemcpyDeviceToHost));
+    return OMI_OK;
+}
+
+static int32_t stage_spheres(int32_t np, int32_t precision, int32_t mode, const double* ca, const double* ra,
+                             const double* cb, const double* Rb, const double* hb, const double* rb, int32_t* count,
+                             double* normal, double* point, double* depth) {
+    if (np < 0 || (precision != 64 && precision != 32)) return fail(OMI_ERR_ARG, "bad arguments");
+    int32_t rc = check_device();
+    if (rc) return rc;
+    if (np == 0) return OMI_OK;
+    DevBuf m;
+    double *dca = m.up(ca, 3 * np), *dra = m.up(ra, np), *dcb = m.up(cb, 3 * np);
+    double *dRb = mode ? m.up(Rb, 9 * np) : nullptr, *dhb = mode ? m.up(hb, 3 * np) : nullptr;
+    double* drb = mode ? nullptr : m.up(rb, np);
+    int32_t* dc = m.up<int32_t>(nullptr, np);
+    double *dn = m.up<double>(nullptr, 3 * np), *dp = m.up<double>(nullptr, 3 * np), *dd = m.up<double>(nullptr, np);
+    if (!dd) return fail(OMI_ERR_HIP, "device allocation failed");
+    launch_stage_spheres(precision, np, mode, dca, dra, dcb, dRb, dhb, drb, dc, dn, dp, dd, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(count, dc, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(normal, dn, sizeof(double) * 3 * np, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(point, dp, sizeof(double) * 3 * np, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(depth, dd, sizeof(double) * np, hipMemcpyDeviceToHost));
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_sphere_sphere(int32_t np, int32_t precision, const double* ca, const double* ra,
+                                     const double* cb, const double* rb, int32_t* count, double* normal,
+                                     double* point, double* depth) {
+    return stage_spheres(np, precision, 0, ca, ra, cb, nullptr, nullptr, rb, count, normal, point, depth);
+}
+
+extern "C" int32_t omi_sphere_box(int32_t np, int32_t precision, const double* cs, const double* rs,
+                                  const double* cb, const double* Rb, const double* hb, int32_t* count,
+                                  double* normal, double* point, double* depth) {
+    return stage_spheres(np, precision, 1, cs, rs, cb, Rb, hb, nullptr, count, normal, point, depth);
+}
+
+// A one-world batch whose arrays the stage entry points fill directly.
+static int32_t mini_batch(int64_t nb, int64_t npairs, int64_t ncon, int32_t precision, int32_t record,
+                          omi_batch** out) {
+    omi_batch_config c{};
+    c.max_worlds = 1;
+    c.precision = precision;
+    c.max_bodies = std::max<int64_t>(nb, 1);
+    c.max_pairs = std::max<int64_t>(npairs, 1);
+    c.max_contacts = std::max<int64_t>(ncon, 1);
+    c.max_hull_pts = 1;
+    c.record_broadphase = record;
+    c.solver_block = 256;
+    return omi_batch_create(&c, out);
+}
+
+extern "C" int32_t omi_refit(int32_t n, const double* pos, const double* quat, const int32_t* kind,
+                             const double* half, const int32_t* hull_off, const int32_t* hull_cnt,
+                             const double* hull_pts, int64_t n_hull_pts, double margin, double* lo, double* hi) {
+    if (n < 0) return fail(OMI_ERR_ARG, "negative body count");
+    if (n == 0) return OMI_OK;
+    omi_batch_config c{};
+    c.max_worlds = 1; c.precision = 64; c.max_bodies = n; c.max_pairs = 1; c.max_contacts = 1;
+    c.max_hull_pts = std::max<int64_t>(n_hull_pts, 1);
+    omi_batch* b = nullptr;
+    int32_t rc = omi_batch_create(&c, &b);
+    if (rc) return rc;
+    Dev& d = b->d;
+    std::vector<uint8_t> k((size_t)n);
+    std::vector<int32_t> ho((size_t)n, 0), hc((size_t)n, 0), wz((size_t)n, 0);
+    for (int32_t i = 0; i < n; i++) {
+        k[i] = (uint8_t)kind[i];
+        if (kind[i] == 3) {
+            if (!hull_off || !hull_cnt || hull_off[i] < 0 || hull_cnt[i] < 1 || hull_off[i] + hull_cnt[i] > n_hull_pts) {
+                omi_batch_destroy(b);
+                return fail(OMI_ERR_ARG, "convex body without valid hull points");
+            }
+            ho[i] = hull_off[i]; hc[i] = hull_cnt[i];
+        }
+    }
+    if (!(rc = upload_soa(b, d.pos, d.ncap, 0, pos, n, 3)) && !(rc = upload_soa(b, d.quat, d.ncap, 0, quat, n, 4)) &&
+        !(rc = upload_soa(b, d.half, d.ncap, 0, half, n, 3)) && !(rc = put(b, d.kind, k.data(), n)) &&
+        !(rc = put(b, d.hoff, ho.data(), n)) && !(rc = put(b, d.hcnt, hc.data(), n)) &&
+        !(rc = put(b, d.world, wz.data(), n)) &&
+        !(rc = (n_hull_pts > 0 ? put(b, d.hull, hull_pts, 3 * n_hull_pts) : OMI_OK))) {
+        launch_refit(d, n, margin, b->stream);
+        if (hipGetLastError() != hipSuccess) rc = fail(OMI_ERR_HIP, "refit launch failed");
+        if (!rc) rc = download_soa(b, lo, d.lo, d.ncap, 0, n, 3);
+        if (!rc) rc = download_soa(b, hi, d.hi, d.ncap, 0, n, 3);
+    }
+    omi_batch_destroy(b);
+    return rc;
+}
+
+extern "C" int32_t omi_sap_pairs(int32_t n, const double* lo, const double* hi, const int32_t* motion_type,
+                                 const int32_t* filt_slot, int32_t n_filt, const uint8_t* filter_table,
+                                 int32_t* pairs, int64_t cap, int64_t* count) {
+    if (n < 0 || n_filt < 1 || !count) return fail(OMI_ERR_ARG, "bad arguments");
+    *count = 0;
+    if (n < 2) return OMI_OK;
+    int64_t pcap = std::max<int64_t>(std::min<int64_t>((int64_t)n * 16, 1 << 26), 1024);
+    for (;;) {
+        omi_batch* b = nullptr;
+        int32_t rc = mini_batch(n, pcap, 1, 64, 1, &b);
+        if (rc) return rc;
+        Dev& d = b->d;
+        std::vector<uint8_t> mt((size_t)n), bf((size_t)n, BF_COLLIDER), aw((size_t)n, 1);
+        std::vector<int32_t> wz((size_t)n, 0);
+        for (int32_t i = 0; i < n; i++) mt[i] = (uint8_t)motion_type[i];
+        int32_t flags = WF_ACTIVE | WF_COLLISION, zero = 0, nn = n;
+        int32_t nf = n_filt;
+        int32_t wl = 0;
+        rc = upload_soa(b, d.lo, d.ncap, 0, lo, n, 3);
+        if (!rc) rc = upload_soa(b, d.hi, d.ncap, 0, hi, n, 3);
+        if (!rc) rc = put(b, d.mt, mt.data(), n);
+        if (!rc) rc = put(b, d.bflags, bf.data(), n);
+        if (!rc) rc = put(b, d.awake, aw.data(), n);
+        if (!rc) rc = put(b, d.world, wz.data(), n);
+        if (!rc) rc = put(b, d.filt, filt_slot, n);
+        if (!rc) rc = put(b, d.filt_tab, filter_table, (int64_t)n_filt * n_filt);
+        if (!rc) rc = put(b, d.w_flags, &flags, 1);
+        if (!rc) rc = put(b, d.w_off, &zero, 1);
+        if (!rc) rc = put(b, d.w_cnt, &nn, 1);
+        if (!rc) rc = put(b, d.w_filt_off, &zero, 1);
+        if (!rc) rc = put(b, d.w_nfilt, &nf, 1);
+        if (!rc) rc = put(b, b->d_wl_small, &wl, 1);
+        if (!rc) rc = put(b, b->d_wl_big, &wl, 1);
+        if (rc) { omi_batch_destroy(b); return rc; }
+        int32_t NP = 1;
+        while (NP < n) NP <<= 1;
+        launch_begin(d, b->stream);
+        if (NP <= b->np_lds_max) launch_sap(d, b->d_wl_small, 1, NP, b->d_wl_big, 0, 1, b->stream);
+        else launch_sap(d, b->d_wl_small, 0, 2, b->d_wl_big, 1, 1, b->stream);
+        if (hipGetLastError() != hipSuccess) { omi_batch_destroy(b); return fail(OMI_ERR_HIP, "broadphase launch failed"); }
+        uint64_t c[C_COUNT];
+        hipStreamSynchronize(b->stream);
+        hipMemcpy(c, d.cnt, sizeof c, hipMemcpyDeviceToHost);
+        if (c[C_OVERFLOW] & 8ull) {
+            omi_batch_destroy(b);
+            pcap = (int64_t)c[C_BPAIRS] + 16;
+            continue;
+        }
+        int64_t np_ = (int64_t)c[C_BPTOTAL];
+        *count = np_;
+        const int64_t m = std::min<int64_t>(np_, cap);
+        std::vector<int32_t> pa((size_t)std::max<int64_t>(m, 1)), pb((size_t)std::max<int64_t>(m, 1));
+        if (m > 0 && pairs) {
+            rc = get(b, pa.data(), d.bpa, m);
+            if (!rc) rc = get(b, pb.data(), d.bpb, m);
+            for (int64_t k = 0; k < m && !rc; k++) { pairs[2 * k] = pa[k]; pairs[2 * k + 1] = pb[k]; }
+        }
+        omi_batch_destroy(b);
+        return rc;
+    }
+}
+
+extern "C" int32_t omi_prepare_and_solve(int64_t K, int32_t N, const int64_t* a_idx, const int64_t* b_idx,
+                                         const double* point, const double* normal, const double* depth, double* pos,
+                                         double* lv, double* av, const double* inv_mass, const double* invIw,
+                                         const double* restitution, const double* friction, double* nImp,
+                                         double* tImp, double rthr, int32_t iters, int32_t warm, double correction,
+                                         double slop, int32_t precision) {
+    if (K < 0 || N < 0 || (precision != 64 && precision != 32)) return fail(OMI_ERR_ARG, "bad arguments");
+    if (K == 0) return OMI_OK;
+    for (int64_t k = 0; k < K; k++)
+        if (a_idx[k] < 0 || a_idx[k] >= N || b_idx[k] < 0 || b_idx[k] >= N || a_idx[k] == b_idx[k])
+            return fail(OMI_ERR_ARG, "contact body index out of range");
+    omi_batch* b = nullptr;
+    int32_t rc = mini_batch(N, 1, K, precision, 0, &b);
+    if (rc) return rc;
+    Dev& d = b->d;
+    const bool f64 = precision == 64;
+    const size_t rs = f64 ? 8 : 4;
+    std::vector<int32_t> ia((size_t)K), ib((size_t)K), wz((size_t)N, 0);
+    std::vector<uint8_t> mt((size_t)N, M_DYNAMIC);
+    for (int64_t k = 0; k < K; k++) { ia[k] = (int32_t)a_idx[k]; ib[k] = (int32_t)b_idx[k]; }
+    // Real-typed uploads (component-major)
+    auto upR = [&](void* dst, int64_t ld, const double* src, int64_t n, int k, int stride_k) -> int32_t {
+        std::vector<unsigned char> tmp((size_t)n * rs);
+        for (int c = 0; c < k; c++) {
+            for (int64_t i = 0; i < n; i++) {
+                double v = src[i * stride_k + c];
+                if (f64) ((double*)tmp.data())[i] = v; else ((float*)tmp.data())[i] = (float)v;
+            }
+            if (hipMemcpy((unsigned char*)dst + (size_t)c * ld * rs, tmp.data(), n * rs, hipMemcpyHostToDevice) != hipSuccess)
+                return fail(OMI_ERR_HIP, "upload failed");
+        }
+        return OMI_OK;
+    };
+    std::vector<double> imp((size_t)K * 3);
+    for (int64_t k = 0; k < K; k++) { imp[3 * k] = nImp[k]; imp[3 * k + 1] = tImp[2 * k]; imp[3 * k + 2] = tImp[2 * k + 1]; }
+    DevBuf tb;
+    double* drest = tb.up(restitution, K);
+    double* dfric = tb.up(friction, K);
+    int32_t Ni = N, Ki = (int32_t)K, zero = 0, flags = WF_ACTIVE;
+    if (!(rc = upload_soa(b, d.pos, d.ncap, 0, pos, N, 3)) && !(rc = upload_soa(b, d.lv, d.ncap, 0, lv, N, 3)) &&
+        !(rc = upload_soa(b, d.av, d.ncap, 0, av, N, 3)) && !(rc = put(b, d.inv_mass, inv_mass, N)) &&
+        !(rc = put(b, d.mt, mt.data(), N)) && !(rc = put(b, d.world, wz.data(), N)) &&
+        !(rc = upR(d.iiw, d.ncap, invIw, N, 9, 9)) && !(rc = put(b, d.ca, ia.data(), K)) &&
+        !(rc = put(b, d.cb, ib.data(), K)) && !(rc = upload_soa(b, d.cpt, d.kcap, 0, point, K, 3)) &&
+        !(rc = upR(d.cnrm, d.kcap, normal, K, 3, 3)) && !(rc = upR(d.cdep, d.kcap, depth, K, 1, 1)) &&
+        !(rc = upR(d.cimp, d.kcap, imp.data(), K, 3, 3)) && !(rc = put(b, d.w_off, &zero, 1)) &&
+        !(rc = put(b, d.w_cnt, &Ni, 1)) && !(rc = put(b, d.w_cs, &zero, 1)) && !(rc = put(b, d.w_cn, &Ki, 1)) &&
+        !(rc = put(b, d.w_iters, &iters, 1)) && !(rc = put(b, d.w_flags, &flags, 1))) {
+        SolveIn in{drest, dfric, rthr, correction, slop, warm};
+        launch_solve(d, precision, 256, in, std::min<int32_t>(N, 12288), true, b->stream);
+        if (hipGetLastError() != hipSuccess) rc = fail(OMI_ERR_HIP, "solver launch failed");
+        if (!rc) rc = download_soa(b, pos, d.pos, d.ncap, 0, N, 3);
+        if (!rc) rc = download_soa(b, lv, d.lv, d.ncap, 0, N, 3);
+        if (!rc) rc = download_soa(b, av, d.av, d.ncap, 0, N, 3);
+        if (!rc) {
+            std::vector<unsigned char> raw((size_t)K * rs);
+            for (int c = 0; c < 3 && !rc; c++) {
+                if (hipMemcpy(raw.data(), (unsigned char*)d.cimp + (size_t)c * d.kcap * rs, K * rs, hipMemcpyDeviceToHost) != hipSuccess) {
+                    rc = fail(OMI_ERR_HIP, "download failed");
+                    break;
+                }
+                for (int64_t k = 0; k < K; k++) {
+                    double v = f64 ? ((double*)raw.data())[k] : (double)((float*)raw.data())[k];
+                    if (c == 0) nImp[k] = v; else tImp[2 * k + c - 1] = v;
+                }
+            }
+        }
+    }
+    omi_batch_destroy(b);
+    return rc;
+}
+
+extern "C" int32_t omi_collide_convex(int32_t, int32_t, const int32_t*, const double*, const double*, const double*,
+                                      const int32_t*, const int32_t*, const int32_t*, const double*, const double*,
+                                      const double*, const int32_t*, const int32_t*, const double*, int64_t,
+                                      int32_t*, double*, double*, double*) {
+    return fail(OMI_ERR_STATE, "convex contacts are not built into this library yet");
+}

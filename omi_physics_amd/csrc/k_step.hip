@@ -1,0 +1,1106 @@
+// k_step.hip — the per-step kernels of the contact pipeline on gfx950.
+//
+// Stage map (reference -> kernel):
+//   world.step prev-pose copy, integrate_forces, refit  -> k_pre        (1 thread / body)
+//   BroadPhase.pairs + solid filter                     -> k_sap        (1 block / world)
+//   NarrowPhase.generate (box/sphere)                   -> k_narrow     (1 thread / pair)
+//   contact list in pair-then-manifold order            -> k_compact    (1 block / world)
+//   SequentialImpulseSolver._wake_sleepers              -> k_wake       (1 block / world)
+//   _solve_native_full + prepare_and_solve              -> k_solve      (1 block / world)
+//   warm-start cache rebuild                            -> k_cache      (1 block / world)
+//   integrate_positions                                 -> k_post       (1 thread / body)
+//   _update_sleep + _contact_islands                    -> k_sleep_*    (contacts, bodies)
+#include "omi_math.hpp"
+#include "omi_batch.hpp"
+
+namespace omi {
+
+#define BODY(a, c, i) (a)[(size_t)(c) * d.ncap + (size_t)(i)]
+#define CON(a, c, k) (a)[(size_t)(c) * d.kcap + (size_t)(k)]
+
+__device__ __forceinline__ int64_t gtid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ int64_t gstride() { return (int64_t)gridDim.x * blockDim.x; }
+
+// Orders doubles as unsigned integers; -0.0 and +0.0 map to one key, like
+// numpy's comparison-based argsort sees them.
+__device__ __forceinline__ uint64_t order_key(double x) {
+    if (x == 0.0) x = 0.0;
+    uint64_t u = (uint64_t)__double_as_longlong(x);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ uint64_t hash_mix(uint64_t x) {
+    x ^= x >> 33; x *= 0xff51afd7ed558ccdull; x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ull; x ^= x >> 33;
+    return x;
+}
+
+// ---------------------------------------------------------- block scans --
+// Exclusive scan of a[0..n) in place (LDS or global), BLOCK threads; returns
+// the total to every thread.  scratch: BLOCK/64 + 2 ints of LDS.
+template <int BLOCK>
+__device__ int32_t block_exscan(int32_t* a, int32_t n, int32_t* scratch) {
+    constexpr int NW = BLOCK / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int32_t running = 0;
+    for (int32_t base = 0; base < n; base += BLOCK) {
+        int32_t v = base + tid < n ? a[base + tid] : 0, x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            int32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) scratch[wv] = x;
+        __syncthreads();
+        int32_t wbase = 0, tot = 0;
+#pragma unroll
+        for (int q = 0; q < NW; q++) { int32_t s = scratch[q]; if (q < wv) wbase += s; tot += s; }
+        if (base + tid < n) a[base + tid] = running + wbase + x - v;
+        running += tot;
+        __syncthreads();
+    }
+    return running;
+}
+
+// Exclusive prefix of a 0/1 flag over the block (one chunk); returns the
+// prefix, sets `total`.
+template <int BLOCK>
+__device__ __forceinline__ int32_t block_flag_prefix(bool f, int32_t* scratch, int32_t& total) {
+    constexpr int NW = BLOCK / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    uint64_t m = __ballot(f);
+    int32_t pre = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) scratch[wv] = __popcll(m);
+    __syncthreads();
+    int32_t wbase = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < NW; q++) { int32_t s = scratch[q]; if (q < wv) wbase += s; tot += s; }
+    __syncthreads();
+    total = tot;
+    return wbase + pre;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ int64_t block_sum(int64_t v, int64_t* scratch) {
+    constexpr int NW = BLOCK / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) scratch[wv] = v;
+    __syncthreads();
+    int64_t t = 0;
+#pragma unroll
+    for (int q = 0; q < NW; q++) t += scratch[q];
+    __syncthreads();
+    return t;
+}
+
+// ------------------------------------------------------------ per step --
+
+__global__ void k_begin(Dev d) {
+    int64_t i = gtid();
+    if (i < C_COUNT) d.cnt[i] = 0;
+    for (int64_t w = i; w < d.wcap; w += gstride()) d.w_flags[w] &= ~WF_SEQWAKE;
+}
+
+// Refit of body i from pose (world.py:471-508; body.py:115-123 for hulls).
+__device__ __forceinline__ void refit_body(const Dev& d, int64_t i, const double Rm[9], double margin) {
+    const int kind = d.kind[i];
+    double p[3] = {BODY(d.pos, 0, i), BODY(d.pos, 1, i), BODY(d.pos, 2, i)};
+    double h[3] = {BODY(d.half, 0, i), BODY(d.half, 1, i), BODY(d.half, 2, i)};
+    double lo[3], hi[3];
+    if (kind == K_NONE) {
+        for (int r = 0; r < 3; r++) { lo[r] = p[r]; hi[r] = p[r]; }
+    } else if (kind == K_CONVEX) {
+        const int32_t off = d.hoff[i], m = d.hcnt[i];
+        for (int r = 0; r < 3; r++) { lo[r] = 0.0; hi[r] = 0.0; }
+        for (int32_t k = 0; k < m; k++) {
+            const double* l = d.hull + 3 * (size_t)(off + k);
+            for (int r = 0; r < 3; r++) {
+                double v = ((l[0] * Rm[3 * r] + l[1] * Rm[3 * r + 1]) + l[2] * Rm[3 * r + 2]) + p[r];
+                if (k == 0 || v < lo[r]) lo[r] = v;
+                if (k == 0 || v > hi[r]) hi[r] = v;
+            }
+        }
+        for (int r = 0; r < 3; r++) { lo[r] = lo[r] - margin; hi[r] = hi[r] + margin; }
+    } else {
+        for (int r = 0; r < 3; r++) {
+            double hw = kind == K_SPHERE ? h[r] : box_hw(Rm, h, r);
+            lo[r] = (p[r] - hw) - margin;
+            hi[r] = (p[r] + hw) + margin;
+        }
+    }
+    for (int r = 0; r < 3; r++) { BODY(d.lo, r, i) = lo[r]; BODY(d.hi, r, i) = hi[r]; }
+}
+
+// Start of step for every live body: previous pose, sleep scratch reset,
+// integrate_forces (backend.py:26-40) for awake dynamic bodies, then the
+// collision-side per-body data: AABB, rotation, world inverse inertia
+// (solver.py:215-220, einsum order (p0 + p2) + p1).
+template <typename R>
+__global__ void k_pre(Dev d, int64_t nhw, double dt, double margin) {
+    for (int64_t i = gtid(); i < nhw; i += gstride()) {
+        const int32_t w = d.world[i];
+        if (w < 0) continue;
+        const int32_t wf = d.w_flags[w];
+        double q[4];
+        for (int c = 0; c < 3; c++) BODY(d.ppos, c, i) = BODY(d.pos, c, i);
+        for (int c = 0; c < 4; c++) { q[c] = BODY(d.quat, c, i); BODY(d.pquat, c, i) = q[c]; }
+        const int mt = d.mt[i];
+        const uint8_t aw = d.awake[i];
+        d.awake0[i] = aw;
+        d.parent[i] = (int32_t)i;
+        d.contacted[i] = 0;
+        d.notready[i] = 0;
+        double v[3] = {BODY(d.lv, 0, i), BODY(d.lv, 1, i), BODY(d.lv, 2, i)};
+        if (mt == M_DYNAMIC && aw) {
+            const double gf = d.gf[i], dt_ = dt;
+            for (int r = 0; r < 3; r++) v[r] = v[r] + (d.w_grav[(size_t)r * d.wcap + w] * gf) * dt_;
+            double fl = maxd(1.0 - (d.ld[i] + d.w_damp[w]) * dt_, 0.0);
+            for (int r = 0; r < 3; r++) v[r] = v[r] * fl;
+            double sp = sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+            const double qd = d.qd[i];
+            for (int r = 0; r < 3; r++) v[r] = v[r] - ((qd * sp) * v[r]) * dt_;
+            double fa = maxd(1.0 - (d.ad[i] + d.w_damp[d.wcap + w]) * dt_, 0.0);
+            for (int r = 0; r < 3; r++) BODY(d.lv, r, i) = v[r];
+            for (int r = 0; r < 3; r++) BODY(d.av, r, i) = BODY(d.av, r, i) * fa;
+        } else if (mt == M_DYNAMIC && (wf & WF_COLLISION)) {
+            // A sleeping body with a wake-grade speed makes the wake order matter.
+            double v2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+            if (v2 > 0.15 * 0.15) atomicOr(&d.w_flags[w], WF_SEQWAKE);
+        }
+        if (wf & WF_COLLISION) {
+            double Rm[9];
+            quat_to_matrix(q, Rm);
+            refit_body(d, i, Rm, margin);
+            R* rot = (R*)d.rot;
+            R* iiw = (R*)d.iiw;
+            const double dg[3] = {BODY(d.inv_inertia, 0, i), BODY(d.inv_inertia, 1, i), BODY(d.inv_inertia, 2, i)};
+            for (int r = 0; r < 9; r++) BODY(rot, r, i) = (R)Rm[r];
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) {
+                    double p0 = (Rm[3 * r] * dg[0]) * Rm[3 * c], p1 = (Rm[3 * r + 1] * dg[1]) * Rm[3 * c + 1];
+                    double p2 = (Rm[3 * r + 2] * dg[2]) * Rm[3 * c + 2];
+                    BODY(iiw, 3 * r + c, i) = (R)((p0 + p2) + p1);
+                }
+        }
+    }
+}
+
+// Refit only (PhysicsWorld.refit_aabbs called outside a step).
+__global__ void k_refit(Dev d, int64_t nhw, double margin) {
+    for (int64_t i = gtid(); i < nhw; i += gstride()) {
+        const int32_t w = d.world[i];
+        if (w < 0) continue;
+        double q[4], Rm[9];
+        for (int c = 0; c < 4; c++) q[c] = BODY(d.quat, c, i);
+        quat_to_matrix(q, Rm);
+        refit_body(d, i, Rm, margin);
+    }
+}
+
+// ------------------------------------------------------------ broadphase --
+
+// Pair test of the reference broadphase (broadphase.py:280-294) and the solid
+// filter (world.py:663-670) for world-local sorted pair (a, b).
+__device__ __forceinline__ void sap_eval(const Dev& d, int64_t a, int64_t b, int32_t fo, int32_t nf,
+                                         bool& bp, bool& solid) {
+    bool keep = (BODY(d.hi, 1, a) >= BODY(d.lo, 1, b)) && (BODY(d.hi, 1, b) >= BODY(d.lo, 1, a)) &&
+                (BODY(d.hi, 2, a) >= BODY(d.lo, 2, b)) && (BODY(d.hi, 2, b) >= BODY(d.lo, 2, a));
+    const int ma = d.mt[a], mb = d.mt[b];
+    keep = keep && (ma == M_DYNAMIC || mb == M_DYNAMIC);
+    const int64_t p = a < b ? a : b, q = a < b ? b : a;
+    keep = keep && d.filt_tab[fo + d.filt[p] * nf + d.filt[q]] != 0;
+    bp = keep;
+    const uint8_t fa = d.bflags[a], fb = d.bflags[b];
+    const bool am = (ma == M_DYNAMIC && d.awake[a]) || ma == M_KINEMATIC;
+    const bool bm = (mb == M_DYNAMIC && d.awake[b]) || mb == M_KINEMATIC;
+    solid = keep && !((fa | fb) & BF_TRIGGER) && (fa & BF_COLLIDER) && (fb & BF_COLLIDER) && (am || bm);
+}
+
+// BroadPhase.pairs (broadphase.py:243-294) for one world per block: bitonic
+// sort of (lo.x key, index) — the stable argsort order — then the candidate
+// windows [k+1, searchsorted(lo_sx, hi_sx[k], 'right')) flattened over the
+// block so one long window (a ground box) does not serialise a thread, then
+// an order-preserving compaction of the kept pairs into this world's slice.
+template <int BLOCK, bool GLOBAL>
+__global__ __launch_bounds__(BLOCK) void k_sap(Dev d, const int32_t* wlist, int32_t nw, int32_t np_max,
+                                               int32_t record_bp) {
+    extern __shared__ unsigned char smem[];
+    __shared__ int32_t s_scan[BLOCK / 64 + 2];
+    __shared__ int64_t s_sum[BLOCK / 64 + 2];
+    __shared__ int32_t s_base[4];
+    if ((int32_t)blockIdx.x >= nw) return;
+    const int32_t w = wlist[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int32_t flags = d.w_flags[w], n = d.w_cnt[w];
+    const int64_t off = d.w_off[w];
+    if (!(flags & WF_ACTIVE) || !(flags & WF_COLLISION) || n < 2) {
+        if (tid == 0) { d.w_pn[w] = 0; d.w_bpn[w] = 0; }
+        return;
+    }
+    int32_t NP = 1;
+    while (NP < n) NP <<= 1;
+    uint64_t* key;
+    int32_t *idx, *coff;
+    if (GLOBAL) {
+        key = d.skey + 2 * off; idx = d.sidx + 2 * off; coff = d.soff + 2 * off;
+    } else {
+        key = (uint64_t*)smem; idx = (int32_t*)(key + np_max); coff = idx + np_max;
+    }
+    for (int32_t t = tid; t < NP; t += BLOCK) {
+        if (t < n) { key[t] = order_key(BODY(d.lo, 0, off + t)); idx[t] = t; }
+        else { key[t] = ~0ull; idx[t] = 0x7fffffff; }
+    }
+    __syncthreads();
+    for (int32_t k = 2; k <= NP; k <<= 1)
+        for (int32_t j = k >> 1; j > 0; j >>= 1) {
+            for (int32_t t = tid; t < (NP >> 1); t += BLOCK) {
+                int32_t i = 2 * t - (t & (j - 1));
+                int32_t l = i + j;
+                uint64_t ki = key[i], kl = key[l];
+                int32_t ii = idx[i], il = idx[l];
+                bool gt = ki > kl || (ki == kl && ii > il);
+                bool up = (i & k) == 0;
+                if (gt == up) { key[i] = kl; key[l] = ki; idx[i] = il; idx[l] = ii; }
+            }
+            __syncthreads();
+        }
+    for (int32_t k = tid; k < n; k += BLOCK) {
+        uint64_t hk = order_key(BODY(d.hi, 0, off + idx[k]));
+        int32_t lo_i = 0, hi_i = n;
+        while (lo_i < hi_i) {
+            int32_t mid = (lo_i + hi_i) >> 1;
+            if (key[mid] <= hk) lo_i = mid + 1; else hi_i = mid;
+        }
+        int32_t c = lo_i - k - 1;
+        coff[k] = c > 0 ? c : 0;
+    }
+    __syncthreads();
+    const int32_t X = block_exscan<BLOCK>(coff, n, s_scan);
+    const int32_t fo = d.w_filt_off[w], nf = d.w_nfilt[w];
+    auto candidate = [&](int32_t c, int64_t& a, int64_t& b) {
+        int32_t lo_i = 0, hi_i = n;            // last k with coff[k] <= c
+        while (lo_i < hi_i) {
+            int32_t mid = (lo_i + hi_i) >> 1;
+            if (coff[mid] <= c) lo_i = mid + 1; else hi_i = mid;
+        }
+        int32_t k = lo_i - 1;
+        int32_t j = k + 1 + (c - coff[k]);
+        a = off + idx[k];
+        b = off + idx[j];
+    };
+    int64_t nbp = 0, nsol = 0;
+    for (int32_t c = tid; c < X; c += BLOCK) {
+        int64_t a, b;
+        bool bp, so;
+        candidate(c, a, b);
+        sap_eval(d, a, b, fo, nf, bp, so);
+        nbp += bp; nsol += so;
+    }
+    const int64_t tbp = block_sum<BLOCK>(nbp, s_sum);
+    const int64_t tsol = block_sum<BLOCK>(nsol, s_sum);
+    if (tid == 0) {
+        uint64_t base = atomicAdd((unsigned long long*)&d.cnt[C_PAIRS], (unsigned long long)tsol);
+        int32_t ok = base + tsol <= (uint64_t)d.pcap;
+        if (!ok) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 1ull);
+        s_base[0] = (int32_t)base; s_base[1] = ok;
+        d.w_ps[w] = (int32_t)base; d.w_pn[w] = ok ? (int32_t)tsol : 0;
+        int32_t okb = 0;
+        if (record_bp) {
+            uint64_t bb = atomicAdd((unsigned long long*)&d.cnt[C_BPAIRS], (unsigned long long)tbp);
+            okb = bb + tbp <= (uint64_t)d.pcap;
+            if (!okb) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 8ull);
+            s_base[2] = (int32_t)bb;
+            d.w_bps[w] = (int32_t)bb;
+        }
+        s_base[3] = okb;
+        d.w_bpn[w] = okb ? (int32_t)tbp : 0;
+        atomicAdd((unsigned long long*)&d.cnt[C_XCAND], (unsigned long long)X);
+        atomicAdd((unsigned long long*)&d.cnt[C_BPTOTAL], (unsigned long long)tbp);
+    }
+    __syncthreads();
+    const bool ok = s_base[1] != 0, okb = s_base[3] != 0;
+    int32_t ws = s_base[0], wb = s_base[2];
+    for (int32_t cb = 0; cb < X; cb += BLOCK) {
+        int32_t c = cb + tid;
+        int64_t a = 0, b = 0;
+        bool bp = false, so = false;
+        if (c < X) { candidate(c, a, b); sap_eval(d, a, b, fo, nf, bp, so); }
+        const int64_t p = a < b ? a : b, q = a < b ? b : a;
+        int32_t tot;
+        int32_t pre = block_flag_prefix<BLOCK>(so, s_scan, tot);
+        if (so && ok) { d.pa[ws + pre] = (int32_t)p; d.pb[ws + pre] = (int32_t)q; }
+        ws += tot;
+        if (record_bp) {
+            int32_t totb;
+            int32_t preb = block_flag_prefix<BLOCK>(bp, s_scan, totb);
+            if (bp && okb) { d.bpa[wb + preb] = (int32_t)p; d.bpb[wb + preb] = (int32_t)q; }
+            wb += totb;
+        }
+    }
+}
+
+// ----------------------------------------------------------- narrowphase --
+
+// NarrowPhase.generate dispatch (narrowphase.py:78-100) for one solid pair.
+// FP64 runs in world coordinates exactly like the reference; FP32 first moves
+// the pair into a frame at body a's centre (offset taken in FP64).
+template <typename R>
+__device__ __forceinline__ int narrow_pair(const Dev& d, int64_t a, int64_t b, R* nrm, R* pts, R* dep,
+                                           double* org) {
+    const R* rot = (const R*)d.rot;
+    const int ka = d.kind[a], kb = d.kind[b];
+    R Ra[9], Rb[9], ha[3], hb[3], ca[3], cb[3];
+    for (int r = 0; r < 9; r++) { Ra[r] = BODY(rot, r, a); Rb[r] = BODY(rot, r, b); }
+    for (int r = 0; r < 3; r++) { ha[r] = (R)BODY(d.half, r, a); hb[r] = (R)BODY(d.half, r, b); }
+    for (int r = 0; r < 3; r++) {
+        double pa = BODY(d.pos, r, a), pb = BODY(d.pos, r, b);
+        if (sizeof(R) == sizeof(double)) { ca[r] = (R)pa; cb[r] = (R)pb; org[r] = 0.0; }
+        else { ca[r] = R(0); cb[r] = (R)(pb - pa); org[r] = pa; }
+    }
+    int cnt = 0;
+    if (ka == K_BOX && kb == K_BOX) cnt = box_box(ca, Ra, ha, cb, Rb, hb, nrm, pts, dep);
+    else if (ka == K_SPHERE && kb == K_SPHERE) cnt = sphere_sphere(ca, ha[0], cb, hb[0], nrm, pts, dep);
+    else if (ka == K_SPHERE && kb == K_BOX) cnt = sphere_box(ca, ha[0], cb, Rb, hb, nrm, pts, dep);
+    else if (ka == K_BOX && kb == K_SPHERE) {
+        cnt = sphere_box(cb, hb[0], ca, Ra, ha, nrm, pts, dep);
+        nrm[0] = -nrm[0]; nrm[1] = -nrm[1]; nrm[2] = -nrm[2];
+    }
+    return cnt;
+}
+
+template <typename R>
+__global__ void k_narrow(Dev d) {
+    const uint64_t tot = d.cnt[C_PAIRS];
+    const int64_t P = tot < (uint64_t)d.pcap ? (int64_t)tot : d.pcap;
+    R* mn = (R*)d.mnrm;
+    R* md = (R*)d.mdep;
+    for (int64_t p = gtid(); p < P; p += gstride()) {
+        const int64_t a = d.pa[p], b = d.pb[p];
+        R nrm[3] = {R(0), R(0), R(0)}, pts[12], dep[4];
+        double org[3];
+        int cnt = narrow_pair<R>(d, a, b, nrm, pts, dep, org);
+        d.mcnt[p] = (int8_t)cnt;
+        for (int r = 0; r < 3; r++) mn[(size_t)r * d.pcap + p] = nrm[r];
+        for (int m = 0; m < cnt; m++) {
+            for (int r = 0; r < 3; r++) d.mpts[(size_t)r * 4 * d.pcap + 4 * p + m] = org[r] + (double)pts[3 * m + r];
+            md[4 * p + m] = dep[m];
+        }
+    }
+}
+
+// Contacts in the reference's order (pair order, then manifold order) packed
+// into one contiguous slice per world.
+template <typename R, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_compact(Dev d, int32_t record) {
+    __shared__ int32_t s_scan[BLOCK / 64 + 2];
+    __shared__ int64_t s_sum[BLOCK / 64 + 2];
+    __shared__ int32_t s_base[2];
+    const int32_t w = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (w >= d.wcap) return;
+    const int32_t pn = (d.w_flags[w] & WF_ACTIVE) ? d.w_pn[w] : 0, ps = d.w_ps[w];
+    if (pn == 0) { if (tid == 0) d.w_cn[w] = 0; return; }
+    int64_t kc = 0, mf = 0;
+    for (int32_t i = tid; i < pn; i += BLOCK) { int c = d.mcnt[ps + i]; kc += c; mf += c > 0; }
+    const int64_t K = block_sum<BLOCK>(kc, s_sum);
+    const int64_t M = block_sum<BLOCK>(mf, s_sum);
+    if (tid == 0) {
+        uint64_t base = atomicAdd((unsigned long long*)&d.cnt[C_CONTACTS], (unsigned long long)K);
+        int32_t ok = base + K <= (uint64_t)d.kcap;
+        if (!ok) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 2ull);
+        s_base[0] = (int32_t)base; s_base[1] = ok;
+        d.w_cs[w] = (int32_t)base; d.w_cn[w] = ok ? (int32_t)K : 0;
+        atomicAdd((unsigned long long*)&d.cnt[C_MANIFOLDS], (unsigned long long)M);
+    }
+    __syncthreads();
+    if (!s_base[1]) return;
+    int32_t run = s_base[0];
+    const R* mn = (const R*)d.mnrm;
+    const R* md = (const R*)d.mdep;
+    R* cn = (R*)d.cnrm;
+    R* cdp = (R*)d.cdep;
+    R* imp = (R*)d.cimp;
+    for (int32_t cb = 0; cb < pn; cb += BLOCK) {
+        const int32_t i = cb + tid;
+        const int32_t p = ps + i;
+        const int32_t c = i < pn ? d.mcnt[p] : 0;
+        // exclusive prefix of the counts over the chunk (counts are 0..4)
+        int32_t x = c;
+        const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) { int32_t y = __shfl_up(x, o, 64); if (lane >= o) x += y; }
+        if (lane == 63) s_scan[wv] = x;
+        __syncthreads();
+        int32_t wbase = 0, tot = 0;
+        for (int q = 0; q < BLOCK / 64; q++) { int32_t s = s_scan[q]; if (q < wv) wbase += s; tot += s; }
+        const int32_t dst = run + wbase + x - c;
+        for (int m = 0; m < c; m++) {
+            const int64_t k = dst + m;
+            d.ca[k] = d.pa[p]; d.cb[k] = d.pb[p];
+            for (int r = 0; r < 3; r++) {
+                CON(d.cpt, r, k) = d.mpts[(size_t)r * 4 * d.pcap + 4 * (size_t)p + m];
+                CON(cn, r, k) = mn[(size_t)r * d.pcap + p];
+                CON(imp, r, k) = R(0);
+            }
+            cdp[k] = md[4 * (size_t)p + m];
+        }
+        run += tot;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ wake --
+
+// SequentialImpulseSolver._wake_sleepers (solver.py:321-337).  Waking is
+// idempotent and, unless a sleeping dynamic body carries a wake-grade speed
+// (flag WF_SEQWAKE, set by k_pre), independent of contact order, so the
+// contacts are processed in parallel; otherwise one thread replays the
+// reference's sequential loop.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_wake(Dev d) {
+    const int32_t w = blockIdx.x;
+    if (w >= d.wcap || !(d.w_flags[w] & WF_ACTIVE)) return;
+    const int32_t K = d.w_cn[w], cs = d.w_cs[w];
+    if (K == 0) return;
+    const double th = 0.15 * 0.15;
+    auto v2 = [&](int64_t i) {
+        double x = BODY(d.lv, 0, i), y = BODY(d.lv, 1, i), z = BODY(d.lv, 2, i);
+        return (x * x + y * y) + z * z;
+    };
+    if (d.w_flags[w] & WF_SEQWAKE) {
+        if (threadIdx.x != 0) return;
+        for (int32_t k = 0; k < K; k++) {
+            const int64_t a = d.ca[cs + k], b = d.cb[cs + k];
+            const int ma = d.mt[a], mb = d.mt[b];
+            bool wa = ma == M_KINEMATIC || (ma == M_DYNAMIC && d.awake[a] && v2(a) > th);
+            if (!d.awake[b] && wa) { d.awake[b] = 1; d.timer[b] = 0.0; }
+            bool wb = mb == M_KINEMATIC || (mb == M_DYNAMIC && d.awake[b] && v2(b) > th);
+            if (!d.awake[a] && wb) { d.awake[a] = 1; d.timer[a] = 0.0; }
+        }
+        return;
+    }
+    for (int32_t k = threadIdx.x; k < K; k += BLOCK) {
+        const int64_t a = d.ca[cs + k], b = d.cb[cs + k];
+        const int ma = d.mt[a], mb = d.mt[b];
+        const bool wa = ma == M_KINEMATIC || (ma == M_DYNAMIC && d.awake0[a] && v2(a) > th);
+        const bool wb = mb == M_KINEMATIC || (mb == M_DYNAMIC && d.awake0[b] && v2(b) > th);
+        if (!d.awake0[b] && wa) { d.awake[b] = 1; d.timer[b] = 0.0; }
+        if (!d.awake0[a] && wb) { d.awake[a] = 1; d.timer[a] = 0.0; }
+    }
+}
+
+// ---------------------------------------------------------------- solver --
+
+// Solver row layout per contact (R [SR_N][kcap]).
+enum { SR_RA = 0, SR_RB = 3, SR_T1 = 6, SR_T2 = 9, SR_NM = 12, SR_T1M = 13, SR_T2M = 14, SR_VB = 15,
+       SR_FR = 16, SR_IMA = 17, SR_IMB = 18, SR_N = 19 };
+
+// _solver_native._effmass (_solver_native.pyx:154-176)
+template <typename R>
+__device__ __forceinline__ R effmass(const R* ra, const R* rb, R dx, R dy, R dz, const R* Ia, const R* Ib,
+                                     R ima, R imb) {
+    R ax = ra[1] * dz - ra[2] * dy, ay = ra[2] * dx - ra[0] * dz, az = ra[0] * dy - ra[1] * dx;
+    R bx = rb[1] * dz - rb[2] * dy, by = rb[2] * dx - rb[0] * dz, bz = rb[0] * dy - rb[1] * dx;
+    R kk = ima + imb, wx, wy, wz;
+    wx = Ia[0] * ax + Ia[1] * ay + Ia[2] * az;
+    wy = Ia[3] * ax + Ia[4] * ay + Ia[5] * az;
+    wz = Ia[6] * ax + Ia[7] * ay + Ia[8] * az;
+    kk += ax * wx + ay * wy + az * wz;
+    wx = Ib[0] * bx + Ib[1] * by + Ib[2] * bz;
+    wy = Ib[3] * bx + Ib[4] * by + Ib[5] * bz;
+    wz = Ib[6] * bx + Ib[7] * by + Ib[8] * bz;
+    kk += bx * wx + by * wy + bz * wz;
+    return kk > R(1e-12) ? R(1) / kk : R(0);
+}
+
+// _apply_w (_solver_native.pyx:132-151) on register copies of the two bodies.
+template <typename R>
+__device__ __forceinline__ void apply_w(R* va, R* vb, R ima, R imb, const R* Ia, const R* Ib, const R* rA,
+                                        const R* rB, R Px, R Py, R Pz) {
+    R crx, cry, crz;
+    va[0] -= ima * Px; va[1] -= ima * Py; va[2] -= ima * Pz;
+    vb[0] += imb * Px; vb[1] += imb * Py; vb[2] += imb * Pz;
+    crx = rA[1] * Pz - rA[2] * Py; cry = rA[2] * Px - rA[0] * Pz; crz = rA[0] * Py - rA[1] * Px;
+    va[3] -= Ia[0] * crx + Ia[1] * cry + Ia[2] * crz;
+    va[4] -= Ia[3] * crx + Ia[4] * cry + Ia[5] * crz;
+    va[5] -= Ia[6] * crx + Ia[7] * cry + Ia[8] * crz;
+    crx = rB[1] * Pz - rB[2] * Py; cry = rB[2] * Px - rB[0] * Pz; crz = rB[0] * Py - rB[1] * Px;
+    vb[3] += Ib[0] * crx + Ib[1] * cry + Ib[2] * crz;
+    vb[4] += Ib[3] * crx + Ib[4] * cry + Ib[5] * crz;
+    vb[5] += Ib[6] * crx + Ib[7] * cry + Ib[8] * crz;
+}
+
+template <typename R>
+__device__ __forceinline__ void relvel(const R* va, const R* vb, const R* rA, const R* rB, R& x, R& y, R& z) {
+    x = (vb[0] + vb[4] * rB[2] - vb[5] * rB[1]) - (va[0] + va[4] * rA[2] - va[5] * rA[1]);
+    y = (vb[1] + vb[5] * rB[0] - vb[3] * rB[2]) - (va[1] + va[5] * rA[0] - va[3] * rA[2]);
+    z = (vb[2] + vb[3] * rB[1] - vb[4] * rB[0]) - (va[2] + va[3] * rA[1] - va[4] * rA[0]);
+}
+
+// Cached (nImp, t0, t1) for contact k, solver._seed_pair (solver.py:222-233):
+// nearest cached point of the same (a, b) with d² < 0.04, first strict min.
+template <typename R>
+__device__ __forceinline__ void seed_lookup(const Dev& d, int32_t w, int32_t old, int64_t ci, int64_t a, int64_t b,
+                                            R& n0, R& t0, R& t1) {
+    const uint64_t key = ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
+    const int64_t hm = d.hcap - 1;
+    int64_t h = (int64_t)(hash_mix(key) & (uint64_t)hm);
+    const int32_t rs = d.w_cache_s[(size_t)old * d.wcap + w], rn = d.w_cache_n[(size_t)old * d.wcap + w];
+    for (int probe = 0; probe < 64; probe++) {
+        const uint64_t kk = d.hkey[old][h];
+        if (kk == ~0ull) return;
+        if (kk == key) {
+            int32_t v = d.hval[old][h];
+            if (v < rs || v >= rs + rn) return;
+            const R* kimp = (const R*)d.kimp[old];
+            double best = 0.04;
+            for (int32_t q = v; q < rs + rn && d.ka[old][q] == a && d.kb[old][q] == b; q++) {
+                double dx = CON(d.cpt, 0, ci) - CON(d.kpt[old], 0, q);
+                double dy = CON(d.cpt, 1, ci) - CON(d.kpt[old], 1, q);
+                double dz = CON(d.cpt, 2, ci) - CON(d.kpt[old], 2, q);
+                double dd = dx * dx + dy * dy + dz * dz;
+                if (dd < best) { best = dd; n0 = CON(kimp, 0, q); t0 = CON(kimp, 1, q); t1 = CON(kimp, 2, q); }
+            }
+            return;
+        }
+        h = (h + 1) & hm;
+    }
+}
+
+// One world's contacts solved exactly in the reference's Gauss-Seidel order
+// (_solver_native.pyx:179-312).  Contact k depends only on the previous contact
+// (in sequence, across sweeps) that touched one of its dynamic bodies; level
+// L(k) = 1 + max(L of those) puts every dependency on an earlier level, so the
+// contacts of one level touch disjoint dynamic bodies and run concurrently
+// while every contact still reads exactly the velocities the sequential loop
+// would.  Static/kinematic bodies are read-only (their inverse mass and
+// inertia are zero, so the reference's writes to them leave them unchanged).
+template <typename R, int BLOCK, bool STAGE>
+__global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, int32_t lds_bodies) {
+    extern __shared__ int32_t s_last[];
+    __shared__ int32_t s_scan[BLOCK / 64 + 2];
+    __shared__ int32_t s_D;
+    const int32_t w = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (w >= d.wcap || !(d.w_flags[w] & WF_ACTIVE)) return;
+    const int32_t K = d.w_cn[w], cs = d.w_cs[w], n = d.w_cnt[w];
+    const int64_t off = d.w_off[w];
+    if (K == 0) { if (tid == 0) d.w_levels[w] = 0; return; }
+    const bool fp64 = sizeof(R) == sizeof(double);
+    // velocities the solver works on: the FP64 state itself, or an FP32 copy
+    R* V = fp64 ? (R*)d.lv : (R*)d.svel;
+    if (!fp64)
+        for (int32_t i = tid; i < n; i += BLOCK)
+            for (int c = 0; c < 6; c++) BODY(V, c, off + i) = (R)BODY(d.lv, c, off + i);
+    const R* iiw = (const R*)d.iiw;
+    R* S = (R*)d.csolv;
+    R* imp = (R*)d.cimp;
+    const R* cn = (const R*)d.cnrm;
+    const int32_t old = *d.sel;
+    const int32_t mo = d.w_mat_off[w], nm = d.w_nmat[w];
+    __syncthreads();
+    // prepare every contact from the pre-solve velocities
+    for (int32_t k = tid; k < K; k += BLOCK) {
+        const int64_t ci = cs + k, a = d.ca[ci], b = d.cb[ci];
+        R rest, fric;
+        if (STAGE) {
+            rest = (R)in.st_rest[k]; fric = (R)in.st_fric[k];
+        } else {
+            const int32_t t = mo + d.mat[a] * nm + d.mat[b];
+            rest = (R)d.mat_rest[t]; fric = (R)d.mat_fric[t];
+            R n0 = R(0), t0 = R(0), t1 = R(0);
+            if (in.warm) seed_lookup<R>(d, w, old, ci, a, b, n0, t0, t1);
+            CON(imp, 0, ci) = n0; CON(imp, 1, ci) = t0; CON(imp, 2, ci) = t1;
+        }
+        const R ima = (R)d.inv_mass[a], imb = (R)d.inv_mass[b];
+        R rA[3], rB[3], Ia[9], Ib[9], va[6], vb[6];
+        for (int r = 0; r < 3; r++) {
+            double p = CON(d.cpt, r, ci);
+            rA[r] = (R)(p - BODY(d.pos, r, a));
+            rB[r] = (R)(p - BODY(d.pos, r, b));
+        }
+        for (int r = 0; r < 9; r++) { Ia[r] = BODY(iiw, r, a); Ib[r] = BODY(iiw, r, b); }
+        for (int c = 0; c < 6; c++) { va[c] = BODY(V, c, a); vb[c] = BODY(V, c, b); }
+        const R nx = CON(cn, 0, ci), ny = CON(cn, 1, ci), nz = CON(cn, 2, ci);
+        R axx, ayy, azz;
+        if ((nx >= R(0) ? nx : -nx) < R(0.9)) { axx = R(1); ayy = R(0); azz = R(0); }
+        else { axx = R(0); ayy = R(1); azz = R(0); }
+        R tx = ayy * nz - azz * ny, ty = azz * nx - axx * nz, tz = axx * ny - ayy * nx;
+        R ln = sqrt(tx * tx + ty * ty + tz * tz);
+        R inv = ln > R(1e-12) ? R(1) / ln : R(0);
+        tx *= inv; ty *= inv; tz *= inv;
+        const R ux = ny * tz - nz * ty, uy = nz * tx - nx * tz, uz = nx * ty - ny * tx;
+        R dvx, dvy, dvz;
+        relvel(va, vb, rA, rB, dvx, dvy, dvz);
+        const R vn0 = dvx * nx + dvy * ny + dvz * nz;
+        for (int r = 0; r < 3; r++) { CON(S, SR_RA + r, ci) = rA[r]; CON(S, SR_RB + r, ci) = rB[r]; }
+        CON(S, SR_T1, ci) = tx; CON(S, SR_T1 + 1, ci) = ty; CON(S, SR_T1 + 2, ci) = tz;
+        CON(S, SR_T2, ci) = ux; CON(S, SR_T2 + 1, ci) = uy; CON(S, SR_T2 + 2, ci) = uz;
+        CON(S, SR_NM, ci) = effmass(rA, rB, nx, ny, nz, Ia, Ib, ima, imb);
+        CON(S, SR_T1M, ci) = effmass(rA, rB, tx, ty, tz, Ia, Ib, ima, imb);
+        CON(S, SR_T2M, ci) = effmass(rA, rB, ux, uy, uz, Ia, Ib, ima, imb);
+        CON(S, SR_VB, ci) = vn0 < -(R)in.rthr ? -rest * vn0 : R(0);
+        CON(S, SR_FR, ci) = fric;
+        CON(S, SR_IMA, ci) = ima; CON(S, SR_IMB, ci) = imb;
+    }
+    // level schedule (one thread; `last` in LDS for small worlds)
+    int32_t* last = lds_bodies >= n ? s_last : d.last + off;
+    for (int32_t i = tid; i < n; i += BLOCK) last[i] = 0;
+    for (int32_t k = tid; k < K; k += BLOCK) d.lcount[cs + k] = 0;
+    __syncthreads();
+    if (tid == 0) {
+        int32_t D = 0;
+        for (int32_t k = 0; k < K; k++) {
+            const int64_t a = d.ca[cs + k], b = d.cb[cs + k];
+            const bool da = d.mt[a] == M_DYNAMIC, db = d.mt[b] == M_DYNAMIC;
+            const int32_t la = da ? last[a - off] : 0, lb = db ? last[b - off] : 0;
+            const int32_t L = (la > lb ? la : lb) + 1;
+            if (da) last[a - off] = L;
+            if (db) last[b - off] = L;
+            d.clevel[cs + k] = L;
+            d.lcount[cs + L - 1] += 1;
+            D = L > D ? L : D;
+        }
+        s_D = D;
+        d.w_levels[w] = D;
+        atomicMax((unsigned long long*)&d.cnt[C_MAXLEVEL], (unsigned long long)D);
+    }
+    __syncthreads();
+    const int32_t D = s_D;
+    block_exscan<BLOCK>(d.lcount + cs, D, s_scan);
+    for (int32_t k = tid; k < K; k += BLOCK) {
+        const int32_t L = d.clevel[cs + k];
+        const int32_t slot = atomicAdd(&d.lcount[cs + L - 1], 1);
+        d.corder[cs + slot] = k;
+    }
+    __syncthreads();
+    // after the scatter lcount[l] is the end of level l
+    auto level_begin = [&](int32_t l) { return l == 0 ? 0 : d.lcount[cs + l - 1]; };
+    auto body_io = [&](int64_t a, int64_t b, R* va, R* vb) {
+        for (int c = 0; c < 6; c++) { va[c] = BODY(V, c, a); vb[c] = BODY(V, c, b); }
+    };
+    auto body_store = [&](int64_t a, int64_t b, const R* va, const R* vb) {
+        if (d.mt[a] == M_DYNAMIC) for (int c = 0; c < 6; c++) BODY(V, c, a) = va[c];
+        if (d.mt[b] == M_DYNAMIC) for (int c = 0; c < 6; c++) BODY(V, c, b) = vb[c];
+    };
+    // warm start
+    if (in.warm) {
+        for (int32_t l = 0; l < D; l++) {
+            const int32_t e = d.lcount[cs + l];
+            for (int32_t t = level_begin(l) + tid; t < e; t += BLOCK) {
+                const int64_t ci = cs + d.corder[cs + t], a = d.ca[ci], b = d.cb[ci];
+                R va[6], vb[6], rA[3], rB[3], Ia[9], Ib[9];
+                body_io(a, b, va, vb);
+                for (int r = 0; r < 3; r++) { rA[r] = CON(S, SR_RA + r, ci); rB[r] = CON(S, SR_RB + r, ci); }
+                for (int r = 0; r < 9; r++) { Ia[r] = BODY(iiw, r, a); Ib[r] = BODY(iiw, r, b); }
+                const R n0 = CON(imp, 0, ci), u0 = CON(imp, 1, ci), w0 = CON(imp, 2, ci);
+                const R P0 = n0 * CON(cn, 0, ci) + u0 * CON(S, SR_T1, ci) + w0 * CON(S, SR_T2, ci);
+                const R P1 = n0 * CON(cn, 1, ci) + u0 * CON(S, SR_T1 + 1, ci) + w0 * CON(S, SR_T2 + 1, ci);
+                const R P2 = n0 * CON(cn, 2, ci) + u0 * CON(S, SR_T1 + 2, ci) + w0 * CON(S, SR_T2 + 2, ci);
+                apply_w(va, vb, CON(S, SR_IMA, ci), CON(S, SR_IMB, ci), Ia, Ib, rA, rB, P0, P1, P2);
+                body_store(a, b, va, vb);
+            }
+            __syncthreads();
+        }
+    }
+    const int32_t iters = d.w_iters[w];
+    for (int32_t it = 0; it < iters; it++) {
+        for (int32_t l = 0; l < D; l++) {
+            const int32_t e = d.lcount[cs + l];
+            for (int32_t t = level_begin(l) + tid; t < e; t += BLOCK) {
+                const int64_t ci = cs + d.corder[cs + t], a = d.ca[ci], b = d.cb[ci];
+                R va[6], vb[6], rA[3], rB[3], Ia[9], Ib[9], t1[3], t2[3];
+                body_io(a, b, va, vb);
+                for (int r = 0; r < 3; r++) {
+                    rA[r] = CON(S, SR_RA + r, ci); rB[r] = CON(S, SR_RB + r, ci);
+                    t1[r] = CON(S, SR_T1 + r, ci); t2[r] = CON(S, SR_T2 + r, ci);
+                }
+                for (int r = 0; r < 9; r++) { Ia[r] = BODY(iiw, r, a); Ib[r] = BODY(iiw, r, b); }
+                const R ima = CON(S, SR_IMA, ci), imb = CON(S, SR_IMB, ci);
+                const R nx = CON(cn, 0, ci), ny = CON(cn, 1, ci), nz = CON(cn, 2, ci);
+                R nI = CON(imp, 0, ci), tI0 = CON(imp, 1, ci), tI1 = CON(imp, 2, ci);
+                R dvx, dvy, dvz;
+                relvel(va, vb, rA, rB, dvx, dvy, dvz);
+                R vn = dvx * nx + dvy * ny + dvz * nz;
+                R nw = nI - CON(S, SR_NM, ci) * (vn - CON(S, SR_VB, ci));
+                if (nw < R(0)) nw = R(0);
+                R dL = nw - nI;
+                nI = nw;
+                apply_w(va, vb, ima, imb, Ia, Ib, rA, rB, dL * nx, dL * ny, dL * nz);
+                const R maxf = CON(S, SR_FR, ci) * nI;
+                relvel(va, vb, rA, rB, dvx, dvy, dvz);
+                R vt = dvx * t1[0] + dvy * t1[1] + dvz * t1[2];
+                R nt = tI0 - CON(S, SR_T1M, ci) * vt;
+                if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
+                dL = nt - tI0; tI0 = nt;
+                apply_w(va, vb, ima, imb, Ia, Ib, rA, rB, dL * t1[0], dL * t1[1], dL * t1[2]);
+                vt = dvx * t2[0] + dvy * t2[1] + dvz * t2[2];
+                nt = tI1 - CON(S, SR_T2M, ci) * vt;
+                if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
+                dL = nt - tI1; tI1 = nt;
+                apply_w(va, vb, ima, imb, Ia, Ib, rA, rB, dL * t2[0], dL * t2[1], dL * t2[2]);
+                CON(imp, 0, ci) = nI; CON(imp, 1, ci) = tI0; CON(imp, 2, ci) = tI1;
+                body_store(a, b, va, vb);
+            }
+            __syncthreads();
+        }
+    }
+    // split-impulse position pass, same order
+    for (int32_t l = 0; l < D; l++) {
+        const int32_t e = d.lcount[cs + l];
+        for (int32_t t = level_begin(l) + tid; t < e; t += BLOCK) {
+            const int64_t ci = cs + d.corder[cs + t];
+            const R dep = ((const R*)d.cdep)[ci];
+            const R vn = dep - (R)in.slop;
+            if (vn <= R(0)) continue;
+            const R ia = CON(S, SR_IMA, ci), ib = CON(S, SR_IMB, ci);
+            if (ia + ib <= R(1e-12)) continue;
+            const R dL = (R)in.correction * vn / (ia + ib);
+            const int64_t a = d.ca[ci], b = d.cb[ci];
+            if (d.mt[a] == M_DYNAMIC)
+                for (int r = 0; r < 3; r++) {
+                    if (fp64) BODY(d.pos, r, a) -= (double)(ia * dL * CON(cn, r, ci));
+                    else BODY(d.pos, r, a) = BODY(d.pos, r, a) - (double)(ia * dL * CON(cn, r, ci));
+                }
+            if (d.mt[b] == M_DYNAMIC)
+                for (int r = 0; r < 3; r++) {
+                    if (fp64) BODY(d.pos, r, b) += (double)(ib * dL * CON(cn, r, ci));
+                    else BODY(d.pos, r, b) = BODY(d.pos, r, b) + (double)(ib * dL * CON(cn, r, ci));
+                }
+        }
+        __syncthreads();
+    }
+    // FP32: fold the solver's velocity change back into the FP64 state
+    if (!fp64)
+        for (int32_t i = tid; i < n; i += BLOCK)
+            if (d.mt[off + i] == M_DYNAMIC)
+                for (int c = 0; c < 6; c++) {
+                    double v0 = BODY(d.lv, c, off + i);
+                    R dv = BODY(V, c, off + i) - (R)v0;
+                    BODY(d.lv, c, off + i) = v0 + (double)dv;
+                }
+}
+
+// ------------------------------------------------------- warm-start cache --
+
+__global__ void k_hash_clear(Dev d) {
+    const int32_t nw = *d.sel ^ 1;
+    for (int64_t i = gtid(); i < d.hcap; i += gstride()) d.hkey[nw][i] = ~0ull;
+}
+
+// The cache after this step (solver.py:271-278): this step's contacts with
+// their impulses, or — when the world had no contacts — last step's cache
+// unchanged (solver.py:197-198).  Group heads go into the hash.
+template <typename R, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_cache(Dev d) {
+    __shared__ int32_t s_base[2];
+    const int32_t w = blockIdx.x, tid = threadIdx.x;
+    if (w >= d.wcap) return;
+    const int32_t old = *d.sel, nw = old ^ 1;
+    const size_t os = (size_t)old * d.wcap + w, ns = (size_t)nw * d.wcap + w;
+    if (!(d.w_flags[w] & WF_ACTIVE)) { if (tid == 0) d.w_cache_n[ns] = 0; return; }
+    const int32_t K = d.w_cn[w];
+    const int32_t cnt = K > 0 ? K : d.w_cache_n[os];
+    if (cnt == 0) { if (tid == 0) d.w_cache_n[ns] = 0; return; }
+    if (tid == 0) {
+        uint64_t base = atomicAdd((unsigned long long*)&d.cnt[C_CACHE], (unsigned long long)cnt);
+        int32_t ok = base + cnt <= (uint64_t)d.kcap;
+        if (!ok) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 4ull);
+        s_base[0] = (int32_t)base; s_base[1] = ok;
+        d.w_cache_s[ns] = (int32_t)base; d.w_cache_n[ns] = ok ? cnt : 0;
+    }
+    __syncthreads();
+    if (!s_base[1]) return;
+    const int32_t base = s_base[0];
+    const int32_t src = K > 0 ? d.w_cs[w] : d.w_cache_s[os];
+    const int32_t* sa = K > 0 ? d.ca : d.ka[old];
+    const int32_t* sb = K > 0 ? d.cb : d.kb[old];
+    const double* sp = K > 0 ? d.cpt : d.kpt[old];
+    const R* si = (const R*)(K > 0 ? d.cimp : d.kimp[old]);
+    R* di = (R*)d.kimp[nw];
+    for (int32_t k = tid; k < cnt; k += BLOCK) {
+        const int64_t s = src + k, t = base + k;
+        const int32_t a = sa[s], b = sb[s];
+        d.ka[nw][t] = a; d.kb[nw][t] = b;
+        for (int r = 0; r < 3; r++) { CON(d.kpt[nw], r, t) = CON(sp, r, s); CON(di, r, t) = CON(si, r, s); }
+        if (k == 0 || sa[s - 1] != a || sb[s - 1] != b) {
+            const uint64_t key = ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
+            const int64_t hm = d.hcap - 1;
+            int64_t h = (int64_t)(hash_mix(key) & (uint64_t)hm);
+            for (int64_t probe = 0; probe < d.hcap; probe++) {
+                unsigned long long prev = atomicCAS((unsigned long long*)&d.hkey[nw][h], ~0ull, (unsigned long long)key);
+                if (prev == ~0ull || prev == key) { d.hval[nw][h] = (int32_t)t; break; }
+                h = (h + 1) & hm;
+            }
+        }
+    }
+}
+
+__global__ void k_toggle(Dev d) {
+    if (gtid() == 0) *d.sel ^= 1;
+}
+
+// ------------------------------------------------------------- integrate --
+
+// NumpyBackend.integrate_positions (backend.py:42-49) with quat_integrate
+// (mathutil.py:108-113) for awake dynamic and kinematic bodies.
+__global__ void k_post(Dev d, int64_t nhw, double dt) {
+    for (int64_t i = gtid(); i < nhw; i += gstride()) {
+        if (d.world[i] < 0) continue;
+        const int mt = d.mt[i];
+        if (!((mt == M_DYNAMIC || mt == M_KINEMATIC) && d.awake[i])) continue;
+        for (int r = 0; r < 3; r++) BODY(d.pos, r, i) = BODY(d.pos, r, i) + BODY(d.lv, r, i) * dt;
+        const double wx = BODY(d.av, 0, i), wy = BODY(d.av, 1, i), wz = BODY(d.av, 2, i);
+        const double bx = BODY(d.quat, 0, i), by = BODY(d.quat, 1, i), bz = BODY(d.quat, 2, i),
+                     bw = BODY(d.quat, 3, i);
+        const double mx = ((0.0 * bx + wx * bw) + wy * bz) - wz * by;
+        const double my = ((0.0 * by - wx * bz) + wy * bw) + wz * bx;
+        const double mz = ((0.0 * bz + wx * by) - wy * bx) + wz * bw;
+        const double mw = ((0.0 * bw - wx * bx) - wy * by) - wz * bz;
+        const double s0 = bx + (0.5 * mx) * dt, s1 = by + (0.5 * my) * dt;
+        const double s2 = bz + (0.5 * mz) * dt, s3 = bw + (0.5 * mw) * dt;
+        const double nn = sqrt(((s0 * s0 + s1 * s1) + s2 * s2) + s3 * s3);
+        const double dd = maxd(nn, 1e-12);
+        BODY(d.quat, 0, i) = s0 / dd; BODY(d.quat, 1, i) = s1 / dd;
+        BODY(d.quat, 2, i) = s2 / dd; BODY(d.quat, 3, i) = s3 / dd;
+    }
+}
+
+// ----------------------------------------------------------------- sleep --
+
+__device__ __forceinline__ int32_t uf_find(int32_t* par, int32_t x) {
+    int32_t p = __atomic_load_n(&par[x], __ATOMIC_RELAXED);
+    while (p != x) {
+        x = p;
+        p = __atomic_load_n(&par[x], __ATOMIC_RELAXED);
+    }
+    return x;
+}
+
+// _contact_islands (world.py:540-567): union of contacts whose two ends are
+// both dynamic; every dynamic end counts as contacted.
+__global__ void k_sleep_union(Dev d) {
+    const uint64_t tot = d.cnt[C_CONTACTS];
+    const int64_t K = tot < (uint64_t)d.kcap ? (int64_t)tot : d.kcap;
+    for (int64_t k = gtid(); k < K; k += gstride()) {
+        const int32_t a = d.ca[k], b = d.cb[k];
+        const int32_t w = d.world[a];
+        if (w < 0 || !(d.w_flags[w] & WF_SLEEP)) continue;
+        const bool da = d.mt[a] == M_DYNAMIC, db = d.mt[b] == M_DYNAMIC;
+        if (da) d.contacted[a] = 1;
+        if (db) d.contacted[b] = 1;
+        if (!(da && db)) continue;
+        int32_t x = a, y = b;
+        for (;;) {
+            x = uf_find(d.parent, x);
+            y = uf_find(d.parent, y);
+            if (x == y) break;
+            if (x < y) { int32_t t = x; x = y; y = t; }
+            if (atomicCAS(&d.parent[x], x, y) == x) break;
+        }
+    }
+}
+
+// _update_sleep (world.py:510-538), first half: speed tests, timers, and an
+// island is marked not-ready if any member is not ready.
+__global__ void k_sleep_body(Dev d, int64_t nhw, double dt) {
+    for (int64_t i = gtid(); i < nhw; i += gstride()) {
+        const int32_t w = d.world[i];
+        if (w < 0 || !(d.w_flags[w] & WF_SLEEP)) continue;
+        const bool dyn = d.mt[i] == M_DYNAMIC;
+        const double x = BODY(d.lv, 0, i), y = BODY(d.lv, 1, i), z = BODY(d.lv, 2, i);
+        const double p = BODY(d.av, 0, i), q = BODY(d.av, 1, i), r = BODY(d.av, 2, i);
+        const double s2 = (x * x + y * y) + z * z, w2 = (p * p + q * q) + r * r;
+        const bool slow = dyn && s2 < 0.08 * 0.08 && w2 < 0.25 * 0.25;
+        double tm = slow ? d.timer[i] + dt : 0.0;
+        d.timer[i] = tm;
+        const bool ready = slow && tm > 0.4;
+        d.awake0[i] = (uint8_t)((slow ? 1 : 0) | (ready ? 2 : 0));   // reuse as per-body flags
+        if (d.contacted[i] && !ready) d.notready[uf_find(d.parent, (int32_t)i)] = 1;
+    }
+}
+
+__global__ void k_sleep_final(Dev d, int64_t nhw) {
+    for (int64_t i = gtid(); i < nhw; i += gstride()) {
+        const int32_t w = d.world[i];
+        if (w < 0 || !(d.w_flags[w] & WF_SLEEP)) continue;
+        const bool dyn = d.mt[i] == M_DYNAMIC;
+        const uint8_t f = d.awake0[i];
+        const bool slow = f & 1, ready = f & 2;
+        const bool to_sleep = d.contacted[i] ? !d.notready[uf_find(d.parent, (int32_t)i)] : (dyn && ready);
+        uint8_t aw = d.awake[i];
+        if (to_sleep) aw = 0;
+        if (dyn && !slow) aw = 1;
+        d.awake[i] = aw;
+        if (!aw && dyn)
+            for (int c = 0; c < 6; c++) BODY(d.lv, c, i) = 0.0;
+    }
+}
+
+// ------------------------------------------------------ stage-level kernels --
+
+template <typename R>
+__global__ void k_stage_boxbox(int32_t np, const double* ca, const double* Ra, const double* ha, const double* cb,
+                               const double* Rb, const double* hb, int32_t* cnt, double* nrm, double* pts,
+                               double* dep) {
+    for (int64_t i = gtid(); i < np; i += gstride()) {
+        R A[9], B[9], h1[3], h2[3], c1[3], c2[3], n[3] = {R(0), R(0), R(0)}, p[12], dd[4];
+        for (int r = 0; r < 9; r++) { A[r] = (R)Ra[9 * i + r]; B[r] = (R)Rb[9 * i + r]; }
+        double org[3];
+        for (int r = 0; r < 3; r++) {
+            h1[r] = (R)ha[3 * i + r]; h2[r] = (R)hb[3 * i + r];
+            if (sizeof(R) == sizeof(double)) { c1[r] = (R)ca[3 * i + r]; c2[r] = (R)cb[3 * i + r]; org[r] = 0.0; }
+            else { c1[r] = R(0); c2[r] = (R)(cb[3 * i + r] - ca[3 * i + r]); org[r] = ca[3 * i + r]; }
+        }
+        int c = box_box(c1, A, h1, c2, B, h2, n, p, dd);
+        cnt[i] = c;
+        for (int r = 0; r < 3; r++) nrm[3 * i + r] = n[r];
+        for (int m = 0; m < 4; m++) {
+            for (int r = 0; r < 3; r++) pts[12 * i + 3 * m + r] = m < c ? org[r] + (double)p[3 * m + r] : 0.0;
+            dep[4 * i + m] = m < c ? (double)dd[m] : 0.0;
+        }
+    }
+}
+
+template <typename R>
+__global__ void k_stage_spheres(int32_t np, int32_t mode, const double* ca, const double* ra, const double* cb,
+                                const double* Rb, const double* hb, const double* rb, int32_t* cnt, double* nrm,
+                                double* pt, double* dep) {
+    for (int64_t i = gtid(); i < np; i += gstride()) {
+        R c1[3], c2[3], n[3] = {R(0), R(0), R(0)}, p[3] = {R(0), R(0), R(0)}, dd = R(0);
+        double org[3];
+        for (int r = 0; r < 3; r++) {
+            if (sizeof(R) == sizeof(double)) { c1[r] = (R)ca[3 * i + r]; c2[r] = (R)cb[3 * i + r]; org[r] = 0.0; }
+            else { c1[r] = R(0); c2[r] = (R)(cb[3 * i + r] - ca[3 * i + r]); org[r] = ca[3 * i + r]; }
+        }
+        int c;
+        if (mode == 0) {
+            c = sphere_sphere(c1, (R)ra[i], c2, (R)rb[i], n, p, &dd);
+        } else {
+            R M[9], h[3];
+            for (int r = 0; r < 9; r++) M[r] = (R)Rb[9 * i + r];
+            for (int r = 0; r < 3; r++) h[r] = (R)hb[3 * i + r];
+            c = sphere_box(c1, (R)ra[i], c2, M, h, n, p, &dd);
+        }
+        cnt[i] = c;
+        for (int r = 0; r < 3; r++) { nrm[3 * i + r] = n[r]; pt[3 * i + r] = c ? org[r] + (double)p[r] : 0.0; }
+        dep[i] = c ? (double)dd : 0.0;
+    }
+}
+
+// ----------------------------------------------------------- launchers --
+
+#define GRID_FOR(n, b) ((int)(((n) + (b)-1) / (b) < 8192 ? ((n) + (b)-1) / (b) : 8192))
+
+struct Launch {
+    static void begin(const Dev& d, hipStream_t s) {
+        int g = GRID_FOR(d.wcap > C_COUNT ? d.wcap : C_COUNT, 256);
+        hipLaunchKernelGGL(k_begin, dim3(g), dim3(256), 0, s, d);
+    }
+};
+
+}  // namespace omi
+
+// host-side entry points into the kernels, used by capi.hip
+namespace omi {
+void launch_begin(const Dev& d, hipStream_t s) { Launch::begin(d, s); }
+
+void launch_pre(const Dev& d, int precision, int64_t nhw, double dt, double margin, hipStream_t s) {
+    if (nhw <= 0) return;
+    int g = GRID_FOR(nhw, 256);
+    if (precision == 64) hipLaunchKernelGGL(k_pre<double>, dim3(g), dim3(256), 0, s, d, nhw, dt, margin);
+    else hipLaunchKernelGGL(k_pre<float>, dim3(g), dim3(256), 0, s, d, nhw, dt, margin);
+}
+
+void launch_refit(const Dev& d, int64_t nhw, double margin, hipStream_t s) {
+    if (nhw <= 0) return;
+    hipLaunchKernelGGL(k_refit, dim3(GRID_FOR(nhw, 256)), dim3(256), 0, s, d, nhw, margin);
+}
+
+void launch_sap(const Dev& d, const int32_t* wl_small, int32_t n_small, int32_t np_small, const int32_t* wl_big,
+                int32_t n_big, int32_t record, hipStream_t s) {
+    if (n_small > 0) {
+        size_t lds = (size_t)np_small * (8 + 4 + 4);
+        hipLaunchKernelGGL((k_sap<256, false>), dim3(n_small), dim3(256), lds, s, d, wl_small, n_small, np_small,
+                           record);
+    }
+    if (n_big > 0)
+        hipLaunchKernelGGL((k_sap<1024, true>), dim3(n_big), dim3(1024), 0, s, d, wl_big, n_big, 0, record);
+}
+
+void launch_narrow(const Dev& d, int precision, hipStream_t s) {
+    int g = GRID_FOR(d.pcap, 256);
+    if (precision == 64) hipLaunchKernelGGL(k_narrow<double>, dim3(g), dim3(256), 0, s, d);
+    else hipLaunchKernelGGL(k_narrow<float>, dim3(g), dim3(256), 0, s, d);
+}
+
+void launch_compact(const Dev& d, int precision, int32_t record, hipStream_t s) {
+    if (precision == 64) hipLaunchKernelGGL((k_compact<double, 256>), dim3(d.wcap), dim3(256), 0, s, d, record);
+    else hipLaunchKernelGGL((k_compact<float, 256>), dim3(d.wcap), dim3(256), 0, s, d, record);
+}
+
+void launch_wake(const Dev& d, hipStream_t s) {
+    hipLaunchKernelGGL((k_wake<256>), dim3(d.wcap), dim3(256), 0, s, d);
+}
+
+void launch_solve(const Dev& d, int precision, int block, SolveIn in, int32_t lds_bodies, bool stage,
+                  hipStream_t s) {
+    size_t lds = (size_t)lds_bodies * 4;
+#define OMI_SOLVE(R, B)                                                                                     \
+    do {                                                                                                    \
+        if (stage) hipLaunchKernelGGL((k_solve<R, B, true>), dim3(d.wcap), dim3(B), lds, s, d, in, lds_bodies); \
+        else hipLaunchKernelGGL((k_solve<R, B, false>), dim3(d.wcap), dim3(B), lds, s, d, in, lds_bodies);      \
+    } while (0)
+    if (precision == 64) {
+        if (block <= 64) OMI_SOLVE(double, 64);
+        else if (block <= 256) OMI_SOLVE(double, 256);
+        else OMI_SOLVE(double, 1024);
+    } else {
+        if (block <= 64) OMI_SOLVE(float, 64);
+        else if (block <= 256) OMI_SOLVE(float, 256);
+        else OMI_SOLVE(float, 1024);
+    }
+#undef OMI_SOLVE
+}
+
+void launch_cache(const Dev& d, int precision, hipStream_t s) {
+    hipLaunchKernelGGL(k_hash_clear, dim3(GRID_FOR(d.hcap, 256)), dim3(256), 0, s, d);
+    if (precision == 64) hipLaunchKernelGGL((k_cache<double, 256>), dim3(d.wcap), dim3(256), 0, s, d);
+    else hipLaunchKernelGGL((k_cache<float, 256>), dim3(d.wcap), dim3(256), 0, s, d);
+}
+
+void launch_post(const Dev& d, int64_t nhw, double dt, hipStream_t s) {
+    if (nhw <= 0) return;
+    hipLaunchKernelGGL(k_post, dim3(GRID_FOR(nhw, 256)), dim3(256), 0, s, d, nhw, dt);
+}
+
+void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s) {
+    if (nhw <= 0) return;
+    hipLaunchKernelGGL(k_sleep_union, dim3(GRID_FOR(d.kcap, 256)), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_sleep_body, dim3(GRID_FOR(nhw, 256)), dim3(256), 0, s, d, nhw, dt);
+    hipLaunchKernelGGL(k_sleep_final, dim3(GRID_FOR(nhw, 256)), dim3(256), 0, s, d, nhw);
+}
+
+void launch_toggle(const Dev& d, hipStream_t s) { hipLaunchKernelGGL(k_toggle, dim3(1), dim3(64), 0, s, d); }
+
+void launch_stage_boxbox(int precision, int32_t np, const double* ca, const double* Ra, const double* ha,
+                         const double* cb, const double* Rb, const double* hb, int32_t* cnt, double* nrm,
+                         double* pts, double* dep, hipStream_t s) {
+    int g = GRID_FOR(np, 256);
+    if (precision == 64)
+        hipLaunchKernelGGL(k_stage_boxbox<double>, dim3(g), dim3(256), 0, s, np, ca, Ra, ha, cb, Rb, hb, cnt, nrm, pts, dep);
+    else
+        hipLaunchKernelGGL(k_stage_boxbox<float>, dim3(g), dim3(256), 0, s, np, ca, Ra, ha, cb, Rb, hb, cnt, nrm, pts, dep);
+}
+
+void launch_stage_spheres(int precision, int32_t np, int32_t mode, const double* ca, const double* ra,
+                          const double* cb, const double* Rb, const double* hb, const double* rb, int32_t* cnt,
+                          double* nrm, double* pt, double* dep, hipStream_t s) {
+    int g = GRID_FOR(np, 256);
+    if (precision == 64)
+        hipLaunchKernelGGL(k_stage_spheres<double>, dim3(g), dim3(256), 0, s, np, mode, ca, ra, cb, Rb, hb, rb, cnt, nrm, pt, dep);
+    else
+        hipLaunchKernelGGL(k_stage_spheres<float>, dim3(g), dim3(256), 0, s, np, mode, ca, ra, cb, Rb, hb, rb, cnt, nrm, pt, dep);
+}
+}  // namespace omi

@@ -1,0 +1,91 @@
+// omi_batch.hpp — device-resident batch layout shared by the kernels and the host.
+//
+// All per-body arrays are structure-of-arrays over a global body index
+// (worlds occupy contiguous ranges), component-major: component c of body i is
+// at [c * ncap + i], so each component load of a wavefront is one coalesced
+// 512-byte (FP64) transaction.  Per-contact arrays use the same scheme over
+// the contact index.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace omi {
+
+enum : int32_t { WF_ACTIVE = 1, WF_COLLISION = 2, WF_SLEEP = 4, WF_SEQWAKE = 8 };
+enum : uint8_t { BF_COLLIDER = 1, BF_TRIGGER = 2 };
+
+// counters (uint64 slots)
+enum : int32_t {
+    C_PAIRS = 0,       // solid pairs reserved
+    C_BPAIRS = 1,      // broadphase pairs reserved (record mode)
+    C_CONTACTS = 2,    // contacts reserved
+    C_CACHE = 3,       // warm-start cache entries reserved (new buffer)
+    C_XCAND = 4,       // x-window candidates
+    C_BPTOTAL = 5,     // broadphase pairs (all worlds)
+    C_MANIFOLDS = 6,
+    C_MAXLEVEL = 7,
+    C_OVERFLOW = 8,    // bit 0 pairs, 1 contacts, 2 cache, 3 broadphase record
+    C_COUNT = 16
+};
+
+struct Dev {
+    int64_t ncap, pcap, kcap, hcap;   // bodies, pairs, contacts, hash slots (pow2)
+    int32_t wcap;
+    // per body
+    double *pos, *quat, *ppos, *pquat;    // [3|4][ncap]
+    double *lv, *av;                      // [3][ncap] FP64 state
+    double *lo, *hi;                      // [3][ncap]
+    double *half, *inv_mass, *inv_inertia;
+    double *gf, *ld, *ad, *qd, *timer;
+    void *rot;                            // R[9][ncap] narrowphase rotation
+    void *iiw;                            // R[9][ncap] world inverse inertia
+    void *svel;                           // R[6][ncap] FP32 solver velocity staging
+    int32_t *world;                       // world slot or -1
+    int32_t *mat, *filt, *hoff, *hcnt;
+    uint8_t *mt, *kind, *bflags, *awake, *awake0, *contacted, *notready;
+    int32_t *parent, *last;               // union-find, level schedule scratch
+    uint64_t *skey;                       // [2*ncap] broadphase sort keys (large worlds)
+    int32_t *sidx, *soff;                 // [2*ncap] broadphase scratch (large worlds)
+    double *hull;                         // [hull_cap][3] local hull points, row-major
+    // per world
+    int32_t *w_off, *w_cnt, *w_flags, *w_iters;
+    int32_t *w_mat_off, *w_nmat, *w_filt_off, *w_nfilt;
+    double *w_grav, *w_damp;              // [3][wcap], [2][wcap]
+    int32_t *w_ps, *w_pn, *w_bps, *w_bpn; // solid / broadphase pair ranges
+    int32_t *w_cs, *w_cn;                 // contact ranges
+    int32_t *w_levels;                    // levels of the last solve
+    int32_t *w_cache_s, *w_cache_n;       // [2][wcap] cache ranges per buffer
+    double *mat_fric, *mat_rest;          // pooled tables
+    uint8_t *filt_tab;
+    // pairs
+    int32_t *pa, *pb, *bpa, *bpb;
+    int8_t *mcnt;
+    double *mpts;                         // [3][4*pcap] absolute points
+    void *mnrm, *mdep;                    // R [3][pcap], R [4*pcap]
+    // contacts
+    int32_t *ca, *cb;
+    double *cpt;                          // [3][kcap]
+    void *cnrm, *cdep, *cimp;             // R [3][kcap], R [kcap], R [3][kcap] (n, t1, t2 impulses)
+    void *csolv;                          // R [20][kcap] solver rows
+    int32_t *clevel, *corder, *lcount;    // level schedule
+    // warm-start cache, two buffers
+    int32_t *ka[2], *kb[2];
+    double *kpt[2];                       // [3][kcap]
+    void *kimp[2];                        // R [3][kcap]
+    uint64_t *hkey[2];
+    int32_t *hval[2];
+    int32_t *sel;                         // current cache buffer index (device)
+    uint64_t *cnt;                        // counters
+};
+
+// Where the solver of one world gets seeds, restitution and friction: batch
+// mode looks seeds up in the warm-start cache and reads the material tables;
+// stage mode (omi_prepare_and_solve) takes them as arrays.
+struct SolveIn {
+    const double* st_rest;   // stage mode: [K] restitution, else null
+    const double* st_fric;
+    double rthr, correction, slop;
+    int32_t warm;
+};
+
+}  // namespace omi
