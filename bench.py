@@ -1,0 +1,249 @@
+"""Benchmark: body-steps/s of the batched contact pipeline on one MI355X.
+
+Workload (default): 4096 independent worlds, each a static ground box
+(half extents 10 x 0.5 x 10) plus n_w dynamic bodies, n_w =
+clip(round(lognormal(ln 64, 1.0)), 8, 2048) (seed 3, ~435k bodies), a 50/50
+mix of unit boxes and r = 0.5 spheres of mass 1 dropped from y in [2, 50]
+over x, z in [-10, 10]; mu = 0.5, e = 0, dt = 1/60, 10 solver iterations,
+sleeping on.  Narrowphase and solver arithmetic in FP32, integration and
+broadphase in FP64.  One "step" advances every world by dt; the timed region
+is steps 1..K of the scene from its initial state (a separate batch takes
+the warm-up steps).
+
+Printed: one JSON line with the batched FP32 throughput as ``value``, the
+plain one-world-at-a-time GPU form and the FP64 batched form beside it,
+p50/p99 step latency, FP32-vs-FP64 position error on a sample of worlds, the
+roofline of the dominant kernel and the CPU baseline (the oracle, a C
+restatement of the reference algorithm, on this host's cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = 'body-steps/s at 1 GPU (batched FP32 vs plain) + p99 step latency, err≤1e-4m'
+DT = 1.0 / 60.0
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_worlds(n_worlds, seed):
+    from omi_physics_amd import scenes
+    sizes = scenes.world_sizes(n_worlds, seed=seed)
+    t = time.perf_counter()
+    arrays = [scenes.arrays(scenes.boxes_on_ground(int(sizes[w]), seed=seed * 100003 + w, sphere_fraction=0.5))
+              for w in range(n_worlds)]
+    log('scene: %d worlds, %d bodies (%.1f s)' % (n_worlds, sum(a['n'] for a in arrays), time.perf_counter() - t))
+    return arrays
+
+
+def algorithmic_bytes(kernel, c, r):
+    """HBM bytes one launch of ``kernel`` must move, from the step's counters.
+
+    Per-unit figures (r = bytes of the narrowphase/solver float):
+      pre     350 + 18r per body (pose, velocity and parameters in; previous pose,
+              velocity, AABB, rotation and world inverse inertia out)
+      sap     16 per body (lo.x, hi.x) + 32 per x-window candidate (y/z bounds)
+              + 8 per solid pair written
+      narrow  106 + 18r per solid pair (indices, kinds, poses, rotations,
+              half extents) + 24 + r per contact point written
+      solve   32 + 7r per contact (indices, point, normal, depth, impulses)
+              + 152 + 9r per body touched (velocity, pose, mass and inertia in
+              and out), bodies touched taken as min(bodies, 2 x manifolds)
+    """
+    N, X, P, M, C = c['bodies'], c['x_candidates'], c['solid_pairs'], c['manifolds'], c['contacts']
+    if kernel == 'pre':
+        return (350 + 18 * r) * N
+    if kernel == 'sap':
+        return 16 * N + 32 * X + 8 * P
+    if kernel == 'narrow':
+        return (106 + 18 * r) * P + (24 + r) * C
+    if kernel == 'solve':
+        return (32 + 7 * r) * C + (152 + 9 * r) * min(N, 2 * M)
+    return 0
+
+
+def run_batched(arrays, precision, steps, warmup, timing=True):
+    """Warm up on one batch, then time ``steps`` steps of a fresh batch from the initial state."""
+    from omi_physics_amd.batch import batch_from_arrays
+    if warmup > 0:
+        w = batch_from_arrays(arrays, precision=precision)
+        w.prepare(DT)
+        w.step(DT, warmup)
+        w.sync()
+        w.close()
+    t0 = time.perf_counter()
+    b = batch_from_arrays(arrays, precision=precision)
+    log('%s batch loaded: %.1f GB device, %.1f s' % (precision, b.device_bytes() / 1e9, time.perf_counter() - t0))
+    if timing:
+        b.set_timing(True)
+    b.prepare(DT)
+    b.sync()
+    lat = []
+    ktot = {}
+    ctot = {}
+    per_kernel_bytes = {}
+    t_start = time.perf_counter()
+    for s in range(steps):
+        t = time.perf_counter()
+        b.step(DT, 1)
+        b.sync()
+        lat.append(time.perf_counter() - t)
+        if timing:
+            kt = b.kernel_times()
+            c = b.counters()
+            for k, v in kt.items():
+                ktot[k] = ktot.get(k, 0.0) + v
+            for k, v in c.items():
+                ctot[k] = ctot.get(k, 0) + v
+            r = 4 if precision == 'fp32' else 8
+            for k in ('pre', 'sap', 'narrow', 'solve'):
+                per_kernel_bytes[k] = per_kernel_bytes.get(k, 0) + algorithmic_bytes(k, c, r)
+        if (s + 1) % 50 == 0:
+            log('  %s step %d/%d  %.2f ms' % (precision, s + 1, steps, 1e3 * lat[-1]))
+    wall = time.perf_counter() - t_start
+    return b, np.array(lat), wall, ktot, ctot, per_kernel_bytes
+
+
+def run_plain(arrays, steps, precision):
+    """The plain form: each sampled world alone on the GPU, one world at a time."""
+    from omi_physics_amd.batch import batch_from_arrays
+    body_steps = 0
+    total = 0.0
+    lat = []
+    for a in arrays:
+        b = batch_from_arrays([a], precision=precision)
+        b.prepare(DT)
+        b.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t = time.perf_counter()
+            b.step(DT, 1)
+            b.sync()
+            lat.append(time.perf_counter() - t)
+        total += time.perf_counter() - t0
+        body_steps += a['n'] * steps
+        b.close()
+    return body_steps / total, np.array(lat)
+
+
+def cpu_baseline(arrays, steps, threads):
+    """The oracle (C, FP64, one world per thread) on a sample of the worlds."""
+    import oracle
+
+    def one(a):
+        ow = oracle.OracleWorld(a)
+        ow.step(DT, steps)
+        return ow.state()['position']
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        pos = list(ex.map(one, arrays))
+    wall = time.perf_counter() - t0
+    return sum(a['n'] for a in arrays) * steps / wall, wall, pos
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=300)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--worlds', type=int, default=4096)
+    ap.add_argument('--seed', type=int, default=3)
+    ap.add_argument('--plain-sample', type=int, default=48, help='worlds stepped alone for the plain form')
+    ap.add_argument('--cpu-sample', type=int, default=192, help='worlds the CPU baseline and error check step')
+    ap.add_argument('--no-plain', action='store_true')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-fp64', action='store_true')
+    ap.add_argument('--no-timing', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
+        print('bench.py measures one GPU only: this build has no multi-GPU path (worlds would shard by '
+              'world id with no collective; see DESIGN.md).', file=sys.stderr)
+        sys.exit(2)
+    from omi_physics_amd import _ffi
+    if _ffi.load().omi_device_count() < 1:
+        print('no HIP device visible', file=sys.stderr)
+        sys.exit(1)
+
+    arrays = make_worlds(args.worlds, args.seed)
+    n_bodies = int(sum(a['n'] for a in arrays))
+    rng = np.random.default_rng(11)
+    sample = np.sort(rng.choice(len(arrays), size=min(args.cpu_sample, len(arrays)), replace=False))
+
+    b, lat, wall, ktot, ctot, kbytes = run_batched(arrays, 'fp32', args.steps, args.warmup,
+                                                   timing=not args.no_timing)
+    value = n_bodies * args.steps / wall
+    gpu_pos = {int(i): b.read(int(i))['position'] for i in sample}
+    b.close()
+    out = {
+        'metric': METRIC, 'value': value, 'unit': 'body-steps/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': 1e3 * wall / args.steps, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': '%d independent box/sphere worlds, lognormal(ln 64, 1.0) bodies clipped '
+                               '[8, 2048], batched in one graph-captured step; FP32 narrowphase + solver, '
+                               'FP64 integration/broadphase; steps 1..%d from rest' % (args.worlds, args.steps),
+                   'worlds': args.worlds, 'bodies': n_bodies, 'dt': DT, 'solver_iterations': 10,
+                   'sleep_enabled': True, 'seed': args.seed},
+        'p50_step_ms': 1e3 * float(np.percentile(lat, 50)),
+        'p99_step_ms': 1e3 * float(np.percentile(lat, 99)),
+    }
+    if ctot:
+        out['contacts_per_s'] = ctot.get('contacts', 0) / wall
+        out['totals'] = {k: int(v) for k, v in ctot.items() if k not in ('overflow', 'max_levels')}
+        out['kernels_ms_per_step'] = {k: v / args.steps for k, v in ktot.items()}
+        dom = max(('pre', 'sap', 'narrow', 'solve'), key=lambda k: ktot.get(k, 0.0))
+        avg_ms = ktot[dom] / args.steps
+        achieved = kbytes[dom] / args.steps / (avg_ms * 1e-3) / 1e9
+        out['roofline'] = {'kernel': dom, 'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+                           'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                           'avg_ms': avg_ms, 'bytes_per_launch': kbytes[dom] / args.steps}
+        out['roofline_all'] = {k: {'avg_ms': ktot[k] / args.steps,
+                                   'GBps': kbytes[k] / max(ktot[k] * 1e-3, 1e-12) / 1e9}
+                               for k in ('pre', 'sap', 'narrow', 'solve')}
+
+    if not args.no_fp64:
+        b64, lat64, wall64, *_ = run_batched(arrays, 'fp64', args.steps, 0, timing=False)
+        out['fp64_batched'] = {'value': n_bodies * args.steps / wall64, 'ms_per_step': 1e3 * wall64 / args.steps,
+                               'p99_step_ms': 1e3 * float(np.percentile(lat64, 99))}
+        fp64_pos = {int(i): b64.read(int(i))['position'] for i in sample}
+        b64.close()
+
+    if not args.no_plain:
+        ps = np.sort(rng.choice(len(arrays), size=min(args.plain_sample, len(arrays)), replace=False))
+        pv, plat = run_plain([arrays[i] for i in ps], args.steps, 'fp32')
+        out['plain'] = {'value': pv, 'unit': 'body-steps/s', 'worlds_sampled': len(ps),
+                        'p50_step_ms': 1e3 * float(np.percentile(plat, 50)),
+                        'p99_step_ms': 1e3 * float(np.percentile(plat, 99)),
+                        'batched_over_plain': value / pv}
+
+    if not args.no_cpu:
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        sub = [arrays[i] for i in sample]
+        cv, cwall, cpos = cpu_baseline(sub, args.steps, threads)
+        out['cpu_baseline'] = {'value': cv, 'unit': 'body-steps/s', 'cores': threads, 'kind': 'port',
+                               'sample': '%d of the %d worlds (%d bodies), %d steps each, one world per '
+                                         'thread, %.1f s wall' % (len(sub), len(arrays),
+                                                                   sum(a['n'] for a in sub), args.steps, cwall)}
+        err = np.array([np.abs(gpu_pos[int(i)] - p).max() for i, p in zip(sample, cpos)])
+        out['err'] = {'fp32_max_m': float(err.max()), 'fp32_median_world_max_m': float(np.median(err)),
+                      'fp32_worlds_within_1e-4': float(np.mean(err <= 1e-4)), 'worlds': len(sample),
+                      'steps': args.steps}
+        if not args.no_fp64:
+            e64 = np.array([np.abs(fp64_pos[int(i)] - p).max() for i, p in zip(sample, cpos)])
+            out['err']['fp64_max_m'] = float(e64.max())
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -160,6 +160,9 @@ int32_t omi_batch_remove_world(omi_batch *b, int32_t slot);
 int32_t omi_batch_set_state(omi_batch *b, int32_t slot, const double *pos, const double *quat,
                             const double *lv, const double *av, const uint8_t *awake,
                             const double *sleep_timer);
+/* Build the world lists and capture the step graph for dt without running it,
+ * so the first omi_batch_step does no set-up work. */
+int32_t omi_batch_prepare(omi_batch *b, double dt);
 /* Enqueue nsteps fixed steps of dt for every loaded world (asynchronous). */
 int32_t omi_batch_step(omi_batch *b, double dt, int32_t nsteps);
 /* Recompute AABBs only (asynchronous). */

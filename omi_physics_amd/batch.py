@@ -1,0 +1,195 @@
+"""Device-resident batches of independent worlds (the batched form of the step).
+
+One :class:`Batch` owns one ``omi_batch``: every loaded world's state stays in
+HBM, and :meth:`Batch.step` enqueues whole steps for all of them at once —
+per-body kernels over every body, one workgroup per world for the
+sweep-and-prune and the solver — captured in a HIP graph after the first step.
+A single world is simply a batch of one (the plain form).
+"""
+import ctypes
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import _d, _i32, _i64, _u8, check, ptr
+
+KERNELS = ('begin', 'pre', 'sap', 'narrow', 'compact', 'wake', 'solve', 'cache', 'post', 'sleep', 'toggle')
+
+
+@dataclass
+class Contact:
+    """One contact point (reference ``collide.py:37-53``): normal a -> b, depth > 0."""
+    a: int
+    b: int
+    point: np.ndarray
+    normal: np.ndarray
+    depth: float
+    normal_impulse: float = 0.0
+    tangent_impulse: np.ndarray = field(default_factory=lambda: np.zeros(2))
+
+
+def capacities_for(n_bodies: int, n_worlds: int = 1, pairs_per_body: float = 12.0,
+                   contacts_per_body: float = 24.0) -> Dict[str, int]:
+    """Pair/contact capacities that cover piles of boxes and spheres with margin."""
+    return dict(max_pairs=int(pairs_per_body * n_bodies) + 4096 * max(1, n_worlds // 64 + 1),
+                max_contacts=int(contacts_per_body * n_bodies) + 8192 * max(1, n_worlds // 64 + 1))
+
+
+class Batch:
+    """Many worlds stepped together on one GPU."""
+
+    def __init__(self, max_worlds: int, max_bodies: int, precision: str = 'fp64',
+                 max_pairs: Optional[int] = None, max_contacts: Optional[int] = None,
+                 max_hull_pts: int = 1, record_broadphase: bool = False,
+                 solver_block: int = 0, use_graph: bool = True) -> None:
+        self.lib = _ffi.load()
+        caps = capacities_for(max_bodies, max_worlds)
+        cfg = _ffi.BatchConfig()
+        cfg.max_worlds = int(max_worlds)
+        cfg.precision = _ffi.OMI_FP64 if precision in ('fp64', 64) else _ffi.OMI_FP32
+        cfg.max_bodies = int(max_bodies)
+        cfg.max_pairs = int(max_pairs or caps['max_pairs'])
+        cfg.max_contacts = int(max_contacts or caps['max_contacts'])
+        cfg.max_hull_pts = int(max(1, max_hull_pts))
+        cfg.record_broadphase = int(bool(record_broadphase))
+        cfg.solver_block = int(solver_block)
+        self.config = cfg
+        self.precision = 'fp64' if cfg.precision == 64 else 'fp32'
+        h = ctypes.c_void_p()
+        check(self.lib.omi_batch_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self.sizes: Dict[int, int] = {}
+        check(self.lib.omi_batch_use_graph(self._h, int(bool(use_graph))))
+
+    def close(self) -> None:
+        if getattr(self, '_h', None):
+            self.lib.omi_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- worlds -------------------------------------------------------------
+    def load(self, slot: int, arrays: dict) -> None:
+        """Place world ``arrays`` (``export_world_arrays`` output) in ``slot``."""
+        desc, keep = _ffi.world_desc(arrays)
+        check(self.lib.omi_batch_load_world(self._h, int(slot), ctypes.byref(desc)))
+        self.sizes[int(slot)] = int(arrays['n'])
+
+    def remove(self, slot: int) -> None:
+        check(self.lib.omi_batch_remove_world(self._h, int(slot)))
+        self.sizes.pop(int(slot), None)
+
+    def set_state(self, slot: int, position=None, orientation=None, linear_velocity=None,
+                  angular_velocity=None, awake=None, sleep_timer=None) -> None:
+        keep = [None if x is None else _ffi.c64(x) for x in (position, orientation, linear_velocity,
+                                                              angular_velocity)]
+        aw = None if awake is None else np.ascontiguousarray(awake, dtype=np.uint8)
+        tm = None if sleep_timer is None else _ffi.c64(sleep_timer)
+        check(self.lib.omi_batch_set_state(self._h, int(slot), ptr(keep[0], _d), ptr(keep[1], _d),
+                                           ptr(keep[2], _d), ptr(keep[3], _d), ptr(aw, _u8), ptr(tm, _d)))
+
+    # -- stepping -----------------------------------------------------------
+    def step(self, dt: float, steps: int = 1) -> None:
+        """Enqueue ``steps`` steps of ``dt`` for every loaded world (asynchronous)."""
+        check(self.lib.omi_batch_step(self._h, float(dt), int(steps)))
+
+    def prepare(self, dt: float) -> None:
+        """Capture the step graph for ``dt`` ahead of the first step."""
+        check(self.lib.omi_batch_prepare(self._h, float(dt)))
+
+    def refit(self, margin: float = 0.05) -> None:
+        check(self.lib.omi_batch_refit(self._h, float(margin)))
+
+    def sync(self) -> None:
+        """Wait for queued steps; raises if a step ran out of pair/contact capacity."""
+        check(self.lib.omi_batch_sync(self._h))
+
+    def use_graph(self, on: bool) -> None:
+        check(self.lib.omi_batch_use_graph(self._h, int(bool(on))))
+
+    def set_timing(self, on: bool) -> None:
+        check(self.lib.omi_batch_set_timing(self._h, int(bool(on))))
+
+    def kernel_times(self) -> Dict[str, float]:
+        """Per-stage device milliseconds of the last timed step."""
+        ms = np.zeros(len(KERNELS))
+        check(self.lib.omi_batch_kernel_times(self._h, len(KERNELS), ptr(ms, _d)))
+        return dict(zip(KERNELS, ms.tolist()))
+
+    # -- readback -----------------------------------------------------------
+    def read(self, slot: int) -> Dict[str, np.ndarray]:
+        """Copies of one world's mutable state."""
+        n = self.sizes[int(slot)]
+        out = dict(position=np.empty((n, 3)), orientation=np.empty((n, 4)),
+                   linear_velocity=np.empty((n, 3)), angular_velocity=np.empty((n, 3)),
+                   awake=np.empty(n, np.uint8), sleep_timer=np.empty(n),
+                   aabb_min=np.empty((n, 3)), aabb_max=np.empty((n, 3)),
+                   prev_position=np.empty((n, 3)), prev_orientation=np.empty((n, 4)))
+        check(self.lib.omi_batch_read_world(
+            self._h, int(slot), ptr(out['position'], _d), ptr(out['orientation'], _d),
+            ptr(out['linear_velocity'], _d), ptr(out['angular_velocity'], _d), ptr(out['awake'], _u8),
+            ptr(out['sleep_timer'], _d), ptr(out['aabb_min'], _d), ptr(out['aabb_max'], _d),
+            ptr(out['prev_position'], _d), ptr(out['prev_orientation'], _d)))
+        return out
+
+    def contacts(self, slot: int) -> Dict[str, np.ndarray]:
+        """One world's last-step contacts in solver order (world-local indices)."""
+        cnt = np.zeros(1, np.int64)
+        check(self.lib.omi_batch_read_contacts(self._h, int(slot), 0, ptr(cnt, _i64), None, None, None,
+                                               None, None, None, None))
+        K = int(cnt[0])
+        out = dict(a=np.empty(K, np.int32), b=np.empty(K, np.int32), point=np.empty((K, 3)),
+                   normal=np.empty((K, 3)), depth=np.empty(K), normal_impulse=np.empty(K),
+                   tangent_impulse=np.empty((K, 2)))
+        if K:
+            check(self.lib.omi_batch_read_contacts(
+                self._h, int(slot), K, ptr(cnt, _i64), ptr(out['a'], _i32), ptr(out['b'], _i32),
+                ptr(out['point'], _d), ptr(out['normal'], _d), ptr(out['depth'], _d),
+                ptr(out['normal_impulse'], _d), ptr(out['tangent_impulse'], _d)))
+        return out
+
+    def pairs(self, slot: int, solid: bool = True) -> np.ndarray:
+        """One world's last-step pairs ``(P, 2)`` (solid = narrowphase input)."""
+        cnt = np.zeros(1, np.int64)
+        check(self.lib.omi_batch_read_pairs(self._h, int(slot), int(solid), 0, ptr(cnt, _i64), None))
+        P = int(cnt[0])
+        out = np.empty((P, 2), np.int32)
+        if P:
+            check(self.lib.omi_batch_read_pairs(self._h, int(slot), int(solid), P, ptr(cnt, _i64),
+                                                ptr(out, _i32)))
+        return out
+
+    def counters(self) -> Dict[str, int]:
+        """Totals of the last step over all worlds."""
+        o = np.zeros(8, np.int64)
+        check(self.lib.omi_batch_counters(self._h, ptr(o, _i64)))
+        keys = ('bodies', 'x_candidates', 'broadphase_pairs', 'solid_pairs', 'manifolds', 'contacts',
+                'max_levels', 'overflow')
+        return {k: int(v) for k, v in zip(keys, o)}
+
+    def device_bytes(self) -> int:
+        return int(self.lib.omi_batch_device_bytes(self._h))
+
+
+def batch_from_arrays(worlds: Sequence[dict], precision: str = 'fp64', **kw) -> Batch:
+    """A batch holding ``worlds`` in slots 0..len-1, sized for them."""
+    total = int(sum(int(w['n']) for w in worlds))
+    hull = int(sum(len(np.asarray(w['hull_pts']).reshape(-1, 3)) for w in worlds))
+    b = Batch(max_worlds=len(worlds), max_bodies=max(total, 1), precision=precision,
+              max_hull_pts=max(hull, 1), **kw)
+    for i, w in enumerate(worlds):
+        b.load(i, w)
+    return b
+
+
+def contact_list(c: Dict[str, np.ndarray]) -> List[Contact]:
+    """Contact objects from a ``Batch.contacts`` dictionary."""
+    return [Contact(int(c['a'][k]), int(c['b'][k]), c['point'][k].copy(), c['normal'][k].copy(),
+                    float(c['depth'][k]), float(c['normal_impulse'][k]), c['tangent_impulse'][k].copy())
+            for k in range(len(c['a']))]

@@ -35,6 +35,11 @@ void launch_stage_spheres(int precision, int32_t np, int32_t mode, const double*
                           double* nrm, double* pt, double* dep, hipStream_t s);
 void launch_solve(const Dev& d, int precision, int block, SolveIn in, int32_t lds_bodies, bool stage,
                   hipStream_t s);
+size_t epa_work_bytes();
+void launch_stage_convex(int32_t np, const int32_t* ka, const double* ca, const double* Ra, const double* ha,
+                         const int32_t* oa, const int32_t* na, const int32_t* kb, const double* cb, const double* Rb,
+                         const double* hb, const int32_t* ob, const int32_t* nb, const double* hull, void* work,
+                         int32_t slots, int32_t* cnt, double* nrm, double* pt, double* dep, hipStream_t s);
 }  // namespace omi
 
 using namespace omi;
@@ -203,7 +208,17 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     A(d.mat_fric, TAB); A(d.mat_rest, TAB); A(d.filt_tab, TAB);
     A(d.pa, P); A(d.pb, P);
     A(d.bpa, cfg->record_broadphase ? P : 1); A(d.bpb, cfg->record_broadphase ? P : 1);
-    A(d.mcnt, P); A(d.mpts, 3 * 4 * P);
+    A(d.mcnt, P); A(d.mpts, 3 * 4 * P); A(d.cvx, P);
+    if (cfg->max_hull_pts > 1) {
+        // GJK/EPA workspaces: one per thread of the convex narrowphase launch
+        unsigned char* ew;
+        d.epa_slots = 4096;
+        A(ew, (int64_t)d.epa_slots * (int64_t)epa_work_bytes());
+        d.epa_work = ew;
+    } else {
+        d.epa_slots = 0;
+        d.epa_work = nullptr;
+    }
     unsigned char *mn, *md, *cn, *cd, *ci, *cs;
     A(mn, 3 * P * (int64_t)rs); A(md, 4 * P * (int64_t)rs);
     d.mnrm = mn; d.mdep = md;
@@ -475,6 +490,7 @@ static void enqueue_step(omi_batch* b, double dt) {
 extern "C" int32_t omi_batch_set_timing(omi_batch* b, int32_t on) {
     if (!b) return fail(OMI_ERR_ARG, "null batch");
     b->timing = on != 0;
+    invalidate_graph(b);
     if (b->timing && b->ev.empty()) {
         b->ev.resize(kNumKernels + 1);
         for (auto& e : b->ev) HIPCHK(hipEventCreate(&e));
@@ -503,23 +519,38 @@ extern "C" int32_t omi_batch_use_graph(omi_batch* b, int32_t on) {
     return OMI_OK;
 }
 
+// Capture the step (with its timing events when timing is on) in a HIP graph.
+static int32_t ensure_graph(omi_batch* b, double dt) {
+    if (b->graph && b->graph_dt == dt) return OMI_OK;
+    invalidate_graph(b);
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(b, dt);
+    HIPCHK(hipStreamEndCapture(b->stream, &g));
+    hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) { b->graph = nullptr; return fail(OMI_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e)); }
+    b->graph_dt = dt;
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_prepare(omi_batch* b, double dt) {
+    if (!b) return fail(OMI_ERR_ARG, "null batch");
+    if (!(dt > 0.0)) return fail(OMI_ERR_ARG, "dt must be positive");
+    int32_t rc;
+    if (b->lists_dirty && (rc = rebuild_lists(b))) return rc;
+    if (b->use_graph) return ensure_graph(b, dt);
+    return OMI_OK;
+}
+
 extern "C" int32_t omi_batch_step(omi_batch* b, double dt, int32_t nsteps) {
     if (!b) return fail(OMI_ERR_ARG, "null batch");
     if (!(dt > 0.0)) return fail(OMI_ERR_ARG, "dt must be positive");
     int32_t rc;
     if (b->lists_dirty && (rc = rebuild_lists(b))) return rc;
     for (int32_t k = 0; k < nsteps; k++) {
-        if (b->use_graph && !b->timing) {
-            if (!b->graph || b->graph_dt != dt) {
-                invalidate_graph(b);
-                hipGraph_t g;
-                HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-                enqueue_step(b, dt);
-                HIPCHK(hipStreamEndCapture(b->stream, &g));
-                HIPCHK(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
-                hipGraphDestroy(g);
-                b->graph_dt = dt;
-            }
+        if (b->use_graph) {
+            if ((rc = ensure_graph(b, dt))) return rc;
             HIPCHK(hipGraphLaunch(b->graph, b->stream));
         } else {
             enqueue_step(b, dt);
@@ -925,9 +956,46 @@ extern "C" int32_t omi_prepare_and_solve(int64_t K, int32_t N, const int64_t* a_
     return rc;
 }
 
-extern "C" int32_t omi_collide_convex(int32_t, int32_t, const int32_t*, const double*, const double*, const double*,
-                                      const int32_t*, const int32_t*, const int32_t*, const double*, const double*,
-                                      const double*, const int32_t*, const int32_t*, const double*, int64_t,
-                                      int32_t*, double*, double*, double*) {
-    return fail(OMI_ERR_STATE, "convex contacts are not built into this library yet");
+extern "C" int32_t omi_collide_convex(int32_t np, int32_t precision, const int32_t* kind_a, const double* ca,
+                                      const double* Ra, const double* ha, const int32_t* off_a, const int32_t* cnt_a,
+                                      const int32_t* kind_b, const double* cb, const double* Rb, const double* hb,
+                                      const int32_t* off_b, const int32_t* cnt_b, const double* hull_pts,
+                                      int64_t n_hull_pts, int32_t* count, double* normal, double* point,
+                                      double* depth) {
+    if (np < 0 || (precision != 64 && precision != 32)) return fail(OMI_ERR_ARG, "bad arguments");
+    int32_t rc = check_device();
+    if (rc) return rc;
+    if (np == 0) return OMI_OK;
+    for (int32_t i = 0; i < np; i++) {
+        for (int s2 = 0; s2 < 2; s2++) {
+            int k = s2 ? kind_b[i] : kind_a[i];
+            if (k != 1 && k != 2 && k != 3) return fail(OMI_ERR_ARG, "kind must be sphere, box or convex");
+            if (k == 3) {
+                int32_t o = s2 ? off_b[i] : off_a[i], c = s2 ? cnt_b[i] : cnt_a[i];
+                if (o < 0 || c < 1 || (int64_t)o + c > n_hull_pts) return fail(OMI_ERR_ARG, "hull range out of bounds");
+            }
+        }
+    }
+    DevBuf m;
+    std::vector<int32_t> zeros((size_t)np, 0);
+    int32_t *dka = m.up(kind_a, np), *dkb = m.up(kind_b, np);
+    int32_t *doa = m.up(off_a ? off_a : zeros.data(), np), *dna = m.up(cnt_a ? cnt_a : zeros.data(), np);
+    int32_t *dob = m.up(off_b ? off_b : zeros.data(), np), *dnb = m.up(cnt_b ? cnt_b : zeros.data(), np);
+    double *dca = m.up(ca, 3 * np), *dRa = m.up(Ra, 9 * np), *dha = m.up(ha, 3 * np);
+    double *dcb = m.up(cb, 3 * np), *dRb = m.up(Rb, 9 * np), *dhb = m.up(hb, 3 * np);
+    double* dh = m.up(hull_pts, 3 * std::max<int64_t>(n_hull_pts, 1));
+    const int32_t slots = std::min<int32_t>(np, 4096);
+    unsigned char* work = m.up<unsigned char>(nullptr, (int64_t)slots * (int64_t)epa_work_bytes());
+    int32_t* dc = m.up<int32_t>(nullptr, np);
+    double *dn = m.up<double>(nullptr, 3 * np), *dp = m.up<double>(nullptr, 3 * np), *dd = m.up<double>(nullptr, np);
+    if (!dd || !work) return fail(OMI_ERR_HIP, "device allocation failed");
+    launch_stage_convex(np, dka, dca, dRa, dha, doa, dna, dkb, dcb, dRb, dhb, dob, dnb, dh, work, slots, dc, dn, dp,
+                        dd, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(count, dc, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(normal, dn, sizeof(double) * 3 * np, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(point, dp, sizeof(double) * 3 * np, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(depth, dd, sizeof(double) * np, hipMemcpyDeviceToHost));
+    return OMI_OK;
 }

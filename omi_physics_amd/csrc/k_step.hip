@@ -12,6 +12,7 @@
 //   _update_sleep + _contact_islands                    -> k_sleep_*    (contacts, bodies)
 #include "omi_math.hpp"
 #include "omi_batch.hpp"
+#include "omi_gjk.hpp"
 
 namespace omi {
 
@@ -377,6 +378,13 @@ __global__ void k_narrow(Dev d) {
     R* md = (R*)d.mdep;
     for (int64_t p = gtid(); p < P; p += gstride()) {
         const int64_t a = d.pa[p], b = d.pb[p];
+        if (d.kind[a] == K_CONVEX || d.kind[b] == K_CONVEX) {
+            // queued for the GJK/EPA kernel, which writes this pair's manifold
+            d.mcnt[p] = 0;
+            uint64_t slot = atomicAdd((unsigned long long*)&d.cnt[C_CONVEX], 1ull);
+            d.cvx[slot] = (int32_t)p;
+            continue;
+        }
         R nrm[3] = {R(0), R(0), R(0)}, pts[12], dep[4];
         double org[3];
         int cnt = narrow_pair<R>(d, a, b, nrm, pts, dep, org);
@@ -386,6 +394,65 @@ __global__ void k_narrow(Dev d) {
             for (int r = 0; r < 3; r++) d.mpts[(size_t)r * 4 * d.pcap + 4 * p + m] = org[r] + (double)pts[3 * m + r];
             md[4 * p + m] = dep[m];
         }
+    }
+}
+
+__device__ __forceinline__ void make_proxy(const Dev& d, int64_t i, Proxy& P) {
+    P.kind = d.kind[i];
+    for (int r = 0; r < 3; r++) { P.c[r] = BODY(d.pos, r, i); P.h[r] = BODY(d.half, r, i); }
+    double q[4] = {BODY(d.quat, 0, i), BODY(d.quat, 1, i), BODY(d.quat, 2, i), BODY(d.quat, 3, i)};
+    quat_to_matrix(q, P.R);
+    P.local = d.hull + 3 * (size_t)d.hoff[i];
+    P.m = P.kind == K_CONVEX ? d.hcnt[i] : 0;
+}
+
+// collide.collide fallback to gjk.collide_convex (collide.py:251-269) for the
+// pairs k_narrow queued; FP64 in both precisions, one thread per pair, each
+// with its own EPA workspace.
+template <typename R>
+__global__ void k_narrow_convex(Dev d) {
+    const uint64_t tot = d.cnt[C_CONVEX];
+    const int64_t n = tot < (uint64_t)d.pcap ? (int64_t)tot : d.pcap;
+    const int64_t slot = gtid();
+    if (slot >= d.epa_slots) return;
+    EpaWork* W = (EpaWork*)d.epa_work + slot;
+    R* mn = (R*)d.mnrm;
+    R* md = (R*)d.mdep;
+    for (int64_t q = slot; q < n; q += d.epa_slots) {
+        const int64_t p = d.cvx[q];
+        Proxy A, B;
+        make_proxy(d, d.pa[p], A);
+        make_proxy(d, d.pb[p], B);
+        double nrm[3] = {0.0, 0.0, 0.0}, pt[3], dep = 0.0;
+        int cnt = collide_convex(A, B, W, nrm, pt, dep);
+        d.mcnt[p] = (int8_t)cnt;
+        for (int r = 0; r < 3; r++) mn[(size_t)r * d.pcap + p] = (R)nrm[r];
+        if (cnt) {
+            for (int r = 0; r < 3; r++) d.mpts[(size_t)r * 4 * d.pcap + 4 * p] = pt[r];
+            md[4 * p] = (R)dep;
+        }
+    }
+}
+
+__global__ void k_stage_convex(int32_t np, const int32_t* ka, const double* ca, const double* Ra, const double* ha,
+                               const int32_t* oa, const int32_t* na, const int32_t* kb, const double* cb,
+                               const double* Rb, const double* hb, const int32_t* ob, const int32_t* nb,
+                               const double* hull, EpaWork* work, int32_t slots, int32_t* cnt, double* nrm,
+                               double* pt, double* dep) {
+    const int64_t slot = gtid();
+    if (slot >= slots) return;
+    for (int64_t i = slot; i < np; i += slots) {
+        Proxy A, B;
+        A.kind = ka[i]; B.kind = kb[i];
+        for (int r = 0; r < 3; r++) { A.c[r] = ca[3 * i + r]; A.h[r] = ha[3 * i + r]; B.c[r] = cb[3 * i + r]; B.h[r] = hb[3 * i + r]; }
+        for (int r = 0; r < 9; r++) { A.R[r] = Ra[9 * i + r]; B.R[r] = Rb[9 * i + r]; }
+        A.local = hull + 3 * (size_t)(A.kind == K_CONVEX ? oa[i] : 0); A.m = A.kind == K_CONVEX ? na[i] : 0;
+        B.local = hull + 3 * (size_t)(B.kind == K_CONVEX ? ob[i] : 0); B.m = B.kind == K_CONVEX ? nb[i] : 0;
+        double n3[3] = {0.0, 0.0, 0.0}, p3[3] = {0.0, 0.0, 0.0}, dd = 0.0;
+        int c = collide_convex(A, B, work + slot, n3, p3, dd);
+        cnt[i] = c;
+        for (int r = 0; r < 3; r++) { nrm[3 * i + r] = n3[r]; pt[3 * i + r] = p3[r]; }
+        dep[i] = dd;
     }
 }
 
@@ -1033,6 +1100,21 @@ void launch_narrow(const Dev& d, int precision, hipStream_t s) {
     int g = GRID_FOR(d.pcap, 256);
     if (precision == 64) hipLaunchKernelGGL(k_narrow<double>, dim3(g), dim3(256), 0, s, d);
     else hipLaunchKernelGGL(k_narrow<float>, dim3(g), dim3(256), 0, s, d);
+    if (d.epa_slots > 0) {
+        int ge = (d.epa_slots + 63) / 64;
+        if (precision == 64) hipLaunchKernelGGL(k_narrow_convex<double>, dim3(ge), dim3(64), 0, s, d);
+        else hipLaunchKernelGGL(k_narrow_convex<float>, dim3(ge), dim3(64), 0, s, d);
+    }
+}
+
+size_t epa_work_bytes() { return sizeof(EpaWork); }
+
+void launch_stage_convex(int32_t np, const int32_t* ka, const double* ca, const double* Ra, const double* ha,
+                         const int32_t* oa, const int32_t* na, const int32_t* kb, const double* cb, const double* Rb,
+                         const double* hb, const int32_t* ob, const int32_t* nb, const double* hull, void* work,
+                         int32_t slots, int32_t* cnt, double* nrm, double* pt, double* dep, hipStream_t s) {
+    hipLaunchKernelGGL(k_stage_convex, dim3((slots + 63) / 64), dim3(64), 0, s, np, ka, ca, Ra, ha, oa, na, kb, cb,
+                       Rb, hb, ob, nb, hull, (EpaWork*)work, slots, cnt, nrm, pt, dep);
 }
 
 void launch_compact(const Dev& d, int precision, int32_t record, hipStream_t s) {

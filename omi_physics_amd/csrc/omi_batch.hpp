@@ -25,6 +25,7 @@ enum : int32_t {
     C_MANIFOLDS = 6,
     C_MAXLEVEL = 7,
     C_OVERFLOW = 8,    // bit 0 pairs, 1 contacts, 2 cache, 3 broadphase record
+    C_CONVEX = 9,      // convex pairs queued for GJK/EPA
     C_COUNT = 16
 };
 
@@ -60,6 +61,9 @@ struct Dev {
     // pairs
     int32_t *pa, *pb, *bpa, *bpb;
     int8_t *mcnt;
+    int32_t *cvx;                         // [pcap] pairs that need GJK/EPA
+    void *epa_work;                       // EpaWork per GJK/EPA thread
+    int32_t epa_slots;
     double *mpts;                         // [3][4*pcap] absolute points
     void *mnrm, *mdep;                    // R [3][pcap], R [4*pcap]
     // contacts

@@ -118,6 +118,29 @@ def pyramids_and_debris(n_pyramids: int = 20, levels: int = 10, n_debris: int = 
     return spec
 
 
+def convex_heap(n: int = 2000, seed: int = 4, spread: float = 10.0, y_range=(1.0, 30.0),
+                pts_range=(16, 32), axes_range=(0.3, 1.0)) -> Dict[str, Any]:
+    """Convex hulls over a static ground: each hull is 16-32 points on an
+    ellipsoid with semi-axes U[0.3, 1.0], cooked with the reference's hull
+    algorithm (``hull.convex_hull``), mass 1, dropped at y ~ U[1, 30]."""
+    from .hull import cook_convex
+    rng = np.random.default_rng(seed)
+    spec = new_spec()
+    _ground(spec)
+    xz = rng.uniform(-spread, spread, size=(n, 2))
+    y = rng.uniform(y_range[0], y_range[1], size=n)
+    q = _axis_angle_quats(rng, n)
+    for i in range(n):
+        m = int(rng.integers(pts_range[0], pts_range[1] + 1))
+        axes = rng.uniform(axes_range[0], axes_range[1], size=3)
+        d = rng.normal(size=(m, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        spec['shapes'].append(('convex', cook_convex(d * axes)))
+        spec['bodies'].append(dict(type='dynamic', shape=len(spec['shapes']) - 1, mass=1.0,
+                                   position=(xz[i, 0], y[i], xz[i, 1]), orientation=tuple(q[i]), material=0))
+    return spec
+
+
 def world_sizes(n_worlds: int = 4096, median: int = 64, sigma: float = 1.0,
                 lo: int = 8, hi: int = 2048, seed: int = 3) -> np.ndarray:
     """Dynamic-body count per world: clip(round(lognormal(ln median, sigma)), lo, hi)."""
@@ -174,3 +197,69 @@ def build(spec: Dict[str, Any], world_cls: Any, model: Any, **extra) -> Any:
                    position=tuple(float(x) for x in b['position']),
                    orientation=tuple(float(x) for x in b['orientation']))
     return w
+
+
+def arrays(spec: Dict[str, Any], gravity=(0.0, -9.81, 0.0), sleep_enabled: bool = True,
+           velocity_iterations: int = 10) -> Dict[str, Any]:
+    """``export_world_arrays`` of ``build(spec)`` computed directly with numpy.
+
+    Only specs of box/sphere bodies with one material and the default filter
+    (everything the generators above produce); the result is identical to
+    building a ``PhysicsWorld`` and exporting it (checked by the tests), but
+    fast enough for hundreds of thousands of bodies.
+    """
+    from .world import KIND_BOX, KIND_SPHERE
+    from . import model
+    bodies = spec['bodies']
+    n = len(bodies)
+    shapes = spec['shapes']
+    sh = np.array([b['shape'] for b in bodies], dtype=np.int64)
+    is_box = np.array([shapes[s][0] == 'box' for s in range(len(shapes))])[sh] if n else np.zeros(0, bool)
+    if any(s[0] not in ('box', 'sphere') for s in shapes):
+        raise ValueError('arrays() handles box and sphere specs only; use build()')
+    size = np.zeros((len(shapes), 3))
+    rad = np.zeros(len(shapes))
+    for i, s in enumerate(shapes):
+        if s[0] == 'box':
+            size[i] = s[1]
+        else:
+            rad[i] = s[1]
+    dyn = np.array([b['type'] == 'dynamic' for b in bodies])
+    mt = np.array([{'static': 0, 'kinematic': 1, 'dynamic': 2}[b['type']] for b in bodies], np.int32)
+    mass = np.array([max(b.get('mass', 1.0), 1e-9) for b in bodies])
+    half = np.where(is_box[:, None], 0.5 * size[sh], rad[sh][:, None])
+    x, y, z = size[sh, 0], size[sh, 1], size[sh, 2]
+    k = mass / 12.0
+    d_box = np.stack([k * (y * y + z * z), k * (x * x + z * z), k * (x * x + y * y)], axis=1)
+    v = 0.4 * mass * rad[sh] ** 2
+    d_sph = np.stack([v, v, v], axis=1)
+    diag = np.where(is_box[:, None], d_box, d_sph)
+    with np.errstate(divide='ignore'):
+        inv_inertia = np.where(diag > 0, 1.0 / np.maximum(diag, 1e-12), 0.0)
+    inv_inertia[~dyn] = 0.0
+    inv_mass = np.where(dyn, 1.0 / mass, 0.0)
+    pos = np.array([b['position'] for b in bodies], dtype='d').reshape(n, 3)
+    quat = np.array([b['orientation'] for b in bodies], dtype='d').reshape(n, 4)
+    lv = np.array([b.get('linear_velocity', (0.0, 0.0, 0.0)) for b in bodies], dtype='d').reshape(n, 3)
+    av = np.array([b.get('angular_velocity', (0.0, 0.0, 0.0)) for b in bodies], dtype='d').reshape(n, 3)
+    mats = spec['materials']
+    if len(mats) != 1 or any(b.get('material', 0) != 0 for b in bodies):
+        raise ValueError('arrays() handles single-material specs only; use build()')
+    m0 = model.Material(**mats[0])
+    fr = model.combine(m0.dynamicFriction, m0.dynamicFriction, m0.frictionCombine)
+    rs = model.combine(m0.restitution, m0.restitution, m0.restitutionCombine)
+    zeros = np.zeros(n)
+    g = np.asarray(gravity, dtype='d')
+    return dict(
+        n=n, position=pos, orientation=quat, prev_position=pos.copy(), prev_orientation=quat.copy(),
+        linear_velocity=lv, angular_velocity=av, inv_mass=inv_mass, inv_inertia=inv_inertia,
+        gravity_factor=np.ones(n), linear_damping=zeros, angular_damping=zeros.copy(),
+        quadratic_drag=zeros.copy(), sleep_timer=zeros.copy(), motion_type=mt,
+        collider_shape=sh.astype(np.int32), trigger_shape=np.full(n, -1, np.int32),
+        awake=np.ones(n, np.uint8), kind=np.where(is_box, KIND_BOX, KIND_SPHERE).astype(np.int32),
+        half=np.ascontiguousarray(half), hull_off=np.full(n, -1, np.int32), hull_cnt=np.zeros(n, np.int32),
+        hull_pts=np.zeros((0, 3)), mat_slot=np.zeros(n, np.int32), n_mat=1,
+        friction_table=np.array([[fr]]), restitution_table=np.array([[rs]]),
+        filt_slot=np.zeros(n, np.int32), n_filt=1, filter_table=np.ones((1, 1), np.uint8),
+        gravity=g, default_linear_damping=0.0, default_angular_damping=0.0,
+        sleep_enabled=int(sleep_enabled), velocity_iterations=int(velocity_iterations), has_collision=1)

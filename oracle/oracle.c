@@ -568,7 +568,7 @@ static int gjk(const Proxy *A, const Proxy *B, Vtx *s, int *n)
 }
 
 #define EPA_MAXV (4 + 64 + 1)
-#define EPA_MAXF 512
+#define EPA_MAXF 256
 
 /* gjk._closest_face (gjk.py:176-194) */
 static int closest_face(const Vtx *v, const int (*f)[3], int nf, double *bn, double *bd)

@@ -1,0 +1,49 @@
+"""Shared scene builders for the tests (the same specs the golden script used)."""
+import numpy as np
+
+from omi_physics_amd import PhysicsWorld, model, scenes, mathutil
+
+
+def traj_spec(name):
+    """The small scenes of tests/golden/traj_*.npz (see make_golden.py)."""
+    if name == 'dropbox':
+        spec = scenes.new_spec(); scenes._ground(spec); spec['shapes'].append(('box', (1.0, 1.0, 1.0)))
+        spec['bodies'].append(dict(type='dynamic', shape=1, mass=1.0, position=(0.3, 3.0, -0.2),
+                                   orientation=tuple(mathutil.quat_from_axis_angle((1, 2, 3), 0.7)), material=0))
+    elif name == 'stack5':
+        spec = scenes.new_spec(); scenes._ground(spec); spec['shapes'].append(('box', (1.0, 1.0, 1.0)))
+        for k in range(5):
+            spec['bodies'].append(dict(type='dynamic', shape=1, mass=1.0, position=(0.0, 0.5 + k * 1.001, 0.0),
+                                       orientation=(0.0, 0.0, 0.0, 1.0), material=0))
+    elif name == 'rolling':
+        spec = scenes.new_spec(friction=0.3); scenes._ground(spec); spec['shapes'].append(('sphere', 0.5))
+        spec['bodies'].append(dict(type='dynamic', shape=1, mass=2.0, position=(-3.0, 0.6, 0.0),
+                                   orientation=(0.0, 0.0, 0.0, 1.0), material=0, linear_velocity=(2.0, 0.0, 0.5)))
+    elif name == 'mixed24':
+        spec = scenes.boxes_on_ground(24, seed=11, spread=2.0, y_range=(1.0, 8.0), sphere_fraction=0.5)
+    else:
+        raise KeyError(name)
+    return spec
+
+
+def world(spec, **kw):
+    return scenes.build(spec, PhysicsWorld, model, **kw)
+
+
+def sap_filter_table(filt):
+    """Slots and table for the filters used by tests/golden/sap.npz scenes 2-3:
+    0 = systems ('a',) excluding ('b',); 1 = ('b',); 2 = ('c',) colliding only
+    with ('c',); -1 = the default filter."""
+    filters = [model.CollisionFilter(collisionSystems=('a',), notCollideWithSystems=('b',)),
+               model.CollisionFilter(collisionSystems=('b',)),
+               model.CollisionFilter(collisionSystems=('c',), collideWithSystems=('c',))]
+    def get(i):
+        return filters[i] if 0 <= i < len(filters) else model.DEFAULT_FILTER
+    vals, slots = np.unique(filt, return_inverse=True)
+    tab = np.array([[get(int(a)).collides_with(get(int(b))) for b in vals] for a in vals], np.uint8)
+    return slots.astype(np.int32), tab
+
+
+def pile(n=120, seed=3, spheres=0.5, spread=2.0, y=(0.6, 10.0)):
+    """A dense pile that produces many contacts within a few dozen steps."""
+    return scenes.boxes_on_ground(n, seed=seed, spread=spread, y_range=y, sphere_fraction=spheres)

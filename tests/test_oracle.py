@@ -1,0 +1,188 @@
+"""The oracle pinned against the reference's golden vectors, closed forms and brute force (CPU)."""
+import numpy as np
+import pytest
+
+import oracle
+from tests import helpers
+
+
+def test_refit_bit_exact_with_reference(golden):
+    g = golden('refit.npz')
+    lo, hi = oracle.refit(g['pos'], g['quat'], g['kind'], g['half'])
+    assert np.array_equal(lo, g['lo']) and np.array_equal(hi, g['hi'])
+
+
+def test_refit_box_closed_form():
+    # axis-aligned box: AABB = centre ± half ± margin; 90° about z swaps x/y extents
+    q = np.array([[0, 0, 0, 1.0], [0, 0, np.sin(np.pi / 4), np.cos(np.pi / 4)]])
+    pos = np.array([[1.0, 2.0, 3.0], [0.0, 0.0, 0.0]])
+    lo, hi = oracle.refit(pos, q, [2, 2], [[0.5, 1.0, 1.5], [0.5, 1.0, 1.5]], margin=0.05)
+    assert np.allclose(lo[0], [0.45, 0.95, 1.45]) and np.allclose(hi[0], [1.55, 3.05, 4.55])
+    assert np.allclose(hi[1], [1.05, 0.55, 1.55], atol=1e-12)
+
+
+@pytest.mark.parametrize('scene', range(4))
+def test_sap_order_matches_reference(golden, scene):
+    g = golden('sap.npz')
+    filt = g['filt%d' % scene]
+    slots, tab = helpers.sap_filter_table(filt)
+    pairs, _ = oracle.sap_pairs(g['lo%d' % scene], g['hi%d' % scene], g['mt%d' % scene], slots, tab)
+    assert np.array_equal(pairs, g['pairs%d' % scene])
+
+
+def test_sap_equals_brute_force():
+    rng = np.random.default_rng(1)
+    n = 200
+    c = rng.uniform(-3, 3, size=(n, 3)); h = rng.uniform(0.1, 0.6, size=(n, 3))
+    lo, hi = c - h, c + h
+    mt = rng.choice([0, 1, 2], size=n, p=[0.2, 0.1, 0.7]).astype(np.int32)
+    pairs, _ = oracle.sap_pairs(lo, hi, mt)
+    brute = {(i, j) for i in range(n) for j in range(i + 1, n)
+             if (mt[i] == 2 or mt[j] == 2) and np.all(lo[i] <= hi[j]) and np.all(lo[j] <= hi[i])}
+    assert set(map(tuple, pairs.tolist())) == brute
+    assert len(pairs) == len(brute)
+
+
+def test_box_box_matches_reference(golden):
+    g = golden('boxbox.npz')
+    for i in range(len(g['ca'])):
+        n, nrm, pts, dep = oracle.box_box(g['ca'][i], g['Ra'][i], g['ha'][i], g['cb'][i], g['Rb'][i], g['hb'][i])
+        assert n == g['count'][i]
+        if n:
+            np.testing.assert_allclose(nrm, g['normal'][i], atol=1e-12)
+            np.testing.assert_allclose(pts, g['points'][i][:n], atol=1e-12)
+            np.testing.assert_allclose(dep, g['depths'][i][:n], atol=1e-12)
+
+
+def test_box_box_closed_forms():
+    I = np.eye(3)
+    # reference test_collide_primitives.py:53-60: overlap 0.5 on x
+    n, nrm, pts, dep = oracle.box_box([0, 0, 0], I, [1, 1, 1], [1.5, 0, 0], I, [1, 1, 1])
+    assert n >= 1 and np.allclose(np.abs(nrm), [1, 0, 0]) and abs(dep.max() - 0.5) < 1e-12
+    # resting crate: four face contacts, depth 0.01, normal +y
+    n, nrm, pts, dep = oracle.box_box([0, 0, 0], I, [5, 0.5, 5], [0, 0.99, 0], I, [0.5, 0.5, 0.5])
+    assert n == 4 and np.allclose(nrm, [0, 1, 0]) and np.allclose(dep, 0.01)
+    assert np.allclose(pts[:, 1], 0.5)        # seated on the reference face
+    n, *_ = oracle.box_box([0, 0, 0], I, [1, 1, 1], [3, 0, 0], I, [1, 1, 1])
+    assert n == 0
+
+
+def test_spheres_match_reference(golden):
+    g = golden('spheres.npz')
+    for i in range(len(g['sa'])):
+        n, nrm, pt, dep = oracle.sphere_sphere(g['sa'][i], g['ra'][i], g['sb'][i], g['rb'][i])
+        e = g['ss'][i]
+        assert n == int(e[0])
+        if n:
+            np.testing.assert_allclose(np.r_[nrm, pt[0], dep], e[1:], atol=1e-12)
+        n, nrm, pt, dep = oracle.sphere_box(g['sc'][i], g['sr'][i], g['bc'][i], g['bR'][i], g['bh'][i])
+        e = g['sbx'][i]
+        assert n == int(e[0])
+        if n:
+            np.testing.assert_allclose(np.r_[nrm, pt[0], dep], e[1:], atol=1e-12)
+
+
+def test_sphere_closed_forms():
+    n, nrm, pt, dep = oracle.sphere_sphere([0, 0, 0], 1.0, [1.5, 0, 0], 1.0)
+    assert n == 1 and np.allclose(nrm, [1, 0, 0]) and abs(dep[0] - 0.5) < 1e-15 and abs(pt[0][0] - 0.75) < 1e-15
+    n, nrm, pt, dep = oracle.sphere_box([0, 1.3, 0], 0.5, [0, 0, 0], np.eye(3), [1, 1, 1])
+    assert n == 1 and np.allclose(nrm, [0, -1, 0]) and abs(dep[0] - 0.2) < 1e-12
+
+
+def _hull(g, i):
+    return g['hull_pts'][g['hull_off'][i]:g['hull_off'][i + 1]]
+
+
+def test_gjk_epa_matches_reference(golden):
+    g = golden('gjk.npz')
+    for k in range(len(g['idx'])):
+        i, j = g['idx'][k]
+        kb, hb = (2, np.array([0.8, 0.3, 0.6])) if k % 10 == 9 else (3, None)
+        args = (3, g['pos'][k, 0], g['R'][k, 0], None, _hull(g, i), kb, g['pos'][k, 1], g['R'][k, 1], hb, _hull(g, j))
+        assert oracle.gjk_intersect(*args) == bool(g['res'][k, 0])
+        n, nrm, pt, dep = oracle.collide_convex(*args)
+        e = g['res'][k]
+        assert n == int(e[1])
+        if n:
+            np.testing.assert_allclose(nrm, e[2:5], atol=1e-9)
+            np.testing.assert_allclose(pt, e[5:8], atol=1e-9)
+            assert abs(dep - e[8]) < 1e-9
+
+
+def test_epa_closed_form():
+    # reference test_gjk_epa.py:31-38: unit cubes overlapping 0.3 on x
+    cube = np.array([(sx, sy, sz) for sx in (-.5, .5) for sy in (-.5, .5) for sz in (-.5, .5)])
+    n, nrm, pt, dep = oracle.collide_convex(3, [0, 0, 0], np.eye(3), None, cube, 3, [0.7, 0, 0], np.eye(3), None, cube)
+    assert n == 1 and abs(dep - 0.3) < 1e-9 and np.allclose(np.abs(nrm), [1, 0, 0], atol=1e-9)
+
+
+def test_solver_matches_reference(golden):
+    g = golden('solver.npz')
+    for isl in range(12):
+        pos = g['pos_in%d' % isl].copy(); lv = g['lv_in%d' % isl].copy(); av = g['av_in%d' % isl].copy()
+        ni = g['seedn%d' % isl].copy(); ti = g['seedt%d' % isl].copy()
+        oracle.prepare_and_solve(g['a%d' % isl], g['b%d' % isl], g['point%d' % isl], g['normal%d' % isl],
+                                 g['depth%d' % isl], pos, lv, av, g['inv_mass_in%d' % isl], g['invIw%d' % isl],
+                                 g['rest%d' % isl], g['fric%d' % isl], ni, ti)
+        for got, key in ((pos, 'pos_out'), (lv, 'lv_out'), (av, 'av_out'), (ni, 'nimp_out'), (ti, 'timp_out')):
+            np.testing.assert_allclose(got, g['%s%d' % (key, isl)], atol=1e-11, rtol=0)
+
+
+def test_solver_single_contact_closed_form():
+    # a unit-mass sphere resting on static ground: one normal impulse = m g dt
+    g_dt = 9.81 / 60.0
+    pos = np.array([[0, 0, 0], [0, 0.5, 0]], float)
+    lv = np.array([[0, 0, 0], [0, -g_dt, 0]], float); av = np.zeros((2, 3))
+    invIw = np.zeros((2, 3, 3)); invIw[1] = np.eye(3) * 10.0
+    ni = np.zeros(1); ti = np.zeros((1, 2))
+    oracle.prepare_and_solve([0], [1], [[0, 0, 0]], [[0, 1, 0]], [0.0], pos, lv, av, [0.0, 1.0], invIw,
+                             [0.0], [0.5], ni, ti, warm_start=False)
+    assert abs(ni[0] - g_dt) < 1e-12 and abs(lv[1, 1]) < 1e-12 and np.all(lv[0] == 0)
+
+
+@pytest.mark.parametrize('name,tol', [('dropbox', 1e-12), ('stack5', 1e-9), ('rolling', 1e-12), ('mixed24', 1e-9)])
+def test_trajectory_matches_reference(golden, name, tol):
+    g = golden('traj_%s.npz' % name)
+    ow = oracle.OracleWorld(helpers.world(helpers.traj_spec(name)).export_arrays())
+    s = 0
+    for idx, st in enumerate(g['steps']):
+        ow.step(1.0 / 60.0, int(st) - s); s = int(st)
+        S = ow.state()
+        np.testing.assert_allclose(S['position'], g['pos'][idx], atol=tol, rtol=0)
+        np.testing.assert_allclose(S['linear_velocity'], g['lv'][idx], atol=100 * tol, rtol=0)
+        assert np.array_equal(S['awake'].astype(bool), g['awake'][idx])
+        assert ow.counters()['contacts'] == g['ncontacts'][idx]
+
+
+def test_freefall_closed_form():
+    # reference test_physics_integrate.py:16-26: x_n = -g dt² n(n+1)/2
+    from omi_physics_amd import PhysicsWorld, model
+    w = PhysicsWorld(gravity=model.Gravity(gravity=9.81, direction=(0, -1, 0)))
+    w.add_body(model.Motion(type=model.DYNAMIC), position=(0, 0, 0))
+    ow = oracle.OracleWorld(w.export_arrays())
+    dt, n = 1.0 / 240.0, 240
+    ow.step(dt, n)
+    assert ow.state()['position'][0, 1] == pytest.approx(-0.5 * 9.81 * dt * dt * n * (n + 1), rel=1e-9)
+
+
+def test_sleep_settles_a_row_of_boxes():
+    # reference test_physics_benchmark.py:61-74
+    from omi_physics_amd import PhysicsWorld, model
+    w = PhysicsWorld(gravity=model.Gravity(gravity=9.81, direction=(0, -1, 0)))
+    box = w.add_shape(model.Shape.box((1, 1, 1)))
+    w.add_body(model.Motion(type=model.STATIC), collider=model.Collider(shape=box), position=(0, -0.5, 0))
+    for k in range(20):
+        w.add_body(model.Motion(type=model.DYNAMIC), collider=model.Collider(shape=box), position=(k * 2.0, 0.5, 0))
+    ow = oracle.OracleWorld(w.export_arrays())
+    ow.step(1 / 120, 240)
+    assert int(ow.state()['awake'][1:].sum()) < 20
+
+
+def test_mutation_is_caught():
+    """A plausible mistake planted in the oracle's inputs (a flipped normal) fails the golden check."""
+    g = np.load(__import__('os').path.join(__import__('tests.conftest', fromlist=['GOLDEN']).GOLDEN, 'solver.npz'))
+    pos = g['pos_in0'].copy(); lv = g['lv_in0'].copy(); av = g['av_in0'].copy()
+    ni = g['seedn0'].copy(); ti = g['seedt0'].copy()
+    oracle.prepare_and_solve(g['a0'], g['b0'], g['point0'], -g['normal0'], g['depth0'], pos, lv, av,
+                             g['inv_mass_in0'], g['invIw0'], g['rest0'], g['fric0'], ni, ti)
+    assert np.abs(lv - g['lv_out0']).max() > 1e-3
