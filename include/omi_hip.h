@@ -43,6 +43,9 @@ const char *omi_last_error(void);
 int32_t omi_abi_version(void);
 /* Number of visible HIP devices (0 on a host without a GPU). */
 int32_t omi_device_count(void);
+/* Free and total device memory of the current device, in bytes (the serving
+ * loop sizes its arena from it). */
+int32_t omi_device_memory(int64_t *free_bytes, int64_t *total_bytes);
 
 /* ---- 1. stage-level entry points ------------------------------------- */
 
@@ -146,6 +149,11 @@ typedef struct {
     int64_t max_hull_pts;
     int32_t record_broadphase;  /* also keep the unfiltered broadphase pair list */
     int32_t solver_block;       /* threads per world in the solver launch (64..1024, 0 = auto) */
+    /* Per size class (worlds of <= 256, <= 1024, <= 4096 bodies, larger):
+     * threads per world for the broadphase and the solver launches, one of
+     * 64, 128, 256, 512, 1024; 0 picks the default.  The autotuner sets these. */
+    int32_t sap_block[4];
+    int32_t solve_block[4];
 } omi_batch_config;
 
 typedef struct omi_batch omi_batch;

@@ -53,12 +53,13 @@ class BatchConfig(ctypes.Structure):
         ('max_bodies', ctypes.c_int64), ('max_pairs', ctypes.c_int64),
         ('max_contacts', ctypes.c_int64), ('max_hull_pts', ctypes.c_int64),
         ('record_broadphase', ctypes.c_int32), ('solver_block', ctypes.c_int32),
+        ('sap_block', ctypes.c_int32 * 4), ('solve_block', ctypes.c_int32 * 4),
     ]
 
 
 #: every symbol include/omi_hip.h declares
 EXPORTS = (
-    'omi_last_error', 'omi_abi_version', 'omi_device_count',
+    'omi_last_error', 'omi_abi_version', 'omi_device_count', 'omi_device_memory',
     'omi_refit', 'omi_sap_pairs', 'omi_box_box', 'omi_sphere_sphere', 'omi_sphere_box',
     'omi_collide_convex', 'omi_prepare_and_solve',
     'omi_batch_create', 'omi_batch_destroy', 'omi_batch_load_world', 'omi_batch_remove_world',
@@ -90,6 +91,7 @@ def load() -> ctypes.CDLL:
         'omi_last_error': (ctypes.c_char_p, []),
         'omi_abi_version': (ctypes.c_int32, []),
         'omi_device_count': (ctypes.c_int32, []),
+        'omi_device_memory': (ctypes.c_int32, [_i64, _i64]),
         'omi_refit': (ctypes.c_int32, [ctypes.c_int32, _d, _d, _i32, _d, _i32, _i32, _d, ctypes.c_int64,
                                        ctypes.c_double, _d, _d]),
         'omi_sap_pairs': (ctypes.c_int32, [ctypes.c_int32, _d, _d, _i32, _i32, ctypes.c_int32, _u8, _i32,

@@ -43,7 +43,9 @@ class Batch:
     def __init__(self, max_worlds: int, max_bodies: int, precision: str = 'fp64',
                  max_pairs: Optional[int] = None, max_contacts: Optional[int] = None,
                  max_hull_pts: int = 1, record_broadphase: bool = False,
-                 solver_block: int = 0, use_graph: bool = True) -> None:
+                 solver_block: int = 0, use_graph: bool = True,
+                 sap_block: Optional[Sequence[int]] = None,
+                 solve_block: Optional[Sequence[int]] = None) -> None:
         self.lib = _ffi.load()
         caps = capacities_for(max_bodies, max_worlds)
         cfg = _ffi.BatchConfig()
@@ -55,6 +57,11 @@ class Batch:
         cfg.max_hull_pts = int(max(1, max_hull_pts))
         cfg.record_broadphase = int(bool(record_broadphase))
         cfg.solver_block = int(solver_block)
+        from .autotune import load_tuned
+        tuned = load_tuned()
+        for c in range(4):
+            cfg.sap_block[c] = int((sap_block or tuned['sap_block'])[c])
+            cfg.solve_block[c] = int((solve_block or tuned['solve_block'])[c])
         self.config = cfg
         self.precision = 'fp64' if cfg.precision == 64 else 'fp32'
         h = ctypes.c_void_p()

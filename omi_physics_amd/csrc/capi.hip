@@ -18,8 +18,8 @@ namespace omi {
 void launch_begin(const Dev& d, hipStream_t s);
 void launch_pre(const Dev& d, int precision, int64_t nhw, double dt, double margin, hipStream_t s);
 void launch_refit(const Dev& d, int64_t nhw, double margin, hipStream_t s);
-void launch_sap(const Dev& d, const int32_t* wl_small, int32_t n_small, int32_t np_small, const int32_t* wl_big,
-                int32_t n_big, int32_t record, hipStream_t s);
+void launch_sap(const Dev& d, const int32_t* wl, const int32_t* wn, const int32_t* np_class, const int32_t* blocks,
+                const int32_t* live, int32_t record, hipStream_t s);
 void launch_narrow(const Dev& d, int precision, hipStream_t s);
 void launch_compact(const Dev& d, int precision, int32_t record, hipStream_t s);
 void launch_wake(const Dev& d, hipStream_t s);
@@ -33,8 +33,8 @@ void launch_stage_boxbox(int precision, int32_t np, const double* ca, const doub
 void launch_stage_spheres(int precision, int32_t np, int32_t mode, const double* ca, const double* ra,
                           const double* cb, const double* Rb, const double* hb, const double* rb, int32_t* cnt,
                           double* nrm, double* pt, double* dep, hipStream_t s);
-void launch_solve(const Dev& d, int precision, int block, SolveIn in, int32_t lds_bodies, bool stage,
-                  hipStream_t s);
+void launch_solve(const Dev& d, int precision, const int32_t* blocks, SolveIn in, const int32_t* wl,
+                  const int32_t* wn, const int32_t* lds_bodies, const int32_t* live, bool stage, hipStream_t s);
 size_t epa_work_bytes();
 void launch_stage_convex(int32_t np, const int32_t* ka, const double* ca, const double* Ra, const double* ha,
                          const int32_t* oa, const int32_t* na, const int32_t* kb, const double* cb, const double* Rb,
@@ -110,9 +110,14 @@ struct omi_batch {
     int64_t bytes = 0;
     Ranges bodies, mats, filts, hulls;
     std::vector<WorldSlot> slots;
-    int32_t* d_wl_small = nullptr;
-    int32_t* d_wl_big = nullptr;
-    int32_t n_small = 0, n_big = 0, np_small = 0, lds_bodies = 0, np_lds_max = 4096;
+    int32_t* d_wl = nullptr;               // [4][max_worlds] world slots per size class
+    int32_t* d_wn = nullptr;               // [4] class counts
+    int32_t np_class[4] = {256, 1024, 4096, 0};
+    int32_t lds_bodies[4] = {256, 1024, 4096, 12288};
+    int32_t live[4] = {0, 0, 0, 0};        // classes with a launch in the step
+    int32_t sap_block[4] = {64, 128, 256, 1024};
+    int32_t solve_block[4] = {64, 128, 256, 256};
+    int32_t np_lds_max = 4096;
     bool lists_dirty = true;
     int64_t nhw = 0;
     bool use_graph = true;
@@ -142,6 +147,15 @@ extern "C" int32_t omi_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+extern "C" int32_t omi_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(OMI_ERR_ARG, "null argument");
+    size_t f = 0, t = 0;
+    HIPCHK(hipMemGetInfo(&f, &t));
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
+    return OMI_OK;
 }
 
 static int32_t check_device() {
@@ -183,8 +197,15 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     int smem = 65536;
     hipDeviceGetAttribute(&smem, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     b->np_lds_max = smem >= 4096 * 16 + 1024 ? 4096 : 2048;
-    b->lds_bodies = 0;
+    b->np_class[2] = b->np_lds_max;
+    b->lds_bodies[2] = b->np_lds_max;
     b->solver_block = cfg->solver_block > 0 ? cfg->solver_block : 256;
+    auto okb = [](int32_t x) { return x == 64 || x == 128 || x == 256 || x == 512 || x == 1024; };
+    for (int c = 0; c < 4; c++) {
+        if (okb(cfg->sap_block[c])) b->sap_block[c] = cfg->sap_block[c];
+        if (okb(cfg->solve_block[c])) b->solve_block[c] = cfg->solve_block[c];
+        else if (cfg->solver_block > 0 && okb(cfg->solver_block)) b->solve_block[c] = cfg->solver_block;
+    }
 #define A(p, n) do { if ((rc = b->alloc(&(p), (n))) != OMI_OK) { free_all(b); delete b; return rc; } } while (0)
     A(d.pos, 3 * N); A(d.quat, 4 * N); A(d.ppos, 3 * N); A(d.pquat, 4 * N);
     A(d.lv, 6 * N); d.av = d.lv + 3 * N;
@@ -233,7 +254,7 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
         HIPCHK(hipMemset(d.hkey[q], 0xff, sizeof(uint64_t) * H));
     }
     A(d.sel, 1); A(d.cnt, C_COUNT);
-    A(b->d_wl_small, W); A(b->d_wl_big, W);
+    A(b->d_wl, 4 * W); A(b->d_wn, 4);
 #undef A
     // every body slot starts free (world = -1)
     {
@@ -440,28 +461,30 @@ extern "C" int32_t omi_batch_set_state(omi_batch* b, int32_t slot, const double*
     return OMI_OK;
 }
 
+// World lists per size class.  Lists live in device memory and every class
+// that has ever held a world keeps its launch, so loading or removing worlds
+// does not invalidate a captured step graph unless a new class appears.
 static int32_t rebuild_lists(omi_batch* b) {
-    std::vector<int32_t> small, big;
-    int32_t np_small = 2, maxn = 0;
+    std::vector<int32_t> lists[4];
     for (int32_t w = 0; w < b->cfg.max_worlds; w++) {
         const WorldSlot& s = b->slots[w];
         if (!s.live) continue;
         int32_t NP = 1;
         while (NP < s.n) NP <<= 1;
-        if (NP <= b->np_lds_max) { small.push_back(w); np_small = std::max(np_small, NP); }
-        else big.push_back(w);
-        maxn = std::max<int32_t>(maxn, (int32_t)s.n);
+        int c = NP <= b->np_class[0] ? 0 : NP <= b->np_class[1] ? 1 : NP <= b->np_class[2] ? 2 : 3;
+        lists[c].push_back(w);
     }
-    int32_t rc;
-    if ((rc = put(b, b->d_wl_small, small.data(), (int64_t)small.size()))) return rc;
-    if ((rc = put(b, b->d_wl_big, big.data(), (int64_t)big.size()))) return rc;
-    b->n_small = (int32_t)small.size();
-    b->n_big = (int32_t)big.size();
-    b->np_small = np_small;
-    b->lds_bodies = std::min<int32_t>(maxn, 12288);
-    b->nhw = b->bodies.high_water();
+    int32_t rc, counts[4];
+    bool grew = false;
+    for (int c = 0; c < 4; c++) {
+        counts[c] = (int32_t)lists[c].size();
+        if ((rc = put(b, b->d_wl + (size_t)c * b->cfg.max_worlds, lists[c].data(), counts[c]))) return rc;
+        if (counts[c] > 0 && !b->live[c]) { b->live[c] = 1; grew = true; }
+    }
+    if ((rc = put(b, b->d_wn, counts, 4))) return rc;
+    b->nhw = b->d.ncap;
     b->lists_dirty = false;
-    invalidate_graph(b);
+    if (grew) invalidate_graph(b);
     return OMI_OK;
 }
 
@@ -475,12 +498,12 @@ static void enqueue_step(omi_batch* b, double dt) {
     mark();
     launch_begin(d, s); mark();
     launch_pre(d, prec, b->nhw, dt, 0.05, s); mark();
-    launch_sap(d, b->d_wl_small, b->n_small, b->np_small, b->d_wl_big, b->n_big, b->cfg.record_broadphase, s); mark();
+    launch_sap(d, b->d_wl, b->d_wn, b->np_class, b->sap_block, b->live, b->cfg.record_broadphase, s); mark();
     launch_narrow(d, prec, s); mark();
     launch_compact(d, prec, b->cfg.record_broadphase, s); mark();
     launch_wake(d, s); mark();
     SolveIn in{nullptr, nullptr, 0.5, 0.8, 0.005, 1};
-    launch_solve(d, prec, b->solver_block, in, b->lds_bodies, false, s); mark();
+    launch_solve(d, prec, b->solve_block, in, b->d_wl, b->d_wn, b->lds_bodies, b->live, false, s); mark();
     launch_cache(d, prec, s); mark();
     launch_post(d, b->nhw, dt, s); mark();
     launch_sleep(d, b->nhw, dt, s); mark();
@@ -853,14 +876,12 @@ extern "C" int32_t omi_sap_pairs(int32_t n, const double* lo, const double* hi, 
         if (!rc) rc = put(b, d.w_cnt, &nn, 1);
         if (!rc) rc = put(b, d.w_filt_off, &zero, 1);
         if (!rc) rc = put(b, d.w_nfilt, &nf, 1);
-        if (!rc) rc = put(b, b->d_wl_small, &wl, 1);
-        if (!rc) rc = put(b, b->d_wl_big, &wl, 1);
         if (rc) { omi_batch_destroy(b); return rc; }
-        int32_t NP = 1;
-        while (NP < n) NP <<= 1;
+        b->slots[0].live = true;
+        b->slots[0].n = n;
+        if ((rc = rebuild_lists(b))) { omi_batch_destroy(b); return rc; }
         launch_begin(d, b->stream);
-        if (NP <= b->np_lds_max) launch_sap(d, b->d_wl_small, 1, NP, b->d_wl_big, 0, 1, b->stream);
-        else launch_sap(d, b->d_wl_small, 0, 2, b->d_wl_big, 1, 1, b->stream);
+        launch_sap(d, b->d_wl, b->d_wn, b->np_class, b->sap_block, b->live, 1, b->stream);
         if (hipGetLastError() != hipSuccess) { omi_batch_destroy(b); return fail(OMI_ERR_HIP, "broadphase launch failed"); }
         uint64_t c[C_COUNT];
         hipStreamSynchronize(b->stream);
@@ -933,8 +954,11 @@ extern "C" int32_t omi_prepare_and_solve(int64_t K, int32_t N, const int64_t* a_
         !(rc = put(b, d.w_cnt, &Ni, 1)) && !(rc = put(b, d.w_cs, &zero, 1)) && !(rc = put(b, d.w_cn, &Ki, 1)) &&
         !(rc = put(b, d.w_iters, &iters, 1)) && !(rc = put(b, d.w_flags, &flags, 1))) {
         SolveIn in{drest, dfric, rthr, correction, slop, warm};
-        launch_solve(d, precision, 256, in, std::min<int32_t>(N, 12288), true, b->stream);
-        if (hipGetLastError() != hipSuccess) rc = fail(OMI_ERR_HIP, "solver launch failed");
+        b->slots[0].live = true;
+        b->slots[0].n = N;
+        if (!(rc = rebuild_lists(b)))
+            launch_solve(d, precision, b->solve_block, in, b->d_wl, b->d_wn, b->lds_bodies, b->live, true, b->stream);
+        if (!rc && hipGetLastError() != hipSuccess) rc = fail(OMI_ERR_HIP, "solver launch failed");
         if (!rc) rc = download_soa(b, pos, d.pos, d.ncap, 0, N, 3);
         if (!rc) rc = download_soa(b, lv, d.lv, d.ncap, 0, N, 3);
         if (!rc) rc = download_soa(b, av, d.av, d.ncap, 0, N, 3);

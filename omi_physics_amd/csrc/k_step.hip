@@ -224,13 +224,13 @@ __device__ __forceinline__ void sap_eval(const Dev& d, int64_t a, int64_t b, int
 // block so one long window (a ground box) does not serialise a thread, then
 // an order-preserving compaction of the kept pairs into this world's slice.
 template <int BLOCK, bool GLOBAL>
-__global__ __launch_bounds__(BLOCK) void k_sap(Dev d, const int32_t* wlist, int32_t nw, int32_t np_max,
+__global__ __launch_bounds__(BLOCK) void k_sap(Dev d, const int32_t* wlist, const int32_t* wcount, int32_t np_max,
                                                int32_t record_bp) {
     extern __shared__ unsigned char smem[];
     __shared__ int32_t s_scan[BLOCK / 64 + 2];
     __shared__ int64_t s_sum[BLOCK / 64 + 2];
     __shared__ int32_t s_base[4];
-    if ((int32_t)blockIdx.x >= nw) return;
+    if ((int32_t)blockIdx.x >= *wcount) return;
     const int32_t w = wlist[blockIdx.x];
     const int tid = threadIdx.x;
     const int32_t flags = d.w_flags[w], n = d.w_cnt[w];
@@ -644,11 +644,13 @@ __device__ __forceinline__ void seed_lookup(const Dev& d, int32_t w, int32_t old
 // would.  Static/kinematic bodies are read-only (their inverse mass and
 // inertia are zero, so the reference's writes to them leave them unchanged).
 template <typename R, int BLOCK, bool STAGE>
-__global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, int32_t lds_bodies) {
+__global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_t* wlist, const int32_t* wcount,
+                                                 int32_t lds_bodies) {
     extern __shared__ int32_t s_last[];
     __shared__ int32_t s_scan[BLOCK / 64 + 2];
     __shared__ int32_t s_D;
-    const int32_t w = blockIdx.x;
+    if ((int32_t)blockIdx.x >= *wcount) return;
+    const int32_t w = wlist[blockIdx.x];
     const int tid = threadIdx.x;
     if (w >= d.wcap || !(d.w_flags[w] & WF_ACTIVE)) return;
     const int32_t K = d.w_cn[w], cs = d.w_cs[w], n = d.w_cnt[w];
@@ -711,28 +713,57 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, int32_t lds_
         CON(S, SR_FR, ci) = fric;
         CON(S, SR_IMA, ci) = ima; CON(S, SR_IMB, ci) = imb;
     }
-    // level schedule (one thread; `last` in LDS for small worlds)
+    // Level schedule.  Wave 0 walks the contacts 64 at a time: lane i starts
+    // from the last levels recorded for its dynamic bodies, then takes the
+    // final levels of the earlier lanes in this group (broadcast in order) and
+    // raises its own past any that share a dynamic body.  `last` (per body,
+    // LDS for worlds that fit) keeps the running maximum with atomicMax.
     int32_t* last = lds_bodies >= n ? s_last : d.last + off;
     for (int32_t i = tid; i < n; i += BLOCK) last[i] = 0;
     for (int32_t k = tid; k < K; k += BLOCK) d.lcount[cs + k] = 0;
     __syncthreads();
-    if (tid == 0) {
+    if (tid < 64) {
         int32_t D = 0;
-        for (int32_t k = 0; k < K; k++) {
-            const int64_t a = d.ca[cs + k], b = d.cb[cs + k];
-            const bool da = d.mt[a] == M_DYNAMIC, db = d.mt[b] == M_DYNAMIC;
-            const int32_t la = da ? last[a - off] : 0, lb = db ? last[b - off] : 0;
-            const int32_t L = (la > lb ? la : lb) + 1;
-            if (da) last[a - off] = L;
-            if (db) last[b - off] = L;
-            d.clevel[cs + k] = L;
-            d.lcount[cs + L - 1] += 1;
-            D = L > D ? L : D;
+        for (int32_t base = 0; base < K; base += 64) {
+            const int32_t k = base + tid;
+            const bool valid = k < K;
+            int32_t al = -1, bl = -1;
+            if (valid) {
+                const int64_t a = d.ca[cs + k], b = d.cb[cs + k];
+                if (d.mt[a] == M_DYNAMIC) al = (int32_t)(a - off);
+                if (d.mt[b] == M_DYNAMIC) bl = (int32_t)(b - off);
+            }
+            // atomic loads: the global-memory `last` must not be read from a stale L1 line
+            const int32_t la = al >= 0 ? __hip_atomic_load(&last[al], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+            const int32_t lb = bl >= 0 ? __hip_atomic_load(&last[bl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+            int32_t L = (la > lb ? la : lb) + 1;
+            const int32_t m = K - base < 64 ? K - base : 64;
+            for (int32_t j = 0; j < m - 1; j++) {
+                const int32_t aj = __builtin_amdgcn_readlane(al, j);
+                const int32_t bj = __builtin_amdgcn_readlane(bl, j);
+                const int32_t Lj = __builtin_amdgcn_readlane(L, j);
+                const bool share = (al >= 0 && (al == aj || al == bj)) || (bl >= 0 && (bl == aj || bl == bj));
+                if (tid > j && share && Lj + 1 > L) L = Lj + 1;
+            }
+            if (valid) {
+                if (al >= 0) atomicMax(&last[al], L);
+                if (bl >= 0) atomicMax(&last[bl], L);
+                d.clevel[cs + k] = L;
+                D = L > D ? L : D;
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_s_waitcnt(0);
         }
-        s_D = D;
-        d.w_levels[w] = D;
-        atomicMax((unsigned long long*)&d.cnt[C_MAXLEVEL], (unsigned long long)D);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) { int32_t x = __shfl_xor(D, o, 64); D = x > D ? x : D; }
+        if (tid == 0) {
+            s_D = D;
+            d.w_levels[w] = D;
+            atomicMax((unsigned long long*)&d.cnt[C_MAXLEVEL], (unsigned long long)D);
+        }
     }
+    __syncthreads();
+    for (int32_t k = tid; k < K; k += BLOCK) atomicAdd(&d.lcount[cs + d.clevel[cs + k] - 1], 1);
     __syncthreads();
     const int32_t D = s_D;
     block_exscan<BLOCK>(d.lcount + cs, D, s_scan);
@@ -1085,15 +1116,32 @@ void launch_refit(const Dev& d, int64_t nhw, double margin, hipStream_t s) {
     hipLaunchKernelGGL(k_refit, dim3(GRID_FOR(nhw, 256)), dim3(256), 0, s, d, nhw, margin);
 }
 
-void launch_sap(const Dev& d, const int32_t* wl_small, int32_t n_small, int32_t np_small, const int32_t* wl_big,
-                int32_t n_big, int32_t record, hipStream_t s) {
-    if (n_small > 0) {
-        size_t lds = (size_t)np_small * (8 + 4 + 4);
-        hipLaunchKernelGGL((k_sap<256, false>), dim3(n_small), dim3(256), lds, s, d, wl_small, n_small, np_small,
-                           record);
+// Per-world launches go by size class: c = 0..2 hold worlds whose padded size
+// fits np_class[c] bodies in LDS, c = 3 the larger ones (global-memory
+// scratch).  Each class's world list and count live in device memory, so a
+// captured graph stays valid as worlds come and go.
+template <int B>
+static void sap_one(const Dev& d, const int32_t* wl, const int32_t* wn, int32_t np, bool global, int32_t record,
+                    hipStream_t s) {
+    if (global) hipLaunchKernelGGL((k_sap<B, true>), dim3(d.wcap), dim3(B), 0, s, d, wl, wn, 0, record);
+    else hipLaunchKernelGGL((k_sap<B, false>), dim3(d.wcap), dim3(B), (size_t)np * 16, s, d, wl, wn, np, record);
+}
+
+void launch_sap(const Dev& d, const int32_t* wl, const int32_t* wn, const int32_t* np_class, const int32_t* blocks,
+                const int32_t* live, int32_t record, hipStream_t s) {
+    for (int c = 0; c < 4; c++) {
+        if (!live[c]) continue;
+        const bool global = c == 3;
+        const int32_t* l = wl + (size_t)c * d.wcap;
+        const int32_t* n = wn + c;
+        switch (blocks[c]) {
+            case 64: sap_one<64>(d, l, n, np_class[c], global, record, s); break;
+            case 128: sap_one<128>(d, l, n, np_class[c], global, record, s); break;
+            case 512: sap_one<512>(d, l, n, np_class[c], global, record, s); break;
+            case 1024: sap_one<1024>(d, l, n, np_class[c], global, record, s); break;
+            default: sap_one<256>(d, l, n, np_class[c], global, record, s); break;
+        }
     }
-    if (n_big > 0)
-        hipLaunchKernelGGL((k_sap<1024, true>), dim3(n_big), dim3(1024), 0, s, d, wl_big, n_big, 0, record);
 }
 
 void launch_narrow(const Dev& d, int precision, hipStream_t s) {
@@ -1126,24 +1174,36 @@ void launch_wake(const Dev& d, hipStream_t s) {
     hipLaunchKernelGGL((k_wake<256>), dim3(d.wcap), dim3(256), 0, s, d);
 }
 
-void launch_solve(const Dev& d, int precision, int block, SolveIn in, int32_t lds_bodies, bool stage,
-                  hipStream_t s) {
+template <typename R, int B>
+static void solve_one(const Dev& d, SolveIn in, const int32_t* wl, const int32_t* wn, int32_t lds_bodies, bool stage,
+                      hipStream_t s) {
     size_t lds = (size_t)lds_bodies * 4;
-#define OMI_SOLVE(R, B)                                                                                     \
-    do {                                                                                                    \
-        if (stage) hipLaunchKernelGGL((k_solve<R, B, true>), dim3(d.wcap), dim3(B), lds, s, d, in, lds_bodies); \
-        else hipLaunchKernelGGL((k_solve<R, B, false>), dim3(d.wcap), dim3(B), lds, s, d, in, lds_bodies);      \
-    } while (0)
-    if (precision == 64) {
-        if (block <= 64) OMI_SOLVE(double, 64);
-        else if (block <= 256) OMI_SOLVE(double, 256);
-        else OMI_SOLVE(double, 1024);
-    } else {
-        if (block <= 64) OMI_SOLVE(float, 64);
-        else if (block <= 256) OMI_SOLVE(float, 256);
-        else OMI_SOLVE(float, 1024);
+    if (stage) hipLaunchKernelGGL((k_solve<R, B, true>), dim3(d.wcap), dim3(B), lds, s, d, in, wl, wn, lds_bodies);
+    else hipLaunchKernelGGL((k_solve<R, B, false>), dim3(d.wcap), dim3(B), lds, s, d, in, wl, wn, lds_bodies);
+}
+
+template <typename R>
+static void solve_block(const Dev& d, int block, SolveIn in, const int32_t* wl, const int32_t* wn,
+                        int32_t lds_bodies, bool stage, hipStream_t s) {
+    switch (block) {
+        case 64: solve_one<R, 64>(d, in, wl, wn, lds_bodies, stage, s); break;
+        case 128: solve_one<R, 128>(d, in, wl, wn, lds_bodies, stage, s); break;
+        case 512: solve_one<R, 512>(d, in, wl, wn, lds_bodies, stage, s); break;
+        case 1024: solve_one<R, 1024>(d, in, wl, wn, lds_bodies, stage, s); break;
+        default: solve_one<R, 256>(d, in, wl, wn, lds_bodies, stage, s); break;
     }
-#undef OMI_SOLVE
+}
+
+// lds_bodies[c]: bodies whose level-schedule counters fit in LDS for class c
+// (worlds above it keep them in global memory).
+void launch_solve(const Dev& d, int precision, const int32_t* blocks, SolveIn in, const int32_t* wl,
+                  const int32_t* wn, const int32_t* lds_bodies, const int32_t* live, bool stage, hipStream_t s) {
+    for (int c = 0; c < 4; c++) {
+        if (!live[c]) continue;
+        const int32_t* l = wl + (size_t)c * d.wcap;
+        if (precision == 64) solve_block<double>(d, blocks[c], in, l, wn + c, lds_bodies[c], stage, s);
+        else solve_block<float>(d, blocks[c], in, l, wn + c, lds_bodies[c], stage, s);
+    }
 }
 
 void launch_cache(const Dev& d, int precision, hipStream_t s) {
