@@ -153,8 +153,74 @@ def cpu_baseline(arrays, steps, threads):
     return sum(a['n'] for a in arrays) * steps / wall, wall, pos
 
 
+def single_world(name, arrays, precision, steps, warmup, cpu):
+    """One world stepped alone on the GPU (the plain form), vs the oracle on one core."""
+    from omi_physics_amd.batch import batch_from_arrays
+    n = int(arrays['n'])
+    if warmup:
+        w = batch_from_arrays([arrays], precision=precision)
+        w.step(DT, warmup); w.sync(); w.close()
+    b = batch_from_arrays([arrays], precision=precision)
+    b.prepare(DT); b.sync()
+    lat = []
+    cmax = 0
+    t0 = time.perf_counter()
+    for s in range(steps):
+        t = time.perf_counter()
+        b.step(DT, 1); b.sync()
+        lat.append(time.perf_counter() - t)
+        cmax = max(cmax, b.counters()['contacts'])
+    wall = time.perf_counter() - t0
+    pos = b.read(0)['position']
+    b.close()
+    out = {'metric': METRIC, 'value': n * steps / wall, 'unit': 'body-steps/s', 'n_gpus': 1, 'steps': steps,
+           'warmup': warmup, 'ms_per_step': 1e3 * wall / steps, 'higher_is_better': True, 'scaling': 'weak',
+           'vs_baseline': None, 'dtype': 'f64' if precision == 'fp64' else 'f32', 'data': 'synthetic',
+           'config': {'workload': name, 'bodies': n, 'dt': DT},
+           'p50_step_ms': 1e3 * float(np.percentile(lat, 50)), 'p99_step_ms': 1e3 * float(np.percentile(lat, 99)),
+           'max_contacts': cmax}
+    if cpu:
+        import oracle
+        ow = oracle.OracleWorld(arrays)
+        t0 = time.perf_counter()
+        ow.step(DT, steps)
+        cw = time.perf_counter() - t0
+        err = float(np.abs(ow.state()['position'] - pos).max())
+        out['cpu_baseline'] = {'value': n * steps / cw, 'unit': 'body-steps/s', 'cores': 1, 'kind': 'port',
+                               'sample': 'the whole world, %d steps, one core (%.1f s)' % (steps, cw)}
+        out['err'] = {'max_pos_m_vs_fp64_oracle': err}
+    return out
+
+
+def serving(rates, n_requests, precision, seed=5):
+    from omi_physics_amd import scenes
+    from omi_physics_amd.serve import Request, RolloutServer, run_plain
+    res = {}
+    for rate in rates:
+        reqs_spec = scenes.serving_requests(n_requests, rate, seed=seed)
+        reqs = [Request(id=i, arrays=scenes.arrays(scenes.boxes_on_ground(r['n'], seed=r['seed'],
+                                                                              sphere_fraction=0.5)),
+                        steps=r['steps'], arrival=r['arrival']) for i, r in enumerate(reqs_spec)]
+        srv = RolloutServer(max_worlds=1024, max_bodies=1 << 19, precision=precision)
+        st = srv.run(reqs).summary()
+        srv.close()
+        wait = [r.admitted_at - r.arrival for r in reqs]
+        st['queue_wait_p99_s'] = float(np.percentile(wait, 99))
+        res[str(rate)] = st
+        log('serving rate %s/s: %s' % (rate, st))
+    plain_reqs = [Request(id=i, arrays=scenes.arrays(scenes.boxes_on_ground(r['n'], seed=r['seed'],
+                                                                            sphere_fraction=0.5)),
+                          steps=r['steps']) for i, r in enumerate(scenes.serving_requests(24, 1.0, seed=seed))]
+    res['plain'] = run_plain(plain_reqs, precision=precision)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument('--workload', default='batched4096',
+                    choices=['batched4096', 'boxes1000', 'pyramids', 'convex', 'serving', 'copy'])
+    ap.add_argument('--copy-mib', type=int, default=1024, help='buffer size of the copy workload')
+    ap.add_argument('--precision', default=None, help='fp32 or fp64 for the single-world workloads')
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=300)
     ap.add_argument('--warmup', type=int, default=3)
@@ -175,6 +241,49 @@ def main():
     if _ffi.load().omi_device_count() < 1:
         print('no HIP device visible', file=sys.stderr)
         sys.exit(1)
+    if args.workload == 'copy':
+        # a plain device-to-device copy of known size: the achievable HBM rate and
+        # the calibration of the FETCH_SIZE / WRITE_SIZE counters
+        import torch
+        n = args.copy_mib * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device='cuda')
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 20
+        e0.record()
+        for _ in range(reps):
+            y.copy_(x)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        print(json.dumps({'workload': 'copy', 'bytes_per_copy': 2 * 4 * n, 'ms_per_copy': ms,
+                          'GBps': 2 * 4 * n / (ms * 1e-3) / 1e9}), flush=True)
+        return
+    if args.workload != 'batched4096':
+        from omi_physics_amd import scenes, PhysicsWorld, model
+        if args.workload == 'serving':
+            out = {'metric': METRIC, 'workload': 'serving', 'unit': 'body-steps/s',
+                   'rates': serving([100.0, 400.0, 1600.0], 160, args.precision or 'fp32')}
+            print(json.dumps(out), flush=True)
+            return
+        if args.workload == 'boxes1000':
+            arr = scenes.arrays(scenes.boxes_on_ground(1000, seed=1))
+            prec = args.precision or 'fp64'
+            name = '1000 unit boxes on a 20x20 m ground, one world, %s' % prec
+        elif args.workload == 'pyramids':
+            arr = scenes.arrays(scenes.pyramids_and_debris())
+            prec = args.precision or 'fp32'
+            name = '20 ten-level box pyramids + 8000 box/sphere debris, one world, %s' % prec
+        else:
+            arr = scenes.build(scenes.convex_heap(2000), PhysicsWorld, model).export_arrays()
+            prec = args.precision or 'fp32'
+            name = '2000 cooked convex hulls (16-32 points) on a ground box, one world, %s (no joints)' % prec
+        out = single_world(name, arr, prec, args.steps, args.warmup, not args.no_cpu)
+        print(json.dumps(out), flush=True)
+        return
 
     arrays = make_worlds(args.worlds, args.seed)
     n_bodies = int(sum(a['n'] for a in arrays))

@@ -34,7 +34,8 @@ void launch_stage_spheres(int precision, int32_t np, int32_t mode, const double*
                           const double* cb, const double* Rb, const double* hb, const double* rb, int32_t* cnt,
                           double* nrm, double* pt, double* dep, hipStream_t s);
 void launch_solve(const Dev& d, int precision, const int32_t* blocks, SolveIn in, const int32_t* wl,
-                  const int32_t* wn, const int32_t* lds_bodies, const int32_t* live, bool stage, hipStream_t s);
+                  const int32_t* wn, const int32_t* lds_last, const int32_t* lds_state, const int32_t* live,
+                  bool stage, hipStream_t s);
 size_t epa_work_bytes();
 void launch_stage_convex(int32_t np, const int32_t* ka, const double* ca, const double* Ra, const double* ha,
                          const int32_t* oa, const int32_t* na, const int32_t* kb, const double* cb, const double* Rb,
@@ -113,7 +114,8 @@ struct omi_batch {
     int32_t* d_wl = nullptr;               // [4][max_worlds] world slots per size class
     int32_t* d_wn = nullptr;               // [4] class counts
     int32_t np_class[4] = {256, 1024, 4096, 0};
-    int32_t lds_bodies[4] = {256, 1024, 4096, 12288};
+    int32_t lds_bodies[4] = {256, 1024, 4096, 12288};   // level counters in LDS
+    int32_t lds_state[4] = {0, 0, 0, 0};                // velocities + inverse inertia in LDS
     int32_t live[4] = {0, 0, 0, 0};        // classes with a launch in the step
     int32_t sap_block[4] = {64, 128, 256, 1024};
     int32_t solve_block[4] = {64, 128, 256, 256};
@@ -199,6 +201,15 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     b->np_lds_max = smem >= 4096 * 16 + 1024 ? 4096 : 2048;
     b->np_class[2] = b->np_lds_max;
     b->lds_bodies[2] = b->np_lds_max;
+    {
+        // stage a class's bodies in LDS when the class bound fits the budget
+        const int64_t budget = std::min<int64_t>((int64_t)smem - 4096, 96 * 1024);
+        for (int c = 0; c < 4; c++) {
+            int64_t nb = c < 3 ? b->np_class[c] : 0;
+            int64_t bytes = ((b->lds_bodies[c] * 4 + 15) & ~15) + nb * 15 * (int64_t)rs;
+            b->lds_state[c] = (nb > 0 && bytes <= budget) ? (int32_t)nb : 0;
+        }
+    }
     b->solver_block = cfg->solver_block > 0 ? cfg->solver_block : 256;
     auto okb = [](int32_t x) { return x == 64 || x == 128 || x == 256 || x == 512 || x == 1024; };
     for (int c = 0; c < 4; c++) {
@@ -244,7 +255,8 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     A(mn, 3 * P * (int64_t)rs); A(md, 4 * P * (int64_t)rs);
     d.mnrm = mn; d.mdep = md;
     A(d.ca, K); A(d.cb, K); A(d.cpt, 3 * K);
-    A(cn, 3 * K * (int64_t)rs); A(cd, K * (int64_t)rs); A(ci, 3 * K * (int64_t)rs); A(cs, 19 * K * (int64_t)rs);
+    A(cn, 3 * K * (int64_t)rs); A(cd, K * (int64_t)rs); A(ci, 3 * K * (int64_t)rs); A(cs, 26 * K * (int64_t)rs);
+    A(d.sab, 2 * K);
     d.cnrm = cn; d.cdep = cd; d.cimp = ci; d.csolv = cs;
     A(d.clevel, K); A(d.corder, K); A(d.lcount, K);
     for (int q = 0; q < 2; q++) {
@@ -503,7 +515,7 @@ static void enqueue_step(omi_batch* b, double dt) {
     launch_compact(d, prec, b->cfg.record_broadphase, s); mark();
     launch_wake(d, s); mark();
     SolveIn in{nullptr, nullptr, 0.5, 0.8, 0.005, 1};
-    launch_solve(d, prec, b->solve_block, in, b->d_wl, b->d_wn, b->lds_bodies, b->live, false, s); mark();
+    launch_solve(d, prec, b->solve_block, in, b->d_wl, b->d_wn, b->lds_bodies, b->lds_state, b->live, false, s); mark();
     launch_cache(d, prec, s); mark();
     launch_post(d, b->nhw, dt, s); mark();
     launch_sleep(d, b->nhw, dt, s); mark();
@@ -957,7 +969,8 @@ extern "C" int32_t omi_prepare_and_solve(int64_t K, int32_t N, const int64_t* a_
         b->slots[0].live = true;
         b->slots[0].n = N;
         if (!(rc = rebuild_lists(b)))
-            launch_solve(d, precision, b->solve_block, in, b->d_wl, b->d_wn, b->lds_bodies, b->live, true, b->stream);
+            launch_solve(d, precision, b->solve_block, in, b->d_wl, b->d_wn, b->lds_bodies, b->lds_state, b->live, true,
+                         b->stream);
         if (!rc && hipGetLastError() != hipSuccess) rc = fail(OMI_ERR_HIP, "solver launch failed");
         if (!rc) rc = download_soa(b, pos, d.pos, d.ncap, 0, N, 3);
         if (!rc) rc = download_soa(b, lv, d.lv, d.ncap, 0, N, 3);

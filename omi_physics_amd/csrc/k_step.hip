@@ -561,7 +561,7 @@ __global__ __launch_bounds__(BLOCK) void k_wake(Dev d) {
 
 // Solver row layout per contact (R [SR_N][kcap]).
 enum { SR_RA = 0, SR_RB = 3, SR_T1 = 6, SR_T2 = 9, SR_NM = 12, SR_T1M = 13, SR_T2M = 14, SR_VB = 15,
-       SR_FR = 16, SR_IMA = 17, SR_IMB = 18, SR_N = 19 };
+       SR_FR = 16, SR_IMA = 17, SR_IMB = 18, SR_NX = 19, SR_IMP = 22, SR_DEP = 25, SR_N = 26 };
 
 // _solver_native._effmass (_solver_native.pyx:154-176)
 template <typename R>
@@ -645,8 +645,8 @@ __device__ __forceinline__ void seed_lookup(const Dev& d, int32_t w, int32_t old
 // inertia are zero, so the reference's writes to them leave them unchanged).
 template <typename R, int BLOCK, bool STAGE>
 __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_t* wlist, const int32_t* wcount,
-                                                 int32_t lds_bodies) {
-    extern __shared__ int32_t s_last[];
+                                                 int32_t lds_last, int32_t lds_state) {
+    extern __shared__ __align__(16) unsigned char s_dyn[];
     __shared__ int32_t s_scan[BLOCK / 64 + 2];
     __shared__ int32_t s_D;
     if ((int32_t)blockIdx.x >= *wcount) return;
@@ -657,68 +657,39 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     const int64_t off = d.w_off[w];
     if (K == 0) { if (tid == 0) d.w_levels[w] = 0; return; }
     const bool fp64 = sizeof(R) == sizeof(double);
-    // velocities the solver works on: the FP64 state itself, or an FP32 copy
-    R* V = fp64 ? (R*)d.lv : (R*)d.svel;
-    if (!fp64)
-        for (int32_t i = tid; i < n; i += BLOCK)
-            for (int c = 0; c < 6; c++) BODY(V, c, off + i) = (R)BODY(d.lv, c, off + i);
+    // Where the bodies' velocities and world inverse inertia live while the
+    // world is solved: in LDS when the world fits the launch's budget, else the
+    // FP64 state itself (FP64) or an FP32 copy (FP32) in HBM.
+    const bool state_lds = lds_state >= n;
+    int32_t* s_last = (int32_t*)s_dyn;
+    R* s_state = (R*)(s_dyn + (((size_t)lds_last * 4 + 15) & ~(size_t)15));
+    R* V;
+    const R* I;
+    int64_t vs, vb;
     const R* iiw = (const R*)d.iiw;
+    if (state_lds) { V = s_state; I = s_state + 6 * (size_t)n; vs = n; vb = off; }
+    else { V = fp64 ? (R*)d.lv : (R*)d.svel; I = iiw; vs = d.ncap; vb = 0; }
+#define VEL(c, i) V[(size_t)(c) * vs + (size_t)((i) - vb)]
+#define IIW(c, i) I[(size_t)(c) * vs + (size_t)((i) - vb)]
+    if (state_lds || !fp64)
+        for (int32_t i = tid; i < n; i += BLOCK) {
+            for (int c = 0; c < 6; c++) VEL(c, off + i) = (R)BODY(d.lv, c, off + i);
+            if (state_lds)
+                for (int c = 0; c < 9; c++) s_state[(size_t)(6 + c) * n + i] = BODY(iiw, c, off + i);
+        }
     R* S = (R*)d.csolv;
     R* imp = (R*)d.cimp;
     const R* cn = (const R*)d.cnrm;
-    const int32_t old = *d.sel;
-    const int32_t mo = d.w_mat_off[w], nm = d.w_nmat[w];
-    __syncthreads();
-    // prepare every contact from the pre-solve velocities
-    for (int32_t k = tid; k < K; k += BLOCK) {
-        const int64_t ci = cs + k, a = d.ca[ci], b = d.cb[ci];
-        R rest, fric;
-        if (STAGE) {
-            rest = (R)in.st_rest[k]; fric = (R)in.st_fric[k];
-        } else {
-            const int32_t t = mo + d.mat[a] * nm + d.mat[b];
-            rest = (R)d.mat_rest[t]; fric = (R)d.mat_fric[t];
-            R n0 = R(0), t0 = R(0), t1 = R(0);
-            if (in.warm) seed_lookup<R>(d, w, old, ci, a, b, n0, t0, t1);
-            CON(imp, 0, ci) = n0; CON(imp, 1, ci) = t0; CON(imp, 2, ci) = t1;
-        }
-        const R ima = (R)d.inv_mass[a], imb = (R)d.inv_mass[b];
-        R rA[3], rB[3], Ia[9], Ib[9], va[6], vb[6];
-        for (int r = 0; r < 3; r++) {
-            double p = CON(d.cpt, r, ci);
-            rA[r] = (R)(p - BODY(d.pos, r, a));
-            rB[r] = (R)(p - BODY(d.pos, r, b));
-        }
-        for (int r = 0; r < 9; r++) { Ia[r] = BODY(iiw, r, a); Ib[r] = BODY(iiw, r, b); }
-        for (int c = 0; c < 6; c++) { va[c] = BODY(V, c, a); vb[c] = BODY(V, c, b); }
-        const R nx = CON(cn, 0, ci), ny = CON(cn, 1, ci), nz = CON(cn, 2, ci);
-        R axx, ayy, azz;
-        if ((nx >= R(0) ? nx : -nx) < R(0.9)) { axx = R(1); ayy = R(0); azz = R(0); }
-        else { axx = R(0); ayy = R(1); azz = R(0); }
-        R tx = ayy * nz - azz * ny, ty = azz * nx - axx * nz, tz = axx * ny - ayy * nx;
-        R ln = sqrt(tx * tx + ty * ty + tz * tz);
-        R inv = ln > R(1e-12) ? R(1) / ln : R(0);
-        tx *= inv; ty *= inv; tz *= inv;
-        const R ux = ny * tz - nz * ty, uy = nz * tx - nx * tz, uz = nx * ty - ny * tx;
-        R dvx, dvy, dvz;
-        relvel(va, vb, rA, rB, dvx, dvy, dvz);
-        const R vn0 = dvx * nx + dvy * ny + dvz * nz;
-        for (int r = 0; r < 3; r++) { CON(S, SR_RA + r, ci) = rA[r]; CON(S, SR_RB + r, ci) = rB[r]; }
-        CON(S, SR_T1, ci) = tx; CON(S, SR_T1 + 1, ci) = ty; CON(S, SR_T1 + 2, ci) = tz;
-        CON(S, SR_T2, ci) = ux; CON(S, SR_T2 + 1, ci) = uy; CON(S, SR_T2 + 2, ci) = uz;
-        CON(S, SR_NM, ci) = effmass(rA, rB, nx, ny, nz, Ia, Ib, ima, imb);
-        CON(S, SR_T1M, ci) = effmass(rA, rB, tx, ty, tz, Ia, Ib, ima, imb);
-        CON(S, SR_T2M, ci) = effmass(rA, rB, ux, uy, uz, Ia, Ib, ima, imb);
-        CON(S, SR_VB, ci) = vn0 < -(R)in.rthr ? -rest * vn0 : R(0);
-        CON(S, SR_FR, ci) = fric;
-        CON(S, SR_IMA, ci) = ima; CON(S, SR_IMB, ci) = imb;
-    }
+    const R* cdep = (const R*)d.cdep;
+    int32_t* sa = d.sab;
+    int32_t* sb = d.sab + d.kcap;
+
     // Level schedule.  Wave 0 walks the contacts 64 at a time: lane i starts
     // from the last levels recorded for its dynamic bodies, then takes the
     // final levels of the earlier lanes in this group (broadcast in order) and
     // raises its own past any that share a dynamic body.  `last` (per body,
     // LDS for worlds that fit) keeps the running maximum with atomicMax.
-    int32_t* last = lds_bodies >= n ? s_last : d.last + off;
+    int32_t* last = lds_last >= n ? s_last : d.last + off;
     for (int32_t i = tid; i < n; i += BLOCK) last[i] = 0;
     for (int32_t k = tid; k < K; k += BLOCK) d.lcount[cs + k] = 0;
     __syncthreads();
@@ -733,7 +704,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
                 if (d.mt[a] == M_DYNAMIC) al = (int32_t)(a - off);
                 if (d.mt[b] == M_DYNAMIC) bl = (int32_t)(b - off);
             }
-            // atomic loads: the global-memory `last` must not be read from a stale L1 line
+            // atomic loads: a global-memory `last` must not be read from a stale L1 line
             const int32_t la = al >= 0 ? __hip_atomic_load(&last[al], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
             const int32_t lb = bl >= 0 ? __hip_atomic_load(&last[bl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
             int32_t L = (la > lb ? la : lb) + 1;
@@ -774,30 +745,80 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     }
     __syncthreads();
     // after the scatter lcount[l] is the end of level l
+
+    // Prepare every contact from the pre-solve velocities, writing its row at
+    // its level-sorted slot so each level's rows are contiguous.
+    const int32_t old = *d.sel;
+    const int32_t mo = d.w_mat_off[w], nm = d.w_nmat[w];
+    for (int32_t t = tid; t < K; t += BLOCK) {
+        const int32_t k = d.corder[cs + t];
+        const int64_t ci = cs + k, si = cs + t, a = d.ca[ci], b = d.cb[ci];
+        R rest, fric, n0 = R(0), u0 = R(0), w0 = R(0);
+        if (STAGE) {
+            rest = (R)in.st_rest[k]; fric = (R)in.st_fric[k];
+            n0 = CON(imp, 0, ci); u0 = CON(imp, 1, ci); w0 = CON(imp, 2, ci);
+        } else {
+            const int32_t tb = mo + d.mat[a] * nm + d.mat[b];
+            rest = (R)d.mat_rest[tb]; fric = (R)d.mat_fric[tb];
+            if (in.warm) seed_lookup<R>(d, w, old, ci, a, b, n0, u0, w0);
+        }
+        const R ima = (R)d.inv_mass[a], imb = (R)d.inv_mass[b];
+        R rA[3], rB[3], Ia[9], Ib[9], va[6], vb_[6];
+        for (int r = 0; r < 3; r++) {
+            double p = CON(d.cpt, r, ci);
+            rA[r] = (R)(p - BODY(d.pos, r, a));
+            rB[r] = (R)(p - BODY(d.pos, r, b));
+        }
+        for (int r = 0; r < 9; r++) { Ia[r] = IIW(r, a); Ib[r] = IIW(r, b); }
+        for (int c = 0; c < 6; c++) { va[c] = VEL(c, a); vb_[c] = VEL(c, b); }
+        const R nx = CON(cn, 0, ci), ny = CON(cn, 1, ci), nz = CON(cn, 2, ci);
+        R axx, ayy, azz;
+        if ((nx >= R(0) ? nx : -nx) < R(0.9)) { axx = R(1); ayy = R(0); azz = R(0); }
+        else { axx = R(0); ayy = R(1); azz = R(0); }
+        R tx = ayy * nz - azz * ny, ty = azz * nx - axx * nz, tz = axx * ny - ayy * nx;
+        R ln = sqrt(tx * tx + ty * ty + tz * tz);
+        R inv = ln > R(1e-12) ? R(1) / ln : R(0);
+        tx *= inv; ty *= inv; tz *= inv;
+        const R ux = ny * tz - nz * ty, uy = nz * tx - nx * tz, uz = nx * ty - ny * tx;
+        R dvx, dvy, dvz;
+        relvel(va, vb_, rA, rB, dvx, dvy, dvz);
+        const R vn0 = dvx * nx + dvy * ny + dvz * nz;
+        for (int r = 0; r < 3; r++) { CON(S, SR_RA + r, si) = rA[r]; CON(S, SR_RB + r, si) = rB[r]; }
+        CON(S, SR_T1, si) = tx; CON(S, SR_T1 + 1, si) = ty; CON(S, SR_T1 + 2, si) = tz;
+        CON(S, SR_T2, si) = ux; CON(S, SR_T2 + 1, si) = uy; CON(S, SR_T2 + 2, si) = uz;
+        CON(S, SR_NM, si) = effmass(rA, rB, nx, ny, nz, Ia, Ib, ima, imb);
+        CON(S, SR_T1M, si) = effmass(rA, rB, tx, ty, tz, Ia, Ib, ima, imb);
+        CON(S, SR_T2M, si) = effmass(rA, rB, ux, uy, uz, Ia, Ib, ima, imb);
+        CON(S, SR_VB, si) = vn0 < -(R)in.rthr ? -rest * vn0 : R(0);
+        CON(S, SR_FR, si) = fric;
+        CON(S, SR_IMA, si) = ima; CON(S, SR_IMB, si) = imb;
+        CON(S, SR_NX, si) = nx; CON(S, SR_NX + 1, si) = ny; CON(S, SR_NX + 2, si) = nz;
+        CON(S, SR_IMP, si) = n0; CON(S, SR_IMP + 1, si) = u0; CON(S, SR_IMP + 2, si) = w0;
+        CON(S, SR_DEP, si) = cdep[ci];
+        sa[si] = (int32_t)a; sb[si] = (int32_t)b;
+    }
+    __syncthreads();
     auto level_begin = [&](int32_t l) { return l == 0 ? 0 : d.lcount[cs + l - 1]; };
-    auto body_io = [&](int64_t a, int64_t b, R* va, R* vb) {
-        for (int c = 0; c < 6; c++) { va[c] = BODY(V, c, a); vb[c] = BODY(V, c, b); }
-    };
-    auto body_store = [&](int64_t a, int64_t b, const R* va, const R* vb) {
-        if (d.mt[a] == M_DYNAMIC) for (int c = 0; c < 6; c++) BODY(V, c, a) = va[c];
-        if (d.mt[b] == M_DYNAMIC) for (int c = 0; c < 6; c++) BODY(V, c, b) = vb[c];
+    auto body_store = [&](int64_t a, int64_t b, const R* va, const R* vb2) {
+        if (d.mt[a] == M_DYNAMIC) for (int c = 0; c < 6; c++) VEL(c, a) = va[c];
+        if (d.mt[b] == M_DYNAMIC) for (int c = 0; c < 6; c++) VEL(c, b) = vb2[c];
     };
     // warm start
     if (in.warm) {
         for (int32_t l = 0; l < D; l++) {
             const int32_t e = d.lcount[cs + l];
             for (int32_t t = level_begin(l) + tid; t < e; t += BLOCK) {
-                const int64_t ci = cs + d.corder[cs + t], a = d.ca[ci], b = d.cb[ci];
-                R va[6], vb[6], rA[3], rB[3], Ia[9], Ib[9];
-                body_io(a, b, va, vb);
-                for (int r = 0; r < 3; r++) { rA[r] = CON(S, SR_RA + r, ci); rB[r] = CON(S, SR_RB + r, ci); }
-                for (int r = 0; r < 9; r++) { Ia[r] = BODY(iiw, r, a); Ib[r] = BODY(iiw, r, b); }
-                const R n0 = CON(imp, 0, ci), u0 = CON(imp, 1, ci), w0 = CON(imp, 2, ci);
-                const R P0 = n0 * CON(cn, 0, ci) + u0 * CON(S, SR_T1, ci) + w0 * CON(S, SR_T2, ci);
-                const R P1 = n0 * CON(cn, 1, ci) + u0 * CON(S, SR_T1 + 1, ci) + w0 * CON(S, SR_T2 + 1, ci);
-                const R P2 = n0 * CON(cn, 2, ci) + u0 * CON(S, SR_T1 + 2, ci) + w0 * CON(S, SR_T2 + 2, ci);
-                apply_w(va, vb, CON(S, SR_IMA, ci), CON(S, SR_IMB, ci), Ia, Ib, rA, rB, P0, P1, P2);
-                body_store(a, b, va, vb);
+                const int64_t si = cs + t, a = sa[si], b = sb[si];
+                R va[6], vb2[6], rA[3], rB[3], Ia[9], Ib[9];
+                for (int c = 0; c < 6; c++) { va[c] = VEL(c, a); vb2[c] = VEL(c, b); }
+                for (int r = 0; r < 3; r++) { rA[r] = CON(S, SR_RA + r, si); rB[r] = CON(S, SR_RB + r, si); }
+                for (int r = 0; r < 9; r++) { Ia[r] = IIW(r, a); Ib[r] = IIW(r, b); }
+                const R n0 = CON(S, SR_IMP, si), u0 = CON(S, SR_IMP + 1, si), w0 = CON(S, SR_IMP + 2, si);
+                const R P0 = n0 * CON(S, SR_NX, si) + u0 * CON(S, SR_T1, si) + w0 * CON(S, SR_T2, si);
+                const R P1 = n0 * CON(S, SR_NX + 1, si) + u0 * CON(S, SR_T1 + 1, si) + w0 * CON(S, SR_T2 + 1, si);
+                const R P2 = n0 * CON(S, SR_NX + 2, si) + u0 * CON(S, SR_T1 + 2, si) + w0 * CON(S, SR_T2 + 2, si);
+                apply_w(va, vb2, CON(S, SR_IMA, si), CON(S, SR_IMB, si), Ia, Ib, rA, rB, P0, P1, P2);
+                body_store(a, b, va, vb2);
             }
             __syncthreads();
         }
@@ -807,39 +828,39 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
         for (int32_t l = 0; l < D; l++) {
             const int32_t e = d.lcount[cs + l];
             for (int32_t t = level_begin(l) + tid; t < e; t += BLOCK) {
-                const int64_t ci = cs + d.corder[cs + t], a = d.ca[ci], b = d.cb[ci];
-                R va[6], vb[6], rA[3], rB[3], Ia[9], Ib[9], t1[3], t2[3];
-                body_io(a, b, va, vb);
+                const int64_t si = cs + t, a = sa[si], b = sb[si];
+                R va[6], vb2[6], rA[3], rB[3], Ia[9], Ib[9], t1[3], t2[3];
+                for (int c = 0; c < 6; c++) { va[c] = VEL(c, a); vb2[c] = VEL(c, b); }
                 for (int r = 0; r < 3; r++) {
-                    rA[r] = CON(S, SR_RA + r, ci); rB[r] = CON(S, SR_RB + r, ci);
-                    t1[r] = CON(S, SR_T1 + r, ci); t2[r] = CON(S, SR_T2 + r, ci);
+                    rA[r] = CON(S, SR_RA + r, si); rB[r] = CON(S, SR_RB + r, si);
+                    t1[r] = CON(S, SR_T1 + r, si); t2[r] = CON(S, SR_T2 + r, si);
                 }
-                for (int r = 0; r < 9; r++) { Ia[r] = BODY(iiw, r, a); Ib[r] = BODY(iiw, r, b); }
-                const R ima = CON(S, SR_IMA, ci), imb = CON(S, SR_IMB, ci);
-                const R nx = CON(cn, 0, ci), ny = CON(cn, 1, ci), nz = CON(cn, 2, ci);
-                R nI = CON(imp, 0, ci), tI0 = CON(imp, 1, ci), tI1 = CON(imp, 2, ci);
+                for (int r = 0; r < 9; r++) { Ia[r] = IIW(r, a); Ib[r] = IIW(r, b); }
+                const R ima = CON(S, SR_IMA, si), imb = CON(S, SR_IMB, si);
+                const R nx = CON(S, SR_NX, si), ny = CON(S, SR_NX + 1, si), nz = CON(S, SR_NX + 2, si);
+                R nI = CON(S, SR_IMP, si), tI0 = CON(S, SR_IMP + 1, si), tI1 = CON(S, SR_IMP + 2, si);
                 R dvx, dvy, dvz;
-                relvel(va, vb, rA, rB, dvx, dvy, dvz);
+                relvel(va, vb2, rA, rB, dvx, dvy, dvz);
                 R vn = dvx * nx + dvy * ny + dvz * nz;
-                R nw = nI - CON(S, SR_NM, ci) * (vn - CON(S, SR_VB, ci));
+                R nw = nI - CON(S, SR_NM, si) * (vn - CON(S, SR_VB, si));
                 if (nw < R(0)) nw = R(0);
                 R dL = nw - nI;
                 nI = nw;
-                apply_w(va, vb, ima, imb, Ia, Ib, rA, rB, dL * nx, dL * ny, dL * nz);
-                const R maxf = CON(S, SR_FR, ci) * nI;
-                relvel(va, vb, rA, rB, dvx, dvy, dvz);
+                apply_w(va, vb2, ima, imb, Ia, Ib, rA, rB, dL * nx, dL * ny, dL * nz);
+                const R maxf = CON(S, SR_FR, si) * nI;
+                relvel(va, vb2, rA, rB, dvx, dvy, dvz);
                 R vt = dvx * t1[0] + dvy * t1[1] + dvz * t1[2];
-                R nt = tI0 - CON(S, SR_T1M, ci) * vt;
+                R nt = tI0 - CON(S, SR_T1M, si) * vt;
                 if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
                 dL = nt - tI0; tI0 = nt;
-                apply_w(va, vb, ima, imb, Ia, Ib, rA, rB, dL * t1[0], dL * t1[1], dL * t1[2]);
+                apply_w(va, vb2, ima, imb, Ia, Ib, rA, rB, dL * t1[0], dL * t1[1], dL * t1[2]);
                 vt = dvx * t2[0] + dvy * t2[1] + dvz * t2[2];
-                nt = tI1 - CON(S, SR_T2M, ci) * vt;
+                nt = tI1 - CON(S, SR_T2M, si) * vt;
                 if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
                 dL = nt - tI1; tI1 = nt;
-                apply_w(va, vb, ima, imb, Ia, Ib, rA, rB, dL * t2[0], dL * t2[1], dL * t2[2]);
-                CON(imp, 0, ci) = nI; CON(imp, 1, ci) = tI0; CON(imp, 2, ci) = tI1;
-                body_store(a, b, va, vb);
+                apply_w(va, vb2, ima, imb, Ia, Ib, rA, rB, dL * t2[0], dL * t2[1], dL * t2[2]);
+                CON(S, SR_IMP, si) = nI; CON(S, SR_IMP + 1, si) = tI0; CON(S, SR_IMP + 2, si) = tI1;
+                body_store(a, b, va, vb2);
             }
             __syncthreads();
         }
@@ -848,36 +869,47 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     for (int32_t l = 0; l < D; l++) {
         const int32_t e = d.lcount[cs + l];
         for (int32_t t = level_begin(l) + tid; t < e; t += BLOCK) {
-            const int64_t ci = cs + d.corder[cs + t];
-            const R dep = ((const R*)d.cdep)[ci];
-            const R vn = dep - (R)in.slop;
+            const int64_t si = cs + t;
+            const R vn = CON(S, SR_DEP, si) - (R)in.slop;
             if (vn <= R(0)) continue;
-            const R ia = CON(S, SR_IMA, ci), ib = CON(S, SR_IMB, ci);
+            const R ia = CON(S, SR_IMA, si), ib = CON(S, SR_IMB, si);
             if (ia + ib <= R(1e-12)) continue;
             const R dL = (R)in.correction * vn / (ia + ib);
-            const int64_t a = d.ca[ci], b = d.cb[ci];
+            const int64_t a = sa[si], b = sb[si];
             if (d.mt[a] == M_DYNAMIC)
                 for (int r = 0; r < 3; r++) {
-                    if (fp64) BODY(d.pos, r, a) -= (double)(ia * dL * CON(cn, r, ci));
-                    else BODY(d.pos, r, a) = BODY(d.pos, r, a) - (double)(ia * dL * CON(cn, r, ci));
+                    if (fp64) BODY(d.pos, r, a) -= (double)(ia * dL * CON(S, SR_NX + r, si));
+                    else BODY(d.pos, r, a) = BODY(d.pos, r, a) - (double)(ia * dL * CON(S, SR_NX + r, si));
                 }
             if (d.mt[b] == M_DYNAMIC)
                 for (int r = 0; r < 3; r++) {
-                    if (fp64) BODY(d.pos, r, b) += (double)(ib * dL * CON(cn, r, ci));
-                    else BODY(d.pos, r, b) = BODY(d.pos, r, b) + (double)(ib * dL * CON(cn, r, ci));
+                    if (fp64) BODY(d.pos, r, b) += (double)(ib * dL * CON(S, SR_NX + r, si));
+                    else BODY(d.pos, r, b) = BODY(d.pos, r, b) + (double)(ib * dL * CON(S, SR_NX + r, si));
                 }
         }
         __syncthreads();
     }
-    // FP32: fold the solver's velocity change back into the FP64 state
-    if (!fp64)
+    // impulses back to contact order (warm-start cache and readback)
+    for (int32_t t = tid; t < K; t += BLOCK) {
+        const int64_t ci = cs + d.corder[cs + t], si = cs + t;
+        for (int r = 0; r < 3; r++) CON(imp, r, ci) = CON(S, SR_IMP + r, si);
+    }
+    // velocities back to the FP64 state: copied from LDS (FP64), or the FP32
+    // solver's change folded in
+    if (state_lds || !fp64)
         for (int32_t i = tid; i < n; i += BLOCK)
             if (d.mt[off + i] == M_DYNAMIC)
                 for (int c = 0; c < 6; c++) {
-                    double v0 = BODY(d.lv, c, off + i);
-                    R dv = BODY(V, c, off + i) - (R)v0;
-                    BODY(d.lv, c, off + i) = v0 + (double)dv;
+                    if (fp64) {
+                        BODY(d.lv, c, off + i) = (double)VEL(c, off + i);
+                    } else {
+                        double v0 = BODY(d.lv, c, off + i);
+                        R dv = VEL(c, off + i) - (R)v0;
+                        BODY(d.lv, c, off + i) = v0 + (double)dv;
+                    }
                 }
+#undef VEL
+#undef IIW
 }
 
 // ------------------------------------------------------- warm-start cache --
@@ -1175,34 +1207,38 @@ void launch_wake(const Dev& d, hipStream_t s) {
 }
 
 template <typename R, int B>
-static void solve_one(const Dev& d, SolveIn in, const int32_t* wl, const int32_t* wn, int32_t lds_bodies, bool stage,
-                      hipStream_t s) {
-    size_t lds = (size_t)lds_bodies * 4;
-    if (stage) hipLaunchKernelGGL((k_solve<R, B, true>), dim3(d.wcap), dim3(B), lds, s, d, in, wl, wn, lds_bodies);
-    else hipLaunchKernelGGL((k_solve<R, B, false>), dim3(d.wcap), dim3(B), lds, s, d, in, wl, wn, lds_bodies);
+static void solve_one(const Dev& d, SolveIn in, const int32_t* wl, const int32_t* wn, int32_t lds_last,
+                      int32_t lds_state, bool stage, hipStream_t s) {
+    size_t lds = (((size_t)lds_last * 4 + 15) & ~(size_t)15) + (size_t)lds_state * 15 * sizeof(R);
+    if (stage)
+        hipLaunchKernelGGL((k_solve<R, B, true>), dim3(d.wcap), dim3(B), lds, s, d, in, wl, wn, lds_last, lds_state);
+    else
+        hipLaunchKernelGGL((k_solve<R, B, false>), dim3(d.wcap), dim3(B), lds, s, d, in, wl, wn, lds_last, lds_state);
 }
 
 template <typename R>
 static void solve_block(const Dev& d, int block, SolveIn in, const int32_t* wl, const int32_t* wn,
-                        int32_t lds_bodies, bool stage, hipStream_t s) {
+                        int32_t lds_last, int32_t lds_state, bool stage, hipStream_t s) {
     switch (block) {
-        case 64: solve_one<R, 64>(d, in, wl, wn, lds_bodies, stage, s); break;
-        case 128: solve_one<R, 128>(d, in, wl, wn, lds_bodies, stage, s); break;
-        case 512: solve_one<R, 512>(d, in, wl, wn, lds_bodies, stage, s); break;
-        case 1024: solve_one<R, 1024>(d, in, wl, wn, lds_bodies, stage, s); break;
-        default: solve_one<R, 256>(d, in, wl, wn, lds_bodies, stage, s); break;
+        case 64: solve_one<R, 64>(d, in, wl, wn, lds_last, lds_state, stage, s); break;
+        case 128: solve_one<R, 128>(d, in, wl, wn, lds_last, lds_state, stage, s); break;
+        case 512: solve_one<R, 512>(d, in, wl, wn, lds_last, lds_state, stage, s); break;
+        case 1024: solve_one<R, 1024>(d, in, wl, wn, lds_last, lds_state, stage, s); break;
+        default: solve_one<R, 256>(d, in, wl, wn, lds_last, lds_state, stage, s); break;
     }
 }
 
-// lds_bodies[c]: bodies whose level-schedule counters fit in LDS for class c
-// (worlds above it keep them in global memory).
+// For size class c, lds_last[c] bodies keep their level counters in LDS and
+// lds_state[c] bodies their velocities and inverse inertia; larger worlds of
+// the class use global memory for them.
 void launch_solve(const Dev& d, int precision, const int32_t* blocks, SolveIn in, const int32_t* wl,
-                  const int32_t* wn, const int32_t* lds_bodies, const int32_t* live, bool stage, hipStream_t s) {
+                  const int32_t* wn, const int32_t* lds_last, const int32_t* lds_state, const int32_t* live,
+                  bool stage, hipStream_t s) {
     for (int c = 0; c < 4; c++) {
         if (!live[c]) continue;
         const int32_t* l = wl + (size_t)c * d.wcap;
-        if (precision == 64) solve_block<double>(d, blocks[c], in, l, wn + c, lds_bodies[c], stage, s);
-        else solve_block<float>(d, blocks[c], in, l, wn + c, lds_bodies[c], stage, s);
+        if (precision == 64) solve_block<double>(d, blocks[c], in, l, wn + c, lds_last[c], lds_state[c], stage, s);
+        else solve_block<float>(d, blocks[c], in, l, wn + c, lds_last[c], lds_state[c], stage, s);
     }
 }
 

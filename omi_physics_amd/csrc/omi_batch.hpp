@@ -70,7 +70,8 @@ struct Dev {
     int32_t *ca, *cb;
     double *cpt;                          // [3][kcap]
     void *cnrm, *cdep, *cimp;             // R [3][kcap], R [kcap], R [3][kcap] (n, t1, t2 impulses)
-    void *csolv;                          // R [20][kcap] solver rows
+    void *csolv;                          // R [26][kcap] solver rows, level-sorted
+    int32_t *sab;                         // [2][kcap] body indices of the sorted rows
     int32_t *clevel, *corder, *lcount;    // level schedule
     // warm-start cache, two buffers
     int32_t *ka[2], *kb[2];
