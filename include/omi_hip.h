@@ -138,6 +138,11 @@ typedef struct {
     double gravity[3];
     double default_linear_damping, default_angular_damping;
     int32_t sleep_enabled, velocity_iterations, has_collision;
+    /* PointConstraint joints (joints.py:60-121) solved after the contacts
+     * (world.py:599-611): body indices (-1 = the world), world anchors [J][3]. */
+    int32_t n_joints, joint_iterations;
+    const int32_t *joint_a, *joint_b;
+    const double *joint_anchor;
 } omi_world_desc;
 
 typedef struct {
@@ -147,6 +152,7 @@ typedef struct {
     int64_t max_pairs;          /* solid pairs per step, all worlds */
     int64_t max_contacts;       /* contact points per step, all worlds */
     int64_t max_hull_pts;
+    int64_t max_joints;
     int32_t record_broadphase;  /* also keep the unfiltered broadphase pair list */
     int32_t solver_block;       /* threads per world in the solver launch (64..1024, 0 = auto) */
     /* Per size class (worlds of <= 256, <= 1024, <= 4096 bodies, larger):

@@ -43,6 +43,8 @@ class WorldDesc(ctypes.Structure):
         ('default_linear_damping', ctypes.c_double), ('default_angular_damping', ctypes.c_double),
         ('sleep_enabled', ctypes.c_int32), ('velocity_iterations', ctypes.c_int32),
         ('has_collision', ctypes.c_int32),
+        ('n_joints', ctypes.c_int32), ('joint_iterations', ctypes.c_int32),
+        ('joint_a', _i32), ('joint_b', _i32), ('joint_anchor', _d),
     ]
 
 
@@ -51,7 +53,7 @@ class BatchConfig(ctypes.Structure):
     _fields_ = [
         ('max_worlds', ctypes.c_int32), ('precision', ctypes.c_int32),
         ('max_bodies', ctypes.c_int64), ('max_pairs', ctypes.c_int64),
-        ('max_contacts', ctypes.c_int64), ('max_hull_pts', ctypes.c_int64),
+        ('max_contacts', ctypes.c_int64), ('max_hull_pts', ctypes.c_int64), ('max_joints', ctypes.c_int64),
         ('record_broadphase', ctypes.c_int32), ('solver_block', ctypes.c_int32),
         ('sap_block', ctypes.c_int32 * 4), ('solve_block', ctypes.c_int32 * 4),
     ]
@@ -199,4 +201,12 @@ def world_desc(arrays: dict):
     d.sleep_enabled = int(arrays['sleep_enabled'])
     d.velocity_iterations = int(arrays['velocity_iterations'])
     d.has_collision = int(arrays['has_collision'])
+    nj = int(arrays.get('n_joints', 0))
+    d.n_joints = nj
+    d.joint_iterations = int(arrays.get('joint_iterations', 10))
+    keep['ja'] = ci32(arrays['joint_a'] if nj else np.zeros(1))
+    keep['jb'] = ci32(arrays['joint_b'] if nj else np.zeros(1))
+    keep['janc'] = c64(np.asarray(arrays['joint_anchor']).reshape(-1, 3) if nj else np.zeros((1, 3)))
+    d.joint_a = ptr(keep['ja'], _i32); d.joint_b = ptr(keep['jb'], _i32)
+    d.joint_anchor = ptr(keep['janc'], _d)
     return d, keep

@@ -60,7 +60,7 @@ class HipBackend:
             if self._batch is not None:
                 self._batch.close()
             self._batch = Batch(1, max(n, 1), precision=prec, max_hull_pts=hull,
-                                record_broadphase=True, **caps)
+                                max_joints=max(1, int(arrays['n_joints'])), record_broadphase=True, **caps)
             if n:
                 self._batch.load(0, arrays)
             self._layout = world._layout_version
@@ -102,6 +102,10 @@ class HipBackend:
     def integrate_positions(self, world: Any, dt: float) -> None:
         """Symplectic position/orientation update (reference ``backend.py:42-49``)."""
         self._stages.add('positions')
+
+    def solve_joints(self, world: Any, dt: float) -> None:
+        """Point joints after the contacts (reference ``world.py:599-611``)."""
+        self._stages.add('joints')
 
     def update_sleep(self, world: Any, dt: float) -> None:
         """Island-aware sleeping (reference ``world.py:510-567``)."""

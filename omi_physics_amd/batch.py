@@ -15,7 +15,8 @@ import numpy as np
 from . import _ffi
 from ._ffi import _d, _i32, _i64, _u8, check, ptr
 
-KERNELS = ('begin', 'pre', 'sap', 'narrow', 'compact', 'wake', 'solve', 'cache', 'post', 'sleep', 'toggle')
+KERNELS = ('begin', 'pre', 'sap', 'narrow', 'compact', 'wake', 'solve', 'cache', 'joints', 'post', 'sleep',
+           'toggle')
 
 
 @dataclass
@@ -42,7 +43,7 @@ class Batch:
 
     def __init__(self, max_worlds: int, max_bodies: int, precision: str = 'fp64',
                  max_pairs: Optional[int] = None, max_contacts: Optional[int] = None,
-                 max_hull_pts: int = 1, record_broadphase: bool = False,
+                 max_hull_pts: int = 1, max_joints: int = 1, record_broadphase: bool = False,
                  solver_block: int = 0, use_graph: bool = True,
                  sap_block: Optional[Sequence[int]] = None,
                  solve_block: Optional[Sequence[int]] = None) -> None:
@@ -55,6 +56,7 @@ class Batch:
         cfg.max_pairs = int(max_pairs or caps['max_pairs'])
         cfg.max_contacts = int(max_contacts or caps['max_contacts'])
         cfg.max_hull_pts = int(max(1, max_hull_pts))
+        cfg.max_joints = int(max(1, max_joints))
         cfg.record_broadphase = int(bool(record_broadphase))
         cfg.solver_block = int(solver_block)
         from .autotune import load_tuned
@@ -188,8 +190,9 @@ def batch_from_arrays(worlds: Sequence[dict], precision: str = 'fp64', **kw) -> 
     """A batch holding ``worlds`` in slots 0..len-1, sized for them."""
     total = int(sum(int(w['n']) for w in worlds))
     hull = int(sum(len(np.asarray(w['hull_pts']).reshape(-1, 3)) for w in worlds))
+    joints = int(sum(int(w.get('n_joints', 0)) for w in worlds))
     b = Batch(max_worlds=len(worlds), max_bodies=max(total, 1), precision=precision,
-              max_hull_pts=max(hull, 1), **kw)
+              max_hull_pts=max(hull, 1), max_joints=max(joints, 1), **kw)
     for i, w in enumerate(worlds):
         b.load(i, w)
     return b

@@ -25,6 +25,7 @@ void launch_compact(const Dev& d, int precision, int32_t record, hipStream_t s);
 void launch_wake(const Dev& d, hipStream_t s);
 void launch_cache(const Dev& d, int precision, hipStream_t s);
 void launch_post(const Dev& d, int64_t nhw, double dt, hipStream_t s);
+void launch_joints(const Dev& d, int precision, double dt, hipStream_t s);
 void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s);
 void launch_toggle(const Dev& d, hipStream_t s);
 void launch_stage_boxbox(int precision, int32_t np, const double* ca, const double* Ra, const double* ha,
@@ -95,13 +96,13 @@ struct Ranges {
 struct WorldSlot {
     bool live = false;
     int64_t off = 0, n = 0;
-    int64_t mat_off = 0, mat_n = 0, filt_off = 0, filt_n = 0, hull_off = 0, hull_n = 0;
+    int64_t mat_off = 0, mat_n = 0, filt_off = 0, filt_n = 0, hull_off = 0, hull_n = 0, j_off = 0, j_n = 0;
     int32_t flags = 0;
 };
 
 static const char* kKernelNames =
-    "begin,pre,sap,narrow,compact,wake,solve,cache,post,sleep,toggle";
-static const int kNumKernels = 11;
+    "begin,pre,sap,narrow,compact,wake,solve,cache,joints,post,sleep,toggle";
+static const int kNumKernels = 12;
 
 struct omi_batch {
     omi_batch_config cfg{};
@@ -109,7 +110,8 @@ struct omi_batch {
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
     int64_t bytes = 0;
-    Ranges bodies, mats, filts, hulls;
+    Ranges bodies, mats, filts, hulls, joints;
+    bool any_joints = false;
     std::vector<WorldSlot> slots;
     int32_t* d_wl = nullptr;               // [4][max_worlds] world slots per size class
     int32_t* d_wn = nullptr;               // [4] class counts
@@ -265,6 +267,14 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
         A(d.hkey[q], H); A(d.hval[q], H);
         HIPCHK(hipMemset(d.hkey[q], 0xff, sizeof(uint64_t) * H));
     }
+    const int64_t JC = std::max<int64_t>(cfg->max_joints, 1);
+    d.jcap = JC;
+    A(d.ja, JC); A(d.jb, JC); A(d.janc, 3 * JC); A(d.jla, 3 * JC); A(d.jlb, 3 * JC); A(d.jhas, JC);
+    unsigned char* jr;
+    A(jr, 18 * JC * (int64_t)rs);
+    d.jrow = jr;
+    A(d.jlevel, JC); A(d.jorder, JC); A(d.jcount, JC);
+    A(d.w_js, W); A(d.w_jn, W); A(d.w_jiters, W);
     A(d.sel, 1); A(d.cnt, C_COUNT);
     A(b->d_wl, 4 * W); A(b->d_wn, 4);
 #undef A
@@ -278,6 +288,7 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     b->mats.init(TAB);
     b->filts.init(TAB);
     b->hulls.init(std::max<int64_t>(cfg->max_hull_pts, 1));
+    b->joints.init(JC);
     b->slots.resize(W);
     *out = b;
     return OMI_OK;
@@ -352,6 +363,7 @@ extern "C" int32_t omi_batch_remove_world(omi_batch* b, int32_t slot) {
     b->mats.release(s.mat_off, s.mat_n);
     b->filts.release(s.filt_off, s.filt_n);
     b->hulls.release(s.hull_off, s.hull_n);
+    b->joints.release(s.j_off, s.j_n);
     s = WorldSlot();
     b->lists_dirty = true;
     return OMI_OK;
@@ -371,6 +383,12 @@ extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_wo
         if (k == 3 && (w->hull_off[i] < 0 || w->hull_cnt[i] < 1 || w->hull_off[i] + w->hull_cnt[i] > w->n_hull_pts))
             return fail(OMI_ERR_ARG, "convex body without valid hull points");
     }
+    if (w->n_joints < 0) return fail(OMI_ERR_ARG, "negative joint count");
+    for (int32_t j = 0; j < w->n_joints; j++) {
+        int32_t a = w->joint_a[j], c = w->joint_b[j];
+        if (a < -1 || a >= w->n || c < -1 || c >= w->n || (a < 0 && c < 0) || a == c)
+            return fail(OMI_ERR_ARG, "joint body index out of range");
+    }
     int32_t rc = omi_batch_remove_world(b, slot);
     if (rc) return rc;
     WorldSlot s;
@@ -383,7 +401,17 @@ extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_wo
     s.mat_off = b->mats.alloc(s.mat_n);
     s.filt_off = b->filts.alloc(s.filt_n);
     s.hull_off = b->hulls.alloc(s.hull_n);
+    s.j_n = w->n_joints;
+    s.j_off = b->joints.alloc(s.j_n);
+    if (s.j_off < 0) {
+        b->bodies.release(s.off, s.n);
+        if (s.mat_off >= 0) b->mats.release(s.mat_off, s.mat_n);
+        if (s.filt_off >= 0) b->filts.release(s.filt_off, s.filt_n);
+        if (s.hull_off >= 0) b->hulls.release(s.hull_off, s.hull_n);
+        return fail(OMI_ERR_CAPACITY, "joint capacity of the batch is full");
+    }
     if (s.mat_off < 0 || s.filt_off < 0 || s.hull_off < 0) {
+        b->joints.release(s.j_off, s.j_n);
         b->bodies.release(s.off, s.n);
         if (s.mat_off >= 0) b->mats.release(s.mat_off, s.mat_n);
         if (s.filt_off >= 0) b->filts.release(s.filt_off, s.filt_n);
@@ -430,7 +458,24 @@ extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_wo
     if (s.hull_n && (rc = put(b, d.hull + 3 * s.hull_off, w->hull_pts, 3 * s.hull_n))) return rc;
     const int32_t W = b->cfg.max_worlds;
     int32_t i32;
-    s.flags = WF_ACTIVE | (w->has_collision ? WF_COLLISION : 0) | (w->sleep_enabled ? WF_SLEEP : 0);
+    if (s.j_n > 0) {
+        std::vector<int32_t> ja((size_t)s.j_n), jb((size_t)s.j_n);
+        std::vector<uint8_t> jh((size_t)s.j_n, 0);
+        for (int64_t j = 0; j < s.j_n; j++) {
+            ja[j] = w->joint_a[j] >= 0 ? (int32_t)(s.off + w->joint_a[j]) : -1;
+            jb[j] = w->joint_b[j] >= 0 ? (int32_t)(s.off + w->joint_b[j]) : -1;
+        }
+        if ((rc = put(b, d.ja + s.j_off, ja.data(), s.j_n))) return rc;
+        if ((rc = put(b, d.jb + s.j_off, jb.data(), s.j_n))) return rc;
+        if ((rc = put(b, d.jhas + s.j_off, jh.data(), s.j_n))) return rc;
+        if ((rc = upload_soa(b, d.janc, d.jcap, s.j_off, w->joint_anchor, s.j_n, 3))) return rc;
+        if (!b->any_joints) { b->any_joints = true; invalidate_graph(b); }
+    }
+    i32 = (int32_t)s.j_off; if ((rc = put(b, d.w_js + slot, &i32, 1))) return rc;
+    i32 = (int32_t)s.j_n; if ((rc = put(b, d.w_jn + slot, &i32, 1))) return rc;
+    i32 = w->joint_iterations; if ((rc = put(b, d.w_jiters + slot, &i32, 1))) return rc;
+    s.flags = WF_ACTIVE | (w->has_collision ? WF_COLLISION : 0) | (w->sleep_enabled ? WF_SLEEP : 0) |
+              (s.j_n > 0 ? WF_JOINTS : 0);
     i32 = (int32_t)s.off; if ((rc = put(b, d.w_off + slot, &i32, 1))) return rc;
     i32 = (int32_t)n; if ((rc = put(b, d.w_cnt + slot, &i32, 1))) return rc;
     i32 = w->velocity_iterations; if ((rc = put(b, d.w_iters + slot, &i32, 1))) return rc;
@@ -517,6 +562,8 @@ static void enqueue_step(omi_batch* b, double dt) {
     SolveIn in{nullptr, nullptr, 0.5, 0.8, 0.005, 1};
     launch_solve(d, prec, b->solve_block, in, b->d_wl, b->d_wn, b->lds_bodies, b->lds_state, b->live, false, s); mark();
     launch_cache(d, prec, s); mark();
+    if (b->any_joints) launch_joints(d, prec, dt, s);
+    mark();
     launch_post(d, b->nhw, dt, s); mark();
     launch_sleep(d, b->nhw, dt, s); mark();
     launch_toggle(d, s); mark();
@@ -574,7 +621,7 @@ extern "C" int32_t omi_batch_prepare(omi_batch* b, double dt) {
     if (!(dt > 0.0)) return fail(OMI_ERR_ARG, "dt must be positive");
     int32_t rc;
     if (b->lists_dirty && (rc = rebuild_lists(b))) return rc;
-    if (b->use_graph) return ensure_graph(b, dt);
+    if (b->use_graph && !b->timing) return ensure_graph(b, dt);
     return OMI_OK;
 }
 
@@ -584,7 +631,9 @@ extern "C" int32_t omi_batch_step(omi_batch* b, double dt, int32_t nsteps) {
     int32_t rc;
     if (b->lists_dirty && (rc = rebuild_lists(b))) return rc;
     for (int32_t k = 0; k < nsteps; k++) {
-        if (b->use_graph) {
+        // per-kernel timing events cannot be timed inside a captured graph: timed
+        // steps launch eagerly
+        if (b->use_graph && !b->timing) {
             if ((rc = ensure_graph(b, dt))) return rc;
             HIPCHK(hipGraphLaunch(b->graph, b->stream));
         } else {

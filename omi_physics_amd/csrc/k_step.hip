@@ -169,10 +169,10 @@ __global__ void k_pre(Dev d, int64_t nhw, double dt, double margin) {
             double v2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
             if (v2 > 0.15 * 0.15) atomicOr(&d.w_flags[w], WF_SEQWAKE);
         }
-        if (wf & WF_COLLISION) {
+        if (wf & (WF_COLLISION | WF_JOINTS)) {
             double Rm[9];
             quat_to_matrix(q, Rm);
-            refit_body(d, i, Rm, margin);
+            if (wf & WF_COLLISION) refit_body(d, i, Rm, margin);
             R* rot = (R*)d.rot;
             R* iiw = (R*)d.iiw;
             const double dg[3] = {BODY(d.inv_inertia, 0, i), BODY(d.inv_inertia, 1, i), BODY(d.inv_inertia, 2, i)};
@@ -912,6 +912,230 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
 #undef IIW
 }
 
+
+// ----------------------------------------------------------------- joints --
+
+// Level schedule shared shape (see k_solve): wave 0 assigns levels 64 items at
+// a time, then every thread counts, scans and scatters the items by level.
+// get(k, al, bl) gives item k's world-local dynamic body indices (-1 = none).
+template <int BLOCK, typename Get>
+__device__ int32_t assign_levels(int32_t K, Get get, int32_t* last, int32_t* level, int32_t* lcount,
+                                 int32_t* order, int32_t* s_scan, int32_t* s_D) {
+    const int tid = threadIdx.x;
+    for (int32_t k = tid; k < K; k += BLOCK) lcount[k] = 0;
+    __syncthreads();
+    if (tid < 64) {
+        int32_t D = 0;
+        for (int32_t base = 0; base < K; base += 64) {
+            const int32_t k = base + tid;
+            const bool valid = k < K;
+            int32_t al = -1, bl = -1;
+            if (valid) get(k, al, bl);
+            const int32_t la = al >= 0 ? __hip_atomic_load(&last[al], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+            const int32_t lb = bl >= 0 ? __hip_atomic_load(&last[bl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+            int32_t L = (la > lb ? la : lb) + 1;
+            const int32_t m = K - base < 64 ? K - base : 64;
+            for (int32_t j = 0; j < m - 1; j++) {
+                const int32_t aj = __builtin_amdgcn_readlane(al, j);
+                const int32_t bj = __builtin_amdgcn_readlane(bl, j);
+                const int32_t Lj = __builtin_amdgcn_readlane(L, j);
+                const bool share = (al >= 0 && (al == aj || al == bj)) || (bl >= 0 && (bl == aj || bl == bj));
+                if (tid > j && share && Lj + 1 > L) L = Lj + 1;
+            }
+            if (valid) {
+                if (al >= 0) atomicMax(&last[al], L);
+                if (bl >= 0) atomicMax(&last[bl], L);
+                level[k] = L;
+                D = L > D ? L : D;
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_s_waitcnt(0);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) { int32_t x = __shfl_xor(D, o, 64); D = x > D ? x : D; }
+        if (tid == 0) *s_D = D;
+    }
+    __syncthreads();
+    const int32_t D = *s_D;
+    for (int32_t k = tid; k < K; k += BLOCK) atomicAdd(&lcount[level[k] - 1], 1);
+    __syncthreads();
+    block_exscan<BLOCK>(lcount, D, s_scan);
+    for (int32_t k = tid; k < K; k += BLOCK) order[atomicAdd(&lcount[level[k] - 1], 1)] = k;
+    __syncthreads();
+    return D;
+}
+
+// 3x3 solve by LU with partial pivoting, LAPACK dgetf2 + dgetrs order
+// (np.linalg.solve in PointConstraint.solve, joints.py:119).
+template <typename R>
+__device__ __forceinline__ void solve3(const R* K, const R* rhs, R* x) {
+    R A[3][3], b[3];
+    for (int r = 0; r < 3; r++) { b[r] = rhs[r]; for (int c = 0; c < 3; c++) A[r][c] = K[3 * r + c]; }
+    int piv[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        int p = j;
+        R m = abs_(A[j][j]);
+        for (int i = j + 1; i < 3; i++) if (abs_(A[i][j]) > m) { m = abs_(A[i][j]); p = i; }
+        piv[j] = p;
+        if (A[p][j] != R(0)) {
+            if (p != j) for (int c = 0; c < 3; c++) { R t = A[j][c]; A[j][c] = A[p][c]; A[p][c] = t; }
+            R inv = R(1) / A[j][j];
+            for (int i = j + 1; i < 3; i++) A[i][j] *= inv;
+        }
+        for (int i = j + 1; i < 3; i++)
+            for (int c = j + 1; c < 3; c++) A[i][c] -= A[i][j] * A[j][c];
+    }
+    for (int j = 0; j < 3; j++) if (piv[j] != j) { R t = b[j]; b[j] = b[piv[j]]; b[piv[j]] = t; }
+    for (int i = 1; i < 3; i++) for (int k = 0; k < i; k++) b[i] -= A[i][k] * b[k];
+    for (int i = 2; i >= 0; i--) {
+        for (int k = i + 1; k < 3; k++) b[i] -= A[i][k] * b[k];
+        b[i] /= A[i][i];
+    }
+    x[0] = b[0]; x[1] = b[1]; x[2] = b[2];
+}
+
+// skew(r) @ I @ skew(r) (joints.py:20-23, 96-97)
+template <typename R>
+__device__ __forceinline__ void skew_sandwich(const R* r, const R* I, R* o) {
+    const R S[9] = {R(0), -r[2], r[1], r[2], R(0), -r[0], -r[1], r[0], R(0)};
+    R M[9];
+    for (int i = 0; i < 3; i++)
+        for (int c = 0; c < 3; c++) M[3 * i + c] = (S[3 * i] * I[c] + S[3 * i + 1] * I[3 + c]) + S[3 * i + 2] * I[6 + c];
+    for (int i = 0; i < 3; i++)
+        for (int c = 0; c < 3; c++) o[3 * i + c] = (M[3 * i] * S[c] + M[3 * i + 1] * S[3 + c]) + M[3 * i + 2] * S[6 + c];
+}
+
+enum { JR_K = 0, JR_RA = 9, JR_RB = 12, JR_BIAS = 15, JR_N = 18 };
+
+// PhysicsWorld._solve_joints (world.py:599-611): wake the joint bodies,
+// PointConstraint.prepare every joint (joints.py:84-101), then
+// joint_iterations Gauss-Seidel sweeps of PointConstraint.solve
+// (joints.py:111-121) in joint order, run level by level like the contacts.
+template <typename R, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_joints(Dev d, double dt) {
+    __shared__ int32_t s_scan[BLOCK / 64 + 2];
+    __shared__ int32_t s_D;
+    const int32_t w = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (w >= d.wcap || (d.w_flags[w] & (WF_ACTIVE | WF_JOINTS)) != (WF_ACTIVE | WF_JOINTS)) return;
+    const int32_t J = d.w_jn[w], js = d.w_js[w], n = d.w_cnt[w];
+    const int64_t off = d.w_off[w];
+    if (J == 0) return;
+    const bool fp64 = sizeof(R) == sizeof(double);
+    R* row = (R*)d.jrow;
+    const R* iiw = (const R*)d.iiw;
+#define JROW(c, j) row[(size_t)(c) * d.jcap + (size_t)(j)]
+    for (int32_t t = tid; t < J; t += BLOCK) {
+        const int64_t j = js + t;
+        const int32_t a = d.ja[j], b = d.jb[j];
+        if (a >= 0) { d.awake[a] = 1; d.timer[a] = 0.0; }
+        if (b >= 0) { d.awake[b] = 1; d.timer[b] = 0.0; }
+        R Ia[9], Ib[9], rA[3], rB[3];
+        double rAd[3], rBd[3];
+        uint8_t has = d.jhas[j];
+        for (int s2 = 0; s2 < 2; s2++) {
+            const int32_t i = s2 ? b : a;
+            R* I = s2 ? Ib : Ia;
+            double* rr = s2 ? rBd : rAd;
+            double* loc = s2 ? d.jlb : d.jla;
+            if (i < 0) {
+                for (int r = 0; r < 9; r++) I[r] = R(0);
+                for (int r = 0; r < 3; r++) rr[r] = 0.0;
+                continue;
+            }
+            for (int r = 0; r < 9; r++) I[r] = BODY(iiw, r, i);
+            double q[4] = {BODY(d.quat, 0, i), BODY(d.quat, 1, i), BODY(d.quat, 2, i), BODY(d.quat, 3, i)}, Rm[9];
+            quat_to_matrix(q, Rm);
+            const double p[3] = {BODY(d.pos, 0, i), BODY(d.pos, 1, i), BODY(d.pos, 2, i)};
+            if (!(has & (1 << s2))) {
+                const double dd[3] = {d.janc[j] - p[0], d.janc[d.jcap + j] - p[1], d.janc[2 * d.jcap + j] - p[2]};
+                for (int c = 0; c < 3; c++)
+                    loc[(size_t)c * d.jcap + j] = (Rm[c] * dd[0] + Rm[3 + c] * dd[1]) + Rm[6 + c] * dd[2];
+                has |= (uint8_t)(1 << s2);
+            }
+            const double l[3] = {loc[j], loc[d.jcap + j], loc[2 * d.jcap + j]};
+            for (int r = 0; r < 3; r++) rr[r] = (p[r] + ((Rm[3 * r] * l[0] + Rm[3 * r + 1] * l[1]) + Rm[3 * r + 2] * l[2])) - p[r];
+        }
+        d.jhas[j] = has;
+        for (int r = 0; r < 3; r++) { rA[r] = (R)rAd[r]; rB[r] = (R)rBd[r]; }
+        const R ima = a < 0 ? R(0) : (R)d.inv_mass[a], imb = b < 0 ? R(0) : (R)d.inv_mass[b];
+        R SA[9], SB[9];
+        skew_sandwich(rA, Ia, SA);
+        skew_sandwich(rB, Ib, SB);
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) {
+                R k = (ima + imb) * (r == c ? R(1) : R(0));
+                k -= SA[3 * r + c];
+                k -= SB[3 * r + c];
+                JROW(JR_K + 3 * r + c, j) = k;
+            }
+        for (int r = 0; r < 3; r++) {
+            const double anc = d.janc[(size_t)r * d.jcap + j];
+            const double pa = (a < 0 ? anc : BODY(d.pos, r, a)) + rAd[r];
+            const double pb = (b < 0 ? anc : BODY(d.pos, r, b)) + rBd[r];
+            JROW(JR_RA + r, j) = rA[r]; JROW(JR_RB + r, j) = rB[r];
+            JROW(JR_BIAS + r, j) = (R)(0.2 * (pb - pa));
+        }
+    }
+    int32_t* last = d.last + off;
+    for (int32_t i = tid; i < n; i += BLOCK) last[i] = 0;
+    __syncthreads();
+    auto get = [&](int32_t t, int32_t& al, int32_t& bl) {
+        const int32_t a = d.ja[js + t], b = d.jb[js + t];
+        al = (a >= 0 && d.mt[a] == M_DYNAMIC) ? (int32_t)(a - off) : -1;
+        bl = (b >= 0 && d.mt[b] == M_DYNAMIC) ? (int32_t)(b - off) : -1;
+    };
+    const int32_t D = assign_levels<BLOCK>(J, get, last, d.jlevel + js, d.jcount + js, d.jorder + js, s_scan, &s_D);
+    const int32_t iters = d.w_jiters[w];
+    const R rdt = (R)dt;
+    for (int32_t it = 0; it < iters; it++) {
+        for (int32_t l = 0; l < D; l++) {
+            const int32_t e = d.jcount[js + l], bg = l == 0 ? 0 : d.jcount[js + l - 1];
+            for (int32_t t = bg + tid; t < e; t += BLOCK) {
+                const int64_t j = js + d.jorder[js + t];
+                const int32_t a = d.ja[j], b = d.jb[j];
+                R va[6], vb[6], rA[3], rB[3], Kj[9], rhs[3], P[3];
+                for (int r = 0; r < 3; r++) { rA[r] = JROW(JR_RA + r, j); rB[r] = JROW(JR_RB + r, j); }
+                for (int r = 0; r < 9; r++) Kj[r] = JROW(JR_K + r, j);
+                for (int c = 0; c < 6; c++) { va[c] = a >= 0 ? (R)BODY(d.lv, c, a) : R(0); vb[c] = b >= 0 ? (R)BODY(d.lv, c, b) : R(0); }
+                R pva[3] = {R(0), R(0), R(0)}, pvb[3] = {R(0), R(0), R(0)};
+                if (a >= 0) {
+                    pva[0] = va[0] + (va[4] * rA[2] - va[5] * rA[1]);
+                    pva[1] = va[1] + (va[5] * rA[0] - va[3] * rA[2]);
+                    pva[2] = va[2] + (va[3] * rA[1] - va[4] * rA[0]);
+                }
+                if (b >= 0) {
+                    pvb[0] = vb[0] + (vb[4] * rB[2] - vb[5] * rB[1]);
+                    pvb[1] = vb[1] + (vb[5] * rB[0] - vb[3] * rB[2]);
+                    pvb[2] = vb[2] + (vb[3] * rB[1] - vb[4] * rB[0]);
+                }
+                for (int r = 0; r < 3; r++) rhs[r] = -((pvb[r] - pva[r]) + JROW(JR_BIAS + r, j) / rdt);
+                solve3(Kj, rhs, P);
+                for (int s2 = 0; s2 < 2; s2++) {
+                    const int32_t i = s2 ? b : a;
+                    if (i < 0 || d.mt[i] != M_DYNAMIC) continue;
+                    R* v = s2 ? vb : va;
+                    const R* rr = s2 ? rB : rA;
+                    const R Q[3] = {s2 ? P[0] : -P[0], s2 ? P[1] : -P[1], s2 ? P[2] : -P[2]};
+                    const R im = (R)d.inv_mass[i];
+                    R nv[6];
+                    for (int r = 0; r < 3; r++) nv[r] = v[r] + im * Q[r];
+                    const R cx = rr[1] * Q[2] - rr[2] * Q[1], cy = rr[2] * Q[0] - rr[0] * Q[2], cz = rr[0] * Q[1] - rr[1] * Q[0];
+                    for (int r = 0; r < 3; r++)
+                        nv[3 + r] = v[3 + r] + ((BODY(iiw, 3 * r, i) * cx + BODY(iiw, 3 * r + 1, i) * cy) + BODY(iiw, 3 * r + 2, i) * cz);
+                    for (int c = 0; c < 6; c++) {
+                        if (fp64) BODY(d.lv, c, i) = (double)nv[c];
+                        else BODY(d.lv, c, i) = BODY(d.lv, c, i) + (double)(nv[c] - v[c]);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+#undef JROW
+}
+
 // ------------------------------------------------------- warm-start cache --
 
 __global__ void k_hash_clear(Dev d) {
@@ -1246,6 +1470,11 @@ void launch_cache(const Dev& d, int precision, hipStream_t s) {
     hipLaunchKernelGGL(k_hash_clear, dim3(GRID_FOR(d.hcap, 256)), dim3(256), 0, s, d);
     if (precision == 64) hipLaunchKernelGGL((k_cache<double, 256>), dim3(d.wcap), dim3(256), 0, s, d);
     else hipLaunchKernelGGL((k_cache<float, 256>), dim3(d.wcap), dim3(256), 0, s, d);
+}
+
+void launch_joints(const Dev& d, int precision, double dt, hipStream_t s) {
+    if (precision == 64) hipLaunchKernelGGL((k_joints<double, 256>), dim3(d.wcap), dim3(256), 0, s, d, dt);
+    else hipLaunchKernelGGL((k_joints<float, 256>), dim3(d.wcap), dim3(256), 0, s, d, dt);
 }
 
 void launch_post(const Dev& d, int64_t nhw, double dt, hipStream_t s) {

@@ -11,7 +11,7 @@
 
 namespace omi {
 
-enum : int32_t { WF_ACTIVE = 1, WF_COLLISION = 2, WF_SLEEP = 4, WF_SEQWAKE = 8 };
+enum : int32_t { WF_ACTIVE = 1, WF_COLLISION = 2, WF_SLEEP = 4, WF_SEQWAKE = 8, WF_JOINTS = 16 };
 enum : uint8_t { BF_COLLIDER = 1, BF_TRIGGER = 2 };
 
 // counters (uint64 slots)
@@ -79,6 +79,14 @@ struct Dev {
     void *kimp[2];                        // R [3][kcap]
     uint64_t *hkey[2];
     int32_t *hval[2];
+    // point joints
+    int64_t jcap;
+    int32_t *ja, *jb;                     // [jcap] body index or -1 (the world)
+    double *janc, *jla, *jlb;             // [3][jcap] world anchor, local anchors
+    uint8_t *jhas;                        // bit 0 / 1: local anchor of a / b fixed
+    void *jrow;                           // R [18][jcap]: K (9), rA, rB, bias
+    int32_t *jlevel, *jorder, *jcount;    // level schedule of the joints
+    int32_t *w_js, *w_jn, *w_jiters;      // per-world joint range, iterations
     int32_t *sel;                         // current cache buffer index (device)
     uint64_t *cnt;                        // counters
 };

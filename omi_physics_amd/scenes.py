@@ -171,6 +171,34 @@ def serving_requests(n_requests: int, rate_hz: float, seed: int = 5,
             for i in range(n_requests)]
 
 
+def hinge_chains(n_chains: int = 20, links: int = 50, seed: int = 4, spacing: float = 3.0,
+                 top: float = 60.0) -> Dict[str, Any]:
+    """Chains of 0.2 x 0.2 x 1.0 box links hanging from world anchors, pivots at
+    the link ends (each chain starts tilted so it swings).  Joints are listed in
+    ``spec['joints']`` as (a, b, anchor) with -1 for the world."""
+    rng = np.random.default_rng(seed)
+    spec = new_spec()
+    _ground(spec, half=(40.0, 0.5, 40.0))
+    spec['shapes'].append(('box', (0.2, 1.0, 0.2)))
+    link = len(spec['shapes']) - 1
+    spec['joints'] = []
+    x0 = -0.5 * spacing * (n_chains - 1)
+    for c in range(n_chains):
+        ax = np.array([x0 + c * spacing, top, 0.0])
+        tilt = rng.uniform(-0.6, 0.6)
+        d = np.array([np.sin(tilt), -np.cos(tilt), 0.0])
+        q = (0.0, 0.0, float(np.sin(0.5 * tilt)), float(np.cos(0.5 * tilt)))
+        prev = -1
+        for k in range(links):
+            centre = ax + d * (k + 0.5)
+            spec['bodies'].append(dict(type='dynamic', shape=link, mass=1.0, position=tuple(centre),
+                                       orientation=q, material=0))
+            idx = len(spec['bodies']) - 1
+            spec['joints'].append((prev, idx, tuple(ax + d * k)))
+            prev = idx
+    return spec
+
+
 def build(spec: Dict[str, Any], world_cls: Any, model: Any, **extra) -> Any:
     """Build ``spec`` into ``world_cls(**spec['world'], **extra)`` using ``model``'s types."""
     kwargs = dict(spec['world'])
@@ -196,7 +224,21 @@ def build(spec: Dict[str, Any], world_cls: Any, model: Any, **extra) -> Any:
                                            physicsMaterial=mats[b['material']] if b.get('material', -1) >= 0 else -1),
                    position=tuple(float(x) for x in b['position']),
                    orientation=tuple(float(x) for x in b['orientation']))
+    if spec.get('joints'):
+        joint_cls = extra_joint_cls or _joint_class(world_cls)
+        for a, b, anchor in spec['joints']:
+            w.add_joint_constraint(joint_cls(a, b, anchor))
     return w
+
+
+extra_joint_cls = None
+
+
+def _joint_class(world_cls: Any) -> Any:
+    """The PointConstraint class that belongs with ``world_cls``'s package."""
+    import importlib
+    pkg = world_cls.__module__.rsplit('.', 1)[0]
+    return importlib.import_module(pkg + '.joints').PointConstraint
 
 
 def arrays(spec: Dict[str, Any], gravity=(0.0, -9.81, 0.0), sleep_enabled: bool = True,
@@ -262,4 +304,6 @@ def arrays(spec: Dict[str, Any], gravity=(0.0, -9.81, 0.0), sleep_enabled: bool 
         friction_table=np.array([[fr]]), restitution_table=np.array([[rs]]),
         filt_slot=np.zeros(n, np.int32), n_filt=1, filter_table=np.ones((1, 1), np.uint8),
         gravity=g, default_linear_damping=0.0, default_angular_damping=0.0,
-        sleep_enabled=int(sleep_enabled), velocity_iterations=int(velocity_iterations), has_collision=1)
+        sleep_enabled=int(sleep_enabled), velocity_iterations=int(velocity_iterations), has_collision=1,
+        n_joints=0, joint_iterations=10, joint_a=np.zeros(0, np.int32), joint_b=np.zeros(0, np.int32),
+        joint_anchor=np.zeros((0, 3)))

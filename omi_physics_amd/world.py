@@ -80,6 +80,8 @@ class PhysicsWorld:
         self.time = 0.0
         self._alloc(8)
         self._collision: Any = None
+        self.joint_constraints: List[Any] = []
+        self.joint_iterations = 10
         self._layout_version = 0           # bumps when bodies/tables change
         self._host_dirty = True            # host mirror may be newer than device
         self._device_fresh = False         # device state newer than host mirror
@@ -305,6 +307,18 @@ class PhysicsWorld:
         self._awake[i] = True
         self._sleep_timer[i] = 0.0
 
+    def add_joint_constraint(self, constraint: Any) -> Any:
+        """Register a ``joints.PointConstraint``, waking its bodies (reference ``world.py:590-597``)."""
+        from .joints import PointConstraint
+        if not isinstance(constraint, PointConstraint):
+            raise TypeError('only PointConstraint joints run on the device')
+        self.joint_constraints.append(constraint)
+        for i in (constraint.a, constraint.b):
+            if i >= 0:
+                self.wake(i)
+        self._layout_version += 1
+        return constraint
+
     # -- the step --------------------------------------------------------
     def advance(self, real_dt: float) -> float:
         """Fixed-timestep accumulator (reference ``world.py:399-408``)."""
@@ -326,6 +340,8 @@ class PhysicsWorld:
             self.backend.refit_aabbs(self)
             self._collision.run(self, dt)
         self.backend.integrate_positions(self, dt)
+        if self.joint_constraints:
+            self.backend.solve_joints(self, dt)
         if self.sleep_enabled:
             self.backend.update_sleep(self, dt)
         self.backend.end_step(self, dt)
@@ -468,4 +484,8 @@ def export_world_arrays(w: PhysicsWorld) -> Dict[str, Any]:
         sleep_enabled=int(bool(w.sleep_enabled)),
         velocity_iterations=int(w.velocity_iterations),
         has_collision=int(w._collision is not None),
+        n_joints=len(w.joint_constraints), joint_iterations=int(w.joint_iterations),
+        joint_a=np.array([c.a for c in w.joint_constraints], np.int32),
+        joint_b=np.array([c.b for c in w.joint_constraints], np.int32),
+        joint_anchor=np.array([c.anchor for c in w.joint_constraints], dtype='d').reshape(-1, 3),
     )

@@ -51,6 +51,8 @@ class OWDesc(ctypes.Structure):
         ('default_linear_damping', ctypes.c_double), ('default_angular_damping', ctypes.c_double),
         ('sleep_enabled', ctypes.c_int32), ('velocity_iterations', ctypes.c_int32),
         ('has_collision', ctypes.c_int32),
+        ('n_joints', ctypes.c_int32), ('joint_iterations', ctypes.c_int32),
+        ('joint_a', _i32), ('joint_b', _i32), ('joint_anchor', _d),
     ]
 
 
@@ -102,6 +104,7 @@ def lib() -> ctypes.CDLL:
         L.ow_integrate_forces.argtypes = [ctypes.c_int32, _i32, _u8, _d, _d, _d, _d, _d,
                                           ctypes.c_double, ctypes.c_double, _d, _d, ctypes.c_double]
         L.ow_integrate_positions.argtypes = [ctypes.c_int32, _i32, _u8, _d, _d, _d, _d, ctypes.c_double]
+        L.ow_solve3.argtypes = [_d, _d, _d]
         _lib = L
     return _lib
 
@@ -148,6 +151,14 @@ class OracleWorld:
         d.sleep_enabled = int(arrays['sleep_enabled'])
         d.velocity_iterations = int(arrays['velocity_iterations'])
         d.has_collision = int(arrays['has_collision'])
+        nj = int(arrays.get('n_joints', 0))
+        d.n_joints = nj
+        d.joint_iterations = int(arrays.get('joint_iterations', 10))
+        keep['ja'] = _c(arrays['joint_a'] if nj else np.zeros(1), np.int32)
+        keep['jb'] = _c(arrays['joint_b'] if nj else np.zeros(1), np.int32)
+        keep['janc'] = _c(np.asarray(arrays['joint_anchor']).reshape(-1, 3) if nj else np.zeros((1, 3)), np.float64)
+        d.joint_a = _p(keep['ja'], _i32); d.joint_b = _p(keep['jb'], _i32); d.joint_anchor = _p(keep['janc'], _d)
+        self._keep = keep
         self._h = L.ow_create(ctypes.byref(d))
         self.n = d.n
 
@@ -281,6 +292,12 @@ def gjk_intersect(ka, ca, Ra, ha, pts_a, kb, cb, Rb, hb, pts_b) -> bool:
     A, ka_ = _proxy_args(ka, ca, Ra, ha, pts_a)
     B, kb_ = _proxy_args(kb, cb, Rb, hb, pts_b)
     return bool(lib().ow_gjk_intersect(*A, *B))
+
+
+def solve3(K, rhs) -> np.ndarray:
+    K = _c(K, np.float64); rhs = _c(rhs, np.float64); x = np.zeros(3)
+    lib().ow_solve3(_p(K, _d), _p(rhs, _d), _p(x, _d))
+    return x
 
 
 def prepare_and_solve(a_idx, b_idx, point, normal, depth, pos, lv, av, inv_mass, invIw,

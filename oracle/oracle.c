@@ -937,7 +937,13 @@ void ow_integrate_positions(int32_t n, const int32_t *mt, const uint8_t *awake,
 
 /* ------------------------------------------------------------ world -- */
 
-typedef struct { int32_t a, b, start, count; } CacheGroup;
+typedef struct {
+    int32_t a, b;
+    double anchor[3], la[3], lb[3];
+    int has_la, has_lb;
+    /* prepared each step */
+    double rA[3], rB[3], K[9], bias[3], ima, imb, Ia[9], Ib[9];
+} PJoint;
 
 struct OWorld {
     int32_t n;
@@ -963,6 +969,8 @@ struct OWorld {
     double *ccp, *ccn, *cct;
     int32_t *hkey_start;  /* open-addressing table over (a,b) -> first cache index */
     int64_t hcap;
+    PJoint *joints;
+    int32_t nj, jiters;
 };
 
 static void *dupmem(const void *p, size_t sz) { void *q = malloc(sz ? sz : 1); if (p && sz) memcpy(q, p, sz); return q; }
@@ -998,6 +1006,14 @@ OWorld *ow_create(const ow_desc *d)
     w->hi = (double *)calloc((size_t)(3 * n + 1), sizeof(double));
     w->R = (double *)calloc((size_t)(9 * n + 1), sizeof(double));
     w->invIw = (double *)calloc((size_t)(9 * n + 1), sizeof(double));
+    w->nj = d->n_joints;
+    w->jiters = d->joint_iterations;
+    w->joints = (PJoint *)calloc((size_t)(w->nj > 0 ? w->nj : 1), sizeof(PJoint));
+    for (int32_t j = 0; j < w->nj; j++) {
+        w->joints[j].a = d->joint_a[j];
+        w->joints[j].b = d->joint_b[j];
+        for (int r = 0; r < 3; r++) w->joints[j].anchor[r] = d->joint_anchor[3 * j + r];
+    }
     w->pair_cap = 1024;
     w->pairs = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)w->pair_cap);
     w->solid = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)w->pair_cap);
@@ -1011,7 +1027,7 @@ void ow_destroy(OWorld *w)
                   w->timer, w->half, w->lo, w->hi, w->R, w->invIw, w->mt, w->cs, w->ts, w->kind,
                   w->hull_off, w->hull_cnt, w->mat, w->filt, w->awake, w->hull_pts, w->ftab_f, w->rtab,
                   w->ftab, w->pairs, w->solid, w->ca, w->cb, w->cp, w->cn, w->cd, w->cni, w->cti,
-                  w->cca, w->ccb, w->ccp, w->ccn, w->cct, w->hkey_start};
+                  w->cca, w->ccb, w->ccp, w->ccn, w->cct, w->hkey_start, w->joints};
     for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
     free(w);
 }
@@ -1167,6 +1183,141 @@ static void collide_pair(OWorld *w, int32_t a, int32_t b)   /* narrowphase.py:78
     for (int i = 0; i < cnt; i++) push_contact(w, a, b, pts + 3 * i, nrm, dep[i]);
 }
 
+void ow_solve3(const double *K, const double *rhs, double *x)
+{
+    double A[3][3], b[3];
+    for (int r = 0; r < 3; r++) { b[r] = rhs[r]; for (int c = 0; c < 3; c++) A[r][c] = K[3 * r + c]; }
+    int piv[3];
+    for (int j = 0; j < 3; j++) {
+        int p = j;
+        double m = fabs(A[j][j]);
+        for (int i = j + 1; i < 3; i++) if (fabs(A[i][j]) > m) { m = fabs(A[i][j]); p = i; }
+        piv[j] = p;
+        if (A[p][j] != 0.0) {
+            if (p != j) for (int c = 0; c < 3; c++) { double t = A[j][c]; A[j][c] = A[p][c]; A[p][c] = t; }
+            double inv = 1.0 / A[j][j];
+            for (int i = j + 1; i < 3; i++) A[i][j] *= inv;
+        }
+        for (int i = j + 1; i < 3; i++)
+            for (int c = j + 1; c < 3; c++) A[i][c] -= A[i][j] * A[j][c];
+    }
+    for (int j = 0; j < 3; j++) if (piv[j] != j) { double t = b[j]; b[j] = b[piv[j]]; b[piv[j]] = t; }
+    for (int i = 1; i < 3; i++) for (int k = 0; k < i; k++) b[i] -= A[i][k] * b[k];
+    for (int i = 2; i >= 0; i--) {
+        for (int k = i + 1; k < 3; k++) b[i] -= A[i][k] * b[k];
+        b[i] /= A[i][i];
+    }
+    x[0] = b[0]; x[1] = b[1]; x[2] = b[2];
+}
+
+/* _JointBase._inv_inertia_world (joints.py:35-40): R diag Rᵀ; zero for the world. */
+static void joint_inv_inertia(const OWorld *w, int32_t i, double *o)
+{
+    if (i < 0) { for (int r = 0; r < 9; r++) o[r] = 0.0; return; }
+    double R[9];
+    ow_quat_to_matrix(w->quat + 4 * i, R);
+    const double *d = w->inv_inertia + 3 * i;
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            double p0 = (R[3 * r] * d[0]) * R[3 * c], p1 = (R[3 * r + 1] * d[1]) * R[3 * c + 1];
+            double p2 = (R[3 * r + 2] * d[2]) * R[3 * c + 2];
+            o[3 * r + c] = (p0 + p2) + p1;
+        }
+}
+
+/* skew(r) @ I @ skew(r) (joints.py:20-23, 96-97) */
+static void skew_sandwich(const double *r, const double *I, double *o)
+{
+    const double S[9] = {0.0, -r[2], r[1], r[2], 0.0, -r[0], -r[1], r[0], 0.0};
+    double M[9];
+    for (int i = 0; i < 3; i++)
+        for (int c = 0; c < 3; c++) M[3 * i + c] = (S[3 * i] * I[c] + S[3 * i + 1] * I[3 + c]) + S[3 * i + 2] * I[6 + c];
+    for (int i = 0; i < 3; i++)
+        for (int c = 0; c < 3; c++) o[3 * i + c] = (M[3 * i] * S[c] + M[3 * i + 1] * S[3 + c]) + M[3 * i + 2] * S[6 + c];
+}
+
+/* PointConstraint.prepare (joints.py:84-101) */
+static void joint_prepare(OWorld *w, PJoint *J)
+{
+    joint_inv_inertia(w, J->a, J->Ia);
+    joint_inv_inertia(w, J->b, J->Ib);
+    for (int s = 0; s < 2; s++) {
+        int32_t i = s ? J->b : J->a;
+        double *loc = s ? J->lb : J->la;
+        int *has = s ? &J->has_lb : &J->has_la;
+        double *rr = s ? J->rB : J->rA;
+        if (i < 0) { for (int r = 0; r < 3; r++) rr[r] = J->anchor[r] - J->anchor[r]; continue; }
+        double R[9];
+        ow_quat_to_matrix(w->quat + 4 * i, R);
+        const double *p = w->pos + 3 * i;
+        if (!*has) {
+            double d[3] = {J->anchor[0] - p[0], J->anchor[1] - p[1], J->anchor[2] - p[2]};
+            for (int c = 0; c < 3; c++) loc[c] = (R[c] * d[0] + R[3 + c] * d[1]) + R[6 + c] * d[2];
+            *has = 1;
+        }
+        for (int r = 0; r < 3; r++) {
+            double wa = p[r] + ((R[3 * r] * loc[0] + R[3 * r + 1] * loc[1]) + R[3 * r + 2] * loc[2]);
+            rr[r] = wa - p[r];
+        }
+    }
+    J->ima = J->a < 0 ? 0.0 : w->inv_mass[J->a];
+    J->imb = J->b < 0 ? 0.0 : w->inv_mass[J->b];
+    double SA[9], SB[9];
+    skew_sandwich(J->rA, J->Ia, SA);
+    skew_sandwich(J->rB, J->Ib, SB);
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            double k = (J->ima + J->imb) * (r == c ? 1.0 : 0.0);
+            k -= SA[3 * r + c];
+            k -= SB[3 * r + c];
+            J->K[3 * r + c] = k;
+        }
+    for (int r = 0; r < 3; r++) {
+        double pa = (J->a < 0 ? J->anchor[r] : w->pos[3 * J->a + r]) + J->rA[r];
+        double pb = (J->b < 0 ? J->anchor[r] : w->pos[3 * J->b + r]) + J->rB[r];
+        J->bias[r] = 0.2 * (pb - pa);
+    }
+}
+
+/* PointConstraint.solve (joints.py:111-121) */
+static void joint_solve(OWorld *w, PJoint *J, double dt)
+{
+    double va[3] = {0.0, 0.0, 0.0}, vb[3] = {0.0, 0.0, 0.0}, t[3];
+    if (J->a >= 0) {
+        cross3(w->av + 3 * J->a, J->rA, t);
+        for (int r = 0; r < 3; r++) va[r] = w->lv[3 * J->a + r] + t[r];
+    }
+    if (J->b >= 0) {
+        cross3(w->av + 3 * J->b, J->rB, t);
+        for (int r = 0; r < 3; r++) vb[r] = w->lv[3 * J->b + r] + t[r];
+    }
+    double rhs[3], P[3];
+    for (int r = 0; r < 3; r++) rhs[r] = -((vb[r] - va[r]) + J->bias[r] / dt);
+    ow_solve3(J->K, rhs, P);
+    for (int s = 0; s < 2; s++) {
+        int32_t i = s ? J->b : J->a;
+        if (i < 0) continue;
+        const double *rr = s ? J->rB : J->rA, *I = s ? J->Ib : J->Ia;
+        double Q[3] = {s ? P[0] : -P[0], s ? P[1] : -P[1], s ? P[2] : -P[2]}, c[3];
+        for (int r = 0; r < 3; r++) w->lv[3 * i + r] += w->inv_mass[i] * Q[r];
+        cross3(rr, Q, c);
+        for (int r = 0; r < 3; r++) w->av[3 * i + r] += (I[3 * r] * c[0] + I[3 * r + 1] * c[1]) + I[3 * r + 2] * c[2];
+    }
+}
+
+/* PhysicsWorld._solve_joints (world.py:599-611) */
+static void solve_joints(OWorld *w, double dt)
+{
+    for (int32_t j = 0; j < w->nj; j++) {
+        PJoint *J = &w->joints[j];
+        if (J->a >= 0) { w->awake[J->a] = 1; w->timer[J->a] = 0.0; }
+        if (J->b >= 0) { w->awake[J->b] = 1; w->timer[J->b] = 0.0; }
+        joint_prepare(w, J);
+    }
+    for (int32_t it = 0; it < w->jiters; it++)
+        for (int32_t j = 0; j < w->nj; j++) joint_solve(w, &w->joints[j], dt);
+}
+
 /* PhysicsWorld.step (world.py:410-430) with _CollisionStages.run (:658-676),
  * SequentialImpulseSolver.solve (solver.py:192-213) and _update_sleep
  * (world.py:510-567). */
@@ -1238,6 +1389,7 @@ void ow_step(OWorld *w, double dt)
             free(rest); free(fr); free(ai); free(bi);
         }
     }
+    if (w->nj > 0) solve_joints(w, dt);
     ow_integrate_positions(n, w->mt, w->awake, w->lv, w->av, w->pos, w->quat, dt);
     if (w->sleep_enabled) {
         /* _update_sleep + _contact_islands (world.py:510-567) */

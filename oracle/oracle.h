@@ -48,6 +48,11 @@ typedef struct {
     double gravity[3];
     double default_linear_damping, default_angular_damping;
     int32_t sleep_enabled, velocity_iterations, has_collision;
+    /* point (ball) joints, joints.py:60-121: body indices (-1 = the world) and
+     * the world anchor given at construction */
+    int32_t n_joints, joint_iterations;
+    const int32_t *joint_a, *joint_b;
+    const double *joint_anchor;
 } ow_desc;
 
 typedef struct OWorld OWorld;
@@ -102,6 +107,9 @@ void ow_integrate_forces(int32_t n, const int32_t *motion_type, const uint8_t *a
                          const double *linear_damping, const double *angular_damping,
                          const double *quadratic_drag, double dlin, double dang,
                          double *lv, double *av, double dt);
+/* 3x3 solve with LU and partial pivoting (LAPACK dgetf2 + dgetrs order), as
+ * np.linalg.solve does for PointConstraint.solve (joints.py:119). */
+void ow_solve3(const double *K, const double *rhs, double *x);
 void ow_integrate_positions(int32_t n, const int32_t *motion_type, const uint8_t *awake,
                             const double *lv, const double *av, double *pos, double *quat,
                             double dt);

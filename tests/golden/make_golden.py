@@ -16,7 +16,7 @@ and invariants cannot pin to the bit:
   gjk.npz       gjk.gjk_intersect and gjk.collide_convex on cooked hulls
   hull.npz      hull.convex_hull vertex order
   solver.npz    the solver's prepare / warm start / 10 sweeps / position pass
-  traj_*.npz    PhysicsWorld.step trajectories of small scenes
+  traj_*.npz    PhysicsWorld.step trajectories of small scenes (one with joints)
 
 Only data is written; nothing from the reference is copied.
 """
@@ -263,6 +263,8 @@ def main(ref_src: str) -> None:
     traj('rolling', spec)
     traj('mixed24', scenes.boxes_on_ground(24, seed=11, spread=2.0, y_range=(1.0, 8.0), sphere_fraction=0.5),
          steps=120)
+    # PointConstraint chains (joints.py:60-121), hanging links that swing and touch
+    traj('chain', scenes.hinge_chains(n_chains=3, links=8, seed=5, spacing=1.5, top=10.0), steps=120)
 
 
 if __name__ == '__main__':

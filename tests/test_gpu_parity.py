@@ -305,11 +305,24 @@ def test_fp32_trajectory_error(lib):
     err = np.abs(b.read(0)['position'] - ow.state()['position']).max()
     say('fp32 pile 60 steps max err', err)
     assert err < 5e-3
-    fall = helpers.world(scenes.boxes_on_ground(200, seed=3, y_range=(40.0, 50.0))).export_arrays()
+    spec = scenes.new_spec()
+    scenes._ground(spec)
+    spec['shapes'].append(('box', (1.0, 1.0, 1.0)))
+    for i in range(5):
+        for j in range(5):
+            for k in range(4):      # 2.5 m apart, 30-40 m up: no contact within 60 steps
+                spec['bodies'].append(dict(type='dynamic', shape=1, mass=1.0,
+                                           position=(2.5 * i - 5, 30.0 + 2.5 * k, 2.5 * j - 5),
+                                           orientation=(0.0, 0.0, 0.0, 1.0), material=0,
+                                           angular_velocity=(0.3 * i, 0.1 * j, -0.2 * k)))
+    fall = helpers.world(spec).export_arrays()
     b = batch_from_arrays([fall], precision='fp32')
     ow = oracle.OracleWorld(fall)
     b.step(DT, 60); b.sync(); ow.step(DT, 60)
-    assert np.abs(b.read(0)['position'] - ow.state()['position']).max() < 1e-9
+    assert b.counters()['contacts'] == 0
+    # free flight never touches FP32: positions and orientations are the FP64 ones
+    assert np.array_equal(b.read(0)['position'], ow.state()['position'])
+    assert np.array_equal(b.read(0)['orientation'], ow.state()['orientation'])
 
 
 def test_collide_convex_bit_exact(lib, golden):
@@ -345,3 +358,30 @@ def test_convex_world_bit_exact(lib):
         assert_state_equal(b, ow, 'convex step %d' % s)
         contacts = max(contacts, b.counters()['contacts'])
     assert contacts > 10
+
+
+def test_joint_chains_bit_exact(lib):
+    """PointConstraint chains (joints.py:60-121) with contacts against the ground, 120 steps FP64."""
+    arrays = helpers.world(scenes.hinge_chains(n_chains=3, links=8, seed=5, spacing=1.5,
+                                               top=10.0)).export_arrays()
+    say('3 chains x 8 links, fp64 120 steps')
+    assert len(arrays['joint_a']) == 24
+    for s, b, ow in run_both(arrays, 120, check_every=20):
+        assert_state_equal(b, ow, 'chains step %d' % s)
+
+
+def test_physicsworld_pendulum_on_hip(lib):
+    """A ball joint to the world keeps a pendulum's bob at its rod length (joints.py:60-121)."""
+    from omi_physics_amd.joints import PointConstraint
+    w = PhysicsWorld(gravity=model.Gravity(gravity=9.81), sleep_enabled=False)
+    s = w.add_shape(model.Shape.sphere(0.1))
+    bob = w.add_body(model.Motion(mass=1.0), collider=model.Collider(shape=s), position=(1.0, 2.0, 0.0))
+    w.add_joint_constraint(PointConstraint(-1, bob, (0.0, 2.0, 0.0)))
+    ow = oracle.OracleWorld(w.export_arrays())
+    for _ in range(240):
+        w.step(1.0 / 120.0)
+        r = np.linalg.norm(np.asarray(w.position[bob]) - np.array([0.0, 2.0, 0.0]))
+        assert r == pytest.approx(1.0, abs=0.02)
+    ow.step(1.0 / 120.0, 240)
+    assert np.array_equal(np.asarray(w.position), ow.state()['position'])
+    assert w.position[bob][1] < 2.0

@@ -186,3 +186,46 @@ def test_mutation_is_caught():
     oracle.prepare_and_solve(g['a0'], g['b0'], g['point0'], -g['normal0'], g['depth0'], pos, lv, av,
                              g['inv_mass_in0'], g['invIw0'], g['rest0'], g['fric0'], ni, ti)
     assert np.abs(lv - g['lv_out0']).max() > 1e-3
+
+
+def test_solve3_matches_lapack():
+    """3x3 LU with partial pivoting against LAPACK, including systems whose first pivot is zero
+    or smallest (row swaps in both elimination steps)."""
+    rng = np.random.default_rng(5)
+    systems = [(np.array([[0.0, 2.0, 1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 3.0]]), np.array([1.0, 2.0, 3.0])),
+               (np.array([[1e-9, 1.0, 0.0], [1.0, 1.0, 0.0], [0.0, 5.0, 1.0]]), np.array([1.0, 0.0, -1.0]))]
+    for _ in range(2000):
+        K = rng.normal(size=(3, 3)) + 3 * np.eye(3) * rng.uniform(0, 1)
+        systems.append((K, rng.normal(size=3)))
+    for K, r in systems:
+        np.testing.assert_allclose(oracle.solve3(K, r), np.linalg.solve(K, r), rtol=1e-9, atol=1e-11)
+
+
+def test_joint_chain_matches_reference(golden):
+    """PointConstraint chains (joints.py:60-121): within 1e-7 m for 60 steps.  Adjacent links start
+    end-face to end-face at depth ~0, so a last-bit difference (LAPACK/BLAS vs the restatement) can
+    keep or drop a zero-depth clip point: contact counts agree within 2."""
+    from omi_physics_amd import scenes
+    g = golden('traj_chain.npz')
+    ow = oracle.OracleWorld(helpers.world(scenes.hinge_chains(n_chains=3, links=8, seed=5, spacing=1.5,
+                                                               top=10.0)).export_arrays())
+    s = 0
+    for idx, st in enumerate(g['steps']):
+        ow.step(1.0 / 60.0, int(st) - s); s = int(st)
+        assert abs(ow.counters()['contacts'] - g['ncontacts'][idx]) <= 2, st
+        if st <= 60:
+            np.testing.assert_allclose(ow.state()['position'], g['pos'][idx], atol=1e-7, rtol=0)
+
+
+def test_pendulum_keeps_its_length():
+    """Invariant: a ball joint to the world holds the anchor distance (Baumgarte-corrected)."""
+    from omi_physics_amd import PhysicsWorld, model
+    from omi_physics_amd.joints import PointConstraint
+    w = PhysicsWorld(gravity=model.Gravity(gravity=9.81, direction=(0, -1, 0)), sleep_enabled=False)
+    b = w.add_body(model.Motion(mass=1.0), position=(1.0, 5.0, 0.0))
+    w.add_joint_constraint(PointConstraint(-1, b, (0.0, 5.0, 0.0)))
+    ow = oracle.OracleWorld(w.export_arrays())
+    for _ in range(120):
+        ow.step(1.0 / 60.0)
+        p = ow.state()['position'][0]
+        assert abs(np.linalg.norm(p - [0.0, 5.0, 0.0]) - 1.0) < 0.02
