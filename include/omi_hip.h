@@ -160,6 +160,9 @@ typedef struct {
      * 64, 128, 256, 512, 1024; 0 picks the default.  The autotuner sets these. */
     int32_t sap_block[4];
     int32_t solve_block[4];
+    /* x-window candidates of the sweep-and-prune per step, all worlds, that
+     * the broadphase tiles can hold (0 = 32 * max_pairs + 2^20). */
+    int64_t max_candidates;
 } omi_batch_config;
 
 typedef struct omi_batch omi_batch;

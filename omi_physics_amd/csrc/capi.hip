@@ -19,7 +19,7 @@ void launch_begin(const Dev& d, hipStream_t s);
 void launch_pre(const Dev& d, int precision, int64_t nhw, double dt, double margin, hipStream_t s);
 void launch_refit(const Dev& d, int64_t nhw, double margin, hipStream_t s);
 void launch_sap(const Dev& d, const int32_t* wl, const int32_t* wn, const int32_t* np_class, const int32_t* blocks,
-                const int32_t* live, int32_t record, hipStream_t s);
+                const int32_t* live, int32_t record, int32_t grid, hipStream_t s);
 void launch_narrow(const Dev& d, int precision, hipStream_t s);
 void launch_compact(const Dev& d, int precision, int32_t record, hipStream_t s);
 void launch_wake(const Dev& d, hipStream_t s);
@@ -122,6 +122,7 @@ struct omi_batch {
     int32_t sap_block[4] = {64, 128, 256, 1024};
     int32_t solve_block[4] = {64, 128, 256, 256};
     int32_t np_lds_max = 4096;
+    int32_t sap_grid = 2048;               // blocks of the chip-wide broadphase pair passes
     bool lists_dirty = true;
     int64_t nhw = 0;
     bool use_graph = true;
@@ -201,6 +202,9 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     int smem = 65536;
     hipDeviceGetAttribute(&smem, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     b->np_lds_max = smem >= 4096 * 16 + 1024 ? 4096 : 2048;
+    int cus = 256;
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    b->sap_grid = std::max(8, cus) * 8;
     b->np_class[2] = b->np_lds_max;
     b->lds_bodies[2] = b->np_lds_max;
     {
@@ -232,12 +236,20 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     A(d.mt, N); A(d.kind, N); A(d.bflags, N); A(d.awake, N); A(d.awake0, N); A(d.contacted, N); A(d.notready, N);
     A(d.parent, N); A(d.last, N);
     A(d.skey, 2 * N); A(d.sidx, 2 * N); A(d.soff, 2 * N);
+    A(d.sbox, 4 * N); A(d.sinfo, N); A(d.sorig, N);
     A(d.hull, 3 * std::max<int64_t>(cfg->max_hull_pts, 1));
     A(d.w_off, W); A(d.w_cnt, W); A(d.w_flags, W); A(d.w_iters, W);
     A(d.w_mat_off, W); A(d.w_nmat, W); A(d.w_filt_off, W); A(d.w_nfilt, W);
     A(d.w_grav, 3 * W); A(d.w_damp, 2 * W);
     A(d.w_ps, W); A(d.w_pn, W); A(d.w_bps, W); A(d.w_bpn, W); A(d.w_cs, W); A(d.w_cn, W); A(d.w_levels, W);
     A(d.w_cache_s, 2 * W); A(d.w_cache_n, 2 * W);
+    A(d.w_x, W); A(d.w_t0, W); A(d.w_tn, W);
+    {
+        // broadphase candidate tiles: max_candidates x-window candidates per step
+        const int64_t xc = cfg->max_candidates > 0 ? cfg->max_candidates : 32 * P + (1 << 20);
+        d.tcap = std::min<int64_t>(xc / SAP_TILE + W + 1, (int64_t)1 << 30);
+    }
+    A(d.tworld, d.tcap); A(d.tcnt, 2 * (d.tcap + 1));
     const int64_t TAB = std::max<int64_t>(64 * (int64_t)W, 1024);
     A(d.mat_fric, TAB); A(d.mat_rest, TAB); A(d.filt_tab, TAB);
     A(d.pa, P); A(d.pb, P);
@@ -555,7 +567,8 @@ static void enqueue_step(omi_batch* b, double dt) {
     mark();
     launch_begin(d, s); mark();
     launch_pre(d, prec, b->nhw, dt, 0.05, s); mark();
-    launch_sap(d, b->d_wl, b->d_wn, b->np_class, b->sap_block, b->live, b->cfg.record_broadphase, s); mark();
+    launch_sap(d, b->d_wl, b->d_wn, b->np_class, b->sap_block, b->live, b->cfg.record_broadphase, b->sap_grid, s);
+    mark();
     launch_narrow(d, prec, s); mark();
     launch_compact(d, prec, b->cfg.record_broadphase, s); mark();
     launch_wake(d, s); mark();
@@ -659,11 +672,13 @@ extern "C" int32_t omi_batch_sync(omi_batch* b) {
     uint64_t ov = 0;
     HIPCHK(hipMemcpy(&ov, b->d.cnt + C_OVERFLOW, sizeof(uint64_t), hipMemcpyDeviceToHost));
     if (ov) {
-        char msg[256];
+        // the flags are sticky over the steps since the last sync; report and clear them
+        HIPCHK(hipMemset(b->d.cnt + C_OVERFLOW, 0, sizeof(uint64_t)));
+        char msg[320];
         snprintf(msg, sizeof msg,
-                 "batch capacity exceeded during the last step (flags 0x%llx: 1 pairs, 2 contacts, 4 cache, "
-                 "8 broadphase record); results of that step are invalid — recreate the batch with larger "
-                 "max_pairs/max_contacts",
+                 "batch capacity exceeded in a step since the last sync (flags 0x%llx: 1 pairs, 2 contacts, "
+                 "4 cache, 8 broadphase record, 16 broadphase candidates); results of that step are invalid "
+                 "— recreate the batch with larger max_pairs/max_contacts/max_candidates",
                  (unsigned long long)ov);
         return fail(OMI_ERR_CAPACITY, msg);
     }
@@ -855,8 +870,9 @@ extern "C" int32_t omi_sphere_box(int32_t np, int32_t precision, const double* c
 
 // A one-world batch whose arrays the stage entry points fill directly.
 static int32_t mini_batch(int64_t nb, int64_t npairs, int64_t ncon, int32_t precision, int32_t record,
-                          omi_batch** out) {
+                          omi_batch** out, int64_t ncand = 0) {
     omi_batch_config c{};
+    c.max_candidates = ncand;
     c.max_worlds = 1;
     c.precision = precision;
     c.max_bodies = std::max<int64_t>(nb, 1);
@@ -913,9 +929,10 @@ extern "C" int32_t omi_sap_pairs(int32_t n, const double* lo, const double* hi, 
     *count = 0;
     if (n < 2) return OMI_OK;
     int64_t pcap = std::max<int64_t>(std::min<int64_t>((int64_t)n * 16, 1 << 26), 1024);
+    int64_t xcap = std::max<int64_t>((int64_t)n * 64, 1 << 16);
     for (;;) {
         omi_batch* b = nullptr;
-        int32_t rc = mini_batch(n, pcap, 1, 64, 1, &b);
+        int32_t rc = mini_batch(n, pcap, 1, 64, 1, &b, xcap);
         if (rc) return rc;
         Dev& d = b->d;
         std::vector<uint8_t> mt((size_t)n), bf((size_t)n, BF_COLLIDER), aw((size_t)n, 1);
@@ -942,14 +959,20 @@ extern "C" int32_t omi_sap_pairs(int32_t n, const double* lo, const double* hi, 
         b->slots[0].n = n;
         if ((rc = rebuild_lists(b))) { omi_batch_destroy(b); return rc; }
         launch_begin(d, b->stream);
-        launch_sap(d, b->d_wl, b->d_wn, b->np_class, b->sap_block, b->live, 1, b->stream);
+        launch_sap(d, b->d_wl, b->d_wn, b->np_class, b->sap_block, b->live, 1, b->sap_grid, b->stream);
         if (hipGetLastError() != hipSuccess) { omi_batch_destroy(b); return fail(OMI_ERR_HIP, "broadphase launch failed"); }
         uint64_t c[C_COUNT];
         hipStreamSynchronize(b->stream);
         hipMemcpy(c, d.cnt, sizeof c, hipMemcpyDeviceToHost);
+        if (c[C_OVERFLOW] & 16ull) {
+            omi_batch_destroy(b);
+            if (c[C_XCAND] == 0) return fail(OMI_ERR_CAPACITY, "more than 2^31 x-window candidates in one world");
+            xcap = (int64_t)c[C_XCAND] + 2 * SAP_TILE;
+            continue;
+        }
         if (c[C_OVERFLOW] & 8ull) {
             omi_batch_destroy(b);
-            pcap = (int64_t)c[C_BPAIRS] + 16;
+            pcap = (int64_t)c[C_BPTOTAL] + 16;
             continue;
         }
         int64_t np_ = (int64_t)c[C_BPTOTAL];

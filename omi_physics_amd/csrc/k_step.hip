@@ -99,7 +99,7 @@ __device__ __forceinline__ int64_t block_sum(int64_t v, int64_t* scratch) {
 
 __global__ void k_begin(Dev d) {
     int64_t i = gtid();
-    if (i < C_COUNT) d.cnt[i] = 0;
+    if (i < C_COUNT && i != C_OVERFLOW) d.cnt[i] = 0;   // overflow flags stay until the host reads them
     for (int64_t w = i; w < d.wcap; w += gstride()) d.w_flags[w] &= ~WF_SEQWAKE;
 }
 
@@ -201,42 +201,39 @@ __global__ void k_refit(Dev d, int64_t nhw, double margin) {
 
 // ------------------------------------------------------------ broadphase --
 
-// Pair test of the reference broadphase (broadphase.py:280-294) and the solid
-// filter (world.py:663-670) for world-local sorted pair (a, b).
-__device__ __forceinline__ void sap_eval(const Dev& d, int64_t a, int64_t b, int32_t fo, int32_t nf,
-                                         bool& bp, bool& solid) {
-    bool keep = (BODY(d.hi, 1, a) >= BODY(d.lo, 1, b)) && (BODY(d.hi, 1, b) >= BODY(d.lo, 1, a)) &&
-                (BODY(d.hi, 2, a) >= BODY(d.lo, 2, b)) && (BODY(d.hi, 2, b) >= BODY(d.lo, 2, a));
-    const int ma = d.mt[a], mb = d.mt[b];
-    keep = keep && (ma == M_DYNAMIC || mb == M_DYNAMIC);
-    const int64_t p = a < b ? a : b, q = a < b ? b : a;
-    keep = keep && d.filt_tab[fo + d.filt[p] * nf + d.filt[q]] != 0;
-    bp = keep;
-    const uint8_t fa = d.bflags[a], fb = d.bflags[b];
-    const bool am = (ma == M_DYNAMIC && d.awake[a]) || ma == M_KINEMATIC;
-    const bool bm = (mb == M_DYNAMIC && d.awake[b]) || mb == M_KINEMATIC;
-    solid = keep && !((fa | fb) & BF_TRIGGER) && (fa & BF_COLLIDER) && (fb & BF_COLLIDER) && (am || bm);
-}
+// BroadPhase.pairs (broadphase.py:243-294), in three passes.
+//
+// k_sap_sort, one block per world: bitonic sort of (lo.x key, index) — the
+// stable argsort order — in LDS (global scratch for large worlds); the
+// candidate window of sorted body k is [k+1, searchsorted(lo_sx, hi_sx[k],
+// 'right')); so[k] is the exclusive prefix of the window lengths, so
+// candidate c of the world is (k, k + 1 + c - so[k]) for the last k with
+// so[k] <= c.  The y/z bounds, motion/filter flags and original index are
+// written in sorted order, and the world reserves ceil(X / SAP_TILE) tiles of
+// its X candidates.
+//
+// k_sap_pairs<COUNT>, the whole chip: each wavefront walks a contiguous run of
+// tiles, 64 candidates at a time, and counts the pairs that pass the y/z
+// test, the motion/filter test and the solid filter (world.py:663-670).
+// k_sap_scan turns the per-tile counts into output offsets and per-world
+// pair ranges; k_sap_pairs<WRITE> walks again and writes the pairs in
+// candidate order, i.e. the reference's order.
+enum : uint32_t { SI_DYN = 1, SI_MOVING = 4, SI_COLLIDER = 8, SI_TRIGGER = 16 };   // filter slot << 8
 
-// BroadPhase.pairs (broadphase.py:243-294) for one world per block: bitonic
-// sort of (lo.x key, index) — the stable argsort order — then the candidate
-// windows [k+1, searchsorted(lo_sx, hi_sx[k], 'right')) flattened over the
-// block so one long window (a ground box) does not serialise a thread, then
-// an order-preserving compaction of the kept pairs into this world's slice.
 template <int BLOCK, bool GLOBAL>
-__global__ __launch_bounds__(BLOCK) void k_sap(Dev d, const int32_t* wlist, const int32_t* wcount, int32_t np_max,
-                                               int32_t record_bp) {
+__global__ __launch_bounds__(BLOCK) void k_sap_sort(Dev d, const int32_t* wlist, const int32_t* wcount,
+                                                    int32_t np_max) {
     extern __shared__ unsigned char smem[];
     __shared__ int32_t s_scan[BLOCK / 64 + 2];
     __shared__ int64_t s_sum[BLOCK / 64 + 2];
-    __shared__ int32_t s_base[4];
+    __shared__ int32_t s_t0;
     if ((int32_t)blockIdx.x >= *wcount) return;
     const int32_t w = wlist[blockIdx.x];
     const int tid = threadIdx.x;
     const int32_t flags = d.w_flags[w], n = d.w_cnt[w];
     const int64_t off = d.w_off[w];
     if (!(flags & WF_ACTIVE) || !(flags & WF_COLLISION) || n < 2) {
-        if (tid == 0) { d.w_pn[w] = 0; d.w_bpn[w] = 0; }
+        if (tid == 0) { d.w_x[w] = 0; d.w_tn[w] = 0; }
         return;
     }
     int32_t NP = 1;
@@ -266,78 +263,246 @@ __global__ __launch_bounds__(BLOCK) void k_sap(Dev d, const int32_t* wlist, cons
             }
             __syncthreads();
         }
+    int64_t part = 0;
     for (int32_t k = tid; k < n; k += BLOCK) {
-        uint64_t hk = order_key(BODY(d.hi, 0, off + idx[k]));
+        const int64_t g = off + idx[k];
+        const uint64_t hk = order_key(BODY(d.hi, 0, g));
         int32_t lo_i = 0, hi_i = n;
         while (lo_i < hi_i) {
             int32_t mid = (lo_i + hi_i) >> 1;
             if (key[mid] <= hk) lo_i = mid + 1; else hi_i = mid;
         }
-        int32_t c = lo_i - k - 1;
-        coff[k] = c > 0 ? c : 0;
+        const int32_t c = lo_i - k - 1 > 0 ? lo_i - k - 1 : 0;
+        coff[k] = c;
+        part += c;
+        // the sorted record of body g
+        const int mt = d.mt[g];
+        const uint8_t bf = d.bflags[g];
+        uint32_t info = (uint32_t)d.filt[g] << 8;
+        if (mt == M_DYNAMIC) info |= SI_DYN;
+        if ((mt == M_DYNAMIC && d.awake[g]) || mt == M_KINEMATIC) info |= SI_MOVING;
+        if (bf & BF_COLLIDER) info |= SI_COLLIDER;
+        if (bf & BF_TRIGGER) info |= SI_TRIGGER;
+        const int64_t s = off + k;
+        d.sbox[s] = BODY(d.lo, 1, g);
+        d.sbox[d.ncap + s] = BODY(d.hi, 1, g);
+        d.sbox[2 * d.ncap + s] = BODY(d.lo, 2, g);
+        d.sbox[3 * d.ncap + s] = BODY(d.hi, 2, g);
+        d.sinfo[s] = info;
+        d.sorig[s] = idx[k];
+    }
+    const int64_t X = block_sum<BLOCK>(part, s_sum);
+    if (X > 0x7fffffffll - SAP_TILE) {
+        // more x-window candidates than one world's 32-bit candidate index holds
+        if (tid == 0) { atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 16ull); d.w_x[w] = 0; d.w_tn[w] = 0; }
+        return;
     }
     __syncthreads();
-    const int32_t X = block_exscan<BLOCK>(coff, n, s_scan);
-    const int32_t fo = d.w_filt_off[w], nf = d.w_nfilt[w];
-    auto candidate = [&](int32_t c, int64_t& a, int64_t& b) {
-        int32_t lo_i = 0, hi_i = n;            // last k with coff[k] <= c
-        while (lo_i < hi_i) {
-            int32_t mid = (lo_i + hi_i) >> 1;
-            if (coff[mid] <= c) lo_i = mid + 1; else hi_i = mid;
-        }
-        int32_t k = lo_i - 1;
-        int32_t j = k + 1 + (c - coff[k]);
-        a = off + idx[k];
-        b = off + idx[j];
-    };
-    int64_t nbp = 0, nsol = 0;
-    for (int32_t c = tid; c < X; c += BLOCK) {
-        int64_t a, b;
-        bool bp, so;
-        candidate(c, a, b);
-        sap_eval(d, a, b, fo, nf, bp, so);
-        nbp += bp; nsol += so;
-    }
-    const int64_t tbp = block_sum<BLOCK>(nbp, s_sum);
-    const int64_t tsol = block_sum<BLOCK>(nsol, s_sum);
+    block_exscan<BLOCK>(coff, n, s_scan);
+    int32_t* so = d.soff + 2 * off;
+    if (!GLOBAL)
+        for (int32_t k = tid; k < n; k += BLOCK) so[k] = coff[k];
+    const int32_t T = (int32_t)((X + SAP_TILE - 1) / SAP_TILE);
     if (tid == 0) {
-        uint64_t base = atomicAdd((unsigned long long*)&d.cnt[C_PAIRS], (unsigned long long)tsol);
-        int32_t ok = base + tsol <= (uint64_t)d.pcap;
-        if (!ok) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 1ull);
-        s_base[0] = (int32_t)base; s_base[1] = ok;
-        d.w_ps[w] = (int32_t)base; d.w_pn[w] = ok ? (int32_t)tsol : 0;
-        int32_t okb = 0;
-        if (record_bp) {
-            uint64_t bb = atomicAdd((unsigned long long*)&d.cnt[C_BPAIRS], (unsigned long long)tbp);
-            okb = bb + tbp <= (uint64_t)d.pcap;
-            if (!okb) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 8ull);
-            s_base[2] = (int32_t)bb;
-            d.w_bps[w] = (int32_t)bb;
-        }
-        s_base[3] = okb;
-        d.w_bpn[w] = okb ? (int32_t)tbp : 0;
+        so[n] = (int32_t)X;
+        d.w_x[w] = X;
         atomicAdd((unsigned long long*)&d.cnt[C_XCAND], (unsigned long long)X);
-        atomicAdd((unsigned long long*)&d.cnt[C_BPTOTAL], (unsigned long long)tbp);
+        const uint64_t t0 = T ? atomicAdd((unsigned long long*)&d.cnt[C_TILES], (unsigned long long)T) : 0ull;
+        const bool ok = t0 + (uint64_t)T <= (uint64_t)d.tcap;
+        if (!ok) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 16ull);
+        d.w_t0[w] = (int32_t)(ok ? t0 : 0);
+        d.w_tn[w] = ok ? T : 0;
+        s_t0 = ok ? (int32_t)t0 : -1 - (int32_t)(t0 < (uint64_t)d.tcap ? t0 : (uint64_t)d.tcap);
     }
     __syncthreads();
-    const bool ok = s_base[1] != 0, okb = s_base[3] != 0;
-    int32_t ws = s_base[0], wb = s_base[2];
-    for (int32_t cb = 0; cb < X; cb += BLOCK) {
-        int32_t c = cb + tid;
-        int64_t a = 0, b = 0;
-        bool bp = false, so = false;
-        if (c < X) { candidate(c, a, b); sap_eval(d, a, b, fo, nf, bp, so); }
-        const int64_t p = a < b ? a : b, q = a < b ? b : a;
-        int32_t tot;
-        int32_t pre = block_flag_prefix<BLOCK>(so, s_scan, tot);
-        if (so && ok) { d.pa[ws + pre] = (int32_t)p; d.pb[ws + pre] = (int32_t)q; }
-        ws += tot;
-        if (record_bp) {
-            int32_t totb;
-            int32_t preb = block_flag_prefix<BLOCK>(bp, s_scan, totb);
-            if (bp && okb) { d.bpa[wb + preb] = (int32_t)p; d.bpb[wb + preb] = (int32_t)q; }
-            wb += totb;
+    // tile -> world map; tiles of a world that did not fit are marked empty
+    const int32_t t0 = s_t0 >= 0 ? s_t0 : -1 - s_t0;
+    const int32_t tend = s_t0 >= 0 ? t0 + T : (t0 + T < d.tcap ? t0 + T : (int32_t)d.tcap);
+    for (int32_t t = t0 + tid; t < tend; t += BLOCK) d.tworld[t] = s_t0 >= 0 ? w : -1;
+}
+
+struct SapRec {
+    double ylo, yhi, zlo, zhi;
+    uint32_t info;
+    int32_t orig;
+};
+
+__device__ __forceinline__ SapRec sap_rec(const Dev& d, int64_t s) {
+    SapRec r;
+    r.ylo = d.sbox[s]; r.yhi = d.sbox[d.ncap + s]; r.zlo = d.sbox[2 * d.ncap + s]; r.zhi = d.sbox[3 * d.ncap + s];
+    r.info = d.sinfo[s]; r.orig = d.sorig[s];
+    return r;
+}
+
+// Pair test of the reference broadphase (broadphase.py:280-294) and the solid
+// filter (world.py:663-670) on two sorted records.
+__device__ __forceinline__ void sap_test(const Dev& d, const SapRec& A, const SapRec& B, int32_t fo, int32_t nf,
+                                         bool& bp, bool& solid) {
+    bool keep = (A.yhi >= B.ylo) && (B.yhi >= A.ylo) && (A.zhi >= B.zlo) && (B.zhi >= A.zlo);
+    keep = keep && ((A.info | B.info) & SI_DYN);
+    if (keep) {
+        const bool af = A.orig < B.orig;
+        const uint32_t fp = (af ? A.info : B.info) >> 8, fq = (af ? B.info : A.info) >> 8;
+        keep = d.filt_tab[fo + (int32_t)fp * nf + (int32_t)fq] != 0;
+    }
+    bp = keep;
+    solid = keep && !((A.info | B.info) & SI_TRIGGER) && (A.info & B.info & SI_COLLIDER) &&
+            ((A.info | B.info) & SI_MOVING);
+}
+
+template <int BLOCK, bool WRITE>
+__global__ __launch_bounds__(BLOCK) void k_sap_pairs(Dev d, int32_t record) {
+    constexpr int NW = BLOCK / 64;
+    const int lane = threadIdx.x & 63;
+    const uint64_t tt = d.cnt[C_TILES];
+    const int64_t T = tt < (uint64_t)d.tcap ? (int64_t)tt : d.tcap;
+    // Blocks b and b + 8 run on the same XCD: give each XCD's blocks adjacent
+    // tile ranges so a world's sorted records stay in one L2.
+    const int32_t G = gridDim.x, b = blockIdx.x;
+    const int32_t bx = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+    const int64_t waves = (int64_t)G * NW, wv = (int64_t)bx * NW + (threadIdx.x >> 6);
+    const int64_t t_lo = T * wv / waves, t_hi = T * (wv + 1) / waves;
+    for (int64_t t = t_lo; t < t_hi; t++) {
+        const int32_t w = d.tworld[t];
+        if (w < 0) {
+            if (!WRITE && lane == 0) { d.tcnt[t] = 0; d.tcnt[d.tcap + 1 + t] = 0; }
+            continue;
         }
+        const int64_t off = d.w_off[w];
+        const int32_t n = d.w_cnt[w];
+        const int32_t fo = d.w_filt_off[w], nf = d.w_nfilt[w];
+        const int32_t* so = d.soff + 2 * off;
+        const int32_t X = so[n];
+        const int32_t c0 = (int32_t)(t - d.w_t0[w]) * SAP_TILE;
+        const int32_t c1 = X - c0 < SAP_TILE ? X : c0 + SAP_TILE;
+        // k of candidate c0: the last k with so[k] <= c0
+        int32_t kw;
+        {
+            int32_t lo_i = 0, hi_i = n;
+            while (lo_i < hi_i) {
+                int32_t mid = (lo_i + hi_i) >> 1;
+                if (so[mid] <= c0) lo_i = mid + 1; else hi_i = mid;
+            }
+            kw = lo_i - 1;
+        }
+        int64_t out_s = 0, out_b = 0;
+        if (WRITE) { out_s = d.tcnt[t]; out_b = d.tcnt[d.tcap + 1 + t]; }
+        int32_t cnt_s = 0, cnt_b = 0;
+        for (int32_t cb = c0; cb < c1; cb += 64) {
+            const int32_t c = cb + lane;
+            const bool valid = c < c1;
+            // k of this lane's candidate: with v_l = so[kb + 1 + l] - cb (ascending
+            // over the lanes), k = kb + #{l : v_l <= lane}; rounds of 64 entries
+            // until every valid lane has found its k.
+            int32_t k = -1;
+            for (int32_t kb = kw;; kb += 64) {
+                const int32_t e = kb + 1 + lane;
+                const int32_t v = e <= n ? so[e] - cb : 0x7fffffff;
+                int32_t m = 0;
+#pragma unroll
+                for (int step = 32; step > 0; step >>= 1) {
+                    const int32_t probe = __shfl(v, m + step - 1, 64);
+                    if (probe <= lane) m += step;
+                }
+                if (m == 63 && __shfl(v, 63, 64) <= lane) m = 64;
+                if (k < 0 && m < 64) k = kb + m;
+                if (!__any(valid && k < 0)) break;
+            }
+            bool bp = false, sol = false;
+            int64_t p = 0, q = 0;
+            if (valid) {
+                const int32_t j = k + 1 + (c - so[k]);
+                const SapRec A = sap_rec(d, off + k), B = sap_rec(d, off + j);
+                sap_test(d, A, B, fo, nf, bp, sol);
+                p = off + (A.orig < B.orig ? A.orig : B.orig);
+                q = off + (A.orig < B.orig ? B.orig : A.orig);
+            }
+            const uint64_t ms = __ballot(sol), mb = __ballot(bp);
+            const uint64_t below = (1ull << lane) - 1ull;
+            if (WRITE) {
+                if (sol) {
+                    const int64_t o = out_s + cnt_s + __popcll(ms & below);
+                    if (o < d.pcap) { d.pa[o] = (int32_t)p; d.pb[o] = (int32_t)q; }
+                }
+                if (record && bp) {
+                    const int64_t o = out_b + cnt_b + __popcll(mb & below);
+                    if (o < d.pcap) { d.bpa[o] = (int32_t)p; d.bpb[o] = (int32_t)q; }
+                }
+            }
+            cnt_s += __popcll(ms);
+            cnt_b += __popcll(mb);
+            kw = __builtin_amdgcn_readlane(k, 63);
+        }
+        if (!WRITE && lane == 0) { d.tcnt[t] = cnt_s; d.tcnt[d.tcap + 1 + t] = cnt_b; }
+    }
+}
+
+// Per-tile counts -> exclusive offsets (in place, one block), then every
+// world's solid and broadphase pair ranges.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sap_scan(Dev d, int32_t record) {
+    constexpr int NW = BLOCK / 64, ITEMS = 8;
+    __shared__ int64_t s_w[2][NW];
+    __shared__ int64_t s_run[2];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint64_t tt = d.cnt[C_TILES];
+    const int64_t T = tt < (uint64_t)d.tcap ? (int64_t)tt : d.tcap;
+    int64_t* arr[2] = {d.tcnt, d.tcnt + d.tcap + 1};
+    if (tid < 2) s_run[tid] = 0;
+    __syncthreads();
+    for (int64_t base = 0; base < T; base += (int64_t)BLOCK * ITEMS) {
+        const int64_t i0 = base + (int64_t)tid * ITEMS;
+        int64_t v[2][ITEMS], sum[2] = {0, 0};
+        for (int a = 0; a < 2; a++)
+            for (int r = 0; r < ITEMS; r++) {
+                v[a][r] = i0 + r < T ? arr[a][i0 + r] : 0;
+                sum[a] += v[a][r];
+            }
+        int64_t x[2] = {sum[0], sum[1]};
+        for (int a = 0; a < 2; a++) {
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                int64_t y = __shfl_up(x[a], o, 64);
+                if (lane >= o) x[a] += y;
+            }
+            if (lane == 63) s_w[a][wv] = x[a];
+        }
+        __syncthreads();
+        for (int a = 0; a < 2; a++) {
+            int64_t wbase = 0, tot = 0;
+            for (int q = 0; q < NW; q++) { int64_t s = s_w[a][q]; if (q < wv) wbase += s; tot += s; }
+            int64_t run = s_run[a] + wbase + x[a] - sum[a];
+            for (int r = 0; r < ITEMS; r++) {
+                if (i0 + r < T) arr[a][i0 + r] = run;
+                run += v[a][r];
+            }
+            __syncthreads();
+            if (tid == 0) s_run[a] += tot;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        arr[0][T] = s_run[0];
+        arr[1][T] = s_run[1];
+        d.cnt[C_PAIRS] = (uint64_t)s_run[0];
+        d.cnt[C_BPTOTAL] = (uint64_t)s_run[1];
+        if (record) d.cnt[C_BPAIRS] = (uint64_t)s_run[1];
+        if (s_run[0] > d.pcap) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 1ull);
+        if (record && s_run[1] > d.pcap) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 8ull);
+    }
+    __syncthreads();
+    for (int32_t w = tid; w < d.wcap; w += BLOCK) {
+        const int32_t f = d.w_flags[w];
+        const bool live = (f & WF_ACTIVE) && (f & WF_COLLISION) && d.w_tn[w] > 0;
+        int64_t ps = 0, pe = 0, bs = 0, be = 0;
+        if (live) {
+            const int64_t t0 = d.w_t0[w], t1 = t0 + d.w_tn[w];
+            ps = arr[0][t0]; pe = arr[0][t1]; bs = arr[1][t0]; be = arr[1][t1];
+        }
+        d.w_ps[w] = (int32_t)(ps < d.pcap ? ps : d.pcap);
+        d.w_pn[w] = pe <= d.pcap ? (int32_t)(pe - ps) : 0;
+        d.w_bps[w] = (int32_t)(bs < d.pcap ? bs : d.pcap);
+        d.w_bpn[w] = (record && be <= d.pcap) ? (int32_t)(be - bs) : 0;
     }
 }
 
@@ -1377,27 +1542,29 @@ void launch_refit(const Dev& d, int64_t nhw, double margin, hipStream_t s) {
 // scratch).  Each class's world list and count live in device memory, so a
 // captured graph stays valid as worlds come and go.
 template <int B>
-static void sap_one(const Dev& d, const int32_t* wl, const int32_t* wn, int32_t np, bool global, int32_t record,
-                    hipStream_t s) {
-    if (global) hipLaunchKernelGGL((k_sap<B, true>), dim3(d.wcap), dim3(B), 0, s, d, wl, wn, 0, record);
-    else hipLaunchKernelGGL((k_sap<B, false>), dim3(d.wcap), dim3(B), (size_t)np * 16, s, d, wl, wn, np, record);
+static void sap_sort_one(const Dev& d, const int32_t* wl, const int32_t* wn, int32_t np, bool global, hipStream_t s) {
+    if (global) hipLaunchKernelGGL((k_sap_sort<B, true>), dim3(d.wcap), dim3(B), 0, s, d, wl, wn, 0);
+    else hipLaunchKernelGGL((k_sap_sort<B, false>), dim3(d.wcap), dim3(B), (size_t)np * 16, s, d, wl, wn, np);
 }
 
 void launch_sap(const Dev& d, const int32_t* wl, const int32_t* wn, const int32_t* np_class, const int32_t* blocks,
-                const int32_t* live, int32_t record, hipStream_t s) {
+                const int32_t* live, int32_t record, int32_t grid, hipStream_t s) {
     for (int c = 0; c < 4; c++) {
         if (!live[c]) continue;
         const bool global = c == 3;
         const int32_t* l = wl + (size_t)c * d.wcap;
         const int32_t* n = wn + c;
         switch (blocks[c]) {
-            case 64: sap_one<64>(d, l, n, np_class[c], global, record, s); break;
-            case 128: sap_one<128>(d, l, n, np_class[c], global, record, s); break;
-            case 512: sap_one<512>(d, l, n, np_class[c], global, record, s); break;
-            case 1024: sap_one<1024>(d, l, n, np_class[c], global, record, s); break;
-            default: sap_one<256>(d, l, n, np_class[c], global, record, s); break;
+            case 64: sap_sort_one<64>(d, l, n, np_class[c], global, s); break;
+            case 128: sap_sort_one<128>(d, l, n, np_class[c], global, s); break;
+            case 512: sap_sort_one<512>(d, l, n, np_class[c], global, s); break;
+            case 1024: sap_sort_one<1024>(d, l, n, np_class[c], global, s); break;
+            default: sap_sort_one<256>(d, l, n, np_class[c], global, s); break;
         }
     }
+    hipLaunchKernelGGL((k_sap_pairs<256, false>), dim3(grid), dim3(256), 0, s, d, record);
+    hipLaunchKernelGGL((k_sap_scan<1024>), dim3(1), dim3(1024), 0, s, d, record);
+    hipLaunchKernelGGL((k_sap_pairs<256, true>), dim3(grid), dim3(256), 0, s, d, record);
 }
 
 void launch_narrow(const Dev& d, int precision, hipStream_t s) {

@@ -24,10 +24,14 @@ enum : int32_t {
     C_BPTOTAL = 5,     // broadphase pairs (all worlds)
     C_MANIFOLDS = 6,
     C_MAXLEVEL = 7,
-    C_OVERFLOW = 8,    // bit 0 pairs, 1 contacts, 2 cache, 3 broadphase record
+    C_OVERFLOW = 8,    // bit 0 pairs, 1 contacts, 2 cache, 3 broadphase record, 4 candidate tiles
     C_CONVEX = 9,      // convex pairs queued for GJK/EPA
+    C_TILES = 10,      // broadphase candidate tiles reserved
     C_COUNT = 16
 };
+
+// x-window candidates per broadphase tile (one wavefront task)
+constexpr int32_t SAP_TILE = 512;
 
 struct Dev {
     int64_t ncap, pcap, kcap, hcap;   // bodies, pairs, contacts, hash slots (pow2)
@@ -46,7 +50,10 @@ struct Dev {
     uint8_t *mt, *kind, *bflags, *awake, *awake0, *contacted, *notready;
     int32_t *parent, *last;               // union-find, level schedule scratch
     uint64_t *skey;                       // [2*ncap] broadphase sort keys (large worlds)
-    int32_t *sidx, *soff;                 // [2*ncap] broadphase scratch (large worlds)
+    int32_t *sidx, *soff;                 // [2*ncap] broadphase scratch; soff: window prefix per world
+    double *sbox;                         // [4][ncap] y/z bounds in sorted order (lo.y, hi.y, lo.z, hi.z)
+    uint32_t *sinfo;                      // [ncap] motion/collider/trigger bits | filter slot << 8, sorted
+    int32_t *sorig;                       // [ncap] world-local body index of each sorted position
     double *hull;                         // [hull_cap][3] local hull points, row-major
     // per world
     int32_t *w_off, *w_cnt, *w_flags, *w_iters;
@@ -55,6 +62,11 @@ struct Dev {
     int32_t *w_ps, *w_pn, *w_bps, *w_bpn; // solid / broadphase pair ranges
     int32_t *w_cs, *w_cn;                 // contact ranges
     int32_t *w_levels;                    // levels of the last solve
+    int64_t *w_x;                         // x-window candidates of the step
+    int32_t *w_t0, *w_tn;                 // candidate tiles: first, count
+    int64_t tcap;                         // candidate tiles per step
+    int32_t *tworld;                      // [tcap] world of each tile (-1: none)
+    int64_t *tcnt;                        // [2][tcap + 1] solid / broadphase pairs per tile -> offsets
     int32_t *w_cache_s, *w_cache_n;       // [2][wcap] cache ranges per buffer
     double *mat_fric, *mat_rest;          // pooled tables
     uint8_t *filt_tab;
