@@ -163,6 +163,9 @@ typedef struct {
     /* x-window candidates of the sweep-and-prune per step, all worlds, that
      * the broadphase tiles can hold (0 = 32 * max_pairs + 2^20). */
     int64_t max_candidates;
+    /* largest velocity_iterations of a world this batch will hold, 1..30
+     * (0 = 16): sizes the solver's level schedule. */
+    int32_t max_solver_iterations;
 } omi_batch_config;
 
 typedef struct omi_batch omi_batch;

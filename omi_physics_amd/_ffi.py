@@ -56,7 +56,7 @@ class BatchConfig(ctypes.Structure):
         ('max_contacts', ctypes.c_int64), ('max_hull_pts', ctypes.c_int64), ('max_joints', ctypes.c_int64),
         ('record_broadphase', ctypes.c_int32), ('solver_block', ctypes.c_int32),
         ('sap_block', ctypes.c_int32 * 4), ('solve_block', ctypes.c_int32 * 4),
-        ('max_candidates', ctypes.c_int64),
+        ('max_candidates', ctypes.c_int64), ('max_solver_iterations', ctypes.c_int32),
     ]
 
 

@@ -33,9 +33,10 @@ class Contact:
 
 def capacities_for(n_bodies: int, n_worlds: int = 1, pairs_per_body: float = 12.0,
                    contacts_per_body: float = 24.0) -> Dict[str, int]:
-    """Pair/contact capacities that cover piles of boxes and spheres with margin."""
-    return dict(max_pairs=int(pairs_per_body * n_bodies) + 4096 * max(1, n_worlds // 64 + 1),
-                max_contacts=int(contacts_per_body * n_bodies) + 8192 * max(1, n_worlds // 64 + 1))
+    """Pair/contact capacities that cover piles of boxes and spheres with margin (within the
+    library's limits of 2^30 pairs and 2^24 contacts per step)."""
+    return dict(max_pairs=min(int(pairs_per_body * n_bodies) + 4096 * max(1, n_worlds // 64 + 1), 1 << 30),
+                max_contacts=min(int(contacts_per_body * n_bodies) + 8192 * max(1, n_worlds // 64 + 1), 1 << 24))
 
 
 class Batch:

@@ -35,9 +35,10 @@ void launch_stage_spheres(int precision, int32_t np, int32_t mode, const double*
                           const double* cb, const double* Rb, const double* hb, const double* rb, int32_t* cnt,
                           double* nrm, double* pt, double* dep, hipStream_t s);
 void launch_solve(const Dev& d, int precision, const int32_t* blocks, SolveIn in, const int32_t* wl,
-                  const int32_t* wn, const int32_t* lds_last, const int32_t* lds_state, const int32_t* live,
-                  bool stage, hipStream_t s);
+                  const int32_t* wn, const int32_t* lds_n, const int32_t* live, bool stage, hipStream_t s);
 size_t epa_work_bytes();
+bool allow_dynamic_lds(int bytes);
+int32_t solve_lds_bodies(int precision, int64_t budget, int32_t cap);
 void launch_stage_convex(int32_t np, const int32_t* ka, const double* ca, const double* Ra, const double* ha,
                          const int32_t* oa, const int32_t* na, const int32_t* kb, const double* cb, const double* Rb,
                          const double* hb, const int32_t* ob, const int32_t* nb, const double* hull, void* work,
@@ -116,8 +117,7 @@ struct omi_batch {
     int32_t* d_wl = nullptr;               // [4][max_worlds] world slots per size class
     int32_t* d_wn = nullptr;               // [4] class counts
     int32_t np_class[4] = {256, 1024, 4096, 0};
-    int32_t lds_bodies[4] = {256, 1024, 4096, 12288};   // level counters in LDS
-    int32_t lds_state[4] = {0, 0, 0, 0};                // velocities + inverse inertia in LDS
+    int32_t lds_n[4] = {0, 0, 0, 0};        // solver: largest world of the class kept wholly in LDS
     int32_t live[4] = {0, 0, 0, 0};        // classes with a launch in the step
     int32_t sap_block[4] = {64, 128, 256, 1024};
     int32_t solve_block[4] = {64, 128, 256, 256};
@@ -184,8 +184,10 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
         return fail(OMI_ERR_ARG, "precision must be OMI_FP64 (64) or OMI_FP32 (32)");
     if (cfg->max_worlds <= 0 || cfg->max_bodies <= 0 || cfg->max_pairs <= 0 || cfg->max_contacts <= 0)
         return fail(OMI_ERR_ARG, "capacities must be positive");
-    if (cfg->max_pairs > (1ll << 30) || cfg->max_contacts > (1ll << 29) || cfg->max_bodies > (1ll << 30))
-        return fail(OMI_ERR_ARG, "capacity exceeds the 32-bit index range");
+    if (cfg->max_pairs > (1ll << 30) || cfg->max_contacts > (1ll << 24) || cfg->max_bodies > (1ll << 30))
+        return fail(OMI_ERR_ARG, "capacity exceeds the index range (pairs 2^30, contacts 2^24, bodies 2^30)");
+    if (cfg->max_solver_iterations < 0 || cfg->max_solver_iterations > 30)
+        return fail(OMI_ERR_ARG, "max_solver_iterations must be 0 (default 16) .. 30");
     int32_t rc = check_device();
     if (rc) return rc;
     omi_batch* b = new omi_batch();
@@ -201,21 +203,18 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     hipGetDevice(&dev);
     int smem = 65536;
     hipDeviceGetAttribute(&smem, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    b->np_lds_max = smem >= 4096 * 16 + 1024 ? 4096 : 2048;
     int cus = 256;
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     b->sap_grid = std::max(8, cus) * 8;
+    // dynamic LDS per launch: the device's LDS less 1 KB for the kernels'
+    // static LDS, or 64 KB less that if the runtime refuses the larger size
+    int dyn_lds = smem - 1024;
+    if (dyn_lds > 65536 - 1024 && !allow_dynamic_lds(dyn_lds)) dyn_lds = 65536 - 1024;
+    b->np_lds_max = dyn_lds >= 4096 * 16 ? 4096 : 2048;
     b->np_class[2] = b->np_lds_max;
-    b->lds_bodies[2] = b->np_lds_max;
-    {
-        // stage a class's bodies in LDS when the class bound fits the budget
-        const int64_t budget = std::min<int64_t>((int64_t)smem - 4096, 96 * 1024);
-        for (int c = 0; c < 4; c++) {
-            int64_t nb = c < 3 ? b->np_class[c] : 0;
-            int64_t bytes = ((b->lds_bodies[c] * 4 + 15) & ~15) + nb * 15 * (int64_t)rs;
-            b->lds_state[c] = (nb > 0 && bytes <= budget) ? (int32_t)nb : 0;
-        }
-    }
+    // the solver keeps a world of the class in LDS when it fits the budget
+    for (int c = 0; c < 4; c++)
+        b->lds_n[c] = c < 3 ? solve_lds_bodies(cfg->precision, (int64_t)dyn_lds - 1024, b->np_class[c]) : 0;
     b->solver_block = cfg->solver_block > 0 ? cfg->solver_block : 256;
     auto okb = [](int32_t x) { return x == 64 || x == 128 || x == 256 || x == 512 || x == 1024; };
     for (int c = 0; c < 4; c++) {
@@ -234,14 +233,14 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     d.rot = rot; d.iiw = iiw; d.svel = svel;
     A(d.world, N); A(d.mat, N); A(d.filt, N); A(d.hoff, N); A(d.hcnt, N);
     A(d.mt, N); A(d.kind, N); A(d.bflags, N); A(d.awake, N); A(d.awake0, N); A(d.contacted, N); A(d.notready, N);
-    A(d.parent, N); A(d.last, N);
+    A(d.parent, N); A(d.last, N); A(d.lmask, N);
     A(d.skey, 2 * N); A(d.sidx, 2 * N); A(d.soff, 2 * N);
     A(d.sbox, 4 * N); A(d.sinfo, N); A(d.sorig, N);
     A(d.hull, 3 * std::max<int64_t>(cfg->max_hull_pts, 1));
     A(d.w_off, W); A(d.w_cnt, W); A(d.w_flags, W); A(d.w_iters, W);
     A(d.w_mat_off, W); A(d.w_nmat, W); A(d.w_filt_off, W); A(d.w_nfilt, W);
     A(d.w_grav, 3 * W); A(d.w_damp, 2 * W);
-    A(d.w_ps, W); A(d.w_pn, W); A(d.w_bps, W); A(d.w_bpn, W); A(d.w_cs, W); A(d.w_cn, W); A(d.w_levels, W);
+    A(d.w_ps, W); A(d.w_pn, W); A(d.w_bps, W); A(d.w_bpn, W); A(d.w_cs, W); A(d.w_cn, W); A(d.w_mn, W); A(d.w_levels, W);
     A(d.w_cache_s, 2 * W); A(d.w_cache_n, 2 * W);
     A(d.w_x, W); A(d.w_t0, W); A(d.w_tn, W);
     {
@@ -270,9 +269,11 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     d.mnrm = mn; d.mdep = md;
     A(d.ca, K); A(d.cb, K); A(d.cpt, 3 * K);
     A(cn, 3 * K * (int64_t)rs); A(cd, K * (int64_t)rs); A(ci, 3 * K * (int64_t)rs); A(cs, 26 * K * (int64_t)rs);
-    A(d.sab, 2 * K);
+    A(d.mrec, K);
     d.cnrm = cn; d.cdep = cd; d.cimp = ci; d.csolv = cs;
-    A(d.clevel, K); A(d.corder, K); A(d.lcount, K);
+    d.slmax = 1 + (cfg->max_solver_iterations > 0 ? cfg->max_solver_iterations : 16);
+    A(d.clevel, K * d.slmax); A(d.corder, K * d.slmax); A(d.lcount, K * d.slmax);
+    A(d.cord_a, K * d.slmax); A(d.cord_b, K * d.slmax);
     for (int q = 0; q < 2; q++) {
         unsigned char* ki;
         A(d.ka[q], K); A(d.kb[q], K); A(d.kpt[q], 3 * K); A(ki, 3 * K * (int64_t)rs); d.kimp[q] = ki;
@@ -386,6 +387,8 @@ extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_wo
     if (slot < 0 || slot >= b->cfg.max_worlds) return fail(OMI_ERR_ARG, "world slot out of range");
     if (w->n < 0) return fail(OMI_ERR_ARG, "negative body count");
     if (w->n_mat < 1 || w->n_filt < 1) return fail(OMI_ERR_ARG, "material and filter tables need >= 1 slot");
+    if (w->velocity_iterations < 0 || w->velocity_iterations > b->d.slmax - 1)
+        return fail(OMI_ERR_ARG, "velocity_iterations exceeds the batch's max_solver_iterations");
     const int64_t n = w->n;
     for (int64_t i = 0; i < n; i++) {
         int k = w->kind[i], m = w->motion_type[i];
@@ -496,6 +499,7 @@ extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_wo
     i32 = (int32_t)s.filt_off; if ((rc = put(b, d.w_filt_off + slot, &i32, 1))) return rc;
     i32 = w->n_filt; if ((rc = put(b, d.w_nfilt + slot, &i32, 1))) return rc;
     i32 = 0;
+    if ((rc = put(b, d.w_mn + slot, &i32, 1))) return rc;
     if ((rc = put(b, d.w_cache_n + slot, &i32, 1))) return rc;
     if ((rc = put(b, d.w_cache_n + W + slot, &i32, 1))) return rc;
     if ((rc = put(b, d.w_pn + slot, &i32, 1))) return rc;
@@ -573,7 +577,7 @@ static void enqueue_step(omi_batch* b, double dt) {
     launch_compact(d, prec, b->cfg.record_broadphase, s); mark();
     launch_wake(d, s); mark();
     SolveIn in{nullptr, nullptr, 0.5, 0.8, 0.005, 1};
-    launch_solve(d, prec, b->solve_block, in, b->d_wl, b->d_wn, b->lds_bodies, b->lds_state, b->live, false, s); mark();
+    launch_solve(d, prec, b->solve_block, in, b->d_wl, b->d_wn, b->lds_n, b->live, false, s); mark();
     launch_cache(d, prec, s); mark();
     if (b->any_joints) launch_joints(d, prec, dt, s);
     mark();
@@ -870,9 +874,10 @@ extern "C" int32_t omi_sphere_box(int32_t np, int32_t precision, const double* c
 
 // A one-world batch whose arrays the stage entry points fill directly.
 static int32_t mini_batch(int64_t nb, int64_t npairs, int64_t ncon, int32_t precision, int32_t record,
-                          omi_batch** out, int64_t ncand = 0) {
+                          omi_batch** out, int64_t ncand = 0, int32_t iters = 0) {
     omi_batch_config c{};
     c.max_candidates = ncand;
+    c.max_solver_iterations = iters;
     c.max_worlds = 1;
     c.precision = precision;
     c.max_bodies = std::max<int64_t>(nb, 1);
@@ -996,12 +1001,13 @@ extern "C" int32_t omi_prepare_and_solve(int64_t K, int32_t N, const int64_t* a_
                                          double* tImp, double rthr, int32_t iters, int32_t warm, double correction,
                                          double slop, int32_t precision) {
     if (K < 0 || N < 0 || (precision != 64 && precision != 32)) return fail(OMI_ERR_ARG, "bad arguments");
+    if (iters < 0 || iters > 30 || (warm != 0 && warm != 1)) return fail(OMI_ERR_ARG, "iters must be 0..30, warm 0 or 1");
     if (K == 0) return OMI_OK;
     for (int64_t k = 0; k < K; k++)
         if (a_idx[k] < 0 || a_idx[k] >= N || b_idx[k] < 0 || b_idx[k] >= N || a_idx[k] == b_idx[k])
             return fail(OMI_ERR_ARG, "contact body index out of range");
     omi_batch* b = nullptr;
-    int32_t rc = mini_batch(N, 1, K, precision, 0, &b);
+    int32_t rc = mini_batch(N, 1, K, precision, 0, &b, 0, std::max(iters, 1));
     if (rc) return rc;
     Dev& d = b->d;
     const bool f64 = precision == 64;
@@ -1009,6 +1015,17 @@ extern "C" int32_t omi_prepare_and_solve(int64_t K, int32_t N, const int64_t* a_
     std::vector<int32_t> ia((size_t)K), ib((size_t)K), wz((size_t)N, 0);
     std::vector<uint8_t> mt((size_t)N, M_DYNAMIC);
     for (int64_t k = 0; k < K; k++) { ia[k] = (int32_t)a_idx[k]; ib[k] = (int32_t)b_idx[k]; }
+    // manifolds: runs of contacts of one pair (every body counts as dynamic here)
+    std::vector<int4> mrec;
+    for (int64_t k = 0; k < K; k++) {
+        if (k == 0 || ia[k] != ia[k - 1] || ib[k] != ib[k - 1])
+            mrec.push_back(make_int4((int32_t)k, MR_DYN_A | MR_DYN_B, ia[k], ib[k]));
+        if ((mrec.back().y & MR_COUNT) == MR_COUNT)   // a run longer than a record holds starts a new one
+            mrec.push_back(make_int4((int32_t)k, MR_DYN_A | MR_DYN_B, ia[k], ib[k]));
+        mrec.back().y++;
+    }
+    int32_t Mi = (int32_t)mrec.size();
+    uint64_t kc = (uint64_t)K;
     // Real-typed uploads (component-major)
     auto upR = [&](void* dst, int64_t ld, const double* src, int64_t n, int k, int stride_k) -> int32_t {
         std::vector<unsigned char> tmp((size_t)n * rs);
@@ -1036,13 +1053,14 @@ extern "C" int32_t omi_prepare_and_solve(int64_t K, int32_t N, const int64_t* a_
         !(rc = upR(d.cnrm, d.kcap, normal, K, 3, 3)) && !(rc = upR(d.cdep, d.kcap, depth, K, 1, 1)) &&
         !(rc = upR(d.cimp, d.kcap, imp.data(), K, 3, 3)) && !(rc = put(b, d.w_off, &zero, 1)) &&
         !(rc = put(b, d.w_cnt, &Ni, 1)) && !(rc = put(b, d.w_cs, &zero, 1)) && !(rc = put(b, d.w_cn, &Ki, 1)) &&
-        !(rc = put(b, d.w_iters, &iters, 1)) && !(rc = put(b, d.w_flags, &flags, 1))) {
+        !(rc = put(b, d.w_iters, &iters, 1)) && !(rc = put(b, d.w_flags, &flags, 1)) &&
+        !(rc = put(b, d.mrec, mrec.data(), Mi)) && !(rc = put(b, d.w_mn, &Mi, 1)) &&
+        !(rc = put(b, d.cnt + C_CONTACTS, &kc, 1))) {
         SolveIn in{drest, dfric, rthr, correction, slop, warm};
         b->slots[0].live = true;
         b->slots[0].n = N;
         if (!(rc = rebuild_lists(b)))
-            launch_solve(d, precision, b->solve_block, in, b->d_wl, b->d_wn, b->lds_bodies, b->lds_state, b->live, true,
-                         b->stream);
+            launch_solve(d, precision, b->solve_block, in, b->d_wl, b->d_wn, b->lds_n, b->live, true, b->stream);
         if (!rc && hipGetLastError() != hipSuccess) rc = fail(OMI_ERR_HIP, "solver launch failed");
         if (!rc) rc = download_soa(b, pos, d.pos, d.ncap, 0, N, 3);
         if (!rc) rc = download_soa(b, lv, d.lv, d.ncap, 0, N, 3);

@@ -622,17 +622,19 @@ __global__ void k_stage_convex(int32_t np, const int32_t* ka, const double* ca, 
 }
 
 // Contacts in the reference's order (pair order, then manifold order) packed
-// into one contiguous slice per world.
+// into one contiguous slice per world, with one manifold record per pair
+// that produced contacts: {first contact, contacts | a dynamic << 8 | b
+// dynamic << 9, body a, body b}, all world-local.
 template <typename R, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_compact(Dev d, int32_t record) {
-    __shared__ int32_t s_scan[BLOCK / 64 + 2];
+    __shared__ int32_t s_scan[2][BLOCK / 64 + 2];
     __shared__ int64_t s_sum[BLOCK / 64 + 2];
     __shared__ int32_t s_base[2];
     const int32_t w = blockIdx.x;
     const int tid = threadIdx.x;
     if (w >= d.wcap) return;
     const int32_t pn = (d.w_flags[w] & WF_ACTIVE) ? d.w_pn[w] : 0, ps = d.w_ps[w];
-    if (pn == 0) { if (tid == 0) d.w_cn[w] = 0; return; }
+    if (pn == 0) { if (tid == 0) { d.w_cn[w] = 0; d.w_mn[w] = 0; } return; }
     int64_t kc = 0, mf = 0;
     for (int32_t i = tid; i < pn; i += BLOCK) { int c = d.mcnt[ps + i]; kc += c; mf += c > 0; }
     const int64_t K = block_sum<BLOCK>(kc, s_sum);
@@ -643,32 +645,48 @@ __global__ __launch_bounds__(BLOCK) void k_compact(Dev d, int32_t record) {
         if (!ok) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 2ull);
         s_base[0] = (int32_t)base; s_base[1] = ok;
         d.w_cs[w] = (int32_t)base; d.w_cn[w] = ok ? (int32_t)K : 0;
+        d.w_mn[w] = ok ? (int32_t)M : 0;
         atomicAdd((unsigned long long*)&d.cnt[C_MANIFOLDS], (unsigned long long)M);
     }
     __syncthreads();
     if (!s_base[1]) return;
-    int32_t run = s_base[0];
+    const int32_t cs = s_base[0];
+    const int64_t off = d.w_off[w];
+    int32_t run = 0, mrun = 0;
     const R* mn = (const R*)d.mnrm;
     const R* md = (const R*)d.mdep;
     R* cn = (R*)d.cnrm;
     R* cdp = (R*)d.cdep;
     R* imp = (R*)d.cimp;
+    const int lane = tid & 63, wv = tid >> 6;
     for (int32_t cb = 0; cb < pn; cb += BLOCK) {
         const int32_t i = cb + tid;
         const int32_t p = ps + i;
         const int32_t c = i < pn ? d.mcnt[p] : 0;
-        // exclusive prefix of the counts over the chunk (counts are 0..4)
-        int32_t x = c;
-        const int lane = tid & 63, wv = tid >> 6;
+        // exclusive prefixes over the chunk of the contact counts (0..4) and of
+        // the manifold flags
+        int32_t x = c, y = c > 0;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) { int32_t y = __shfl_up(x, o, 64); if (lane >= o) x += y; }
-        if (lane == 63) s_scan[wv] = x;
+        for (int o = 1; o < 64; o <<= 1) {
+            int32_t u = __shfl_up(x, o, 64), v = __shfl_up(y, o, 64);
+            if (lane >= o) { x += u; y += v; }
+        }
+        if (lane == 63) { s_scan[0][wv] = x; s_scan[1][wv] = y; }
         __syncthreads();
-        int32_t wbase = 0, tot = 0;
-        for (int q = 0; q < BLOCK / 64; q++) { int32_t s = s_scan[q]; if (q < wv) wbase += s; tot += s; }
-        const int32_t dst = run + wbase + x - c;
+        int32_t wbase = 0, tot = 0, mbase = 0, mtot = 0;
+        for (int q = 0; q < BLOCK / 64; q++) {
+            int32_t s0 = s_scan[0][q], s1 = s_scan[1][q];
+            if (q < wv) { wbase += s0; mbase += s1; }
+            tot += s0; mtot += s1;
+        }
+        const int32_t first = run + wbase + x - c;       // world-local first contact
+        if (c > 0) {
+            const int64_t a = d.pa[p], b = d.pb[p];
+            const int32_t fl = (d.mt[a] == M_DYNAMIC ? MR_DYN_A : 0) | (d.mt[b] == M_DYNAMIC ? MR_DYN_B : 0);
+            d.mrec[cs + mrun + mbase + y - 1] = make_int4(first, c | fl, (int32_t)(a - off), (int32_t)(b - off));
+        }
         for (int m = 0; m < c; m++) {
-            const int64_t k = dst + m;
+            const int64_t k = cs + first + m;
             d.ca[k] = d.pa[p]; d.cb[k] = d.pb[p];
             for (int r = 0; r < 3; r++) {
                 CON(d.cpt, r, k) = d.mpts[(size_t)r * 4 * d.pcap + 4 * (size_t)p + m];
@@ -678,6 +696,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact(Dev d, int32_t record) {
             cdp[k] = md[4 * (size_t)p + m];
         }
         run += tot;
+        mrun += mtot;
         __syncthreads();
     }
 }
@@ -801,129 +820,169 @@ __device__ __forceinline__ void seed_lookup(const Dev& d, int32_t w, int32_t old
 }
 
 // One world's contacts solved exactly in the reference's Gauss-Seidel order
-// (_solver_native.pyx:179-312).  Contact k depends only on the previous contact
-// (in sequence, across sweeps) that touched one of its dynamic bodies; level
-// L(k) = 1 + max(L of those) puts every dependency on an earlier level, so the
-// contacts of one level touch disjoint dynamic bodies and run concurrently
-// while every contact still reads exactly the velocities the sequential loop
-// would.  Static/kinematic bodies are read-only (their inverse mass and
-// inertia are zero, so the reference's writes to them leave them unchanged).
-template <typename R, int BLOCK, bool STAGE>
-__global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_t* wlist, const int32_t* wcount,
-                                                 int32_t lds_last, int32_t lds_state) {
-    extern __shared__ __align__(16) unsigned char s_dyn[];
-    __shared__ int32_t s_scan[BLOCK / 64 + 2];
-    __shared__ int32_t s_D;
-    if ((int32_t)blockIdx.x >= *wcount) return;
-    const int32_t w = wlist[blockIdx.x];
-    const int tid = threadIdx.x;
-    if (w >= d.wcap || !(d.w_flags[w] & WF_ACTIVE)) return;
-    const int32_t K = d.w_cn[w], cs = d.w_cs[w], n = d.w_cnt[w];
-    const int64_t off = d.w_off[w];
-    if (K == 0) { if (tid == 0) d.w_levels[w] = 0; return; }
-    const bool fp64 = sizeof(R) == sizeof(double);
-    // Where the bodies' velocities and world inverse inertia live while the
-    // world is solved: in LDS when the world fits the launch's budget, else the
-    // FP64 state itself (FP64) or an FP32 copy (FP32) in HBM.
-    const bool state_lds = lds_state >= n;
-    int32_t* s_last = (int32_t*)s_dyn;
-    R* s_state = (R*)(s_dyn + (((size_t)lds_last * 4 + 15) & ~(size_t)15));
-    R* V;
-    const R* I;
-    int64_t vs, vb;
-    const R* iiw = (const R*)d.iiw;
-    if (state_lds) { V = s_state; I = s_state + 6 * (size_t)n; vs = n; vb = off; }
-    else { V = fp64 ? (R*)d.lv : (R*)d.svel; I = iiw; vs = d.ncap; vb = 0; }
-#define VEL(c, i) V[(size_t)(c) * vs + (size_t)((i) - vb)]
-#define IIW(c, i) I[(size_t)(c) * vs + (size_t)((i) - vb)]
-    if (state_lds || !fp64)
-        for (int32_t i = tid; i < n; i += BLOCK) {
-            for (int c = 0; c < 6; c++) VEL(c, off + i) = (R)BODY(d.lv, c, off + i);
-            if (state_lds)
-                for (int c = 0; c < 9; c++) s_state[(size_t)(6 + c) * n + i] = BODY(iiw, c, off + i);
+// (_solver_native.pyx:179-312).  The warm start and the `iters` sweeps are one
+// sequence of velocity updates over the contacts in narrowphase order; item
+// (s, k) is contact k in pass s.  An item reads and writes only its two bodies,
+// and static/kinematic bodies are read-only (their inverse mass and inertia
+// are zero, so the reference's writes to them leave them unchanged), so item i
+// depends only on the most recent earlier item that touched one of its dynamic
+// bodies.  Level L(i) = 1 + max(L of those) puts every dependency on an earlier
+// level: the items of one level touch disjoint dynamic bodies and run
+// concurrently, every item reads exactly the velocities the sequential loop
+// would, and later passes start on bodies whose previous pass is done.  The
+// split-impulse position pass reads no velocity and writes only positions, so
+// its update of contact k runs with item (0, k), which keeps each body's
+// position updates in contact order.
+//
+// Levels are assigned per manifold (the run of contacts of one pair, which
+// chain through both bodies and so take consecutive levels) by one wavefront,
+// 64 manifolds at a time.  Each lane ORs its bit into a 64-bit mask per dynamic
+// body; the highest lower bit in its bodies' masks is its predecessor inside
+// the group.  Levels start from the per-body table `last` and are relaxed over
+// those predecessors until no lane changes (rounds = the longest chain inside
+// the group); the last lane to touch a body writes its level back.
+template <bool G>
+struct LevelTable {
+    int32_t* last;
+    uint64_t* mask;
+    __device__ __forceinline__ int32_t get_last(int32_t i) const {
+        if (G) return __hip_atomic_load(&last[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return last[i];
+    }
+    __device__ __forceinline__ uint64_t get_mask(int32_t i) const {
+        if (G) return __hip_atomic_load(&mask[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return mask[i];
+    }
+    __device__ __forceinline__ void set_last(int32_t i, int32_t v) const {
+        if (G) __hip_atomic_store(&last[i], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else last[i] = v;
+    }
+    __device__ __forceinline__ void or_mask(int32_t i, uint64_t m) const {
+        __hip_atomic_fetch_or(&mask[i], m, __ATOMIC_RELAXED,
+                              G ? __HIP_MEMORY_SCOPE_AGENT : __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __device__ __forceinline__ void clear_mask(int32_t i) const {
+        if (G) __hip_atomic_store(&mask[i], (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else mask[i] = 0;
+    }
+    // orders this wavefront's table accesses (LDS: in-order, so a compiler
+    // barrier and an LDS wait suffice)
+    __device__ __forceinline__ void fence() const {
+        if (G) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+        else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
+    }
+};
+
+// Levels of manifold items [0, NI) (item i is manifold i mod M of pass
+// i / M) into level[] (the first contact's level); with `lcnt` (LDS, zeroed)
+// also the number of contacts on each level below `cap` (*over is set when a
+// level reaches cap).  Returns the depth.  Called by the 64 threads of one
+// wavefront; the table starts at zero.
+template <bool G>
+__device__ int32_t assign_item_levels(const LevelTable<G>& T, int32_t M, int32_t NI, const int4* mrec,
+                                      int32_t* level, int32_t* lcnt, int32_t cap, bool& over) {
+    const int lane = threadIdx.x & 63;
+    int32_t D = 0;
+    int32_t m = lane;
+    while (m >= M) m -= M;
+    int4 rec = make_int4(0, 1, 0, 0);
+    if (lane < NI) rec = mrec[m];
+    for (int32_t base = 0; base < NI; base += 64) {
+        const bool valid = base + lane < NI;
+        const int4 cur = rec;
+        // the next group's manifold records load while this group resolves
+        m += 64;
+        while (m >= M) m -= M;
+        if (base + 64 + lane < NI) rec = mrec[m];
+        const int32_t al = (valid && (cur.y & MR_DYN_A)) ? cur.z : -1;
+        const int32_t bl = (valid && (cur.y & MR_DYN_B)) ? cur.w : -1;
+        const int32_t c = valid ? (cur.y & MR_COUNT) : 1;
+        const uint64_t bit = 1ull << lane;
+        if (al >= 0) T.or_mask(al, bit);
+        if (bl >= 0) T.or_mask(bl, bit);
+        T.fence();
+        const uint64_t below = bit - 1ull, above = ~((bit << 1) - 1ull);
+        uint64_t ma = 0, mb = 0;
+        int32_t L = 1;
+        if (al >= 0) { ma = T.get_mask(al); const int32_t x = T.get_last(al) + 1; L = x > L ? x : L; }
+        if (bl >= 0) { mb = T.get_mask(bl); const int32_t x = T.get_last(bl) + 1; L = x > L ? x : L; }
+        const int32_t pa = (ma & below) ? 63 - __clzll(ma & below) : lane;
+        const int32_t pb = (mb & below) ? 63 - __clzll(mb & below) : lane;
+        const int32_t ca = __shfl(c, pa, 64), cb = __shfl(c, pb, 64);
+        for (;;) {
+            const int32_t La = __shfl(L, pa, 64), Lb = __shfl(L, pb, 64);
+            int32_t nL = L;
+            if (pa != lane && La + ca > nL) nL = La + ca;
+            if (pb != lane && Lb + cb > nL) nL = Lb + cb;
+            const bool ch = nL != L;
+            L = nL;
+            if (!__any(ch)) break;
         }
-    R* S = (R*)d.csolv;
-    R* imp = (R*)d.cimp;
+        T.fence();
+        const int32_t Le = L + c - 1;   // level of the manifold's last contact
+        if (al >= 0 && !(ma & above)) T.set_last(al, Le);
+        if (bl >= 0 && !(mb & above)) T.set_last(bl, Le);
+        if (al >= 0) T.clear_mask(al);
+        if (bl >= 0) T.clear_mask(bl);
+        T.fence();
+        if (valid) {
+            level[base + lane] = L;
+            D = Le > D ? Le : D;
+            if (lcnt) {
+                if (Le <= cap) for (int32_t j = 0; j < c; j++) atomicAdd(&lcnt[L - 1 + j], 1);
+                else over = true;
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { int32_t x = __shfl_xor(D, o, 64); D = x > D ? x : D; }
+    over = __any(over);
+    return D;
+}
+
+// Schedule item code: contact (bits 31..7), side a dynamic (6), side b
+// dynamic (5), pass (4..0).
+__device__ __forceinline__ int32_t item_code(int32_t k, bool da, bool db, int32_t s) {
+    return (k << 7) | ((int32_t)da << 6) | ((int32_t)db << 5) | s;
+}
+
+// One scheduled item's immutable data: the contact's solver row and the
+// bodies' world inverse inertia.
+template <typename R>
+struct SolveItem {
+    int64_t ci, a, b;
+    int32_t s;
+    bool da, db;
+    R rA[3], rB[3], t1[3], t2[3], nv[3], nm, t1m, t2m, vb, fr, ima, imb, dep;
+    R Ia[9], Ib[9];
+};
+
+// Solver rows for every contact of the step, one thread per contact
+// (_solver_native.pyx:179-256): lever arms, tangent basis, effective masses,
+// restitution bias, friction, seeds (warm-start cache, solver._seed_pair,
+// or the stage call's arrays), from the pre-solve velocities.
+template <typename R, bool STAGE>
+__global__ void k_prepare(Dev d, SolveIn in) {
+    const uint64_t tc = d.cnt[C_CONTACTS];
+    const int64_t T = tc < (uint64_t)d.kcap ? (int64_t)tc : d.kcap;
+    R* S_ = (R*)d.csolv;
+    const R* imp = (const R*)d.cimp;
     const R* cn = (const R*)d.cnrm;
     const R* cdep = (const R*)d.cdep;
-    int32_t* sa = d.sab;
-    int32_t* sb = d.sab + d.kcap;
-
-    // Level schedule.  Wave 0 walks the contacts 64 at a time: lane i starts
-    // from the last levels recorded for its dynamic bodies, then takes the
-    // final levels of the earlier lanes in this group (broadcast in order) and
-    // raises its own past any that share a dynamic body.  `last` (per body,
-    // LDS for worlds that fit) keeps the running maximum with atomicMax.
-    int32_t* last = lds_last >= n ? s_last : d.last + off;
-    for (int32_t i = tid; i < n; i += BLOCK) last[i] = 0;
-    for (int32_t k = tid; k < K; k += BLOCK) d.lcount[cs + k] = 0;
-    __syncthreads();
-    if (tid < 64) {
-        int32_t D = 0;
-        for (int32_t base = 0; base < K; base += 64) {
-            const int32_t k = base + tid;
-            const bool valid = k < K;
-            int32_t al = -1, bl = -1;
-            if (valid) {
-                const int64_t a = d.ca[cs + k], b = d.cb[cs + k];
-                if (d.mt[a] == M_DYNAMIC) al = (int32_t)(a - off);
-                if (d.mt[b] == M_DYNAMIC) bl = (int32_t)(b - off);
-            }
-            // atomic loads: a global-memory `last` must not be read from a stale L1 line
-            const int32_t la = al >= 0 ? __hip_atomic_load(&last[al], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-            const int32_t lb = bl >= 0 ? __hip_atomic_load(&last[bl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-            int32_t L = (la > lb ? la : lb) + 1;
-            const int32_t m = K - base < 64 ? K - base : 64;
-            for (int32_t j = 0; j < m - 1; j++) {
-                const int32_t aj = __builtin_amdgcn_readlane(al, j);
-                const int32_t bj = __builtin_amdgcn_readlane(bl, j);
-                const int32_t Lj = __builtin_amdgcn_readlane(L, j);
-                const bool share = (al >= 0 && (al == aj || al == bj)) || (bl >= 0 && (bl == aj || bl == bj));
-                if (tid > j && share && Lj + 1 > L) L = Lj + 1;
-            }
-            if (valid) {
-                if (al >= 0) atomicMax(&last[al], L);
-                if (bl >= 0) atomicMax(&last[bl], L);
-                d.clevel[cs + k] = L;
-                D = L > D ? L : D;
-            }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_s_waitcnt(0);
-        }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) { int32_t x = __shfl_xor(D, o, 64); D = x > D ? x : D; }
-        if (tid == 0) {
-            s_D = D;
-            d.w_levels[w] = D;
-            atomicMax((unsigned long long*)&d.cnt[C_MAXLEVEL], (unsigned long long)D);
-        }
-    }
-    __syncthreads();
-    for (int32_t k = tid; k < K; k += BLOCK) atomicAdd(&d.lcount[cs + d.clevel[cs + k] - 1], 1);
-    __syncthreads();
-    const int32_t D = s_D;
-    block_exscan<BLOCK>(d.lcount + cs, D, s_scan);
-    for (int32_t k = tid; k < K; k += BLOCK) {
-        const int32_t L = d.clevel[cs + k];
-        const int32_t slot = atomicAdd(&d.lcount[cs + L - 1], 1);
-        d.corder[cs + slot] = k;
-    }
-    __syncthreads();
-    // after the scatter lcount[l] is the end of level l
-
-    // Prepare every contact from the pre-solve velocities, writing its row at
-    // its level-sorted slot so each level's rows are contiguous.
+    const R* iiw = (const R*)d.iiw;
     const int32_t old = *d.sel;
-    const int32_t mo = d.w_mat_off[w], nm = d.w_nmat[w];
-    for (int32_t t = tid; t < K; t += BLOCK) {
-        const int32_t k = d.corder[cs + t];
-        const int64_t ci = cs + k, si = cs + t, a = d.ca[ci], b = d.cb[ci];
+    for (int64_t ci = gtid(); ci < T; ci += gstride()) {
+        const int64_t a = d.ca[ci], b = d.cb[ci];
+        const int32_t w = d.world[a];
+        if (w < 0) continue;
+        const int32_t cs = d.w_cs[w];
+        if (ci < cs || ci >= (int64_t)cs + d.w_cn[w]) continue;   // a world that overflowed
         R rest, fric, n0 = R(0), u0 = R(0), w0 = R(0);
         if (STAGE) {
-            rest = (R)in.st_rest[k]; fric = (R)in.st_fric[k];
+            rest = (R)in.st_rest[ci - cs]; fric = (R)in.st_fric[ci - cs];
             n0 = CON(imp, 0, ci); u0 = CON(imp, 1, ci); w0 = CON(imp, 2, ci);
         } else {
-            const int32_t tb = mo + d.mat[a] * nm + d.mat[b];
+            const int32_t tb = d.w_mat_off[w] + d.mat[a] * d.w_nmat[w] + d.mat[b];
             rest = (R)d.mat_rest[tb]; fric = (R)d.mat_fric[tb];
             if (in.warm) seed_lookup<R>(d, w, old, ci, a, b, n0, u0, w0);
         }
@@ -934,8 +993,8 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
             rA[r] = (R)(p - BODY(d.pos, r, a));
             rB[r] = (R)(p - BODY(d.pos, r, b));
         }
-        for (int r = 0; r < 9; r++) { Ia[r] = IIW(r, a); Ib[r] = IIW(r, b); }
-        for (int c = 0; c < 6; c++) { va[c] = VEL(c, a); vb_[c] = VEL(c, b); }
+        for (int r = 0; r < 9; r++) { Ia[r] = BODY(iiw, r, a); Ib[r] = BODY(iiw, r, b); }
+        for (int c = 0; c < 6; c++) { va[c] = (R)BODY(d.lv, c, a); vb_[c] = (R)BODY(d.lv, c, b); }
         const R nx = CON(cn, 0, ci), ny = CON(cn, 1, ci), nz = CON(cn, 2, ci);
         R axx, ayy, azz;
         if ((nx >= R(0) ? nx : -nx) < R(0.9)) { axx = R(1); ayy = R(0); azz = R(0); }
@@ -948,133 +1007,376 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
         R dvx, dvy, dvz;
         relvel(va, vb_, rA, rB, dvx, dvy, dvz);
         const R vn0 = dvx * nx + dvy * ny + dvz * nz;
-        for (int r = 0; r < 3; r++) { CON(S, SR_RA + r, si) = rA[r]; CON(S, SR_RB + r, si) = rB[r]; }
-        CON(S, SR_T1, si) = tx; CON(S, SR_T1 + 1, si) = ty; CON(S, SR_T1 + 2, si) = tz;
-        CON(S, SR_T2, si) = ux; CON(S, SR_T2 + 1, si) = uy; CON(S, SR_T2 + 2, si) = uz;
-        CON(S, SR_NM, si) = effmass(rA, rB, nx, ny, nz, Ia, Ib, ima, imb);
-        CON(S, SR_T1M, si) = effmass(rA, rB, tx, ty, tz, Ia, Ib, ima, imb);
-        CON(S, SR_T2M, si) = effmass(rA, rB, ux, uy, uz, Ia, Ib, ima, imb);
-        CON(S, SR_VB, si) = vn0 < -(R)in.rthr ? -rest * vn0 : R(0);
-        CON(S, SR_FR, si) = fric;
-        CON(S, SR_IMA, si) = ima; CON(S, SR_IMB, si) = imb;
-        CON(S, SR_NX, si) = nx; CON(S, SR_NX + 1, si) = ny; CON(S, SR_NX + 2, si) = nz;
-        CON(S, SR_IMP, si) = n0; CON(S, SR_IMP + 1, si) = u0; CON(S, SR_IMP + 2, si) = w0;
-        CON(S, SR_DEP, si) = cdep[ci];
-        sa[si] = (int32_t)a; sb[si] = (int32_t)b;
+        for (int r = 0; r < 3; r++) { CON(S_, SR_RA + r, ci) = rA[r]; CON(S_, SR_RB + r, ci) = rB[r]; }
+        CON(S_, SR_T1, ci) = tx; CON(S_, SR_T1 + 1, ci) = ty; CON(S_, SR_T1 + 2, ci) = tz;
+        CON(S_, SR_T2, ci) = ux; CON(S_, SR_T2 + 1, ci) = uy; CON(S_, SR_T2 + 2, ci) = uz;
+        CON(S_, SR_NM, ci) = effmass(rA, rB, nx, ny, nz, Ia, Ib, ima, imb);
+        CON(S_, SR_T1M, ci) = effmass(rA, rB, tx, ty, tz, Ia, Ib, ima, imb);
+        CON(S_, SR_T2M, ci) = effmass(rA, rB, ux, uy, uz, Ia, Ib, ima, imb);
+        CON(S_, SR_VB, ci) = vn0 < -(R)in.rthr ? -rest * vn0 : R(0);
+        CON(S_, SR_FR, ci) = fric;
+        CON(S_, SR_IMA, ci) = ima; CON(S_, SR_IMB, ci) = imb;
+        CON(S_, SR_NX, ci) = nx; CON(S_, SR_NX + 1, ci) = ny; CON(S_, SR_NX + 2, ci) = nz;
+        CON(S_, SR_IMP, ci) = n0; CON(S_, SR_IMP + 1, ci) = u0; CON(S_, SR_IMP + 2, ci) = w0;
+        CON(S_, SR_DEP, ci) = cdep[ci];
+    }
+}
+
+// Level bounds kept in LDS per world (deeper schedules use global memory).
+constexpr int32_t SOLVE_DCAP = 1024;
+
+// LDS layout of a solve launch that keeps worlds of up to lds_n bodies in
+// LDS (byte offsets): level table and body masks (12 B per body; reused for
+// FP32 position corrections), velocities (6 R), impulses of up to 3 contacts
+// per body (9 R), FP64 positions (3 doubles, FP64 only), level bounds.
+struct SolveLds {
+    size_t vel, imp, pos, bound, total;
+};
+__host__ __device__ inline SolveLds solve_lds_layout(int32_t lds_n, size_t rsize) {
+    SolveLds L;
+    L.vel = (((size_t)lds_n * 12 + 15) & ~(size_t)15);
+    L.imp = L.vel + (size_t)lds_n * 6 * rsize;
+    L.pos = L.imp + (size_t)lds_n * 9 * rsize;
+    L.bound = L.pos + (rsize == 8 ? (size_t)lds_n * 24 : 0);
+    L.total = L.bound + 4 * (size_t)SOLVE_DCAP;
+    return L;
+}
+
+// Where one world's solver state lives during the level loop.
+template <typename R>
+struct SolveState {
+    R* V; int64_t vs, vb;          // velocities: V[c * vs + (i - vb)]
+    R* P; int64_t ps, pb;          // impulses: P[c * ps + (ci - pb)]
+    R* pd;                         // FP32 position corrections in LDS [3][n], or null
+    double* pos;                   // FP64 positions in LDS [3][n], or null
+    int64_t n, off;
+};
+
+// Bucket the contact items by level: bound[l] is the start of level l on
+// entry (after the scan) and its end on exit.
+template <int BLOCK>
+__device__ void scatter_items(int32_t* bound, int32_t SL, int32_t M, const int4* mrec, const int32_t* level,
+                              int64_t off, int32_t* corder, int32_t* cord_a, int32_t* cord_b) {
+    for (int32_t s = 0; s < SL; s++)
+        for (int32_t m = threadIdx.x; m < M; m += BLOCK) {
+            const int4 r = mrec[m];
+            const int32_t L = level[s * M + m], c = r.y & MR_COUNT;
+            const bool da = r.y & MR_DYN_A, db = r.y & MR_DYN_B;
+            for (int32_t j = 0; j < c; j++) {
+                const int32_t slot = atomicAdd(&bound[L - 1 + j], 1);
+                corder[slot] = item_code(r.x + j, da, db, s);
+                cord_a[slot] = (int32_t)off + r.z;
+                cord_b[slot] = (int32_t)off + r.w;
+            }
+        }
+}
+
+// The level loop of one world.  Each thread's first item of level l + 2 (code
+// and bodies) and of level l + 1 (row and inverse inertia) load while level l
+// runs.  LIGHT: velocities, positions and impulses are in LDS and a level
+// ends on an LDS barrier; otherwise on a full barrier.
+template <typename R, int BLOCK, bool LIGHT>
+__device__ void run_levels(const Dev& d, const SolveIn& in, SolveState<R> st, const SolveLds& lay, int32_t cs,
+                           int32_t D, int32_t NC, int32_t S, bool lds_bounds, const int32_t* gbound,
+                           const int32_t* corder, const int32_t* cord_a, const int32_t* cord_b) {
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    const int tid = threadIdx.x;
+    if (LIGHT) {
+        // the same LDS addresses, rebuilt here so the compiler sees LDS pointers
+        st.V = (R*)(s_dyn + lay.vel);
+        st.P = (R*)(s_dyn + lay.imp);
+        if (sizeof(R) == 8) st.pos = (double*)(s_dyn + lay.pos);
+        else st.pd = (R*)s_dyn;
+    }
+    const int32_t* lbound = (const int32_t*)(s_dyn + lay.bound);
+    const bool fp64 = sizeof(R) == sizeof(double);
+    const R* S_ = (const R*)d.csolv;
+    const R* iiw = (const R*)d.iiw;
+#define VEL(c, i) st.V[(size_t)(c) * st.vs + (size_t)((i) - st.vb)]
+#define IMP(c, ci) st.P[(size_t)(c) * st.ps + (size_t)((ci) - st.pb)]
+    struct IdRec { int32_t code, a, b; };
+    auto fetch_id = [&](int32_t slot, IdRec& r) {
+        if (slot < NC) { r.code = corder[slot]; r.a = cord_a[slot]; r.b = cord_b[slot]; }
+    };
+    auto fetch_item = [&](const IdRec& r, SolveItem<R>& it) {
+        it.s = r.code & 31;
+        it.da = (r.code >> 6) & 1;
+        it.db = (r.code >> 5) & 1;
+        it.ci = cs + (r.code >> 7);
+        it.a = r.a; it.b = r.b;
+        const int64_t ci = it.ci;
+        for (int q = 0; q < 3; q++) {
+            it.rA[q] = CON(S_, SR_RA + q, ci); it.rB[q] = CON(S_, SR_RB + q, ci);
+            it.t1[q] = CON(S_, SR_T1 + q, ci); it.t2[q] = CON(S_, SR_T2 + q, ci);
+            it.nv[q] = CON(S_, SR_NX + q, ci);
+        }
+        it.nm = CON(S_, SR_NM, ci); it.t1m = CON(S_, SR_T1M, ci); it.t2m = CON(S_, SR_T2M, ci);
+        it.vb = CON(S_, SR_VB, ci); it.fr = CON(S_, SR_FR, ci);
+        it.ima = CON(S_, SR_IMA, ci); it.imb = CON(S_, SR_IMB, ci); it.dep = CON(S_, SR_DEP, ci);
+        for (int q = 0; q < 9; q++) { it.Ia[q] = BODY(iiw, q, it.a); it.Ib[q] = BODY(iiw, q, it.b); }
+    };
+    auto pos_add = [&](int64_t i, int r, R term, bool neg) {
+        // split-impulse correction of body i (global index), component r
+        if (LIGHT) {
+            if (fp64) {
+                double& p = st.pos[(size_t)r * st.n + (size_t)(i - st.off)];
+                if (neg) p -= (double)term; else p += (double)term;
+            } else {
+                R& p = st.pd[(size_t)r * st.n + (size_t)(i - st.off)];
+                if (neg) p -= term; else p += term;
+            }
+        } else {
+            if (fp64) { if (neg) BODY(d.pos, r, i) -= (double)term; else BODY(d.pos, r, i) += (double)term; }
+            else BODY(d.pos, r, i) = neg ? BODY(d.pos, r, i) - (double)term : BODY(d.pos, r, i) + (double)term;
+        }
+    };
+    auto process = [&](const SolveItem<R>& it) {
+        const int64_t a = it.a, b = it.b, ci = it.ci;
+        if (it.s == 0) {
+            // split-impulse position pass (_solver_native.pyx:297-305)
+            const R vn = it.dep - (R)in.slop;
+            if (vn > R(0) && it.ima + it.imb > R(1e-12)) {
+                const R dL = (R)in.correction * vn / (it.ima + it.imb);
+                if (it.da) for (int r = 0; r < 3; r++) pos_add(a, r, it.ima * dL * it.nv[r], true);
+                if (it.db) for (int r = 0; r < 3; r++) pos_add(b, r, it.imb * dL * it.nv[r], false);
+            }
+        }
+        if (it.s >= S) return;
+        R va[6], vb2[6];
+        for (int c = 0; c < 6; c++) { va[c] = VEL(c, a); vb2[c] = VEL(c, b); }
+        if (it.s < in.warm) {
+            // warm start (_solver_native.pyx:258-266)
+            const R n0 = IMP(0, ci), u0 = IMP(1, ci), w0 = IMP(2, ci);
+            const R P0 = n0 * it.nv[0] + u0 * it.t1[0] + w0 * it.t2[0];
+            const R P1 = n0 * it.nv[1] + u0 * it.t1[1] + w0 * it.t2[1];
+            const R P2 = n0 * it.nv[2] + u0 * it.t1[2] + w0 * it.t2[2];
+            apply_w(va, vb2, it.ima, it.imb, it.Ia, it.Ib, it.rA, it.rB, P0, P1, P2);
+        } else {
+            // one sweep over the contact (_solver_native.pyx:268-295)
+            R nI = IMP(0, ci), tI0 = IMP(1, ci), tI1 = IMP(2, ci);
+            R dvx, dvy, dvz;
+            relvel(va, vb2, it.rA, it.rB, dvx, dvy, dvz);
+            R vn = dvx * it.nv[0] + dvy * it.nv[1] + dvz * it.nv[2];
+            R nw = nI - it.nm * (vn - it.vb);
+            if (nw < R(0)) nw = R(0);
+            R dL = nw - nI;
+            nI = nw;
+            apply_w(va, vb2, it.ima, it.imb, it.Ia, it.Ib, it.rA, it.rB, dL * it.nv[0], dL * it.nv[1], dL * it.nv[2]);
+            const R maxf = it.fr * nI;
+            relvel(va, vb2, it.rA, it.rB, dvx, dvy, dvz);
+            R vt = dvx * it.t1[0] + dvy * it.t1[1] + dvz * it.t1[2];
+            R nt = tI0 - it.t1m * vt;
+            if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
+            dL = nt - tI0; tI0 = nt;
+            apply_w(va, vb2, it.ima, it.imb, it.Ia, it.Ib, it.rA, it.rB, dL * it.t1[0], dL * it.t1[1], dL * it.t1[2]);
+            vt = dvx * it.t2[0] + dvy * it.t2[1] + dvz * it.t2[2];
+            nt = tI1 - it.t2m * vt;
+            if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
+            dL = nt - tI1; tI1 = nt;
+            apply_w(va, vb2, it.ima, it.imb, it.Ia, it.Ib, it.rA, it.rB, dL * it.t2[0], dL * it.t2[1], dL * it.t2[2]);
+            IMP(0, ci) = nI; IMP(1, ci) = tI0; IMP(2, ci) = tI1;
+        }
+        if (it.da) for (int c = 0; c < 6; c++) VEL(c, a) = va[c];
+        if (it.db) for (int c = 0; c < 6; c++) VEL(c, b) = vb2[c];
+    };
+    auto bound_at = [&](int32_t l) -> int32_t {
+        if (l >= D) return NC;
+        if (lds_bounds) return lbound[l];
+        // past L1: written by this block's atomics
+        return __hip_atomic_load(&gbound[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    // The loop is unrolled by two so the two item buffers swap roles instead
+    // of being copied (a copy would wait for the loads in flight).
+    int32_t e_prev = 0, e_cur = bound_at(0), e_next = bound_at(1);
+    IdRec idA{0, 0, 0}, idB{0, 0, 0};
+    SolveItem<R> itA, itB;
+    fetch_id(tid, idA);
+    if (tid < e_cur) fetch_item(idA, itA);
+    fetch_id(e_cur + tid, idB);     // first items of level 1
+    auto level_step = [&](int32_t l, SolveItem<R>& cur, SolveItem<R>& nxt, const IdRec& idn, IdRec& id2) {
+        fetch_id(e_next + tid, id2);                                    // level l + 2
+        const int32_t e_next2 = bound_at(l + 2);
+        if (l + 1 < D && e_cur + tid < e_next) fetch_item(idn, nxt);   // level l + 1
+        if (e_prev + tid < e_cur) process(cur);
+        for (int32_t t = e_prev + tid + BLOCK; t < e_cur; t += BLOCK) {
+            IdRec r{0, 0, 0};
+            SolveItem<R> x;
+            fetch_id(t, r);
+            fetch_item(r, x);
+            process(x);
+        }
+        if (LIGHT) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            if (BLOCK > 64) __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        } else {
+            __syncthreads();
+        }
+        e_prev = e_cur; e_cur = e_next; e_next = e_next2;
+    };
+    for (int32_t l = 0; l < D; l += 2) {
+        level_step(l, itA, itB, idB, idA);
+        if (l + 1 < D) level_step(l + 1, itB, itA, idA, idB);
+    }
+#undef VEL
+#undef IMP
+}
+
+template <typename R, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_t* wlist, const int32_t* wcount,
+                                                 int32_t lds_n) {
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    __shared__ int32_t s_scan[BLOCK / 64 + 2];
+    __shared__ int32_t s_D;
+    if ((int32_t)blockIdx.x >= *wcount) return;
+    const int32_t w = wlist[blockIdx.x];
+    const int tid = threadIdx.x;
+    if (w >= d.wcap || !(d.w_flags[w] & WF_ACTIVE)) return;
+    const int32_t K = d.w_cn[w], cs = d.w_cs[w], n = d.w_cnt[w], M = d.w_mn[w];
+    const int64_t off = d.w_off[w];
+    if (K == 0) { if (tid == 0) d.w_levels[w] = 0; return; }
+    const bool fp64 = sizeof(R) == sizeof(double);
+    // LDS when the world fits (solve_lds_layout); otherwise the level table and
+    // masks use global scratch, the velocities the FP64 state itself (FP64) or
+    // an FP32 copy (FP32), the impulses the solver rows, the positions the
+    // FP64 state.
+    const bool in_lds = n <= lds_n;
+    const bool imp_lds = in_lds && K <= 3 * lds_n;
+    const bool light = in_lds && imp_lds;
+    const SolveLds lay = solve_lds_layout(lds_n, sizeof(R));
+    R* S_ = (R*)d.csolv;
+    const int4* mrec = d.mrec + cs;
+    const int32_t iters = d.w_iters[w];
+    const int32_t S = in.warm + iters;          // velocity passes
+    const int32_t SL = S > 0 ? S : 1;           // passes in the schedule (the position pass rides on pass 0)
+    const int32_t NI = SL * M;                  // manifold items
+    const int32_t NC = SL * K;                  // contact items
+    const int64_t ib = (int64_t)cs * d.slmax;   // this world's item slice
+    int32_t* level = d.clevel + ib;
+    int32_t* corder = d.corder + ib;
+    int32_t* cord_a = d.cord_a + ib;
+    int32_t* cord_b = d.cord_b + ib;
+    int32_t* s_bound = (int32_t*)(s_dyn + lay.bound);
+    int32_t* g_bound = d.lcount + ib;
+
+    SolveState<R> st;
+    st.n = n; st.off = off; st.pd = nullptr; st.pos = nullptr;
+    if (in_lds) { st.V = (R*)(s_dyn + lay.vel); st.vs = n; st.vb = off; }
+    else { st.V = fp64 ? (R*)d.lv : (R*)d.svel; st.vs = d.ncap; st.vb = 0; }
+    if (imp_lds) { st.P = (R*)(s_dyn + lay.imp); st.ps = K; st.pb = cs; }
+    else { st.P = S_ + (size_t)SR_IMP * d.kcap; st.ps = d.kcap; st.pb = 0; }
+    if (light) {
+        if (fp64) st.pos = (double*)(s_dyn + lay.pos);
+        else st.pd = (R*)s_dyn;
+    }
+    // stage velocities, positions, impulses
+    if (in_lds) {
+        R* V = (R*)(s_dyn + lay.vel);
+        for (int32_t i = tid; i < n; i += BLOCK)
+            for (int c = 0; c < 6; c++) V[(size_t)c * n + i] = (R)BODY(d.lv, c, off + i);
+    } else if (!fp64) {
+        R* V = (R*)d.svel;
+        for (int32_t i = tid; i < n; i += BLOCK)
+            for (int c = 0; c < 6; c++) V[(size_t)c * d.ncap + off + i] = (R)BODY(d.lv, c, off + i);
+    }
+    if (light && fp64) {
+        double* P = (double*)(s_dyn + lay.pos);
+        for (int32_t i = tid; i < n; i += BLOCK)
+            for (int c = 0; c < 3; c++) P[(size_t)c * n + i] = BODY(d.pos, c, off + i);
+    }
+    if (imp_lds) {
+        R* P = (R*)(s_dyn + lay.imp);
+        for (int32_t k = tid; k < K; k += BLOCK)
+            for (int c = 0; c < 3; c++) P[(size_t)c * K + k] = CON(S_, SR_IMP + c, cs + k);
+    }
+    // level table and masks; LDS level counters
+    if (in_lds) {
+        uint64_t* sm = (uint64_t*)s_dyn;
+        int32_t* sl = (int32_t*)(sm + lds_n);
+        for (int32_t i = tid; i < n; i += BLOCK) { sl[i] = 0; sm[i] = 0; }
+    } else {
+        for (int32_t i = tid; i < n; i += BLOCK) { d.last[off + i] = 0; d.lmask[off + i] = 0; }
+    }
+    for (int32_t l = tid; l < SOLVE_DCAP; l += BLOCK) s_bound[l] = 0;
+    __syncthreads();
+    if (tid < 64) {
+        int32_t D;
+        bool over = false;
+        if (in_lds) {
+            uint64_t* sm = (uint64_t*)s_dyn;
+            int32_t* sl = (int32_t*)(sm + lds_n);
+            D = assign_item_levels<false>(LevelTable<false>{sl, sm}, M, NI, mrec, level, s_bound, SOLVE_DCAP, over);
+        } else {
+            D = assign_item_levels<true>(LevelTable<true>{d.last + off, d.lmask + off}, M, NI, mrec, level, s_bound,
+                                         SOLVE_DCAP, over);
+        }
+        if (tid == 0) {
+            s_D = over ? -D : D;
+            d.w_levels[w] = D;
+            atomicMax((unsigned long long*)&d.cnt[C_MAXLEVEL], (unsigned long long)D);
+        }
     }
     __syncthreads();
-    auto level_begin = [&](int32_t l) { return l == 0 ? 0 : d.lcount[cs + l - 1]; };
-    auto body_store = [&](int64_t a, int64_t b, const R* va, const R* vb2) {
-        if (d.mt[a] == M_DYNAMIC) for (int c = 0; c < 6; c++) VEL(c, a) = va[c];
-        if (d.mt[b] == M_DYNAMIC) for (int c = 0; c < 6; c++) VEL(c, b) = vb2[c];
-    };
-    // warm start
-    if (in.warm) {
-        for (int32_t l = 0; l < D; l++) {
-            const int32_t e = d.lcount[cs + l];
-            for (int32_t t = level_begin(l) + tid; t < e; t += BLOCK) {
-                const int64_t si = cs + t, a = sa[si], b = sb[si];
-                R va[6], vb2[6], rA[3], rB[3], Ia[9], Ib[9];
-                for (int c = 0; c < 6; c++) { va[c] = VEL(c, a); vb2[c] = VEL(c, b); }
-                for (int r = 0; r < 3; r++) { rA[r] = CON(S, SR_RA + r, si); rB[r] = CON(S, SR_RB + r, si); }
-                for (int r = 0; r < 9; r++) { Ia[r] = IIW(r, a); Ib[r] = IIW(r, b); }
-                const R n0 = CON(S, SR_IMP, si), u0 = CON(S, SR_IMP + 1, si), w0 = CON(S, SR_IMP + 2, si);
-                const R P0 = n0 * CON(S, SR_NX, si) + u0 * CON(S, SR_T1, si) + w0 * CON(S, SR_T2, si);
-                const R P1 = n0 * CON(S, SR_NX + 1, si) + u0 * CON(S, SR_T1 + 1, si) + w0 * CON(S, SR_T2 + 1, si);
-                const R P2 = n0 * CON(S, SR_NX + 2, si) + u0 * CON(S, SR_T1 + 2, si) + w0 * CON(S, SR_T2 + 2, si);
-                apply_w(va, vb2, CON(S, SR_IMA, si), CON(S, SR_IMB, si), Ia, Ib, rA, rB, P0, P1, P2);
-                body_store(a, b, va, vb2);
-            }
-            __syncthreads();
-        }
-    }
-    const int32_t iters = d.w_iters[w];
-    for (int32_t it = 0; it < iters; it++) {
-        for (int32_t l = 0; l < D; l++) {
-            const int32_t e = d.lcount[cs + l];
-            for (int32_t t = level_begin(l) + tid; t < e; t += BLOCK) {
-                const int64_t si = cs + t, a = sa[si], b = sb[si];
-                R va[6], vb2[6], rA[3], rB[3], Ia[9], Ib[9], t1[3], t2[3];
-                for (int c = 0; c < 6; c++) { va[c] = VEL(c, a); vb2[c] = VEL(c, b); }
-                for (int r = 0; r < 3; r++) {
-                    rA[r] = CON(S, SR_RA + r, si); rB[r] = CON(S, SR_RB + r, si);
-                    t1[r] = CON(S, SR_T1 + r, si); t2[r] = CON(S, SR_T2 + r, si);
-                }
-                for (int r = 0; r < 9; r++) { Ia[r] = IIW(r, a); Ib[r] = IIW(r, b); }
-                const R ima = CON(S, SR_IMA, si), imb = CON(S, SR_IMB, si);
-                const R nx = CON(S, SR_NX, si), ny = CON(S, SR_NX + 1, si), nz = CON(S, SR_NX + 2, si);
-                R nI = CON(S, SR_IMP, si), tI0 = CON(S, SR_IMP + 1, si), tI1 = CON(S, SR_IMP + 2, si);
-                R dvx, dvy, dvz;
-                relvel(va, vb2, rA, rB, dvx, dvy, dvz);
-                R vn = dvx * nx + dvy * ny + dvz * nz;
-                R nw = nI - CON(S, SR_NM, si) * (vn - CON(S, SR_VB, si));
-                if (nw < R(0)) nw = R(0);
-                R dL = nw - nI;
-                nI = nw;
-                apply_w(va, vb2, ima, imb, Ia, Ib, rA, rB, dL * nx, dL * ny, dL * nz);
-                const R maxf = CON(S, SR_FR, si) * nI;
-                relvel(va, vb2, rA, rB, dvx, dvy, dvz);
-                R vt = dvx * t1[0] + dvy * t1[1] + dvz * t1[2];
-                R nt = tI0 - CON(S, SR_T1M, si) * vt;
-                if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
-                dL = nt - tI0; tI0 = nt;
-                apply_w(va, vb2, ima, imb, Ia, Ib, rA, rB, dL * t1[0], dL * t1[1], dL * t1[2]);
-                vt = dvx * t2[0] + dvy * t2[1] + dvz * t2[2];
-                nt = tI1 - CON(S, SR_T2M, si) * vt;
-                if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
-                dL = nt - tI1; tI1 = nt;
-                apply_w(va, vb2, ima, imb, Ia, Ib, rA, rB, dL * t2[0], dL * t2[1], dL * t2[2]);
-                CON(S, SR_IMP, si) = nI; CON(S, SR_IMP + 1, si) = tI0; CON(S, SR_IMP + 2, si) = tI1;
-                body_store(a, b, va, vb2);
-            }
-            __syncthreads();
-        }
-    }
-    // split-impulse position pass, same order
-    for (int32_t l = 0; l < D; l++) {
-        const int32_t e = d.lcount[cs + l];
-        for (int32_t t = level_begin(l) + tid; t < e; t += BLOCK) {
-            const int64_t si = cs + t;
-            const R vn = CON(S, SR_DEP, si) - (R)in.slop;
-            if (vn <= R(0)) continue;
-            const R ia = CON(S, SR_IMA, si), ib = CON(S, SR_IMB, si);
-            if (ia + ib <= R(1e-12)) continue;
-            const R dL = (R)in.correction * vn / (ia + ib);
-            const int64_t a = sa[si], b = sb[si];
-            if (d.mt[a] == M_DYNAMIC)
-                for (int r = 0; r < 3; r++) {
-                    if (fp64) BODY(d.pos, r, a) -= (double)(ia * dL * CON(S, SR_NX + r, si));
-                    else BODY(d.pos, r, a) = BODY(d.pos, r, a) - (double)(ia * dL * CON(S, SR_NX + r, si));
-                }
-            if (d.mt[b] == M_DYNAMIC)
-                for (int r = 0; r < 3; r++) {
-                    if (fp64) BODY(d.pos, r, b) += (double)(ib * dL * CON(S, SR_NX + r, si));
-                    else BODY(d.pos, r, b) = BODY(d.pos, r, b) + (double)(ib * dL * CON(S, SR_NX + r, si));
-                }
-        }
+    const bool lds_bounds = s_D > 0;
+    const int32_t D = lds_bounds ? s_D : -s_D;
+    if (light && !fp64)
+        for (int32_t i = tid; i < 3 * n; i += BLOCK) ((R*)s_dyn)[i] = R(0);   // position corrections
+    // contacts per level -> level starts -> items scattered by level
+    if (lds_bounds) {
+        block_exscan<BLOCK>(s_bound, D, s_scan);
+        scatter_items<BLOCK>(s_bound, SL, M, mrec, level, off, corder, cord_a, cord_b);
+    } else {
+        for (int32_t l = tid; l < D; l += BLOCK) g_bound[l] = 0;
         __syncthreads();
+        for (int32_t s = 0; s < SL; s++)
+            for (int32_t m = tid; m < M; m += BLOCK) {
+                const int32_t L = level[s * M + m], c = mrec[m].y & MR_COUNT;
+                for (int32_t j = 0; j < c; j++) atomicAdd(&g_bound[L - 1 + j], 1);
+            }
+        __syncthreads();
+        block_exscan<BLOCK>(g_bound, D, s_scan);
+        scatter_items<BLOCK>(g_bound, SL, M, mrec, level, off, corder, cord_a, cord_b);
     }
-    // impulses back to contact order (warm-start cache and readback)
-    for (int32_t t = tid; t < K; t += BLOCK) {
-        const int64_t ci = cs + d.corder[cs + t], si = cs + t;
-        for (int r = 0; r < 3; r++) CON(imp, r, ci) = CON(S, SR_IMP + r, si);
+    __syncthreads();
+    // bound[l] is now the end of level l
+    if (light) run_levels<R, BLOCK, true>(d, in, st, lay, cs, D, NC, S, lds_bounds, g_bound, corder, cord_a, cord_b);
+    else run_levels<R, BLOCK, false>(d, in, st, lay, cs, D, NC, S, lds_bounds, g_bound, corder, cord_a, cord_b);
+    __syncthreads();
+    // impulses back to the contacts (warm-start cache and readback)
+    R* imp = (R*)d.cimp;
+    for (int32_t k = tid; k < K; k += BLOCK) {
+        const int64_t ci = cs + k;
+        for (int r = 0; r < 3; r++)
+            CON(imp, r, ci) = imp_lds ? ((R*)(s_dyn + lay.imp))[(size_t)r * K + k] : CON(S_, SR_IMP + r, ci);
     }
     // velocities back to the FP64 state: copied from LDS (FP64), or the FP32
-    // solver's change folded in
-    if (state_lds || !fp64)
-        for (int32_t i = tid; i < n; i += BLOCK)
-            if (d.mt[off + i] == M_DYNAMIC)
-                for (int c = 0; c < 6; c++) {
-                    if (fp64) {
-                        BODY(d.lv, c, off + i) = (double)VEL(c, off + i);
-                    } else {
-                        double v0 = BODY(d.lv, c, off + i);
-                        R dv = VEL(c, off + i) - (R)v0;
-                        BODY(d.lv, c, off + i) = v0 + (double)dv;
-                    }
+    // solver's change folded in; positions from LDS
+    for (int32_t i = tid; i < n; i += BLOCK) {
+        if (d.mt[off + i] != M_DYNAMIC) continue;
+        if (in_lds || !fp64) {
+            const R* V = in_lds ? (const R*)(s_dyn + lay.vel) : (const R*)d.svel;
+            const size_t vs = in_lds ? (size_t)n : (size_t)d.ncap;
+            const size_t vi = in_lds ? (size_t)i : (size_t)(off + i);
+            for (int c = 0; c < 6; c++) {
+                if (fp64) {
+                    BODY(d.lv, c, off + i) = (double)V[c * vs + vi];
+                } else {
+                    double v0 = BODY(d.lv, c, off + i);
+                    R dv = V[c * vs + vi] - (R)v0;
+                    BODY(d.lv, c, off + i) = v0 + (double)dv;
                 }
-#undef VEL
-#undef IIW
+            }
+        }
+        if (light) {
+            if (fp64) {
+                const double* P = (const double*)(s_dyn + lay.pos);
+                for (int c = 0; c < 3; c++) BODY(d.pos, c, off + i) = P[(size_t)c * n + i];
+            } else {
+                const R* pd = (const R*)s_dyn;
+                for (int c = 0; c < 3; c++) BODY(d.pos, c, off + i) = BODY(d.pos, c, off + i) + (double)pd[(size_t)c * n + i];
+            }
+        }
+    }
 }
 
 
@@ -1598,38 +1900,78 @@ void launch_wake(const Dev& d, hipStream_t s) {
 }
 
 template <typename R, int B>
-static void solve_one(const Dev& d, SolveIn in, const int32_t* wl, const int32_t* wn, int32_t lds_last,
-                      int32_t lds_state, bool stage, hipStream_t s) {
-    size_t lds = (((size_t)lds_last * 4 + 15) & ~(size_t)15) + (size_t)lds_state * 15 * sizeof(R);
-    if (stage)
-        hipLaunchKernelGGL((k_solve<R, B, true>), dim3(d.wcap), dim3(B), lds, s, d, in, wl, wn, lds_last, lds_state);
-    else
-        hipLaunchKernelGGL((k_solve<R, B, false>), dim3(d.wcap), dim3(B), lds, s, d, in, wl, wn, lds_last, lds_state);
+static void solve_one(const Dev& d, SolveIn in, const int32_t* wl, const int32_t* wn, int32_t lds_n, hipStream_t s) {
+    hipLaunchKernelGGL((k_solve<R, B>), dim3(d.wcap), dim3(B), solve_lds_layout(lds_n, sizeof(R)).total, s, d, in, wl, wn,
+                       lds_n);
 }
 
 template <typename R>
-static void solve_block(const Dev& d, int block, SolveIn in, const int32_t* wl, const int32_t* wn,
-                        int32_t lds_last, int32_t lds_state, bool stage, hipStream_t s) {
+static void solve_block(const Dev& d, int block, SolveIn in, const int32_t* wl, const int32_t* wn, int32_t lds_n,
+                        hipStream_t s) {
     switch (block) {
-        case 64: solve_one<R, 64>(d, in, wl, wn, lds_last, lds_state, stage, s); break;
-        case 128: solve_one<R, 128>(d, in, wl, wn, lds_last, lds_state, stage, s); break;
-        case 512: solve_one<R, 512>(d, in, wl, wn, lds_last, lds_state, stage, s); break;
-        case 1024: solve_one<R, 1024>(d, in, wl, wn, lds_last, lds_state, stage, s); break;
-        default: solve_one<R, 256>(d, in, wl, wn, lds_last, lds_state, stage, s); break;
+        case 64: solve_one<R, 64>(d, in, wl, wn, lds_n, s); break;
+        case 128: solve_one<R, 128>(d, in, wl, wn, lds_n, s); break;
+        case 512: solve_one<R, 512>(d, in, wl, wn, lds_n, s); break;
+        case 1024: solve_one<R, 1024>(d, in, wl, wn, lds_n, s); break;
+        default: solve_one<R, 256>(d, in, wl, wn, lds_n, s); break;
     }
 }
 
-// For size class c, lds_last[c] bodies keep their level counters in LDS and
-// lds_state[c] bodies their velocities and inverse inertia; larger worlds of
-// the class use global memory for them.
+// Let the solver and broadphase-sort launches use `bytes` of dynamic LDS;
+// returns false (and clears the error) if the runtime refuses.
+template <typename F>
+static bool allow_lds_fn(F f, int bytes) {
+    if (hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess)
+        return true;
+    (void)hipGetLastError();
+    return false;
+}
+
+template <typename R, int B>
+static bool allow_lds_one(int bytes) { return allow_lds_fn(k_solve<R, B>, bytes); }
+
+bool allow_dynamic_lds(int bytes) {
+    bool ok = true;
+    ok = allow_lds_one<double, 64>(bytes) && ok; ok = allow_lds_one<double, 128>(bytes) && ok;
+    ok = allow_lds_one<double, 256>(bytes) && ok; ok = allow_lds_one<double, 512>(bytes) && ok;
+    ok = allow_lds_one<double, 1024>(bytes) && ok;
+    ok = allow_lds_one<float, 64>(bytes) && ok; ok = allow_lds_one<float, 128>(bytes) && ok;
+    ok = allow_lds_one<float, 256>(bytes) && ok; ok = allow_lds_one<float, 512>(bytes) && ok;
+    ok = allow_lds_one<float, 1024>(bytes) && ok;
+    ok = allow_lds_fn(k_sap_sort<64, false>, bytes) && ok; ok = allow_lds_fn(k_sap_sort<128, false>, bytes) && ok;
+    ok = allow_lds_fn(k_sap_sort<256, false>, bytes) && ok; ok = allow_lds_fn(k_sap_sort<512, false>, bytes) && ok;
+    ok = allow_lds_fn(k_sap_sort<1024, false>, bytes) && ok;
+    return ok;
+}
+
+// Largest world kept wholly in LDS by a solve launch with `budget` bytes.
+int32_t solve_lds_bodies(int precision, int64_t budget, int32_t cap) {
+    int32_t lo = 0, hi = cap;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) / 2;
+        const size_t need = solve_lds_layout(mid, precision == 64 ? 8 : 4).total;
+        if ((int64_t)need <= budget) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Solver rows for every contact, then one block per world by size class:
+// worlds of at most lds_n[c] bodies keep their schedule state in LDS.
 void launch_solve(const Dev& d, int precision, const int32_t* blocks, SolveIn in, const int32_t* wl,
-                  const int32_t* wn, const int32_t* lds_last, const int32_t* lds_state, const int32_t* live,
-                  bool stage, hipStream_t s) {
+                  const int32_t* wn, const int32_t* lds_n, const int32_t* live, bool stage, hipStream_t s) {
+    const int g = GRID_FOR(d.kcap, 256);
+    if (precision == 64) {
+        if (stage) hipLaunchKernelGGL((k_prepare<double, true>), dim3(g), dim3(256), 0, s, d, in);
+        else hipLaunchKernelGGL((k_prepare<double, false>), dim3(g), dim3(256), 0, s, d, in);
+    } else {
+        if (stage) hipLaunchKernelGGL((k_prepare<float, true>), dim3(g), dim3(256), 0, s, d, in);
+        else hipLaunchKernelGGL((k_prepare<float, false>), dim3(g), dim3(256), 0, s, d, in);
+    }
     for (int c = 0; c < 4; c++) {
         if (!live[c]) continue;
         const int32_t* l = wl + (size_t)c * d.wcap;
-        if (precision == 64) solve_block<double>(d, blocks[c], in, l, wn + c, lds_last[c], lds_state[c], stage, s);
-        else solve_block<float>(d, blocks[c], in, l, wn + c, lds_last[c], lds_state[c], stage, s);
+        if (precision == 64) solve_block<double>(d, blocks[c], in, l, wn + c, lds_n[c], s);
+        else solve_block<float>(d, blocks[c], in, l, wn + c, lds_n[c], s);
     }
 }
 

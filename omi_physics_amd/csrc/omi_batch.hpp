@@ -13,6 +13,8 @@ namespace omi {
 
 enum : int32_t { WF_ACTIVE = 1, WF_COLLISION = 2, WF_SLEEP = 4, WF_SEQWAKE = 8, WF_JOINTS = 16 };
 enum : uint8_t { BF_COLLIDER = 1, BF_TRIGGER = 2 };
+// manifold record field y: contacts | side a dynamic | side b dynamic
+enum : int32_t { MR_COUNT = 0xff, MR_DYN_A = 0x100, MR_DYN_B = 0x200 };
 
 // counters (uint64 slots)
 enum : int32_t {
@@ -49,6 +51,7 @@ struct Dev {
     int32_t *mat, *filt, *hoff, *hcnt;
     uint8_t *mt, *kind, *bflags, *awake, *awake0, *contacted, *notready;
     int32_t *parent, *last;               // union-find, level schedule scratch
+    uint64_t *lmask;                      // [ncap] level schedule body masks (worlds too large for LDS)
     uint64_t *skey;                       // [2*ncap] broadphase sort keys (large worlds)
     int32_t *sidx, *soff;                 // [2*ncap] broadphase scratch; soff: window prefix per world
     double *sbox;                         // [4][ncap] y/z bounds in sorted order (lo.y, hi.y, lo.z, hi.z)
@@ -61,6 +64,7 @@ struct Dev {
     double *w_grav, *w_damp;              // [3][wcap], [2][wcap]
     int32_t *w_ps, *w_pn, *w_bps, *w_bpn; // solid / broadphase pair ranges
     int32_t *w_cs, *w_cn;                 // contact ranges
+    int32_t *w_mn;                        // manifolds (runs of contacts of one pair) per world
     int32_t *w_levels;                    // levels of the last solve
     int64_t *w_x;                         // x-window candidates of the step
     int32_t *w_t0, *w_tn;                 // candidate tiles: first, count
@@ -82,9 +86,11 @@ struct Dev {
     int32_t *ca, *cb;
     double *cpt;                          // [3][kcap]
     void *cnrm, *cdep, *cimp;             // R [3][kcap], R [kcap], R [3][kcap] (n, t1, t2 impulses)
-    void *csolv;                          // R [26][kcap] solver rows, level-sorted
-    int32_t *sab;                         // [2][kcap] body indices of the sorted rows
-    int32_t *clevel, *corder, *lcount;    // level schedule
+    void *csolv;                          // R [26][kcap] solver rows
+    int4 *mrec;                           // [kcap] manifolds: first contact, contacts, dynamic a, dynamic b
+    int32_t slmax;                        // passes per solve the schedule holds (iterations + warm start)
+    int32_t *clevel, *corder, *lcount;    // [slmax * kcap] level schedule of the solver passes
+    int32_t *cord_a, *cord_b;             // [slmax * kcap] bodies of each scheduled item
     // warm-start cache, two buffers
     int32_t *ka[2], *kb[2];
     double *kpt[2];                       // [3][kcap]
