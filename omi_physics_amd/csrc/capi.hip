@@ -228,7 +228,7 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     A(d.lo, 3 * N); A(d.hi, 3 * N); A(d.half, 3 * N); A(d.inv_mass, N); A(d.inv_inertia, 3 * N);
     A(d.gf, N); A(d.ld, N); A(d.ad, N); A(d.qd, N); A(d.timer, N);
     unsigned char *rot, *iiw, *svel;
-    A(rot, 9 * N * (int64_t)rs); A(iiw, 9 * N * (int64_t)rs);
+    A(rot, 9 * N * (int64_t)rs); A(iiw, IIW_STRIDE * N * (int64_t)rs);
     A(svel, cfg->precision == OMI_FP32 ? 6 * N * 4 : 1);
     d.rot = rot; d.iiw = iiw; d.svel = svel;
     A(d.world, N); A(d.mat, N); A(d.filt, N); A(d.hoff, N); A(d.hcnt, N);
@@ -268,7 +268,7 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     A(mn, 3 * P * (int64_t)rs); A(md, 4 * P * (int64_t)rs);
     d.mnrm = mn; d.mdep = md;
     A(d.ca, K); A(d.cb, K); A(d.cpt, 3 * K);
-    A(cn, 3 * K * (int64_t)rs); A(cd, K * (int64_t)rs); A(ci, 3 * K * (int64_t)rs); A(cs, 26 * K * (int64_t)rs);
+    A(cn, 3 * K * (int64_t)rs); A(cd, K * (int64_t)rs); A(ci, 3 * K * (int64_t)rs); A(cs, SOLVE_ROW_BYTES * K);
     A(d.mrec, K);
     d.cnrm = cn; d.cdep = cd; d.cimp = ci; d.csolv = cs;
     d.slmax = 1 + (cfg->max_solver_iterations > 0 ? cfg->max_solver_iterations : 16);
@@ -1039,6 +1039,19 @@ extern "C" int32_t omi_prepare_and_solve(int64_t K, int32_t N, const int64_t* a_
         }
         return OMI_OK;
     };
+    // world inverse inertia [N][9] -> one IIW_STRIDE record per body
+    auto upIIW = [&](const double* src) -> int32_t {
+        std::vector<unsigned char> tmp((size_t)N * IIW_STRIDE * rs, 0);
+        for (int64_t i = 0; i < N; i++)
+            for (int c = 0; c < 9; c++) {
+                double v = src[i * 9 + c];
+                if (f64) ((double*)tmp.data())[i * IIW_STRIDE + c] = v;
+                else ((float*)tmp.data())[i * IIW_STRIDE + c] = (float)v;
+            }
+        if (hipMemcpy(d.iiw, tmp.data(), tmp.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(OMI_ERR_HIP, "upload failed");
+        return OMI_OK;
+    };
     std::vector<double> imp((size_t)K * 3);
     for (int64_t k = 0; k < K; k++) { imp[3 * k] = nImp[k]; imp[3 * k + 1] = tImp[2 * k]; imp[3 * k + 2] = tImp[2 * k + 1]; }
     DevBuf tb;
@@ -1048,7 +1061,7 @@ extern "C" int32_t omi_prepare_and_solve(int64_t K, int32_t N, const int64_t* a_
     if (!(rc = upload_soa(b, d.pos, d.ncap, 0, pos, N, 3)) && !(rc = upload_soa(b, d.lv, d.ncap, 0, lv, N, 3)) &&
         !(rc = upload_soa(b, d.av, d.ncap, 0, av, N, 3)) && !(rc = put(b, d.inv_mass, inv_mass, N)) &&
         !(rc = put(b, d.mt, mt.data(), N)) && !(rc = put(b, d.world, wz.data(), N)) &&
-        !(rc = upR(d.iiw, d.ncap, invIw, N, 9, 9)) && !(rc = put(b, d.ca, ia.data(), K)) &&
+        !(rc = upIIW(invIw)) && !(rc = put(b, d.ca, ia.data(), K)) &&
         !(rc = put(b, d.cb, ib.data(), K)) && !(rc = upload_soa(b, d.cpt, d.kcap, 0, point, K, 3)) &&
         !(rc = upR(d.cnrm, d.kcap, normal, K, 3, 3)) && !(rc = upR(d.cdep, d.kcap, depth, K, 1, 1)) &&
         !(rc = upR(d.cimp, d.kcap, imp.data(), K, 3, 3)) && !(rc = put(b, d.w_off, &zero, 1)) &&

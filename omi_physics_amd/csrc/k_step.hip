@@ -18,6 +18,10 @@ namespace omi {
 
 #define BODY(a, c, i) (a)[(size_t)(c) * d.ncap + (size_t)(i)]
 #define CON(a, c, k) (a)[(size_t)(c) * d.kcap + (size_t)(k)]
+// world inverse inertia of body i, row-major 3x3 in a 12-slot record
+#define IIW(a, i, c) (a)[(size_t)(i) * IIW_STRIDE + (size_t)(c)]
+// field f of contact ci's solver row (one record of RowLayout<R>::stride slots)
+#define ROW(a, ci, f) (a)[(size_t)(ci) * RowLayout<R>::stride + (size_t)(f)]
 
 __device__ __forceinline__ int64_t gtid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ int64_t gstride() { return (int64_t)gridDim.x * blockDim.x; }
@@ -181,7 +185,7 @@ __global__ void k_pre(Dev d, int64_t nhw, double dt, double margin) {
                 for (int c = 0; c < 3; c++) {
                     double p0 = (Rm[3 * r] * dg[0]) * Rm[3 * c], p1 = (Rm[3 * r + 1] * dg[1]) * Rm[3 * c + 1];
                     double p2 = (Rm[3 * r + 2] * dg[2]) * Rm[3 * c + 2];
-                    BODY(iiw, 3 * r + c, i) = (R)((p0 + p2) + p1);
+                    IIW(iiw, i, 3 * r + c) = (R)((p0 + p2) + p1);
                 }
         }
     }
@@ -744,8 +748,34 @@ __global__ __launch_bounds__(BLOCK) void k_wake(Dev d) {
 // ---------------------------------------------------------------- solver --
 
 // Solver row layout per contact (R [SR_N][kcap]).
+// FP64 row: the reference's per-contact quantities; the sweep recomputes the
+// angular terms in the reference's operation order.
 enum { SR_RA = 0, SR_RB = 3, SR_T1 = 6, SR_T2 = 9, SR_NM = 12, SR_T1M = 13, SR_T2M = 14, SR_VB = 15,
        SR_FR = 16, SR_IMA = 17, SR_IMB = 18, SR_NX = 19, SR_IMP = 22, SR_DEP = 25, SR_N = 26 };
+// FP32 row: directions d = n, t1, t2 with the Jacobian terms folded in,
+// cA_d = rA x d, cB_d = rB x d, kA_d = IA (rA x d), kB_d = IB (rB x d), so a
+// sweep is dot products and scaled adds (FMA) on the same quantities.
+enum { FR_DIR = 0, FR_CA = 9, FR_CB = 18, FR_KA = 27, FR_KB = 36, FR_NM = 45, FR_T1M = 46, FR_T2M = 47,
+       FR_VB = 48, FR_FR = 49, FR_IMA = 50, FR_IMB = 51, FR_DEP = 52, FR_IMP = 53, FR_STRIDE = 56 };
+template <typename R> struct RowLayout;
+template <> struct RowLayout<double> {
+    static constexpr int stride = SR_STRIDE, imp = SR_IMP, dep = SR_DEP, ima = SR_IMA, imb = SR_IMB, nrm = SR_NX;
+};
+template <> struct RowLayout<float> {
+    static constexpr int stride = FR_STRIDE, imp = FR_IMP, dep = FR_DEP, ima = FR_IMA, imb = FR_IMB, nrm = FR_DIR;
+};
+
+// n values of type R from 16-byte aligned memory as 16-byte loads
+template <typename R, int N>
+__device__ __forceinline__ void load_vec(const R* src, R* dst) {
+    constexpr int P = 16 / sizeof(R);
+    static_assert(N % P == 0, "whole 16-byte chunks");
+#pragma unroll
+    for (int q = 0; q < N / P; q++) {
+        const uint4 t = ((const uint4*)src)[q];
+        __builtin_memcpy(dst + q * P, &t, 16);
+    }
+}
 
 // _solver_native._effmass (_solver_native.pyx:154-176)
 template <typename R>
@@ -953,9 +983,11 @@ struct SolveItem {
     int64_t ci, a, b;
     int32_t s;
     bool da, db;
-    R rA[3], rB[3], t1[3], t2[3], nv[3], nm, t1m, t2m, vb, fr, ima, imb, dep;
-    R Ia[9], Ib[9];
+    R row[RowLayout<R>::stride];
+    R Ia[sizeof(R) == 8 ? IIW_STRIDE : 1], Ib[sizeof(R) == 8 ? IIW_STRIDE : 1];   // FP64 rows only
 };
+static_assert(SR_STRIDE * sizeof(double) == SOLVE_ROW_BYTES && FR_STRIDE * sizeof(float) == SOLVE_ROW_BYTES,
+              "solver rows fill SOLVE_ROW_BYTES in both precisions");
 
 // Solver rows for every contact of the step, one thread per contact
 // (_solver_native.pyx:179-256): lever arms, tangent basis, effective masses,
@@ -993,7 +1025,7 @@ __global__ void k_prepare(Dev d, SolveIn in) {
             rA[r] = (R)(p - BODY(d.pos, r, a));
             rB[r] = (R)(p - BODY(d.pos, r, b));
         }
-        for (int r = 0; r < 9; r++) { Ia[r] = BODY(iiw, r, a); Ib[r] = BODY(iiw, r, b); }
+        for (int r = 0; r < 9; r++) { Ia[r] = IIW(iiw, a, r); Ib[r] = IIW(iiw, b, r); }
         for (int c = 0; c < 6; c++) { va[c] = (R)BODY(d.lv, c, a); vb_[c] = (R)BODY(d.lv, c, b); }
         const R nx = CON(cn, 0, ci), ny = CON(cn, 1, ci), nz = CON(cn, 2, ci);
         R axx, ayy, azz;
@@ -1007,18 +1039,48 @@ __global__ void k_prepare(Dev d, SolveIn in) {
         R dvx, dvy, dvz;
         relvel(va, vb_, rA, rB, dvx, dvy, dvz);
         const R vn0 = dvx * nx + dvy * ny + dvz * nz;
-        for (int r = 0; r < 3; r++) { CON(S_, SR_RA + r, ci) = rA[r]; CON(S_, SR_RB + r, ci) = rB[r]; }
-        CON(S_, SR_T1, ci) = tx; CON(S_, SR_T1 + 1, ci) = ty; CON(S_, SR_T1 + 2, ci) = tz;
-        CON(S_, SR_T2, ci) = ux; CON(S_, SR_T2 + 1, ci) = uy; CON(S_, SR_T2 + 2, ci) = uz;
-        CON(S_, SR_NM, ci) = effmass(rA, rB, nx, ny, nz, Ia, Ib, ima, imb);
-        CON(S_, SR_T1M, ci) = effmass(rA, rB, tx, ty, tz, Ia, Ib, ima, imb);
-        CON(S_, SR_T2M, ci) = effmass(rA, rB, ux, uy, uz, Ia, Ib, ima, imb);
-        CON(S_, SR_VB, ci) = vn0 < -(R)in.rthr ? -rest * vn0 : R(0);
-        CON(S_, SR_FR, ci) = fric;
-        CON(S_, SR_IMA, ci) = ima; CON(S_, SR_IMB, ci) = imb;
-        CON(S_, SR_NX, ci) = nx; CON(S_, SR_NX + 1, ci) = ny; CON(S_, SR_NX + 2, ci) = nz;
-        CON(S_, SR_IMP, ci) = n0; CON(S_, SR_IMP + 1, ci) = u0; CON(S_, SR_IMP + 2, ci) = w0;
-        CON(S_, SR_DEP, ci) = cdep[ci];
+        constexpr int RS = RowLayout<R>::stride;
+        R row[RS];
+        if (sizeof(R) == sizeof(double)) {
+            for (int r = 0; r < 3; r++) { row[SR_RA + r] = rA[r]; row[SR_RB + r] = rB[r]; }
+            row[SR_T1] = tx; row[SR_T1 + 1] = ty; row[SR_T1 + 2] = tz;
+            row[SR_T2] = ux; row[SR_T2 + 1] = uy; row[SR_T2 + 2] = uz;
+            row[SR_NX] = nx; row[SR_NX + 1] = ny; row[SR_NX + 2] = nz;
+            row[SR_N] = R(0); row[SR_N + 1] = R(0);
+        } else {
+            const R dir[9] = {nx, ny, nz, tx, ty, tz, ux, uy, uz};
+            for (int q = 0; q < 9; q++) row[FR_DIR + q] = dir[q];
+            for (int dd = 0; dd < 3; dd++) {
+                const R* e = dir + 3 * dd;
+                const R ca[3] = {rA[1] * e[2] - rA[2] * e[1], rA[2] * e[0] - rA[0] * e[2], rA[0] * e[1] - rA[1] * e[0]};
+                const R cb[3] = {rB[1] * e[2] - rB[2] * e[1], rB[2] * e[0] - rB[0] * e[2], rB[0] * e[1] - rB[1] * e[0]};
+                for (int r = 0; r < 3; r++) {
+                    row[FR_CA + 3 * dd + r] = ca[r];
+                    row[FR_CB + 3 * dd + r] = cb[r];
+                    row[FR_KA + 3 * dd + r] = Ia[3 * r] * ca[0] + Ia[3 * r + 1] * ca[1] + Ia[3 * r + 2] * ca[2];
+                    row[FR_KB + 3 * dd + r] = Ib[3 * r] * cb[0] + Ib[3 * r + 1] * cb[1] + Ib[3 * r + 2] * cb[2];
+                }
+            }
+        }
+        const int NM_ = sizeof(R) == 8 ? (int)SR_NM : (int)FR_NM, T1M_ = sizeof(R) == 8 ? (int)SR_T1M : (int)FR_T1M;
+        const int T2M_ = sizeof(R) == 8 ? (int)SR_T2M : (int)FR_T2M, VB_ = sizeof(R) == 8 ? (int)SR_VB : (int)FR_VB;
+        const int FRI_ = sizeof(R) == 8 ? (int)SR_FR : (int)FR_FR;
+        row[NM_] = effmass(rA, rB, nx, ny, nz, Ia, Ib, ima, imb);
+        row[T1M_] = effmass(rA, rB, tx, ty, tz, Ia, Ib, ima, imb);
+        row[T2M_] = effmass(rA, rB, ux, uy, uz, Ia, Ib, ima, imb);
+        row[VB_] = vn0 < -(R)in.rthr ? -rest * vn0 : R(0);
+        row[FRI_] = fric;
+        row[RowLayout<R>::ima] = ima; row[RowLayout<R>::imb] = imb;
+        row[RowLayout<R>::imp] = n0; row[RowLayout<R>::imp + 1] = u0; row[RowLayout<R>::imp + 2] = w0;
+        row[RowLayout<R>::dep] = cdep[ci];
+        R* dst = &ROW(S_, ci, 0);
+        constexpr int P = 16 / sizeof(R);
+#pragma unroll
+        for (int q = 0; q < RS / P; q++) {
+            uint4 t;
+            __builtin_memcpy(&t, row + q * P, 16);
+            ((uint4*)dst)[q] = t;
+        }
     }
 }
 
@@ -1046,7 +1108,7 @@ __host__ __device__ inline SolveLds solve_lds_layout(int32_t lds_n, size_t rsize
 template <typename R>
 struct SolveState {
     R* V; int64_t vs, vb;          // velocities: V[c * vs + (i - vb)]
-    R* P; int64_t ps, pb;          // impulses: P[c * ps + (ci - pb)]
+    R* P; int64_t ps, pc, pb;      // impulses: P[(ci - pb) * ps + c * pc]
     R* pd;                         // FP32 position corrections in LDS [3][n], or null
     double* pos;                   // FP64 positions in LDS [3][n], or null
     int64_t n, off;
@@ -1093,7 +1155,7 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, SolveState<R> st, co
     const R* S_ = (const R*)d.csolv;
     const R* iiw = (const R*)d.iiw;
 #define VEL(c, i) st.V[(size_t)(c) * st.vs + (size_t)((i) - st.vb)]
-#define IMP(c, ci) st.P[(size_t)(c) * st.ps + (size_t)((ci) - st.pb)]
+#define IMP(c, ci) st.P[(size_t)((ci) - st.pb) * st.ps + (size_t)(c) * st.pc]
     struct IdRec { int32_t code, a, b; };
     auto fetch_id = [&](int32_t slot, IdRec& r) {
         if (slot < NC) { r.code = corder[slot]; r.a = cord_a[slot]; r.b = cord_b[slot]; }
@@ -1104,16 +1166,11 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, SolveState<R> st, co
         it.db = (r.code >> 5) & 1;
         it.ci = cs + (r.code >> 7);
         it.a = r.a; it.b = r.b;
-        const int64_t ci = it.ci;
-        for (int q = 0; q < 3; q++) {
-            it.rA[q] = CON(S_, SR_RA + q, ci); it.rB[q] = CON(S_, SR_RB + q, ci);
-            it.t1[q] = CON(S_, SR_T1 + q, ci); it.t2[q] = CON(S_, SR_T2 + q, ci);
-            it.nv[q] = CON(S_, SR_NX + q, ci);
+        load_vec<R, RowLayout<R>::stride>(&ROW(S_, it.ci, 0), it.row);
+        if constexpr (sizeof(R) == 8) {
+            load_vec<R, IIW_STRIDE>(&IIW(iiw, it.a, 0), it.Ia);
+            load_vec<R, IIW_STRIDE>(&IIW(iiw, it.b, 0), it.Ib);
         }
-        it.nm = CON(S_, SR_NM, ci); it.t1m = CON(S_, SR_T1M, ci); it.t2m = CON(S_, SR_T2M, ci);
-        it.vb = CON(S_, SR_VB, ci); it.fr = CON(S_, SR_FR, ci);
-        it.ima = CON(S_, SR_IMA, ci); it.imb = CON(S_, SR_IMB, ci); it.dep = CON(S_, SR_DEP, ci);
-        for (int q = 0; q < 9; q++) { it.Ia[q] = BODY(iiw, q, it.a); it.Ib[q] = BODY(iiw, q, it.b); }
     };
     auto pos_add = [&](int64_t i, int r, R term, bool neg) {
         // split-impulse correction of body i (global index), component r
@@ -1132,52 +1189,120 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, SolveState<R> st, co
     };
     auto process = [&](const SolveItem<R>& it) {
         const int64_t a = it.a, b = it.b, ci = it.ci;
+        const R ima = it.row[RowLayout<R>::ima], imb = it.row[RowLayout<R>::imb];
+        const R* nv = it.row + RowLayout<R>::nrm;
         if (it.s == 0) {
             // split-impulse position pass (_solver_native.pyx:297-305)
-            const R vn = it.dep - (R)in.slop;
-            if (vn > R(0) && it.ima + it.imb > R(1e-12)) {
-                const R dL = (R)in.correction * vn / (it.ima + it.imb);
-                if (it.da) for (int r = 0; r < 3; r++) pos_add(a, r, it.ima * dL * it.nv[r], true);
-                if (it.db) for (int r = 0; r < 3; r++) pos_add(b, r, it.imb * dL * it.nv[r], false);
+            const R vn = it.row[RowLayout<R>::dep] - (R)in.slop;
+            if (vn > R(0) && ima + imb > R(1e-12)) {
+                const R dL = (R)in.correction * vn / (ima + imb);
+                if (it.da) for (int r = 0; r < 3; r++) pos_add(a, r, ima * dL * nv[r], true);
+                if (it.db) for (int r = 0; r < 3; r++) pos_add(b, r, imb * dL * nv[r], false);
             }
         }
         if (it.s >= S) return;
-        R va[6], vb2[6];
-        for (int c = 0; c < 6; c++) { va[c] = VEL(c, a); vb2[c] = VEL(c, b); }
-        if (it.s < in.warm) {
-            // warm start (_solver_native.pyx:258-266)
-            const R n0 = IMP(0, ci), u0 = IMP(1, ci), w0 = IMP(2, ci);
-            const R P0 = n0 * it.nv[0] + u0 * it.t1[0] + w0 * it.t2[0];
-            const R P1 = n0 * it.nv[1] + u0 * it.t1[1] + w0 * it.t2[1];
-            const R P2 = n0 * it.nv[2] + u0 * it.t1[2] + w0 * it.t2[2];
-            apply_w(va, vb2, it.ima, it.imb, it.Ia, it.Ib, it.rA, it.rB, P0, P1, P2);
+        if constexpr (sizeof(R) == 8) {
+            // the reference's operation order (bit-exact FP64)
+            const R* rA = it.row + SR_RA;
+            const R* rB = it.row + SR_RB;
+            const R* t1 = it.row + SR_T1;
+            const R* t2 = it.row + SR_T2;
+            R va[6], vb2[6];
+            for (int c = 0; c < 6; c++) { va[c] = VEL(c, a); vb2[c] = VEL(c, b); }
+            if (it.s < in.warm) {
+                // warm start (_solver_native.pyx:258-266)
+                const R n0 = IMP(0, ci), u0 = IMP(1, ci), w0 = IMP(2, ci);
+                const R P0 = n0 * nv[0] + u0 * t1[0] + w0 * t2[0];
+                const R P1 = n0 * nv[1] + u0 * t1[1] + w0 * t2[1];
+                const R P2 = n0 * nv[2] + u0 * t1[2] + w0 * t2[2];
+                apply_w(va, vb2, ima, imb, it.Ia, it.Ib, rA, rB, P0, P1, P2);
+            } else {
+                // one sweep over the contact (_solver_native.pyx:268-295)
+                R nI = IMP(0, ci), tI0 = IMP(1, ci), tI1 = IMP(2, ci);
+                R dvx, dvy, dvz;
+                relvel(va, vb2, rA, rB, dvx, dvy, dvz);
+                R vn = dvx * nv[0] + dvy * nv[1] + dvz * nv[2];
+                R nw = nI - it.row[SR_NM] * (vn - it.row[SR_VB]);
+                if (nw < R(0)) nw = R(0);
+                R dL = nw - nI;
+                nI = nw;
+                apply_w(va, vb2, ima, imb, it.Ia, it.Ib, rA, rB, dL * nv[0], dL * nv[1], dL * nv[2]);
+                const R maxf = it.row[SR_FR] * nI;
+                relvel(va, vb2, rA, rB, dvx, dvy, dvz);
+                R vt = dvx * t1[0] + dvy * t1[1] + dvz * t1[2];
+                R nt = tI0 - it.row[SR_T1M] * vt;
+                if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
+                dL = nt - tI0; tI0 = nt;
+                apply_w(va, vb2, ima, imb, it.Ia, it.Ib, rA, rB, dL * t1[0], dL * t1[1], dL * t1[2]);
+                vt = dvx * t2[0] + dvy * t2[1] + dvz * t2[2];
+                nt = tI1 - it.row[SR_T2M] * vt;
+                if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
+                dL = nt - tI1; tI1 = nt;
+                apply_w(va, vb2, ima, imb, it.Ia, it.Ib, rA, rB, dL * t2[0], dL * t2[1], dL * t2[2]);
+                IMP(0, ci) = nI; IMP(1, ci) = tI0; IMP(2, ci) = tI1;
+            }
+            if (it.da) for (int c = 0; c < 6; c++) VEL(c, a) = va[c];
+            if (it.db) for (int c = 0; c < 6; c++) VEL(c, b) = vb2[c];
         } else {
-            // one sweep over the contact (_solver_native.pyx:268-295)
-            R nI = IMP(0, ci), tI0 = IMP(1, ci), tI1 = IMP(2, ci);
-            R dvx, dvy, dvz;
-            relvel(va, vb2, it.rA, it.rB, dvx, dvy, dvz);
-            R vn = dvx * it.nv[0] + dvy * it.nv[1] + dvz * it.nv[2];
-            R nw = nI - it.nm * (vn - it.vb);
-            if (nw < R(0)) nw = R(0);
-            R dL = nw - nI;
-            nI = nw;
-            apply_w(va, vb2, it.ima, it.imb, it.Ia, it.Ib, it.rA, it.rB, dL * it.nv[0], dL * it.nv[1], dL * it.nv[2]);
-            const R maxf = it.fr * nI;
-            relvel(va, vb2, it.rA, it.rB, dvx, dvy, dvz);
-            R vt = dvx * it.t1[0] + dvy * it.t1[1] + dvz * it.t1[2];
-            R nt = tI0 - it.t1m * vt;
-            if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
-            dL = nt - tI0; tI0 = nt;
-            apply_w(va, vb2, it.ima, it.imb, it.Ia, it.Ib, it.rA, it.rB, dL * it.t1[0], dL * it.t1[1], dL * it.t1[2]);
-            vt = dvx * it.t2[0] + dvy * it.t2[1] + dvz * it.t2[2];
-            nt = tI1 - it.t2m * vt;
-            if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
-            dL = nt - tI1; tI1 = nt;
-            apply_w(va, vb2, it.ima, it.imb, it.Ia, it.Ib, it.rA, it.rB, dL * it.t2[0], dL * it.t2[1], dL * it.t2[2]);
-            IMP(0, ci) = nI; IMP(1, ci) = tI0; IMP(2, ci) = tI1;
+            // FP32: the same updates on the folded Jacobian terms, with FMA
+            const R* row = it.row;
+            R va[6], vb2[6];
+            for (int c = 0; c < 6; c++) { va[c] = VEL(c, a); vb2[c] = VEL(c, b); }
+            // J_d . v = (vb - va) . d + wb . (rB x d) - wa . (rA x d)
+            auto jv = [&](int dd) -> R {
+                const R* e = row + FR_DIR + 3 * dd;
+                const R* ca = row + FR_CA + 3 * dd;
+                const R* cb = row + FR_CB + 3 * dd;
+                R x = (vb2[0] - va[0]) * e[0];
+                x = __builtin_fmaf(vb2[1] - va[1], e[1], x);
+                x = __builtin_fmaf(vb2[2] - va[2], e[2], x);
+                x = __builtin_fmaf(vb2[3], cb[0], x);
+                x = __builtin_fmaf(vb2[4], cb[1], x);
+                x = __builtin_fmaf(vb2[5], cb[2], x);
+                x = __builtin_fmaf(-va[3], ca[0], x);
+                x = __builtin_fmaf(-va[4], ca[1], x);
+                return __builtin_fmaf(-va[5], ca[2], x);
+            };
+            // impulse lam along direction dd
+            auto apply = [&](int dd, R lam) {
+                const R* e = row + FR_DIR + 3 * dd;
+                const R* ka = row + FR_KA + 3 * dd;
+                const R* kb = row + FR_KB + 3 * dd;
+                const R la = ima * lam, lb = imb * lam;
+                for (int r = 0; r < 3; r++) {
+                    va[r] = __builtin_fmaf(-la, e[r], va[r]);
+                    vb2[r] = __builtin_fmaf(lb, e[r], vb2[r]);
+                    va[3 + r] = __builtin_fmaf(-lam, ka[r], va[3 + r]);
+                    vb2[3 + r] = __builtin_fmaf(lam, kb[r], vb2[3 + r]);
+                }
+            };
+            if (it.s < in.warm) {
+                // warm start (_solver_native.pyx:258-266)
+                apply(0, IMP(0, ci));
+                apply(1, IMP(1, ci));
+                apply(2, IMP(2, ci));
+            } else {
+                // one sweep over the contact (_solver_native.pyx:268-295)
+                R nI = IMP(0, ci), tI0 = IMP(1, ci), tI1 = IMP(2, ci);
+                R nw = nI - row[FR_NM] * (jv(0) - row[FR_VB]);
+                if (nw < R(0)) nw = R(0);
+                apply(0, nw - nI);
+                nI = nw;
+                const R maxf = row[FR_FR] * nI;
+                const R vt1 = jv(1), vt2 = jv(2);   // both from the velocity after the normal impulse
+                R nt = tI0 - row[FR_T1M] * vt1;
+                if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
+                apply(1, nt - tI0);
+                tI0 = nt;
+                nt = tI1 - row[FR_T2M] * vt2;
+                if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
+                apply(2, nt - tI1);
+                tI1 = nt;
+                IMP(0, ci) = nI; IMP(1, ci) = tI0; IMP(2, ci) = tI1;
+            }
+            if (it.da) for (int c = 0; c < 6; c++) VEL(c, a) = va[c];
+            if (it.db) for (int c = 0; c < 6; c++) VEL(c, b) = vb2[c];
         }
-        if (it.da) for (int c = 0; c < 6; c++) VEL(c, a) = va[c];
-        if (it.db) for (int c = 0; c < 6; c++) VEL(c, b) = vb2[c];
     };
     auto bound_at = [&](int32_t l) -> int32_t {
         if (l >= D) return NC;
@@ -1228,6 +1353,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     extern __shared__ __align__(16) unsigned char s_dyn[];
     __shared__ int32_t s_scan[BLOCK / 64 + 2];
     __shared__ int32_t s_D;
+    __shared__ bool s_over;
     if ((int32_t)blockIdx.x >= *wcount) return;
     const int32_t w = wlist[blockIdx.x];
     const int tid = threadIdx.x;
@@ -1263,31 +1389,11 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     st.n = n; st.off = off; st.pd = nullptr; st.pos = nullptr;
     if (in_lds) { st.V = (R*)(s_dyn + lay.vel); st.vs = n; st.vb = off; }
     else { st.V = fp64 ? (R*)d.lv : (R*)d.svel; st.vs = d.ncap; st.vb = 0; }
-    if (imp_lds) { st.P = (R*)(s_dyn + lay.imp); st.ps = K; st.pb = cs; }
-    else { st.P = S_ + (size_t)SR_IMP * d.kcap; st.ps = d.kcap; st.pb = 0; }
+    if (imp_lds) { st.P = (R*)(s_dyn + lay.imp); st.ps = 1; st.pc = K; st.pb = cs; }
+    else { st.P = S_ + RowLayout<R>::imp; st.ps = RowLayout<R>::stride; st.pc = 1; st.pb = 0; }
     if (light) {
         if (fp64) st.pos = (double*)(s_dyn + lay.pos);
         else st.pd = (R*)s_dyn;
-    }
-    // stage velocities, positions, impulses
-    if (in_lds) {
-        R* V = (R*)(s_dyn + lay.vel);
-        for (int32_t i = tid; i < n; i += BLOCK)
-            for (int c = 0; c < 6; c++) V[(size_t)c * n + i] = (R)BODY(d.lv, c, off + i);
-    } else if (!fp64) {
-        R* V = (R*)d.svel;
-        for (int32_t i = tid; i < n; i += BLOCK)
-            for (int c = 0; c < 6; c++) V[(size_t)c * d.ncap + off + i] = (R)BODY(d.lv, c, off + i);
-    }
-    if (light && fp64) {
-        double* P = (double*)(s_dyn + lay.pos);
-        for (int32_t i = tid; i < n; i += BLOCK)
-            for (int c = 0; c < 3; c++) P[(size_t)c * n + i] = BODY(d.pos, c, off + i);
-    }
-    if (imp_lds) {
-        R* P = (R*)(s_dyn + lay.imp);
-        for (int32_t k = tid; k < K; k += BLOCK)
-            for (int c = 0; c < 3; c++) P[(size_t)c * K + k] = CON(S_, SR_IMP + c, cs + k);
     }
     // level table and masks; LDS level counters
     if (in_lds) {
@@ -1310,11 +1416,34 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
             D = assign_item_levels<true>(LevelTable<true>{d.last + off, d.lmask + off}, M, NI, mrec, level, s_bound,
                                          SOLVE_DCAP, over);
         }
-        if (tid == 0) {
-            s_D = over ? -D : D;
-            d.w_levels[w] = D;
-            atomicMax((unsigned long long*)&d.cnt[C_MAXLEVEL], (unsigned long long)D);
-        }
+        if (tid == 0) { s_D = D; s_over = over; }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        d.w_levels[w] = s_D;
+        atomicMax((unsigned long long*)&d.cnt[C_MAXLEVEL], (unsigned long long)s_D);
+        if (s_over) s_D = -s_D;
+    }
+    __syncthreads();
+    // stage velocities, positions, impulses
+    if (in_lds) {
+        R* V = (R*)(s_dyn + lay.vel);
+        for (int32_t i = tid; i < n; i += BLOCK)
+            for (int c = 0; c < 6; c++) V[(size_t)c * n + i] = (R)BODY(d.lv, c, off + i);
+    } else if (!fp64) {
+        R* V = (R*)d.svel;
+        for (int32_t i = tid; i < n; i += BLOCK)
+            for (int c = 0; c < 6; c++) V[(size_t)c * d.ncap + off + i] = (R)BODY(d.lv, c, off + i);
+    }
+    if (light && fp64) {
+        double* P = (double*)(s_dyn + lay.pos);
+        for (int32_t i = tid; i < n; i += BLOCK)
+            for (int c = 0; c < 3; c++) P[(size_t)c * n + i] = BODY(d.pos, c, off + i);
+    }
+    if (imp_lds) {
+        R* P = (R*)(s_dyn + lay.imp);
+        for (int32_t k = tid; k < K; k += BLOCK)
+            for (int c = 0; c < 3; c++) P[(size_t)c * K + k] = ROW(S_, cs + k, RowLayout<R>::imp + c);
     }
     __syncthreads();
     const bool lds_bounds = s_D > 0;
@@ -1347,7 +1476,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     for (int32_t k = tid; k < K; k += BLOCK) {
         const int64_t ci = cs + k;
         for (int r = 0; r < 3; r++)
-            CON(imp, r, ci) = imp_lds ? ((R*)(s_dyn + lay.imp))[(size_t)r * K + k] : CON(S_, SR_IMP + r, ci);
+            CON(imp, r, ci) = imp_lds ? ((R*)(s_dyn + lay.imp))[(size_t)r * K + k] : ROW(S_, ci, RowLayout<R>::imp + r);
     }
     // velocities back to the FP64 state: copied from LDS (FP64), or the FP32
     // solver's change folded in; positions from LDS
@@ -1511,7 +1640,7 @@ __global__ __launch_bounds__(BLOCK) void k_joints(Dev d, double dt) {
                 for (int r = 0; r < 3; r++) rr[r] = 0.0;
                 continue;
             }
-            for (int r = 0; r < 9; r++) I[r] = BODY(iiw, r, i);
+            for (int r = 0; r < 9; r++) I[r] = IIW(iiw, i, r);
             double q[4] = {BODY(d.quat, 0, i), BODY(d.quat, 1, i), BODY(d.quat, 2, i), BODY(d.quat, 3, i)}, Rm[9];
             quat_to_matrix(q, Rm);
             const double p[3] = {BODY(d.pos, 0, i), BODY(d.pos, 1, i), BODY(d.pos, 2, i)};
@@ -1590,7 +1719,7 @@ __global__ __launch_bounds__(BLOCK) void k_joints(Dev d, double dt) {
                     for (int r = 0; r < 3; r++) nv[r] = v[r] + im * Q[r];
                     const R cx = rr[1] * Q[2] - rr[2] * Q[1], cy = rr[2] * Q[0] - rr[0] * Q[2], cz = rr[0] * Q[1] - rr[1] * Q[0];
                     for (int r = 0; r < 3; r++)
-                        nv[3 + r] = v[3 + r] + ((BODY(iiw, 3 * r, i) * cx + BODY(iiw, 3 * r + 1, i) * cy) + BODY(iiw, 3 * r + 2, i) * cz);
+                        nv[3 + r] = v[3 + r] + ((IIW(iiw, i, 3 * r) * cx + IIW(iiw, i, 3 * r + 1) * cy) + IIW(iiw, i, 3 * r + 2) * cz);
                     for (int c = 0; c < 6; c++) {
                         if (fp64) BODY(d.lv, c, i) = (double)nv[c];
                         else BODY(d.lv, c, i) = BODY(d.lv, c, i) + (double)(nv[c] - v[c]);

@@ -34,6 +34,12 @@ enum : int32_t {
 
 // x-window candidates per broadphase tile (one wavefront task)
 constexpr int32_t SAP_TILE = 512;
+// slots of one contact's solver row (26 used) and of one body's world inverse
+// inertia (9 used): whole 16-byte chunks in FP32 and FP64
+constexpr int32_t SR_STRIDE = 28;
+constexpr int32_t IIW_STRIDE = 12;
+// bytes of one contact's solver row (FP64: 28 doubles; FP32: 56 floats)
+constexpr int32_t SOLVE_ROW_BYTES = 224;
 
 struct Dev {
     int64_t ncap, pcap, kcap, hcap;   // bodies, pairs, contacts, hash slots (pow2)
@@ -45,7 +51,7 @@ struct Dev {
     double *half, *inv_mass, *inv_inertia;
     double *gf, *ld, *ad, *qd, *timer;
     void *rot;                            // R[9][ncap] narrowphase rotation
-    void *iiw;                            // R[9][ncap] world inverse inertia
+    void *iiw;                            // R[ncap][IIW_STRIDE] world inverse inertia, row-major 3x3
     void *svel;                           // R[6][ncap] FP32 solver velocity staging
     int32_t *world;                       // world slot or -1
     int32_t *mat, *filt, *hoff, *hcnt;
@@ -86,7 +92,7 @@ struct Dev {
     int32_t *ca, *cb;
     double *cpt;                          // [3][kcap]
     void *cnrm, *cdep, *cimp;             // R [3][kcap], R [kcap], R [3][kcap] (n, t1, t2 impulses)
-    void *csolv;                          // R [26][kcap] solver rows
+    void *csolv;                          // R [kcap][SR_STRIDE] solver rows
     int4 *mrec;                           // [kcap] manifolds: first contact, contacts, dynamic a, dynamic b
     int32_t slmax;                        // passes per solve the schedule holds (iterations + warm start)
     int32_t *clevel, *corder, *lcount;    // [slmax * kcap] level schedule of the solver passes
