@@ -34,8 +34,9 @@ void launch_stage_boxbox(int precision, int32_t np, const double* ca, const doub
 void launch_stage_spheres(int precision, int32_t np, int32_t mode, const double* ca, const double* ra,
                           const double* cb, const double* Rb, const double* hb, const double* rb, int32_t* cnt,
                           double* nrm, double* pt, double* dep, hipStream_t s);
-void launch_solve(const Dev& d, int precision, const int32_t* blocks, SolveIn in, const int32_t* wl,
-                  const int32_t* wn, const int32_t* lds_n, const int32_t* live, bool stage, hipStream_t s);
+void launch_prepare(const Dev& d, int precision, SolveIn in, bool stage, hipStream_t s);
+void launch_solve_class(const Dev& d, int precision, int block, SolveIn in, const int32_t* wl, const int32_t* wn,
+                        int32_t lds_n, hipStream_t s);
 size_t epa_work_bytes();
 bool allow_dynamic_lds(int bytes);
 int32_t solve_lds_bodies(int precision, int64_t budget, int32_t cap);
@@ -109,6 +110,8 @@ struct omi_batch {
     omi_batch_config cfg{};
     Dev d{};
     hipStream_t stream = nullptr;
+    hipStream_t aux[3] = {nullptr, nullptr, nullptr};   // size-class branches of the solve (default priority)
+    hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
     std::vector<void*> allocs;
     int64_t bytes = 0;
     Ranges bodies, mats, filts, hulls, joints;
@@ -175,6 +178,11 @@ static void free_all(omi_batch* b) {
     for (auto e : b->ev) hipEventDestroy(e);
     for (void* p : b->allocs) hipFree(p);
     if (b->stream) hipStreamDestroy(b->stream);
+    for (int q = 0; q < 3; q++) {
+        if (b->aux[q]) hipStreamDestroy(b->aux[q]);
+        if (b->join[q]) hipEventDestroy(b->join[q]);
+    }
+    if (b->fork) hipEventDestroy(b->fork);
     b->allocs.clear();
 }
 
@@ -297,6 +305,11 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
         HIPCHK(hipMemcpy(d.world, m1.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
     }
     HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    for (int q = 0; q < 3; q++) {
+        HIPCHK(hipStreamCreateWithFlags(&b->aux[q], hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&b->join[q], hipEventDisableTiming));
+    }
+    HIPCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
     b->bodies.init(N);
     b->mats.init(TAB);
     b->filts.init(TAB);
@@ -561,6 +574,35 @@ static int32_t rebuild_lists(omi_batch* b) {
     return OMI_OK;
 }
 
+// Rows for every contact, then the size classes' solve launches: the largest
+// class on the batch stream, the others on branch streams forked and joined
+// with events, so a captured graph runs them side by side.
+static void enqueue_solve(omi_batch* b, SolveIn in, bool stage) {
+    const Dev& d = b->d;
+    hipStream_t s = b->stream;
+    const int prec = b->cfg.precision;
+    launch_prepare(d, prec, in, stage, s);
+    int top = -1;
+    for (int c = 3; c >= 0; c--)
+        if (b->live[c]) { top = c; break; }
+    if (top < 0) return;
+    int used[3] = {0, 0, 0}, q = 0;
+    hipEventRecord(b->fork, s);
+    for (int c = top; c >= 0; c--) {
+        if (!b->live[c]) continue;
+        hipStream_t sc = s;
+        if (c != top) {
+            sc = b->aux[q];
+            hipStreamWaitEvent(sc, b->fork, 0);
+        }
+        launch_solve_class(d, prec, b->solve_block[c], in, b->d_wl + (size_t)c * d.wcap, b->d_wn + c, b->lds_n[c],
+                           sc);
+        if (c != top) { hipEventRecord(b->join[q], sc); used[q] = 1; q++; }
+    }
+    for (int k = 0; k < 3; k++)
+        if (used[k]) hipStreamWaitEvent(s, b->join[k], 0);
+}
+
 static void enqueue_step(omi_batch* b, double dt) {
     const Dev& d = b->d;
     hipStream_t s = b->stream;
@@ -577,7 +619,8 @@ static void enqueue_step(omi_batch* b, double dt) {
     launch_compact(d, prec, b->cfg.record_broadphase, s); mark();
     launch_wake(d, s); mark();
     SolveIn in{nullptr, nullptr, 0.5, 0.8, 0.005, 1};
-    launch_solve(d, prec, b->solve_block, in, b->d_wl, b->d_wn, b->lds_n, b->live, false, s); mark();
+    enqueue_solve(b, in, false);
+    mark();
     launch_cache(d, prec, s); mark();
     if (b->any_joints) launch_joints(d, prec, dt, s);
     mark();
@@ -1073,7 +1116,7 @@ extern "C" int32_t omi_prepare_and_solve(int64_t K, int32_t N, const int64_t* a_
         b->slots[0].live = true;
         b->slots[0].n = N;
         if (!(rc = rebuild_lists(b)))
-            launch_solve(d, precision, b->solve_block, in, b->d_wl, b->d_wn, b->lds_n, b->live, true, b->stream);
+            enqueue_solve(b, in, true);
         if (!rc && hipGetLastError() != hipSuccess) rc = fail(OMI_ERR_HIP, "solver launch failed");
         if (!rc) rc = download_soa(b, pos, d.pos, d.ncap, 0, N, 3);
         if (!rc) rc = download_soa(b, lv, d.lv, d.ncap, 0, N, 3);

@@ -2084,10 +2084,8 @@ int32_t solve_lds_bodies(int precision, int64_t budget, int32_t cap) {
     return lo;
 }
 
-// Solver rows for every contact, then one block per world by size class:
-// worlds of at most lds_n[c] bodies keep their schedule state in LDS.
-void launch_solve(const Dev& d, int precision, const int32_t* blocks, SolveIn in, const int32_t* wl,
-                  const int32_t* wn, const int32_t* lds_n, const int32_t* live, bool stage, hipStream_t s) {
+// Solver rows for every contact (one thread per contact).
+void launch_prepare(const Dev& d, int precision, SolveIn in, bool stage, hipStream_t s) {
     const int g = GRID_FOR(d.kcap, 256);
     if (precision == 64) {
         if (stage) hipLaunchKernelGGL((k_prepare<double, true>), dim3(g), dim3(256), 0, s, d, in);
@@ -2096,12 +2094,14 @@ void launch_solve(const Dev& d, int precision, const int32_t* blocks, SolveIn in
         if (stage) hipLaunchKernelGGL((k_prepare<float, true>), dim3(g), dim3(256), 0, s, d, in);
         else hipLaunchKernelGGL((k_prepare<float, false>), dim3(g), dim3(256), 0, s, d, in);
     }
-    for (int c = 0; c < 4; c++) {
-        if (!live[c]) continue;
-        const int32_t* l = wl + (size_t)c * d.wcap;
-        if (precision == 64) solve_block<double>(d, blocks[c], in, l, wn + c, lds_n[c], s);
-        else solve_block<float>(d, blocks[c], in, l, wn + c, lds_n[c], s);
-    }
+}
+
+// One block per world of size class c (world list wl, count *wn); worlds of
+// at most lds_n bodies keep their schedule state in LDS.
+void launch_solve_class(const Dev& d, int precision, int block, SolveIn in, const int32_t* wl, const int32_t* wn,
+                        int32_t lds_n, hipStream_t s) {
+    if (precision == 64) solve_block<double>(d, block, in, wl, wn, lds_n, s);
+    else solve_block<float>(d, block, in, wl, wn, lds_n, s);
 }
 
 void launch_cache(const Dev& d, int precision, hipStream_t s) {
