@@ -213,6 +213,10 @@ int32_t omi_batch_kernel_times(omi_batch *b, int32_t n, double *ms);
 const char *omi_batch_kernel_names(void);
 /* Capture the step in a HIP graph (1) or launch kernels eagerly (0). */
 int32_t omi_batch_use_graph(omi_batch *b, int32_t on);
+/* Threads per world of the broadphase sort and of the solver for each size
+ * class (as in omi_batch_config; 0 keeps the current value).  Batch-layer
+ * tuning knob with no counterpart in the reference; the autotuner uses it. */
+int32_t omi_batch_set_blocks(omi_batch *b, const int32_t *sap_block, const int32_t *solve_block);
 /* Device bytes held by the batch. */
 int64_t omi_batch_device_bytes(const omi_batch *b);
 

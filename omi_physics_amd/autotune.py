@@ -1,11 +1,14 @@
 """Autotuner for the per-world launches (world-to-workgroup packing).
 
 Worlds are launched by size class (<= 256, <= 1024, <= 4096 bodies, larger):
-the broadphase and the solver run one workgroup per world, and the best
-workgroup size depends on how many bodies, candidates and contacts per level a
-world of that class has.  For each class present in a workload, the tuner
-steps a sample of that class's worlds with every candidate size, times the
-``sap`` and ``solve`` kernels with HIP events, and keeps the fastest.
+the broadphase sort and the solver run one workgroup per world, the classes
+side by side.  The best workgroup size of a class depends on its worlds'
+bodies, contacts per level and on how many of them share a CU, so the tuner
+times the whole step on the whole workload: after the worlds have settled
+into contact it tries every candidate size for one class and kernel at a
+time (coordinate descent), interleaving the candidates over a few rounds so
+the slowly changing contact state weighs on all of them alike, and keeps the
+fastest.
 
 The result is a small JSON table (``tuned.json`` beside this file is the one
 measured for the 4096-world workload on MI355X) that ``Batch`` takes as
@@ -40,48 +43,44 @@ def load_tuned(path: str = TUNED) -> Dict[str, List[int]]:
         with open(path) as f:
             t = json.load(f)
         return {'sap_block': list(t['sap_block']), 'solve_block': list(t['solve_block'])}
-    return {'sap_block': [64, 128, 256, 1024], 'solve_block': [64, 128, 256, 256]}
+    return {'sap_block': [64, 128, 1024, 1024], 'solve_block': [64, 128, 256, 256]}
 
 
-def tune(worlds: Sequence[dict], precision: str = 'fp32', dt: float = 1.0 / 60.0, sample: int = 192,
-         warm_steps: int = 90, timed_steps: int = 20, candidates: Sequence[int] = CANDIDATES,
-         seed: int = 0, log=print) -> Dict[str, object]:
+def tune(worlds: Sequence[dict], precision: str = 'fp32', dt: float = 1.0 / 60.0, warm_steps: int = 120,
+         timed_steps: int = 6, rounds: int = 3, candidates: Sequence[int] = CANDIDATES,
+         log=print) -> Dict[str, object]:
     """Pick per-class block sizes for ``worlds`` (list of ``export_world_arrays`` dicts)."""
     from .batch import batch_from_arrays
-    rng = np.random.default_rng(seed)
-    by_class: Dict[int, List[int]] = {}
-    for i, w in enumerate(worlds):
-        by_class.setdefault(size_class(int(w['n'])), []).append(i)
+    classes = sorted({size_class(int(w['n'])) for w in worlds})
     best = load_tuned()
+    b = batch_from_arrays(worlds, precision=precision, sap_block=best['sap_block'], solve_block=best['solve_block'])
+    b.prepare(dt)
+    b.step(dt, warm_steps)          # let the worlds settle into contact
+    b.sync()
     report = {}
-    for c, idx in sorted(by_class.items()):
-        pick = rng.choice(idx, size=min(sample, len(idx)), replace=False)
-        arrs = [worlds[i] for i in pick]
-        times = {}
-        for B in candidates:
-            sap = [best['sap_block'][k] if k != c else B for k in range(4)]
-            sol = [best['solve_block'][k] if k != c else B for k in range(4)]
-            b = batch_from_arrays(arrs, precision=precision, sap_block=sap, solve_block=sol)
-            b.prepare(dt)
-            b.step(dt, warm_steps)          # let the worlds settle into contact
-            b.sync()
-            b.set_timing(True)
-            tot = {'sap': 0.0, 'solve': 0.0}
-            for _ in range(timed_steps):
-                b.step(dt, 1)
-                b.sync()
-                kt = b.kernel_times()
-                tot['sap'] += kt['sap']
-                tot['solve'] += kt['solve']
-            b.close()
-            times[B] = {k: v / timed_steps for k, v in tot.items()}
-            log('class %d (%d worlds) block %4d: sap %.3f ms  solve %.3f ms'
-                % (c, len(arrs), B, times[B]['sap'], times[B]['solve']))
-        best['sap_block'][c] = int(min(times, key=lambda B: times[B]['sap']))
-        best['solve_block'][c] = int(min(times, key=lambda B: times[B]['solve']))
-        report[str(c)] = {str(B): t for B, t in times.items()}
+    for c in classes:
+        for kind in ('sap_block', 'solve_block'):
+            times = {B: 0.0 for B in candidates}
+            for _ in range(rounds):
+                for B in candidates:
+                    cfg = {k: list(v) for k, v in best.items()}
+                    cfg[kind][c] = B
+                    b.set_blocks(cfg['sap_block'], cfg['solve_block'])
+                    b.prepare(dt)
+                    b.sync()
+                    t = time.perf_counter()
+                    b.step(dt, timed_steps)
+                    b.sync()
+                    times[B] += (time.perf_counter() - t) / (rounds * timed_steps)
+            pick = int(min(times, key=times.get))
+            best[kind][c] = pick
+            report['%s[%d]' % (kind, c)] = {str(B): round(v * 1e3, 4) for B, v in times.items()}
+            log('class %d %s: %s ms/step -> %d' % (c, kind, ' '.join('%d:%.3f' % (B, v * 1e3)
+                                                                      for B, v in times.items()), pick))
+    b.close()
     return {'sap_block': best['sap_block'], 'solve_block': best['solve_block'], 'precision': precision,
-            'measured': report, 'when': time.strftime('%Y-%m-%d %H:%M:%S')}
+            'workload_worlds': len(worlds), 'measured_ms_per_step': report,
+            'when': time.strftime('%Y-%m-%d %H:%M:%S')}
 
 
 def save(result: Dict[str, object], path: str = TUNED) -> None:

@@ -123,6 +123,12 @@ class Batch:
     def use_graph(self, on: bool) -> None:
         check(self.lib.omi_batch_use_graph(self._h, int(bool(on))))
 
+    def set_blocks(self, sap_block: Sequence[int], solve_block: Sequence[int]) -> None:
+        """Threads per world of the broadphase sort / solver per size class (0 keeps a value)."""
+        sb = np.ascontiguousarray(sap_block, dtype=np.int32)
+        vb = np.ascontiguousarray(solve_block, dtype=np.int32)
+        check(self.lib.omi_batch_set_blocks(self._h, ptr(sb, _i32), ptr(vb, _i32)))
+
     def set_timing(self, on: bool) -> None:
         check(self.lib.omi_batch_set_timing(self._h, int(bool(on))))
 

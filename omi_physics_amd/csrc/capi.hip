@@ -18,8 +18,9 @@ namespace omi {
 void launch_begin(const Dev& d, hipStream_t s);
 void launch_pre(const Dev& d, int precision, int64_t nhw, double dt, double margin, hipStream_t s);
 void launch_refit(const Dev& d, int64_t nhw, double margin, hipStream_t s);
-void launch_sap(const Dev& d, const int32_t* wl, const int32_t* wn, const int32_t* np_class, const int32_t* blocks,
-                const int32_t* live, int32_t record, int32_t grid, hipStream_t s);
+void launch_sap_sort(const Dev& d, int c, const int32_t* wl, const int32_t* wn, int32_t np, int32_t block,
+                     hipStream_t s);
+void launch_sap_pairs(const Dev& d, int32_t record, int32_t grid, hipStream_t s);
 void launch_narrow(const Dev& d, int precision, hipStream_t s);
 void launch_compact(const Dev& d, int precision, int32_t record, hipStream_t s);
 void launch_wake(const Dev& d, hipStream_t s);
@@ -122,7 +123,7 @@ struct omi_batch {
     int32_t np_class[4] = {256, 1024, 4096, 0};
     int32_t lds_n[4] = {0, 0, 0, 0};        // solver: largest world of the class kept wholly in LDS
     int32_t live[4] = {0, 0, 0, 0};        // classes with a launch in the step
-    int32_t sap_block[4] = {64, 128, 256, 1024};
+    int32_t sap_block[4] = {64, 128, 1024, 1024};
     int32_t solve_block[4] = {64, 128, 256, 256};
     int32_t np_lds_max = 4096;
     int32_t sap_grid = 2048;               // blocks of the chip-wide broadphase pair passes
@@ -285,7 +286,7 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     for (int q = 0; q < 2; q++) {
         unsigned char* ki;
         A(d.ka[q], K); A(d.kb[q], K); A(d.kpt[q], 3 * K); A(ki, 3 * K * (int64_t)rs); d.kimp[q] = ki;
-        A(d.hkey[q], H); A(d.hval[q], H);
+        A(d.hkey[q], H); A(d.hval[q], H); A(d.hslot[q], K);
         HIPCHK(hipMemset(d.hkey[q], 0xff, sizeof(uint64_t) * H));
     }
     const int64_t JC = std::max<int64_t>(cfg->max_joints, 1);
@@ -296,7 +297,7 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     d.jrow = jr;
     A(d.jlevel, JC); A(d.jorder, JC); A(d.jcount, JC);
     A(d.w_js, W); A(d.w_jn, W); A(d.w_jiters, W);
-    A(d.sel, 1); A(d.cnt, C_COUNT);
+    A(d.sel, 1); A(d.cnt, C_COUNT); A(d.hslot_n, 2);
     A(b->d_wl, 4 * W); A(b->d_wn, 4);
 #undef A
     // every body slot starts free (world = -1)
@@ -574,14 +575,12 @@ static int32_t rebuild_lists(omi_batch* b) {
     return OMI_OK;
 }
 
-// Rows for every contact, then the size classes' solve launches: the largest
-// class on the batch stream, the others on branch streams forked and joined
-// with events, so a captured graph runs them side by side.
-static void enqueue_solve(omi_batch* b, SolveIn in, bool stage) {
-    const Dev& d = b->d;
+// Launches of the size classes c in `live`, largest first: the largest on the
+// batch stream, the others on branch streams forked and joined with events,
+// so a captured graph runs them side by side.
+template <typename F>
+static void fork_classes(omi_batch* b, F launch_class) {
     hipStream_t s = b->stream;
-    const int prec = b->cfg.precision;
-    launch_prepare(d, prec, in, stage, s);
     int top = -1;
     for (int c = 3; c >= 0; c--)
         if (b->live[c]) { top = c; break; }
@@ -595,12 +594,34 @@ static void enqueue_solve(omi_batch* b, SolveIn in, bool stage) {
             sc = b->aux[q];
             hipStreamWaitEvent(sc, b->fork, 0);
         }
-        launch_solve_class(d, prec, b->solve_block[c], in, b->d_wl + (size_t)c * d.wcap, b->d_wn + c, b->lds_n[c],
-                           sc);
+        launch_class(c, sc);
         if (c != top) { hipEventRecord(b->join[q], sc); used[q] = 1; q++; }
     }
     for (int k = 0; k < 3; k++)
         if (used[k]) hipStreamWaitEvent(s, b->join[k], 0);
+}
+
+// Broadphase: the size classes' sorts side by side, then the chip-wide passes.
+static void enqueue_sap(omi_batch* b, int32_t record) {
+    const Dev& d = b->d;
+    fork_classes(b, [&](int c, hipStream_t sc) {
+        launch_sap_sort(d, c, b->d_wl + (size_t)c * d.wcap, b->d_wn + c, b->np_class[c], b->sap_block[c], sc);
+    });
+    launch_sap_pairs(d, record, b->sap_grid, b->stream);
+}
+
+// Rows for every contact, then the size classes' solve launches: the largest
+// class on the batch stream, the others on branch streams forked and joined
+// with events, so a captured graph runs them side by side.
+static void enqueue_solve(omi_batch* b, SolveIn in, bool stage) {
+    const Dev& d = b->d;
+    hipStream_t s = b->stream;
+    const int prec = b->cfg.precision;
+    launch_prepare(d, prec, in, stage, s);
+    fork_classes(b, [&](int c, hipStream_t sc) {
+        launch_solve_class(d, prec, b->solve_block[c], in, b->d_wl + (size_t)c * d.wcap, b->d_wn + c, b->lds_n[c],
+                           sc);
+    });
 }
 
 static void enqueue_step(omi_batch* b, double dt) {
@@ -613,7 +634,7 @@ static void enqueue_step(omi_batch* b, double dt) {
     mark();
     launch_begin(d, s); mark();
     launch_pre(d, prec, b->nhw, dt, 0.05, s); mark();
-    launch_sap(d, b->d_wl, b->d_wn, b->np_class, b->sap_block, b->live, b->cfg.record_broadphase, b->sap_grid, s);
+    enqueue_sap(b, b->cfg.record_broadphase);
     mark();
     launch_narrow(d, prec, s); mark();
     launch_compact(d, prec, b->cfg.record_broadphase, s); mark();
@@ -651,6 +672,20 @@ extern "C" int32_t omi_batch_kernel_times(omi_batch* b, int32_t n, double* ms) {
         HIPCHK(hipEventElapsedTime(&t, b->ev[i], b->ev[i + 1]));
         ms[i] = t;
     }
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_set_blocks(omi_batch* b, const int32_t* sap_block, const int32_t* solve_block) {
+    if (!b || !sap_block || !solve_block) return fail(OMI_ERR_ARG, "null argument");
+    auto okb = [](int32_t x) { return x == 0 || x == 64 || x == 128 || x == 256 || x == 512 || x == 1024; };
+    for (int c = 0; c < 4; c++)
+        if (!okb(sap_block[c]) || !okb(solve_block[c]))
+            return fail(OMI_ERR_ARG, "block sizes must be 0, 64, 128, 256, 512 or 1024");
+    for (int c = 0; c < 4; c++) {
+        if (sap_block[c]) b->sap_block[c] = sap_block[c];
+        if (solve_block[c]) b->solve_block[c] = solve_block[c];
+    }
+    invalidate_graph(b);
     return OMI_OK;
 }
 
@@ -1007,7 +1042,7 @@ extern "C" int32_t omi_sap_pairs(int32_t n, const double* lo, const double* hi, 
         b->slots[0].n = n;
         if ((rc = rebuild_lists(b))) { omi_batch_destroy(b); return rc; }
         launch_begin(d, b->stream);
-        launch_sap(d, b->d_wl, b->d_wn, b->np_class, b->sap_block, b->live, 1, b->sap_grid, b->stream);
+        enqueue_sap(b, 1);
         if (hipGetLastError() != hipSuccess) { omi_batch_destroy(b); return fail(OMI_ERR_HIP, "broadphase launch failed"); }
         uint64_t c[C_COUNT];
         hipStreamSynchronize(b->stream);

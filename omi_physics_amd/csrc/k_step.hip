@@ -1734,9 +1734,15 @@ __global__ __launch_bounds__(BLOCK) void k_joints(Dev d, double dt) {
 
 // ------------------------------------------------------- warm-start cache --
 
+// Empty the slots of the buffer this step writes: each cache entry of the
+// step that last filled it (two steps ago) recorded its slot, or -1.
 __global__ void k_hash_clear(Dev d) {
     const int32_t nw = *d.sel ^ 1;
-    for (int64_t i = gtid(); i < d.hcap; i += gstride()) d.hkey[nw][i] = ~0ull;
+    const int32_t n = d.hslot_n[nw];
+    for (int64_t i = gtid(); i < n; i += gstride()) {
+        const int32_t h = d.hslot[nw][i];
+        if (h >= 0) d.hkey[nw][h] = ~0ull;
+    }
 }
 
 // The cache after this step (solver.py:271-278): this step's contacts with
@@ -1774,21 +1780,31 @@ __global__ __launch_bounds__(BLOCK) void k_cache(Dev d) {
         const int32_t a = sa[s], b = sb[s];
         d.ka[nw][t] = a; d.kb[nw][t] = b;
         for (int r = 0; r < 3; r++) { CON(d.kpt[nw], r, t) = CON(sp, r, s); CON(di, r, t) = CON(si, r, s); }
+        int32_t slot = -1;
         if (k == 0 || sa[s - 1] != a || sb[s - 1] != b) {
             const uint64_t key = ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
             const int64_t hm = d.hcap - 1;
             int64_t h = (int64_t)(hash_mix(key) & (uint64_t)hm);
             for (int64_t probe = 0; probe < d.hcap; probe++) {
                 unsigned long long prev = atomicCAS((unsigned long long*)&d.hkey[nw][h], ~0ull, (unsigned long long)key);
+                if (prev == ~0ull) slot = (int32_t)h;
                 if (prev == ~0ull || prev == key) { d.hval[nw][h] = (int32_t)t; break; }
                 h = (h + 1) & hm;
             }
         }
+        d.hslot[nw][t] = slot;
     }
 }
 
+// End of step: the cache buffer just written keeps its entry count (for
+// emptying its hash slots before its next use) and becomes the current one.
 __global__ void k_toggle(Dev d) {
-    if (gtid() == 0) *d.sel ^= 1;
+    if (gtid() == 0) {
+        const int32_t nw = *d.sel ^ 1;
+        const uint64_t c = d.cnt[C_CACHE];
+        d.hslot_n[nw] = (int32_t)(c < (uint64_t)d.kcap ? c : (uint64_t)d.kcap);
+        *d.sel = nw;
+    }
 }
 
 // ------------------------------------------------------------- integrate --
@@ -1978,21 +1994,21 @@ static void sap_sort_one(const Dev& d, const int32_t* wl, const int32_t* wn, int
     else hipLaunchKernelGGL((k_sap_sort<B, false>), dim3(d.wcap), dim3(B), (size_t)np * 16, s, d, wl, wn, np);
 }
 
-void launch_sap(const Dev& d, const int32_t* wl, const int32_t* wn, const int32_t* np_class, const int32_t* blocks,
-                const int32_t* live, int32_t record, int32_t grid, hipStream_t s) {
-    for (int c = 0; c < 4; c++) {
-        if (!live[c]) continue;
-        const bool global = c == 3;
-        const int32_t* l = wl + (size_t)c * d.wcap;
-        const int32_t* n = wn + c;
-        switch (blocks[c]) {
-            case 64: sap_sort_one<64>(d, l, n, np_class[c], global, s); break;
-            case 128: sap_sort_one<128>(d, l, n, np_class[c], global, s); break;
-            case 512: sap_sort_one<512>(d, l, n, np_class[c], global, s); break;
-            case 1024: sap_sort_one<1024>(d, l, n, np_class[c], global, s); break;
-            default: sap_sort_one<256>(d, l, n, np_class[c], global, s); break;
-        }
+// The broadphase sort of size class c (LDS for c < 3, global scratch above).
+void launch_sap_sort(const Dev& d, int c, const int32_t* wl, const int32_t* wn, int32_t np, int32_t block,
+                     hipStream_t s) {
+    const bool global = c == 3;
+    switch (block) {
+        case 64: sap_sort_one<64>(d, wl, wn, np, global, s); break;
+        case 128: sap_sort_one<128>(d, wl, wn, np, global, s); break;
+        case 512: sap_sort_one<512>(d, wl, wn, np, global, s); break;
+        case 1024: sap_sort_one<1024>(d, wl, wn, np, global, s); break;
+        default: sap_sort_one<256>(d, wl, wn, np, global, s); break;
     }
+}
+
+// The chip-wide candidate passes after every class is sorted.
+void launch_sap_pairs(const Dev& d, int32_t record, int32_t grid, hipStream_t s) {
     hipLaunchKernelGGL((k_sap_pairs<256, false>), dim3(grid), dim3(256), 0, s, d, record);
     hipLaunchKernelGGL((k_sap_scan<1024>), dim3(1), dim3(1024), 0, s, d, record);
     hipLaunchKernelGGL((k_sap_pairs<256, true>), dim3(grid), dim3(256), 0, s, d, record);
@@ -2105,7 +2121,7 @@ void launch_solve_class(const Dev& d, int precision, int block, SolveIn in, cons
 }
 
 void launch_cache(const Dev& d, int precision, hipStream_t s) {
-    hipLaunchKernelGGL(k_hash_clear, dim3(GRID_FOR(d.hcap, 256)), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_hash_clear, dim3(GRID_FOR(d.kcap, 256)), dim3(256), 0, s, d);
     if (precision == 64) hipLaunchKernelGGL((k_cache<double, 256>), dim3(d.wcap), dim3(256), 0, s, d);
     else hipLaunchKernelGGL((k_cache<float, 256>), dim3(d.wcap), dim3(256), 0, s, d);
 }

@@ -103,6 +103,8 @@ struct Dev {
     void *kimp[2];                        // R [3][kcap]
     uint64_t *hkey[2];
     int32_t *hval[2];
+    int32_t *hslot[2];                    // [kcap] hash slot each cache entry filled, or -1
+    int32_t *hslot_n;                     // [2] cache entries of each buffer's last fill
     // point joints
     int64_t jcap;
     int32_t *ja, *jb;                     // [jcap] body index or -1 (the world)
