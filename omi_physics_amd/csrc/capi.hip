@@ -37,10 +37,9 @@ void launch_stage_spheres(int precision, int32_t np, int32_t mode, const double*
                           double* nrm, double* pt, double* dep, hipStream_t s);
 void launch_prepare(const Dev& d, int precision, SolveIn in, bool stage, hipStream_t s);
 void launch_solve_class(const Dev& d, int precision, int block, SolveIn in, const int32_t* wl, const int32_t* wn,
-                        int32_t lds_n, hipStream_t s);
+                        int32_t lds_bytes, hipStream_t s);
 size_t epa_work_bytes();
 bool allow_dynamic_lds(int bytes);
-int32_t solve_lds_bodies(int precision, int64_t budget, int32_t cap);
 void launch_stage_convex(int32_t np, const int32_t* ka, const double* ca, const double* Ra, const double* ha,
                          const int32_t* oa, const int32_t* na, const int32_t* kb, const double* cb, const double* Rb,
                          const double* hb, const int32_t* ob, const int32_t* nb, const double* hull, void* work,
@@ -121,7 +120,7 @@ struct omi_batch {
     int32_t* d_wl = nullptr;               // [4][max_worlds] world slots per size class
     int32_t* d_wn = nullptr;               // [4] class counts
     int32_t np_class[4] = {256, 1024, 4096, 0};
-    int32_t lds_n[4] = {0, 0, 0, 0};        // solver: largest world of the class kept wholly in LDS
+    int32_t lds_bytes[4] = {0, 0, 0, 0};    // solver: dynamic LDS per world of each size class
     int32_t live[4] = {0, 0, 0, 0};        // classes with a launch in the step
     int32_t sap_block[4] = {64, 128, 1024, 1024};
     int32_t solve_block[4] = {64, 128, 256, 256};
@@ -221,9 +220,15 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     if (dyn_lds > 65536 - 1024 && !allow_dynamic_lds(dyn_lds)) dyn_lds = 65536 - 1024;
     b->np_lds_max = dyn_lds >= 4096 * 16 ? 4096 : 2048;
     b->np_class[2] = b->np_lds_max;
-    // the solver keeps a world of the class in LDS when it fits the budget
-    for (int c = 0; c < 4; c++)
-        b->lds_n[c] = c < 3 ? solve_lds_bodies(cfg->precision, (int64_t)dyn_lds - 1024, b->np_class[c]) : 0;
+    // the solver's LDS per world of a size class: small worlds take less so
+    // that more of them share a CU (a world that does not fit keeps its
+    // solver state in global memory)
+    {
+        const int32_t full = (dyn_lds - 1024) & ~15;
+        const bool f64 = cfg->precision == OMI_FP64;
+        const int32_t want[4] = {f64 ? 40 << 10 : 24 << 10, f64 ? 80 << 10 : 64 << 10, full, full};
+        for (int c = 0; c < 4; c++) b->lds_bytes[c] = std::min(want[c], full);
+    }
     b->solver_block = cfg->solver_block > 0 ? cfg->solver_block : 256;
     auto okb = [](int32_t x) { return x == 64 || x == 128 || x == 256 || x == 512 || x == 1024; };
     for (int c = 0; c < 4; c++) {
@@ -619,8 +624,8 @@ static void enqueue_solve(omi_batch* b, SolveIn in, bool stage) {
     const int prec = b->cfg.precision;
     launch_prepare(d, prec, in, stage, s);
     fork_classes(b, [&](int c, hipStream_t sc) {
-        launch_solve_class(d, prec, b->solve_block[c], in, b->d_wl + (size_t)c * d.wcap, b->d_wn + c, b->lds_n[c],
-                           sc);
+        launch_solve_class(d, prec, b->solve_block[c], in, b->d_wl + (size_t)c * d.wcap, b->d_wn + c,
+                           b->lds_bytes[c], sc);
     });
 }
 

@@ -905,14 +905,15 @@ struct LevelTable {
 
 // Levels of manifold items [0, NI) (item i is manifold i mod M of pass
 // i / M) into level[] (the first contact's level); with `lcnt` (LDS, zeroed)
-// also the number of contacts on each level below `cap` (*over is set when a
-// level reaches cap).  Returns the depth.  Called by the 64 threads of one
-// wavefront; the table starts at zero.
+// also the number of contacts on each level below `cap` (`over` is set when a
+// level reaches cap).  Returns the depth; d0 gets the deepest level holding a
+// pass-0 contact.  Called by the 64 threads of one wavefront; the table starts
+// at zero.
 template <bool G>
 __device__ int32_t assign_item_levels(const LevelTable<G>& T, int32_t M, int32_t NI, const int4* mrec,
-                                      int32_t* level, int32_t* lcnt, int32_t cap, bool& over) {
+                                      int32_t* level, int32_t* lcnt, int32_t cap, bool& over, int32_t& d0) {
     const int lane = threadIdx.x & 63;
-    int32_t D = 0;
+    int32_t D = 0, Dp = 0;
     int32_t m = lane;
     while (m >= M) m -= M;
     int4 rec = make_int4(0, 1, 0, 0);
@@ -958,6 +959,7 @@ __device__ int32_t assign_item_levels(const LevelTable<G>& T, int32_t M, int32_t
         if (valid) {
             level[base + lane] = L;
             D = Le > D ? Le : D;
+            if (base + lane < M) Dp = Le > Dp ? Le : Dp;
             if (lcnt) {
                 if (Le <= cap) for (int32_t j = 0; j < c; j++) atomicAdd(&lcnt[L - 1 + j], 1);
                 else over = true;
@@ -965,8 +967,12 @@ __device__ int32_t assign_item_levels(const LevelTable<G>& T, int32_t M, int32_t
         }
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) { int32_t x = __shfl_xor(D, o, 64); D = x > D ? x : D; }
+    for (int o = 32; o > 0; o >>= 1) {
+        int32_t x = __shfl_xor(D, o, 64); D = x > D ? x : D;
+        x = __shfl_xor(Dp, o, 64); Dp = x > Dp ? x : Dp;
+    }
     over = __any(over);
+    d0 = Dp;
     return D;
 }
 
@@ -981,7 +987,7 @@ __device__ __forceinline__ int32_t item_code(int32_t k, bool da, bool db, int32_
 template <typename R>
 struct SolveItem {
     int64_t ci, a, b;
-    int32_t s;
+    int32_t s, cida, cidb;          // pass; compact ids of the sides (LDS state)
     bool da, db;
     R row[RowLayout<R>::stride];
     R Ia[sizeof(R) == 8 ? IIW_STRIDE : 1], Ib[sizeof(R) == 8 ? IIW_STRIDE : 1];   // FP64 rows only
@@ -1087,31 +1093,22 @@ __global__ void k_prepare(Dev d, SolveIn in) {
 // Level bounds kept in LDS per world (deeper schedules use global memory).
 constexpr int32_t SOLVE_DCAP = 1024;
 
-// LDS layout of a solve launch that keeps worlds of up to lds_n bodies in
-// LDS (byte offsets): level table and body masks (12 B per body; reused for
-// FP32 position corrections), velocities (6 R), impulses of up to 3 contacts
-// per body (9 R), FP64 positions (3 doubles, FP64 only), level bounds.
-struct SolveLds {
-    size_t vel, imp, pos, bound, total;
-};
-__host__ __device__ inline SolveLds solve_lds_layout(int32_t lds_n, size_t rsize) {
-    SolveLds L;
-    L.vel = (((size_t)lds_n * 12 + 15) & ~(size_t)15);
-    L.imp = L.vel + (size_t)lds_n * 6 * rsize;
-    L.pos = L.imp + (size_t)lds_n * 9 * rsize;
-    L.bound = L.pos + (rsize == 8 ? (size_t)lds_n * 24 : 0);
-    L.total = L.bound + 4 * (size_t)SOLVE_DCAP;
-    return L;
-}
+__device__ __forceinline__ size_t al16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-// Where one world's solver state lives during the level loop.
+// Where one world's solver state lives during the level loop.  In LDS
+// (byte offsets into the launch's dynamic LDS): the compact id of every body in
+// contact (map, world-local index -> id), their velocities (6 R per id), the
+// contacts' impulses (3 R per contact), and positions (FP64) or position
+// corrections (FP32) of the bodies in contact (3 R per id); otherwise the
+// velocities, impulses and positions of the batch in global memory.
 template <typename R>
 struct SolveState {
-    R* V; int64_t vs, vb;          // velocities: V[c * vs + (i - vb)]
-    R* P; int64_t ps, pc, pb;      // impulses: P[(ci - pb) * ps + c * pc]
-    R* pd;                         // FP32 position corrections in LDS [3][n], or null
-    double* pos;                   // FP64 positions in LDS [3][n], or null
-    int64_t n, off;
+    size_t map, vel, imp, pos;     // LDS offsets
+    int32_t nd;                    // bodies in contact (compact ids)
+    bool pos_lds;
+    R* V; int64_t vs;              // global velocities: V[c * vs + i]
+    R* P; int64_t ps, pc, pb;      // global impulses: P[(ci - pb) * ps + c * pc]
+    int64_t off;
 };
 
 // Bucket the contact items by level: bound[l] is the start of level l on
@@ -1138,24 +1135,26 @@ __device__ void scatter_items(int32_t* bound, int32_t SL, int32_t M, const int4*
 // runs.  LIGHT: velocities, positions and impulses are in LDS and a level
 // ends on an LDS barrier; otherwise on a full barrier.
 template <typename R, int BLOCK, bool LIGHT>
-__device__ void run_levels(const Dev& d, const SolveIn& in, SolveState<R> st, const SolveLds& lay, int32_t cs,
-                           int32_t D, int32_t NC, int32_t S, bool lds_bounds, const int32_t* gbound,
-                           const int32_t* corder, const int32_t* cord_a, const int32_t* cord_b) {
+__device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>& st, size_t bound_off, int32_t cs,
+                           int32_t K, int32_t D, int32_t D0, int32_t NC, int32_t S, bool lds_bounds,
+                           const int32_t* gbound, const int32_t* corder, const int32_t* cord_a,
+                           const int32_t* cord_b) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     const int tid = threadIdx.x;
-    if (LIGHT) {
-        // the same LDS addresses, rebuilt here so the compiler sees LDS pointers
-        st.V = (R*)(s_dyn + lay.vel);
-        st.P = (R*)(s_dyn + lay.imp);
-        if (sizeof(R) == 8) st.pos = (double*)(s_dyn + lay.pos);
-        else st.pd = (R*)s_dyn;
-    }
-    const int32_t* lbound = (const int32_t*)(s_dyn + lay.bound);
     const bool fp64 = sizeof(R) == sizeof(double);
+    // LIGHT: LDS pointers built here from the shared array so the compiler
+    // addresses them as LDS
+    const int32_t* lmap = (const int32_t*)(s_dyn + st.map);
+    R* lV = (R*)(s_dyn + st.vel);
+    R* lP = (R*)(s_dyn + st.imp);
+    R* lpos = (R*)(s_dyn + st.pos);
+    const int32_t* lbound = (const int32_t*)(s_dyn + bound_off);
     const R* S_ = (const R*)d.csolv;
     const R* iiw = (const R*)d.iiw;
-#define VEL(c, i) st.V[(size_t)(c) * st.vs + (size_t)((i) - st.vb)]
-#define IMP(c, ci) st.P[(size_t)((ci) - st.pb) * st.ps + (size_t)(c) * st.pc]
+    // velocity c of side body i (global index) with compact id ci_
+#define VEL(c, i, ci_) (LIGHT ? lV[(size_t)(c) * st.nd + (size_t)(ci_)] : st.V[(size_t)(c) * st.vs + (size_t)(i)])
+#define IMP(c, ci) (LIGHT ? lP[(size_t)(c) * K + (size_t)((ci) - cs)] \
+                          : st.P[(size_t)((ci) - st.pb) * st.ps + (size_t)(c) * st.pc])
     struct IdRec { int32_t code, a, b; };
     auto fetch_id = [&](int32_t slot, IdRec& r) {
         if (slot < NC) { r.code = corder[slot]; r.a = cord_a[slot]; r.b = cord_b[slot]; }
@@ -1166,22 +1165,18 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, SolveState<R> st, co
         it.db = (r.code >> 5) & 1;
         it.ci = cs + (r.code >> 7);
         it.a = r.a; it.b = r.b;
+        if (LIGHT) { it.cida = lmap[r.a - st.off]; it.cidb = lmap[r.b - st.off]; }
         load_vec<R, RowLayout<R>::stride>(&ROW(S_, it.ci, 0), it.row);
         if constexpr (sizeof(R) == 8) {
             load_vec<R, IIW_STRIDE>(&IIW(iiw, it.a, 0), it.Ia);
             load_vec<R, IIW_STRIDE>(&IIW(iiw, it.b, 0), it.Ib);
         }
     };
-    auto pos_add = [&](int64_t i, int r, R term, bool neg) {
-        // split-impulse correction of body i (global index), component r
-        if (LIGHT) {
-            if (fp64) {
-                double& p = st.pos[(size_t)r * st.n + (size_t)(i - st.off)];
-                if (neg) p -= (double)term; else p += (double)term;
-            } else {
-                R& p = st.pd[(size_t)r * st.n + (size_t)(i - st.off)];
-                if (neg) p -= term; else p += term;
-            }
+    auto pos_add = [&](int64_t i, int32_t cid, int r, R term, bool neg) {
+        // split-impulse correction of body i (global index, compact id cid), component r
+        if (LIGHT && st.pos_lds) {
+            R& p = lpos[(size_t)r * st.nd + (size_t)cid];
+            if (neg) p -= term; else p += term;
         } else {
             if (fp64) { if (neg) BODY(d.pos, r, i) -= (double)term; else BODY(d.pos, r, i) += (double)term; }
             else BODY(d.pos, r, i) = neg ? BODY(d.pos, r, i) - (double)term : BODY(d.pos, r, i) + (double)term;
@@ -1196,8 +1191,8 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, SolveState<R> st, co
             const R vn = it.row[RowLayout<R>::dep] - (R)in.slop;
             if (vn > R(0) && ima + imb > R(1e-12)) {
                 const R dL = (R)in.correction * vn / (ima + imb);
-                if (it.da) for (int r = 0; r < 3; r++) pos_add(a, r, ima * dL * nv[r], true);
-                if (it.db) for (int r = 0; r < 3; r++) pos_add(b, r, imb * dL * nv[r], false);
+                if (it.da) for (int r = 0; r < 3; r++) pos_add(a, it.cida, r, ima * dL * nv[r], true);
+                if (it.db) for (int r = 0; r < 3; r++) pos_add(b, it.cidb, r, imb * dL * nv[r], false);
             }
         }
         if (it.s >= S) return;
@@ -1208,7 +1203,7 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, SolveState<R> st, co
             const R* t1 = it.row + SR_T1;
             const R* t2 = it.row + SR_T2;
             R va[6], vb2[6];
-            for (int c = 0; c < 6; c++) { va[c] = VEL(c, a); vb2[c] = VEL(c, b); }
+            for (int c = 0; c < 6; c++) { va[c] = VEL(c, a, it.cida); vb2[c] = VEL(c, b, it.cidb); }
             if (it.s < in.warm) {
                 // warm start (_solver_native.pyx:258-266)
                 const R n0 = IMP(0, ci), u0 = IMP(1, ci), w0 = IMP(2, ci);
@@ -1241,13 +1236,13 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, SolveState<R> st, co
                 apply_w(va, vb2, ima, imb, it.Ia, it.Ib, rA, rB, dL * t2[0], dL * t2[1], dL * t2[2]);
                 IMP(0, ci) = nI; IMP(1, ci) = tI0; IMP(2, ci) = tI1;
             }
-            if (it.da) for (int c = 0; c < 6; c++) VEL(c, a) = va[c];
-            if (it.db) for (int c = 0; c < 6; c++) VEL(c, b) = vb2[c];
+            if (it.da) for (int c = 0; c < 6; c++) VEL(c, a, it.cida) = va[c];
+            if (it.db) for (int c = 0; c < 6; c++) VEL(c, b, it.cidb) = vb2[c];
         } else {
             // FP32: the same updates on the folded Jacobian terms, with FMA
             const R* row = it.row;
             R va[6], vb2[6];
-            for (int c = 0; c < 6; c++) { va[c] = VEL(c, a); vb2[c] = VEL(c, b); }
+            for (int c = 0; c < 6; c++) { va[c] = VEL(c, a, it.cida); vb2[c] = VEL(c, b, it.cidb); }
             // J_d . v = (vb - va) . d + wb . (rB x d) - wa . (rA x d)
             auto jv = [&](int dd) -> R {
                 const R* e = row + FR_DIR + 3 * dd;
@@ -1300,8 +1295,8 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, SolveState<R> st, co
                 tI1 = nt;
                 IMP(0, ci) = nI; IMP(1, ci) = tI0; IMP(2, ci) = tI1;
             }
-            if (it.da) for (int c = 0; c < 6; c++) VEL(c, a) = va[c];
-            if (it.db) for (int c = 0; c < 6; c++) VEL(c, b) = vb2[c];
+            if (it.da) for (int c = 0; c < 6; c++) VEL(c, a, it.cida) = va[c];
+            if (it.db) for (int c = 0; c < 6; c++) VEL(c, b, it.cidb) = vb2[c];
         }
     };
     auto bound_at = [&](int32_t l) -> int32_t {
@@ -1330,7 +1325,9 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, SolveState<R> st, co
             fetch_item(r, x);
             process(x);
         }
-        if (LIGHT) {
+        // position corrections in global memory end their levels (up to the
+        // last level holding a pass-0 contact) on a full barrier
+        if (LIGHT && (st.pos_lds || l >= D0)) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
             if (BLOCK > 64) __builtin_amdgcn_s_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
@@ -1349,10 +1346,10 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, SolveState<R> st, co
 
 template <typename R, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_t* wlist, const int32_t* wcount,
-                                                 int32_t lds_n) {
+                                                 int32_t lds_bytes) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     __shared__ int32_t s_scan[BLOCK / 64 + 2];
-    __shared__ int32_t s_D;
+    __shared__ int32_t s_D, s_D0, s_nd;
     __shared__ bool s_over;
     if ((int32_t)blockIdx.x >= *wcount) return;
     const int32_t w = wlist[blockIdx.x];
@@ -1362,14 +1359,12 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     const int64_t off = d.w_off[w];
     if (K == 0) { if (tid == 0) d.w_levels[w] = 0; return; }
     const bool fp64 = sizeof(R) == sizeof(double);
-    // LDS when the world fits (solve_lds_layout); otherwise the level table and
-    // masks use global scratch, the velocities the FP64 state itself (FP64) or
-    // an FP32 copy (FP32), the impulses the solver rows, the positions the
-    // FP64 state.
-    const bool in_lds = n <= lds_n;
-    const bool imp_lds = in_lds && K <= 3 * lds_n;
-    const bool light = in_lds && imp_lds;
-    const SolveLds lay = solve_lds_layout(lds_n, sizeof(R));
+    constexpr size_t RB = sizeof(R);
+    // The launch's dynamic LDS: the last 4 * SOLVE_DCAP bytes hold the level
+    // bounds; the rest holds the level table during the assignment, then the
+    // level-loop state, each when it fits (global memory otherwise).
+    const size_t avail = (size_t)lds_bytes - 4 * (size_t)SOLVE_DCAP;
+    int32_t* s_bound = (int32_t*)(s_dyn + avail);
     R* S_ = (R*)d.csolv;
     const int4* mrec = d.mrec + cs;
     const int32_t iters = d.w_iters[w];
@@ -1382,23 +1377,13 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     int32_t* corder = d.corder + ib;
     int32_t* cord_a = d.cord_a + ib;
     int32_t* cord_b = d.cord_b + ib;
-    int32_t* s_bound = (int32_t*)(s_dyn + lay.bound);
     int32_t* g_bound = d.lcount + ib;
 
-    SolveState<R> st;
-    st.n = n; st.off = off; st.pd = nullptr; st.pos = nullptr;
-    if (in_lds) { st.V = (R*)(s_dyn + lay.vel); st.vs = n; st.vb = off; }
-    else { st.V = fp64 ? (R*)d.lv : (R*)d.svel; st.vs = d.ncap; st.vb = 0; }
-    if (imp_lds) { st.P = (R*)(s_dyn + lay.imp); st.ps = 1; st.pc = K; st.pb = cs; }
-    else { st.P = S_ + RowLayout<R>::imp; st.ps = RowLayout<R>::stride; st.pc = 1; st.pb = 0; }
-    if (light) {
-        if (fp64) st.pos = (double*)(s_dyn + lay.pos);
-        else st.pd = (R*)s_dyn;
-    }
-    // level table and masks; LDS level counters
-    if (in_lds) {
+    // ---- level assignment: table (body masks 8 B, last levels 4 B per body)
+    const bool table_lds = al16((size_t)n * 8) + (size_t)n * 4 <= avail;
+    if (table_lds) {
         uint64_t* sm = (uint64_t*)s_dyn;
-        int32_t* sl = (int32_t*)(sm + lds_n);
+        int32_t* sl = (int32_t*)(s_dyn + al16((size_t)n * 8));
         for (int32_t i = tid; i < n; i += BLOCK) { sl[i] = 0; sm[i] = 0; }
     } else {
         for (int32_t i = tid; i < n; i += BLOCK) { d.last[off + i] = 0; d.lmask[off + i] = 0; }
@@ -1406,51 +1391,73 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     for (int32_t l = tid; l < SOLVE_DCAP; l += BLOCK) s_bound[l] = 0;
     __syncthreads();
     if (tid < 64) {
-        int32_t D;
+        int32_t D, D0;
         bool over = false;
-        if (in_lds) {
+        if (table_lds) {
             uint64_t* sm = (uint64_t*)s_dyn;
-            int32_t* sl = (int32_t*)(sm + lds_n);
-            D = assign_item_levels<false>(LevelTable<false>{sl, sm}, M, NI, mrec, level, s_bound, SOLVE_DCAP, over);
+            int32_t* sl = (int32_t*)(s_dyn + al16((size_t)n * 8));
+            D = assign_item_levels<false>(LevelTable<false>{sl, sm}, M, NI, mrec, level, s_bound, SOLVE_DCAP, over,
+                                          D0);
         } else {
             D = assign_item_levels<true>(LevelTable<true>{d.last + off, d.lmask + off}, M, NI, mrec, level, s_bound,
-                                         SOLVE_DCAP, over);
+                                         SOLVE_DCAP, over, D0);
         }
-        if (tid == 0) { s_D = D; s_over = over; }
+        if (tid == 0) {
+            s_D = D; s_D0 = D0; s_over = over;
+            d.w_levels[w] = D;
+            atomicMax((unsigned long long*)&d.cnt[C_MAXLEVEL], (unsigned long long)D);
+        }
     }
     __syncthreads();
-    if (tid == 0) {
-        d.w_levels[w] = s_D;
-        atomicMax((unsigned long long*)&d.cnt[C_MAXLEVEL], (unsigned long long)s_D);
-        if (s_over) s_D = -s_D;
+    const int32_t D = s_D, D0 = s_D0;
+    const bool lds_bounds = !s_over;
+
+    // ---- compact ids of the bodies in contact (the table is no longer needed)
+    const bool map_lds = al16((size_t)n * 4) <= avail;
+    int32_t* cmap = map_lds ? (int32_t*)s_dyn : d.last + off;
+    for (int32_t i = tid; i < n; i += BLOCK) cmap[i] = 0;
+    __syncthreads();
+    for (int32_t m = tid; m < M; m += BLOCK) {
+        const int4 r = mrec[m];
+        cmap[r.z] = 1;
+        cmap[r.w] = 1;
     }
     __syncthreads();
-    // stage velocities, positions, impulses
-    if (in_lds) {
-        R* V = (R*)(s_dyn + lay.vel);
-        for (int32_t i = tid; i < n; i += BLOCK)
-            for (int c = 0; c < 6; c++) V[(size_t)c * n + i] = (R)BODY(d.lv, c, off + i);
+    const int32_t nd = block_exscan<BLOCK>(cmap, n, s_scan);
+    auto touched = [&](int32_t i) { return (i + 1 < n ? cmap[i + 1] : nd) > cmap[i]; };
+    SolveState<R> st;
+    st.map = 0;
+    st.vel = al16((size_t)n * 4);
+    st.imp = st.vel + al16((size_t)nd * 6 * RB);
+    st.pos = st.imp + al16((size_t)K * 3 * RB);
+    st.nd = nd;
+    st.off = off;
+    const bool state_lds = map_lds && st.pos <= avail;
+    st.pos_lds = state_lds && st.pos + al16((size_t)nd * 3 * RB) <= avail;
+    st.V = fp64 ? (R*)d.lv : (R*)d.svel; st.vs = d.ncap;
+    st.P = S_ + RowLayout<R>::imp; st.ps = RowLayout<R>::stride; st.pc = 1; st.pb = 0;
+
+    // ---- stage velocities, positions, impulses
+    if (state_lds) {
+        R* V = (R*)(s_dyn + st.vel);
+        R* P = (R*)(s_dyn + st.pos);
+        for (int32_t i = tid; i < n; i += BLOCK) {
+            if (!touched(i)) continue;
+            const int32_t c = cmap[i];
+            for (int q = 0; q < 6; q++) V[(size_t)q * nd + c] = (R)BODY(d.lv, q, off + i);
+            if (st.pos_lds)
+                for (int q = 0; q < 3; q++) P[(size_t)q * nd + c] = fp64 ? (R)BODY(d.pos, q, off + i) : R(0);
+        }
+        R* I = (R*)(s_dyn + st.imp);
+        for (int32_t k = tid; k < K; k += BLOCK)
+            for (int q = 0; q < 3; q++) I[(size_t)q * K + k] = ROW(S_, cs + k, RowLayout<R>::imp + q);
     } else if (!fp64) {
         R* V = (R*)d.svel;
         for (int32_t i = tid; i < n; i += BLOCK)
-            for (int c = 0; c < 6; c++) V[(size_t)c * d.ncap + off + i] = (R)BODY(d.lv, c, off + i);
+            for (int q = 0; q < 6; q++) V[(size_t)q * d.ncap + off + i] = (R)BODY(d.lv, q, off + i);
     }
-    if (light && fp64) {
-        double* P = (double*)(s_dyn + lay.pos);
-        for (int32_t i = tid; i < n; i += BLOCK)
-            for (int c = 0; c < 3; c++) P[(size_t)c * n + i] = BODY(d.pos, c, off + i);
-    }
-    if (imp_lds) {
-        R* P = (R*)(s_dyn + lay.imp);
-        for (int32_t k = tid; k < K; k += BLOCK)
-            for (int c = 0; c < 3; c++) P[(size_t)c * K + k] = ROW(S_, cs + k, RowLayout<R>::imp + c);
-    }
-    __syncthreads();
-    const bool lds_bounds = s_D > 0;
-    const int32_t D = lds_bounds ? s_D : -s_D;
-    if (light && !fp64)
-        for (int32_t i = tid; i < 3 * n; i += BLOCK) ((R*)s_dyn)[i] = R(0);   // position corrections
-    // contacts per level -> level starts -> items scattered by level
+
+    // ---- contacts per level -> level starts -> items scattered by level
     if (lds_bounds) {
         block_exscan<BLOCK>(s_bound, D, s_scan);
         scatter_items<BLOCK>(s_bound, SL, M, mrec, level, off, corder, cord_a, cord_b);
@@ -1468,41 +1475,48 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     }
     __syncthreads();
     // bound[l] is now the end of level l
-    if (light) run_levels<R, BLOCK, true>(d, in, st, lay, cs, D, NC, S, lds_bounds, g_bound, corder, cord_a, cord_b);
-    else run_levels<R, BLOCK, false>(d, in, st, lay, cs, D, NC, S, lds_bounds, g_bound, corder, cord_a, cord_b);
+    if (state_lds)
+        run_levels<R, BLOCK, true>(d, in, st, avail, cs, K, D, D0, NC, S, lds_bounds, g_bound, corder, cord_a, cord_b);
+    else
+        run_levels<R, BLOCK, false>(d, in, st, avail, cs, K, D, D0, NC, S, lds_bounds, g_bound, corder, cord_a, cord_b);
     __syncthreads();
-    // impulses back to the contacts (warm-start cache and readback)
+
+    // ---- results back: impulses to the contacts (warm-start cache and
+    // readback); velocities to the FP64 state (copied in FP64, the FP32
+    // solver's change folded in for FP32); positions
     R* imp = (R*)d.cimp;
     for (int32_t k = tid; k < K; k += BLOCK) {
         const int64_t ci = cs + k;
         for (int r = 0; r < 3; r++)
-            CON(imp, r, ci) = imp_lds ? ((R*)(s_dyn + lay.imp))[(size_t)r * K + k] : ROW(S_, ci, RowLayout<R>::imp + r);
+            CON(imp, r, ci) = state_lds ? ((R*)(s_dyn + st.imp))[(size_t)r * K + k] : ROW(S_, ci, RowLayout<R>::imp + r);
     }
-    // velocities back to the FP64 state: copied from LDS (FP64), or the FP32
-    // solver's change folded in; positions from LDS
     for (int32_t i = tid; i < n; i += BLOCK) {
         if (d.mt[off + i] != M_DYNAMIC) continue;
-        if (in_lds || !fp64) {
-            const R* V = in_lds ? (const R*)(s_dyn + lay.vel) : (const R*)d.svel;
-            const size_t vs = in_lds ? (size_t)n : (size_t)d.ncap;
-            const size_t vi = in_lds ? (size_t)i : (size_t)(off + i);
-            for (int c = 0; c < 6; c++) {
+        if (state_lds) {
+            if (!touched(i)) continue;
+            const int32_t c = cmap[i];
+            const R* V = (const R*)(s_dyn + st.vel);
+            for (int q = 0; q < 6; q++) {
                 if (fp64) {
-                    BODY(d.lv, c, off + i) = (double)V[c * vs + vi];
+                    BODY(d.lv, q, off + i) = (double)V[(size_t)q * nd + c];
                 } else {
-                    double v0 = BODY(d.lv, c, off + i);
-                    R dv = V[c * vs + vi] - (R)v0;
-                    BODY(d.lv, c, off + i) = v0 + (double)dv;
+                    const double v0 = BODY(d.lv, q, off + i);
+                    const R dv = V[(size_t)q * nd + c] - (R)v0;
+                    BODY(d.lv, q, off + i) = v0 + (double)dv;
                 }
             }
-        }
-        if (light) {
-            if (fp64) {
-                const double* P = (const double*)(s_dyn + lay.pos);
-                for (int c = 0; c < 3; c++) BODY(d.pos, c, off + i) = P[(size_t)c * n + i];
-            } else {
-                const R* pd = (const R*)s_dyn;
-                for (int c = 0; c < 3; c++) BODY(d.pos, c, off + i) = BODY(d.pos, c, off + i) + (double)pd[(size_t)c * n + i];
+            if (st.pos_lds) {
+                const R* P = (const R*)(s_dyn + st.pos);
+                for (int q = 0; q < 3; q++) {
+                    if (fp64) BODY(d.pos, q, off + i) = (double)P[(size_t)q * nd + c];
+                    else BODY(d.pos, q, off + i) = BODY(d.pos, q, off + i) + (double)P[(size_t)q * nd + c];
+                }
+            }
+        } else if (!fp64) {
+            for (int q = 0; q < 6; q++) {
+                const double v0 = BODY(d.lv, q, off + i);
+                const R dv = ((const R*)d.svel)[(size_t)q * d.ncap + off + i] - (R)v0;
+                BODY(d.lv, q, off + i) = v0 + (double)dv;
             }
         }
     }
@@ -2045,20 +2059,20 @@ void launch_wake(const Dev& d, hipStream_t s) {
 }
 
 template <typename R, int B>
-static void solve_one(const Dev& d, SolveIn in, const int32_t* wl, const int32_t* wn, int32_t lds_n, hipStream_t s) {
-    hipLaunchKernelGGL((k_solve<R, B>), dim3(d.wcap), dim3(B), solve_lds_layout(lds_n, sizeof(R)).total, s, d, in, wl, wn,
-                       lds_n);
+static void solve_one(const Dev& d, SolveIn in, const int32_t* wl, const int32_t* wn, int32_t lds_bytes,
+                      hipStream_t s) {
+    hipLaunchKernelGGL((k_solve<R, B>), dim3(d.wcap), dim3(B), (size_t)lds_bytes, s, d, in, wl, wn, lds_bytes);
 }
 
 template <typename R>
-static void solve_block(const Dev& d, int block, SolveIn in, const int32_t* wl, const int32_t* wn, int32_t lds_n,
-                        hipStream_t s) {
+static void solve_block(const Dev& d, int block, SolveIn in, const int32_t* wl, const int32_t* wn,
+                        int32_t lds_bytes, hipStream_t s) {
     switch (block) {
-        case 64: solve_one<R, 64>(d, in, wl, wn, lds_n, s); break;
-        case 128: solve_one<R, 128>(d, in, wl, wn, lds_n, s); break;
-        case 512: solve_one<R, 512>(d, in, wl, wn, lds_n, s); break;
-        case 1024: solve_one<R, 1024>(d, in, wl, wn, lds_n, s); break;
-        default: solve_one<R, 256>(d, in, wl, wn, lds_n, s); break;
+        case 64: solve_one<R, 64>(d, in, wl, wn, lds_bytes, s); break;
+        case 128: solve_one<R, 128>(d, in, wl, wn, lds_bytes, s); break;
+        case 512: solve_one<R, 512>(d, in, wl, wn, lds_bytes, s); break;
+        case 1024: solve_one<R, 1024>(d, in, wl, wn, lds_bytes, s); break;
+        default: solve_one<R, 256>(d, in, wl, wn, lds_bytes, s); break;
     }
 }
 
@@ -2089,17 +2103,6 @@ bool allow_dynamic_lds(int bytes) {
     return ok;
 }
 
-// Largest world kept wholly in LDS by a solve launch with `budget` bytes.
-int32_t solve_lds_bodies(int precision, int64_t budget, int32_t cap) {
-    int32_t lo = 0, hi = cap;
-    while (lo < hi) {
-        const int32_t mid = (lo + hi + 1) / 2;
-        const size_t need = solve_lds_layout(mid, precision == 64 ? 8 : 4).total;
-        if ((int64_t)need <= budget) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
 // Solver rows for every contact (one thread per contact).
 void launch_prepare(const Dev& d, int precision, SolveIn in, bool stage, hipStream_t s) {
     const int g = GRID_FOR(d.kcap, 256);
@@ -2112,12 +2115,12 @@ void launch_prepare(const Dev& d, int precision, SolveIn in, bool stage, hipStre
     }
 }
 
-// One block per world of size class c (world list wl, count *wn); worlds of
-// at most lds_n bodies keep their schedule state in LDS.
+// One block per world of size class c (world list wl, count *wn) with
+// lds_bytes of dynamic LDS (at least 4 * SOLVE_DCAP + 16, a multiple of 16).
 void launch_solve_class(const Dev& d, int precision, int block, SolveIn in, const int32_t* wl, const int32_t* wn,
-                        int32_t lds_n, hipStream_t s) {
-    if (precision == 64) solve_block<double>(d, block, in, wl, wn, lds_n, s);
-    else solve_block<float>(d, block, in, wl, wn, lds_n, s);
+                        int32_t lds_bytes, hipStream_t s) {
+    if (precision == 64) solve_block<double>(d, block, in, wl, wn, lds_bytes, s);
+    else solve_block<float>(d, block, in, wl, wn, lds_bytes, s);
 }
 
 void launch_cache(const Dev& d, int precision, hipStream_t s) {
