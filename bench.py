@@ -2,19 +2,23 @@
 
 Workload (default): 4096 independent worlds, each a static ground box
 (half extents 10 x 0.5 x 10) plus n_w dynamic bodies, n_w =
-clip(round(lognormal(ln 64, 1.0)), 8, 2048) (seed 3, ~435k bodies), a 50/50
+clip(round(lognormal(ln 64, 1.0)), 8, 2048) (seed 3, ~437k bodies), a 50/50
 mix of unit boxes and r = 0.5 spheres of mass 1 dropped from y in [2, 50]
 over x, z in [-10, 10]; mu = 0.5, e = 0, dt = 1/60, 10 solver iterations,
-sleeping on.  Narrowphase and solver arithmetic in FP32, integration and
-broadphase in FP64.  One "step" advances every world by dt; the timed region
-is steps 1..K of the scene from its initial state (a separate batch takes
-the warm-up steps).
+sleeping on.  One "step" advances every world by dt; the timed region is steps
+1..K of the scene from its initial state (a separate batch takes the warm-up
+steps), inputs resident in HBM, one graph launch per step.
 
-Printed: one JSON line with the batched FP32 throughput as ``value``, the
-plain one-world-at-a-time GPU form and the FP64 batched form beside it,
-p50/p99 step latency, FP32-vs-FP64 position error on a sample of worlds, the
-roofline of the dominant kernel and the CPU baseline (the oracle, a C
-restatement of the reference algorithm, on this host's cores).
+Printed: one JSON line.  ``value`` is the batched FP64 form, which is
+bit-identical to the FP64 oracle (the reference's algorithm) and so meets the
+300-step position-error bound exactly; the batched FP32 form (FP32
+narrowphase and solver) is reported beside it with its errors against FP64
+(after one step from the same state, and after 300 steps, next to the
+divergence that a 1-ulp change of the initial positions causes in FP64 alone
+-- the piles are chaotic).  Also: the plain one-world-at-a-time GPU form,
+p50/p99 step latency, per-stage device time, the roofline of the dominant
+kernel, and the CPU baseline (the oracle, a C restatement of the reference
+algorithm, one world per thread on this host's cores).
 """
 import argparse
 import json
@@ -73,6 +77,22 @@ def algorithmic_bytes(kernel, c, r):
     return 0
 
 
+def solve_top_bytes(stats, r):
+    """Algorithmic bytes of one solver launch of the largest size class present.
+
+    Per contact: its solver row (224 B, both precisions) in and its three
+    impulses (3r) out; per manifold its 16-byte record; per body in contact
+    (taken as min(bodies, 2 x manifolds) per world): FP64 velocity in and out
+    (96 B), FP64 position in and out (48 B) and world inverse inertia (12r).
+    """
+    top = int(stats['size_class'].max())
+    sel = stats['size_class'] == top
+    k = int(stats['contacts'][sel].sum())
+    m = int(stats['manifolds'][sel].sum())
+    nd = int(np.minimum(stats['bodies'][sel], 2 * stats['manifolds'][sel]).sum())
+    return k * (224 + 3 * r) + 16 * m + nd * (96 + 48 + 12 * r), top, int(sel.sum())
+
+
 def run_batched(arrays, precision, steps, warmup, timing=True):
     """Warm up on one batch, then time ``steps`` steps of a fresh batch from the initial state.
 
@@ -114,6 +134,9 @@ def run_batched(arrays, precision, steps, warmup, timing=True):
             r = 4 if precision == 'fp32' else 8
             for k in ('pre', 'sap', 'narrow', 'solve'):
                 per_kernel_bytes[k] = per_kernel_bytes.get(k, 0) + algorithmic_bytes(k, c, r)
+            tb, top, ntop = solve_top_bytes(b.world_stats(), r)
+            per_kernel_bytes['solve_top'] = per_kernel_bytes.get('solve_top', 0) + tb
+            per_kernel_bytes['top_class'] = (top, ntop)
         if (s + 1) % 50 == 0:
             log('  %s step %d/%d  %.2f ms' % (precision, s + 1, steps, 1e3 * lat[-1]))
     wall = time.perf_counter() - t_start
@@ -197,6 +220,34 @@ def single_world(name, arrays, precision, steps, warmup, cpu):
     return out
 
 
+def one_step_error(worlds, states):
+    """FP32 against FP64 after one step from the same state (the worlds at step K):
+    the precision of the FP32 narrowphase + solver before any chaotic growth."""
+    import oracle
+    from omi_physics_amd.batch import batch_from_arrays
+    start = []
+    for a, st in zip(worlds, states):
+        a = dict(a)
+        for k_arr, k_st in (('position', 'position'), ('orientation', 'orientation'),
+                            ('linear_velocity', 'linear_velocity'), ('angular_velocity', 'angular_velocity'),
+                            ('sleep_timer', 'sleep_timer')):
+            a[k_arr] = np.ascontiguousarray(st[k_st], dtype=np.float64)
+        a['awake'] = np.ascontiguousarray(st['awake'], dtype=np.uint8)
+        start.append(a)
+    b = batch_from_arrays(start, precision='fp32')
+    b.step(DT, 1)
+    b.sync()
+    ep, ev = 0.0, 0.0
+    for i, a in enumerate(start):
+        ow = oracle.OracleWorld(a)
+        ow.step(DT, 1)
+        o, g = ow.state(), b.read(i)
+        ep = max(ep, float(np.abs(o['position'] - g['position']).max()))
+        ev = max(ev, float(np.abs(o['linear_velocity'] - g['linear_velocity']).max()))
+    b.close()
+    return {'fp32_one_step_pos_max_m': ep, 'fp32_one_step_vel_max_mps': ev}
+
+
 def serving(rates, n_requests, precision, seed=5):
     from omi_physics_amd import scenes
     from omi_physics_amd.serve import Request, RolloutServer, run_plain
@@ -235,7 +286,7 @@ def main():
     ap.add_argument('--cpu-sample', type=int, default=192, help='worlds the CPU baseline and error check step')
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--no-fp64', action='store_true')
+    ap.add_argument('--no-fp32', action='store_true', help='skip the batched FP32 form')
     ap.add_argument('--no-timing', action='store_true')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
@@ -247,25 +298,37 @@ def main():
         print('no HIP device visible', file=sys.stderr)
         sys.exit(1)
     if args.workload == 'copy':
-        # a plain device-to-device copy of known size: the achievable HBM rate and
-        # the calibration of the FETCH_SIZE / WRITE_SIZE counters
-        import torch
-        n = args.copy_mib * (1 << 20) // 4
-        x = torch.ones(n, dtype=torch.float32, device='cuda')
-        y = torch.empty_like(x)
+        # a plain device-to-device copy of known size through the HIP runtime (no
+        # framework): the achievable HBM rate and the calibration of the
+        # FETCH_SIZE / WRITE_SIZE counters
+        import ctypes
+        hip = ctypes.CDLL(os.path.join(os.environ.get('ROCM_PATH', '/opt/rocm'), 'lib', 'libamdhip64.so'))
+        nbytes = ctypes.c_size_t(args.copy_mib << 20)
+        x, y = ctypes.c_void_p(), ctypes.c_void_p()
+        if hip.hipMalloc(ctypes.byref(x), nbytes) or hip.hipMalloc(ctypes.byref(y), nbytes):
+            raise RuntimeError('hipMalloc failed')
+        hip.hipMemset(x, 1, nbytes)
+        d2d = 3  # hipMemcpyDeviceToDevice
         for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            hip.hipMemcpyAsync(y, x, nbytes, d2d, None)
+        hip.hipDeviceSynchronize()
+        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        hip.hipEventCreate(ctypes.byref(e0))
+        hip.hipEventCreate(ctypes.byref(e1))
         reps = 20
-        e0.record()
+        hip.hipEventRecord(e0, None)
         for _ in range(reps):
-            y.copy_(x)
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / reps
-        print(json.dumps({'workload': 'copy', 'bytes_per_copy': 2 * 4 * n, 'ms_per_copy': ms,
-                          'GBps': 2 * 4 * n / (ms * 1e-3) / 1e9}), flush=True)
+            hip.hipMemcpyAsync(y, x, nbytes, d2d, None)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        per = ms.value / reps
+        print(json.dumps({'workload': 'copy', 'bytes_read_per_copy': nbytes.value, 'bytes_written_per_copy':
+                          nbytes.value, 'ms_per_copy': per, 'GBps': 2 * nbytes.value / (per * 1e-3) / 1e9}),
+              flush=True)
+        hip.hipFree(x)
+        hip.hipFree(y)
         return
     if args.workload != 'batched4096':
         from omi_physics_amd import scenes, PhysicsWorld, model
@@ -295,58 +358,68 @@ def main():
     rng = np.random.default_rng(11)
     sample = np.sort(rng.choice(len(arrays), size=min(args.cpu_sample, len(arrays)), replace=False))
 
-    # the timed region: graph-launched steps 1..K
-    b, lat, wall, _, _, _ = run_batched(arrays, 'fp32', args.steps, args.warmup, timing=False)
+    # headline: the batched FP64 form, graph-launched steps 1..K
+    b, lat, wall, _, _, _ = run_batched(arrays, 'fp64', args.steps, args.warmup, timing=False)
     value = n_bodies * args.steps / wall
-    gpu_pos = {int(i): b.read(int(i))['position'] for i in sample}
+    gpu64 = {int(i): b.read(int(i)) for i in sample}
     b.close()
-    ktot, ctot, kbytes = {}, {}, {}
-    if not args.no_timing:
-        # the same K steps again from the initial state, launched eagerly with HIP
-        # events between kernels: per-kernel device time and counters
-        bt, lat_t, wall_t, ktot, ctot, kbytes = run_batched(arrays, 'fp32', args.steps, 0, timing=True)
-        same = all(np.array_equal(bt.read(int(i))['position'], gpu_pos[int(i)]) for i in sample[:8])
-        bt.close()
-        log('timed pass: %.2f ms/step eager (graph %.2f), identical results: %s'
-            % (1e3 * wall_t / args.steps, 1e3 * wall / args.steps, same))
     out = {
         'metric': METRIC, 'value': value, 'unit': 'body-steps/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': 1e3 * wall / args.steps, 'higher_is_better': True,
-        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
-        'config': {'workload': '%d independent box/sphere worlds, lognormal(ln 64, 1.0) bodies clipped '
-                               '[8, 2048], batched in one graph-captured step; FP32 narrowphase + solver, '
-                               'FP64 integration/broadphase; steps 1..%d from rest' % (args.worlds, args.steps),
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic',
+        'config': {'workload': '%d independent box/sphere worlds, lognormal(ln 64, 1.0) bodies clipped [8, 2048], '
+                               'batched in one graph-captured step, FP64 (bit-identical to the oracle); steps '
+                               '1..%d from rest' % (args.worlds, args.steps),
                    'worlds': args.worlds, 'bodies': n_bodies, 'dt': DT, 'solver_iterations': 10,
                    'sleep_enabled': True, 'seed': args.seed},
         'p50_step_ms': 1e3 * float(np.percentile(lat, 50)),
         'p99_step_ms': 1e3 * float(np.percentile(lat, 99)),
     }
-    if ctot:
+    if not args.no_timing:
+        # the same K steps again from the initial state, launched eagerly with HIP
+        # events between the stages on the batch stream: device time per stage,
+        # counters and the roofline of the dominant kernel
+        bt, lat_t, wall_t, ktot, ctot, kbytes = run_batched(arrays, 'fp64', args.steps, 0, timing=True)
+        same = all(np.array_equal(bt.read(int(i))['position'], gpu64[int(i)]['position']) for i in sample[:8])
+        bt.close()
+        log('timed pass: %.2f ms/step eager (graph %.2f), identical results: %s'
+            % (1e3 * wall_t / args.steps, 1e3 * wall / args.steps, same))
+        ktot['solve'] = ktot['prepare'] + ktot['solve_top'] + ktot['solve_rest']
+        out['eager_ms_per_step'] = 1e3 * wall_t / args.steps
         out['contacts_per_s'] = ctot.get('contacts', 0) / wall
         out['totals'] = {k: int(v) for k, v in ctot.items() if k not in ('overflow', 'max_levels')}
         out['kernels_ms_per_step'] = {k: v / args.steps for k, v in ktot.items()}
-        dom = max(('pre', 'sap', 'narrow', 'solve'), key=lambda k: ktot.get(k, 0.0))
-        avg_ms = ktot[dom] / args.steps
-        achieved = kbytes[dom] / args.steps / (avg_ms * 1e-3) / 1e9
-        out['roofline'] = {'kernel': dom, 'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
-                           'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
-                           'avg_ms': avg_ms, 'bytes_per_launch': kbytes[dom] / args.steps}
-        out['eager_ms_per_step'] = 1e3 * wall_t / args.steps
-        out['roofline_all'] = {k: {'avg_ms': ktot[k] / args.steps,
-                                   'GBps': kbytes[k] / max(ktot[k] * 1e-3, 1e-12) / 1e9}
-                               for k in ('pre', 'sap', 'narrow', 'solve')}
+        top, ntop = kbytes['top_class']
+        avg_ms = ktot['solve_top'] / args.steps
+        per_launch = kbytes['solve_top'] / args.steps
+        achieved = per_launch / (avg_ms * 1e-3) / 1e9
+        traffic = None
+        tf = os.path.join(HERE, 'profiles', 'traffic.json')
+        if os.path.exists(tf):
+            with open(tf) as f:
+                traffic = json.load(f).get('k_solve_top_fp64_bytes_per_launch')
+        out['roofline'] = {'kernel': 'k_solve<double> size class %d (%d worlds, the largest; alone on the batch '
+                                     'stream)' % (top, ntop),
+                           'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                           'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic, 'avg_ms': avg_ms,
+                           'bytes_per_launch': per_launch}
+        out['roofline_stages'] = {k: {'avg_ms': ktot[k] / args.steps,
+                                      'GBps': kbytes[k] / max(ktot[k] * 1e-3, 1e-12) / 1e9}
+                                  for k in ('pre', 'sap', 'narrow', 'solve')}
 
-    if not args.no_fp64:
-        b64, lat64, wall64, *_ = run_batched(arrays, 'fp64', args.steps, 0, timing=False)
-        out['fp64_batched'] = {'value': n_bodies * args.steps / wall64, 'ms_per_step': 1e3 * wall64 / args.steps,
-                               'p99_step_ms': 1e3 * float(np.percentile(lat64, 99))}
-        fp64_pos = {int(i): b64.read(int(i))['position'] for i in sample}
-        b64.close()
+    if not args.no_fp32:
+        b32, lat32, wall32, *_ = run_batched(arrays, 'fp32', args.steps, 0, timing=False)
+        gpu32 = {int(i): b32.read(int(i))['position'] for i in sample}
+        b32.close()
+        out['fp32_batched'] = {'value': n_bodies * args.steps / wall32, 'ms_per_step': 1e3 * wall32 / args.steps,
+                               'p50_step_ms': 1e3 * float(np.percentile(lat32, 50)),
+                               'p99_step_ms': 1e3 * float(np.percentile(lat32, 99)), 'dtype': 'f32',
+                               'over_fp64': wall / wall32}
 
     if not args.no_plain:
         ps = np.sort(rng.choice(len(arrays), size=min(args.plain_sample, len(arrays)), replace=False))
-        pv, plat = run_plain([arrays[i] for i in ps], args.steps, 'fp32')
-        out['plain'] = {'value': pv, 'unit': 'body-steps/s', 'worlds_sampled': len(ps),
+        pv, plat = run_plain([arrays[i] for i in ps], args.steps, 'fp64')
+        out['plain'] = {'value': pv, 'unit': 'body-steps/s', 'dtype': 'f64', 'worlds_sampled': len(ps),
                         'p50_step_ms': 1e3 * float(np.percentile(plat, 50)),
                         'p99_step_ms': 1e3 * float(np.percentile(plat, 99)),
                         'batched_over_plain': value / pv}
@@ -359,13 +432,27 @@ def main():
                                'sample': '%d of the %d worlds (%d bodies), %d steps each, one world per '
                                          'thread, %.1f s wall' % (len(sub), len(arrays),
                                                                    sum(a['n'] for a in sub), args.steps, cwall)}
-        err = np.array([np.abs(gpu_pos[int(i)] - p).max() for i, p in zip(sample, cpos)])
-        out['err'] = {'fp32_max_m': float(err.max()), 'fp32_median_world_max_m': float(np.median(err)),
-                      'fp32_worlds_within_1e-4': float(np.mean(err <= 1e-4)), 'worlds': len(sample),
-                      'steps': args.steps}
-        if not args.no_fp64:
-            e64 = np.array([np.abs(fp64_pos[int(i)] - p).max() for i, p in zip(sample, cpos)])
-            out['err']['fp64_max_m'] = float(e64.max())
+        e64 = np.array([np.abs(gpu64[int(i)]['position'] - p).max() for i, p in zip(sample, cpos)])
+        err = {'worlds': len(sample), 'steps': args.steps, 'fp64_max_m': float(e64.max())}
+        if not args.no_fp32:
+            # FP64 alone: the same worlds with every dynamic body's initial position
+            # moved by one ulp -- how far the chaotic piles carry a last-bit change
+            def nudged(a):
+                a = dict(a)
+                p = a['position'].copy()
+                dyn = a['motion_type'] == 2
+                p[dyn] = np.nextafter(p[dyn], np.inf)
+                a['position'] = p
+                return a
+            _, _, upos = cpu_baseline([nudged(a) for a in sub], args.steps, threads)
+            e32 = np.array([np.abs(gpu32[int(i)] - p).max() for i, p in zip(sample, cpos)])
+            eu = np.array([np.abs(u - p).max() for u, p in zip(upos, cpos)])
+            err.update({'fp32_max_m': float(e32.max()), 'fp32_median_world_max_m': float(np.median(e32)),
+                        'fp32_worlds_within_1e-4': float(np.mean(e32 <= 1e-4)),
+                        'fp64_1ulp_max_m': float(eu.max()), 'fp64_1ulp_median_world_max_m': float(np.median(eu)),
+                        'fp64_1ulp_worlds_within_1e-4': float(np.mean(eu <= 1e-4))})
+            err.update(one_step_error(sub, [gpu64[int(i)] for i in sample]))
+        out['err'] = err
     print(json.dumps(out), flush=True)
 
 

@@ -213,6 +213,11 @@ int32_t omi_batch_kernel_times(omi_batch *b, int32_t n, double *ms);
 const char *omi_batch_kernel_names(void);
 /* Capture the step in a HIP graph (1) or launch kernels eagerly (0). */
 int32_t omi_batch_use_graph(omi_batch *b, int32_t on);
+/* Per-world figures of the last step for slots [0, n): bodies, contacts,
+ * manifolds, solver levels, size class (-1 = free slot); any pointer may be
+ * NULL.  Batch-layer diagnostics with no counterpart in the reference. */
+int32_t omi_batch_world_stats(omi_batch *b, int32_t n, int32_t *bodies, int32_t *contacts,
+                              int32_t *manifolds, int32_t *levels, int32_t *size_class);
 /* Threads per world of the broadphase sort and of the solver for each size
  * class (as in omi_batch_config; 0 keeps the current value).  Batch-layer
  * tuning knob with no counterpart in the reference; the autotuner uses it. */

@@ -69,7 +69,8 @@ EXPORTS = (
     'omi_batch_set_state', 'omi_batch_prepare', 'omi_batch_step', 'omi_batch_refit', 'omi_batch_sync',
     'omi_batch_read_world', 'omi_batch_read_contacts', 'omi_batch_read_pairs',
     'omi_batch_counters', 'omi_batch_set_timing', 'omi_batch_kernel_times',
-    'omi_batch_kernel_names', 'omi_batch_use_graph', 'omi_batch_set_blocks', 'omi_batch_device_bytes',
+    'omi_batch_kernel_names', 'omi_batch_use_graph', 'omi_batch_set_blocks', 'omi_batch_world_stats',
+    'omi_batch_device_bytes',
 )
 
 _lib: Optional[ctypes.CDLL] = None
@@ -129,6 +130,7 @@ def load() -> ctypes.CDLL:
         'omi_batch_kernel_names': (ctypes.c_char_p, []),
         'omi_batch_use_graph': (ctypes.c_int32, [_vp, ctypes.c_int32]),
         'omi_batch_set_blocks': (ctypes.c_int32, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+        'omi_batch_world_stats': (ctypes.c_int32, [_vp, ctypes.c_int32] + [ctypes.POINTER(ctypes.c_int32)] * 5),
         'omi_batch_device_bytes': (ctypes.c_int64, [_vp]),
     }
     for name, (res, args) in sig.items():

@@ -15,8 +15,8 @@ import numpy as np
 from . import _ffi
 from ._ffi import _d, _i32, _i64, _u8, check, ptr
 
-KERNELS = ('begin', 'pre', 'sap', 'narrow', 'compact', 'wake', 'solve', 'cache', 'joints', 'post', 'sleep',
-           'toggle')
+KERNELS = ('begin', 'pre', 'sap', 'narrow', 'compact', 'wake', 'prepare', 'solve_top', 'solve_rest', 'cache',
+           'joints', 'post', 'sleep', 'toggle')
 
 
 @dataclass
@@ -67,6 +67,7 @@ class Batch:
             cfg.solve_block[c] = int((solve_block or tuned['solve_block'])[c])
         self.config = cfg
         self.precision = 'fp64' if cfg.precision == 64 else 'fp32'
+        self.max_worlds = int(max_worlds)
         h = ctypes.c_void_p()
         check(self.lib.omi_batch_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
@@ -188,6 +189,16 @@ class Batch:
         keys = ('bodies', 'x_candidates', 'broadphase_pairs', 'solid_pairs', 'manifolds', 'contacts',
                 'max_levels', 'overflow')
         return {k: int(v) for k, v in zip(keys, o)}
+
+    def world_stats(self, n: Optional[int] = None) -> Dict[str, np.ndarray]:
+        """Per-world figures of the last step for slots [0, n): bodies, contacts, manifolds,
+        solver levels and size class (-1 for a free slot)."""
+        n = self.max_worlds if n is None else int(n)
+        out = {k: np.zeros(n, np.int32) for k in ('bodies', 'contacts', 'manifolds', 'levels', 'size_class')}
+        check(self.lib.omi_batch_world_stats(self._h, n, *(ptr(out[k], _i32) for k in
+                                                            ('bodies', 'contacts', 'manifolds', 'levels',
+                                                             'size_class'))))
+        return out
 
     def device_bytes(self) -> int:
         return int(self.lib.omi_batch_device_bytes(self._h))

@@ -102,9 +102,11 @@ struct WorldSlot {
     int32_t flags = 0;
 };
 
+// Timed stages of a step; solve_top is the solver launch of the largest size
+// class alone on the batch stream, solve_rest the wait for the other classes.
 static const char* kKernelNames =
-    "begin,pre,sap,narrow,compact,wake,solve,cache,joints,post,sleep,toggle";
-static const int kNumKernels = 12;
+    "begin,pre,sap,narrow,compact,wake,prepare,solve_top,solve_rest,cache,joints,post,sleep,toggle";
+static const int kNumKernels = 14;
 
 struct omi_batch {
     omi_batch_config cfg{};
@@ -583,8 +585,8 @@ static int32_t rebuild_lists(omi_batch* b) {
 // Launches of the size classes c in `live`, largest first: the largest on the
 // batch stream, the others on branch streams forked and joined with events,
 // so a captured graph runs them side by side.
-template <typename F>
-static void fork_classes(omi_batch* b, F launch_class) {
+template <typename F, typename G>
+static void fork_classes(omi_batch* b, F launch_class, G after_top) {
     hipStream_t s = b->stream;
     int top = -1;
     for (int c = 3; c >= 0; c--)
@@ -600,6 +602,7 @@ static void fork_classes(omi_batch* b, F launch_class) {
             hipStreamWaitEvent(sc, b->fork, 0);
         }
         launch_class(c, sc);
+        if (c == top) after_top();
         if (c != top) { hipEventRecord(b->join[q], sc); used[q] = 1; q++; }
     }
     for (int k = 0; k < 3; k++)
@@ -611,22 +614,24 @@ static void enqueue_sap(omi_batch* b, int32_t record) {
     const Dev& d = b->d;
     fork_classes(b, [&](int c, hipStream_t sc) {
         launch_sap_sort(d, c, b->d_wl + (size_t)c * d.wcap, b->d_wn + c, b->np_class[c], b->sap_block[c], sc);
-    });
+    }, [] {});
     launch_sap_pairs(d, record, b->sap_grid, b->stream);
 }
 
 // Rows for every contact, then the size classes' solve launches: the largest
 // class on the batch stream, the others on branch streams forked and joined
 // with events, so a captured graph runs them side by side.
-static void enqueue_solve(omi_batch* b, SolveIn in, bool stage) {
+template <typename Mark>
+static void enqueue_solve(omi_batch* b, SolveIn in, bool stage, Mark mark) {
     const Dev& d = b->d;
     hipStream_t s = b->stream;
     const int prec = b->cfg.precision;
     launch_prepare(d, prec, in, stage, s);
+    mark();
     fork_classes(b, [&](int c, hipStream_t sc) {
         launch_solve_class(d, prec, b->solve_block[c], in, b->d_wl + (size_t)c * d.wcap, b->d_wn + c,
                            b->lds_bytes[c], sc);
-    });
+    }, mark);
 }
 
 static void enqueue_step(omi_batch* b, double dt) {
@@ -645,7 +650,7 @@ static void enqueue_step(omi_batch* b, double dt) {
     launch_compact(d, prec, b->cfg.record_broadphase, s); mark();
     launch_wake(d, s); mark();
     SolveIn in{nullptr, nullptr, 0.5, 0.8, 0.005, 1};
-    enqueue_solve(b, in, false);
+    enqueue_solve(b, in, false, mark);
     mark();
     launch_cache(d, prec, s); mark();
     if (b->any_joints) launch_joints(d, prec, dt, s);
@@ -676,6 +681,36 @@ extern "C" int32_t omi_batch_kernel_times(omi_batch* b, int32_t n, double* ms) {
         float t = 0.f;
         HIPCHK(hipEventElapsedTime(&t, b->ev[i], b->ev[i + 1]));
         ms[i] = t;
+    }
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_world_stats(omi_batch* b, int32_t n, int32_t* bodies, int32_t* contacts,
+                                         int32_t* manifolds, int32_t* levels, int32_t* size_class) {
+    if (!b || n < 0 || n > b->cfg.max_worlds) return fail(OMI_ERR_ARG, "bad arguments");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const Dev& d = b->d;
+    std::vector<int32_t> fl((size_t)std::max(n, 1));
+    int32_t rc;
+    if (n == 0) return OMI_OK;
+    if ((rc = get(b, fl.data(), d.w_flags, n))) return rc;
+    if (bodies && (rc = get(b, bodies, d.w_cnt, n))) return rc;
+    if (contacts && (rc = get(b, contacts, d.w_cn, n))) return rc;
+    if (manifolds && (rc = get(b, manifolds, d.w_mn, n))) return rc;
+    if (levels && (rc = get(b, levels, d.w_levels, n))) return rc;
+    for (int32_t w = 0; w < n; w++) {
+        const bool live = b->slots[w].live;
+        if (!live) {
+            if (bodies) bodies[w] = 0;
+            if (contacts) contacts[w] = 0;
+            if (manifolds) manifolds[w] = 0;
+            if (levels) levels[w] = 0;
+        }
+        if (size_class) {
+            int32_t NP = 1;
+            while (NP < b->slots[w].n) NP <<= 1;
+            size_class[w] = !live ? -1 : NP <= b->np_class[0] ? 0 : NP <= b->np_class[1] ? 1 : NP <= b->np_class[2] ? 2 : 3;
+        }
     }
     return OMI_OK;
 }
@@ -1156,7 +1191,7 @@ extern "C" int32_t omi_prepare_and_solve(int64_t K, int32_t N, const int64_t* a_
         b->slots[0].live = true;
         b->slots[0].n = N;
         if (!(rc = rebuild_lists(b)))
-            enqueue_solve(b, in, true);
+            enqueue_solve(b, in, true, [] {});
         if (!rc && hipGetLastError() != hipSuccess) rc = fail(OMI_ERR_HIP, "solver launch failed");
         if (!rc) rc = download_soa(b, pos, d.pos, d.ncap, 0, N, 3);
         if (!rc) rc = download_soa(b, lv, d.lv, d.ncap, 0, N, 3);
