@@ -157,7 +157,8 @@ typedef struct {
     int32_t solver_block;       /* threads per world in the solver launch (64..1024, 0 = auto) */
     /* Per size class (worlds of <= 256, <= 1024, <= 4096 bodies, larger):
      * threads per world for the broadphase and the solver launches, one of
-     * 64, 128, 256, 512, 1024; 0 picks the default.  The autotuner sets these. */
+     * 64, 128, 256, 512, 1024 (FP64 solver: 64, 128, 256; larger is
+     * OMI_ERR_ARG); 0 picks the default.  The autotuner sets these. */
     int32_t sap_block[4];
     int32_t solve_block[4];
     /* x-window candidates of the sweep-and-prune per step, all worlds, that

@@ -10,9 +10,11 @@ time (coordinate descent), interleaving the candidates over a few rounds so
 the slowly changing contact state weighs on all of them alike, and keeps the
 fastest.
 
-The result is a small JSON table (``tuned.json`` beside this file is the one
-measured for the 4096-world workload on MI355X) that ``Batch`` takes as
-``sap_block`` / ``solve_block``.
+The result is a small JSON table per precision (``tuned.json`` beside this
+file holds the ones measured for the 4096-world workload on MI355X) that
+``Batch`` takes as ``sap_block`` / ``solve_block``.  FP64 solver blocks stop
+at 256 threads: the FP64 sweep needs more registers per thread than a larger
+block leaves.
 """
 import json
 import os
@@ -24,7 +26,14 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 TUNED = os.path.join(HERE, 'tuned.json')
 CANDIDATES = (64, 128, 256, 512, 1024)
+FP64_SOLVE_BLOCK_MAX = 256
 CLASS_LIMITS = (256, 1024, 4096)
+DEFAULTS = {'fp32': {'sap_block': [64, 128, 1024, 1024], 'solve_block': [64, 128, 256, 256]},
+            'fp64': {'sap_block': [64, 128, 1024, 1024], 'solve_block': [64, 128, 256, 256]}}
+
+
+def _prec(precision) -> str:
+    return 'fp64' if precision in ('fp64', 64) else 'fp32'
 
 
 def size_class(n: int) -> int:
@@ -37,13 +46,15 @@ def size_class(n: int) -> int:
     return 3
 
 
-def load_tuned(path: str = TUNED) -> Dict[str, List[int]]:
-    """The tuned table, or defaults when none has been measured."""
+def load_tuned(precision='fp32', path: str = TUNED) -> Dict[str, List[int]]:
+    """The tuned table of ``precision``, or the defaults when none has been measured."""
+    p = _prec(precision)
+    t = {}
     if os.path.exists(path):
         with open(path) as f:
-            t = json.load(f)
-        return {'sap_block': list(t['sap_block']), 'solve_block': list(t['solve_block'])}
-    return {'sap_block': [64, 128, 1024, 1024], 'solve_block': [64, 128, 256, 256]}
+            t = json.load(f).get(p, {})
+    d = DEFAULTS[p]
+    return {k: [int(x) for x in t.get(k, d[k])] for k in ('sap_block', 'solve_block')}
 
 
 def tune(worlds: Sequence[dict], precision: str = 'fp32', dt: float = 1.0 / 60.0, warm_steps: int = 120,
@@ -52,7 +63,7 @@ def tune(worlds: Sequence[dict], precision: str = 'fp32', dt: float = 1.0 / 60.0
     """Pick per-class block sizes for ``worlds`` (list of ``export_world_arrays`` dicts)."""
     from .batch import batch_from_arrays
     classes = sorted({size_class(int(w['n'])) for w in worlds})
-    best = load_tuned()
+    best = load_tuned(precision)
     b = batch_from_arrays(worlds, precision=precision, sap_block=best['sap_block'], solve_block=best['solve_block'])
     b.prepare(dt)
     b.step(dt, warm_steps)          # let the worlds settle into contact
@@ -60,9 +71,11 @@ def tune(worlds: Sequence[dict], precision: str = 'fp32', dt: float = 1.0 / 60.0
     report = {}
     for c in classes:
         for kind in ('sap_block', 'solve_block'):
-            times = {B: 0.0 for B in candidates}
+            cands = [B for B in candidates
+                     if not (kind == 'solve_block' and _prec(precision) == 'fp64' and B > FP64_SOLVE_BLOCK_MAX)]
+            times = {B: 0.0 for B in cands}
             for _ in range(rounds):
-                for B in candidates:
+                for B in cands:
                     cfg = {k: list(v) for k, v in best.items()}
                     cfg[kind][c] = B
                     b.set_blocks(cfg['sap_block'], cfg['solve_block'])
@@ -78,14 +91,20 @@ def tune(worlds: Sequence[dict], precision: str = 'fp32', dt: float = 1.0 / 60.0
             log('class %d %s: %s ms/step -> %d' % (c, kind, ' '.join('%d:%.3f' % (B, v * 1e3)
                                                                       for B, v in times.items()), pick))
     b.close()
-    return {'sap_block': best['sap_block'], 'solve_block': best['solve_block'], 'precision': precision,
+    return {'sap_block': best['sap_block'], 'solve_block': best['solve_block'], 'precision': _prec(precision),
             'workload_worlds': len(worlds), 'measured_ms_per_step': report,
             'when': time.strftime('%Y-%m-%d %H:%M:%S')}
 
 
 def save(result: Dict[str, object], path: str = TUNED) -> None:
+    """Store ``result`` as its precision's table, keeping the other precision's."""
+    tables = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            tables = {k: v for k, v in json.load(f).items() if k in DEFAULTS}
+    tables[_prec(result['precision'])] = result
     with open(path, 'w') as f:
-        json.dump(result, f, indent=1)
+        json.dump(tables, f, indent=1)
 
 
 if __name__ == '__main__':
@@ -96,10 +115,13 @@ if __name__ == '__main__':
     ap = argparse.ArgumentParser(description='tune per-class block sizes on the 4096-world workload')
     ap.add_argument('--worlds', type=int, default=4096)
     ap.add_argument('--out', default=TUNED)
+    ap.add_argument('--precision', choices=('fp32', 'fp64', 'both'), default='both')
     a = ap.parse_args()
     sizes = scenes.world_sizes(a.worlds)
     worlds = [scenes.arrays(scenes.boxes_on_ground(int(sizes[w]), seed=3 * 100003 + w, sphere_fraction=0.5))
               for w in range(a.worlds)]
-    res = tune(worlds, log=lambda *x: print(*x, flush=True))
-    save(res, a.out)
-    print(json.dumps({k: res[k] for k in ('sap_block', 'solve_block')}))
+    for prec in (('fp32', 'fp64') if a.precision == 'both' else (a.precision,)):
+        res = tune(worlds, precision=prec, log=lambda *x: print(prec, *x, flush=True))
+        save(res, a.out)
+        print(json.dumps({'precision': prec, 'sap_block': res['sap_block'], 'solve_block': res['solve_block']}),
+              flush=True)

@@ -61,7 +61,7 @@ class Batch:
         cfg.record_broadphase = int(bool(record_broadphase))
         cfg.solver_block = int(solver_block)
         from .autotune import load_tuned
-        tuned = load_tuned()
+        tuned = load_tuned(precision)
         for c in range(4):
             cfg.sap_block[c] = int((sap_block or tuned['sap_block'])[c])
             cfg.solve_block[c] = int((solve_block or tuned['solve_block'])[c])

@@ -198,6 +198,12 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
         return fail(OMI_ERR_ARG, "capacity exceeds the index range (pairs 2^30, contacts 2^24, bodies 2^30)");
     if (cfg->max_solver_iterations < 0 || cfg->max_solver_iterations > 30)
         return fail(OMI_ERR_ARG, "max_solver_iterations must be 0 (default 16) .. 30");
+    if (cfg->precision == OMI_FP64)
+        for (int c = 0; c < 4; c++) {
+            const int32_t blk = cfg->solve_block[c] > 0 ? cfg->solve_block[c] : cfg->solver_block;
+            if (blk > FP64_SOLVE_BLOCK_MAX)
+                return fail(OMI_ERR_ARG, "FP64 solver blocks are 64, 128 or 256 threads");
+        }
     int32_t rc = check_device();
     if (rc) return rc;
     omi_batch* b = new omi_batch();
@@ -718,9 +724,12 @@ extern "C" int32_t omi_batch_world_stats(omi_batch* b, int32_t n, int32_t* bodie
 extern "C" int32_t omi_batch_set_blocks(omi_batch* b, const int32_t* sap_block, const int32_t* solve_block) {
     if (!b || !sap_block || !solve_block) return fail(OMI_ERR_ARG, "null argument");
     auto okb = [](int32_t x) { return x == 0 || x == 64 || x == 128 || x == 256 || x == 512 || x == 1024; };
-    for (int c = 0; c < 4; c++)
+    for (int c = 0; c < 4; c++) {
         if (!okb(sap_block[c]) || !okb(solve_block[c]))
             return fail(OMI_ERR_ARG, "block sizes must be 0, 64, 128, 256, 512 or 1024");
+        if (b->cfg.precision == OMI_FP64 && solve_block[c] > FP64_SOLVE_BLOCK_MAX)
+            return fail(OMI_ERR_ARG, "FP64 solver blocks are 64, 128 or 256 threads");
+    }
     for (int c = 0; c < 4; c++) {
         if (sap_block[c]) b->sap_block[c] = sap_block[c];
         if (solve_block[c]) b->solve_block[c] = solve_block[c];

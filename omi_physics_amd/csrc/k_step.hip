@@ -747,11 +747,13 @@ __global__ __launch_bounds__(BLOCK) void k_wake(Dev d) {
 
 // ---------------------------------------------------------------- solver --
 
-// Solver row layout per contact (R [SR_N][kcap]).
+// Solver row layout per contact (R [kcap][stride]).
 // FP64 row: the reference's per-contact quantities; the sweep recomputes the
-// angular terms in the reference's operation order.
+// angular terms in the reference's operation order. A sweep item loads only
+// the first SR_HOT values (everything but the seeds, which the level loop
+// reads from the impulse state), so fewer registers hold a prefetched item.
 enum { SR_RA = 0, SR_RB = 3, SR_T1 = 6, SR_T2 = 9, SR_NM = 12, SR_T1M = 13, SR_T2M = 14, SR_VB = 15,
-       SR_FR = 16, SR_IMA = 17, SR_IMB = 18, SR_NX = 19, SR_IMP = 22, SR_DEP = 25, SR_N = 26 };
+       SR_FR = 16, SR_IMA = 17, SR_IMB = 18, SR_NX = 19, SR_DEP = 22, SR_IMP = 23, SR_N = 26, SR_HOT = 24 };
 // FP32 row: directions d = n, t1, t2 with the Jacobian terms folded in,
 // cA_d = rA x d, cB_d = rB x d, kA_d = IA (rA x d), kB_d = IB (rB x d), so a
 // sweep is dot products and scaled adds (FMA) on the same quantities.
@@ -759,22 +761,27 @@ enum { FR_DIR = 0, FR_CA = 9, FR_CB = 18, FR_KA = 27, FR_KB = 36, FR_NM = 45, FR
        FR_VB = 48, FR_FR = 49, FR_IMA = 50, FR_IMB = 51, FR_DEP = 52, FR_IMP = 53, FR_STRIDE = 56 };
 template <typename R> struct RowLayout;
 template <> struct RowLayout<double> {
-    static constexpr int stride = SR_STRIDE, imp = SR_IMP, dep = SR_DEP, ima = SR_IMA, imb = SR_IMB, nrm = SR_NX;
+    static constexpr int stride = SR_STRIDE, hot = SR_HOT, imp = SR_IMP, dep = SR_DEP, ima = SR_IMA, imb = SR_IMB,
+                         nrm = SR_NX;
 };
 template <> struct RowLayout<float> {
-    static constexpr int stride = FR_STRIDE, imp = FR_IMP, dep = FR_DEP, ima = FR_IMA, imb = FR_IMB, nrm = FR_DIR;
+    static constexpr int stride = FR_STRIDE, hot = FR_STRIDE, imp = FR_IMP, dep = FR_DEP, ima = FR_IMA, imb = FR_IMB,
+                         nrm = FR_DIR;
 };
+static_assert(SR_DEP < SR_HOT && SR_NX + 3 <= SR_HOT && SR_HOT % 2 == 0, "the FP64 sweep's fields lie in the hot prefix");
 
 // n values of type R from 16-byte aligned memory as 16-byte loads
+// (a last partial chunk is loaded value by value)
 template <typename R, int N>
 __device__ __forceinline__ void load_vec(const R* src, R* dst) {
     constexpr int P = 16 / sizeof(R);
-    static_assert(N % P == 0, "whole 16-byte chunks");
 #pragma unroll
     for (int q = 0; q < N / P; q++) {
         const uint4 t = ((const uint4*)src)[q];
         __builtin_memcpy(dst + q * P, &t, 16);
     }
+#pragma unroll
+    for (int q = (N / P) * P; q < N; q++) dst[q] = src[q];
 }
 
 // _solver_native._effmass (_solver_native.pyx:154-176)
@@ -989,8 +996,8 @@ struct SolveItem {
     int64_t ci, a, b;
     int32_t s, cida, cidb;          // pass; compact ids of the sides (LDS state)
     bool da, db;
-    R row[RowLayout<R>::stride];
-    R Ia[sizeof(R) == 8 ? IIW_STRIDE : 1], Ib[sizeof(R) == 8 ? IIW_STRIDE : 1];   // FP64 rows only
+    R row[RowLayout<R>::hot];
+    R Ia[sizeof(R) == 8 ? 9 : 1], Ib[sizeof(R) == 8 ? 9 : 1];   // FP64 rows only: world inverse inertia
 };
 static_assert(SR_STRIDE * sizeof(double) == SOLVE_ROW_BYTES && FR_STRIDE * sizeof(float) == SOLVE_ROW_BYTES,
               "solver rows fill SOLVE_ROW_BYTES in both precisions");
@@ -1166,10 +1173,10 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>&
         it.ci = cs + (r.code >> 7);
         it.a = r.a; it.b = r.b;
         if (LIGHT) { it.cida = lmap[r.a - st.off]; it.cidb = lmap[r.b - st.off]; }
-        load_vec<R, RowLayout<R>::stride>(&ROW(S_, it.ci, 0), it.row);
+        load_vec<R, RowLayout<R>::hot>(&ROW(S_, it.ci, 0), it.row);
         if constexpr (sizeof(R) == 8) {
-            load_vec<R, IIW_STRIDE>(&IIW(iiw, it.a, 0), it.Ia);
-            load_vec<R, IIW_STRIDE>(&IIW(iiw, it.b, 0), it.Ib);
+            load_vec<R, 9>(&IIW(iiw, it.a, 0), it.Ia);
+            load_vec<R, 9>(&IIW(iiw, it.b, 0), it.Ib);
         }
     };
     auto pos_add = [&](int64_t i, int32_t cid, int r, R term, bool neg) {
@@ -2070,9 +2077,16 @@ static void solve_block(const Dev& d, int block, SolveIn in, const int32_t* wl, 
     switch (block) {
         case 64: solve_one<R, 64>(d, in, wl, wn, lds_bytes, s); break;
         case 128: solve_one<R, 128>(d, in, wl, wn, lds_bytes, s); break;
-        case 512: solve_one<R, 512>(d, in, wl, wn, lds_bytes, s); break;
-        case 1024: solve_one<R, 1024>(d, in, wl, wn, lds_bytes, s); break;
         default: solve_one<R, 256>(d, in, wl, wn, lds_bytes, s); break;
+        // the FP64 sweep holds two prefetched items in ~300 VGPRs per thread,
+        // more than a 512-thread block leaves it: FP64 runs up to 256 threads
+        // (the C-ABI rejects larger FP64 blocks)
+        case 512:
+            if constexpr (sizeof(R) == 4) solve_one<R, 512>(d, in, wl, wn, lds_bytes, s);
+            break;
+        case 1024:
+            if constexpr (sizeof(R) == 4) solve_one<R, 1024>(d, in, wl, wn, lds_bytes, s);
+            break;
     }
 }
 
@@ -2092,8 +2106,7 @@ static bool allow_lds_one(int bytes) { return allow_lds_fn(k_solve<R, B>, bytes)
 bool allow_dynamic_lds(int bytes) {
     bool ok = true;
     ok = allow_lds_one<double, 64>(bytes) && ok; ok = allow_lds_one<double, 128>(bytes) && ok;
-    ok = allow_lds_one<double, 256>(bytes) && ok; ok = allow_lds_one<double, 512>(bytes) && ok;
-    ok = allow_lds_one<double, 1024>(bytes) && ok;
+    ok = allow_lds_one<double, 256>(bytes) && ok;
     ok = allow_lds_one<float, 64>(bytes) && ok; ok = allow_lds_one<float, 128>(bytes) && ok;
     ok = allow_lds_one<float, 256>(bytes) && ok; ok = allow_lds_one<float, 512>(bytes) && ok;
     ok = allow_lds_one<float, 1024>(bytes) && ok;

@@ -38,6 +38,9 @@ constexpr int32_t SAP_TILE = 512;
 // inertia (9 used): whole 16-byte chunks in FP32 and FP64
 constexpr int32_t SR_STRIDE = 28;
 constexpr int32_t IIW_STRIDE = 12;
+// largest solver block in FP64: two prefetched sweep items need more
+// registers per thread than a 512-thread block leaves
+constexpr int32_t FP64_SOLVE_BLOCK_MAX = 256;
 // bytes of one contact's solver row (FP64: 28 doubles; FP32: 56 floats)
 constexpr int32_t SOLVE_ROW_BYTES = 224;
 

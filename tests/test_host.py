@@ -34,6 +34,15 @@ def test_binding_loads_without_gpu():
     assert tuple(L.omi_batch_kernel_names().decode().split(',')) == KERNELS
 
 
+def test_fp64_solver_block_limit_rejected_before_device_use():
+    # argument checks run before the library touches a device
+    from omi_physics_amd.batch import Batch
+    from omi_physics_amd.autotune import load_tuned
+    with pytest.raises(RuntimeError, match='FP64 solver blocks'):
+        Batch(4, 64, precision='fp64', solve_block=[64, 128, 512, 256])
+    assert max(load_tuned('fp64')['solve_block']) <= 256
+
+
 def test_mass_properties_follow_reference():
     w = PhysicsWorld()
     box = w.add_shape(model.Shape.box((1.0, 2.0, 3.0)))
