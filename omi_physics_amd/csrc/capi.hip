@@ -22,7 +22,7 @@ void launch_sap_sort(const Dev& d, int c, const int32_t* wl, const int32_t* wn, 
                      hipStream_t s);
 void launch_sap_pairs(const Dev& d, int32_t record, int32_t grid, hipStream_t s);
 void launch_narrow(const Dev& d, int precision, hipStream_t s);
-void launch_compact(const Dev& d, int precision, int32_t record, hipStream_t s);
+void launch_compact(const Dev& d, int precision, int grid, hipStream_t s);
 void launch_wake(const Dev& d, hipStream_t s);
 void launch_cache(const Dev& d, int precision, hipStream_t s);
 void launch_post(const Dev& d, int64_t nhw, double dt, hipStream_t s);
@@ -262,7 +262,7 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     A(d.w_off, W); A(d.w_cnt, W); A(d.w_flags, W); A(d.w_iters, W);
     A(d.w_mat_off, W); A(d.w_nmat, W); A(d.w_filt_off, W); A(d.w_nfilt, W);
     A(d.w_grav, 3 * W); A(d.w_damp, 2 * W);
-    A(d.w_ps, W); A(d.w_pn, W); A(d.w_bps, W); A(d.w_bpn, W); A(d.w_cs, W); A(d.w_cn, W); A(d.w_mn, W); A(d.w_levels, W);
+    A(d.w_ps, W); A(d.w_pn, W); A(d.w_bps, W); A(d.w_bpn, W); A(d.w_cs, W); A(d.w_cn, W); A(d.w_mn, W); A(d.w_ms, W); A(d.w_levels, W);
     A(d.w_cache_s, 2 * W); A(d.w_cache_n, 2 * W);
     A(d.w_x, W); A(d.w_t0, W); A(d.w_tn, W);
     {
@@ -276,6 +276,8 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     A(d.pa, P); A(d.pb, P);
     A(d.bpa, cfg->record_broadphase ? P : 1); A(d.bpb, cfg->record_broadphase ? P : 1);
     A(d.mcnt, P); A(d.mpts, 3 * 4 * P); A(d.cvx, P);
+    d.pchunk_cap = (P + CPT_CHUNK - 1) / CPT_CHUNK + 1;
+    A(d.pfx, 2 * (P + 1)); A(d.pchunk, 2 * d.pchunk_cap);
     if (cfg->max_hull_pts > 1) {
         // GJK/EPA workspaces: one per thread of the convex narrowphase launch
         unsigned char* ew;
@@ -527,6 +529,7 @@ extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_wo
     i32 = w->n_filt; if ((rc = put(b, d.w_nfilt + slot, &i32, 1))) return rc;
     i32 = 0;
     if ((rc = put(b, d.w_mn + slot, &i32, 1))) return rc;
+    if ((rc = put(b, d.w_ms + slot, &i32, 1))) return rc;
     if ((rc = put(b, d.w_cache_n + slot, &i32, 1))) return rc;
     if ((rc = put(b, d.w_cache_n + W + slot, &i32, 1))) return rc;
     if ((rc = put(b, d.w_pn + slot, &i32, 1))) return rc;
@@ -653,7 +656,7 @@ static void enqueue_step(omi_batch* b, double dt) {
     enqueue_sap(b, b->cfg.record_broadphase);
     mark();
     launch_narrow(d, prec, s); mark();
-    launch_compact(d, prec, b->cfg.record_broadphase, s); mark();
+    launch_compact(d, prec, b->sap_grid, s); mark();
     launch_wake(d, s); mark();
     SolveIn in{nullptr, nullptr, 0.5, 0.8, 0.005, 1};
     enqueue_solve(b, in, false, mark);
@@ -1194,7 +1197,7 @@ extern "C" int32_t omi_prepare_and_solve(int64_t K, int32_t N, const int64_t* a_
         !(rc = upR(d.cimp, d.kcap, imp.data(), K, 3, 3)) && !(rc = put(b, d.w_off, &zero, 1)) &&
         !(rc = put(b, d.w_cnt, &Ni, 1)) && !(rc = put(b, d.w_cs, &zero, 1)) && !(rc = put(b, d.w_cn, &Ki, 1)) &&
         !(rc = put(b, d.w_iters, &iters, 1)) && !(rc = put(b, d.w_flags, &flags, 1)) &&
-        !(rc = put(b, d.mrec, mrec.data(), Mi)) && !(rc = put(b, d.w_mn, &Mi, 1)) &&
+        !(rc = put(b, d.mrec, mrec.data(), Mi)) && !(rc = put(b, d.w_mn, &Mi, 1)) && !(rc = put(b, d.w_ms, &zero, 1)) &&
         !(rc = put(b, d.cnt + C_CONTACTS, &kc, 1))) {
         SolveIn in{drest, dfric, rthr, correction, slop, warm};
         b->slots[0].live = true;

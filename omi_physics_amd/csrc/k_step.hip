@@ -4,7 +4,7 @@
 //   world.step prev-pose copy, integrate_forces, refit  -> k_pre        (1 thread / body)
 //   BroadPhase.pairs + solid filter                     -> k_sap        (1 block / world)
 //   NarrowPhase.generate (box/sphere)                   -> k_narrow     (1 thread / pair)
-//   contact list in pair-then-manifold order            -> k_compact    (1 block / world)
+//   contact list in pair-then-manifold order            -> k_pair_*     (scan over all pairs)
 //   SequentialImpulseSolver._wake_sleepers              -> k_wake       (1 block / world)
 //   _solve_native_full + prepare_and_solve              -> k_solve      (1 block / world)
 //   warm-start cache rebuild                            -> k_cache      (1 block / world)
@@ -441,17 +441,15 @@ __global__ __launch_bounds__(BLOCK) void k_sap_pairs(Dev d, int32_t record) {
     }
 }
 
-// Per-tile counts -> exclusive offsets (in place, one block), then every
-// world's solid and broadphase pair ranges.
+// Exclusive scans of a0[0..T) and a1[0..T) in place by one block; the totals
+// land in totals[0], totals[1] (LDS) for every thread.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_sap_scan(Dev d, int32_t record) {
+__device__ void block_exscan2(int64_t* a0, int64_t* a1, int64_t T, int64_t* totals) {
     constexpr int NW = BLOCK / 64, ITEMS = 8;
     __shared__ int64_t s_w[2][NW];
-    __shared__ int64_t s_run[2];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const uint64_t tt = d.cnt[C_TILES];
-    const int64_t T = tt < (uint64_t)d.tcap ? (int64_t)tt : d.tcap;
-    int64_t* arr[2] = {d.tcnt, d.tcnt + d.tcap + 1};
+    int64_t* arr[2] = {a0, a1};
+    int64_t* s_run = totals;
     if (tid < 2) s_run[tid] = 0;
     __syncthreads();
     for (int64_t base = 0; base < T; base += (int64_t)BLOCK * ITEMS) {
@@ -485,11 +483,26 @@ __global__ __launch_bounds__(BLOCK) void k_sap_scan(Dev d, int32_t record) {
         }
         __syncthreads();
     }
+}
+
+// Per-tile counts -> exclusive offsets (in place, one block), then every
+// world's solid and broadphase pair ranges.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sap_scan(Dev d, int32_t record) {
+    __shared__ int64_t s_run[2];
+    const int tid = threadIdx.x;
+    const uint64_t tt = d.cnt[C_TILES];
+    const int64_t T = tt < (uint64_t)d.tcap ? (int64_t)tt : d.tcap;
+    int64_t* arr[2] = {d.tcnt, d.tcnt + d.tcap + 1};
+    block_exscan2<BLOCK>(arr[0], arr[1], T, s_run);
     if (tid == 0) {
         arr[0][T] = s_run[0];
         arr[1][T] = s_run[1];
         d.cnt[C_PAIRS] = (uint64_t)s_run[0];
         d.cnt[C_BPTOTAL] = (uint64_t)s_run[1];
+        // pairs of the worlds whose lists fit: all of them, or those below
+        // the world that crosses the capacity (set in the loop below)
+        d.cnt[C_PVALID] = (uint64_t)(s_run[0] < d.pcap ? s_run[0] : d.pcap);
         if (record) d.cnt[C_BPAIRS] = (uint64_t)s_run[1];
         if (s_run[0] > d.pcap) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 1ull);
         if (record && s_run[1] > d.pcap) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 8ull);
@@ -503,12 +516,16 @@ __global__ __launch_bounds__(BLOCK) void k_sap_scan(Dev d, int32_t record) {
             const int64_t t0 = d.w_t0[w], t1 = t0 + d.w_tn[w];
             ps = arr[0][t0]; pe = arr[0][t1]; bs = arr[1][t0]; be = arr[1][t1];
         }
+        // pair ranges are disjoint and ascending in tile order: at most one
+        // world crosses the capacity
+        if (live && ps < d.pcap && pe > d.pcap) d.cnt[C_PVALID] = (uint64_t)ps;
         d.w_ps[w] = (int32_t)(ps < d.pcap ? ps : d.pcap);
         d.w_pn[w] = pe <= d.pcap ? (int32_t)(pe - ps) : 0;
         d.w_bps[w] = (int32_t)(bs < d.pcap ? bs : d.pcap);
         d.w_bpn[w] = (record && be <= d.pcap) ? (int32_t)(be - bs) : 0;
     }
 }
+
 
 // ----------------------------------------------------------- narrowphase --
 
@@ -628,80 +645,144 @@ __global__ void k_stage_convex(int32_t np, const int32_t* ka, const double* ca, 
 // Contacts in the reference's order (pair order, then manifold order) packed
 // into one contiguous slice per world, with one manifold record per pair
 // that produced contacts: {first contact, contacts | a dynamic << 8 | b
-// dynamic << 9, body a, body b}, all world-local.
-template <typename R, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_compact(Dev d, int32_t record) {
-    __shared__ int32_t s_scan[2][BLOCK / 64 + 2];
-    __shared__ int64_t s_sum[BLOCK / 64 + 2];
-    __shared__ int32_t s_base[2];
-    const int32_t w = blockIdx.x;
-    const int tid = threadIdx.x;
-    if (w >= d.wcap) return;
-    const int32_t pn = (d.w_flags[w] & WF_ACTIVE) ? d.w_pn[w] : 0, ps = d.w_ps[w];
-    if (pn == 0) { if (tid == 0) { d.w_cn[w] = 0; d.w_mn[w] = 0; } return; }
-    int64_t kc = 0, mf = 0;
-    for (int32_t i = tid; i < pn; i += BLOCK) { int c = d.mcnt[ps + i]; kc += c; mf += c > 0; }
-    const int64_t K = block_sum<BLOCK>(kc, s_sum);
-    const int64_t M = block_sum<BLOCK>(mf, s_sum);
-    if (tid == 0) {
-        uint64_t base = atomicAdd((unsigned long long*)&d.cnt[C_CONTACTS], (unsigned long long)K);
-        int32_t ok = base + K <= (uint64_t)d.kcap;
-        if (!ok) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 2ull);
-        s_base[0] = (int32_t)base; s_base[1] = ok;
-        d.w_cs[w] = (int32_t)base; d.w_cn[w] = ok ? (int32_t)K : 0;
-        d.w_mn[w] = ok ? (int32_t)M : 0;
-        atomicAdd((unsigned long long*)&d.cnt[C_MANIFOLDS], (unsigned long long)M);
+// dynamic << 9, body a, body b}; first contact is a batch index, the bodies
+// are world-local.  Pair lists are world-contiguous and ascending, so one
+// exclusive scan of the contact counts over all valid pairs places every
+// world's contacts: chunk sums (k_pair_count), a scan of the chunk sums
+// (k_pair_scan), then every pair writes its contacts and record at its own
+// prefix (k_pair_write) and every world takes its slice from the prefixes
+// at its first and one-past-last pair (k_world_slices).  A world whose
+// slice ends past the contact capacity gets no contacts (overflow flag 2),
+// and so do all worlds after it.
+__device__ __forceinline__ int64_t pair_chunks(const Dev& d, int64_t& P) {
+    P = (int64_t)d.cnt[C_PVALID];
+    return (P + CPT_CHUNK - 1) / CPT_CHUNK;
+}
+
+__global__ __launch_bounds__(CPT_BLOCK) void k_pair_count(Dev d) {
+    __shared__ int64_t s_sum[CPT_BLOCK / 64 + 2];
+    int64_t P;
+    const int64_t nch = pair_chunks(d, P);
+    for (int64_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+        int64_t c = 0, m = 0;
+#pragma unroll
+        for (int j = 0; j < CPT_ITEMS; j++) {
+            const int64_t p = ch * CPT_CHUNK + j * CPT_BLOCK + threadIdx.x;
+            const int v = p < P ? d.mcnt[p] : 0;
+            c += v;
+            m += v > 0;
+        }
+        // contacts in the high half, manifolds in the low (both < 2^31 per chunk)
+        const int64_t t = block_sum<CPT_BLOCK>((c << 32) | m, s_sum);
+        if (threadIdx.x == 0) { d.pchunk[ch] = t >> 32; d.pchunk[d.pchunk_cap + ch] = t & 0xffffffffll; }
     }
-    __syncthreads();
-    if (!s_base[1]) return;
-    const int32_t cs = s_base[0];
-    const int64_t off = d.w_off[w];
-    int32_t run = 0, mrun = 0;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_pair_scan(Dev d) {
+    __shared__ int64_t s_run[2];
+    int64_t P;
+    const int64_t nch = pair_chunks(d, P);
+    block_exscan2<BLOCK>(d.pchunk, d.pchunk + d.pchunk_cap, nch, s_run);
+    if (threadIdx.x == 0) {
+        const int64_t K = s_run[0], M = s_run[1];
+        d.cnt[C_CONTACTS] = (uint64_t)K;
+        d.cnt[C_MANIFOLDS] = (uint64_t)M;
+        if (K > d.kcap) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 2ull);
+        d.pfx[P] = (int32_t)(K < INT32_MAX ? K : INT32_MAX);
+        d.pfx[d.pcap + 1 + P] = (int32_t)(M < INT32_MAX ? M : INT32_MAX);
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(CPT_BLOCK) void k_pair_write(Dev d) {
+    constexpr int NW = CPT_BLOCK / 64;
+    __shared__ int32_t s_w[2][CPT_ITEMS][NW];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int64_t P;
+    const int64_t nch = pair_chunks(d, P);
     const R* mn = (const R*)d.mnrm;
     const R* md = (const R*)d.mdep;
     R* cn = (R*)d.cnrm;
     R* cdp = (R*)d.cdep;
     R* imp = (R*)d.cimp;
-    const int lane = tid & 63, wv = tid >> 6;
-    for (int32_t cb = 0; cb < pn; cb += BLOCK) {
-        const int32_t i = cb + tid;
-        const int32_t p = ps + i;
-        const int32_t c = i < pn ? d.mcnt[p] : 0;
-        // exclusive prefixes over the chunk of the contact counts (0..4) and of
-        // the manifold flags
-        int32_t x = c, y = c > 0;
+    int32_t* pfx_m = d.pfx + d.pcap + 1;
+    for (int64_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+        int32_t c[CPT_ITEMS], xc[CPT_ITEMS], xm[CPT_ITEMS];
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            int32_t u = __shfl_up(x, o, 64), v = __shfl_up(y, o, 64);
-            if (lane >= o) { x += u; y += v; }
+        for (int j = 0; j < CPT_ITEMS; j++) {
+            const int64_t p = ch * CPT_CHUNK + j * CPT_BLOCK + tid;
+            c[j] = p < P ? d.mcnt[p] : 0;
+            xc[j] = c[j];
+            xm[j] = c[j] > 0;
         }
-        if (lane == 63) { s_scan[0][wv] = x; s_scan[1][wv] = y; }
-        __syncthreads();
-        int32_t wbase = 0, tot = 0, mbase = 0, mtot = 0;
-        for (int q = 0; q < BLOCK / 64; q++) {
-            int32_t s0 = s_scan[0][q], s1 = s_scan[1][q];
-            if (q < wv) { wbase += s0; mbase += s1; }
-            tot += s0; mtot += s1;
-        }
-        const int32_t first = run + wbase + x - c;       // world-local first contact
-        if (c > 0) {
-            const int64_t a = d.pa[p], b = d.pb[p];
-            const int32_t fl = (d.mt[a] == M_DYNAMIC ? MR_DYN_A : 0) | (d.mt[b] == M_DYNAMIC ? MR_DYN_B : 0);
-            d.mrec[cs + mrun + mbase + y - 1] = make_int4(first, c | fl, (int32_t)(a - off), (int32_t)(b - off));
-        }
-        for (int m = 0; m < c; m++) {
-            const int64_t k = cs + first + m;
-            d.ca[k] = d.pa[p]; d.cb[k] = d.pb[p];
-            for (int r = 0; r < 3; r++) {
-                CON(d.cpt, r, k) = d.mpts[(size_t)r * 4 * d.pcap + 4 * (size_t)p + m];
-                CON(cn, r, k) = mn[(size_t)r * d.pcap + p];
-                CON(imp, r, k) = R(0);
+        // inclusive prefixes within each wavefront row
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+#pragma unroll
+            for (int j = 0; j < CPT_ITEMS; j++) {
+                const int32_t u = __shfl_up(xc[j], o, 64), v = __shfl_up(xm[j], o, 64);
+                if (lane >= o) { xc[j] += u; xm[j] += v; }
             }
-            cdp[k] = md[4 * (size_t)p + m];
-        }
-        run += tot;
-        mrun += mtot;
+        if (lane == 63)
+#pragma unroll
+            for (int j = 0; j < CPT_ITEMS; j++) { s_w[0][j][wv] = xc[j]; s_w[1][j][wv] = xm[j]; }
         __syncthreads();
+        // pair order within the chunk is row j, then thread: rows before j,
+        // then the wavefronts before this one in row j
+        int64_t bc = d.pchunk[ch], bm = d.pchunk[d.pchunk_cap + ch];
+#pragma unroll
+        for (int j = 0; j < CPT_ITEMS; j++) {
+            int32_t rc = 0, rm = 0, wc = 0, wm = 0;
+#pragma unroll
+            for (int q = 0; q < NW; q++) {
+                const int32_t sc = s_w[0][j][q], sm = s_w[1][j][q];
+                rc += sc; rm += sm;
+                if (q < wv) { wc += sc; wm += sm; }
+            }
+            const int64_t p = ch * CPT_CHUNK + j * CPT_BLOCK + tid;
+            const int64_t k0 = bc + wc + xc[j] - c[j];          // first contact of the pair (batch index)
+            const int64_t m0 = bm + wm + xm[j] - (c[j] > 0);    // its manifold record
+            if (p < P) {
+                d.pfx[p] = (int32_t)(k0 < INT32_MAX ? k0 : INT32_MAX);
+                pfx_m[p] = (int32_t)(m0 < INT32_MAX ? m0 : INT32_MAX);
+            }
+            if (c[j] > 0 && k0 + c[j] <= d.kcap) {
+                const int64_t a = d.pa[p], b = d.pb[p];
+                const int64_t off = d.w_off[d.world[a]];
+                const int32_t fl = (d.mt[a] == M_DYNAMIC ? MR_DYN_A : 0) | (d.mt[b] == M_DYNAMIC ? MR_DYN_B : 0);
+                d.mrec[m0] = make_int4((int32_t)k0, c[j] | fl, (int32_t)(a - off), (int32_t)(b - off));
+                for (int m = 0; m < c[j]; m++) {
+                    const int64_t k = k0 + m;
+                    d.ca[k] = (int32_t)a; d.cb[k] = (int32_t)b;
+                    for (int r = 0; r < 3; r++) {
+                        CON(d.cpt, r, k) = d.mpts[(size_t)r * 4 * d.pcap + 4 * (size_t)p + m];
+                        CON(cn, r, k) = mn[(size_t)r * d.pcap + p];
+                        CON(imp, r, k) = R(0);
+                    }
+                    cdp[k] = md[4 * (size_t)p + m];
+                }
+            }
+            bc += rc; bm += rm;
+        }
+        __syncthreads();
+    }
+}
+
+// Every world's contact slice and manifold records from the pair prefixes.
+__global__ void k_world_slices(Dev d) {
+    const int32_t* pfx_m = d.pfx + d.pcap + 1;
+    for (int64_t w = gtid(); w < d.wcap; w += gstride()) {
+        const int32_t pn = (d.w_flags[w] & WF_ACTIVE) ? d.w_pn[w] : 0, ps = d.w_ps[w];
+        int32_t cs = 0, cn = 0, ms = 0, mn = 0;
+        if (pn > 0) {
+            cs = d.pfx[ps];
+            const int32_t ce = d.pfx[ps + pn];
+            ms = pfx_m[ps];
+            if (ce <= d.kcap) { cn = ce - cs; mn = pfx_m[ps + pn] - ms; }
+            else cs = 0;
+        }
+        d.w_cs[w] = cs; d.w_cn[w] = cn; d.w_ms[w] = ms; d.w_mn[w] = mn;
     }
 }
 
@@ -1122,7 +1203,7 @@ struct SolveState {
 // entry (after the scan) and its end on exit.
 template <int BLOCK>
 __device__ void scatter_items(int32_t* bound, int32_t SL, int32_t M, const int4* mrec, const int32_t* level,
-                              int64_t off, int32_t* corder, int32_t* cord_a, int32_t* cord_b) {
+                              int32_t cs, int64_t off, int32_t* corder, int32_t* cord_a, int32_t* cord_b) {
     for (int32_t s = 0; s < SL; s++)
         for (int32_t m = threadIdx.x; m < M; m += BLOCK) {
             const int4 r = mrec[m];
@@ -1130,7 +1211,7 @@ __device__ void scatter_items(int32_t* bound, int32_t SL, int32_t M, const int4*
             const bool da = r.y & MR_DYN_A, db = r.y & MR_DYN_B;
             for (int32_t j = 0; j < c; j++) {
                 const int32_t slot = atomicAdd(&bound[L - 1 + j], 1);
-                corder[slot] = item_code(r.x + j, da, db, s);
+                corder[slot] = item_code(r.x - cs + j, da, db, s);
                 cord_a[slot] = (int32_t)off + r.z;
                 cord_b[slot] = (int32_t)off + r.w;
             }
@@ -1373,7 +1454,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     const size_t avail = (size_t)lds_bytes - 4 * (size_t)SOLVE_DCAP;
     int32_t* s_bound = (int32_t*)(s_dyn + avail);
     R* S_ = (R*)d.csolv;
-    const int4* mrec = d.mrec + cs;
+    const int4* mrec = d.mrec + d.w_ms[w];
     const int32_t iters = d.w_iters[w];
     const int32_t S = in.warm + iters;          // velocity passes
     const int32_t SL = S > 0 ? S : 1;           // passes in the schedule (the position pass rides on pass 0)
@@ -1467,7 +1548,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     // ---- contacts per level -> level starts -> items scattered by level
     if (lds_bounds) {
         block_exscan<BLOCK>(s_bound, D, s_scan);
-        scatter_items<BLOCK>(s_bound, SL, M, mrec, level, off, corder, cord_a, cord_b);
+        scatter_items<BLOCK>(s_bound, SL, M, mrec, level, cs, off, corder, cord_a, cord_b);
     } else {
         for (int32_t l = tid; l < D; l += BLOCK) g_bound[l] = 0;
         __syncthreads();
@@ -1478,7 +1559,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
             }
         __syncthreads();
         block_exscan<BLOCK>(g_bound, D, s_scan);
-        scatter_items<BLOCK>(g_bound, SL, M, mrec, level, off, corder, cord_a, cord_b);
+        scatter_items<BLOCK>(g_bound, SL, M, mrec, level, cs, off, corder, cord_a, cord_b);
     }
     __syncthreads();
     // bound[l] is now the end of level l
@@ -1874,6 +1955,7 @@ __global__ void k_sleep_union(Dev d) {
         const int32_t a = d.ca[k], b = d.cb[k];
         const int32_t w = d.world[a];
         if (w < 0 || !(d.w_flags[w] & WF_SLEEP)) continue;
+        if (k < d.w_cs[w] || k >= (int64_t)d.w_cs[w] + d.w_cn[w]) continue;   // a world that overflowed
         const bool da = d.mt[a] == M_DYNAMIC, db = d.mt[b] == M_DYNAMIC;
         if (da) d.contacted[a] = 1;
         if (db) d.contacted[b] = 1;
@@ -2056,9 +2138,14 @@ void launch_stage_convex(int32_t np, const int32_t* ka, const double* ca, const 
                        Rb, hb, ob, nb, hull, (EpaWork*)work, slots, cnt, nrm, pt, dep);
 }
 
-void launch_compact(const Dev& d, int precision, int32_t record, hipStream_t s) {
-    if (precision == 64) hipLaunchKernelGGL((k_compact<double, 256>), dim3(d.wcap), dim3(256), 0, s, d, record);
-    else hipLaunchKernelGGL((k_compact<float, 256>), dim3(d.wcap), dim3(256), 0, s, d, record);
+void launch_compact(const Dev& d, int precision, int grid, hipStream_t s) {
+    const int64_t nch = (d.pcap + CPT_CHUNK - 1) / CPT_CHUNK;
+    const int g = (int)(nch < grid ? nch : grid);
+    hipLaunchKernelGGL(k_pair_count, dim3(g), dim3(CPT_BLOCK), 0, s, d);
+    hipLaunchKernelGGL((k_pair_scan<1024>), dim3(1), dim3(1024), 0, s, d);
+    if (precision == 64) hipLaunchKernelGGL((k_pair_write<double>), dim3(g), dim3(CPT_BLOCK), 0, s, d);
+    else hipLaunchKernelGGL((k_pair_write<float>), dim3(g), dim3(CPT_BLOCK), 0, s, d);
+    hipLaunchKernelGGL(k_world_slices, dim3(GRID_FOR(d.wcap, 256)), dim3(256), 0, s, d);
 }
 
 void launch_wake(const Dev& d, hipStream_t s) {

@@ -29,11 +29,14 @@ enum : int32_t {
     C_OVERFLOW = 8,    // bit 0 pairs, 1 contacts, 2 cache, 3 broadphase record, 4 candidate tiles
     C_CONVEX = 9,      // convex pairs queued for GJK/EPA
     C_TILES = 10,      // broadphase candidate tiles reserved
+    C_PVALID = 11,     // solid pairs of the worlds whose pair lists fit (a prefix of the pair list)
     C_COUNT = 16
 };
 
 // x-window candidates per broadphase tile (one wavefront task)
 constexpr int32_t SAP_TILE = 512;
+// contact compaction: pairs per chunk (CPT_ITEMS rows of CPT_BLOCK threads)
+constexpr int32_t CPT_BLOCK = 256, CPT_ITEMS = 4, CPT_CHUNK = CPT_BLOCK * CPT_ITEMS;
 // slots of one contact's solver row (26 used) and of one body's world inverse
 // inertia (9 used): whole 16-byte chunks in FP32 and FP64
 constexpr int32_t SR_STRIDE = 28;
@@ -74,6 +77,7 @@ struct Dev {
     int32_t *w_ps, *w_pn, *w_bps, *w_bpn; // solid / broadphase pair ranges
     int32_t *w_cs, *w_cn;                 // contact ranges
     int32_t *w_mn;                        // manifolds (runs of contacts of one pair) per world
+    int32_t *w_ms;                        // first manifold record of each world
     int32_t *w_levels;                    // levels of the last solve
     int64_t *w_x;                         // x-window candidates of the step
     int32_t *w_t0, *w_tn;                 // candidate tiles: first, count
@@ -86,6 +90,9 @@ struct Dev {
     // pairs
     int32_t *pa, *pb, *bpa, *bpb;
     int8_t *mcnt;
+    int32_t *pfx;                         // [2][pcap + 1] first contact / manifold record of each pair
+    int64_t *pchunk;                      // [2][pchunk_cap] contacts / manifolds per pair chunk -> offsets
+    int64_t pchunk_cap;
     int32_t *cvx;                         // [pcap] pairs that need GJK/EPA
     void *epa_work;                       // EpaWork per GJK/EPA thread
     int32_t epa_slots;

@@ -255,6 +255,48 @@ def test_batch_equals_alone(lib):
             assert np.array_equal(x[key], y[key]), (i, key)
 
 
+@pytest.mark.parametrize('kind', ['max_contacts', 'max_pairs'])
+def test_capacity_overflow_drops_whole_worlds(lib, kind):
+    """A step that runs out of contact (or pair) capacity raises on sync; every world
+    either keeps all of its contacts, with a state bit-equal to an unconstrained batch,
+    or gets none."""
+    specs = [helpers.pile(n, seed=s) for n, s in ((40, 1), (90, 2), (15, 3), (130, 4), (70, 5), (110, 6))]
+    arrays = [helpers.world(sp).export_arrays() for sp in specs]
+    W = len(arrays)
+    probe = batch_from_arrays(arrays)
+    peak = 0
+    for _ in range(12):
+        probe.step(DT, 10); probe.sync()
+        peak = max(peak, int(probe.counters()['contacts' if kind == 'max_contacts' else 'solid_pairs']))
+    cap = int(peak * 0.7)
+    say('overflow of %s: peak %d, capacity %d' % (kind, peak, cap))
+    big = batch_from_arrays(arrays)
+    small = batch_from_arrays(arrays, **{kind: cap})
+    for s in range(120):
+        big.step(DT); big.sync()
+        small.step(DT)
+        try:
+            small.sync()
+            over = False
+        except RuntimeError as e:
+            assert 'capacity' in str(e)
+            over = True
+        kb, ks = big.world_stats(W)['contacts'], small.world_stats(W)['contacts']
+        if not over:
+            assert np.array_equal(kb, ks), s
+            continue
+        fit = ks > 0
+        assert np.all((ks == kb) | (ks == 0)), (s, kb, ks)
+        assert (kb[~fit] > 0).any(), (s, kb, ks)
+        if kind == 'max_contacts':
+            assert ks.sum() <= cap
+        for i in np.flatnonzero(fit):
+            for key in ('position', 'orientation', 'linear_velocity', 'angular_velocity'):
+                assert np.array_equal(small.read(i)[key], big.read(i)[key]), (s, i, key)
+        return
+    pytest.fail('the capacity was never exceeded')
+
+
 def test_world_reload_resets_slot(lib):
     """Removing a world and loading another into the freed slot gives the fresh result."""
     a = helpers.world(helpers.pile(60, seed=8)).export_arrays()
