@@ -557,7 +557,7 @@ __device__ __forceinline__ int narrow_pair(const Dev& d, int64_t a, int64_t b, R
 }
 
 template <typename R>
-__global__ void k_narrow(Dev d) {
+__global__ __launch_bounds__(NARROW_BLOCK) void k_narrow(Dev d) {
     const uint64_t tot = d.cnt[C_PAIRS];
     const int64_t P = tot < (uint64_t)d.pcap ? (int64_t)tot : d.pcap;
     R* mn = (R*)d.mnrm;
@@ -571,12 +571,14 @@ __global__ void k_narrow(Dev d) {
             d.cvx[slot] = (int32_t)p;
             continue;
         }
-        R nrm[3] = {R(0), R(0), R(0)}, pts[12], dep[4];
+        R nrm[3] = {R(0), R(0), R(0)}, pts[12] = {}, dep[4] = {};
         double org[3];
         int cnt = narrow_pair<R>(d, a, b, nrm, pts, dep, org);
         d.mcnt[p] = (int8_t)cnt;
         for (int r = 0; r < 3; r++) mn[(size_t)r * d.pcap + p] = nrm[r];
-        for (int m = 0; m < cnt; m++) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            if (m >= cnt) break;
             for (int r = 0; r < 3; r++) d.mpts[(size_t)r * 4 * d.pcap + 4 * p + m] = org[r] + (double)pts[3 * m + r];
             md[4 * p + m] = dep[m];
         }
@@ -596,7 +598,7 @@ __device__ __forceinline__ void make_proxy(const Dev& d, int64_t i, Proxy& P) {
 // pairs k_narrow queued; FP64 in both precisions, one thread per pair, each
 // with its own EPA workspace.
 template <typename R>
-__global__ void k_narrow_convex(Dev d) {
+__global__ __launch_bounds__(64) void k_narrow_convex(Dev d) {
     const uint64_t tot = d.cnt[C_CONVEX];
     const int64_t n = tot < (uint64_t)d.pcap ? (int64_t)tot : d.pcap;
     const int64_t slot = gtid();
@@ -620,7 +622,7 @@ __global__ void k_narrow_convex(Dev d) {
     }
 }
 
-__global__ void k_stage_convex(int32_t np, const int32_t* ka, const double* ca, const double* Ra, const double* ha,
+__global__ __launch_bounds__(64) void k_stage_convex(int32_t np, const int32_t* ka, const double* ca, const double* Ra, const double* ha,
                                const int32_t* oa, const int32_t* na, const int32_t* kb, const double* cb,
                                const double* Rb, const double* hb, const int32_t* ob, const int32_t* nb,
                                const double* hull, EpaWork* work, int32_t slots, int32_t* cnt, double* nrm,
@@ -2010,7 +2012,7 @@ __global__ void k_sleep_final(Dev d, int64_t nhw) {
 // ------------------------------------------------------ stage-level kernels --
 
 template <typename R>
-__global__ void k_stage_boxbox(int32_t np, const double* ca, const double* Ra, const double* ha, const double* cb,
+__global__ __launch_bounds__(NARROW_BLOCK) void k_stage_boxbox(int32_t np, const double* ca, const double* Ra, const double* ha, const double* cb,
                                const double* Rb, const double* hb, int32_t* cnt, double* nrm, double* pts,
                                double* dep) {
     for (int64_t i = gtid(); i < np; i += gstride()) {
@@ -2118,9 +2120,9 @@ void launch_sap_pairs(const Dev& d, int32_t record, int32_t grid, hipStream_t s)
 }
 
 void launch_narrow(const Dev& d, int precision, hipStream_t s) {
-    int g = GRID_FOR(d.pcap, 256);
-    if (precision == 64) hipLaunchKernelGGL(k_narrow<double>, dim3(g), dim3(256), 0, s, d);
-    else hipLaunchKernelGGL(k_narrow<float>, dim3(g), dim3(256), 0, s, d);
+    int g = GRID_FOR(d.pcap, NARROW_BLOCK);
+    if (precision == 64) hipLaunchKernelGGL(k_narrow<double>, dim3(g), dim3(NARROW_BLOCK), 0, s, d);
+    else hipLaunchKernelGGL(k_narrow<float>, dim3(g), dim3(NARROW_BLOCK), 0, s, d);
     if (d.epa_slots > 0) {
         int ge = (d.epa_slots + 63) / 64;
         if (precision == 64) hipLaunchKernelGGL(k_narrow_convex<double>, dim3(ge), dim3(64), 0, s, d);
@@ -2251,11 +2253,11 @@ void launch_toggle(const Dev& d, hipStream_t s) { hipLaunchKernelGGL(k_toggle, d
 void launch_stage_boxbox(int precision, int32_t np, const double* ca, const double* Ra, const double* ha,
                          const double* cb, const double* Rb, const double* hb, int32_t* cnt, double* nrm,
                          double* pts, double* dep, hipStream_t s) {
-    int g = GRID_FOR(np, 256);
+    int g = GRID_FOR(np, NARROW_BLOCK);
     if (precision == 64)
-        hipLaunchKernelGGL(k_stage_boxbox<double>, dim3(g), dim3(256), 0, s, np, ca, Ra, ha, cb, Rb, hb, cnt, nrm, pts, dep);
+        hipLaunchKernelGGL(k_stage_boxbox<double>, dim3(g), dim3(NARROW_BLOCK), 0, s, np, ca, Ra, ha, cb, Rb, hb, cnt, nrm, pts, dep);
     else
-        hipLaunchKernelGGL(k_stage_boxbox<float>, dim3(g), dim3(256), 0, s, np, ca, Ra, ha, cb, Rb, hb, cnt, nrm, pts, dep);
+        hipLaunchKernelGGL(k_stage_boxbox<float>, dim3(g), dim3(NARROW_BLOCK), 0, s, np, ca, Ra, ha, cb, Rb, hb, cnt, nrm, pts, dep);
 }
 
 void launch_stage_spheres(int precision, int32_t np, int32_t mode, const double* ca, const double* ra,

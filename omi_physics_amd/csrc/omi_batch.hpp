@@ -35,6 +35,8 @@ enum : int32_t {
 
 // x-window candidates per broadphase tile (one wavefront task)
 constexpr int32_t SAP_TILE = 512;
+// threads per block of the box/sphere narrowphase (one pair per thread)
+constexpr int32_t NARROW_BLOCK = 256;
 // contact compaction: pairs per chunk (CPT_ITEMS rows of CPT_BLOCK threads)
 constexpr int32_t CPT_BLOCK = 256, CPT_ITEMS = 4, CPT_CHUNK = CPT_BLOCK * CPT_ITEMS;
 // slots of one contact's solver row (26 used) and of one body's world inverse

@@ -9,6 +9,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace omi {
 
@@ -69,18 +70,29 @@ __device__ __forceinline__ int face_axis(const R* M, R d0, R d1, R d2, R& sign) 
 }
 
 // _collide_native._face_vertices (_collide_native.pyx:92-110); v[12]
+// A value the compiler must treat as computed, not loaded: a select between
+// two such values stays a select, instead of becoming one load from a
+// run-time address (which would pin the source array in private memory).
+template <typename R>
+__device__ __forceinline__ R opaque(R x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 template <typename R>
 __device__ __forceinline__ void face_vertices(const R* c, const R* M, const R* h, int axis, R sign, R* v) {
     int o0 = axis == 0 ? 1 : 0, o1 = axis != 2 ? 2 : 1;
     if (axis == 1) { o0 = 0; o1 = 2; }
-    R ha = axis == 0 ? h[0] : (axis == 1 ? h[1] : h[2]);
-    R h0 = o0 == 0 ? h[0] : h[1];
-    R h1 = o1 == 1 ? h[1] : h[2];
+    const R hx = opaque(h[0]), hy = opaque(h[1]), hz = opaque(h[2]);
+    R ha = axis == 0 ? hx : (axis == 1 ? hy : hz);
+    R h0 = o0 == 0 ? hx : hy;
+    R h1 = o1 == 1 ? hy : hz;
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        R col = axis == 0 ? M[3 * r] : (axis == 1 ? M[3 * r + 1] : M[3 * r + 2]);
-        R c0 = o0 == 0 ? M[3 * r] : M[3 * r + 1];
-        R c1 = o1 == 1 ? M[3 * r + 1] : M[3 * r + 2];
+        const R m0 = opaque(M[3 * r]), m1 = opaque(M[3 * r + 1]), m2 = opaque(M[3 * r + 2]);
+        R col = axis == 0 ? m0 : (axis == 1 ? m1 : m2);
+        R c0 = o0 == 0 ? m0 : m1;
+        R c1 = o1 == 1 ? m1 : m2;
         R fc = c[r] + col * sign * ha;
         R u = c0 * h0, w = c1 * h1;
         v[r] = fc + u + w;
@@ -141,10 +153,12 @@ __device__ int reduce4(const R* P, const R* D, int nc, R* op, R* od) {
 
 #define OMI_POLY_MAX 16
 
-// _collide_native._clip (_collide_native.pyx:113-209)
+// _collide_native._clip (_collide_native.pyx:113-209) on 16-vertex buffers in
+// private memory: the path for the rare polygon that outgrows the 8 register
+// slots of clip_manifold below.
 template <typename R>
-__device__ int clip_manifold(const R* ca, const R* Ra, const R* ha, const R* cb, const R* Rb, const R* hb,
-                             const R* nrm, R depth, R* op, R* od) {
+__device__ __noinline__ int clip_manifold_general(const R* ca, const R* Ra, const R* ha, const R* cb, const R* Rb,
+                                                  const R* hb, const R* nrm, R depth, R* op, R* od) {
     R a_align = R(-1), b_align = R(-1);
 #pragma unroll
     for (int m = 0; m < 3; m++) {
@@ -210,12 +224,270 @@ __device__ int clip_manifold(const R* ca, const R* Ra, const R* ha, const R* cb,
     return reduce4(allp, alld, nc, op, od);
 }
 
+// A clip polygon of up to 8 vertices in registers.  Slots are written through
+// one-hot selects and read at indices that are static after unrolling, so no
+// slot is ever addressed at run time (which would put the polygon in private
+// memory).  A convex quadrilateral clipped by 4 half-planes has at most 8
+// vertices; more only arise from rounding on near-degenerate input, and then
+// the clip reports overflow and the caller takes the 16-vertex path.
+template <typename R>
+struct Poly8 {
+    R x[8], y[8], z[8];
+};
+
+// f(integral_constant<int, I>) for I = B .. N-1: loop indices that are
+// constant expressions
+template <int B, int N, typename F>
+__device__ __forceinline__ void sfor(F&& f) {
+    if constexpr (B < N) {
+        f(std::integral_constant<int, B>{});
+        sfor<B + 1, N>(f);
+    }
+}
+
+// Slot k of P with k run-time: a select over the 8 slots.  The selects act
+// on opaque values and the writes below store every slot unconditionally, so
+// the compiler cannot fold them into one access at a run-time index.
+template <typename R>
+__device__ __forceinline__ R sel8(const R (&a)[8], int k) {
+    R x = opaque(a[0]);
+#pragma unroll
+    for (int s = 1; s < 8; s++) x = k == s ? opaque(a[s]) : x;
+    return x;
+}
+
+template <typename R>
+__device__ __forceinline__ void get8(const Poly8<R>& P, int k, R& x, R& y, R& z) {
+    x = sel8(P.x, k); y = sel8(P.y, k); z = sel8(P.z, k);
+}
+
+// write (x, y, z) to slot k of Q; KMAX bounds the slots k can reach
+template <int KMAX, typename R>
+__device__ __forceinline__ void put8(Poly8<R>& Q, int k, R x, R y, R z) {
+#pragma unroll
+    for (int s = 0; s < (KMAX < 8 ? KMAX : 8); s++) {
+        const bool h = k == s;
+        Q.x[s] = h ? x : Q.x[s]; Q.y[s] = h ? y : Q.y[s]; Q.z[s] = h ? z : Q.z[s];
+    }
+}
+
+// One Sutherland-Hodgman side plane (_collide_native.pyx:171-188): P (n
+// vertices) into Q, returning Q's count.  The distance of vertex (i + 1) % n
+// is the same expression on the same operands as vertex i + 1's own, so it is
+// reused.
+template <typename R>
+__device__ __forceinline__ int clip_side8(const Poly8<R>& P, int n, R p0, R p1, R p2, R off, Poly8<R>& Q,
+                                          bool& over) {
+    R dc[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) dc[i] = p0 * P.x[i] + p1 * P.y[i] + p2 * P.z[i] - off;
+    int nb = 0;
+    sfor<0, 8>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        constexpr int j = i + 1 < 8 ? i + 1 : 0;
+        if (i < n) {
+            const bool wrap = i + 1 == n;
+            const R dn = wrap ? dc[0] : dc[j];
+            const R xj = wrap ? P.x[0] : P.x[j], yj = wrap ? P.y[0] : P.y[j], zj = wrap ? P.z[0] : P.z[j];
+            if (dc[i] <= R(0)) {
+                over = over || nb >= 8;
+                put8<2 * i + 1>(Q, nb, P.x[i], P.y[i], P.z[i]);
+                nb++;
+            }
+            if (dc[i] * dn < R(0)) {
+                const R t = dc[i] / (dc[i] - dn);
+                over = over || nb >= 8;
+                put8<2 * i + 2>(Q, nb, P.x[i] + t * (xj - P.x[i]), P.y[i] + t * (yj - P.y[i]),
+                                P.z[i] + t * (zj - P.z[i]));
+                nb++;
+            }
+        }
+    });
+    return nb;
+}
+
+// _collide_native._reduce (_collide_native.pyx:212-263) on up to 8 points in
+// registers; the same selection order as reduce4.
+template <typename R>
+__device__ __forceinline__ int reduce8(const Poly8<R>& P, const R (&D)[8], int nc, R* op, R* od) {
+    if (nc <= 4) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (i < nc) { op[3 * i] = P.x[i]; op[3 * i + 1] = P.y[i]; op[3 * i + 2] = P.z[i]; od[i] = D[i]; }
+        return nc;
+    }
+    int k0 = 0, k1 = 0, k2 = 0, k3 = 0;
+    R best = P.x[0];
+#pragma unroll
+    for (int i = 1; i < 8; i++) if (i < nc && P.x[i] > best) { best = P.x[i]; k0 = i; }
+    R c0x, c0y, c0z, c1x, c1y, c1z, c2x, c2y, c2z;
+    get8(P, k0, c0x, c0y, c0z);
+    best = R(-1);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        if (i >= nc) continue;
+        R dd = R(0), d0;
+        d0 = P.x[i] - c0x; dd += d0 * d0;
+        d0 = P.y[i] - c0y; dd += d0 * d0;
+        d0 = P.z[i] - c0z; dd += d0 * d0;
+        if (dd > best) { best = dd; k1 = i; }
+    }
+    get8(P, k1, c1x, c1y, c1z);
+    best = R(-1);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        if (i >= nc) continue;
+        R ds = R(0), dd = R(0), d0;
+        d0 = P.x[i] - c1x; ds += d0 * d0;
+        d0 = P.x[i] - c0x; dd += d0 * d0;
+        d0 = P.y[i] - c1y; ds += d0 * d0;
+        d0 = P.y[i] - c0y; dd += d0 * d0;
+        d0 = P.z[i] - c1z; ds += d0 * d0;
+        d0 = P.z[i] - c0z; dd += d0 * d0;
+        const R sd = sqrt(ds) + sqrt(dd);
+        if (sd > best) { best = sd; k2 = i; }
+    }
+    get8(P, k2, c2x, c2y, c2z);
+    best = R(-1);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        if (i >= nc) continue;
+        R dd = R(0), d0;
+        d0 = P.x[i] - c2x; dd += d0 * d0;
+        d0 = P.y[i] - c2y; dd += d0 * d0;
+        d0 = P.z[i] - c2z; dd += d0 * d0;
+        const R sd = sqrt(dd);
+        if (sd > best) { best = sd; k3 = i; }
+    }
+    // first occurrences of k0..k3, in that order
+    const int keep[4] = {k0, k1, k2, k3};
+    int nk = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        bool dup = false;
+#pragma unroll
+        for (int j = 0; j < i; j++) dup = dup || keep[i] == keep[j];
+        if (!dup) {
+            R x, y, z;
+            get8(P, keep[i], x, y, z);
+            const R dv = sel8(D, keep[i]);
+#pragma unroll
+            for (int q = 0; q <= i; q++) {
+                const bool h = nk == q;
+                op[3 * q] = h ? x : op[3 * q]; op[3 * q + 1] = h ? y : op[3 * q + 1];
+                op[3 * q + 2] = h ? z : op[3 * q + 2]; od[q] = h ? dv : od[q];
+            }
+            nk++;
+        }
+    }
+    return nk;
+}
+
+// _collide_native._clip (_collide_native.pyx:113-209): reference face = the
+// box face most aligned with the normal, incident face = the other box's face
+// most anti-parallel to it, clipped against the reference face's side planes;
+// points below the reference face are seated on it and reduced to 4.
+template <typename R>
+__device__ __forceinline__ int clip_manifold(const R* ca, const R* Ra, const R* ha, const R* cb, const R* Rb,
+                                             const R* hb, const R* nrm, R depth, R* op, R* od) {
+    R a_align = R(-1), b_align = R(-1);
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        R dv = abs_(Ra[m] * nrm[0] + Ra[3 + m] * nrm[1] + Ra[6 + m] * nrm[2]);
+        if (dv > a_align) a_align = dv;
+        dv = abs_(Rb[m] * nrm[0] + Rb[3 + m] * nrm[1] + Rb[6 + m] * nrm[2]);
+        if (dv > b_align) b_align = dv;
+    }
+    const bool flip = !(a_align >= b_align);
+    R rc[3], rR[9], rh[3], ic[3], iR[9], ih[3], rn[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        rc[r] = flip ? cb[r] : ca[r]; rh[r] = flip ? hb[r] : ha[r];
+        ic[r] = flip ? ca[r] : cb[r]; ih[r] = flip ? ha[r] : hb[r];
+        rn[r] = flip ? -nrm[r] : nrm[r];
+    }
+#pragma unroll
+    for (int r = 0; r < 9; r++) { rR[r] = flip ? Rb[r] : Ra[r]; iR[r] = flip ? Ra[r] : Rb[r]; }
+    R isign = R(1), rsign = R(1);
+    const int iaxis = face_axis(iR, -rn[0], -rn[1], -rn[2], isign);
+    R v[12], refv[12];
+    face_vertices(ic, iR, ih, iaxis, isign, v);
+    const int raxis = face_axis(rR, rn[0], rn[1], rn[2], rsign);
+    face_vertices(rc, rR, rh, raxis, rsign, refv);
+    Poly8<R> A, B;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        A.x[i] = i < 4 ? v[3 * i] : R(0); A.y[i] = i < 4 ? v[3 * i + 1] : R(0); A.z[i] = i < 4 ? v[3 * i + 2] : R(0);
+        B.x[i] = R(0); B.y[i] = R(0); B.z[i] = R(0);
+    }
+    // side planes: the two axes other than raxis, + then - each
+    const int o0 = raxis == 0 ? 1 : 0, o1 = raxis == 2 ? 1 : 2;
+    bool over = false;
+    int np = 4;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int other = q < 2 ? o0 : o1;
+        const R s = (q & 1) ? R(-1) : R(1);
+        const R c0 = other == 0 ? rR[0] : (other == 1 ? rR[1] : rR[2]);
+        const R c1 = other == 0 ? rR[3] : (other == 1 ? rR[4] : rR[5]);
+        const R c2 = other == 0 ? rR[6] : (other == 1 ? rR[7] : rR[8]);
+        const R hw = other == 0 ? rh[0] : (other == 1 ? rh[1] : rh[2]);
+        const R p0 = c0 * s, p1 = c1 * s, p2 = c2 * s;
+        const R offset = (rc[0] * p0 + rc[1] * p1 + rc[2] * p2) + hw;
+        // an empty polygon stays empty (the reference stops clipping)
+        if (q & 1) np = clip_side8(B, np, p0, p1, p2, offset, A, over);
+        else np = clip_side8(A, np, p0, p1, p2, offset, B, over);
+    }
+    if (over) {
+        // copies that only this rare branch hands to the out-of-line path, so
+        // the caller's arrays stay in registers
+        R gca[3], gRa[9], gha[3], gcb[3], gRb[9], ghb[3], gn[3], gp[12], gd[4];
+#pragma unroll
+        for (int r = 0; r < 3; r++) { gca[r] = ca[r]; gha[r] = ha[r]; gcb[r] = cb[r]; ghb[r] = hb[r]; gn[r] = nrm[r]; }
+#pragma unroll
+        for (int r = 0; r < 9; r++) { gRa[r] = Ra[r]; gRb[r] = Rb[r]; }
+        const int n = clip_manifold_general(gca, gRa, gha, gcb, gRb, ghb, gn, depth, gp, gd);
+#pragma unroll
+        for (int r = 0; r < 12; r++) op[r] = gp[r];
+#pragma unroll
+        for (int r = 0; r < 4; r++) od[r] = gd[r];
+        return n;
+    }
+    // the polygon is in A (an even number of planes)
+    Poly8<R> C;
+    R dc[8];
+    int nc = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        C.x[i] = R(0); C.y[i] = R(0); C.z[i] = R(0); dc[i] = R(0);
+    }
+    sfor<0, 8>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        if (i < np) {
+            const R pen = (refv[0] - A.x[i]) * rn[0] + (refv[1] - A.y[i]) * rn[1] + (refv[2] - A.z[i]) * rn[2];
+            if (pen >= R(-1e-6)) {
+                const R d = pen > R(0) ? pen : R(0);
+                put8<i + 1>(C, nc, A.x[i] + rn[0] * pen, A.y[i] + rn[1] * pen, A.z[i] + rn[2] * pen);
+#pragma unroll
+                for (int s = 0; s <= i; s++) dc[s] = nc == s ? d : dc[s];
+                nc++;
+            }
+        }
+    });
+    if (nc == 0) {
+        for (int r = 0; r < 3; r++) op[r] = R(0.5) * (ca[r] + cb[r]);
+        od[0] = depth;
+        return 1;
+    }
+    return reduce8(C, dc, nc, op, od);
+}
+
 // _collide_native._box_box (_collide_native.pyx:20-76): 15 SAT axes generated
 // on the fly (A's faces, B's faces, the non-degenerate edge crosses), first
 // minimum overlap wins, sign from the centre offset.
 template <typename R>
-__device__ int box_box(const R* ca, const R* Ra, const R* ha, const R* cb, const R* Rb, const R* hb,
-                       R* normal, R* pts, R* deps) {
+__device__ __forceinline__ int box_box(const R* ca, const R* Ra, const R* ha, const R* cb, const R* Rb,
+                                       const R* hb, R* normal, R* pts, R* deps) {
     const R d0 = cb[0] - ca[0], d1 = cb[1] - ca[1], d2 = cb[2] - ca[2];
     R best = Lim<R>::big, b0 = R(0), b1 = R(0), b2 = R(0);
     bool sep = false;
@@ -259,7 +531,7 @@ __device__ int box_box(const R* ca, const R* Ra, const R* ha, const R* cb, const
 
 // _collide_native.sphere_sphere (_collide_native.pyx:285-299)
 template <typename R>
-__device__ int sphere_sphere(const R* ca, R ra, const R* cb, R rb, R* n, R* pt, R* dep) {
+__device__ __forceinline__ int sphere_sphere(const R* ca, R ra, const R* cb, R rb, R* n, R* pt, R* dep) {
     R dx = cb[0] - ca[0], dy = cb[1] - ca[1], dz = cb[2] - ca[2];
     R dist = sqrt(dx * dx + dy * dy + dz * dz);
     R r = ra + rb;
@@ -274,7 +546,8 @@ __device__ int sphere_sphere(const R* ca, R ra, const R* cb, R rb, R* n, R* pt, 
 
 // _collide_native.sphere_box (_collide_native.pyx:302-344); normal sphere -> box
 template <typename R>
-__device__ int sphere_box(const R* cs, R rs, const R* cb, const R* M, const R* hb, R* n, R* pt, R* dep) {
+__device__ __forceinline__ int sphere_box(const R* cs, R rs, const R* cb, const R* M, const R* hb, R* n, R* pt,
+                                          R* dep) {
     R sx = cs[0] - cb[0], sy = cs[1] - cb[1], sz = cs[2] - cb[2];
     R lx = M[0] * sx + M[3] * sy + M[6] * sz;
     R ly = M[1] * sx + M[4] * sy + M[7] * sz;
