@@ -19,7 +19,7 @@ void launch_begin(const Dev& d, hipStream_t s);
 void launch_pre(const Dev& d, int precision, int64_t nhw, double dt, double margin, hipStream_t s);
 void launch_refit(const Dev& d, int64_t nhw, double margin, hipStream_t s);
 void launch_sap_sort(const Dev& d, int c, const int32_t* wl, const int32_t* wn, int32_t np, int32_t block,
-                     hipStream_t s);
+                     int grid, hipStream_t s);
 void launch_sap_pairs(const Dev& d, int32_t record, int32_t grid, hipStream_t s);
 void launch_narrow(const Dev& d, int precision, hipStream_t s);
 void launch_compact(const Dev& d, int precision, int grid, hipStream_t s);
@@ -37,7 +37,7 @@ void launch_stage_spheres(int precision, int32_t np, int32_t mode, const double*
                           double* nrm, double* pt, double* dep, hipStream_t s);
 void launch_prepare(const Dev& d, int precision, SolveIn in, bool stage, hipStream_t s);
 void launch_solve_class(const Dev& d, int precision, int block, SolveIn in, const int32_t* wl, const int32_t* wn,
-                        int32_t lds_bytes, hipStream_t s);
+                        int32_t lds_bytes, int grid, hipStream_t s);
 size_t epa_work_bytes();
 bool allow_dynamic_lds(int bytes);
 void launch_stage_convex(int32_t np, const int32_t* ka, const double* ca, const double* Ra, const double* ha,
@@ -123,7 +123,7 @@ struct omi_batch {
     int32_t* d_wn = nullptr;               // [4] class counts
     int32_t np_class[4] = {256, 1024, 4096, 0};
     int32_t lds_bytes[4] = {0, 0, 0, 0};    // solver: dynamic LDS per world of each size class
-    int32_t live[4] = {0, 0, 0, 0};        // classes with a launch in the step
+    int32_t live[4] = {0, 0, 0, 0};        // grid of each class's per-world launches (0: none)
     int32_t sap_block[4] = {64, 128, 1024, 1024};
     int32_t solve_block[4] = {64, 128, 256, 256};
     int32_t np_lds_max = 4096;
@@ -582,7 +582,13 @@ static int32_t rebuild_lists(omi_batch* b) {
     for (int c = 0; c < 4; c++) {
         counts[c] = (int32_t)lists[c].size();
         if ((rc = put(b, b->d_wl + (size_t)c * b->cfg.max_worlds, lists[c].data(), counts[c]))) return rc;
-        if (counts[c] > 0 && !b->live[c]) { b->live[c] = 1; grew = true; }
+        // a class's launches take a grid of its worlds rounded up to a power of
+        // two, kept as a high-water mark so the captured graph changes rarely;
+        // blocks past the class's count exit at once, but each still needs a
+        // CU with the launch's LDS free
+        int32_t g = 1;
+        while (g < counts[c]) g <<= 1;
+        if (counts[c] > 0 && g > b->live[c]) { b->live[c] = g; grew = true; }
     }
     if ((rc = put(b, b->d_wn, counts, 4))) return rc;
     b->nhw = b->d.ncap;
@@ -622,7 +628,8 @@ static void fork_classes(omi_batch* b, F launch_class, G after_top) {
 static void enqueue_sap(omi_batch* b, int32_t record) {
     const Dev& d = b->d;
     fork_classes(b, [&](int c, hipStream_t sc) {
-        launch_sap_sort(d, c, b->d_wl + (size_t)c * d.wcap, b->d_wn + c, b->np_class[c], b->sap_block[c], sc);
+        launch_sap_sort(d, c, b->d_wl + (size_t)c * d.wcap, b->d_wn + c, b->np_class[c], b->sap_block[c],
+                        b->live[c], sc);
     }, [] {});
     launch_sap_pairs(d, record, b->sap_grid, b->stream);
 }
@@ -639,7 +646,7 @@ static void enqueue_solve(omi_batch* b, SolveIn in, bool stage, Mark mark) {
     mark();
     fork_classes(b, [&](int c, hipStream_t sc) {
         launch_solve_class(d, prec, b->solve_block[c], in, b->d_wl + (size_t)c * d.wcap, b->d_wn + c,
-                           b->lds_bytes[c], sc);
+                           b->lds_bytes[c], b->live[c], sc);
     }, mark);
 }
 

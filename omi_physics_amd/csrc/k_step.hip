@@ -835,8 +835,11 @@ __global__ __launch_bounds__(BLOCK) void k_wake(Dev d) {
 // angular terms in the reference's operation order. A sweep item loads only
 // the first SR_HOT values (everything but the seeds, which the level loop
 // reads from the impulse state), so fewer registers hold a prefetched item.
+// SR_TA / SR_TB: the split-impulse position terms ima dL and imb dL of the
+// contact (SR_TA = -1: no correction).
 enum { SR_RA = 0, SR_RB = 3, SR_T1 = 6, SR_T2 = 9, SR_NM = 12, SR_T1M = 13, SR_T2M = 14, SR_VB = 15,
-       SR_FR = 16, SR_IMA = 17, SR_IMB = 18, SR_NX = 19, SR_DEP = 22, SR_IMP = 23, SR_N = 26, SR_HOT = 24 };
+       SR_FR = 16, SR_IMA = 17, SR_IMB = 18, SR_NX = 19, SR_DEP = 22, SR_IMP = 23, SR_TA = 26, SR_TB = 27,
+       SR_HOT = 24 };
 // FP32 row: directions d = n, t1, t2 with the Jacobian terms folded in,
 // cA_d = rA x d, cB_d = rB x d, kA_d = IA (rA x d), kB_d = IB (rB x d), so a
 // sweep is dot products and scaled adds (FMA) on the same quantities.
@@ -1085,6 +1088,19 @@ struct SolveItem {
 static_assert(SR_STRIDE * sizeof(double) == SOLVE_ROW_BYTES && FR_STRIDE * sizeof(float) == SOLVE_ROW_BYTES,
               "solver rows fill SOLVE_ROW_BYTES in both precisions");
 
+// The split-impulse position terms of one contact (_solver_native.pyx:297-305):
+// dL = correction (depth - slop) / (ima + imb), ta = ima dL, tb = imb dL;
+// false (no correction) unless depth - slop > 0 and ima + imb > 1e-12.
+template <typename R>
+__device__ __forceinline__ bool pos_terms(R ima, R imb, R dep, const SolveIn& in, R& ta, R& tb) {
+    const R vn = dep - (R)in.slop;
+    if (!(vn > R(0) && ima + imb > R(1e-12))) return false;
+    const R dL = (R)in.correction * vn / (ima + imb);
+    ta = ima * dL;
+    tb = imb * dL;
+    return true;
+}
+
 // Solver rows for every contact of the step, one thread per contact
 // (_solver_native.pyx:179-256): lever arms, tangent basis, effective masses,
 // restitution bias, friction, seeds (warm-start cache, solver._seed_pair,
@@ -1142,7 +1158,9 @@ __global__ void k_prepare(Dev d, SolveIn in) {
             row[SR_T1] = tx; row[SR_T1 + 1] = ty; row[SR_T1 + 2] = tz;
             row[SR_T2] = ux; row[SR_T2 + 1] = uy; row[SR_T2 + 2] = uz;
             row[SR_NX] = nx; row[SR_NX + 1] = ny; row[SR_NX + 2] = nz;
-            row[SR_N] = R(0); row[SR_N + 1] = R(0);
+            R ta = R(-1), tb = R(0);
+            pos_terms<R>(ima, imb, (R)cdep[ci], in, ta, tb);
+            row[SR_TA] = ta; row[SR_TB] = tb;
         } else {
             const R dir[9] = {nx, ny, nz, tx, ty, tz, ux, uy, uz};
             for (int q = 0; q < 9; q++) row[FR_DIR + q] = dir[q];
@@ -1183,7 +1201,7 @@ __global__ void k_prepare(Dev d, SolveIn in) {
 // Level bounds kept in LDS per world (deeper schedules use global memory).
 constexpr int32_t SOLVE_DCAP = 1024;
 
-__device__ __forceinline__ size_t al16(size_t x) { return (x + 15) & ~(size_t)15; }
+__host__ __device__ __forceinline__ size_t al16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 // Where one world's solver state lives during the level loop.  In LDS
 // (byte offsets into the launch's dynamic LDS): the compact id of every body in
@@ -1222,13 +1240,17 @@ __device__ void scatter_items(int32_t* bound, int32_t SL, int32_t M, const int4*
 
 // The level loop of one world.  Each thread's first item of level l + 2 (code
 // and bodies) and of level l + 1 (row and inverse inertia) load while level l
-// runs.  LIGHT: velocities, positions and impulses are in LDS and a level
-// ends on an LDS barrier; otherwise on a full barrier.
-template <typename R, int BLOCK, bool LIGHT>
+// runs.  POS = false: the velocity passes (warm start and sweeps) over all D
+// levels; POS = true: the split-impulse position pass, which reads only the
+// contacts' depth, masses and normal and so runs on its own after the
+// velocity passes, over the levels holding pass-0 items (D is then that
+// count).  LIGHT: the state the mode updates (velocities and impulses, or
+// positions) is in LDS and a level ends on an LDS barrier; otherwise on a
+// full barrier.
+template <typename R, int BLOCK, bool LIGHT, bool POS>
 __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>& st, size_t bound_off, int32_t cs,
-                           int32_t K, int32_t D, int32_t D0, int32_t NC, int32_t S, bool lds_bounds,
-                           const int32_t* gbound, const int32_t* corder, const int32_t* cord_a,
-                           const int32_t* cord_b) {
+                           int32_t K, int32_t D, int32_t NC, int32_t S, bool lds_bounds, const int32_t* gbound,
+                           const int32_t* corder, const int32_t* cord_a, const int32_t* cord_b) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     const int tid = threadIdx.x;
     const bool fp64 = sizeof(R) == sizeof(double);
@@ -1257,33 +1279,47 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>&
         it.a = r.a; it.b = r.b;
         if (LIGHT) { it.cida = lmap[r.a - st.off]; it.cidb = lmap[r.b - st.off]; }
         load_vec<R, RowLayout<R>::hot>(&ROW(S_, it.ci, 0), it.row);
-        if constexpr (sizeof(R) == 8) {
+        if constexpr (sizeof(R) == 8 && !POS) {
             load_vec<R, 9>(&IIW(iiw, it.a, 0), it.Ia);
             load_vec<R, 9>(&IIW(iiw, it.b, 0), it.Ib);
         }
     };
-    auto pos_add = [&](int64_t i, int32_t cid, int r, R term, bool neg) {
-        // split-impulse correction of body i (global index, compact id cid), component r
-        if (LIGHT && st.pos_lds) {
-            R& p = lpos[(size_t)r * st.nd + (size_t)cid];
-            if (neg) p -= term; else p += term;
-        } else {
-            if (fp64) { if (neg) BODY(d.pos, r, i) -= (double)term; else BODY(d.pos, r, i) += (double)term; }
+    auto pos_add = [&](int64_t i, int32_t cid, int r, R term, bool neg, bool dyn) {
+        // split-impulse correction of body i (global index, compact id cid),
+        // component r.  FP64 positions are the state itself; FP32 accumulates
+        // the correction from zero and adds it to the FP64 position after.
+        R* pp = LIGHT ? &lpos[(size_t)r * st.nd + (size_t)cid] : nullptr;
+        if (!fp64) {
+            if (!dyn) return;
+            if (LIGHT) { if (neg) *pp -= term; else *pp += term; }
             else BODY(d.pos, r, i) = neg ? BODY(d.pos, r, i) - (double)term : BODY(d.pos, r, i) + (double)term;
+            return;
+        }
+        const double p = LIGHT ? (double)*pp : BODY(d.pos, r, i);
+        const double q = neg ? p - (double)term : p + (double)term;
+        if (dyn) {
+            if (LIGHT) *pp = (R)q; else BODY(d.pos, r, i) = q;
+        } else if (__double_as_longlong(q) != __double_as_longlong(p)) {
+            // a static or kinematic side (inverse mass 0, term +-0): the only
+            // change is -0.0 -> +0.0, written idempotently since several
+            // items of one level may touch the same such body
+            if (LIGHT) *pp = (R)q; else BODY(d.pos, r, i) = q;
         }
     };
     auto process = [&](const SolveItem<R>& it) {
         const int64_t a = it.a, b = it.b, ci = it.ci;
         const R ima = it.row[RowLayout<R>::ima], imb = it.row[RowLayout<R>::imb];
         const R* nv = it.row + RowLayout<R>::nrm;
-        if (it.s == 0) {
+        if constexpr (POS) {
             // split-impulse position pass (_solver_native.pyx:297-305)
+            if (it.s != 0) return;
             const R vn = it.row[RowLayout<R>::dep] - (R)in.slop;
             if (vn > R(0) && ima + imb > R(1e-12)) {
                 const R dL = (R)in.correction * vn / (ima + imb);
-                if (it.da) for (int r = 0; r < 3; r++) pos_add(a, it.cida, r, ima * dL * nv[r], true);
-                if (it.db) for (int r = 0; r < 3; r++) pos_add(b, it.cidb, r, imb * dL * nv[r], false);
+                for (int r = 0; r < 3; r++) pos_add(a, it.cida, r, ima * dL * nv[r], true, it.da);
+                for (int r = 0; r < 3; r++) pos_add(b, it.cidb, r, imb * dL * nv[r], false, it.db);
             }
+            return;
         }
         if (it.s >= S) return;
         if constexpr (sizeof(R) == 8) {
@@ -1415,9 +1451,7 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>&
             fetch_item(r, x);
             process(x);
         }
-        // position corrections in global memory end their levels (up to the
-        // last level holding a pass-0 contact) on a full barrier
-        if (LIGHT && (st.pos_lds || l >= D0)) {
+        if (LIGHT) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
             if (BLOCK > 64) __builtin_amdgcn_s_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
@@ -1432,6 +1466,207 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>&
     }
 #undef VEL
 #undef IMP
+}
+
+// LDS of the per-body position pass for a world of n bodies, K contacts and M
+// manifolds: list starts (n + 1), entries (2K), active contacts per manifold
+// (M), active flags (K bits) and, with rb = sizeof(R), the contacts' terms
+// and normals (5K of R).
+__host__ __device__ __forceinline__ size_t split_pos_bytes(int32_t K, int32_t M, int32_t n, size_t rb) {
+    return al16((size_t)(n + 1) * 4) + al16((size_t)K * 8) + al16((size_t)M) + al16((size_t)(K + 31) / 32 * 4) +
+           al16((size_t)K * 5 * rb);
+}
+
+// The split-impulse position pass (_solver_native.pyx:297-305) of one world,
+// by the block, in `lds` (split_pos_bytes(K, M, n, 0) bytes at least; the
+// contacts' terms and normals are staged there too when `staged`, else each
+// entry reads them from its row).  The pass reads only contact depth, masses
+// and normal, and bodies do not interact in it: each dynamic body takes the
+// corrections of its contacts in contact order.  So the block builds per-body
+// entry lists in that order, then every body folds its list in registers: as
+// many steps as the longest list, where a level schedule of the pass-0 items
+// would take the depth of the velocity pass.  The lists fill a manifold at a
+// time (its contacts are consecutive and share the pair): a wavefront of
+// manifolds per step, each lane ranked among the earlier lanes naming the
+// same body, weighted by their active contacts.  A static or kinematic side
+// (inverse mass 0, terms +-0) can only turn a -0.0 coordinate into +0.0; that
+// is written directly, idempotently.  FP32 accumulates the corrections from
+// zero and adds them to the FP64 position.
+template <typename R, int BLOCK>
+__device__ void split_positions(const Dev& d, const SolveIn& in, int32_t cs, int32_t K, int32_t M, int32_t n,
+                                int64_t off, const int4* mrec, unsigned char* lds, bool staged, int32_t* s_scan) {
+    constexpr int NW = BLOCK / 64;
+    constexpr bool fp64 = sizeof(R) == sizeof(double);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int32_t* cur = (int32_t*)lds;                                          // [n + 1]
+    unsigned char* q = lds + al16((size_t)(n + 1) * 4);
+    int32_t* ent = (int32_t*)q;                                            // [2K]: contact << 1 | side b
+    q += al16((size_t)K * 8);
+    uint8_t* nact = (uint8_t*)q;                                           // [M]
+    q += al16((size_t)M);
+    uint32_t* act = (uint32_t*)q;                                          // [K / 32]
+    q += al16((size_t)(K + 31) / 32 * 4);
+    R* T = (R*)q;                                                          // [5][K]: ima dL, imb dL, normal
+    const R* S_ = (const R*)d.csolv;
+    for (int32_t i = tid; i <= n; i += BLOCK) cur[i] = 0;
+    for (int32_t i = tid; i < (K + 31) / 32; i += BLOCK) act[i] = 0;
+    __syncthreads();
+    // contact k's terms (FP64: computed by k_prepare) and normal
+    auto terms = [&](int32_t k, R& ta, R& tb, R* nv) -> bool {
+        const int64_t ci = cs + k;
+        for (int r = 0; r < 3; r++) nv[r] = ROW(S_, ci, RowLayout<R>::nrm + r);
+        if constexpr (fp64) {
+            ta = ROW(S_, ci, SR_TA);
+            tb = ROW(S_, ci, SR_TB);
+            return ta >= R(0);
+        } else {
+            return pos_terms<R>(ROW(S_, ci, RowLayout<R>::ima), ROW(S_, ci, RowLayout<R>::imb),
+                                ROW(S_, ci, RowLayout<R>::dep), in, ta, tb);
+        }
+    };
+    // per contact: active flag, and its terms and normal when staged
+#pragma unroll 4
+    for (int32_t k = tid; k < K; k += BLOCK) {
+        R ta = R(0), tb = R(0), nv[3];
+        if (!terms(k, ta, tb, nv)) continue;
+        atomicOr(&act[k >> 5], 1u << (k & 31));
+        if (staged) {
+            T[k] = ta; T[K + k] = tb;
+            for (int r = 0; r < 3; r++) T[(size_t)(2 + r) * K + k] = nv[r];
+        }
+    }
+    __syncthreads();
+    auto active = [&](int32_t k) { return (act[k >> 5] >> (k & 31) & 1u) != 0; };
+    // per manifold: active contacts; entries per dynamic body; the -0.0 rule
+    // on the other sides
+    for (int32_t m = tid; m < M; m += BLOCK) {
+        const int4 r4 = mrec[m];
+        const int32_t c = r4.y & MR_COUNT, k0 = r4.x - cs;
+        int32_t na = 0;
+        for (int32_t j = 0; j < c; j++) {
+            const int32_t k = k0 + j;
+            if (!active(k)) continue;
+            na++;
+            if (fp64 && (r4.y & (MR_DYN_A | MR_DYN_B)) != (MR_DYN_A | MR_DYN_B)) {
+                R ta, tb, nv[3];
+                if (staged) {
+                    ta = T[k]; tb = T[K + k];
+                    for (int r = 0; r < 3; r++) nv[r] = T[(size_t)(2 + r) * K + k];
+                } else {
+                    terms(k, ta, tb, nv);
+                }
+                for (int side = 0; side < 2; side++) {
+                    if (r4.y & (side ? MR_DYN_B : MR_DYN_A)) continue;
+                    const int64_t x = off + (side ? r4.w : r4.z);
+                    const R t = side ? tb : ta;
+                    for (int r = 0; r < 3; r++) {
+                        const double p = BODY(d.pos, r, x);
+                        const double qv = side ? p + (double)(t * nv[r]) : p - (double)(t * nv[r]);
+                        if (__double_as_longlong(qv) != __double_as_longlong(p)) BODY(d.pos, r, x) = qv;
+                    }
+                }
+            }
+        }
+        nact[m] = (uint8_t)na;
+        if (na) {
+            if (r4.y & MR_DYN_A) atomicAdd(&cur[r4.z], na);
+            if (r4.y & MR_DYN_B) atomicAdd(&cur[r4.w], na);
+        }
+    }
+    __syncthreads();
+    const int32_t tot = block_exscan<BLOCK>(cur, n, s_scan);
+    if (tid == 0) cur[n] = tot;
+    __syncthreads();
+    // fill in contact order: a block of manifolds at a time.  Every
+    // wavefront ranks its lanes at once: a lane's peers (lanes naming its
+    // body on either side) come from one ballot per bit of the body index,
+    // its rank is their active contacts in the lanes before it (the counts
+    // 1..4 bit-sliced into three ballots).  Then the wavefronts in turn
+    // place their entries after the body's cursor and move it.
+    int32_t nbits = 1;
+    while ((1 << nbits) <= n) nbits++;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    int4 nxt = tid < M ? mrec[tid] : make_int4(cs, 0, 0, 0);
+    for (int32_t base = 0; base < M; base += BLOCK) {
+        const int32_t m = base + tid;
+        const int4 r4 = nxt;
+        nxt = m + BLOCK < M ? mrec[m + BLOCK] : make_int4(cs, 0, 0, 0);   // the next block's record
+        int32_t A = n, B = n, na = 0, k0 = 0, c = 0;
+        if (m < M) {
+            na = nact[m];
+            k0 = r4.x - cs;
+            c = r4.y & MR_COUNT;
+            if (na && (r4.y & MR_DYN_A)) A = r4.z;
+            if (na && (r4.y & MR_DYN_B)) B = r4.w;
+        }
+        uint64_t pAA = ~0ull, pAB = ~0ull, pBA = ~0ull, pBB = ~0ull;
+        for (int b = 0; b < nbits; b++) {
+            const bool xa = (A >> b) & 1, xb = (B >> b) & 1;
+            const uint64_t ba = __ballot(xa), bb = __ballot(xb);
+            pAA &= xa ? ba : ~ba; pAB &= xa ? bb : ~bb;
+            pBA &= xb ? ba : ~ba; pBB &= xb ? bb : ~bb;
+        }
+        const uint64_t w1 = __ballot(na & 1), w2 = __ballot(na & 2), w4 = __ballot(na & 4);
+        auto wsum = [&](uint64_t x) { return __popcll(x & w1) + 2 * __popcll(x & w2) + 4 * __popcll(x & w4); };
+        const int32_t ra = wsum(pAA & lt) + wsum(pAB & lt), ta = wsum(pAA) + wsum(pAB);
+        const int32_t rb = wsum(pBA & lt) + wsum(pBB & lt), tb = wsum(pBA) + wsum(pBB);
+        const bool ha = A < n, hb = B < n;
+        for (int w = 0; w < NW; w++) {
+            if (wv == w) {
+                const int32_t ca0 = ha ? cur[A] + ra : 0, cb0 = hb ? cur[B] + rb : 0;
+                int32_t jj = 0;
+                for (int32_t j = 0; j < c; j++) {
+                    const int32_t k = k0 + j;
+                    if (!active(k)) continue;
+                    if (ha) ent[ca0 + jj] = k << 1;
+                    if (hb) ent[cb0 + jj] = (k << 1) | 1;
+                    jj++;
+                }
+                // the lane with a body's last entries in the wavefront moves
+                // its cursor
+                __builtin_amdgcn_wave_barrier();
+                if (ha && ra + na == ta) cur[A] = ca0 + na;
+                if (hb && rb + na == tb) cur[B] = cb0 + na;
+            }
+            __syncthreads();
+        }
+    }
+    // cur[i] is now the end of body i's list, cur[i - 1] its start
+    for (int32_t i = tid; i < n; i += BLOCK) {
+        const int32_t e0 = i ? cur[i - 1] : 0, e1 = cur[i];
+        if (e0 == e1) continue;
+        const int64_t x = off + i;
+        double p[3] = {BODY(d.pos, 0, x), BODY(d.pos, 1, x), BODY(d.pos, 2, x)};
+        R acc[3] = {R(0), R(0), R(0)};
+        // each entry's term and normal, the next entry's loaded before this
+        // one is folded
+        auto fetch = [&](int32_t e, R& t, R* nv, bool& sb) {
+            const int32_t v = ent[e], k = v >> 1;
+            sb = v & 1;
+            if (staged) {
+                t = sb ? T[K + k] : T[k];
+                for (int r = 0; r < 3; r++) nv[r] = T[(size_t)(2 + r) * K + k];
+            } else {
+                R ta = R(0), tb = R(0);
+                terms(k, ta, tb, nv);
+                t = sb ? tb : ta;
+            }
+        };
+        R tn, nn[3];
+        bool sn;
+        fetch(e0, tn, nn, sn);
+        for (int32_t e = e0; e < e1; e++) {
+            const R t = tn, nv[3] = {nn[0], nn[1], nn[2]};
+            const bool sb = sn;
+            if (e + 1 < e1) fetch(e + 1, tn, nn, sn);
+            for (int r = 0; r < 3; r++) {
+                const R term = t * nv[r];
+                if (fp64) p[r] = sb ? p[r] + (double)term : p[r] - (double)term;
+                else acc[r] = sb ? acc[r] + term : acc[r] - term;
+            }
+        }
+        for (int r = 0; r < 3; r++) BODY(d.pos, r, x) = fp64 ? p[r] : p[r] + (double)acc[r];
+    }
 }
 
 template <typename R, int BLOCK>
@@ -1522,21 +1757,19 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     st.pos = st.imp + al16((size_t)K * 3 * RB);
     st.nd = nd;
     st.off = off;
+    // velocities and impulses (positions come later, in the same space)
     const bool state_lds = map_lds && st.pos <= avail;
-    st.pos_lds = state_lds && st.pos + al16((size_t)nd * 3 * RB) <= avail;
+    st.pos_lds = false;
     st.V = fp64 ? (R*)d.lv : (R*)d.svel; st.vs = d.ncap;
     st.P = S_ + RowLayout<R>::imp; st.ps = RowLayout<R>::stride; st.pc = 1; st.pb = 0;
 
-    // ---- stage velocities, positions, impulses
+    // ---- stage velocities, impulses
     if (state_lds) {
         R* V = (R*)(s_dyn + st.vel);
-        R* P = (R*)(s_dyn + st.pos);
         for (int32_t i = tid; i < n; i += BLOCK) {
             if (!touched(i)) continue;
             const int32_t c = cmap[i];
             for (int q = 0; q < 6; q++) V[(size_t)q * nd + c] = (R)BODY(d.lv, q, off + i);
-            if (st.pos_lds)
-                for (int q = 0; q < 3; q++) P[(size_t)q * nd + c] = fp64 ? (R)BODY(d.pos, q, off + i) : R(0);
         }
         R* I = (R*)(s_dyn + st.imp);
         for (int32_t k = tid; k < K; k += BLOCK)
@@ -1565,15 +1798,19 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     }
     __syncthreads();
     // bound[l] is now the end of level l
-    if (state_lds)
-        run_levels<R, BLOCK, true>(d, in, st, avail, cs, K, D, D0, NC, S, lds_bounds, g_bound, corder, cord_a, cord_b);
-    else
-        run_levels<R, BLOCK, false>(d, in, st, avail, cs, K, D, D0, NC, S, lds_bounds, g_bound, corder, cord_a, cord_b);
+    if (S > 0) {
+        if (state_lds)
+            run_levels<R, BLOCK, true, false>(d, in, st, avail, cs, K, D, NC, S, lds_bounds, g_bound, corder,
+                                              cord_a, cord_b);
+        else
+            run_levels<R, BLOCK, false, false>(d, in, st, avail, cs, K, D, NC, S, lds_bounds, g_bound, corder,
+                                               cord_a, cord_b);
+    }
     __syncthreads();
 
     // ---- results back: impulses to the contacts (warm-start cache and
     // readback); velocities to the FP64 state (copied in FP64, the FP32
-    // solver's change folded in for FP32); positions
+    // solver's change folded in for FP32)
     R* imp = (R*)d.cimp;
     for (int32_t k = tid; k < K; k += BLOCK) {
         const int64_t ci = cs + k;
@@ -1595,13 +1832,6 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
                     BODY(d.lv, q, off + i) = v0 + (double)dv;
                 }
             }
-            if (st.pos_lds) {
-                const R* P = (const R*)(s_dyn + st.pos);
-                for (int q = 0; q < 3; q++) {
-                    if (fp64) BODY(d.pos, q, off + i) = (double)P[(size_t)q * nd + c];
-                    else BODY(d.pos, q, off + i) = BODY(d.pos, q, off + i) + (double)P[(size_t)q * nd + c];
-                }
-            }
         } else if (!fp64) {
             for (int q = 0; q < 6; q++) {
                 const double v0 = BODY(d.lv, q, off + i);
@@ -1610,8 +1840,45 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
             }
         }
     }
+    // ---- the split-impulse position pass: per body, in the LDS the velocity
+    // state used, when its lists fit; else over the levels holding pass-0
+    // items, positions of the bodies in contact in LDS when they fit (FP64
+    // holds the positions themselves, FP32 the corrections from zero)
+    if (D0 > 0) {
+        __syncthreads();
+        if (split_pos_bytes(K, M, n, 0) <= avail) {
+            split_positions<R, BLOCK>(d, in, cs, K, M, n, off, mrec, s_dyn, split_pos_bytes(K, M, n, RB) <= avail,
+                                      s_scan);
+        } else {
+            st.pos = st.vel;
+            st.pos_lds = map_lds && st.pos + al16((size_t)nd * 3 * RB) <= avail;
+            if (st.pos_lds) {
+                R* P = (R*)(s_dyn + st.pos);
+                for (int32_t i = tid; i < n; i += BLOCK) {
+                    if (!touched(i)) continue;
+                    const int32_t c = cmap[i];
+                    for (int q = 0; q < 3; q++) P[(size_t)q * nd + c] = fp64 ? (R)BODY(d.pos, q, off + i) : R(0);
+                }
+                __syncthreads();
+                run_levels<R, BLOCK, true, true>(d, in, st, avail, cs, K, D0, NC, S, lds_bounds, g_bound, corder,
+                                                 cord_a, cord_b);
+                __syncthreads();
+                for (int32_t i = tid; i < n; i += BLOCK) {
+                    if (!touched(i)) continue;
+                    const int32_t c = cmap[i];
+                    for (int q = 0; q < 3; q++) {
+                        if (fp64) BODY(d.pos, q, off + i) = (double)P[(size_t)q * nd + c];
+                        else if (d.mt[off + i] == M_DYNAMIC)
+                            BODY(d.pos, q, off + i) = BODY(d.pos, q, off + i) + (double)P[(size_t)q * nd + c];
+                    }
+                }
+            } else {
+                run_levels<R, BLOCK, false, true>(d, in, st, avail, cs, K, D0, NC, S, lds_bounds, g_bound, corder,
+                                                  cord_a, cord_b);
+            }
+        }
+    }
 }
-
 
 // ----------------------------------------------------------------- joints --
 
@@ -2094,21 +2361,22 @@ void launch_refit(const Dev& d, int64_t nhw, double margin, hipStream_t s) {
 // scratch).  Each class's world list and count live in device memory, so a
 // captured graph stays valid as worlds come and go.
 template <int B>
-static void sap_sort_one(const Dev& d, const int32_t* wl, const int32_t* wn, int32_t np, bool global, hipStream_t s) {
-    if (global) hipLaunchKernelGGL((k_sap_sort<B, true>), dim3(d.wcap), dim3(B), 0, s, d, wl, wn, 0);
-    else hipLaunchKernelGGL((k_sap_sort<B, false>), dim3(d.wcap), dim3(B), (size_t)np * 16, s, d, wl, wn, np);
+static void sap_sort_one(const Dev& d, const int32_t* wl, const int32_t* wn, int32_t np, bool global, int grid,
+                         hipStream_t s) {
+    if (global) hipLaunchKernelGGL((k_sap_sort<B, true>), dim3(grid), dim3(B), 0, s, d, wl, wn, 0);
+    else hipLaunchKernelGGL((k_sap_sort<B, false>), dim3(grid), dim3(B), (size_t)np * 16, s, d, wl, wn, np);
 }
 
 // The broadphase sort of size class c (LDS for c < 3, global scratch above).
 void launch_sap_sort(const Dev& d, int c, const int32_t* wl, const int32_t* wn, int32_t np, int32_t block,
-                     hipStream_t s) {
+                     int grid, hipStream_t s) {
     const bool global = c == 3;
     switch (block) {
-        case 64: sap_sort_one<64>(d, wl, wn, np, global, s); break;
-        case 128: sap_sort_one<128>(d, wl, wn, np, global, s); break;
-        case 512: sap_sort_one<512>(d, wl, wn, np, global, s); break;
-        case 1024: sap_sort_one<1024>(d, wl, wn, np, global, s); break;
-        default: sap_sort_one<256>(d, wl, wn, np, global, s); break;
+        case 64: sap_sort_one<64>(d, wl, wn, np, global, grid, s); break;
+        case 128: sap_sort_one<128>(d, wl, wn, np, global, grid, s); break;
+        case 512: sap_sort_one<512>(d, wl, wn, np, global, grid, s); break;
+        case 1024: sap_sort_one<1024>(d, wl, wn, np, global, grid, s); break;
+        default: sap_sort_one<256>(d, wl, wn, np, global, grid, s); break;
     }
 }
 
@@ -2156,25 +2424,25 @@ void launch_wake(const Dev& d, hipStream_t s) {
 
 template <typename R, int B>
 static void solve_one(const Dev& d, SolveIn in, const int32_t* wl, const int32_t* wn, int32_t lds_bytes,
-                      hipStream_t s) {
-    hipLaunchKernelGGL((k_solve<R, B>), dim3(d.wcap), dim3(B), (size_t)lds_bytes, s, d, in, wl, wn, lds_bytes);
+                      int grid, hipStream_t s) {
+    hipLaunchKernelGGL((k_solve<R, B>), dim3(grid), dim3(B), (size_t)lds_bytes, s, d, in, wl, wn, lds_bytes);
 }
 
 template <typename R>
 static void solve_block(const Dev& d, int block, SolveIn in, const int32_t* wl, const int32_t* wn,
-                        int32_t lds_bytes, hipStream_t s) {
+                        int32_t lds_bytes, int grid, hipStream_t s) {
     switch (block) {
-        case 64: solve_one<R, 64>(d, in, wl, wn, lds_bytes, s); break;
-        case 128: solve_one<R, 128>(d, in, wl, wn, lds_bytes, s); break;
-        default: solve_one<R, 256>(d, in, wl, wn, lds_bytes, s); break;
+        case 64: solve_one<R, 64>(d, in, wl, wn, lds_bytes, grid, s); break;
+        case 128: solve_one<R, 128>(d, in, wl, wn, lds_bytes, grid, s); break;
+        default: solve_one<R, 256>(d, in, wl, wn, lds_bytes, grid, s); break;
         // the FP64 sweep holds two prefetched items in ~300 VGPRs per thread,
         // more than a 512-thread block leaves it: FP64 runs up to 256 threads
         // (the C-ABI rejects larger FP64 blocks)
         case 512:
-            if constexpr (sizeof(R) == 4) solve_one<R, 512>(d, in, wl, wn, lds_bytes, s);
+            if constexpr (sizeof(R) == 4) solve_one<R, 512>(d, in, wl, wn, lds_bytes, grid, s);
             break;
         case 1024:
-            if constexpr (sizeof(R) == 4) solve_one<R, 1024>(d, in, wl, wn, lds_bytes, s);
+            if constexpr (sizeof(R) == 4) solve_one<R, 1024>(d, in, wl, wn, lds_bytes, grid, s);
             break;
     }
 }
@@ -2220,9 +2488,9 @@ void launch_prepare(const Dev& d, int precision, SolveIn in, bool stage, hipStre
 // One block per world of size class c (world list wl, count *wn) with
 // lds_bytes of dynamic LDS (at least 4 * SOLVE_DCAP + 16, a multiple of 16).
 void launch_solve_class(const Dev& d, int precision, int block, SolveIn in, const int32_t* wl, const int32_t* wn,
-                        int32_t lds_bytes, hipStream_t s) {
-    if (precision == 64) solve_block<double>(d, block, in, wl, wn, lds_bytes, s);
-    else solve_block<float>(d, block, in, wl, wn, lds_bytes, s);
+                        int32_t lds_bytes, int grid, hipStream_t s) {
+    if (precision == 64) solve_block<double>(d, block, in, wl, wn, lds_bytes, grid, s);
+    else solve_block<float>(d, block, in, wl, wn, lds_bytes, grid, s);
 }
 
 void launch_cache(const Dev& d, int precision, hipStream_t s) {
