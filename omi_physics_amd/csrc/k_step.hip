@@ -833,13 +833,14 @@ __global__ __launch_bounds__(BLOCK) void k_wake(Dev d) {
 // Solver row layout per contact (R [kcap][stride]).
 // FP64 row: the reference's per-contact quantities; the sweep recomputes the
 // angular terms in the reference's operation order. A sweep item loads only
-// the first SR_HOT values (everything but the seeds, which the level loop
-// reads from the impulse state), so fewer registers hold a prefetched item.
-// SR_TA / SR_TB: the split-impulse position terms ima dL and imb dL of the
-// contact (SR_TA = -1: no correction).
-enum { SR_RA = 0, SR_RB = 3, SR_T1 = 6, SR_T2 = 9, SR_NM = 12, SR_T1M = 13, SR_T2M = 14, SR_VB = 15,
-       SR_FR = 16, SR_IMA = 17, SR_IMB = 18, SR_NX = 19, SR_DEP = 22, SR_IMP = 23, SR_TA = 26, SR_TB = 27,
-       SR_HOT = 24 };
+// the first SR_HOT values, so fewer registers hold a prefetched item: not the
+// seeds (the level loop reads the impulse state) nor the second tangent
+// (recomputed as n x t1, the expression that made it).  SR_TA / SR_TB: the
+// split-impulse position terms ima dL and imb dL of the contact (SR_TA = -1:
+// no correction).
+enum { SR_RA = 0, SR_RB = 3, SR_T1 = 6, SR_NX = 9, SR_NM = 12, SR_T1M = 13, SR_T2M = 14, SR_VB = 15,
+       SR_FR = 16, SR_IMA = 17, SR_IMB = 18, SR_DEP = 19, SR_T2 = 20, SR_IMP = 23, SR_TA = 26, SR_TB = 27,
+       SR_HOT = 20 };
 // FP32 row: directions d = n, t1, t2 with the Jacobian terms folded in,
 // cA_d = rA x d, cB_d = rB x d, kA_d = IA (rA x d), kB_d = IB (rB x d), so a
 // sweep is dot products and scaled adds (FMA) on the same quantities.
@@ -1079,11 +1080,11 @@ __device__ __forceinline__ int32_t item_code(int32_t k, bool da, bool db, int32_
 // bodies' world inverse inertia.
 template <typename R>
 struct SolveItem {
-    int64_t ci, a, b;
+    int32_t ci, a, b;               // contact, bodies (batch indices)
     int32_t s, cida, cidb;          // pass; compact ids of the sides (LDS state)
     bool da, db;
     R row[RowLayout<R>::hot];
-    R Ia[sizeof(R) == 8 ? 9 : 1], Ib[sizeof(R) == 8 ? 9 : 1];   // FP64 rows only: world inverse inertia
+    R Ia[sizeof(R) == 8 ? 9 : 1], Ib[sizeof(R) == 8 ? 9 : 1];   // FP64 rows only
 };
 static_assert(SR_STRIDE * sizeof(double) == SOLVE_ROW_BYTES && FR_STRIDE * sizeof(float) == SOLVE_ROW_BYTES,
               "solver rows fill SOLVE_ROW_BYTES in both precisions");
@@ -1327,7 +1328,11 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>&
             const R* rA = it.row + SR_RA;
             const R* rB = it.row + SR_RB;
             const R* t1 = it.row + SR_T1;
-            const R* t2 = it.row + SR_T2;
+            // t2 = n x t1 exactly as the row's preparation formed it
+            const R t2[3] = {nv[1] * t1[2] - nv[2] * t1[1], nv[2] * t1[0] - nv[0] * t1[2],
+                             nv[0] * t1[1] - nv[1] * t1[0]};
+            const R* Ia = it.Ia;
+            const R* Ib = it.Ib;
             R va[6], vb2[6];
             for (int c = 0; c < 6; c++) { va[c] = VEL(c, a, it.cida); vb2[c] = VEL(c, b, it.cidb); }
             if (it.s < in.warm) {
@@ -1336,7 +1341,7 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>&
                 const R P0 = n0 * nv[0] + u0 * t1[0] + w0 * t2[0];
                 const R P1 = n0 * nv[1] + u0 * t1[1] + w0 * t2[1];
                 const R P2 = n0 * nv[2] + u0 * t1[2] + w0 * t2[2];
-                apply_w(va, vb2, ima, imb, it.Ia, it.Ib, rA, rB, P0, P1, P2);
+                apply_w(va, vb2, ima, imb, Ia, Ib, rA, rB, P0, P1, P2);
             } else {
                 // one sweep over the contact (_solver_native.pyx:268-295)
                 R nI = IMP(0, ci), tI0 = IMP(1, ci), tI1 = IMP(2, ci);
@@ -1347,19 +1352,19 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>&
                 if (nw < R(0)) nw = R(0);
                 R dL = nw - nI;
                 nI = nw;
-                apply_w(va, vb2, ima, imb, it.Ia, it.Ib, rA, rB, dL * nv[0], dL * nv[1], dL * nv[2]);
+                apply_w(va, vb2, ima, imb, Ia, Ib, rA, rB, dL * nv[0], dL * nv[1], dL * nv[2]);
                 const R maxf = it.row[SR_FR] * nI;
                 relvel(va, vb2, rA, rB, dvx, dvy, dvz);
                 R vt = dvx * t1[0] + dvy * t1[1] + dvz * t1[2];
                 R nt = tI0 - it.row[SR_T1M] * vt;
                 if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
                 dL = nt - tI0; tI0 = nt;
-                apply_w(va, vb2, ima, imb, it.Ia, it.Ib, rA, rB, dL * t1[0], dL * t1[1], dL * t1[2]);
+                apply_w(va, vb2, ima, imb, Ia, Ib, rA, rB, dL * t1[0], dL * t1[1], dL * t1[2]);
                 vt = dvx * t2[0] + dvy * t2[1] + dvz * t2[2];
                 nt = tI1 - it.row[SR_T2M] * vt;
                 if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
                 dL = nt - tI1; tI1 = nt;
-                apply_w(va, vb2, ima, imb, it.Ia, it.Ib, rA, rB, dL * t2[0], dL * t2[1], dL * t2[2]);
+                apply_w(va, vb2, ima, imb, Ia, Ib, rA, rB, dL * t2[0], dL * t2[1], dL * t2[2]);
                 IMP(0, ci) = nI; IMP(1, ci) = tI0; IMP(2, ci) = tI1;
             }
             if (it.da) for (int c = 0; c < 6; c++) VEL(c, a, it.cida) = va[c];
@@ -1439,7 +1444,8 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>&
     fetch_id(tid, idA);
     if (tid < e_cur) fetch_item(idA, itA);
     fetch_id(e_cur + tid, idB);     // first items of level 1
-    auto level_step = [&](int32_t l, SolveItem<R>& cur, SolveItem<R>& nxt, const IdRec& idn, IdRec& id2) {
+    auto level_step = [&](int32_t l, SolveItem<R>& cur, SolveItem<R>& nxt, const IdRec& idn,
+                          IdRec& id2) {
         fetch_id(e_next + tid, id2);                                    // level l + 2
         const int32_t e_next2 = bound_at(l + 2);
         if (l + 1 < D && e_cur + tid < e_next) fetch_item(idn, nxt);   // level l + 1
@@ -1800,11 +1806,11 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     // bound[l] is now the end of level l
     if (S > 0) {
         if (state_lds)
-            run_levels<R, BLOCK, true, false>(d, in, st, avail, cs, K, D, NC, S, lds_bounds, g_bound, corder,
-                                              cord_a, cord_b);
+            run_levels<R, BLOCK, true, false>(d, in, st, avail, cs, K, D, NC, S, lds_bounds, g_bound,
+                                                  corder, cord_a, cord_b);
         else
-            run_levels<R, BLOCK, false, false>(d, in, st, avail, cs, K, D, NC, S, lds_bounds, g_bound, corder,
-                                               cord_a, cord_b);
+            run_levels<R, BLOCK, false, false>(d, in, st, avail, cs, K, D, NC, S, lds_bounds, g_bound,
+                                                   corder, cord_a, cord_b);
     }
     __syncthreads();
 
@@ -1860,8 +1866,8 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
                     for (int q = 0; q < 3; q++) P[(size_t)q * nd + c] = fp64 ? (R)BODY(d.pos, q, off + i) : R(0);
                 }
                 __syncthreads();
-                run_levels<R, BLOCK, true, true>(d, in, st, avail, cs, K, D0, NC, S, lds_bounds, g_bound, corder,
-                                                 cord_a, cord_b);
+                run_levels<R, BLOCK, true, true>(d, in, st, avail, cs, K, D0, NC, S, lds_bounds, g_bound,
+                                                     corder, cord_a, cord_b);
                 __syncthreads();
                 for (int32_t i = tid; i < n; i += BLOCK) {
                     if (!touched(i)) continue;
@@ -1873,8 +1879,8 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
                     }
                 }
             } else {
-                run_levels<R, BLOCK, false, true>(d, in, st, avail, cs, K, D0, NC, S, lds_bounds, g_bound, corder,
-                                                  cord_a, cord_b);
+                run_levels<R, BLOCK, false, true>(d, in, st, avail, cs, K, D0, NC, S, lds_bounds, g_bound,
+                                                      corder, cord_a, cord_b);
             }
         }
     }
