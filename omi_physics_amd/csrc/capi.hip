@@ -234,7 +234,7 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     {
         const int32_t full = (dyn_lds - 1024) & ~15;
         const bool f64 = cfg->precision == OMI_FP64;
-        const int32_t want[4] = {f64 ? 40 << 10 : 24 << 10, f64 ? 80 << 10 : 64 << 10, full, full};
+        const int32_t want[4] = {f64 ? 32 << 10 : 24 << 10, f64 ? 80 << 10 : 64 << 10, full, full};
         for (int c = 0; c < 4; c++) b->lds_bytes[c] = std::min(want[c], full);
     }
     b->solver_block = cfg->solver_block > 0 ? cfg->solver_block : 256;
