@@ -97,8 +97,9 @@ def run_batched(arrays, precision, steps, warmup, timing=True):
     """Warm up on one batch, then time ``steps`` steps of a fresh batch from the initial state.
 
     With ``timing`` the steps launch eagerly with a HIP event between kernels on
-    the batch's stream (events cannot be timed inside a captured graph); without
-    it every step is one graph launch.
+    each lane's stream (events cannot be timed inside a captured graph); without
+    it every step is one graph launch.  Stage times, counters and algorithmic
+    bytes come back per lane (one lane for a single batch).
     """
     from omi_physics_amd.batch import batch_from_arrays
     if warmup > 0:
@@ -115,9 +116,11 @@ def run_batched(arrays, precision, steps, warmup, timing=True):
     b.prepare(DT)
     b.sync()
     lat = []
-    ktot = {}
-    ctot = {}
-    per_kernel_bytes = {}
+    lanes = b.lanes if hasattr(b, 'lanes') else [b]
+    ktot = [{} for _ in lanes]
+    ctot = [{} for _ in lanes]
+    per_kernel_bytes = [{} for _ in lanes]
+    r = 4 if precision == 'fp32' else 8
     t_start = time.perf_counter()
     for s in range(steps):
         t = time.perf_counter()
@@ -125,18 +128,19 @@ def run_batched(arrays, precision, steps, warmup, timing=True):
         b.sync()
         lat.append(time.perf_counter() - t)
         if timing:
-            kt = b.kernel_times()
-            c = b.counters()
-            for k, v in kt.items():
-                ktot[k] = ktot.get(k, 0.0) + v
-            for k, v in c.items():
-                ctot[k] = ctot.get(k, 0) + v
-            r = 4 if precision == 'fp32' else 8
-            for k in ('pre', 'sap', 'narrow', 'solve'):
-                per_kernel_bytes[k] = per_kernel_bytes.get(k, 0) + algorithmic_bytes(k, c, r)
-            tb, top, ntop = solve_top_bytes(b.world_stats(), r)
-            per_kernel_bytes['solve_top'] = per_kernel_bytes.get('solve_top', 0) + tb
-            per_kernel_bytes['top_class'] = (top, ntop)
+            for li, lb in enumerate(lanes):
+                kt = lb.kernel_times()
+                c = lb.counters()
+                for k, v in kt.items():
+                    ktot[li][k] = ktot[li].get(k, 0.0) + v
+                for k, v in c.items():
+                    ctot[li][k] = ctot[li].get(k, 0) + v
+                kb = per_kernel_bytes[li]
+                for k in ('pre', 'sap', 'narrow', 'solve'):
+                    kb[k] = kb.get(k, 0) + algorithmic_bytes(k, c, r)
+                tb, top, ntop = solve_top_bytes(lb.world_stats(), r)
+                kb['solve_top'] = kb.get('solve_top', 0) + tb
+                kb['top_class'] = (top, ntop)
         if (s + 1) % 50 == 0:
             log('  %s step %d/%d  %.2f ms' % (precision, s + 1, steps, 1e3 * lat[-1]))
     wall = time.perf_counter() - t_start
@@ -384,28 +388,41 @@ def main():
         bt.close()
         log('timed pass: %.2f ms/step eager (graph %.2f), identical results: %s'
             % (1e3 * wall_t / args.steps, 1e3 * wall / args.steps, same))
-        ktot['solve'] = ktot['prepare'] + ktot['solve_top'] + ktot['solve_rest']
         out['eager_ms_per_step'] = 1e3 * wall_t / args.steps
-        out['contacts_per_s'] = ctot.get('contacts', 0) / wall
-        out['totals'] = {k: int(v) for k, v in ctot.items() if k not in ('overflow', 'max_levels')}
-        out['kernels_ms_per_step'] = {k: v / args.steps for k, v in ktot.items()}
-        top, ntop = kbytes['top_class']
-        avg_ms = ktot['solve_top'] / args.steps
-        per_launch = kbytes['solve_top'] / args.steps
+        tot = {}
+        for c in ctot:
+            for k, v in c.items():
+                if k not in ('overflow', 'max_levels'):
+                    tot[k] = tot.get(k, 0) + int(v)
+        out['contacts_per_s'] = tot.get('contacts', 0) / wall
+        out['totals'] = tot
+        # one lane per size class present (the worlds of each class are a member
+        # batch of one group; the lanes' steps run side by side)
+        names = []
+        for li, kt in enumerate(ktot):
+            kt['solve'] = kt['prepare'] + kt['solve_top'] + kt['solve_rest']
+            cls, nw = kbytes[li]['top_class']
+            names.append('lane %d: %d worlds of size class %d' % (li, nw, cls))
+        out['kernels_ms_per_step'] = {nm: {k: v / args.steps for k, v in kt.items()} for nm, kt in zip(names, ktot)}
+        crit = int(np.argmax([kt['solve_top'] for kt in ktot]))
+        top, ntop = kbytes[crit]['top_class']
+        avg_ms = ktot[crit]['solve_top'] / args.steps
+        per_launch = kbytes[crit]['solve_top'] / args.steps
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
         traffic = None
         tf = os.path.join(HERE, 'profiles', 'traffic.json')
         if os.path.exists(tf):
             with open(tf) as f:
                 traffic = json.load(f).get('k_solve_top_fp64_bytes_per_launch')
-        out['roofline'] = {'kernel': 'k_solve<double> size class %d (%d worlds, the largest; alone on the batch '
-                                     'stream)' % (top, ntop),
+        out['roofline'] = {'kernel': 'k_solve<double> size class %d (%d worlds, the largest; alone on its '
+                                     'lane stream)' % (top, ntop),
                            'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                            'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic, 'avg_ms': avg_ms,
                            'bytes_per_launch': per_launch}
-        out['roofline_stages'] = {k: {'avg_ms': ktot[k] / args.steps,
-                                      'GBps': kbytes[k] / max(ktot[k] * 1e-3, 1e-12) / 1e9}
-                                  for k in ('pre', 'sap', 'narrow', 'solve')}
+        out['roofline_stages'] = {nm: {k: {'avg_ms': kt[k] / args.steps,
+                                           'GBps': kb[k] / max(kt[k] * 1e-3, 1e-12) / 1e9}
+                                       for k in ('pre', 'sap', 'narrow', 'solve')}
+                                  for nm, kt, kb in zip(names, ktot, kbytes)}
 
     if not args.no_fp32:
         b32, lat32, wall32, *_ = run_batched(arrays, 'fp32', args.steps, 0, timing=False)

@@ -194,6 +194,16 @@ int32_t omi_batch_sync(omi_batch *b);
 int32_t omi_batch_read_world(omi_batch *b, int32_t slot, double *pos, double *quat, double *lv,
                              double *av, uint8_t *awake, double *sleep_timer, double *aabb_min,
                              double *aabb_max, double *prev_pos, double *prev_quat);
+/* Pose snapshot of the worlds slots[0..nslots), packed world after world:
+ * position [n][3], orientation as VRML axis-angle [n][4] (x, y, z, angle), awake
+ * [n] and dynamic [n] (motion type DYNAMIC), with n the listed worlds' total
+ * bodies and world slots[k] at rows offsets[k] .. offsets[k+1] (offsets holds
+ * nslots + 1 entries).  Pass NULL data pointers to size the buffers from the
+ * offsets first.  Replaces ThreadedSimulation._publish (threaded.py:149-158),
+ * quat_to_axis_angle (mathutil.py:90-105) computed on the device, one packed
+ * download for every listed world. */
+int32_t omi_batch_snapshot(omi_batch *b, int32_t nslots, const int32_t *slots, int64_t *offsets,
+                           double *pos, double *axis_angle, uint8_t *awake, uint8_t *dynamic);
 /* Contacts of one world's last step in solver order: count first (pass NULLs),
  * then copy up to `cap` of them. Body indices are world-local. */
 int32_t omi_batch_read_contacts(omi_batch *b, int32_t slot, int64_t cap, int64_t *count,
@@ -225,6 +235,29 @@ int32_t omi_batch_world_stats(omi_batch *b, int32_t n, int32_t *bodies, int32_t 
 int32_t omi_batch_set_blocks(omi_batch *b, const int32_t *sap_block, const int32_t *solve_block);
 /* Device bytes held by the batch. */
 int64_t omi_batch_device_bytes(const omi_batch *b);
+
+/* ---- groups: batches stepped together ---------------------------------
+ * Batch-layer scheduling with no counterpart in the reference (which steps one
+ * world at a time, world.py:410-430).  A group steps its member batches side
+ * by side: each step forks from the group's stream to every member's stream
+ * (one hardware queue each), launches the member's captured step graph there
+ * (its kernels eagerly when it times them) and joins back, so a member holding
+ * a few large worlds runs its solver while another member's chip-wide stages
+ * run over many small worlds.
+ * Results are those of stepping each member alone.  The group does not own
+ * its members: destroy the group before them.  Member reads, state changes
+ * and loads stay on the member's own calls and are ordered after the group's
+ * queued steps. */
+typedef struct omi_group omi_group;
+int32_t omi_group_create(omi_batch *const *members, int32_t n, omi_group **out);
+int32_t omi_group_destroy(omi_group *g);
+/* Capture the group's step graph for dt ahead of the first step. */
+int32_t omi_group_prepare(omi_group *g, double dt);
+/* Enqueue nsteps steps of dt for every member (asynchronous). */
+int32_t omi_group_step(omi_group *g, double dt, int32_t nsteps);
+/* Wait for the group's queued steps; reports the first member whose capacity
+ * overflowed (every member's flags are read and cleared). */
+int32_t omi_group_sync(omi_group *g);
 
 #ifdef __cplusplus
 }

@@ -67,10 +67,11 @@ EXPORTS = (
     'omi_collide_convex', 'omi_prepare_and_solve',
     'omi_batch_create', 'omi_batch_destroy', 'omi_batch_load_world', 'omi_batch_remove_world',
     'omi_batch_set_state', 'omi_batch_prepare', 'omi_batch_step', 'omi_batch_refit', 'omi_batch_sync',
-    'omi_batch_read_world', 'omi_batch_read_contacts', 'omi_batch_read_pairs',
+    'omi_batch_read_world', 'omi_batch_snapshot', 'omi_batch_read_contacts', 'omi_batch_read_pairs',
     'omi_batch_counters', 'omi_batch_set_timing', 'omi_batch_kernel_times',
     'omi_batch_kernel_names', 'omi_batch_use_graph', 'omi_batch_set_blocks', 'omi_batch_world_stats',
     'omi_batch_device_bytes',
+    'omi_group_create', 'omi_group_destroy', 'omi_group_prepare', 'omi_group_step', 'omi_group_sync',
 )
 
 _lib: Optional[ctypes.CDLL] = None
@@ -120,6 +121,7 @@ def load() -> ctypes.CDLL:
         'omi_batch_refit': (ctypes.c_int32, [_vp, ctypes.c_double]),
         'omi_batch_sync': (ctypes.c_int32, [_vp]),
         'omi_batch_read_world': (ctypes.c_int32, [_vp, ctypes.c_int32, _d, _d, _d, _d, _u8, _d, _d, _d, _d, _d]),
+        'omi_batch_snapshot': (ctypes.c_int32, [_vp, ctypes.c_int32, _i32, _i64, _d, _d, _u8, _u8]),
         'omi_batch_read_contacts': (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_int64, _i64, _i32, _i32,
                                                      _d, _d, _d, _d, _d]),
         'omi_batch_read_pairs': (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, _i64,
@@ -132,6 +134,11 @@ def load() -> ctypes.CDLL:
         'omi_batch_set_blocks': (ctypes.c_int32, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
         'omi_batch_world_stats': (ctypes.c_int32, [_vp, ctypes.c_int32] + [ctypes.POINTER(ctypes.c_int32)] * 5),
         'omi_batch_device_bytes': (ctypes.c_int64, [_vp]),
+        'omi_group_create': (ctypes.c_int32, [ctypes.POINTER(_vp), ctypes.c_int32, ctypes.POINTER(_vp)]),
+        'omi_group_destroy': (ctypes.c_int32, [_vp]),
+        'omi_group_prepare': (ctypes.c_int32, [_vp, ctypes.c_double]),
+        'omi_group_step': (ctypes.c_int32, [_vp, ctypes.c_double, ctypes.c_int32]),
+        'omi_group_sync': (ctypes.c_int32, [_vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

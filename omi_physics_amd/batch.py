@@ -4,7 +4,9 @@ One :class:`Batch` owns one ``omi_batch``: every loaded world's state stays in
 HBM, and :meth:`Batch.step` enqueues whole steps for all of them at once —
 per-body kernels over every body, one workgroup per world for the
 sweep-and-prune and the solver — captured in a HIP graph after the first step.
-A single world is simply a batch of one (the plain form).
+A single world is simply a batch of one (the plain form).  A
+:class:`BatchGroup` splits worlds of different size classes over member batches
+whose steps run side by side.
 """
 import ctypes
 from dataclasses import dataclass, field
@@ -14,6 +16,7 @@ import numpy as np
 
 from . import _ffi
 from ._ffi import _d, _i32, _i64, _u8, check, ptr
+from .autotune import size_class
 
 KERNELS = ('begin', 'pre', 'sap', 'narrow', 'compact', 'wake', 'prepare', 'solve_top', 'solve_rest', 'cache',
            'joints', 'post', 'sleep', 'toggle')
@@ -155,6 +158,23 @@ class Batch:
             ptr(out['prev_position'], _d), ptr(out['prev_orientation'], _d)))
         return out
 
+    def snapshot(self, slots: Optional[Sequence[int]] = None) -> List[tuple]:
+        """Pose snapshots ``(position (n,3), axis_angle (n,4), awake (n,), dynamic (n,))``
+        of ``slots`` (default: every loaded world), in the reference's snapshot format
+        (``threaded.py:16-21``), packed on the device and downloaded at once."""
+        slots = sorted(self.sizes) if slots is None else [int(s) for s in slots]
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        off = np.zeros(len(slots) + 1, np.int64)
+        check(self.lib.omi_batch_snapshot(self._h, len(slots), ptr(sl, _i32), ptr(off, _i64), None, None,
+                                          None, None))
+        n = int(off[-1])
+        pos, aa = np.empty((n, 3)), np.empty((n, 4))
+        aw, dy = np.empty(n, np.uint8), np.empty(n, np.uint8)
+        check(self.lib.omi_batch_snapshot(self._h, len(slots), ptr(sl, _i32), ptr(off, _i64), ptr(pos, _d),
+                                          ptr(aa, _d), ptr(aw, _u8), ptr(dy, _u8)))
+        return [(pos[a:b], aa[a:b], aw[a:b].astype(bool), dy[a:b].astype(bool))
+                for a, b in zip(off[:-1], off[1:])]
+
     def contacts(self, slot: int) -> Dict[str, np.ndarray]:
         """One world's last-step contacts in solver order (world-local indices)."""
         cnt = np.zeros(1, np.int64)
@@ -204,16 +224,170 @@ class Batch:
         return int(self.lib.omi_batch_device_bytes(self._h))
 
 
-def batch_from_arrays(worlds: Sequence[dict], precision: str = 'fp64', **kw) -> Batch:
-    """A batch holding ``worlds`` in slots 0..len-1, sized for them."""
+def _single_batch(worlds: Sequence[dict], precision: str, **kw) -> Batch:
     total = int(sum(int(w['n']) for w in worlds))
     hull = int(sum(len(np.asarray(w['hull_pts']).reshape(-1, 3)) for w in worlds))
     joints = int(sum(int(w.get('n_joints', 0)) for w in worlds))
-    b = Batch(max_worlds=len(worlds), max_bodies=max(total, 1), precision=precision,
+    b = Batch(max_worlds=max(len(worlds), 1), max_bodies=max(total, 1), precision=precision,
               max_hull_pts=max(hull, 1), max_joints=max(joints, 1), **kw)
     for i, w in enumerate(worlds):
         b.load(i, w)
     return b
+
+
+class BatchGroup:
+    """Worlds split over member batches (lanes) by size class and stepped together.
+
+    Each lane's step is an independent branch of one captured step (``omi_group_*``),
+    so the few largest worlds solve while the lanes of smaller worlds run their
+    chip-wide stages: a world's step no longer waits at stage boundaries shared with
+    every other world.  Worlds are independent, so every world's results equal those of
+    a single batch.  World ``i`` keeps index ``i`` in every method.
+    """
+
+    def __init__(self, worlds: Sequence[dict], lanes: Sequence[Sequence[int]], precision: str = 'fp64',
+                 **kw) -> None:
+        self.lib = _ffi.load()
+        self.precision = 'fp64' if precision in ('fp64', 64) else 'fp32'
+        self.lanes: List[Batch] = []
+        self.lane_worlds: List[List[int]] = [list(map(int, l)) for l in lanes if len(l)]
+        self.where: Dict[int, tuple] = {}
+        self._h = None
+        try:
+            for li, idx in enumerate(self.lane_worlds):
+                self.lanes.append(_single_batch([worlds[i] for i in idx], precision, **kw))
+                for slot, i in enumerate(idx):
+                    self.where[i] = (li, slot)
+            arr = (ctypes.c_void_p * len(self.lanes))(*[b._h.value for b in self.lanes])
+            h = ctypes.c_void_p()
+            check(self.lib.omi_group_create(arr, len(self.lanes), ctypes.byref(h)))
+            self._h = h
+        except Exception:
+            self.close()
+            raise
+        self.max_worlds = len(worlds)
+        self.sizes = {i: int(worlds[i]['n']) for i in range(len(worlds))}
+
+    def close(self) -> None:
+        if getattr(self, '_h', None):
+            self.lib.omi_group_destroy(self._h)
+            self._h = None
+        for b in getattr(self, 'lanes', []):
+            b.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _at(self, i: int):
+        li, slot = self.where[int(i)]
+        return self.lanes[li], slot
+
+    # -- stepping -----------------------------------------------------------
+    def step(self, dt: float, steps: int = 1) -> None:
+        check(self.lib.omi_group_step(self._h, float(dt), int(steps)))
+
+    def prepare(self, dt: float) -> None:
+        check(self.lib.omi_group_prepare(self._h, float(dt)))
+
+    def sync(self) -> None:
+        check(self.lib.omi_group_sync(self._h))
+
+    def refit(self, margin: float = 0.05) -> None:
+        for b in self.lanes:
+            b.refit(margin)
+
+    def use_graph(self, on: bool) -> None:
+        for b in self.lanes:
+            b.use_graph(on)
+
+    def set_blocks(self, sap_block: Sequence[int], solve_block: Sequence[int]) -> None:
+        for b in self.lanes:
+            b.set_blocks(sap_block, solve_block)
+
+    def set_timing(self, on: bool) -> None:
+        for b in self.lanes:
+            b.set_timing(on)
+
+    def lane_kernel_times(self) -> List[Dict[str, float]]:
+        """Per-stage device milliseconds of the last timed step, one dictionary per lane."""
+        return [b.kernel_times() for b in self.lanes]
+
+    # -- worlds -------------------------------------------------------------
+    def set_state(self, i: int, **state) -> None:
+        b, slot = self._at(i)
+        b.set_state(slot, **state)
+
+    def read(self, i: int) -> Dict[str, np.ndarray]:
+        b, slot = self._at(i)
+        return b.read(slot)
+
+    def snapshot(self, worlds: Optional[Sequence[int]] = None) -> List[tuple]:
+        """As :meth:`Batch.snapshot`, indexed by world (one packed download per lane)."""
+        worlds = list(range(self.max_worlds)) if worlds is None else [int(i) for i in worlds]
+        out: List[Optional[tuple]] = [None] * len(worlds)
+        for li, b in enumerate(self.lanes):
+            pick = [(k, self.where[i][1]) for k, i in enumerate(worlds) if self.where[i][0] == li]
+            if pick:
+                for (k, _), snap in zip(pick, b.snapshot([s for _, s in pick])):
+                    out[k] = snap
+        return out
+
+    def contacts(self, i: int) -> Dict[str, np.ndarray]:
+        b, slot = self._at(i)
+        return b.contacts(slot)
+
+    def pairs(self, i: int, solid: bool = True) -> np.ndarray:
+        b, slot = self._at(i)
+        return b.pairs(slot, solid)
+
+    def counters(self) -> Dict[str, int]:
+        """Totals of the last step over all lanes (``max_levels`` the largest)."""
+        out: Dict[str, int] = {}
+        for b in self.lanes:
+            for k, v in b.counters().items():
+                out[k] = max(out.get(k, 0), v) if k == 'max_levels' else (out.get(k, 0) | v if k == 'overflow'
+                                                                           else out.get(k, 0) + v)
+        return out
+
+    def lane_counters(self) -> List[Dict[str, int]]:
+        return [b.counters() for b in self.lanes]
+
+    def world_stats(self, n: Optional[int] = None) -> Dict[str, np.ndarray]:
+        """As :meth:`Batch.world_stats`, indexed by world."""
+        n = self.max_worlds if n is None else int(n)
+        keys = ('bodies', 'contacts', 'manifolds', 'levels', 'size_class')
+        out = {k: np.zeros(n, np.int32) for k in keys}
+        out['size_class'][:] = -1
+        for b, idx in zip(self.lanes, self.lane_worlds):
+            st = b.world_stats(len(idx))
+            sel = [(slot, i) for slot, i in enumerate(idx) if i < n]
+            if sel:
+                s_, i_ = np.array(sel).T
+                for k in keys:
+                    out[k][i_] = st[k][s_]
+        return out
+
+    def device_bytes(self) -> int:
+        return int(sum(b.device_bytes() for b in self.lanes))
+
+
+def batch_from_arrays(worlds: Sequence[dict], precision: str = 'fp64', lanes: Optional[bool] = None, **kw):
+    """A batch holding ``worlds`` as worlds 0..len-1, sized for them.
+
+    With ``lanes`` (default: whenever the worlds span more than one size class and no
+    explicit pair/contact capacity is given) the worlds are split by size class into
+    the lanes of a :class:`BatchGroup`; otherwise one :class:`Batch` holds them all.
+    """
+    classes = [size_class(int(w['n'])) for w in worlds]
+    if lanes is None:
+        lanes = len(set(classes)) > 1 and 'max_pairs' not in kw and 'max_contacts' not in kw
+    if not lanes:
+        return _single_batch(worlds, precision, **kw)
+    groups = [[i for i, c in enumerate(classes) if c == k] for k in (3, 2, 1, 0)]
+    return BatchGroup(worlds, [g for g in groups if g], precision=precision, **kw)
 
 
 def contact_list(c: Dict[str, np.ndarray]) -> List[Contact]:

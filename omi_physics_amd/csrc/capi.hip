@@ -29,6 +29,8 @@ void launch_post(const Dev& d, int64_t nhw, double dt, hipStream_t s);
 void launch_joints(const Dev& d, int precision, double dt, hipStream_t s);
 void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s);
 void launch_toggle(const Dev& d, hipStream_t s);
+void launch_snapshot(const Dev& d, const int32_t* wl, const int64_t* woff, int32_t nw, double* pos, double* aa,
+                     uint8_t* awake, uint8_t* dyn, hipStream_t s);
 void launch_stage_boxbox(int precision, int32_t np, const double* ca, const double* Ra, const double* ha,
                          const double* cb, const double* Rb, const double* hb, int32_t* cnt, double* nrm,
                          double* pts, double* dep, hipStream_t s);
@@ -137,6 +139,14 @@ struct omi_batch {
     std::vector<hipEvent_t> ev;
     std::vector<double> kms;
     int solver_block = 256;
+    // pose snapshots: packed device rows, their world list, pinned host copy
+    // (grown on demand, outside any captured step)
+    unsigned char* snap_dev = nullptr;
+    unsigned char* snap_host = nullptr;
+    int64_t snap_cap = 0;                  // bodies
+    int32_t* snap_wl = nullptr;
+    int64_t* snap_woff = nullptr;
+    int32_t snap_wcap = 0;
 
     template <typename T>
     int32_t alloc(T** p, int64_t count) {
@@ -177,6 +187,13 @@ static int32_t check_device() {
 
 static void free_all(omi_batch* b) {
     if (b->graph) hipGraphExecDestroy(b->graph);
+    if (b->snap_dev) hipFree(b->snap_dev);
+    if (b->snap_host) hipHostFree(b->snap_host);
+    if (b->snap_wl) hipFree(b->snap_wl);
+    if (b->snap_woff) hipFree(b->snap_woff);
+    b->snap_dev = b->snap_host = nullptr;
+    b->snap_wl = nullptr;
+    b->snap_woff = nullptr;
     for (auto e : b->ev) hipEventDestroy(e);
     for (void* p : b->allocs) hipFree(p);
     if (b->stream) hipStreamDestroy(b->stream);
@@ -321,10 +338,10 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
         HIPCHK(hipMemcpy(d.world, m1.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
     }
     HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    for (int q = 0; q < 3; q++) {
-        HIPCHK(hipStreamCreateWithFlags(&b->aux[q], hipStreamNonBlocking));
-        HIPCHK(hipEventCreateWithFlags(&b->join[q], hipEventDisableTiming));
-    }
+    // the size classes' branch streams are created once a second class holds
+    // worlds: a batch of one class (a member of a group, say) then holds a
+    // single stream, and so a single hardware queue
+    for (int q = 0; q < 3; q++) HIPCHK(hipEventCreateWithFlags(&b->join[q], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
     b->bodies.init(N);
     b->mats.init(TAB);
@@ -591,6 +608,10 @@ static int32_t rebuild_lists(omi_batch* b) {
         if (counts[c] > 0 && g > b->live[c]) { b->live[c] = g; grew = true; }
     }
     if ((rc = put(b, b->d_wn, counts, 4))) return rc;
+    int32_t nlive = 0;
+    for (int c = 0; c < 4; c++) nlive += b->live[c] > 0;
+    for (int q = 0; q + 1 < nlive; q++)
+        if (!b->aux[q]) HIPCHK(hipStreamCreateWithFlags(&b->aux[q], hipStreamNonBlocking));
     b->nhw = b->d.ncap;
     b->lists_dirty = false;
     if (grew) invalidate_graph(b);
@@ -798,6 +819,117 @@ extern "C" int32_t omi_batch_step(omi_batch* b, double dt, int32_t nsteps) {
     return OMI_OK;
 }
 
+// ---- groups: batches stepped together ---------------------------------
+//
+// Each member's step runs on the member's own stream (its captured step graph,
+// or its kernels launched eagerly when it times them), forked from the group's
+// stream and joined back at the end of the step.  The members' streams sit on
+// different hardware queues, so their kernels overlap on the device: a member
+// holding a few large worlds runs its long solver launch while another
+// member's chip-wide stages run over many small worlds, instead of every world
+// waiting at the same stage boundaries.
+
+struct omi_group {
+    std::vector<omi_batch*> m;
+    std::vector<hipEvent_t> join;          // per member
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr;
+};
+
+static void group_free(omi_group* g) {
+    for (auto e : g->join) if (e) hipEventDestroy(e);
+    if (g->fork) hipEventDestroy(g->fork);
+    if (g->stream) hipStreamDestroy(g->stream);
+}
+
+extern "C" int32_t omi_group_create(omi_batch* const* members, int32_t n, omi_group** out) {
+    if (!members || !out || n <= 0) return fail(OMI_ERR_ARG, "a group needs at least one batch");
+    for (int32_t i = 0; i < n; i++) {
+        if (!members[i]) return fail(OMI_ERR_ARG, "null member batch");
+        for (int32_t j = 0; j < i; j++)
+            if (members[j] == members[i]) return fail(OMI_ERR_ARG, "a batch appears twice in the group");
+    }
+    omi_group* g = new omi_group();
+    g->m.assign(members, members + n);
+    g->join.assign((size_t)n, nullptr);
+    hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g->fork, hipEventDisableTiming);
+    for (int32_t i = 0; i < n && e == hipSuccess; i++) e = hipEventCreateWithFlags(&g->join[i], hipEventDisableTiming);
+    if (e != hipSuccess) {
+        group_free(g);
+        delete g;
+        return fail(OMI_ERR_HIP, std::string("group streams/events: ") + hipGetErrorString(e));
+    }
+    *out = g;
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_group_destroy(omi_group* g) {
+    if (!g) return OMI_OK;
+    hipStreamSynchronize(g->stream);
+    group_free(g);
+    delete g;
+    return OMI_OK;
+}
+
+static int32_t group_ready(omi_group* g, double dt) {
+    int32_t rc;
+    for (omi_batch* b : g->m) {
+        if (b->lists_dirty && (rc = rebuild_lists(b))) return rc;
+        if (b->use_graph && !b->timing && (rc = ensure_graph(b, dt))) return rc;
+    }
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_group_prepare(omi_group* g, double dt) {
+    if (!g) return fail(OMI_ERR_ARG, "null group");
+    if (!(dt > 0.0)) return fail(OMI_ERR_ARG, "dt must be positive");
+    return group_ready(g, dt);
+}
+
+extern "C" int32_t omi_group_step(omi_group* g, double dt, int32_t nsteps) {
+    if (!g) return fail(OMI_ERR_ARG, "null group");
+    if (!(dt > 0.0)) return fail(OMI_ERR_ARG, "dt must be positive");
+    int32_t rc;
+    if ((rc = group_ready(g, dt))) return rc;
+    // the steps start after work already queued on the members' streams
+    // (uploads, state changes)
+    for (size_t i = 0; i < g->m.size(); i++) {
+        HIPCHK(hipEventRecord(g->join[i], g->m[i]->stream));
+        HIPCHK(hipStreamWaitEvent(g->stream, g->join[i], 0));
+    }
+    for (int32_t k = 0; k < nsteps; k++) {
+        HIPCHK(hipEventRecord(g->fork, g->stream));
+        for (size_t i = 0; i < g->m.size(); i++) {
+            omi_batch* b = g->m[i];
+            HIPCHK(hipStreamWaitEvent(b->stream, g->fork, 0));
+            if (b->use_graph && !b->timing) {
+                HIPCHK(hipGraphLaunch(b->graph, b->stream));
+            } else {
+                enqueue_step(b, dt);
+                HIPCHK(hipGetLastError());
+            }
+            HIPCHK(hipEventRecord(g->join[i], b->stream));
+            HIPCHK(hipStreamWaitEvent(g->stream, g->join[i], 0));
+        }
+    }
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_group_sync(omi_group* g) {
+    if (!g) return fail(OMI_ERR_ARG, "null group");
+    HIPCHK(hipStreamSynchronize(g->stream));
+    // every member's capacity flags, reporting the first member that overflowed
+    int32_t first = OMI_OK;
+    std::string msg;
+    for (omi_batch* b : g->m) {
+        int32_t rc = omi_batch_sync(b);
+        if (rc && first == OMI_OK) { first = rc; msg = g_err; }
+    }
+    if (first) return fail(first, msg);
+    return OMI_OK;
+}
+
 extern "C" int32_t omi_batch_refit(omi_batch* b, double margin) {
     if (!b) return fail(OMI_ERR_ARG, "null batch");
     int32_t rc;
@@ -857,6 +989,60 @@ extern "C" int32_t omi_batch_read_world(omi_batch* b, int32_t slot, double* pos,
     if (pquat && (rc = download_soa(b, pquat, d.pquat, d.ncap, s.off, s.n, 4))) return rc;
     if (awake && (rc = get(b, awake, d.awake + s.off, s.n))) return rc;
     if (timer && (rc = get(b, timer, d.timer + s.off, s.n))) return rc;
+    return OMI_OK;
+}
+
+// Bytes per snapshot row: position 3 and axis-angle 4 doubles, awake and dynamic.
+static const int64_t kSnapRow = 7 * 8 + 2;
+
+extern "C" int32_t omi_batch_snapshot(omi_batch* b, int32_t nslots, const int32_t* slots, int64_t* offsets,
+                                      double* pos, double* axis_angle, uint8_t* awake, uint8_t* dynamic) {
+    if (!b || nslots < 0 || (nslots > 0 && (!slots || !offsets))) return fail(OMI_ERR_ARG, "bad arguments");
+    int64_t total = 0;
+    for (int32_t k = 0; k < nslots; k++) {
+        const int32_t w = slots[k];
+        if (w < 0 || w >= b->cfg.max_worlds || !b->slots[w].live) return fail(OMI_ERR_ARG, "bad world slot");
+        offsets[k] = total;
+        total += b->slots[w].n;
+    }
+    if (offsets) offsets[nslots] = total;
+    if (!pos && !axis_angle && !awake && !dynamic) return OMI_OK;
+    if (total == 0) return OMI_OK;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (total > b->snap_cap) {
+        if (b->snap_dev) { HIPCHK(hipFree(b->snap_dev)); b->snap_dev = nullptr; }
+        if (b->snap_host) { HIPCHK(hipHostFree(b->snap_host)); b->snap_host = nullptr; }
+        b->snap_cap = 0;
+        const int64_t cap = std::max<int64_t>(total, 1024);
+        HIPCHK(hipMalloc((void**)&b->snap_dev, (size_t)(cap * kSnapRow)));
+        HIPCHK(hipHostMalloc((void**)&b->snap_host, (size_t)(cap * kSnapRow), hipHostMallocDefault));
+        b->snap_cap = cap;
+    }
+    if (nslots > b->snap_wcap) {
+        if (b->snap_wl) { HIPCHK(hipFree(b->snap_wl)); b->snap_wl = nullptr; }
+        if (b->snap_woff) { HIPCHK(hipFree(b->snap_woff)); b->snap_woff = nullptr; }
+        b->snap_wcap = 0;
+        HIPCHK(hipMalloc((void**)&b->snap_wl, sizeof(int32_t) * (size_t)nslots));
+        HIPCHK(hipMalloc((void**)&b->snap_woff, sizeof(int64_t) * (size_t)nslots));
+        b->snap_wcap = nslots;
+    }
+    int32_t rc;
+    if ((rc = put(b, b->snap_wl, slots, nslots))) return rc;
+    if ((rc = put(b, b->snap_woff, offsets, nslots))) return rc;
+    // packed rows: [total][3] position, [total][4] axis-angle, [total] awake, [total] dynamic
+    double* dpos = (double*)b->snap_dev;
+    double* daa = dpos + 3 * total;
+    uint8_t* daw = (uint8_t*)(daa + 4 * total);
+    uint8_t* ddy = daw + total;
+    launch_snapshot(b->d, b->snap_wl, b->snap_woff, nslots, dpos, daa, daw, ddy, b->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(b->snap_host, b->snap_dev, (size_t)(total * kSnapRow), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const unsigned char* h = b->snap_host;
+    if (pos) std::memcpy(pos, h, sizeof(double) * 3 * total);
+    if (axis_angle) std::memcpy(axis_angle, h + sizeof(double) * 3 * total, sizeof(double) * 4 * total);
+    if (awake) std::memcpy(awake, h + sizeof(double) * 7 * total, (size_t)total);
+    if (dynamic) std::memcpy(dynamic, h + sizeof(double) * 7 * total + total, (size_t)total);
     return OMI_OK;
 }
 

@@ -2210,6 +2210,41 @@ __global__ void k_post(Dev d, int64_t nhw, double dt) {
     }
 }
 
+// -------------------------------------------------------------- snapshot --
+
+// ThreadedSimulation._publish (threaded.py:149-158): per body the position,
+// the orientation as VRML axis-angle by quat_to_axis_angle (mathutil.py:90-105,
+// with quat_normalize = q / max(|q|, 1e-12), the norm summed left to right),
+// awake, and motion type == DYNAMIC.  One block per listed world; world k's
+// bodies go to packed rows woff[k] .. woff[k] + n.
+__global__ void k_snapshot(Dev d, const int32_t* wl, const int64_t* woff, int32_t nw, double* pos, double* aa,
+                           uint8_t* awake, uint8_t* dyn) {
+    const int32_t k = blockIdx.x;
+    if (k >= nw) return;
+    const int32_t w = wl[k];
+    const int64_t off = d.w_off[w], o = woff[k];
+    const int32_t n = d.w_cnt[w];
+    for (int32_t j = threadIdx.x; j < n; j += blockDim.x) {
+        const int64_t i = off + j, r = o + j;
+        for (int c = 0; c < 3; c++) pos[3 * r + c] = BODY(d.pos, c, i);
+        const double qx = BODY(d.quat, 0, i), qy = BODY(d.quat, 1, i), qz = BODY(d.quat, 2, i),
+                     qw = BODY(d.quat, 3, i);
+        const double nn = maxd(sqrt(((qx * qx + qy * qy) + qz * qz) + qw * qw), 1e-12);
+        const double ux = qx / nn, uy = qy / nn, uz = qz / nn;
+        double cw = qw / nn;
+        cw = cw < -1.0 ? -1.0 : cw > 1.0 ? 1.0 : cw;
+        const double sn = sqrt(maxd(1.0 - cw * cw, 0.0));
+        const bool small = sn < 1e-9;
+        const double ss = small ? 1.0 : sn;
+        aa[4 * r + 0] = small ? 0.0 : ux / ss;
+        aa[4 * r + 1] = small ? 1.0 : uy / ss;
+        aa[4 * r + 2] = small ? 0.0 : uz / ss;
+        aa[4 * r + 3] = small ? 0.0 : 2.0 * acos(cw);
+        awake[r] = d.awake[i];
+        dyn[r] = d.mt[i] == M_DYNAMIC ? 1 : 0;
+    }
+}
+
 // ----------------------------------------------------------------- sleep --
 
 __device__ __forceinline__ int32_t uf_find(int32_t* par, int32_t x) {
@@ -2523,6 +2558,12 @@ void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s) {
 }
 
 void launch_toggle(const Dev& d, hipStream_t s) { hipLaunchKernelGGL(k_toggle, dim3(1), dim3(64), 0, s, d); }
+
+void launch_snapshot(const Dev& d, const int32_t* wl, const int64_t* woff, int32_t nw, double* pos, double* aa,
+                     uint8_t* awake, uint8_t* dyn, hipStream_t s) {
+    if (nw <= 0) return;
+    hipLaunchKernelGGL(k_snapshot, dim3(nw), dim3(256), 0, s, d, wl, woff, nw, pos, aa, awake, dyn);
+}
 
 void launch_stage_boxbox(int precision, int32_t np, const double* ca, const double* Ra, const double* ha,
                          const double* cb, const double* Rb, const double* hb, int32_t* cnt, double* nrm,

@@ -105,6 +105,7 @@ def lib() -> ctypes.CDLL:
                                           ctypes.c_double, ctypes.c_double, _d, _d, ctypes.c_double]
         L.ow_integrate_positions.argtypes = [ctypes.c_int32, _i32, _u8, _d, _d, _d, _d, ctypes.c_double]
         L.ow_solve3.argtypes = [_d, _d, _d]
+        L.ow_snapshot.argtypes = [ctypes.c_int32, _d, _d, _u8, _i32, _d, _d, _u8, _u8]
         _lib = L
     return _lib
 
@@ -221,6 +222,16 @@ def quat_to_matrix(q) -> np.ndarray:
     q = _c(q, np.float64); R = np.empty(9)
     lib().ow_quat_to_matrix(_p(q, _d), _p(R, _d))
     return R.reshape(3, 3)
+
+
+def snapshot(pos, quat, awake, motion_type):
+    """``(position, axis_angle, awake, dynamic)`` of n bodies (the reference's snapshot)."""
+    pos = _c(pos, np.float64); quat = _c(quat, np.float64)
+    aw = _c(awake, np.uint8); mt = _c(motion_type, np.int32); n = len(pos)
+    op = np.empty((n, 3)); aa = np.empty((n, 4)); oa = np.empty(n, np.uint8); dy = np.empty(n, np.uint8)
+    lib().ow_snapshot(n, _p(pos, _d), _p(quat, _d), _p(aw, _u8), _p(mt, _i32), _p(op, _d), _p(aa, _d),
+                      _p(oa, _u8), _p(dy, _u8))
+    return op, aa, oa.astype(bool), dy.astype(bool)
 
 
 def refit(pos, quat, kind, half, margin=0.05, hull_off=None, hull_cnt=None, hull_pts=None):

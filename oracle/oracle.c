@@ -20,6 +20,35 @@ static double dmax(double a, double b) { return a > b ? a : b; }
 
 /* ---------------------------------------------------------------- math -- */
 
+/* ThreadedSimulation._publish (threaded.py:149-158): the pose snapshot of n
+ * bodies.  Orientation by quat_to_axis_angle (mathutil.py:90-105): normalise
+ * (norm added left to right, divided by max(norm, 1e-12)), clip w to [-1, 1],
+ * s = sqrt(max(1 - w w, 0)); s < 1e-9 gives (0, 1, 0, 0), else (q_xyz / s,
+ * 2 acos w).  dynamic = motion type DYNAMIC. */
+void ow_snapshot(int32_t n, const double *pos, const double *quat, const uint8_t *awake,
+                 const int32_t *motion_type, double *out_pos, double *axis_angle, uint8_t *out_awake,
+                 uint8_t *dynamic)
+{
+    for (int32_t i = 0; i < n; i++) {
+        const double *q = quat + 4 * i;
+        double nn = sqrt(((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + q[3] * q[3]);
+        double d = dmax(nn, 1e-12);
+        double x = q[0] / d, y = q[1] / d, z = q[2] / d, w = q[3] / d;
+        if (w < -1.0) w = -1.0;
+        if (w > 1.0) w = 1.0;
+        double s = sqrt(dmax(1.0 - w * w, 0.0));
+        double *o = axis_angle + 4 * i;
+        if (s < 1e-9) {
+            o[0] = 0.0; o[1] = 1.0; o[2] = 0.0; o[3] = 0.0;
+        } else {
+            o[0] = x / s; o[1] = y / s; o[2] = z / s; o[3] = 2.0 * acos(w);
+        }
+        for (int c = 0; c < 3; c++) out_pos[3 * i + c] = pos[3 * i + c];
+        out_awake[i] = awake[i];
+        dynamic[i] = motion_type[i] == M_DYNAMIC;
+    }
+}
+
 /* mathutil.quat_to_matrix (mathutil.py:59-76) after quat_normalize
  * (mathutil.py:21-27): norm adds the squares left to right. R row-major,
  * columns are the body axes. */

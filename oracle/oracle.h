@@ -74,6 +74,9 @@ void ow_counters(const OWorld *w, int64_t *out5);
 
 /* ---- stage-level restatements (each cites its reference function) ---- */
 void ow_quat_to_matrix(const double *q, double *R);
+void ow_snapshot(int32_t n, const double *pos, const double *quat, const uint8_t *awake,
+                 const int32_t *motion_type, double *out_pos, double *axis_angle, uint8_t *out_awake,
+                 uint8_t *dynamic);
 void ow_refit(int32_t n, const double *pos, const double *quat, const int32_t *kind,
               const double *half, const int32_t *hull_off, const int32_t *hull_cnt,
               const double *hull_pts, double margin, double *lo, double *hi);

@@ -255,6 +255,77 @@ def test_batch_equals_alone(lib):
             assert np.array_equal(x[key], y[key]), (i, key)
 
 
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+def test_group_lanes_equal_single_batch(lib, precision):
+    """Worlds of three size classes split into lanes (one member batch per class,
+    stepped side by side) end bit-identical to one batch holding them all: states,
+    contacts, per-world statistics and counters, in graph mode, in eager timed mode
+    and in graph mode again after it."""
+    sizes = ((40, 1), (600, 2), (1500, 3), (130, 4))
+    arrays = [scenes.arrays(scenes.boxes_on_ground(n, seed=s, spread=10.0, y_range=(0.6, 8.0),
+                                                   sphere_fraction=0.5)) for n, s in sizes]
+    say('group of lanes vs one batch', precision, [a['n'] for a in arrays])
+    grp = batch_from_arrays(arrays, precision=precision)
+    one = batch_from_arrays(arrays, precision=precision, lanes=False)
+    assert hasattr(grp, 'lanes') and len(grp.lanes) == 3 and not hasattr(one, 'lanes')
+    grp.prepare(DT)
+    for phase, timed in enumerate((False, True, False)):
+        grp.set_timing(timed); one.set_timing(timed)
+        grp.step(DT, 15); one.step(DT, 15)
+        grp.sync(); one.sync()
+        say('  phase', phase, 'timed' if timed else 'graph', grp.counters())
+        for i in range(len(arrays)):
+            x, y = grp.read(i), one.read(i)
+            for key in ('position', 'orientation', 'linear_velocity', 'angular_velocity', 'awake'):
+                assert np.array_equal(x[key], y[key]), (phase, i, key)
+            cx, cy = grp.contacts(i), one.contacts(i)
+            for key in ('a', 'b', 'normal', 'depth', 'normal_impulse'):
+                assert np.array_equal(cx[key], cy[key]), (phase, i, key)
+        sg, so = grp.world_stats(), one.world_stats()
+        for key in ('bodies', 'contacts', 'manifolds', 'levels', 'size_class'):
+            assert np.array_equal(sg[key], so[key]), (phase, key)
+        cg, co = grp.counters(), one.counters()
+        for key in ('bodies', 'x_candidates', 'broadphase_pairs', 'solid_pairs', 'manifolds', 'contacts',
+                    'max_levels'):
+            assert cg[key] == co[key], (phase, key)
+        if timed:
+            lt = grp.lane_kernel_times()
+            assert len(lt) == 3 and all(t['solve_top'] > 0 for t in lt)
+    assert grp.counters()['contacts'] > 0
+    for x, y in zip(grp.snapshot(), one.snapshot()):
+        for u, v in zip(x, y):
+            assert np.array_equal(u, v)
+    grp.close(); one.close()
+
+
+def test_snapshot_matches_oracle(lib):
+    """Pose snapshots packed on the device (threaded.py:149-158 format) against the
+    oracle's restatement on the same state.  Positions, awake and dynamic flags are
+    exact; the axis, a quotient of correctly rounded operations, within 1e-15; the
+    angle 2 acos(w) within 1e-14 rad (the device's acos against the host's, each
+    within an ulp or two).  A subset in another order gives the same rows."""
+    arrays = [helpers.world(helpers.pile(n, seed=s)).export_arrays() for n, s in ((40, 1), (90, 2), (15, 3))]
+    say('snapshot of 3 worlds after 40 steps')
+    b = batch_from_arrays(arrays)
+    b.step(DT, 40); b.sync()
+    snaps = b.snapshot()
+    sub = b.snapshot([2, 0])
+    assert len(snaps) == 3 and len(sub) == 2
+    for i, (p, aa, aw, dy) in enumerate(snaps):
+        st = b.read(i)
+        op, oaa, oaw, ody = oracle.snapshot(st['position'], st['orientation'], st['awake'],
+                                            arrays[i]['motion_type'])
+        assert p.shape == (arrays[i]['n'], 3) and aa.shape == (arrays[i]['n'], 4)
+        assert np.array_equal(p, op) and np.array_equal(aw, oaw) and np.array_equal(dy, ody)
+        assert np.abs(aa[:, :3] - oaa[:, :3]).max() <= 1e-15, i
+        assert np.abs(aa[:, 3] - oaa[:, 3]).max() <= 1e-14, i
+        assert dy.sum() == arrays[i]['n'] - 1                 # every body but the ground
+    for (k, i) in ((0, 2), (1, 0)):
+        for u, v in zip(sub[k], snaps[i]):
+            assert np.array_equal(u, v)
+    b.close()
+
+
 @pytest.mark.parametrize('kind', ['max_contacts', 'max_pairs'])
 def test_capacity_overflow_drops_whole_worlds(lib, kind):
     """A step that runs out of contact (or pair) capacity raises on sync; every world

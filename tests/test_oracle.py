@@ -229,3 +229,36 @@ def test_pendulum_keeps_its_length():
         ow.step(1.0 / 60.0)
         p = ow.state()['position'][0]
         assert abs(np.linalg.norm(p - [0.0, 5.0, 0.0]) - 1.0) < 0.02
+
+
+def test_snapshot_axis_angle_closed_forms():
+    """ow_snapshot against what the rotation fixes: a quaternion built from a known
+    unit axis and angle in [0.1, 2 pi - 0.1] returns that axis and angle (1e-12); any positive
+    scaling of the quaternion gives the same result; the identity (either sign) and a
+    rotation whose w rounds to 1 give the canonical (0, 1, 0, 0); a small rotation
+    keeps its axis; pi about
+    z is (0, 0, 1, pi); positions, awake flags and dynamic = motion type 2 pass
+    through."""
+    rng = np.random.default_rng(5)
+    n = 400
+    axis = rng.normal(size=(n, 3))
+    axis /= np.linalg.norm(axis, axis=1, keepdims=True)
+    # away from 0 and 2 pi, where sqrt(1 - w^2) cancels (error ~ eps / sin^2(angle / 2))
+    ang = rng.uniform(0.1, 2 * np.pi - 0.1, size=n)
+    q = np.concatenate([axis * np.sin(ang / 2)[:, None], np.cos(ang / 2)[:, None]], axis=1)
+    pos = rng.normal(size=(n, 3))
+    awake = (rng.uniform(size=n) < 0.5).astype(np.uint8)
+    mt = rng.integers(0, 3, size=n).astype(np.int32)
+    op, aa, oa, dy = oracle.snapshot(pos, q, awake, mt)
+    assert np.array_equal(op, pos) and np.array_equal(oa, awake.astype(bool)) and np.array_equal(dy, mt == 2)
+    assert np.abs(aa[:, :3] - axis).max() < 1e-12
+    assert np.abs(aa[:, 3] - ang).max() < 1e-12
+    _, aa2, _, _ = oracle.snapshot(pos, q * rng.uniform(0.2, 5.0, size=(n, 1)), awake, mt)
+    assert np.abs(aa2 - aa).max() < 1e-12
+    special = np.array([[0, 0, 0, 1.0], [0, 0, 0, -1.0], [1e-12, 0, 0, 1.0], [0, 0, 1.0, 0.0],
+                        [0, 1e-4, 0, 1.0]])
+    _, sa, _, _ = oracle.snapshot(np.zeros((5, 3)), special, np.ones(5, np.uint8), np.full(5, 2, np.int32))
+    assert np.array_equal(sa[:3], np.tile([0.0, 1.0, 0.0, 0.0], (3, 1)))
+    assert np.allclose(sa[3], [0, 0, 1, np.pi], atol=1e-15)
+    # a small rotation keeps its axis (sqrt(1 - w^2) carries ~1e-8 relative error here)
+    assert np.allclose(sa[4], [0, 1, 0, 2 * np.arctan2(1e-4, 1.0)], rtol=1e-6, atol=0)
