@@ -167,6 +167,9 @@ typedef struct {
     /* largest velocity_iterations of a world this batch will hold, 1..30
      * (0 = 16): sizes the solver's level schedule. */
     int32_t max_solver_iterations;
+    /* broadphase pairs with a trigger body per step, all worlds
+     * (0 = max(1024, max_pairs / 16)). */
+    int64_t max_trigger_pairs;
 } omi_batch_config;
 
 typedef struct omi_batch omi_batch;
@@ -204,6 +207,13 @@ int32_t omi_batch_read_world(omi_batch *b, int32_t slot, double *pos, double *qu
  * download for every listed world. */
 int32_t omi_batch_snapshot(omi_batch *b, int32_t nslots, const int32_t *slots, int64_t *offsets,
                            double *pos, double *axis_angle, uint8_t *awake, uint8_t *dynamic);
+/* Trigger overlaps of one world's last step (TriggerSystem.update's overlap
+ * set, triggers.py:24-70): pairs (trigger body, other body), world-local,
+ * sorted; count first (pass NULLs), then copy up to `cap`.  The enter / stay /
+ * exit diff against the previous step is the caller's (omi_physics_amd
+ * .triggers.TriggerSystem). */
+int32_t omi_batch_read_trigger_overlaps(omi_batch *b, int32_t slot, int64_t cap, int64_t *count,
+                                        int32_t *trigger, int32_t *other);
 /* Contacts of one world's last step in solver order: count first (pass NULLs),
  * then copy up to `cap` of them. Body indices are world-local. */
 int32_t omi_batch_read_contacts(omi_batch *b, int32_t slot, int64_t cap, int64_t *count,

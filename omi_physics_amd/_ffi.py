@@ -57,6 +57,7 @@ class BatchConfig(ctypes.Structure):
         ('record_broadphase', ctypes.c_int32), ('solver_block', ctypes.c_int32),
         ('sap_block', ctypes.c_int32 * 4), ('solve_block', ctypes.c_int32 * 4),
         ('max_candidates', ctypes.c_int64), ('max_solver_iterations', ctypes.c_int32),
+        ('max_trigger_pairs', ctypes.c_int64),
     ]
 
 
@@ -68,6 +69,7 @@ EXPORTS = (
     'omi_batch_create', 'omi_batch_destroy', 'omi_batch_load_world', 'omi_batch_remove_world',
     'omi_batch_set_state', 'omi_batch_prepare', 'omi_batch_step', 'omi_batch_refit', 'omi_batch_sync',
     'omi_batch_read_world', 'omi_batch_snapshot', 'omi_batch_read_contacts', 'omi_batch_read_pairs',
+    'omi_batch_read_trigger_overlaps',
     'omi_batch_counters', 'omi_batch_set_timing', 'omi_batch_kernel_times',
     'omi_batch_kernel_names', 'omi_batch_use_graph', 'omi_batch_set_blocks', 'omi_batch_world_stats',
     'omi_batch_device_bytes',
@@ -122,6 +124,7 @@ def load() -> ctypes.CDLL:
         'omi_batch_sync': (ctypes.c_int32, [_vp]),
         'omi_batch_read_world': (ctypes.c_int32, [_vp, ctypes.c_int32, _d, _d, _d, _d, _u8, _d, _d, _d, _d, _d]),
         'omi_batch_snapshot': (ctypes.c_int32, [_vp, ctypes.c_int32, _i32, _i64, _d, _d, _u8, _u8]),
+        'omi_batch_read_trigger_overlaps': (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_int64, _i64, _i32, _i32]),
         'omi_batch_read_contacts': (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_int64, _i64, _i32, _i32,
                                                      _d, _d, _d, _d, _d]),
         'omi_batch_read_pairs': (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, _i64,

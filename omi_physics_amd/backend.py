@@ -22,13 +22,17 @@ import numpy as np
 
 from . import _ffi
 from .batch import Batch, capacities_for, contact_list
+from .triggers import TriggerSystem
 
 
 class HipCollisionStages:
-    """``world._collision``: broadphase -> narrowphase -> solver, run on the device by the step."""
+    """``world._collision``: broadphase -> narrowphase -> solver -> trigger overlaps, run on the
+    device by the step; ``triggers`` keeps the overlap set across steps (reference
+    ``world.py:640-676``)."""
 
     def __init__(self, backend: 'HipBackend') -> None:
         self.backend = backend
+        self.triggers = TriggerSystem()
 
     def run(self, world: Any, dt: float) -> None:
         """Marks the collision stages for this step (they run in the fused device step)."""
@@ -47,6 +51,7 @@ class HipBackend:
         self._layout = -1
         self._stages = set()
         self._in_step = False
+        self._has_triggers = False
 
     # -- device state ---------------------------------------------------------
     def _ensure(self, world: Any) -> Batch:
@@ -63,6 +68,7 @@ class HipBackend:
                                 max_joints=max(1, int(arrays['n_joints'])), record_broadphase=True, **caps)
             if n:
                 self._batch.load(0, arrays)
+            self._has_triggers = bool(n and (np.asarray(arrays['trigger_shape']) >= 0).any())
             self._layout = world._layout_version
             world._host_dirty = False
         elif world._host_dirty and world._n:
@@ -115,6 +121,13 @@ class HipBackend:
         self._in_step = False
         if world._n:
             self._batch.step(dt, 1)
+            col = world._collision
+            if col is not None and (self._has_triggers or col.triggers.overlaps):
+                # trigger events need this step's overlap set: the one wait of a
+                # step with sensors (reference world.py:674-676)
+                self._batch.sync()
+                for event in col.triggers.update(map(tuple, self._batch.trigger_overlaps(0))):
+                    world.dispatch_trigger(*event)
 
     # -- host mirror --------------------------------------------------------------
     def download(self, world: Any) -> None:

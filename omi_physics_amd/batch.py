@@ -191,6 +191,18 @@ class Batch:
                 ptr(out['normal_impulse'], _d), ptr(out['tangent_impulse'], _d)))
         return out
 
+    def trigger_overlaps(self, slot: int) -> np.ndarray:
+        """One world's last-step trigger overlaps ``(k, 2)``: (trigger body, other body),
+        world-local, sorted."""
+        cnt = np.zeros(1, np.int64)
+        check(self.lib.omi_batch_read_trigger_overlaps(self._h, int(slot), 0, ptr(cnt, _i64), None, None))
+        k = int(cnt[0])
+        t, o = np.empty(k, np.int32), np.empty(k, np.int32)
+        if k:
+            check(self.lib.omi_batch_read_trigger_overlaps(self._h, int(slot), k, ptr(cnt, _i64), ptr(t, _i32),
+                                                           ptr(o, _i32)))
+        return np.stack([t, o], axis=1) if k else np.zeros((0, 2), np.int32)
+
     def pairs(self, slot: int, solid: bool = True) -> np.ndarray:
         """One world's last-step pairs ``(P, 2)`` (solid = narrowphase input)."""
         cnt = np.zeros(1, np.int64)
@@ -342,6 +354,10 @@ class BatchGroup:
     def pairs(self, i: int, solid: bool = True) -> np.ndarray:
         b, slot = self._at(i)
         return b.pairs(slot, solid)
+
+    def trigger_overlaps(self, i: int) -> np.ndarray:
+        b, slot = self._at(i)
+        return b.trigger_overlaps(slot)
 
     def counters(self) -> Dict[str, int]:
         """Totals of the last step over all lanes (``max_levels`` the largest)."""

@@ -29,6 +29,7 @@ void launch_post(const Dev& d, int64_t nhw, double dt, hipStream_t s);
 void launch_joints(const Dev& d, int precision, double dt, hipStream_t s);
 void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s);
 void launch_toggle(const Dev& d, hipStream_t s);
+void launch_trigger(const Dev& d, hipStream_t s);
 void launch_snapshot(const Dev& d, const int32_t* wl, const int64_t* woff, int32_t nw, double* pos, double* aa,
                      uint8_t* awake, uint8_t* dyn, hipStream_t s);
 void launch_stage_boxbox(int precision, int32_t np, const double* ca, const double* Ra, const double* ha,
@@ -120,6 +121,7 @@ struct omi_batch {
     int64_t bytes = 0;
     Ranges bodies, mats, filts, hulls, joints;
     bool any_joints = false;
+    bool any_triggers = false;
     std::vector<WorldSlot> slots;
     int32_t* d_wl = nullptr;               // [4][max_worlds] world slots per size class
     int32_t* d_wn = nullptr;               // [4] class counts
@@ -291,6 +293,8 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     const int64_t TAB = std::max<int64_t>(64 * (int64_t)W, 1024);
     A(d.mat_fric, TAB); A(d.mat_rest, TAB); A(d.filt_tab, TAB);
     A(d.pa, P); A(d.pb, P);
+    d.tpcap = cfg->max_trigger_pairs > 0 ? cfg->max_trigger_pairs : std::max<int64_t>(1024, P / 16);
+    A(d.tpa, d.tpcap); A(d.tpb, d.tpcap); A(d.tva, d.tpcap); A(d.tvb, d.tpcap);
     A(d.bpa, cfg->record_broadphase ? P : 1); A(d.bpb, cfg->record_broadphase ? P : 1);
     A(d.mcnt, P); A(d.mpts, 3 * 4 * P); A(d.cvx, P);
     d.pchunk_cap = (P + CPT_CHUNK - 1) / CPT_CHUNK + 1;
@@ -503,6 +507,7 @@ extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_wo
         mt[i] = (uint8_t)w->motion_type[i];
         kind[i] = (uint8_t)w->kind[i];
         bf[i] = (uint8_t)((w->collider_shape[i] >= 0 ? BF_COLLIDER : 0) | (w->trigger_shape[i] >= 0 ? BF_TRIGGER : 0));
+        if (w->trigger_shape[i] >= 0 && w->has_collision && !b->any_triggers) { b->any_triggers = true; invalidate_graph(b); }
         hoff[i] = w->kind[i] == 3 ? (int32_t)(s.hull_off + w->hull_off[i]) : 0;
         hcnt[i] = w->kind[i] == 3 ? w->hull_cnt[i] : 0;
     }
@@ -690,6 +695,7 @@ static void enqueue_step(omi_batch* b, double dt) {
     enqueue_solve(b, in, false, mark);
     mark();
     launch_cache(d, prec, s); mark();
+    if (b->any_triggers) launch_trigger(d, s);
     if (b->any_joints) launch_joints(d, prec, dt, s);
     mark();
     launch_post(d, b->nhw, dt, s); mark();
@@ -950,7 +956,7 @@ extern "C" int32_t omi_batch_sync(omi_batch* b) {
         char msg[320];
         snprintf(msg, sizeof msg,
                  "batch capacity exceeded in a step since the last sync (flags 0x%llx: 1 pairs, 2 contacts, "
-                 "4 cache, 8 broadphase record, 16 broadphase candidates); results of that step are invalid "
+                 "4 cache, 8 broadphase record, 16 broadphase candidates, 32 trigger pairs); results of that step are invalid "
                  "— recreate the batch with larger max_pairs/max_contacts/max_candidates",
                  (unsigned long long)ov);
         return fail(OMI_ERR_CAPACITY, msg);
@@ -989,6 +995,33 @@ extern "C" int32_t omi_batch_read_world(omi_batch* b, int32_t slot, double* pos,
     if (pquat && (rc = download_soa(b, pquat, d.pquat, d.ncap, s.off, s.n, 4))) return rc;
     if (awake && (rc = get(b, awake, d.awake + s.off, s.n))) return rc;
     if (timer && (rc = get(b, timer, d.timer + s.off, s.n))) return rc;
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_read_trigger_overlaps(omi_batch* b, int32_t slot, int64_t cap, int64_t* count,
+                                                   int32_t* trigger, int32_t* other) {
+    if (!b || slot < 0 || slot >= b->cfg.max_worlds || !b->slots[slot].live || !count)
+        return fail(OMI_ERR_ARG, "bad arguments");
+    const WorldSlot& s = b->slots[slot];
+    const Dev& d = b->d;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    uint64_t tot = 0;
+    int32_t rc;
+    if ((rc = get(b, &tot, d.cnt + C_TOVER, 1))) return rc;
+    const int64_t T = tot < (uint64_t)d.tpcap ? (int64_t)tot : d.tpcap;
+    std::vector<int32_t> ta((size_t)T), tb((size_t)T);
+    if ((rc = get(b, ta.data(), d.tva, T))) return rc;
+    if ((rc = get(b, tb.data(), d.tvb, T))) return rc;
+    std::vector<std::pair<int32_t, int32_t>> mine;
+    for (int64_t k = 0; k < T; k++)
+        if (ta[k] >= s.off && ta[k] < s.off + s.n)
+            mine.push_back({ta[k] - (int32_t)s.off, tb[k] - (int32_t)s.off});
+    std::sort(mine.begin(), mine.end());
+    *count = (int64_t)mine.size();
+    for (int64_t k = 0; k < cap && k < (int64_t)mine.size(); k++) {
+        if (trigger) trigger[k] = mine[k].first;
+        if (other) other[k] = mine[k].second;
+    }
     return OMI_OK;
 }
 

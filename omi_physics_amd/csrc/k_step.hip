@@ -412,12 +412,13 @@ __global__ __launch_bounds__(BLOCK) void k_sap_pairs(Dev d, int32_t record) {
                 if (k < 0 && m < 64) k = kb + m;
                 if (!__any(valid && k < 0)) break;
             }
-            bool bp = false, sol = false;
+            bool bp = false, sol = false, trig = false;
             int64_t p = 0, q = 0;
             if (valid) {
                 const int32_t j = k + 1 + (c - so[k]);
                 const SapRec A = sap_rec(d, off + k), B = sap_rec(d, off + j);
                 sap_test(d, A, B, fo, nf, bp, sol);
+                trig = bp && ((A.info | B.info) & SI_TRIGGER);
                 p = off + (A.orig < B.orig ? A.orig : B.orig);
                 q = off + (A.orig < B.orig ? B.orig : A.orig);
             }
@@ -431,6 +432,12 @@ __global__ __launch_bounds__(BLOCK) void k_sap_pairs(Dev d, int32_t record) {
                 if (record && bp) {
                     const int64_t o = out_b + cnt_b + __popcll(mb & below);
                     if (o < d.pcap) { d.bpa[o] = (int32_t)p; d.bpb[o] = (int32_t)q; }
+                }
+                // pairs with a trigger body (the sensor pairs of the step; an
+                // unordered list, they are few)
+                if (trig) {
+                    const uint64_t o = atomicAdd((unsigned long long*)&d.cnt[C_TPAIRS], 1ull);
+                    if (o < (uint64_t)d.tpcap) { d.tpa[o] = (int32_t)p; d.tpb[o] = (int32_t)q; }
                 }
             }
             cnt_s += __popcll(ms);
@@ -641,6 +648,47 @@ __global__ __launch_bounds__(64) void k_stage_convex(int32_t np, const int32_t* 
         cnt[i] = c;
         for (int r = 0; r < 3; r++) { nrm[3 * i + r] = n3[r]; pt[3 * i + r] = p3[r]; }
         dep[i] = dd;
+    }
+}
+
+// TriggerSystem.update's overlap test (triggers.py:24-70) over the step's
+// broadphase pairs with a trigger body, after the contact solve (the poses
+// _CollisionStages.run hands it, world.py:657-676).  _sensor_pair orders a
+// pair (i < j) as (trigger, other): i when i carries a trigger shape, else j;
+// an other body with neither collider nor trigger shape is skipped.
+// _overlapping collides the two shapes at their poses (collide.collide:
+// box-box, sphere-sphere, sphere-box either way round, GJK/EPA otherwise,
+// always FP64) and the pair overlaps when that yields a contact.  A body's
+// collider and trigger shapes are one shape here (the loader enforces it).
+// Overlaps go to an unordered list (trigger body, other body); the enter /
+// stay / exit diff against the previous step runs on the host.  One thread
+// per pair, each with its own EPA workspace slot when the batch holds hulls.
+__global__ __launch_bounds__(64) void k_trigger(Dev d, int32_t threads) {
+    const uint64_t tot = d.cnt[C_TPAIRS];
+    const int64_t T = tot < (uint64_t)d.tpcap ? (int64_t)tot : d.tpcap;
+    const int64_t slot = gtid();
+    if (slot == 0 && tot > (uint64_t)d.tpcap) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 32ull);
+    if (slot >= threads) return;
+    for (int64_t t = slot; t < T; t += threads) {
+        const int64_t i = d.tpa[t], j = d.tpb[t];
+        const bool ti = d.bflags[i] & BF_TRIGGER;
+        const int64_t trg = ti ? i : j, oth = ti ? j : i;
+        if (d.kind[oth] == K_NONE) continue;
+        Proxy A, B;
+        make_proxy(d, trg, A);
+        make_proxy(d, oth, B);
+        double nrm[3] = {0.0, 0.0, 0.0}, pts[12], dep[4];
+        int cnt;
+        if (A.kind == K_BOX && B.kind == K_BOX) cnt = box_box<double>(A.c, A.R, A.h, B.c, B.R, B.h, nrm, pts, dep);
+        else if (A.kind == K_SPHERE && B.kind == K_SPHERE) cnt = sphere_sphere<double>(A.c, A.h[0], B.c, B.h[0], nrm, pts, dep);
+        else if (A.kind == K_SPHERE && B.kind == K_BOX) cnt = sphere_box<double>(A.c, A.h[0], B.c, B.R, B.h, nrm, pts, dep);
+        else if (A.kind == K_BOX && B.kind == K_SPHERE) cnt = sphere_box<double>(B.c, B.h[0], A.c, A.R, A.h, nrm, pts, dep);
+        else if (d.epa_work) cnt = collide_convex(A, B, (EpaWork*)d.epa_work + slot, nrm, pts, dep[0]);
+        else cnt = 0;
+        if (cnt > 0) {
+            const uint64_t o = atomicAdd((unsigned long long*)&d.cnt[C_TOVER], 1ull);
+            if (o < (uint64_t)d.tpcap) { d.tva[o] = (int32_t)trg; d.tvb[o] = (int32_t)oth; }
+        }
     }
 }
 
@@ -2558,6 +2606,12 @@ void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s) {
 }
 
 void launch_toggle(const Dev& d, hipStream_t s) { hipLaunchKernelGGL(k_toggle, dim3(1), dim3(64), 0, s, d); }
+
+void launch_trigger(const Dev& d, hipStream_t s) {
+    // with hulls, one thread per EPA workspace slot; else a fixed grid
+    const int32_t threads = d.epa_work ? d.epa_slots : 4096;
+    hipLaunchKernelGGL(k_trigger, dim3((threads + 63) / 64), dim3(64), 0, s, d, threads);
+}
 
 void launch_snapshot(const Dev& d, const int32_t* wl, const int64_t* woff, int32_t nw, double* pos, double* aa,
                      uint8_t* awake, uint8_t* dyn, hipStream_t s) {

@@ -26,10 +26,12 @@ enum : int32_t {
     C_BPTOTAL = 5,     // broadphase pairs (all worlds)
     C_MANIFOLDS = 6,
     C_MAXLEVEL = 7,
-    C_OVERFLOW = 8,    // bit 0 pairs, 1 contacts, 2 cache, 3 broadphase record, 4 candidate tiles
+    C_OVERFLOW = 8,    // bit 0 pairs, 1 contacts, 2 cache, 3 broadphase record, 4 candidate tiles, 5 trigger pairs
     C_CONVEX = 9,      // convex pairs queued for GJK/EPA
     C_TILES = 10,      // broadphase candidate tiles reserved
     C_PVALID = 11,     // solid pairs of the worlds whose pair lists fit (a prefix of the pair list)
+    C_TPAIRS = 12,     // broadphase pairs with a trigger body (in no particular order)
+    C_TOVER = 13,      // trigger overlaps found
     C_COUNT = 16
 };
 
@@ -91,6 +93,9 @@ struct Dev {
     uint8_t *filt_tab;
     // pairs
     int32_t *pa, *pb, *bpa, *bpb;
+    int32_t *tpa, *tpb;                   // [tpcap] broadphase pairs with a trigger body
+    int32_t *tva, *tvb;                   // [tpcap] trigger overlaps: (trigger body, other body)
+    int64_t tpcap;
     int8_t *mcnt;
     int32_t *pfx;                         // [2][pcap + 1] first contact / manifold record of each pair
     int64_t *pchunk;                      // [2][pchunk_cap] contacts / manifolds per pair chunk -> offsets

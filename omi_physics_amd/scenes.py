@@ -217,11 +217,15 @@ def build(spec: Dict[str, Any], world_cls: Any, model: Any, **extra) -> Any:
         else:
             raise ValueError(s[0])
     for b in spec['bodies']:
+        # a body with 'trigger' carries its shape as a sensor (no collider)
+        sensor = bool(b.get('trigger', False))
         w.add_body(model.Motion(type=b['type'], mass=b.get('mass', 1.0),
                                 linearVelocity=tuple(b.get('linear_velocity', (0.0, 0.0, 0.0))),
                                 angularVelocity=tuple(b.get('angular_velocity', (0.0, 0.0, 0.0)))),
-                   collider=model.Collider(shape=shapes[b['shape']],
-                                           physicsMaterial=mats[b['material']] if b.get('material', -1) >= 0 else -1),
+                   collider=None if sensor else model.Collider(
+                       shape=shapes[b['shape']],
+                       physicsMaterial=mats[b['material']] if b.get('material', -1) >= 0 else -1),
+                   trigger=model.Trigger(shape=shapes[b['shape']]) if sensor else None,
                    position=tuple(float(x) for x in b['position']),
                    orientation=tuple(float(x) for x in b['orientation']))
     if spec.get('joints'):

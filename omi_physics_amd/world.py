@@ -80,6 +80,7 @@ class PhysicsWorld:
         self.time = 0.0
         self._alloc(8)
         self._collision: Any = None
+        self.trigger_listeners: List[Any] = []
         self.joint_constraints: List[Any] = []
         self.joint_iterations = 10
         self._layout_version = 0           # bumps when bodies/tables change
@@ -307,6 +308,16 @@ class PhysicsWorld:
         self._awake[i] = True
         self._sleep_timer[i] = 0.0
 
+    # -- triggers --------------------------------------------------------
+    def add_trigger_listener(self, fn: Any) -> None:
+        """Register ``fn(event_type, trigger_index, other_index)`` (reference ``world.py:580-582``)."""
+        self.trigger_listeners.append(fn)
+
+    def dispatch_trigger(self, event_type: str, trigger_index: int, other_index: int) -> None:
+        """Notify every registered trigger listener of one overlap event (reference ``world.py:584-588``)."""
+        for fn in self.trigger_listeners:
+            fn(event_type, trigger_index, other_index)
+
     def add_joint_constraint(self, constraint: Any) -> Any:
         """Register a ``joints.PointConstraint``, waking its bodies (reference ``world.py:590-597``)."""
         from .joints import PointConstraint
@@ -426,6 +437,10 @@ def export_world_arrays(w: PhysicsWorld) -> Dict[str, Any]:
         idx = int(cs[i]) if cs[i] >= 0 else int(ts[i])
         if idx < 0:
             continue
+        if cs[i] >= 0 and ts[i] >= 0 and cs[i] != ts[i]:
+            # one shape per body on the device: its collider's, which is also
+            # what the trigger overlap test uses
+            raise NotImplementedError('body %d has a collider and a different trigger shape' % i)
         sh = w.shapes[idx]
         if sh.type == 'sphere':
             kind[i] = KIND_SPHERE

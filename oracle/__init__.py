@@ -81,6 +81,9 @@ def lib() -> ctypes.CDLL:
         L.ow_num_pairs.argtypes = [ctypes.c_void_p, ctypes.c_int32]
         L.ow_get_pairs.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32]
         L.ow_counters.argtypes = [ctypes.c_void_p, _i64]
+        L.ow_num_trigger_overlaps.restype = ctypes.c_int64
+        L.ow_num_trigger_overlaps.argtypes = [ctypes.c_void_p]
+        L.ow_get_trigger_overlaps.argtypes = [ctypes.c_void_p, _i32]
         L.ow_quat_to_matrix.argtypes = [_d, _d]
         L.ow_refit.argtypes = [ctypes.c_int32, _d, _d, _i32, _d, _i32, _i32, _d, ctypes.c_double, _d, _d]
         L.ow_sap_pairs.restype = ctypes.c_int64
@@ -207,6 +210,15 @@ class OracleWorld:
         out = np.empty((P, 2), np.int32)
         if P:
             lib().ow_get_pairs(self._h, int(solid), _p(out, _i32))
+        return out
+
+    def trigger_overlaps(self) -> np.ndarray:
+        """The last step's trigger overlaps ``(k, 2)``: (trigger body, other body), sorted."""
+        k = int(lib().ow_num_trigger_overlaps(self._h))
+        out = np.empty((k, 2), np.int32)
+        if k:
+            lib().ow_get_trigger_overlaps(self._h, _p(out, _i32))
+            out = out[np.lexsort((out[:, 1], out[:, 0]))]
         return out
 
     def counters(self) -> Dict[str, int]:

@@ -1000,6 +1000,9 @@ struct OWorld {
     int64_t hcap;
     PJoint *joints;
     int32_t nj, jiters;
+    /* trigger overlaps of the last step: (trigger body, other body) */
+    int32_t *tov;
+    int64_t ntov, tov_cap;
 };
 
 static void *dupmem(const void *p, size_t sz) { void *q = malloc(sz ? sz : 1); if (p && sz) memcpy(q, p, sz); return q; }
@@ -1056,7 +1059,7 @@ void ow_destroy(OWorld *w)
                   w->timer, w->half, w->lo, w->hi, w->R, w->invIw, w->mt, w->cs, w->ts, w->kind,
                   w->hull_off, w->hull_cnt, w->mat, w->filt, w->awake, w->hull_pts, w->ftab_f, w->rtab,
                   w->ftab, w->pairs, w->solid, w->ca, w->cb, w->cp, w->cn, w->cd, w->cni, w->cti,
-                  w->cca, w->ccb, w->ccp, w->ccn, w->cct, w->hkey_start, w->joints};
+                  w->cca, w->ccb, w->ccp, w->ccn, w->cct, w->hkey_start, w->joints, w->tov};
     for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
     free(w);
 }
@@ -1210,6 +1213,60 @@ static void collide_pair(OWorld *w, int32_t a, int32_t b)   /* narrowphase.py:78
     }
     if (cnt > 0) w->nmanifold++;
     for (int i = 0; i < cnt; i++) push_contact(w, a, b, pts + 3 * i, nrm, dep[i]);
+}
+
+/* TriggerSystem.update's overlap test (triggers.py:24-70) for one broadphase
+ * pair (i < j) with a trigger body: _sensor_pair orders it as (trigger,
+ * other) -- i when i carries a trigger shape, else j -- and skips an other
+ * body with neither collider nor trigger shape; _overlapping collides the
+ * trigger's shape at its pose with the other's shape (collide.collide:
+ * box-box, sphere-sphere, sphere-box either way round, GJK/EPA otherwise)
+ * and overlaps when that yields a contact.  A body's collider and trigger
+ * shapes are one shape here (kind, half, hull), which the loader enforces.
+ * Poses are those after the contact solve, as in _CollisionStages.run. */
+static int trigger_overlap(OWorld *w, int32_t i, int32_t j, int32_t *trg, int32_t *oth)
+{
+    int32_t t = w->ts[i] >= 0 ? i : j, o = w->ts[i] >= 0 ? j : i;
+    *trg = t; *oth = o;
+    if (w->kind[o] == K_NONE) return 0;
+    int kt = w->kind[t], ko = w->kind[o];
+    double nrm[3], pts[12], dep[4];
+    const double *pt_ = w->pos + 3 * t, *po = w->pos + 3 * o;
+    const double *Rt = w->R + 9 * t, *Ro = w->R + 9 * o, *ht = w->half + 3 * t, *ho = w->half + 3 * o;
+    int cnt;
+    if (kt == K_BOX && ko == K_BOX) cnt = ow_box_box(pt_, Rt, ht, po, Ro, ho, nrm, pts, dep);
+    else if (kt == K_SPHERE && ko == K_SPHERE) cnt = ow_sphere_sphere(pt_, ht[0], po, ho[0], nrm, pts, dep);
+    else if (kt == K_SPHERE && ko == K_BOX) cnt = ow_sphere_box(pt_, ht[0], po, Ro, ho, nrm, pts, dep);
+    else if (kt == K_BOX && ko == K_SPHERE) cnt = ow_sphere_box(po, ho[0], pt_, Rt, ht, nrm, pts, dep);
+    else {
+        const double *pa = kt == K_CONVEX ? w->hull_pts + 3 * w->hull_off[t] : NULL;
+        const double *pb = ko == K_CONVEX ? w->hull_pts + 3 * w->hull_off[o] : NULL;
+        cnt = ow_collide_convex(kt, pt_, Rt, ht, pa, kt == K_CONVEX ? w->hull_cnt[t] : 0,
+                                ko, po, Ro, ho, pb, ko == K_CONVEX ? w->hull_cnt[o] : 0, nrm, pts, dep);
+    }
+    return cnt > 0;
+}
+
+static void find_trigger_overlaps(OWorld *w)
+{
+    w->ntov = 0;
+    for (int64_t p = 0; p < w->npairs; p++) {
+        int32_t i = w->pairs[2 * p], j = w->pairs[2 * p + 1], t, o;
+        if (w->ts[i] < 0 && w->ts[j] < 0) continue;
+        if (!trigger_overlap(w, i, j, &t, &o)) continue;
+        if (w->ntov == w->tov_cap) {
+            w->tov_cap = w->tov_cap ? 2 * w->tov_cap : 64;
+            w->tov = realloc(w->tov, sizeof(int32_t) * 2 * (size_t)w->tov_cap);
+        }
+        w->tov[2 * w->ntov] = t; w->tov[2 * w->ntov + 1] = o; w->ntov++;
+    }
+}
+
+int64_t ow_num_trigger_overlaps(const OWorld *w) { return w->ntov; }
+
+void ow_get_trigger_overlaps(const OWorld *w, int32_t *out)
+{
+    for (int64_t k = 0; k < 2 * w->ntov; k++) out[k] = w->tov[k];
 }
 
 void ow_solve3(const double *K, const double *rhs, double *x)
@@ -1417,6 +1474,9 @@ void ow_step(OWorld *w, double dt)
             store_cache(w);
             free(rest); free(fr); free(ai); free(bi);
         }
+        find_trigger_overlaps(w);
+    } else {
+        w->ntov = 0;
     }
     if (w->nj > 0) solve_joints(w, dt);
     ow_integrate_positions(n, w->mt, w->awake, w->lv, w->av, w->pos, w->quat, dt);

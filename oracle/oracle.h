@@ -71,6 +71,8 @@ int32_t ow_num_pairs(const OWorld *w, int32_t solid);
 void ow_get_pairs(const OWorld *w, int32_t solid, int32_t *pairs);
 /* counters of the last step: [pairs, solid pairs, x-candidates, manifolds, contacts] */
 void ow_counters(const OWorld *w, int64_t *out5);
+int64_t ow_num_trigger_overlaps(const OWorld *w);
+void ow_get_trigger_overlaps(const OWorld *w, int32_t *out);
 
 /* ---- stage-level restatements (each cites its reference function) ---- */
 void ow_quat_to_matrix(const double *q, double *R);

@@ -47,3 +47,21 @@ def sap_filter_table(filt):
 def pile(n=120, seed=3, spheres=0.5, spread=2.0, y=(0.6, 10.0)):
     """A dense pile that produces many contacts within a few dozen steps."""
     return scenes.boxes_on_ground(n, seed=seed, spread=spread, y_range=y, sphere_fraction=spheres)
+
+
+def trigger_scene(n=60, seed=4):
+    """A pile falling through sensors: static box and sphere trigger zones on and above
+    the ground, a rotated box zone, and a dynamic sphere that is itself a trigger."""
+    spec = scenes.boxes_on_ground(n, seed=seed, spread=3.0, y_range=(1.0, 9.0), sphere_fraction=0.5)
+    spec['shapes'] += [('box', (3.0, 2.0, 3.0)), ('sphere', 1.5), ('box', (6.0, 0.6, 1.2)), ('sphere', 0.4)]
+    zb, zs, zr, ts = range(len(spec['shapes']) - 4, len(spec['shapes']))
+    ident = (0.0, 0.0, 0.0, 1.0)
+    rot = tuple(mathutil.quat_from_axis_angle((0.3, 1.0, 0.2), 0.7))
+    spec['bodies'] += [
+        dict(type='static', shape=zb, position=(-1.0, 1.5, 0.5), orientation=ident, trigger=True),
+        dict(type='static', shape=zs, position=(1.5, 4.0, -1.0), orientation=ident, trigger=True),
+        dict(type='static', shape=zr, position=(0.0, 2.5, 0.0), orientation=rot, trigger=True),
+        dict(type='dynamic', shape=ts, position=(0.2, 6.0, 0.1), orientation=ident, trigger=True,
+             linear_velocity=(0.0, -1.0, 0.0)),
+    ]
+    return spec

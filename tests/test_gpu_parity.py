@@ -498,3 +498,60 @@ def test_physicsworld_pendulum_on_hip(lib):
     ow.step(1.0 / 120.0, 240)
     assert np.array_equal(np.asarray(w.position), ow.state()['position'])
     assert w.position[bob][1] < 2.0
+
+
+def test_trigger_overlaps_match_oracle(lib):
+    """Sensors on the device against the oracle, step by step: a pile falling through
+    static box, sphere and rotated-box trigger zones and past a dynamic sensor sphere.
+    The overlap sets (trigger body, other body) are identical every step, and the bodies'
+    states stay bit-identical (a sensor gives no impulse)."""
+    arrays = helpers.world(helpers.trigger_scene()).export_arrays()
+    say('trigger scene: %d bodies' % arrays['n'])
+    b = batch_from_arrays([arrays], precision='fp64')
+    ow = oracle.OracleWorld(arrays)
+    total = 0
+    kinds = set()
+    for s in range(90):
+        b.step(DT, 1); b.sync()
+        ow.step(DT, 1)
+        g, o = b.trigger_overlaps(0), ow.trigger_overlaps()
+        assert np.array_equal(g, o), (s, g.tolist(), o.tolist())
+        total += len(g)
+        kinds |= {int(t) for t in g[:, 0]}
+        if s % 10 == 9:
+            assert_state_equal(b, ow, 'trigger scene step %d' % s)
+    say('  overlaps over 90 steps:', total, 'sensors seen:', sorted(kinds))
+    assert total > 50 and len(kinds) >= 3
+    b.close()
+
+
+def test_physicsworld_trigger_events_on_hip(lib):
+    """The reference's trigger tests (tests/test_triggers.py) through PhysicsWorld on the
+    device: a ball crossing a box zone fires enter before exit and keeps its velocity
+    and line; a ball resting inside a zone gets enter, then stays."""
+    def scene(zone_size, r, pos, vel):
+        w = PhysicsWorld(gravity=model.Gravity(gravity=0.0))
+        zone = w.add_shape(model.Shape.box(zone_size))
+        w.add_body(model.Motion(type=model.STATIC), trigger=model.Trigger(shape=zone), position=(0, 0, 0))
+        s = w.add_shape(model.Shape.sphere(r))
+        ball = w.add_body(model.Motion(type=model.DYNAMIC, linearVelocity=vel), collider=model.Collider(shape=s),
+                          position=pos)
+        return w, ball
+    w, ball = scene((1, 1, 1), 0.2, (-3.0, 0, 0), (2.0, 0, 0))
+    events = []
+    w.add_trigger_listener(lambda kind, t, o: events.append((kind, t, o)))
+    for _ in range(120):
+        w.step(DT)
+    kinds = [e[0] for e in events]
+    say('crossing:', kinds.count('enter'), 'enter', kinds.count('stay'), 'stay', kinds.count('exit'), 'exit')
+    assert 'enter' in kinds and 'exit' in kinds and kinds.index('enter') < kinds.index('exit')
+    assert all(t == 0 and o == ball for _, t, o in events)
+    assert np.allclose(w.linear_velocity[ball], (2.0, 0, 0))
+    assert w.position[ball][1] == pytest.approx(0.0, abs=1e-9)
+    assert w.position[ball][2] == pytest.approx(0.0, abs=1e-9)
+    w2, _ = scene((2, 2, 2), 0.2, (0, 0, 0), (0, 0, 0))
+    seen = []
+    w2.add_trigger_listener(lambda kind, t, o: seen.append(kind))
+    for _ in range(5):
+        w2.step(DT)
+    assert seen[0] == 'enter' and seen.count('stay') >= 3

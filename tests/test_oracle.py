@@ -262,3 +262,85 @@ def test_snapshot_axis_angle_closed_forms():
     assert np.allclose(sa[3], [0, 0, 1, np.pi], atol=1e-15)
     # a small rotation keeps its axis (sqrt(1 - w^2) carries ~1e-8 relative error here)
     assert np.allclose(sa[4], [0, 1, 0, 2 * np.arctan2(1e-4, 1.0)], rtol=1e-6, atol=0)
+
+
+def _trigger_world(zone_shape, ball_radius, ball_pos, ball_vel):
+    """The reference's trigger scenarios (tests/test_triggers.py): zero gravity, a
+    static sensor at the origin, one dynamic ball."""
+    from omi_physics_amd import PhysicsWorld, model
+    w = PhysicsWorld(gravity=model.Gravity(gravity=0.0))
+    zone = w.add_shape(zone_shape)
+    w.add_body(model.Motion(type=model.STATIC), trigger=model.Trigger(shape=zone), position=(0, 0, 0))
+    ball = w.add_shape(model.Shape.sphere(ball_radius))
+    w.add_body(model.Motion(type=model.DYNAMIC, linearVelocity=ball_vel), collider=model.Collider(shape=ball),
+               position=ball_pos)
+    return w.export_arrays()
+
+
+def test_trigger_events_on_the_reference_scenarios():
+    """The oracle's trigger overlaps, diffed by TriggerSystem, on the reference's own
+    trigger tests: a ball crossing a unit box zone fires enter before exit and flies
+    straight through (velocity and y, z untouched); overlap holds exactly while the
+    ball's centre is within r of the box (|x| < 0.5 + 0.2, away from the boundary);
+    a ball sitting inside a zone gives enter then stays."""
+    from omi_physics_amd import model
+    from omi_physics_amd.triggers import TriggerSystem
+    a = _trigger_world(model.Shape.box((1, 1, 1)), 0.2, (-3.0, 0, 0), (2.0, 0, 0))
+    ow = oracle.OracleWorld(a)
+    ts = TriggerSystem()
+    events = []
+    for k in range(120):
+        x = ow.state()['position'][1, 0]           # the ball's centre when the step tests overlap
+        ow.step(1.0 / 60.0)
+        ov = ow.trigger_overlaps()
+        if abs(abs(x) - 0.7) > 1e-6:
+            assert (len(ov) == 1) == (abs(x) < 0.7), (k, x, ov)
+        if len(ov):
+            assert ov.tolist() == [[0, 1]]
+        events += [(k, e) for e in ts.update(map(tuple, ov))]
+    kinds = [e[0] for _, e in events]
+    assert 'enter' in kinds and 'exit' in kinds and kinds.index('enter') < kinds.index('exit')
+    assert kinds.count('enter') == 1 and kinds.count('exit') == 1
+    st = ow.state()
+    assert np.array_equal(st['linear_velocity'][1], [2.0, 0.0, 0.0])
+    assert st['position'][1, 1] == 0.0 and st['position'][1, 2] == 0.0
+    # a ball at rest inside a 2 m zone
+    b = _trigger_world(model.Shape.box((2, 2, 2)), 0.2, (0, 0, 0), (0, 0, 0))
+    ow = oracle.OracleWorld(b)
+    ts = TriggerSystem()
+    seen = []
+    for _ in range(5):
+        ow.step(1.0 / 60.0)
+        seen += [e[0] for e in ts.update(map(tuple, ow.trigger_overlaps()))]
+    assert seen[0] == 'enter' and seen.count('stay') >= 3
+
+
+def test_trigger_box_zone_closed_form():
+    """Axis-aligned unit box crossing a 2 x 2 x 2 box zone along x: the overlap holds
+    exactly while the centres are closer than the half extents' sum (1.5) on x."""
+    from omi_physics_amd import PhysicsWorld, model
+    w = PhysicsWorld(gravity=model.Gravity(gravity=0.0))
+    zone = w.add_shape(model.Shape.box((2, 2, 2)))
+    w.add_body(model.Motion(type=model.STATIC), trigger=model.Trigger(shape=zone), position=(0, 0.2, -0.1))
+    box = w.add_shape(model.Shape.box((1, 1, 1)))
+    w.add_body(model.Motion(type=model.DYNAMIC, linearVelocity=(1.5, 0, 0)), collider=model.Collider(shape=box),
+               position=(-3.0, 0, 0))
+    ow = oracle.OracleWorld(w.export_arrays())
+    hits = 0
+    for k in range(240):
+        x = ow.state()['position'][1, 0]
+        ow.step(1.0 / 60.0)
+        ov = ow.trigger_overlaps()
+        if abs(abs(x) - 1.5) > 1e-6:
+            assert (len(ov) == 1) == (abs(x) < 1.5), (k, x)
+        hits += len(ov)
+    assert hits > 100
+
+
+def test_trigger_system_diff():
+    from omi_physics_amd.triggers import TriggerSystem
+    ts = TriggerSystem()
+    assert ts.update([(3, 1), (0, 2)]) == [('enter', 0, 2), ('enter', 3, 1)]
+    assert ts.update([(0, 2), (0, 5)]) == [('enter', 0, 5), ('stay', 0, 2), ('exit', 3, 1)]
+    assert ts.update([]) == [('exit', 0, 2), ('exit', 0, 5)]
+    assert ts.update([]) == [] and ts.overlaps == set()
