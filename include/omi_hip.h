@@ -86,6 +86,17 @@ int32_t omi_sphere_box(int32_t npairs, int32_t precision, const double *cs, cons
                        const double *cb, const double *Rb, const double *hb, int32_t *count,
                        double *normal, double *point, double *depth);
 
+/* Replaces _collide_native.capsule_triangles (_collide_native.pyx:454-574),
+ * the character controller's query behind collide.capsule_triangle_batch
+ * (collide.py:438-462): the capsule segment p0 p1 with `radius` against n
+ * triangles verts[n][3][3] (triangle, vertex, axis).  Outputs for every
+ * triangle: hit[n] 0/1, points[n][3] (on the triangle), normals[n][3]
+ * (triangle -> capsule), depths[n]; zeros where hit is 0.  FP64, bit-exact
+ * with the Cython loop. */
+int32_t omi_capsule_triangles(int64_t n, const double *p0, const double *p1, double radius,
+                              const double *verts, uint8_t *hit, double *points, double *normals,
+                              double *depths);
+
 /* Replaces gjk.collide_convex (gjk.py:249-264) for one batch of pairs.
  * kind_* is OMI_KIND_SPHERE/BOX/CONVEX; hulls are [*_cnt] points starting at
  * *_off in hull_pts (local frame); R* [npairs][3][3]; half for boxes, half[0]

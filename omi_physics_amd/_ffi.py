@@ -65,7 +65,7 @@ class BatchConfig(ctypes.Structure):
 EXPORTS = (
     'omi_last_error', 'omi_abi_version', 'omi_device_count', 'omi_device_memory',
     'omi_refit', 'omi_sap_pairs', 'omi_box_box', 'omi_sphere_sphere', 'omi_sphere_box',
-    'omi_collide_convex', 'omi_prepare_and_solve',
+    'omi_collide_convex', 'omi_capsule_triangles', 'omi_prepare_and_solve',
     'omi_batch_create', 'omi_batch_destroy', 'omi_batch_load_world', 'omi_batch_remove_world',
     'omi_batch_set_state', 'omi_batch_prepare', 'omi_batch_step', 'omi_batch_refit', 'omi_batch_sync',
     'omi_batch_read_world', 'omi_batch_snapshot', 'omi_batch_read_contacts', 'omi_batch_read_pairs',
@@ -109,6 +109,7 @@ def load() -> ctypes.CDLL:
         'omi_collide_convex': (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int32, _i32, _d, _d, _d, _i32, _i32,
                                                 _i32, _d, _d, _d, _i32, _i32, _d, ctypes.c_int64,
                                                 _i32, _d, _d, _d]),
+        'omi_capsule_triangles': (ctypes.c_int32, [ctypes.c_int64, _d, _d, ctypes.c_double, _d, _u8, _d, _d, _d]),
         'omi_prepare_and_solve': (ctypes.c_int32, [ctypes.c_int64, ctypes.c_int32, _i64, _i64, _d, _d, _d, _d, _d,
                                                    _d, _d, _d, _d, _d, _d, _d, ctypes.c_double, ctypes.c_int32,
                                                    ctypes.c_int32, ctypes.c_double, ctypes.c_double,

@@ -30,6 +30,8 @@ void launch_joints(const Dev& d, int precision, double dt, hipStream_t s);
 void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s);
 void launch_toggle(const Dev& d, hipStream_t s);
 void launch_trigger(const Dev& d, hipStream_t s);
+void launch_capsule_triangles(int64_t n, const double* c0, const double* c1, double radius, const double* verts,
+                              uint8_t* hit, double* points, double* normals, double* depths, hipStream_t s);
 void launch_snapshot(const Dev& d, const int32_t* wl, const int64_t* woff, int32_t nw, double* pos, double* aa,
                      uint8_t* awake, uint8_t* dyn, hipStream_t s);
 void launch_stage_boxbox(int precision, int32_t np, const double* ca, const double* Ra, const double* ha,
@@ -1168,6 +1170,29 @@ struct DevBuf {
         return (T*)q;
     }
 };
+
+extern "C" int32_t omi_capsule_triangles(int64_t n, const double* p0, const double* p1, double radius,
+                                         const double* verts, uint8_t* hit, double* points, double* normals,
+                                         double* depths) {
+    if (n < 0 || (n > 0 && (!p0 || !p1 || !verts || !hit || !points || !normals || !depths)))
+        return fail(OMI_ERR_ARG, "bad arguments");
+    int32_t rc = check_device();
+    if (rc) return rc;
+    if (n == 0) return OMI_OK;
+    DevBuf m;
+    double *dc0 = m.up(p0, 3), *dc1 = m.up(p1, 3), *dv = m.up(verts, 9 * n);
+    uint8_t* dh = m.up<uint8_t>(nullptr, n);
+    double *dp = m.up<double>(nullptr, 3 * n), *dn = m.up<double>(nullptr, 3 * n), *dd = m.up<double>(nullptr, n);
+    if (!dd) return fail(OMI_ERR_HIP, "device allocation failed");
+    launch_capsule_triangles(n, dc0, dc1, radius, dv, dh, dp, dn, dd, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(hit, dh, (size_t)n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(points, dp, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(normals, dn, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(depths, dd, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return OMI_OK;
+}
 
 extern "C" int32_t omi_box_box(int32_t np, int32_t precision, const double* ca, const double* Ra, const double* ha,
                                const double* cb, const double* Rb, const double* hb, int32_t* count, double* normal,

@@ -13,6 +13,7 @@
 #include "omi_math.hpp"
 #include "omi_batch.hpp"
 #include "omi_gjk.hpp"
+#include "omi_capsule.hpp"
 
 namespace omi {
 
@@ -2606,6 +2607,31 @@ void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s) {
 }
 
 void launch_toggle(const Dev& d, hipStream_t s) { hipLaunchKernelGGL(k_toggle, dim3(1), dim3(64), 0, s, d); }
+
+// _collide_native.capsule_triangles for one capsule and n triangles
+// verts[n][3][3], one thread per triangle; non-contacts give zeros.
+__global__ void k_capsule_triangles(int64_t n, const double* c0, const double* c1, double radius,
+                                    const double* verts, uint8_t* hit, double* points, double* normals,
+                                    double* depths) {
+    __shared__ double cap[6];
+    if (threadIdx.x < 3) { cap[threadIdx.x] = c0[threadIdx.x]; cap[3 + threadIdx.x] = c1[threadIdx.x]; }
+    __syncthreads();
+    for (int64_t i = gtid(); i < n; i += gstride()) {
+        const double* v = verts + 9 * i;
+        double pt[3] = {0.0, 0.0, 0.0}, nr[3] = {0.0, 0.0, 0.0}, dp = 0.0;
+        const int h = capsule_triangle(cap, cap + 3, radius, v, v + 3, v + 6, pt, nr, dp);
+        hit[i] = (uint8_t)h;
+        for (int m = 0; m < 3; m++) { points[3 * i + m] = pt[m]; normals[3 * i + m] = nr[m]; }
+        depths[i] = dp;
+    }
+}
+
+void launch_capsule_triangles(int64_t n, const double* c0, const double* c1, double radius, const double* verts,
+                              uint8_t* hit, double* points, double* normals, double* depths, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_capsule_triangles, dim3(GRID_FOR(n, 256)), dim3(256), 0, s, n, c0, c1, radius, verts, hit,
+                       points, normals, depths);
+}
 
 void launch_trigger(const Dev& d, hipStream_t s) {
     // with hulls, one thread per EPA workspace slot; else a fixed grid

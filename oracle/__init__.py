@@ -109,6 +109,7 @@ def lib() -> ctypes.CDLL:
         L.ow_integrate_positions.argtypes = [ctypes.c_int32, _i32, _u8, _d, _d, _d, _d, ctypes.c_double]
         L.ow_solve3.argtypes = [_d, _d, _d]
         L.ow_snapshot.argtypes = [ctypes.c_int32, _d, _d, _u8, _i32, _d, _d, _u8, _u8]
+        L.ow_capsule_triangles.argtypes = [ctypes.c_int64, _d, _d, ctypes.c_double, _d, _u8, _d, _d, _d]
         _lib = L
     return _lib
 
@@ -234,6 +235,17 @@ def quat_to_matrix(q) -> np.ndarray:
     q = _c(q, np.float64); R = np.empty(9)
     lib().ow_quat_to_matrix(_p(q, _d), _p(R, _d))
     return R.reshape(3, 3)
+
+
+def capsule_triangles(p0, p1, radius, verts):
+    """``(hit, points, normals, depths)`` of the capsule (segment p0 p1, radius) against
+    triangles ``verts`` ``(n, 3, 3)``."""
+    p0 = _c(p0, np.float64); p1 = _c(p1, np.float64)
+    verts = _c(np.asarray(verts, dtype=np.float64).reshape(-1, 3, 3), np.float64); n = len(verts)
+    hit = np.zeros(n, np.uint8); pts = np.zeros((n, 3)); nrm = np.zeros((n, 3)); dep = np.zeros(n)
+    lib().ow_capsule_triangles(n, _p(p0, _d), _p(p1, _d), float(radius), _p(verts, _d), _p(hit, _u8),
+                               _p(pts, _d), _p(nrm, _d), _p(dep, _d))
+    return hit.astype(bool), pts, nrm, dep
 
 
 def snapshot(pos, quat, awake, motion_type):

@@ -20,6 +20,157 @@ static double dmax(double a, double b) { return a > b ? a : b; }
 
 /* ---------------------------------------------------------------- math -- */
 
+/* ---------------------------------------------------- capsule/triangles -- */
+
+static double cdot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* _collide_native._closest_on_tri (_collide_native.pyx:361-402): closest point
+ * on triangle abc to p by Ericson's Voronoi regions. */
+static void closest_on_tri(const double *p, const double *a, const double *b, const double *c, double *out)
+{
+    double ab[3], ac[3], ap[3], bp[3], cp[3], bc[3];
+    for (int m = 0; m < 3; m++) { ab[m] = b[m] - a[m]; ac[m] = c[m] - a[m]; ap[m] = p[m] - a[m]; }
+    double d1 = cdot3(ab, ap), d2 = cdot3(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) { memcpy(out, a, 24); return; }
+    for (int m = 0; m < 3; m++) bp[m] = p[m] - b[m];
+    double d3 = cdot3(ab, bp), d4 = cdot3(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) { memcpy(out, b, 24); return; }
+    double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+        double v = d1 / (d1 - d3);
+        for (int m = 0; m < 3; m++) out[m] = a[m] + v * ab[m];
+        return;
+    }
+    for (int m = 0; m < 3; m++) cp[m] = p[m] - c[m];
+    double d5 = cdot3(ab, cp), d6 = cdot3(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) { memcpy(out, c, 24); return; }
+    double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        double w = d2 / (d2 - d6);
+        for (int m = 0; m < 3; m++) out[m] = a[m] + w * ac[m];
+        return;
+    }
+    double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+        double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        for (int m = 0; m < 3; m++) { bc[m] = c[m] - b[m]; out[m] = b[m] + w * bc[m]; }
+        return;
+    }
+    double denom = 1.0 / (va + vb + vc);
+    double v = vb * denom, w = vc * denom;
+    for (int m = 0; m < 3; m++) out[m] = a[m] + ab[m] * v + ac[m] * w;
+}
+
+static double clamp01(double x) { return x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x); }
+
+/* _collide_native._closest_seg_seg (_collide_native.pyx:405-451). */
+static void closest_seg_seg(const double *p1, const double *q1, const double *p2, const double *q2,
+                            double *o1, double *o2)
+{
+    double d1[3], d2[3], r[3], s, t;
+    for (int m = 0; m < 3; m++) { d1[m] = q1[m] - p1[m]; d2[m] = q2[m] - p2[m]; r[m] = p1[m] - p2[m]; }
+    double aa = cdot3(d1, d1), e = cdot3(d2, d2), f = cdot3(d2, r);
+    if (aa <= EPS_COLLIDE && e <= EPS_COLLIDE) { memcpy(o1, p1, 24); memcpy(o2, p2, 24); return; }
+    if (aa <= EPS_COLLIDE) {
+        s = 0.0;
+        t = clamp01(f / e);
+    } else {
+        double cc = cdot3(d1, r);
+        if (e <= EPS_COLLIDE) {
+            t = 0.0;
+            s = clamp01(-cc / aa);
+        } else {
+            double bb = cdot3(d1, d2), denom = aa * e - bb * bb;
+            s = denom > EPS_COLLIDE ? clamp01((bb * f - cc * e) / denom) : 0.0;
+            t = (bb * s + f) / e;
+            if (t < 0.0) { t = 0.0; s = clamp01(-cc / aa); }
+            else if (t > 1.0) { t = 1.0; s = clamp01((bb - cc) / aa); }
+        }
+    }
+    for (int m = 0; m < 3; m++) { o1[m] = p1[m] + d1[m] * s; o2[m] = p2[m] + d2[m] * t; }
+}
+
+/* _collide_native.capsule_triangles (_collide_native.pyx:454-574): the capsule
+ * (segment c0 c1, radius) against n triangles verts[n][3][3].  Face contact
+ * first -- the side is the capsule centre's, the depth unbounded: the cap
+ * endpoint whose foot lies on the triangle and reaches deepest -- else the
+ * nearest feature pair (endpoint to triangle, axis to each edge) inside the
+ * radius, the normal from the triangle to the capsule. */
+void ow_capsule_triangles(int64_t n, const double *c0, const double *c1, double radius, const double *verts,
+                          uint8_t *hit, double *points, double *normals, double *depths)
+{
+    double centre[3];
+    const double *caps[2] = {c0, c1};
+    for (int m = 0; m < 3; m++) centre[m] = (c0[m] + c1[m]) * 0.5;
+    for (int64_t i = 0; i < n; i++) {
+        const double *v0 = verts + 9 * i, *v1 = v0 + 3, *v2 = v0 + 6;
+        double e1[3], e2[3], face[3], fn[3], sp[3], tp[3], foot[3], bestpt[3], ap[3], ep[3];
+        double bsp[3] = {0.0, 0.0, 0.0}, btp[3] = {0.0, 0.0, 0.0};
+        for (int m = 0; m < 3; m++) { e1[m] = v1[m] - v0[m]; e2[m] = v2[m] - v0[m]; }
+        face[0] = e1[1] * e2[2] - e1[2] * e2[1];
+        face[1] = e1[2] * e2[0] - e1[0] * e2[2];
+        face[2] = e1[0] * e2[1] - e1[1] * e2[0];
+        double face_len = sqrt(cdot3(face, face));
+        int face_found = 0;
+        double best_reach = -1e300;
+        if (face_len > EPS_COLLIDE) {
+            for (int m = 0; m < 3; m++) fn[m] = face[m] / face_len;
+            double gx = centre[0] - v0[0], gy = centre[1] - v0[1], gz = centre[2] - v0[2];
+            if (fn[0] * gx + fn[1] * gy + fn[2] * gz < 0.0)
+                for (int m = 0; m < 3; m++) fn[m] = -fn[m];
+            for (int k = 0; k < 2; k++) {
+                for (int m = 0; m < 3; m++) sp[m] = caps[k][m];
+                gx = sp[0] - v0[0]; gy = sp[1] - v0[1]; gz = sp[2] - v0[2];
+                double height = fn[0] * gx + fn[1] * gy + fn[2] * gz;
+                for (int m = 0; m < 3; m++) foot[m] = sp[m] - fn[m] * height;
+                closest_on_tri(sp, v0, v1, v2, tp);
+                gx = foot[0] - tp[0]; gy = foot[1] - tp[1]; gz = foot[2] - tp[2];
+                if (gx * gx + gy * gy + gz * gz > EPS_COLLIDE * EPS_COLLIDE) continue;
+                double reach = radius - height;
+                if (reach > 0.0 && reach > best_reach) {
+                    best_reach = reach;
+                    face_found = 1;
+                    memcpy(bestpt, tp, 24);
+                }
+            }
+        }
+        if (face_found) {
+            hit[i] = 1;
+            depths[i] = best_reach;
+            for (int m = 0; m < 3; m++) { points[3 * i + m] = bestpt[m]; normals[3 * i + m] = fn[m]; }
+            continue;
+        }
+        double best_d2 = 1e300;
+        for (int k = 0; k < 2; k++) {
+            for (int m = 0; m < 3; m++) sp[m] = caps[k][m];
+            closest_on_tri(sp, v0, v1, v2, tp);
+            double gx = sp[0] - tp[0], gy = sp[1] - tp[1], gz = sp[2] - tp[2];
+            double d2 = gx * gx + gy * gy + gz * gz;
+            if (d2 < best_d2) { best_d2 = d2; memcpy(bsp, sp, 24); memcpy(btp, tp, 24); }
+        }
+        for (int k = 0; k < 3; k++) {
+            const double *pa = k == 0 ? v0 : (k == 1 ? v1 : v2), *pb = k == 0 ? v1 : (k == 1 ? v2 : v0);
+            closest_seg_seg(c0, c1, pa, pb, ap, ep);
+            double gx = ap[0] - ep[0], gy = ap[1] - ep[1], gz = ap[2] - ep[2];
+            double d2 = gx * gx + gy * gy + gz * gz;
+            if (d2 < best_d2) { best_d2 = d2; memcpy(bsp, ap, 24); memcpy(btp, ep, 24); }
+        }
+        if (best_d2 >= radius * radius) { hit[i] = 0; continue; }
+        double dist = sqrt(best_d2), nx, ny, nz;
+        if (dist > EPS_COLLIDE) {
+            nx = (bsp[0] - btp[0]) / dist; ny = (bsp[1] - btp[1]) / dist; nz = (bsp[2] - btp[2]) / dist;
+        } else if (face_len > EPS_COLLIDE) {
+            nx = face[0] / face_len; ny = face[1] / face_len; nz = face[2] / face_len;
+        } else {
+            nx = ny = nz = 0.0;
+        }
+        hit[i] = 1;
+        depths[i] = radius - dist;
+        for (int m = 0; m < 3; m++) points[3 * i + m] = btp[m];
+        normals[3 * i] = nx; normals[3 * i + 1] = ny; normals[3 * i + 2] = nz;
+    }
+}
+
 /* ThreadedSimulation._publish (threaded.py:149-158): the pose snapshot of n
  * bodies.  Orientation by quat_to_axis_angle (mathutil.py:90-105): normalise
  * (norm added left to right, divided by max(norm, 1e-12)), clip w to [-1, 1],

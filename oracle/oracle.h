@@ -76,6 +76,8 @@ void ow_get_trigger_overlaps(const OWorld *w, int32_t *out);
 
 /* ---- stage-level restatements (each cites its reference function) ---- */
 void ow_quat_to_matrix(const double *q, double *R);
+void ow_capsule_triangles(int64_t n, const double *c0, const double *c1, double radius, const double *verts,
+                          uint8_t *hit, double *points, double *normals, double *depths);
 void ow_snapshot(int32_t n, const double *pos, const double *quat, const uint8_t *awake,
                  const int32_t *motion_type, double *out_pos, double *axis_angle, uint8_t *out_awake,
                  uint8_t *dynamic);

@@ -555,3 +555,34 @@ def test_physicsworld_trigger_events_on_hip(lib):
     for _ in range(5):
         w2.step(DT)
     assert seen[0] == 'enter' and seen.count('stay') >= 3
+
+
+def test_capsule_triangles_bit_exact(lib):
+    """omi_capsule_triangles against the oracle's restatement of the Cython loop, bit for
+    bit: 5000 random triangles around an upright and a tilted capsule, the reference's
+    hand-worked floor / wall / ceiling / rim / sliver cases, and an empty batch."""
+    from omi_physics_amd.capsule import CapsuleProxy, capsule_triangle_batch, capsule_triangles
+    rng = np.random.default_rng(3)
+    tris = rng.normal(scale=1.2, size=(5000, 3, 3))
+    a = 0.6
+    tilt = np.array([[np.cos(a), -np.sin(a), 0.0], [np.sin(a), np.cos(a), 0.0], [0.0, 0.0, 1.0]])
+    caps = [CapsuleProxy((0.1, 0.2, -0.1), 0.5, 0.7, np.eye(3)), CapsuleProxy((0.0, 0.3, 0.0), 0.5, 0.4, tilt)]
+    hand = np.array([((-10.0, 0.0, -10.0), (10.0, 0.0, -10.0), (0.0, 0.0, 10.0)),
+                     ((1.0, -10.0, -10.0), (1.0, -10.0, 10.0), (1.0, 10.0, 0.0)),
+                     ((-10.0, 0.0, -10.0), (0.0, 0.0, 10.0), (10.0, 0.0, -10.0)),
+                     ((0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, 0.0, 1.0)),
+                     ((0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (2.0, 0.0, 0.0))])
+    for cap in caps + [CapsuleProxy(c, h, r, np.eye(3)) for c, h, r in
+                       (((0.0, 0.7, 0.0), 0.5, 0.4), ((0.0, -0.3, 0.0), 0.5, 0.4), ((0.75, 0.0, 0.0), 0.5, 0.4),
+                        ((-0.2, 0.0, -0.2), 0.01, 0.4), ((0.5, 0.0, 0.0), 0.01, 0.4))]:
+        for batch in (tris, hand):
+            g = capsule_triangle_batch(cap, batch)
+            o = oracle.capsule_triangles(cap.p0, cap.p1, cap.radius, batch)
+            assert np.array_equal(g[0], o[0])
+            for x, y in zip(g[1:], o[1:]):
+                assert np.array_equal(x, y)
+    h, p, n, d = capsule_triangles(caps[0], tris[:, 0], tris[:, 1], tris[:, 2])
+    say('capsule: %d of 5000 random triangles hit' % int(h.sum()))
+    assert np.array_equal(h, oracle.capsule_triangles(caps[0].p0, caps[0].p1, 0.7, tris)[0])
+    empty = capsule_triangle_batch(caps[0], np.zeros((0, 3, 3)))
+    assert all(len(x) == 0 for x in empty)

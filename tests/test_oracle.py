@@ -344,3 +344,89 @@ def test_trigger_system_diff():
     assert ts.update([(0, 2), (0, 5)]) == [('enter', 0, 5), ('stay', 0, 2), ('exit', 3, 1)]
     assert ts.update([]) == [('exit', 0, 2), ('exit', 0, 5)]
     assert ts.update([]) == [] and ts.overlaps == set()
+
+
+# the reference's capsule-triangle geometry (tests/test_collide_batch.py)
+CAP_FLOOR = ((-10.0, 0.0, -10.0), (10.0, 0.0, -10.0), (0.0, 0.0, 10.0))
+CAP_WALL = ((1.0, -10.0, -10.0), (1.0, -10.0, 10.0), (1.0, 10.0, 0.0))
+
+
+def _cap(centre, half=0.5, radius=0.4, rot=None):
+    from omi_physics_amd.capsule import CapsuleProxy
+    return CapsuleProxy(centre, half, radius, np.eye(3) if rot is None else rot)
+
+
+def _ocap(cap, tris):
+    return oracle.capsule_triangles(cap.p0, cap.p1, cap.radius, np.asarray(tris, dtype=float))
+
+
+def test_capsule_triangles_hand_worked():
+    """The oracle's capsule_triangles on the reference's hand-worked cases: resting
+    0.2 into a floor (normal up, depth 0.2, point on the plane), clear of it, driven
+    deep (depth 0.6 pushed down), wholly below, under a ceiling wound the other way,
+    against a wall (depth 0.15), off a triangle's rim (push toward the corner), and a
+    degenerate sliver (never a NaN normal)."""
+    h, p, nrm, d = _ocap(_cap((0.0, 0.7, 0.0)), [CAP_FLOOR])
+    assert h[0] and np.allclose(nrm[0], [0, 1, 0]) and d[0] == pytest.approx(0.2) and p[0][1] == 0.0
+    assert not _ocap(_cap((0.0, 1.0, 0.0)), [CAP_FLOOR])[0][0]
+    h, p, nrm, d = _ocap(_cap((0.0, -0.3, 0.0)), [CAP_FLOOR])
+    assert h[0] and np.allclose(nrm[0], [0, -1, 0]) and d[0] == pytest.approx(0.6)
+    assert not _ocap(_cap((0.0, -1.0, 0.0)), [CAP_FLOOR])[0][0]
+    ceiling = ((-10.0, 0.0, -10.0), (0.0, 0.0, 10.0), (10.0, 0.0, -10.0))
+    h, p, nrm, d = _ocap(_cap((0.0, -0.7, 0.0)), [ceiling])
+    assert np.allclose(nrm[0], [0, -1, 0]) and d[0] == pytest.approx(0.2)
+    h, p, nrm, d = _ocap(_cap((0.75, 0.0, 0.0)), [CAP_WALL])
+    assert np.allclose(nrm[0], [-1, 0, 0]) and d[0] == pytest.approx(0.15)
+    small = ((0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, 0.0, 1.0))
+    h, p, nrm, d = _ocap(_cap((-0.2, 0.0, -0.2), half=0.01, radius=0.4), [small])
+    assert h[0] and np.allclose(p[0], [0, 0, 0]) and np.allclose(nrm[0], [-np.sqrt(0.5), 0, -np.sqrt(0.5)], atol=1e-6)
+    sliver = ((0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (2.0, 0.0, 0.0))
+    h, p, nrm, d = _ocap(_cap((0.5, 0.0, 0.0), half=0.01, radius=0.4), [sliver])
+    assert np.all(np.isfinite(nrm[0]))
+    # no triangles; every triangle answered even when none touch
+    assert len(_ocap(_cap((0, 0, 0)), np.zeros((0, 3, 3)))[0]) == 0
+    h = _ocap(_cap((0.0, 50.0, 0.0)), [CAP_FLOOR, CAP_WALL])[0]
+    assert len(h) == 2 and not h.any()
+
+
+def _seg_tri_distance(p0, p1, tri, samples=400):
+    """Brute force: the distance between a segment and a triangle, sampled densely."""
+    a, b, c = (np.asarray(v, float) for v in tri)
+    u = np.linspace(0.0, 1.0, samples)
+    seg = p0[None, :] + u[:, None] * (p1 - p0)[None, :]
+    g = np.linspace(0.0, 1.0, 120)
+    s, t = np.meshgrid(g, g)
+    keep = s + t <= 1.0
+    pts = a + s[keep][:, None] * (b - a) + t[keep][:, None] * (c - a)
+    best = np.inf
+    for q in seg:
+        best = min(best, np.sqrt(((pts - q) ** 2).sum(1)).min())
+    return best
+
+
+def test_capsule_triangles_against_brute_force():
+    """Independent checks on random triangles: a face contact has a unit normal equal to
+    +-the face normal and depth radius - height of a cap endpoint above the plane; a rim
+    contact's depth is radius - the segment-triangle distance (brute force, 2e-2); no
+    contact means the capsule is farther than its radius (brute force)."""
+    rng = np.random.default_rng(7)
+    cap = _cap((0.1, 0.2, -0.1), half=0.5, radius=0.7)
+    tris = rng.normal(scale=1.2, size=(60, 3, 3))
+    hit, pts, nrm, dep = _ocap(cap, tris)
+    kinds = {'face': 0, 'rim': 0, 'none': 0}
+    for i, tri in enumerate(tris):
+        e = np.cross(tri[1] - tri[0], tri[2] - tri[0])
+        fnrm = e / np.linalg.norm(e)
+        if not hit[i]:
+            kinds['none'] += 1
+            assert _seg_tri_distance(cap.p0, cap.p1, tri) > cap.radius - 2e-2, i
+            continue
+        assert abs(np.linalg.norm(nrm[i]) - 1.0) < 1e-12, i
+        if abs(abs(np.dot(nrm[i], fnrm)) - 1.0) < 1e-12:
+            kinds['face'] += 1
+            heights = [np.dot(nrm[i], q - tri[0]) for q in (cap.p0, cap.p1)]
+            assert min(abs(dep[i] - (cap.radius - h)) for h in heights) < 1e-12, i
+        else:
+            kinds['rim'] += 1
+            assert abs(dep[i] - (cap.radius - _seg_tri_distance(cap.p0, cap.p1, tri))) < 2e-2, i
+    assert min(kinds.values()) > 0, kinds
