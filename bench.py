@@ -93,7 +93,7 @@ def solve_top_bytes(stats, r):
     return k * (224 + 3 * r) + 16 * m + nd * (96 + 48 + 12 * r), top, int(sel.sum())
 
 
-def run_batched(arrays, precision, steps, warmup, timing=True):
+def run_batched(arrays, precision, steps, warmup, timing=True, lanes=False):
     """Warm up on one batch, then time ``steps`` steps of a fresh batch from the initial state.
 
     With ``timing`` the steps launch eagerly with a HIP event between kernels on
@@ -103,13 +103,13 @@ def run_batched(arrays, precision, steps, warmup, timing=True):
     """
     from omi_physics_amd.batch import batch_from_arrays
     if warmup > 0:
-        w = batch_from_arrays(arrays, precision=precision)
+        w = batch_from_arrays(arrays, precision=precision, lanes=lanes)
         w.prepare(DT)
         w.step(DT, warmup)
         w.sync()
         w.close()
     t0 = time.perf_counter()
-    b = batch_from_arrays(arrays, precision=precision)
+    b = batch_from_arrays(arrays, precision=precision, lanes=lanes)
     log('%s batch loaded: %.1f GB device, %.1f s' % (precision, b.device_bytes() / 1e9, time.perf_counter() - t0))
     if timing:
         b.set_timing(True)
@@ -292,6 +292,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-fp32', action='store_true', help='skip the batched FP32 form')
     ap.add_argument('--no-timing', action='store_true')
+    ap.add_argument('--lanes', action='store_true', help='split the worlds by size class into lanes stepped side by side')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU only: this build has no multi-GPU path (worlds would shard by '
@@ -363,7 +364,7 @@ def main():
     sample = np.sort(rng.choice(len(arrays), size=min(args.cpu_sample, len(arrays)), replace=False))
 
     # headline: the batched FP64 form, graph-launched steps 1..K
-    b, lat, wall, _, _, _ = run_batched(arrays, 'fp64', args.steps, args.warmup, timing=False)
+    b, lat, wall, _, _, _ = run_batched(arrays, 'fp64', args.steps, args.warmup, timing=False, lanes=args.lanes)
     value = n_bodies * args.steps / wall
     gpu64 = {int(i): b.read(int(i)) for i in sample}
     b.close()
@@ -372,10 +373,10 @@ def main():
         'warmup': args.warmup, 'ms_per_step': 1e3 * wall / args.steps, 'higher_is_better': True,
         'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic',
         'config': {'workload': '%d independent box/sphere worlds, lognormal(ln 64, 1.0) bodies clipped [8, 2048], '
-                               'batched in one graph-captured step, FP64 (bit-identical to the oracle); steps '
-                               '1..%d from rest' % (args.worlds, args.steps),
+                               'batched in one graph-captured step%s, FP64 (bit-identical to the oracle); steps '
+                               '1..%d from rest' % (args.worlds, ' per size-class lane' if args.lanes else '', args.steps),
                    'worlds': args.worlds, 'bodies': n_bodies, 'dt': DT, 'solver_iterations': 10,
-                   'sleep_enabled': True, 'seed': args.seed},
+                   'sleep_enabled': True, 'seed': args.seed, 'lanes': bool(args.lanes)},
         'p50_step_ms': 1e3 * float(np.percentile(lat, 50)),
         'p99_step_ms': 1e3 * float(np.percentile(lat, 99)),
     }
@@ -383,7 +384,8 @@ def main():
         # the same K steps again from the initial state, launched eagerly with HIP
         # events between the stages on the batch stream: device time per stage,
         # counters and the roofline of the dominant kernel
-        bt, lat_t, wall_t, ktot, ctot, kbytes = run_batched(arrays, 'fp64', args.steps, 0, timing=True)
+        bt, lat_t, wall_t, ktot, ctot, kbytes = run_batched(arrays, 'fp64', args.steps, 0, timing=True,
+                                                            lanes=args.lanes)
         same = all(np.array_equal(bt.read(int(i))['position'], gpu64[int(i)]['position']) for i in sample[:8])
         bt.close()
         log('timed pass: %.2f ms/step eager (graph %.2f), identical results: %s'
@@ -415,7 +417,7 @@ def main():
             with open(tf) as f:
                 traffic = json.load(f).get('k_solve_top_fp64_bytes_per_launch')
         out['roofline'] = {'kernel': 'k_solve<double> size class %d (%d worlds, the largest; alone on its '
-                                     'lane stream)' % (top, ntop),
+                                     'stream)' % (top, ntop),
                            'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                            'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic, 'avg_ms': avg_ms,
                            'bytes_per_launch': per_launch}
@@ -425,7 +427,7 @@ def main():
                                   for nm, kt, kb in zip(names, ktot, kbytes)}
 
     if not args.no_fp32:
-        b32, lat32, wall32, *_ = run_batched(arrays, 'fp32', args.steps, 0, timing=False)
+        b32, lat32, wall32, *_ = run_batched(arrays, 'fp32', args.steps, 0, timing=False, lanes=args.lanes)
         gpu32 = {int(i): b32.read(int(i))['position'] for i in sample}
         b32.close()
         out['fp32_batched'] = {'value': n_bodies * args.steps / wall32, 'ms_per_step': 1e3 * wall32 / args.steps,

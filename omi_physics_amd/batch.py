@@ -390,16 +390,16 @@ class BatchGroup:
         return int(sum(b.device_bytes() for b in self.lanes))
 
 
-def batch_from_arrays(worlds: Sequence[dict], precision: str = 'fp64', lanes: Optional[bool] = None, **kw):
+def batch_from_arrays(worlds: Sequence[dict], precision: str = 'fp64', lanes: bool = False, **kw):
     """A batch holding ``worlds`` as worlds 0..len-1, sized for them.
 
-    With ``lanes`` (default: whenever the worlds span more than one size class and no
-    explicit pair/contact capacity is given) the worlds are split by size class into
-    the lanes of a :class:`BatchGroup`; otherwise one :class:`Batch` holds them all.
+    With ``lanes`` the worlds are split by size class into the lanes of a
+    :class:`BatchGroup` (when they span more than one class); otherwise one
+    :class:`Batch` holds them all.
     """
     classes = [size_class(int(w['n'])) for w in worlds]
-    if lanes is None:
-        lanes = len(set(classes)) > 1 and 'max_pairs' not in kw and 'max_contacts' not in kw
+    if lanes and len(set(classes)) < 2:
+        lanes = False
     if not lanes:
         return _single_batch(worlds, precision, **kw)
     groups = [[i for i, c in enumerate(classes) if c == k] for k in (3, 2, 1, 0)]

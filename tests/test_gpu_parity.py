@@ -265,8 +265,8 @@ def test_group_lanes_equal_single_batch(lib, precision):
     arrays = [scenes.arrays(scenes.boxes_on_ground(n, seed=s, spread=10.0, y_range=(0.6, 8.0),
                                                    sphere_fraction=0.5)) for n, s in sizes]
     say('group of lanes vs one batch', precision, [a['n'] for a in arrays])
-    grp = batch_from_arrays(arrays, precision=precision)
-    one = batch_from_arrays(arrays, precision=precision, lanes=False)
+    grp = batch_from_arrays(arrays, precision=precision, lanes=True)
+    one = batch_from_arrays(arrays, precision=precision)
     assert hasattr(grp, 'lanes') and len(grp.lanes) == 3 and not hasattr(one, 'lanes')
     grp.prepare(DT)
     for phase, timed in enumerate((False, True, False)):
