@@ -1255,10 +1255,13 @@ __host__ __device__ __forceinline__ size_t al16(size_t x) { return (x + 15) & ~(
 
 // Where one world's solver state lives during the level loop.  In LDS
 // (byte offsets into the launch's dynamic LDS): the compact id of every body in
-// contact (map, world-local index -> id), their velocities (6 R per id), the
-// contacts' impulses (3 R per contact), and positions (FP64) or position
-// corrections (FP32) of the bodies in contact (3 R per id); otherwise the
-// velocities, impulses and positions of the batch in global memory.
+// contact (map, world-local index -> id), their velocities (one record of
+// VelRec<R>::stride R per id: 6 doubles, or 6 floats padded to 8, so a body's
+// velocity is three or two 16-byte LDS accesses), the contacts' impulses (3 R
+// per contact, contact-major), and positions (FP64) or position corrections
+// (FP32) of the bodies in contact (3 R per id); otherwise the velocities,
+// impulses and positions of the batch in global memory.
+template <typename R> struct VelRec { static constexpr int stride = sizeof(R) == 8 ? 6 : 8; };
 template <typename R>
 struct SolveState {
     size_t map, vel, imp, pos;     // LDS offsets
@@ -1314,8 +1317,7 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>&
     const R* S_ = (const R*)d.csolv;
     const R* iiw = (const R*)d.iiw;
     // velocity c of side body i (global index) with compact id ci_
-#define VEL(c, i, ci_) (LIGHT ? lV[(size_t)(c) * st.nd + (size_t)(ci_)] : st.V[(size_t)(c) * st.vs + (size_t)(i)])
-#define IMP(c, ci) (LIGHT ? lP[(size_t)(c) * K + (size_t)((ci) - cs)] \
+#define IMP(c, ci) (LIGHT ? lP[(size_t)((ci) - cs) * 3 + (size_t)(c)] \
                           : st.P[(size_t)((ci) - st.pb) * st.ps + (size_t)(c) * st.pc])
     struct IdRec { int32_t code, a, b; };
     auto fetch_id = [&](int32_t slot, IdRec& r) {
@@ -1332,6 +1334,36 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>&
         if constexpr (sizeof(R) == 8 && !POS) {
             load_vec<R, 9>(&IIW(iiw, it.a, 0), it.Ia);
             load_vec<R, 9>(&IIW(iiw, it.b, 0), it.Ib);
+        }
+    };
+    // a body's six velocity components: 16-byte LDS accesses in LIGHT mode
+    auto vel_load = [&](int64_t i, int32_t cid, R* v) {
+        if constexpr (LIGHT) {
+            const R* p = lV + (size_t)cid * VelRec<R>::stride;
+            if constexpr (sizeof(R) == 8) {
+#pragma unroll
+                for (int q = 0; q < 3; q++) { const double2 t = ((const double2*)p)[q]; v[2 * q] = t.x; v[2 * q + 1] = t.y; }
+            } else {
+                const float4 t0 = ((const float4*)p)[0];
+                const float2 t1 = ((const float2*)p)[2];
+                v[0] = t0.x; v[1] = t0.y; v[2] = t0.z; v[3] = t0.w; v[4] = t1.x; v[5] = t1.y;
+            }
+        } else {
+            for (int c = 0; c < 6; c++) v[c] = st.V[(size_t)c * st.vs + (size_t)i];
+        }
+    };
+    auto vel_store = [&](int64_t i, int32_t cid, const R* v) {
+        if constexpr (LIGHT) {
+            R* p = lV + (size_t)cid * VelRec<R>::stride;
+            if constexpr (sizeof(R) == 8) {
+#pragma unroll
+                for (int q = 0; q < 3; q++) ((double2*)p)[q] = make_double2(v[2 * q], v[2 * q + 1]);
+            } else {
+                ((float4*)p)[0] = make_float4(v[0], v[1], v[2], v[3]);
+                ((float2*)p)[2] = make_float2(v[4], v[5]);
+            }
+        } else {
+            for (int c = 0; c < 6; c++) st.V[(size_t)c * st.vs + (size_t)i] = v[c];
         }
     };
     auto pos_add = [&](int64_t i, int32_t cid, int r, R term, bool neg, bool dyn) {
@@ -1383,25 +1415,34 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>&
             const R* Ia = it.Ia;
             const R* Ib = it.Ib;
             R va[6], vb2[6];
-            for (int c = 0; c < 6; c++) { va[c] = VEL(c, a, it.cida); vb2[c] = VEL(c, b, it.cidb); }
-            if (it.s < in.warm) {
-                // warm start (_solver_native.pyx:258-266)
-                const R n0 = IMP(0, ci), u0 = IMP(1, ci), w0 = IMP(2, ci);
-                const R P0 = n0 * nv[0] + u0 * t1[0] + w0 * t2[0];
-                const R P1 = n0 * nv[1] + u0 * t1[1] + w0 * t2[1];
-                const R P2 = n0 * nv[2] + u0 * t1[2] + w0 * t2[2];
-                apply_w(va, vb2, ima, imb, Ia, Ib, rA, rB, P0, P1, P2);
+            vel_load(a, it.cida, va);
+            vel_load(b, it.cidb, vb2);
+            // A wavefront's lanes can hold warm-start and sweep items at once:
+            // both impulses go through one apply_w, so such a wavefront runs
+            // that update once instead of once per kind.
+            const bool warm = it.s < in.warm;
+            R nI = IMP(0, ci), tI0 = IMP(1, ci), tI1 = IMP(2, ci);
+            R Px, Py, Pz;
+            if (warm) {
+                // warm start (_solver_native.pyx:258-266): the seeds' impulse
+                Px = nI * nv[0] + tI0 * t1[0] + tI1 * t2[0];
+                Py = nI * nv[1] + tI0 * t1[1] + tI1 * t2[1];
+                Pz = nI * nv[2] + tI0 * t1[2] + tI1 * t2[2];
             } else {
-                // one sweep over the contact (_solver_native.pyx:268-295)
-                R nI = IMP(0, ci), tI0 = IMP(1, ci), tI1 = IMP(2, ci);
+                // one sweep over the contact (_solver_native.pyx:268-295),
+                // normal impulse first
                 R dvx, dvy, dvz;
                 relvel(va, vb2, rA, rB, dvx, dvy, dvz);
                 R vn = dvx * nv[0] + dvy * nv[1] + dvz * nv[2];
                 R nw = nI - it.row[SR_NM] * (vn - it.row[SR_VB]);
                 if (nw < R(0)) nw = R(0);
-                R dL = nw - nI;
+                const R dL = nw - nI;
                 nI = nw;
-                apply_w(va, vb2, ima, imb, Ia, Ib, rA, rB, dL * nv[0], dL * nv[1], dL * nv[2]);
+                Px = dL * nv[0]; Py = dL * nv[1]; Pz = dL * nv[2];
+            }
+            apply_w(va, vb2, ima, imb, Ia, Ib, rA, rB, Px, Py, Pz);
+            if (!warm) {
+                R dvx, dvy, dvz, dL;
                 const R maxf = it.row[SR_FR] * nI;
                 relvel(va, vb2, rA, rB, dvx, dvy, dvz);
                 R vt = dvx * t1[0] + dvy * t1[1] + dvz * t1[2];
@@ -1416,13 +1457,14 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>&
                 apply_w(va, vb2, ima, imb, Ia, Ib, rA, rB, dL * t2[0], dL * t2[1], dL * t2[2]);
                 IMP(0, ci) = nI; IMP(1, ci) = tI0; IMP(2, ci) = tI1;
             }
-            if (it.da) for (int c = 0; c < 6; c++) VEL(c, a, it.cida) = va[c];
-            if (it.db) for (int c = 0; c < 6; c++) VEL(c, b, it.cidb) = vb2[c];
+            if (it.da) vel_store(a, it.cida, va);
+            if (it.db) vel_store(b, it.cidb, vb2);
         } else {
             // FP32: the same updates on the folded Jacobian terms, with FMA
             const R* row = it.row;
             R va[6], vb2[6];
-            for (int c = 0; c < 6; c++) { va[c] = VEL(c, a, it.cida); vb2[c] = VEL(c, b, it.cidb); }
+            vel_load(a, it.cida, va);
+            vel_load(b, it.cidb, vb2);
             // J_d . v = (vb - va) . d + wb . (rB x d) - wa . (rA x d)
             auto jv = [&](int dd) -> R {
                 const R* e = row + FR_DIR + 3 * dd;
@@ -1451,32 +1493,37 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>&
                     vb2[3 + r] = __builtin_fmaf(lam, kb[r], vb2[3 + r]);
                 }
             };
-            if (it.s < in.warm) {
-                // warm start (_solver_native.pyx:258-266)
-                apply(0, IMP(0, ci));
-                apply(1, IMP(1, ci));
-                apply(2, IMP(2, ci));
-            } else {
-                // one sweep over the contact (_solver_native.pyx:268-295)
-                R nI = IMP(0, ci), tI0 = IMP(1, ci), tI1 = IMP(2, ci);
+            // warm start (_solver_native.pyx:258-266): the seeds along n, t1,
+            // t2; a sweep (_solver_native.pyx:268-295): the normal impulse,
+            // then both tangent ones from the velocity after it.  The three
+            // updates are shared by both kinds of item.
+            const bool warm = it.s < in.warm;
+            R nI = IMP(0, ci), tI0 = IMP(1, ci), tI1 = IMP(2, ci);
+            R lam0 = nI, lam1 = tI0, lam2 = tI1;
+            if (!warm) {
                 R nw = nI - row[FR_NM] * (jv(0) - row[FR_VB]);
                 if (nw < R(0)) nw = R(0);
-                apply(0, nw - nI);
+                lam0 = nw - nI;
                 nI = nw;
+            }
+            apply(0, lam0);
+            if (!warm) {
                 const R maxf = row[FR_FR] * nI;
-                const R vt1 = jv(1), vt2 = jv(2);   // both from the velocity after the normal impulse
+                const R vt1 = jv(1), vt2 = jv(2);
                 R nt = tI0 - row[FR_T1M] * vt1;
                 if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
-                apply(1, nt - tI0);
+                lam1 = nt - tI0;
                 tI0 = nt;
                 nt = tI1 - row[FR_T2M] * vt2;
                 if (nt > maxf) nt = maxf; else if (nt < -maxf) nt = -maxf;
-                apply(2, nt - tI1);
+                lam2 = nt - tI1;
                 tI1 = nt;
-                IMP(0, ci) = nI; IMP(1, ci) = tI0; IMP(2, ci) = tI1;
             }
-            if (it.da) for (int c = 0; c < 6; c++) VEL(c, a, it.cida) = va[c];
-            if (it.db) for (int c = 0; c < 6; c++) VEL(c, b, it.cidb) = vb2[c];
+            apply(1, lam1);
+            apply(2, lam2);
+            if (!warm) { IMP(0, ci) = nI; IMP(1, ci) = tI0; IMP(2, ci) = tI1; }
+            if (it.da) vel_store(a, it.cida, va);
+            if (it.db) vel_store(b, it.cidb, vb2);
         }
     };
     auto bound_at = [&](int32_t l) -> int32_t {
@@ -1519,7 +1566,6 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>&
         level_step(l, itA, itB, idB, idA);
         if (l + 1 < D) level_step(l + 1, itB, itA, idA, idB);
     }
-#undef VEL
 #undef IMP
 }
 
@@ -1808,7 +1854,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     SolveState<R> st;
     st.map = 0;
     st.vel = al16((size_t)n * 4);
-    st.imp = st.vel + al16((size_t)nd * 6 * RB);
+    st.imp = st.vel + al16((size_t)nd * VelRec<R>::stride * RB);
     st.pos = st.imp + al16((size_t)K * 3 * RB);
     st.nd = nd;
     st.off = off;
@@ -1824,11 +1870,11 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
         for (int32_t i = tid; i < n; i += BLOCK) {
             if (!touched(i)) continue;
             const int32_t c = cmap[i];
-            for (int q = 0; q < 6; q++) V[(size_t)q * nd + c] = (R)BODY(d.lv, q, off + i);
+            for (int q = 0; q < 6; q++) V[(size_t)c * VelRec<R>::stride + q] = (R)BODY(d.lv, q, off + i);
         }
         R* I = (R*)(s_dyn + st.imp);
         for (int32_t k = tid; k < K; k += BLOCK)
-            for (int q = 0; q < 3; q++) I[(size_t)q * K + k] = ROW(S_, cs + k, RowLayout<R>::imp + q);
+            for (int q = 0; q < 3; q++) I[(size_t)k * 3 + q] = ROW(S_, cs + k, RowLayout<R>::imp + q);
     } else if (!fp64) {
         R* V = (R*)d.svel;
         for (int32_t i = tid; i < n; i += BLOCK)
@@ -1870,7 +1916,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     for (int32_t k = tid; k < K; k += BLOCK) {
         const int64_t ci = cs + k;
         for (int r = 0; r < 3; r++)
-            CON(imp, r, ci) = state_lds ? ((R*)(s_dyn + st.imp))[(size_t)r * K + k] : ROW(S_, ci, RowLayout<R>::imp + r);
+            CON(imp, r, ci) = state_lds ? ((R*)(s_dyn + st.imp))[(size_t)k * 3 + r] : ROW(S_, ci, RowLayout<R>::imp + r);
     }
     for (int32_t i = tid; i < n; i += BLOCK) {
         if (d.mt[off + i] != M_DYNAMIC) continue;
@@ -1880,10 +1926,10 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
             const R* V = (const R*)(s_dyn + st.vel);
             for (int q = 0; q < 6; q++) {
                 if (fp64) {
-                    BODY(d.lv, q, off + i) = (double)V[(size_t)q * nd + c];
+                    BODY(d.lv, q, off + i) = (double)V[(size_t)c * VelRec<R>::stride + q];
                 } else {
                     const double v0 = BODY(d.lv, q, off + i);
-                    const R dv = V[(size_t)q * nd + c] - (R)v0;
+                    const R dv = V[(size_t)c * VelRec<R>::stride + q] - (R)v0;
                     BODY(d.lv, q, off + i) = v0 + (double)dv;
                 }
             }
