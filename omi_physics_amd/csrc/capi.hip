@@ -601,6 +601,12 @@ static int32_t rebuild_lists(omi_batch* b) {
         int c = NP <= b->np_class[0] ? 0 : NP <= b->np_class[1] ? 1 : NP <= b->np_class[2] ? 2 : 3;
         lists[c].push_back(w);
     }
+    // largest worlds first within a class: their blocks are dispatched first,
+    // so the longest solver (and sort) workgroups start before the short ones
+    // rather than queueing behind them for CU space
+    for (int c = 0; c < 4; c++)
+        std::stable_sort(lists[c].begin(), lists[c].end(),
+                         [&](int32_t x, int32_t y) { return b->slots[x].n > b->slots[y].n; });
     int32_t rc, counts[4];
     bool grew = false;
     for (int c = 0; c < 4; c++) {
