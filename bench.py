@@ -351,9 +351,10 @@ def main():
             prec = args.precision or 'fp32'
             name = '20 ten-level box pyramids + 8000 box/sphere debris, one world, %s' % prec
         else:
-            arr = scenes.build(scenes.convex_heap(2000), PhysicsWorld, model).export_arrays()
+            arr = scenes.build(scenes.convex_and_chains(), PhysicsWorld, model).export_arrays()
             prec = args.precision or 'fp32'
-            name = '2000 cooked convex hulls (16-32 points) on a ground box, one world, %s (no joints)' % prec
+            name = ('2000 cooked convex hulls (16-32 points) and 20 hinge chains of 50 links on a ground box, '
+                    'one world, %s' % prec)
         out = single_world(name, arr, prec, args.steps, args.warmup, not args.no_cpu)
         print(json.dumps(out), flush=True)
         return

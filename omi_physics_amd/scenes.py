@@ -199,6 +199,24 @@ def hinge_chains(n_chains: int = 20, links: int = 50, seed: int = 4, spacing: fl
     return spec
 
 
+def convex_and_chains(n_hulls: int = 2000, n_chains: int = 20, links: int = 50, seed: int = 4) -> Dict[str, Any]:
+    """The convex-and-joints workload: :func:`convex_heap`'s hulls and
+    :func:`hinge_chains`' chains (anchored to the world) in one world over one
+    ground box of half extents (40, 0.5, 40)."""
+    spec = convex_heap(n_hulls, seed=seed)
+    spec['shapes'][0] = ('box', (80.0, 1.0, 80.0))          # the heap's ground, enlarged
+    chains = hinge_chains(n_chains, links, seed=seed)
+    s0, b0 = len(spec['shapes']), len(spec['bodies'])
+    # the chains' own ground is body 0 / shape 0: dropped, the rest renumbered
+    spec['shapes'] += chains['shapes'][1:]
+    for b in chains['bodies'][1:]:
+        b = dict(b)
+        b['shape'] = b['shape'] - 1 + s0
+        spec['bodies'].append(b)
+    spec['joints'] = [(a - 1 + b0 if a >= 0 else -1, c - 1 + b0, anchor) for a, c, anchor in chains['joints']]
+    return spec
+
+
 def build(spec: Dict[str, Any], world_cls: Any, model: Any, **extra) -> Any:
     """Build ``spec`` into ``world_cls(**spec['world'], **extra)`` using ``model``'s types."""
     kwargs = dict(spec['world'])
