@@ -26,7 +26,7 @@ void launch_compact(const Dev& d, int precision, int grid, hipStream_t s);
 void launch_wake(const Dev& d, hipStream_t s);
 void launch_cache(const Dev& d, int precision, hipStream_t s);
 void launch_post(const Dev& d, int64_t nhw, double dt, hipStream_t s);
-void launch_joints(const Dev& d, int precision, double dt, hipStream_t s);
+void launch_joints(const Dev& d, int precision, double dt, int32_t lds_bytes, hipStream_t s);
 void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s);
 void launch_toggle(const Dev& d, hipStream_t s);
 void launch_trigger(const Dev& d, hipStream_t s);
@@ -45,6 +45,7 @@ void launch_solve_class(const Dev& d, int precision, int block, SolveIn in, cons
                         int32_t lds_bytes, int grid, hipStream_t s);
 size_t epa_work_bytes();
 bool allow_dynamic_lds(int bytes);
+bool allow_joint_lds(int bytes);
 void launch_stage_convex(int32_t np, const int32_t* ka, const double* ca, const double* Ra, const double* ha,
                          const int32_t* oa, const int32_t* na, const int32_t* kb, const double* cb, const double* Rb,
                          const double* hb, const int32_t* ob, const int32_t* nb, const double* hull, void* work,
@@ -129,6 +130,8 @@ struct omi_batch {
     int32_t* d_wn = nullptr;               // [4] class counts
     int32_t np_class[4] = {256, 1024, 4096, 0};
     int32_t lds_bytes[4] = {0, 0, 0, 0};    // solver: dynamic LDS per world of each size class
+    int32_t joint_lds_max = 0;              // dynamic LDS a joint launch may take
+    int32_t joint_lds = 0;                  // joints: LDS for the joint bodies of the largest joint world
     int32_t live[4] = {0, 0, 0, 0};        // grid of each class's per-world launches (0: none)
     int32_t sap_block[4] = {64, 128, 1024, 1024};
     int32_t solve_block[4] = {64, 128, 256, 256};
@@ -257,6 +260,10 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
         const bool f64 = cfg->precision == OMI_FP64;
         const int32_t want[4] = {f64 ? 32 << 10 : 24 << 10, f64 ? 80 << 10 : 64 << 10, full, full};
         for (int c = 0; c < 4; c++) b->lds_bytes[c] = std::min(want[c], full);
+        // the joint sweeps: the LDS less 8 KB for their static LDS
+        int32_t jl = std::max(0, full - 8192);
+        if (jl > 65536 - 8192 && !allow_joint_lds(jl)) jl = 65536 - 8192;
+        b->joint_lds_max = jl;
     }
     b->solver_block = cfg->solver_block > 0 ? cfg->solver_block : 256;
     auto okb = [](int32_t x) { return x == 64 || x == 128 || x == 256 || x == 512 || x == 1024; };
@@ -331,7 +338,7 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     d.jcap = JC;
     A(d.ja, JC); A(d.jb, JC); A(d.janc, 3 * JC); A(d.jla, 3 * JC); A(d.jlb, 3 * JC); A(d.jhas, JC);
     unsigned char* jr;
-    A(jr, 18 * JC * (int64_t)rs);
+    A(jr, 19 * JC * (int64_t)rs);
     d.jrow = jr;
     A(d.jlevel, JC); A(d.jorder, JC); A(d.jcount, JC);
     A(d.w_js, W); A(d.w_jn, W); A(d.w_jiters, W);
@@ -538,6 +545,18 @@ extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_wo
         if ((rc = put(b, d.jhas + s.j_off, jh.data(), s.j_n))) return rc;
         if ((rc = upload_soa(b, d.janc, d.jcap, s.j_off, w->joint_anchor, s.j_n, 3))) return rc;
         if (!b->any_joints) { b->any_joints = true; invalidate_graph(b); }
+        // LDS for the records of this world's joint bodies during the sweeps
+        std::vector<int32_t> ends;
+        ends.reserve(2 * (size_t)s.j_n);
+        for (int64_t j = 0; j < s.j_n; j++) {
+            if (ja[j] >= 0) ends.push_back(ja[j]);
+            if (jb[j] >= 0) ends.push_back(jb[j]);
+        }
+        std::sort(ends.begin(), ends.end());
+        const int64_t nb = std::unique(ends.begin(), ends.end()) - ends.begin();
+        const int64_t per_body = b->cfg.precision == OMI_FP64 ? 132 : 100;   // JointBody<R>::lds
+        const int64_t want = std::min<int64_t>((nb * per_body + 15) & ~15, (int64_t)b->joint_lds_max);
+        if (want > b->joint_lds) { b->joint_lds = (int32_t)want; invalidate_graph(b); }
     }
     i32 = (int32_t)s.j_off; if ((rc = put(b, d.w_js + slot, &i32, 1))) return rc;
     i32 = (int32_t)s.j_n; if ((rc = put(b, d.w_jn + slot, &i32, 1))) return rc;
@@ -704,7 +723,7 @@ static void enqueue_step(omi_batch* b, double dt) {
     mark();
     launch_cache(d, prec, s); mark();
     if (b->any_triggers) launch_trigger(d, s);
-    if (b->any_joints) launch_joints(d, prec, dt, s);
+    if (b->any_joints) launch_joints(d, prec, dt, b->joint_lds, s);
     mark();
     launch_post(d, b->nhw, dt, s); mark();
     launch_sleep(d, b->nhw, dt, s); mark();

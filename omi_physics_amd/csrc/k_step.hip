@@ -10,6 +10,8 @@
 //   warm-start cache rebuild                            -> k_cache      (1 block / world)
 //   integrate_positions                                 -> k_post       (1 thread / body)
 //   _update_sleep + _contact_islands                    -> k_sleep_*    (contacts, bodies)
+#include <type_traits>
+
 #include "omi_math.hpp"
 #include "omi_batch.hpp"
 #include "omi_gjk.hpp"
@@ -2033,13 +2035,15 @@ __device__ int32_t assign_levels(int32_t K, Get get, int32_t* last, int32_t* lev
     return D;
 }
 
-// 3x3 solve by LU with partial pivoting, LAPACK dgetf2 + dgetrs order
-// (np.linalg.solve in PointConstraint.solve, joints.py:119).
+// 3x3 LU factorisation with partial pivoting in LAPACK dgetf2 order; with
+// lu3_solve (dgetrs order) it is np.linalg.solve in PointConstraint.solve
+// (joints.py:119).  K does not change between the solve sweeps of a step, so
+// the factors are computed once per step and give the same result as
+// factoring K at every call.
 template <typename R>
-__device__ __forceinline__ void solve3(const R* K, const R* rhs, R* x) {
-    R A[3][3], b[3];
-    for (int r = 0; r < 3; r++) { b[r] = rhs[r]; for (int c = 0; c < 3; c++) A[r][c] = K[3 * r + c]; }
-    int piv[3];
+__device__ __forceinline__ void lu3(const R* K, R* LU, int* piv) {
+    R A[3][3];
+    for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) A[r][c] = K[3 * r + c];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         int p = j;
@@ -2054,11 +2058,18 @@ __device__ __forceinline__ void solve3(const R* K, const R* rhs, R* x) {
         for (int i = j + 1; i < 3; i++)
             for (int c = j + 1; c < 3; c++) A[i][c] -= A[i][j] * A[j][c];
     }
-    for (int j = 0; j < 3; j++) if (piv[j] != j) { R t = b[j]; b[j] = b[piv[j]]; b[piv[j]] = t; }
-    for (int i = 1; i < 3; i++) for (int k = 0; k < i; k++) b[i] -= A[i][k] * b[k];
+    for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) LU[3 * r + c] = A[r][c];
+}
+
+template <typename R>
+__device__ __forceinline__ void lu3_solve(const R* LU, int pv, const R* rhs, R* x) {
+    const int piv[3] = {pv & 3, (pv >> 2) & 3, 2};
+    R b[3] = {rhs[0], rhs[1], rhs[2]};
+    for (int j = 0; j < 2; j++) if (piv[j] != j) { R t = b[j]; b[j] = b[piv[j]]; b[piv[j]] = t; }
+    for (int i = 1; i < 3; i++) for (int k = 0; k < i; k++) b[i] -= LU[3 * i + k] * b[k];
     for (int i = 2; i >= 0; i--) {
-        for (int k = i + 1; k < 3; k++) b[i] -= A[i][k] * b[k];
-        b[i] /= A[i][i];
+        for (int k = i + 1; k < 3; k++) b[i] -= LU[3 * i + k] * b[k];
+        b[i] /= LU[3 * i + i];
     }
     x[0] = b[0]; x[1] = b[1]; x[2] = b[2];
 }
@@ -2074,16 +2085,41 @@ __device__ __forceinline__ void skew_sandwich(const R* r, const R* I, R* o) {
         for (int c = 0; c < 3; c++) o[3 * i + c] = (M[3 * i] * S[c] + M[3 * i + 1] * S[3 + c]) + M[3 * i + 2] * S[6 + c];
 }
 
-enum { JR_K = 0, JR_RA = 9, JR_RB = 12, JR_BIAS = 15, JR_N = 18 };
+// One joint's solver row, stored at its position in the level order:
+// LU factors of K, the pivots (p0 + 4 p1), rA, rB, bias / dt.
+enum { JR_LU = 0, JR_PIV = 9, JR_RA = 10, JR_RB = 13, JR_BDT = 16, JR_N = 19 };
+
+template <typename R>
+struct JointItem {
+    R v[JR_N];
+    int32_t ea, eb;   // end records: -1 = the world, else (index << 1) | dynamic
+};
+
+// Per body a joint solve stages in LDS: a record of six FP64 velocities,
+// the world inverse inertia (9 R) and inverse mass (R), padded to 16 bytes,
+// and the body's world-local index.
+template <typename R>
+struct JointBody {
+    static constexpr int vel = 48;
+    static constexpr int rec = 48 + (10 * (int)sizeof(R) + 15) / 16 * 16;
+    static constexpr int lds = rec + 4;
+};
+constexpr int JOINT_LEVELS = 1024;
 
 // PhysicsWorld._solve_joints (world.py:599-611): wake the joint bodies,
 // PointConstraint.prepare every joint (joints.py:84-101), then
 // joint_iterations Gauss-Seidel sweeps of PointConstraint.solve
 // (joints.py:111-121) in joint order, run level by level like the contacts.
+// The velocities of the bodies the joints touch live in LDS during the
+// sweeps when they fit in lds_bytes and the schedule has at most
+// JOINT_LEVELS levels (else the sweeps update the body arrays); each thread
+// loads its item of the next level while it solves the current one.
 template <typename R, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_joints(Dev d, double dt) {
+__global__ __launch_bounds__(BLOCK) void k_joints(Dev d, double dt, int32_t lds_bytes) {
+    extern __shared__ __align__(16) unsigned char s_dyn[];
     __shared__ int32_t s_scan[BLOCK / 64 + 2];
-    __shared__ int32_t s_D;
+    __shared__ int32_t s_D, s_ns;
+    __shared__ int32_t s_lend[JOINT_LEVELS];
     const int32_t w = blockIdx.x;
     const int tid = threadIdx.x;
     if (w >= d.wcap || (d.w_flags[w] & (WF_ACTIVE | WF_JOINTS)) != (WF_ACTIVE | WF_JOINTS)) return;
@@ -2093,9 +2129,56 @@ __global__ __launch_bounds__(BLOCK) void k_joints(Dev d, double dt) {
     const bool fp64 = sizeof(R) == sizeof(double);
     R* row = (R*)d.jrow;
     const R* iiw = (const R*)d.iiw;
-#define JROW(c, j) row[(size_t)(c) * d.jcap + (size_t)(j)]
+    using JB = JointBody<R>;
+    const int32_t cap = lds_bytes / JB::lds;
+    unsigned char* s_rec = s_dyn;
+    int32_t* s_body = (int32_t*)(s_dyn + (size_t)cap * JB::rec);
+#define JROW(c, p) row[(size_t)(c) * d.jcap + (size_t)(p)]
+    int32_t* last = d.last + off;
+    for (int32_t i = tid; i < n; i += BLOCK) last[i] = 0;
+    if (tid == 0) s_ns = 0;
+    __syncthreads();
+    auto get = [&](int32_t t, int32_t& al, int32_t& bl) {
+        const int32_t a = d.ja[js + t], b = d.jb[js + t];
+        al = (a >= 0 && d.mt[a] == M_DYNAMIC) ? (int32_t)(a - off) : -1;
+        bl = (b >= 0 && d.mt[b] == M_DYNAMIC) ? (int32_t)(b - off) : -1;
+    };
+    const int32_t D = assign_levels<BLOCK>(J, get, last, d.jlevel + js, d.jcount + js, d.jorder + js, s_scan, &s_D);
+    // number the bodies the joints touch (static and kinematic ends included:
+    // their velocities are read)
+    for (int32_t i = tid; i < n; i += BLOCK) last[i] = -1;
+    __syncthreads();
     for (int32_t t = tid; t < J; t += BLOCK) {
-        const int64_t j = js + t;
+        const int32_t a = d.ja[js + t], b = d.jb[js + t];
+        if (a >= 0) last[a - off] = 0;
+        if (b >= 0) last[b - off] = 0;
+    }
+    __syncthreads();
+    for (int32_t i = tid; i < n; i += BLOCK)
+        if (last[i] == 0) {
+            const int32_t s = atomicAdd(&s_ns, 1);
+            last[i] = s;
+            if (s < cap) s_body[s] = i;
+        }
+    __syncthreads();
+    const int32_t ns = s_ns;
+    const bool in_lds = ns <= cap && D <= JOINT_LEVELS;
+    if (in_lds)
+        for (int32_t s = tid; s < ns; s += BLOCK) {
+            const int64_t i = off + s_body[s];
+            unsigned char* rec = s_rec + (size_t)s * JB::rec;
+            double2* v = (double2*)rec;
+            v[0] = make_double2(BODY(d.lv, 0, i), BODY(d.lv, 1, i));
+            v[1] = make_double2(BODY(d.lv, 2, i), BODY(d.lv, 3, i));
+            v[2] = make_double2(BODY(d.lv, 4, i), BODY(d.lv, 5, i));
+            R* I = (R*)(rec + JB::vel);
+            for (int r = 0; r < 9; r++) I[r] = IIW(iiw, i, r);
+            I[9] = (R)d.inv_mass[i];
+        }
+    const R rdt = (R)dt;
+    for (int32_t t = tid; t < J; t += BLOCK) {
+        const int32_t jo = d.jorder[js + t];
+        const int64_t j = js + jo, p = js + t;
         const int32_t a = d.ja[j], b = d.jb[j];
         if (a >= 0) { d.awake[a] = 1; d.timer[a] = 0.0; }
         if (b >= 0) { d.awake[b] = 1; d.timer[b] = 0.0; }
@@ -2115,20 +2198,20 @@ __global__ __launch_bounds__(BLOCK) void k_joints(Dev d, double dt) {
             for (int r = 0; r < 9; r++) I[r] = IIW(iiw, i, r);
             double q[4] = {BODY(d.quat, 0, i), BODY(d.quat, 1, i), BODY(d.quat, 2, i), BODY(d.quat, 3, i)}, Rm[9];
             quat_to_matrix(q, Rm);
-            const double p[3] = {BODY(d.pos, 0, i), BODY(d.pos, 1, i), BODY(d.pos, 2, i)};
+            const double pp[3] = {BODY(d.pos, 0, i), BODY(d.pos, 1, i), BODY(d.pos, 2, i)};
             if (!(has & (1 << s2))) {
-                const double dd[3] = {d.janc[j] - p[0], d.janc[d.jcap + j] - p[1], d.janc[2 * d.jcap + j] - p[2]};
+                const double dd[3] = {d.janc[j] - pp[0], d.janc[d.jcap + j] - pp[1], d.janc[2 * d.jcap + j] - pp[2]};
                 for (int c = 0; c < 3; c++)
                     loc[(size_t)c * d.jcap + j] = (Rm[c] * dd[0] + Rm[3 + c] * dd[1]) + Rm[6 + c] * dd[2];
                 has |= (uint8_t)(1 << s2);
             }
             const double l[3] = {loc[j], loc[d.jcap + j], loc[2 * d.jcap + j]};
-            for (int r = 0; r < 3; r++) rr[r] = (p[r] + ((Rm[3 * r] * l[0] + Rm[3 * r + 1] * l[1]) + Rm[3 * r + 2] * l[2])) - p[r];
+            for (int r = 0; r < 3; r++) rr[r] = (pp[r] + ((Rm[3 * r] * l[0] + Rm[3 * r + 1] * l[1]) + Rm[3 * r + 2] * l[2])) - pp[r];
         }
         d.jhas[j] = has;
         for (int r = 0; r < 3; r++) { rA[r] = (R)rAd[r]; rB[r] = (R)rBd[r]; }
         const R ima = a < 0 ? R(0) : (R)d.inv_mass[a], imb = b < 0 ? R(0) : (R)d.inv_mass[b];
-        R SA[9], SB[9];
+        R SA[9], SB[9], K[9], LU[9];
         skew_sandwich(rA, Ia, SA);
         skew_sandwich(rB, Ib, SB);
         for (int r = 0; r < 3; r++)
@@ -2136,71 +2219,163 @@ __global__ __launch_bounds__(BLOCK) void k_joints(Dev d, double dt) {
                 R k = (ima + imb) * (r == c ? R(1) : R(0));
                 k -= SA[3 * r + c];
                 k -= SB[3 * r + c];
-                JROW(JR_K + 3 * r + c, j) = k;
+                K[3 * r + c] = k;
             }
+        int piv[3];
+        lu3(K, LU, piv);
+        for (int r = 0; r < 9; r++) {
+            JROW(JR_LU + r, p) = LU[r];
+        }
+        JROW(JR_PIV, p) = (R)(piv[0] + 4 * piv[1]);
         for (int r = 0; r < 3; r++) {
             const double anc = d.janc[(size_t)r * d.jcap + j];
             const double pa = (a < 0 ? anc : BODY(d.pos, r, a)) + rAd[r];
             const double pb = (b < 0 ? anc : BODY(d.pos, r, b)) + rBd[r];
-            JROW(JR_RA + r, j) = rA[r]; JROW(JR_RB + r, j) = rB[r];
-            JROW(JR_BIAS + r, j) = (R)(0.2 * (pb - pa));
+            JROW(JR_RA + r, p) = rA[r]; JROW(JR_RB + r, p) = rB[r];
+            JROW(JR_BDT + r, p) = (R)(0.2 * (pb - pa)) / rdt;
         }
+        auto end = [&](int32_t i) -> int32_t {
+            if (i < 0) return -1;
+            const int32_t x = in_lds ? last[i - off] : i;
+            return (x << 1) | (d.mt[i] == M_DYNAMIC ? 1 : 0);
+        };
+        // jlevel and jorder of this position are free once the level order
+        // is read: they carry the end records through the sweeps
+        d.jlevel[p] = end(a);
+        d.jorder[p] = end(b);
     }
-    int32_t* last = d.last + off;
-    for (int32_t i = tid; i < n; i += BLOCK) last[i] = 0;
     __syncthreads();
-    auto get = [&](int32_t t, int32_t& al, int32_t& bl) {
-        const int32_t a = d.ja[js + t], b = d.jb[js + t];
-        al = (a >= 0 && d.mt[a] == M_DYNAMIC) ? (int32_t)(a - off) : -1;
-        bl = (b >= 0 && d.mt[b] == M_DYNAMIC) ? (int32_t)(b - off) : -1;
+    auto load_item = [&](int64_t p, JointItem<R>& it) {
+#pragma unroll
+        for (int c = 0; c < JR_N; c++) it.v[c] = JROW(c, p);
+        it.ea = d.jlevel[p];
+        it.eb = d.jorder[p];
     };
-    const int32_t D = assign_levels<BLOCK>(J, get, last, d.jlevel + js, d.jcount + js, d.jorder + js, s_scan, &s_D);
-    const int32_t iters = d.w_jiters[w];
-    const R rdt = (R)dt;
-    for (int32_t it = 0; it < iters; it++) {
-        for (int32_t l = 0; l < D; l++) {
-            const int32_t e = d.jcount[js + l], bg = l == 0 ? 0 : d.jcount[js + l - 1];
-            for (int32_t t = bg + tid; t < e; t += BLOCK) {
-                const int64_t j = js + d.jorder[js + t];
-                const int32_t a = d.ja[j], b = d.jb[j];
-                R va[6], vb[6], rA[3], rB[3], Kj[9], rhs[3], P[3];
-                for (int r = 0; r < 3; r++) { rA[r] = JROW(JR_RA + r, j); rB[r] = JROW(JR_RB + r, j); }
-                for (int r = 0; r < 9; r++) Kj[r] = JROW(JR_K + r, j);
-                for (int c = 0; c < 6; c++) { va[c] = a >= 0 ? (R)BODY(d.lv, c, a) : R(0); vb[c] = b >= 0 ? (R)BODY(d.lv, c, b) : R(0); }
-                R pva[3] = {R(0), R(0), R(0)}, pvb[3] = {R(0), R(0), R(0)};
-                if (a >= 0) {
-                    pva[0] = va[0] + (va[4] * rA[2] - va[5] * rA[1]);
-                    pva[1] = va[1] + (va[5] * rA[0] - va[3] * rA[2]);
-                    pva[2] = va[2] + (va[3] * rA[1] - va[4] * rA[0]);
-                }
-                if (b >= 0) {
-                    pvb[0] = vb[0] + (vb[4] * rB[2] - vb[5] * rB[1]);
-                    pvb[1] = vb[1] + (vb[5] * rB[0] - vb[3] * rB[2]);
-                    pvb[2] = vb[2] + (vb[3] * rB[1] - vb[4] * rB[0]);
-                }
-                for (int r = 0; r < 3; r++) rhs[r] = -((pvb[r] - pva[r]) + JROW(JR_BIAS + r, j) / rdt);
-                solve3(Kj, rhs, P);
-                for (int s2 = 0; s2 < 2; s2++) {
-                    const int32_t i = s2 ? b : a;
-                    if (i < 0 || d.mt[i] != M_DYNAMIC) continue;
-                    R* v = s2 ? vb : va;
-                    const R* rr = s2 ? rB : rA;
-                    const R Q[3] = {s2 ? P[0] : -P[0], s2 ? P[1] : -P[1], s2 ? P[2] : -P[2]};
-                    const R im = (R)d.inv_mass[i];
-                    R nv[6];
-                    for (int r = 0; r < 3; r++) nv[r] = v[r] + im * Q[r];
-                    const R cx = rr[1] * Q[2] - rr[2] * Q[1], cy = rr[2] * Q[0] - rr[0] * Q[2], cz = rr[0] * Q[1] - rr[1] * Q[0];
-                    for (int r = 0; r < 3; r++)
-                        nv[3 + r] = v[3 + r] + ((IIW(iiw, i, 3 * r) * cx + IIW(iiw, i, 3 * r + 1) * cy) + IIW(iiw, i, 3 * r + 2) * cz);
-                    for (int c = 0; c < 6; c++) {
-                        if (fp64) BODY(d.lv, c, i) = (double)nv[c];
-                        else BODY(d.lv, c, i) = BODY(d.lv, c, i) + (double)(nv[c] - v[c]);
-                    }
-                }
+    // LDS and body-array forms of the sweeps are compiled apart (L), so that
+    // the LDS form never waits for the item loads in flight.
+    auto vel_get = [&](auto L, int32_t e, double* v) {
+        const int32_t x = e >> 1;
+        if constexpr (decltype(L)::value) {
+            const double2* q = (const double2*)(s_rec + (size_t)x * JB::rec);
+            const double2 q0 = q[0], q1 = q[1], q2 = q[2];
+            v[0] = q0.x; v[1] = q0.y; v[2] = q1.x; v[3] = q1.y; v[4] = q2.x; v[5] = q2.y;
+        } else {
+            for (int c = 0; c < 6; c++) v[c] = BODY(d.lv, c, x);
+        }
+    };
+    auto vel_put = [&](auto L, int32_t e, const double* v) {
+        const int32_t x = e >> 1;
+        if constexpr (decltype(L)::value) {
+            double2* q = (double2*)(s_rec + (size_t)x * JB::rec);
+            q[0] = make_double2(v[0], v[1]); q[1] = make_double2(v[2], v[3]); q[2] = make_double2(v[4], v[5]);
+        } else {
+            for (int c = 0; c < 6; c++) BODY(d.lv, c, x) = v[c];
+        }
+    };
+    auto solve_item = [&](auto L, const JointItem<R>& it) {
+        const R* rA = it.v + JR_RA;
+        const R* rB = it.v + JR_RB;
+        double wa[6] = {0, 0, 0, 0, 0, 0}, wb[6] = {0, 0, 0, 0, 0, 0};
+        R va[6], vb[6], rhs[3], P[3];
+        if (it.ea >= 0) vel_get(L, it.ea, wa);
+        if (it.eb >= 0) vel_get(L, it.eb, wb);
+        for (int c = 0; c < 6; c++) { va[c] = (R)wa[c]; vb[c] = (R)wb[c]; }
+        R pva[3] = {R(0), R(0), R(0)}, pvb[3] = {R(0), R(0), R(0)};
+        if (it.ea >= 0) {
+            pva[0] = va[0] + (va[4] * rA[2] - va[5] * rA[1]);
+            pva[1] = va[1] + (va[5] * rA[0] - va[3] * rA[2]);
+            pva[2] = va[2] + (va[3] * rA[1] - va[4] * rA[0]);
+        }
+        if (it.eb >= 0) {
+            pvb[0] = vb[0] + (vb[4] * rB[2] - vb[5] * rB[1]);
+            pvb[1] = vb[1] + (vb[5] * rB[0] - vb[3] * rB[2]);
+            pvb[2] = vb[2] + (vb[3] * rB[1] - vb[4] * rB[0]);
+        }
+        for (int r = 0; r < 3; r++) rhs[r] = -((pvb[r] - pva[r]) + it.v[JR_BDT + r]);
+        lu3_solve(it.v + JR_LU, (int)it.v[JR_PIV], rhs, P);
+        for (int s2 = 0; s2 < 2; s2++) {
+            const int32_t e = s2 ? it.eb : it.ea;
+            if (e < 0 || !(e & 1)) continue;
+            const R* v = s2 ? vb : va;
+            double* wv = s2 ? wb : wa;
+            const R* rr = s2 ? rB : rA;
+            R I[10];   // world inverse inertia, inverse mass
+            if constexpr (decltype(L)::value) {
+                const R* q = (const R*)(s_rec + (size_t)(e >> 1) * JB::rec + JB::vel);
+                for (int r = 0; r < 10; r++) I[r] = q[r];
+            } else {
+                for (int r = 0; r < 9; r++) I[r] = IIW(iiw, e >> 1, r);
+                I[9] = (R)d.inv_mass[e >> 1];
             }
+            const R Q[3] = {s2 ? P[0] : -P[0], s2 ? P[1] : -P[1], s2 ? P[2] : -P[2]};
+            const R im = I[9];
+            R nv[6];
+            for (int r = 0; r < 3; r++) nv[r] = v[r] + im * Q[r];
+            const R cx = rr[1] * Q[2] - rr[2] * Q[1], cy = rr[2] * Q[0] - rr[0] * Q[2], cz = rr[0] * Q[1] - rr[1] * Q[0];
+            for (int r = 0; r < 3; r++) nv[3 + r] = v[3 + r] + ((I[3 * r] * cx + I[3 * r + 1] * cy) + I[3 * r + 2] * cz);
+            for (int c = 0; c < 6; c++) wv[c] = fp64 ? (double)nv[c] : wv[c] + (double)(nv[c] - v[c]);
+            vel_put(L, e, wv);
+        }
+    };
+    const int32_t iters = d.w_jiters[w];
+    // end of each level in the order (an LDS copy in the LDS form)
+    if (in_lds)
+        for (int32_t l = tid; l < D; l += BLOCK) s_lend[l] = d.jcount[js + l];
+    __syncthreads();
+    // One step per (sweep, level).  A step loads this thread's item of the
+    // next step while it solves its item of this one; with the velocities in
+    // LDS the barrier between steps orders LDS only, so those loads stay in
+    // flight across it.  Unrolled by two so the item buffers swap roles
+    // instead of being copied (a copy would wait for the loads).
+    const int32_t G = iters * D;
+    auto jstep = [&](auto L, int32_t l, bool more, JointItem<R>& cur, JointItem<R>& nxt) {
+        auto lend = [&](int32_t k) -> int32_t {
+            if constexpr (decltype(L)::value) return s_lend[k];
+            else return d.jcount[js + k];
+        };
+        const int32_t bg = l == 0 ? 0 : lend(l - 1), e = lend(l);
+        if (more) {
+            const int32_t l2 = l + 1 == D ? 0 : l + 1;
+            const int32_t bg2 = l2 == 0 ? 0 : lend(l2 - 1), e2 = lend(l2);
+            if (bg2 + tid < e2) load_item(js + bg2 + tid, nxt);
+        }
+        if (bg + tid < e) solve_item(L, cur);
+        for (int32_t t = bg + tid + BLOCK; t < e; t += BLOCK) {
+            JointItem<R> x;
+            load_item(js + t, x);
+            solve_item(L, x);
+        }
+        if constexpr (decltype(L)::value) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        } else {
             __syncthreads();
         }
-    }
+    };
+    auto sweeps = [&](auto L) {
+        JointItem<R> itA, itB;
+        if (G > 0 && tid < d.jcount[js]) load_item(js + tid, itA);
+        int32_t l = 0;
+        for (int32_t g = 0; g < G; g += 2) {
+            jstep(L, l, g + 1 < G, itA, itB);
+            l = l + 1 == D ? 0 : l + 1;
+            if (g + 1 < G) {
+                jstep(L, l, g + 2 < G, itB, itA);
+                l = l + 1 == D ? 0 : l + 1;
+            }
+        }
+    };
+    if (in_lds) sweeps(std::true_type{});
+    else sweeps(std::false_type{});
+    __syncthreads();
+    if (in_lds)
+        for (int32_t s = tid; s < ns; s += BLOCK) {
+            const int64_t i = off + s_body[s];
+            if (d.mt[i] != M_DYNAMIC) continue;
+            const double* v = (const double*)(s_rec + (size_t)s * JB::rec);
+            for (int c = 0; c < 6; c++) BODY(d.lv, c, i) = v[c];
+        }
 #undef JROW
 }
 
@@ -2609,6 +2784,11 @@ bool allow_dynamic_lds(int bytes) {
     return ok;
 }
 
+// The joint launches' dynamic LDS (they also hold about 4 KB of static LDS).
+bool allow_joint_lds(int bytes) {
+    return allow_lds_fn(k_joints<double, 256>, bytes) && allow_lds_fn(k_joints<float, 256>, bytes);
+}
+
 // Solver rows for every contact (one thread per contact).
 void launch_prepare(const Dev& d, int precision, SolveIn in, bool stage, hipStream_t s) {
     const int g = GRID_FOR(d.kcap, 256);
@@ -2635,9 +2815,13 @@ void launch_cache(const Dev& d, int precision, hipStream_t s) {
     else hipLaunchKernelGGL((k_cache<float, 256>), dim3(d.wcap), dim3(256), 0, s, d);
 }
 
-void launch_joints(const Dev& d, int precision, double dt, hipStream_t s) {
-    if (precision == 64) hipLaunchKernelGGL((k_joints<double, 256>), dim3(d.wcap), dim3(256), 0, s, d, dt);
-    else hipLaunchKernelGGL((k_joints<float, 256>), dim3(d.wcap), dim3(256), 0, s, d, dt);
+// One block per world; lds_bytes (a multiple of 16) of dynamic LDS holds
+// the staged records of up to lds_bytes / 132 (FP64) or / 100 (FP32) joint
+// bodies per world.
+void launch_joints(const Dev& d, int precision, double dt, int32_t lds_bytes, hipStream_t s) {
+    if (precision == 64)
+        hipLaunchKernelGGL((k_joints<double, 256>), dim3(d.wcap), dim3(256), lds_bytes, s, d, dt, lds_bytes);
+    else hipLaunchKernelGGL((k_joints<float, 256>), dim3(d.wcap), dim3(256), lds_bytes, s, d, dt, lds_bytes);
 }
 
 void launch_post(const Dev& d, int64_t nhw, double dt, hipStream_t s) {

@@ -127,7 +127,7 @@ struct Dev {
     int32_t *ja, *jb;                     // [jcap] body index or -1 (the world)
     double *janc, *jla, *jlb;             // [3][jcap] world anchor, local anchors
     uint8_t *jhas;                        // bit 0 / 1: local anchor of a / b fixed
-    void *jrow;                           // R [18][jcap]: K (9), rA, rB, bias
+    void *jrow;                           // R [19][jcap]: per level-order position, the joint solver rows
     int32_t *jlevel, *jorder, *jcount;    // level schedule of the joints
     int32_t *w_js, *w_jn, *w_jiters;      // per-world joint range, iterations
     int32_t *sel;                         // current cache buffer index (device)

@@ -483,6 +483,27 @@ def test_joint_chains_bit_exact(lib):
         assert_state_equal(b, ow, 'chains step %d' % s)
 
 
+def test_joint_sweeps_in_lds_and_in_body_arrays(lib):
+    """60 chains of 50 links touch more joint bodies (3000) than the joint sweeps
+    stage in LDS, so that world's sweeps update the body arrays; the small chain
+    world in the same batch sweeps in LDS.  Both FP64 bit-exact for 40 steps."""
+    big = helpers.world(scenes.hinge_chains(n_chains=60, links=50, seed=7, top=30.0)).export_arrays()
+    small = helpers.world(scenes.hinge_chains(n_chains=2, links=6, seed=8, spacing=1.5, top=4.0)).export_arrays()
+    assert len(big['joint_a']) == 3000
+    say('chains 3000 + 12 links, fp64 40 steps')
+    b = batch_from_arrays([big, small], precision='fp64')
+    ows = [oracle.OracleWorld(big), oracle.OracleWorld(small)]
+    for _ in range(4):
+        b.step(DT, 10)
+        b.sync()
+        for slot, ow in enumerate(ows):
+            ow.step(DT, 10)
+            g, o = b.read(slot), ow.state()
+            for key in ('position', 'orientation', 'linear_velocity', 'angular_velocity'):
+                assert np.array_equal(g[key], o[key]), (slot, key, np.abs(g[key] - o[key]).max())
+    assert np.abs(ows[0].state()['angular_velocity']).max() > 0.1
+
+
 def test_physicsworld_pendulum_on_hip(lib):
     """A ball joint to the world keeps a pendulum's bob at its rod length (joints.py:60-121)."""
     from omi_physics_amd.joints import PointConstraint
