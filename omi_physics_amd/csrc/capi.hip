@@ -338,10 +338,10 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     d.jcap = JC;
     A(d.ja, JC); A(d.jb, JC); A(d.janc, 3 * JC); A(d.jla, 3 * JC); A(d.jlb, 3 * JC); A(d.jhas, JC);
     unsigned char* jr;
-    A(jr, 19 * JC * (int64_t)rs);
+    A(jr, (rs == 8 ? 160 : 96) * JC);     // JointItem<R>
     d.jrow = jr;
-    A(d.jlevel, JC); A(d.jorder, JC); A(d.jcount, JC);
-    A(d.w_js, W); A(d.w_jn, W); A(d.w_jiters, W);
+    A(d.jlevel, JC); A(d.jorder, JC); A(d.jcount, JC); A(d.jend, JC); A(d.jbody, 2 * JC);
+    A(d.w_js, W); A(d.w_jn, W); A(d.w_jiters, W); A(d.w_jsched, W); A(d.w_jns, W);
     A(d.sel, 1); A(d.cnt, C_COUNT); A(d.hslot_n, 2);
     A(b->d_wl, 4 * W); A(b->d_wn, 4);
 #undef A
@@ -561,6 +561,7 @@ extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_wo
     i32 = (int32_t)s.j_off; if ((rc = put(b, d.w_js + slot, &i32, 1))) return rc;
     i32 = (int32_t)s.j_n; if ((rc = put(b, d.w_jn + slot, &i32, 1))) return rc;
     i32 = w->joint_iterations; if ((rc = put(b, d.w_jiters + slot, &i32, 1))) return rc;
+    i32 = 0; if ((rc = put(b, d.w_jsched + slot, &i32, 1))) return rc;   // joint schedule rebuilt
     s.flags = WF_ACTIVE | (w->has_collision ? WF_COLLISION : 0) | (w->sleep_enabled ? WF_SLEEP : 0) |
               (s.j_n > 0 ? WF_JOINTS : 0);
     i32 = (int32_t)s.off; if ((rc = put(b, d.w_off + slot, &i32, 1))) return rc;

@@ -2090,10 +2090,11 @@ __device__ __forceinline__ void skew_sandwich(const R* r, const R* I, R* o) {
 enum { JR_LU = 0, JR_PIV = 9, JR_RA = 10, JR_RB = 13, JR_BDT = 16, JR_N = 19 };
 
 template <typename R>
-struct JointItem {
+struct alignas(16) JointItem {
     R v[JR_N];
     int32_t ea, eb;   // end records: -1 = the world, else (index << 1) | dynamic
 };
+static_assert(sizeof(JointItem<double>) == 160 && sizeof(JointItem<float>) == 96, "joint row sizes the host allocates");
 
 // Per body a joint solve stages in LDS: a record of six FP64 velocities,
 // the world inverse inertia (9 R) and inverse mass (R), padded to 16 bytes,
@@ -2110,10 +2111,13 @@ constexpr int JOINT_LEVELS = 1024;
 // PointConstraint.prepare every joint (joints.py:84-101), then
 // joint_iterations Gauss-Seidel sweeps of PointConstraint.solve
 // (joints.py:111-121) in joint order, run level by level like the contacts.
-// The velocities of the bodies the joints touch live in LDS during the
-// sweeps when they fit in lds_bytes and the schedule has at most
-// JOINT_LEVELS levels (else the sweeps update the body arrays); each thread
-// loads its item of the next level while it solves the current one.
+// The level schedule depends only on the joint list and the motion types,
+// which are fixed while the world is loaded: it is built on the world's first
+// step and kept (w_jsched = 2 levels + LDS form, 0 = none).  The velocities of
+// the bodies the joints touch live in LDS during the sweeps when they fit in
+// lds_bytes and the schedule has at most JOINT_LEVELS levels (else the sweeps
+// update the body arrays); each thread loads its item of the next level while
+// it solves the current one.
 template <typename R, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_joints(Dev d, double dt, int32_t lds_bytes) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
@@ -2127,45 +2131,68 @@ __global__ __launch_bounds__(BLOCK) void k_joints(Dev d, double dt, int32_t lds_
     const int64_t off = d.w_off[w];
     if (J == 0) return;
     const bool fp64 = sizeof(R) == sizeof(double);
-    R* row = (R*)d.jrow;
+    JointItem<R>* rows = (JointItem<R>*)d.jrow;
     const R* iiw = (const R*)d.iiw;
     using JB = JointBody<R>;
     const int32_t cap = lds_bytes / JB::lds;
     unsigned char* s_rec = s_dyn;
     int32_t* s_body = (int32_t*)(s_dyn + (size_t)cap * JB::rec);
-#define JROW(c, p) row[(size_t)(c) * d.jcap + (size_t)(p)]
-    int32_t* last = d.last + off;
-    for (int32_t i = tid; i < n; i += BLOCK) last[i] = 0;
-    if (tid == 0) s_ns = 0;
-    __syncthreads();
-    auto get = [&](int32_t t, int32_t& al, int32_t& bl) {
-        const int32_t a = d.ja[js + t], b = d.jb[js + t];
-        al = (a >= 0 && d.mt[a] == M_DYNAMIC) ? (int32_t)(a - off) : -1;
-        bl = (b >= 0 && d.mt[b] == M_DYNAMIC) ? (int32_t)(b - off) : -1;
-    };
-    const int32_t D = assign_levels<BLOCK>(J, get, last, d.jlevel + js, d.jcount + js, d.jorder + js, s_scan, &s_D);
-    // number the bodies the joints touch (static and kinematic ends included:
-    // their velocities are read)
-    for (int32_t i = tid; i < n; i += BLOCK) last[i] = -1;
-    __syncthreads();
-    for (int32_t t = tid; t < J; t += BLOCK) {
-        const int32_t a = d.ja[js + t], b = d.jb[js + t];
-        if (a >= 0) last[a - off] = 0;
-        if (b >= 0) last[b - off] = 0;
+    int32_t sched = d.w_jsched[w];
+    {
+        const int32_t ns0 = d.w_jns[w], D0 = sched >> 1;
+        if (sched != 0 && (sched & 1) != (ns0 <= cap && D0 <= JOINT_LEVELS)) sched = 0;   // LDS budget changed
     }
-    __syncthreads();
-    for (int32_t i = tid; i < n; i += BLOCK)
-        if (last[i] == 0) {
-            const int32_t s = atomicAdd(&s_ns, 1);
-            last[i] = s;
-            if (s < cap) s_body[s] = i;
+    if (sched == 0) {
+        // ---- build the schedule: levels, joint order, the joint bodies
+        int32_t* last = d.last + off;
+        for (int32_t i = tid; i < n; i += BLOCK) last[i] = 0;
+        if (tid == 0) s_ns = 0;
+        __syncthreads();
+        auto get = [&](int32_t t, int32_t& al, int32_t& bl) {
+            const int32_t a = d.ja[js + t], b = d.jb[js + t];
+            al = (a >= 0 && d.mt[a] == M_DYNAMIC) ? (int32_t)(a - off) : -1;
+            bl = (b >= 0 && d.mt[b] == M_DYNAMIC) ? (int32_t)(b - off) : -1;
+        };
+        const int32_t D = assign_levels<BLOCK>(J, get, last, d.jlevel + js, d.jcount + js, d.jorder + js, s_scan, &s_D);
+        // number the bodies the joints touch (static and kinematic ends
+        // included: their velocities are read)
+        for (int32_t i = tid; i < n; i += BLOCK) last[i] = -1;
+        __syncthreads();
+        for (int32_t t = tid; t < J; t += BLOCK) {
+            const int32_t a = d.ja[js + t], b = d.jb[js + t];
+            if (a >= 0) last[a - off] = 0;
+            if (b >= 0) last[b - off] = 0;
         }
-    __syncthreads();
-    const int32_t ns = s_ns;
-    const bool in_lds = ns <= cap && D <= JOINT_LEVELS;
+        __syncthreads();
+        for (int32_t i = tid; i < n; i += BLOCK)
+            if (last[i] == 0) {
+                const int32_t s2 = atomicAdd(&s_ns, 1);
+                last[i] = s2;
+                d.jbody[2 * (int64_t)js + s2] = i;
+            }
+        __syncthreads();
+        const int32_t ns = s_ns;
+        const bool lds = ns <= cap && D <= JOINT_LEVELS;
+        auto end = [&](int32_t i) -> int32_t {
+            if (i < 0) return -1;
+            const int32_t x = lds ? last[i - off] : i;
+            return (x << 1) | (d.mt[i] == M_DYNAMIC ? 1 : 0);
+        };
+        for (int32_t t = tid; t < J; t += BLOCK) {
+            const int64_t j = js + d.jorder[js + t];
+            d.jend[js + t] = make_int2(end(d.ja[j]), end(d.jb[j]));
+        }
+        sched = 2 * D + (lds ? 1 : 0);
+        if (tid == 0) { d.w_jsched[w] = sched; d.w_jns[w] = ns; }
+        __syncthreads();
+    }
+    const int32_t D = sched >> 1, ns = d.w_jns[w];
+    const bool in_lds = sched & 1;
     if (in_lds)
         for (int32_t s = tid; s < ns; s += BLOCK) {
-            const int64_t i = off + s_body[s];
+            const int32_t bl = d.jbody[2 * (int64_t)js + s];
+            s_body[s] = bl;
+            const int64_t i = off + bl;
             unsigned char* rec = s_rec + (size_t)s * JB::rec;
             double2* v = (double2*)rec;
             v[0] = make_double2(BODY(d.lv, 0, i), BODY(d.lv, 1, i));
@@ -2211,7 +2238,7 @@ __global__ __launch_bounds__(BLOCK) void k_joints(Dev d, double dt, int32_t lds_
         d.jhas[j] = has;
         for (int r = 0; r < 3; r++) { rA[r] = (R)rAd[r]; rB[r] = (R)rBd[r]; }
         const R ima = a < 0 ? R(0) : (R)d.inv_mass[a], imb = b < 0 ? R(0) : (R)d.inv_mass[b];
-        R SA[9], SB[9], K[9], LU[9];
+        R SA[9], SB[9], K[9];
         skew_sandwich(rA, Ia, SA);
         skew_sandwich(rB, Ib, SB);
         for (int r = 0; r < 3; r++)
@@ -2222,35 +2249,22 @@ __global__ __launch_bounds__(BLOCK) void k_joints(Dev d, double dt, int32_t lds_
                 K[3 * r + c] = k;
             }
         int piv[3];
-        lu3(K, LU, piv);
-        for (int r = 0; r < 9; r++) {
-            JROW(JR_LU + r, p) = LU[r];
-        }
-        JROW(JR_PIV, p) = (R)(piv[0] + 4 * piv[1]);
+        JointItem<R> it;
+        lu3(K, it.v + JR_LU, piv);
+        it.v[JR_PIV] = (R)(piv[0] + 4 * piv[1]);
         for (int r = 0; r < 3; r++) {
             const double anc = d.janc[(size_t)r * d.jcap + j];
             const double pa = (a < 0 ? anc : BODY(d.pos, r, a)) + rAd[r];
             const double pb = (b < 0 ? anc : BODY(d.pos, r, b)) + rBd[r];
-            JROW(JR_RA + r, p) = rA[r]; JROW(JR_RB + r, p) = rB[r];
-            JROW(JR_BDT + r, p) = (R)(0.2 * (pb - pa)) / rdt;
+            it.v[JR_RA + r] = rA[r]; it.v[JR_RB + r] = rB[r];
+            it.v[JR_BDT + r] = (R)(0.2 * (pb - pa)) / rdt;
         }
-        auto end = [&](int32_t i) -> int32_t {
-            if (i < 0) return -1;
-            const int32_t x = in_lds ? last[i - off] : i;
-            return (x << 1) | (d.mt[i] == M_DYNAMIC ? 1 : 0);
-        };
-        // jlevel and jorder of this position are free once the level order
-        // is read: they carry the end records through the sweeps
-        d.jlevel[p] = end(a);
-        d.jorder[p] = end(b);
+        const int2 ends = d.jend[p];
+        it.ea = ends.x; it.eb = ends.y;
+        rows[p] = it;
     }
     __syncthreads();
-    auto load_item = [&](int64_t p, JointItem<R>& it) {
-#pragma unroll
-        for (int c = 0; c < JR_N; c++) it.v[c] = JROW(c, p);
-        it.ea = d.jlevel[p];
-        it.eb = d.jorder[p];
-    };
+    auto load_item = [&](int64_t p, JointItem<R>& it) { it = rows[p]; };
     // LDS and body-array forms of the sweeps are compiled apart (L), so that
     // the LDS form never waits for the item loads in flight.
     auto vel_get = [&](auto L, int32_t e, double* v) {
@@ -2376,7 +2390,6 @@ __global__ __launch_bounds__(BLOCK) void k_joints(Dev d, double dt, int32_t lds_
             const double* v = (const double*)(s_rec + (size_t)s * JB::rec);
             for (int c = 0; c < 6; c++) BODY(d.lv, c, i) = v[c];
         }
-#undef JROW
 }
 
 // ------------------------------------------------------- warm-start cache --

@@ -127,9 +127,12 @@ struct Dev {
     int32_t *ja, *jb;                     // [jcap] body index or -1 (the world)
     double *janc, *jla, *jlb;             // [3][jcap] world anchor, local anchors
     uint8_t *jhas;                        // bit 0 / 1: local anchor of a / b fixed
-    void *jrow;                           // R [19][jcap]: per level-order position, the joint solver rows
-    int32_t *jlevel, *jorder, *jcount;    // level schedule of the joints
+    void *jrow;                           // [jcap] joint solver rows in level order (160 B FP64, 96 B FP32)
+    int32_t *jlevel, *jorder, *jcount;    // level schedule of the joints: level, order -> joint, level ends
+    int2 *jend;                           // [jcap] end records of each level-order position
+    int32_t *jbody;                       // [2*jcap] world-local body of each joint-body slot
     int32_t *w_js, *w_jn, *w_jiters;      // per-world joint range, iterations
+    int32_t *w_jsched, *w_jns;            // per world: 2 levels + LDS form (0: none), joint bodies
     int32_t *sel;                         // current cache buffer index (device)
     uint64_t *cnt;                        // counters
 };
