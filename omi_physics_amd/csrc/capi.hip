@@ -309,9 +309,12 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     d.pchunk_cap = (P + CPT_CHUNK - 1) / CPT_CHUNK + 1;
     A(d.pfx, 2 * (P + 1)); A(d.pchunk, 2 * d.pchunk_cap);
     if (cfg->max_hull_pts > 1) {
-        // GJK/EPA workspaces: one per thread of the convex narrowphase launch
+        // GJK/EPA workspaces: one per thread of the convex narrowphase
+        // launch, the pair capacity clamped to [4096, 16384] (16384 take
+        // about 300 MB), so that a large world's queued pairs spread over
+        // every CU
         unsigned char* ew;
-        d.epa_slots = 4096;
+        d.epa_slots = (int32_t)std::min<int64_t>(16384, std::max<int64_t>(4096, (P + 63) / 64 * 64));
         A(ew, (int64_t)d.epa_slots * (int64_t)epa_work_bytes());
         d.epa_work = ew;
     } else {
