@@ -46,6 +46,7 @@ void launch_solve_class(const Dev& d, int precision, int block, SolveIn in, cons
 size_t epa_work_bytes();
 bool allow_dynamic_lds(int bytes);
 bool allow_joint_lds(int bytes);
+int32_t allow_convex_lds();
 void launch_stage_convex(int32_t np, const int32_t* ka, const double* ca, const double* Ra, const double* ha,
                          const int32_t* oa, const int32_t* na, const int32_t* kb, const double* cb, const double* Rb,
                          const double* hb, const int32_t* ob, const int32_t* nb, const double* hull, void* work,
@@ -311,12 +312,13 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     if (cfg->max_hull_pts > 1) {
         // GJK/EPA workspaces: one per thread of the convex narrowphase
         // launch, the pair capacity clamped to [4096, 16384] (16384 take
-        // about 300 MB), so that a large world's queued pairs spread over
+        // about 380 MB), so that a large world's queued pairs spread over
         // every CU
         unsigned char* ew;
         d.epa_slots = (int32_t)std::min<int64_t>(16384, std::max<int64_t>(4096, (P + 63) / 64 * 64));
         A(ew, (int64_t)d.epa_slots * (int64_t)epa_work_bytes());
         d.epa_work = ew;
+        d.cvx_lds = allow_convex_lds();
     } else {
         d.epa_slots = 0;
         d.epa_work = nullptr;

@@ -606,9 +606,13 @@ __device__ __forceinline__ void make_proxy(const Dev& d, int64_t i, Proxy& P) {
 
 // collide.collide fallback to gjk.collide_convex (collide.py:251-269) for the
 // pairs k_narrow queued; FP64 in both precisions, one thread per pair, each
-// with its own EPA workspace.
+// with its own EPA workspace.  With wp_lds bytes of dynamic LDS
+// (2 * 3 * HULL_STAGE_PTS doubles per thread) each thread first stages its
+// two hulls' world points there, element-major so that the lanes of the
+// wavefront read consecutive words, and the support loops read LDS.
 template <typename R>
-__global__ __launch_bounds__(64) void k_narrow_convex(Dev d) {
+__global__ __launch_bounds__(64) void k_narrow_convex(Dev d, int32_t wp_lds) {
+    extern __shared__ __align__(16) unsigned char s_dyn[];
     const uint64_t tot = d.cnt[C_CONVEX];
     const int64_t n = tot < (uint64_t)d.pcap ? (int64_t)tot : d.pcap;
     const int64_t slot = gtid();
@@ -621,6 +625,11 @@ __global__ __launch_bounds__(64) void k_narrow_convex(Dev d) {
         Proxy A, B;
         make_proxy(d, d.pa[p], A);
         make_proxy(d, d.pb[p], B);
+        if (wp_lds) {
+            double* s = (double*)s_dyn + threadIdx.x;
+            stage_world_pts(A, s, 64);
+            stage_world_pts(B, s + 3 * HULL_STAGE_PTS * 64, 64);
+        }
         double nrm[3] = {0.0, 0.0, 0.0}, pt[3], dep = 0.0;
         int cnt = collide_convex(A, B, W, nrm, pt, dep);
         d.mcnt[p] = (int8_t)cnt;
@@ -2717,8 +2726,8 @@ void launch_narrow(const Dev& d, int precision, hipStream_t s) {
     else hipLaunchKernelGGL(k_narrow<float>, dim3(g), dim3(NARROW_BLOCK), 0, s, d);
     if (d.epa_slots > 0) {
         int ge = (d.epa_slots + 63) / 64;
-        if (precision == 64) hipLaunchKernelGGL(k_narrow_convex<double>, dim3(ge), dim3(64), 0, s, d);
-        else hipLaunchKernelGGL(k_narrow_convex<float>, dim3(ge), dim3(64), 0, s, d);
+        if (precision == 64) hipLaunchKernelGGL(k_narrow_convex<double>, dim3(ge), dim3(64), d.cvx_lds, s, d, d.cvx_lds);
+        else hipLaunchKernelGGL(k_narrow_convex<float>, dim3(ge), dim3(64), d.cvx_lds, s, d, d.cvx_lds);
     }
 }
 
@@ -2795,6 +2804,13 @@ bool allow_dynamic_lds(int bytes) {
     ok = allow_lds_fn(k_sap_sort<256, false>, bytes) && ok; ok = allow_lds_fn(k_sap_sort<512, false>, bytes) && ok;
     ok = allow_lds_fn(k_sap_sort<1024, false>, bytes) && ok;
     return ok;
+}
+
+// The convex narrowphase's staged hull points: returns the dynamic LDS a
+// launch takes, or 0 when the runtime refuses it.
+int32_t allow_convex_lds() {
+    constexpr int bytes = 2 * 3 * HULL_STAGE_PTS * 64 * 8;
+    return allow_lds_fn(k_narrow_convex<double>, bytes) && allow_lds_fn(k_narrow_convex<float>, bytes) ? bytes : 0;
 }
 
 // The joint launches' dynamic LDS (they also hold about 4 KB of static LDS).

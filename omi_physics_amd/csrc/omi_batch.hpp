@@ -103,6 +103,7 @@ struct Dev {
     int32_t *cvx;                         // [pcap] pairs that need GJK/EPA
     void *epa_work;                       // EpaWork per GJK/EPA thread
     int32_t epa_slots;
+    int32_t cvx_lds;                      // dynamic LDS of the convex narrowphase (0: hull points from HBM)
     double *mpts;                         // [3][4*pcap] absolute points
     void *mnrm, *mdep;                    // R [3][pcap], R [4*pcap]
     // contacts

@@ -19,12 +19,21 @@ struct Proxy {
     double c[3], R[9], h[3];
     const double* local;      // convex: local points [m][3]
     int m;
+    const double* wp = nullptr;   // convex: world points, element k at wp[k * ws] (else from local)
+    int ws = 0;
 };
+
+// Hulls of up to this many points get their world points staged once per
+// pair (the convex narrowphase keeps them in LDS).
+constexpr int HULL_STAGE_PTS = 32;
 
 struct EpaWork {              // one per thread
     double v[EPA_MAXV][3], a[EPA_MAXV][3], b[EPA_MAXV][3];
     int f[EPA_MAXF][3];
-    int kept[EPA_MAXF][3];
+    // per face, computed once when the face is made: the unit normal
+    // oriented away from the origin and its distance (+inf for a degenerate
+    // face), which _closest_face recomputes from the same vertices each time
+    double fn[EPA_MAXF][3], fd[EPA_MAXF];
     int ek[3 * EPA_MAXF][2];
     unsigned char ealive[3 * EPA_MAXF];
 };
@@ -42,6 +51,28 @@ __device__ __forceinline__ void sub3(const double* a, const double* b, double* o
 __device__ __forceinline__ void convex_pt(const Proxy& P, int p, double* o) {
     const double* l = P.local + 3 * p;
     for (int r = 0; r < 3; r++) o[r] = ((l[0] * P.R[3 * r] + l[1] * P.R[3 * r + 1]) + l[2] * P.R[3 * r + 2]) + P.c[r];
+}
+
+// The same point from the staged world points when there are any.
+__device__ __forceinline__ void convex_w(const Proxy& P, int p, double* o) {
+    if (P.wp) {
+        for (int r = 0; r < 3; r++) o[r] = P.wp[(size_t)(3 * p + r) * P.ws];
+    } else {
+        convex_pt(P, p, o);
+    }
+}
+
+// Stage a convex proxy's world points at base (element k at base[k * stride])
+// when it has at most HULL_STAGE_PTS of them; the values are convex_pt's.
+__device__ __forceinline__ void stage_world_pts(Proxy& P, double* base, int stride) {
+    if (P.kind != K_CONVEX || P.m > HULL_STAGE_PTS) return;
+    for (int p = 0; p < P.m; p++) {
+        double o[3];
+        convex_pt(P, p, o);
+        for (int r = 0; r < 3; r++) base[(size_t)(3 * p + r) * stride] = o[r];
+    }
+    P.wp = base;
+    P.ws = stride;
 }
 
 // SphereProxy/BoxProxy/ConvexProxy.support (body.py:46-48, 69-73, 125-128)
@@ -62,11 +93,11 @@ __device__ void support1(const Proxy& P, const double* d, double* out) {
         int best = 0;
         double bv = 0.0, w[3];
         for (int p = 0; p < P.m; p++) {
-            convex_pt(P, p, w);
+            convex_w(P, p, w);
             double v = (w[0] * d[0] + w[1] * d[1]) + w[2] * d[2];
             if (p == 0 || v > bv) { bv = v; best = p; }
         }
-        convex_pt(P, best, out);
+        convex_w(P, best, out);
     }
 }
 
@@ -74,7 +105,7 @@ __device__ void support1(const Proxy& P, const double* d, double* out) {
 __device__ void center_hint(const Proxy& P, double* out) {
     if (P.kind != K_CONVEX) { out[0] = P.c[0]; out[1] = P.c[1]; out[2] = P.c[2]; return; }
     double s[3] = {0.0, 0.0, 0.0}, w[3];
-    for (int p = 0; p < P.m; p++) { convex_pt(P, p, w); for (int r = 0; r < 3; r++) s[r] += w[r]; }
+    for (int p = 0; p < P.m; p++) { convex_w(P, p, w); for (int r = 0; r < 3; r++) s[r] += w[r]; }
     for (int r = 0; r < 3; r++) out[r] = s[r] / (double)P.m;
 }
 
@@ -176,26 +207,37 @@ __device__ __forceinline__ void wput(EpaWork* W, int i, const Vtx& o) {
     for (int r = 0; r < 3; r++) { W->v[i][r] = o.v[r]; W->a[i][r] = o.a[r]; W->b[i][r] = o.b[r]; }
 }
 
-// gjk._closest_face (gjk.py:176-194)
+// One face's term of gjk._closest_face (gjk.py:176-194): unit normal and
+// distance, skipped (+inf) when the face is degenerate.
+__device__ __forceinline__ void face_eval(EpaWork* W, int i) {
+    const double *p0 = W->v[W->f[i][0]], *p1 = W->v[W->f[i][1]], *p2 = W->v[W->f[i][2]];
+    double e1[3], e2[3], nn[3];
+    sub3(p1, p0, e1); sub3(p2, p0, e2);
+    cross3(e1, e2, nn);
+    double ln = sqrt((nn[0] * nn[0] + nn[1] * nn[1]) + nn[2] * nn[2]);
+    if (ln < GJK_EPS) { W->fd[i] = __longlong_as_double(0x7ff0000000000000ll); return; }
+    nn[0] /= ln; nn[1] /= ln; nn[2] /= ln;
+    double d = dot3(nn, p0);
+    if (d < 0) { nn[0] = -nn[0]; nn[1] = -nn[1]; nn[2] = -nn[2]; d = -d; }
+    W->fn[i][0] = nn[0]; W->fn[i][1] = nn[1]; W->fn[i][2] = nn[2]; W->fd[i] = d;
+}
+
+// gjk._closest_face (gjk.py:176-194) over the faces' stored terms: the first
+// face of least distance
 __device__ int closest_face(const EpaWork* W, int nf, double* bn, double& bd) {
     int best = -1;
     bd = __longlong_as_double(0x7ff0000000000000ll);
     for (int i = 0; i < nf; i++) {
-        const double *p0 = W->v[W->f[i][0]], *p1 = W->v[W->f[i][1]], *p2 = W->v[W->f[i][2]];
-        double e1[3], e2[3], nn[3];
-        sub3(p1, p0, e1); sub3(p2, p0, e2);
-        cross3(e1, e2, nn);
-        double ln = sqrt((nn[0] * nn[0] + nn[1] * nn[1]) + nn[2] * nn[2]);
-        if (ln < GJK_EPS) continue;
-        nn[0] /= ln; nn[1] /= ln; nn[2] /= ln;
-        double d = dot3(nn, p0);
-        if (d < 0) { nn[0] = -nn[0]; nn[1] = -nn[1]; nn[2] = -nn[2]; d = -d; }
-        if (d < bd) { bd = d; bn[0] = nn[0]; bn[1] = nn[1]; bn[2] = nn[2]; best = i; }
+        const double d = W->fd[i];
+        if (d < bd) { bd = d; best = i; }
     }
+    if (best >= 0) { bn[0] = W->fn[best][0]; bn[1] = W->fn[best][1]; bn[2] = W->fn[best][2]; }
     return best;
 }
 
-// gjk._rebuild (gjk.py:197-217): horizon edges in first-seen order
+// gjk._rebuild (gjk.py:197-217): horizon edges in first-seen order.  The
+// faces that stay keep their order and are packed in place (the write
+// position never passes the face being read), then the new faces follow.
 __device__ int rebuild(EpaWork* W, int nf, int ni) {
     int ne = 0, nk = 0;
     const double* w = W->v[ni];
@@ -213,15 +255,20 @@ __device__ int rebuild(EpaWork* W, int nf, int ni) {
                 else { W->ek[ne][0] = es[e][0]; W->ek[ne][1] = es[e][1]; W->ealive[ne] = 1; ne++; }
             }
         } else {
-            W->kept[nk][0] = W->f[i][0]; W->kept[nk][1] = W->f[i][1]; W->kept[nk][2] = W->f[i][2]; nk++;
+            if (nk != i) {
+                for (int r = 0; r < 3; r++) { W->f[nk][r] = W->f[i][r]; W->fn[nk][r] = W->fn[i][r]; }
+                W->fd[nk] = W->fd[i];
+            }
+            nk++;
         }
     }
     for (int q = 0; q < ne; q++)
         if (W->ealive[q]) {
             if (nk >= EPA_MAXF) return -1;
-            W->kept[nk][0] = W->ek[q][0]; W->kept[nk][1] = W->ek[q][1]; W->kept[nk][2] = ni; nk++;
+            W->f[nk][0] = W->ek[q][0]; W->f[nk][1] = W->ek[q][1]; W->f[nk][2] = ni;
+            face_eval(W, nk);
+            nk++;
         }
-    for (int i = 0; i < nk; i++) { W->f[i][0] = W->kept[i][0]; W->f[i][1] = W->kept[i][1]; W->f[i][2] = W->kept[i][2]; }
     return nk;
 }
 
@@ -273,7 +320,10 @@ __device__ bool epa(const Proxy& A, const Proxy& B, const Vtx* simplex, int ns, 
     }
     nv = 4;
     const int init[4][3] = {{0, 1, 2}, {0, 2, 3}, {0, 3, 1}, {1, 3, 2}};
-    for (int i = 0; i < 4; i++) for (int j = 0; j < 3; j++) W->f[i][j] = init[i][j];
+    for (int i = 0; i < 4; i++) {
+        for (int j = 0; j < 3; j++) W->f[i][j] = init[i][j];
+        face_eval(W, i);
+    }
     int nf = 4;
     double bn[3], bd;
     for (int it = 0; it < 64; it++) {
