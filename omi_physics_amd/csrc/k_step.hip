@@ -2417,27 +2417,47 @@ __global__ void k_hash_clear(Dev d) {
 // The cache after this step (solver.py:271-278): this step's contacts with
 // their impulses, or — when the world had no contacts — last step's cache
 // unchanged (solver.py:197-198).  Group heads go into the hash.
+// Where each world's cache entries go this step: its contacts, or — when it
+// has none — last step's entries, placed by an exclusive scan over the worlds
+// in slot order; a world whose entries pass the capacity keeps none (and the
+// overflow is reported).  C_CACHE gets the total asked for, at most twice
+// the capacity (this step's contact slices and last step's entries each fit
+// in it), so the int32 scan cannot overflow.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_cache_plan(Dev d) {
+    __shared__ int32_t s_scan[BLOCK / 64 + 2];
+    const int32_t old = *d.sel, nw = old ^ 1;
+    int32_t* start = d.w_cache_s + (size_t)nw * d.wcap;
+    int32_t* count = d.w_cache_n + (size_t)nw * d.wcap;
+    for (int32_t w = threadIdx.x; w < d.wcap; w += BLOCK) {
+        int32_t c = 0;
+        if (d.w_flags[w] & WF_ACTIVE) {
+            const int32_t K = d.w_cn[w];
+            c = K > 0 ? K : d.w_cache_n[(size_t)old * d.wcap + w];
+        }
+        start[w] = c;
+        count[w] = c;
+    }
+    __syncthreads();
+    const int32_t tot = block_exscan<BLOCK>(start, d.wcap, s_scan);
+    bool over = false;
+    for (int32_t w = threadIdx.x; w < d.wcap; w += BLOCK)
+        if ((int64_t)start[w] + count[w] > d.kcap) { count[w] = 0; over = true; }
+    if (over) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 4ull);
+    if (threadIdx.x == 0) d.cnt[C_CACHE] = (uint64_t)tot;
+}
+
 template <typename R, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_cache(Dev d) {
-    __shared__ int32_t s_base[2];
     const int32_t w = blockIdx.x, tid = threadIdx.x;
     if (w >= d.wcap) return;
     const int32_t old = *d.sel, nw = old ^ 1;
     const size_t os = (size_t)old * d.wcap + w, ns = (size_t)nw * d.wcap + w;
-    if (!(d.w_flags[w] & WF_ACTIVE)) { if (tid == 0) d.w_cache_n[ns] = 0; return; }
+    if (!(d.w_flags[w] & WF_ACTIVE)) return;
     const int32_t K = d.w_cn[w];
     const int32_t cnt = K > 0 ? K : d.w_cache_n[os];
-    if (cnt == 0) { if (tid == 0) d.w_cache_n[ns] = 0; return; }
-    if (tid == 0) {
-        uint64_t base = atomicAdd((unsigned long long*)&d.cnt[C_CACHE], (unsigned long long)cnt);
-        int32_t ok = base + cnt <= (uint64_t)d.kcap;
-        if (!ok) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 4ull);
-        s_base[0] = (int32_t)base; s_base[1] = ok;
-        d.w_cache_s[ns] = (int32_t)base; d.w_cache_n[ns] = ok ? cnt : 0;
-    }
-    __syncthreads();
-    if (!s_base[1]) return;
-    const int32_t base = s_base[0];
+    if (cnt == 0 || d.w_cache_n[ns] != cnt) return;   // nothing to keep, or past the capacity
+    const int32_t base = d.w_cache_s[ns];
     const int32_t src = K > 0 ? d.w_cs[w] : d.w_cache_s[os];
     const int32_t* sa = K > 0 ? d.ca : d.ka[old];
     const int32_t* sb = K > 0 ? d.cb : d.kb[old];
@@ -2840,6 +2860,7 @@ void launch_solve_class(const Dev& d, int precision, int block, SolveIn in, cons
 
 void launch_cache(const Dev& d, int precision, hipStream_t s) {
     hipLaunchKernelGGL(k_hash_clear, dim3(GRID_FOR(d.kcap, 256)), dim3(256), 0, s, d);
+    hipLaunchKernelGGL((k_cache_plan<1024>), dim3(1), dim3(1024), 0, s, d);
     if (precision == 64) hipLaunchKernelGGL((k_cache<double, 256>), dim3(d.wcap), dim3(256), 0, s, d);
     else hipLaunchKernelGGL((k_cache<float, 256>), dim3(d.wcap), dim3(256), 0, s, d);
 }
