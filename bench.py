@@ -295,8 +295,8 @@ def main():
     ap.add_argument('--lanes', action='store_true', help='split the worlds by size class into lanes stepped side by side')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
-        print('bench.py measures one GPU only: this build has no multi-GPU path (worlds would shard by '
-              'world id with no collective; see DESIGN.md).', file=sys.stderr)
+        print('bench.py measures one GPU only: there is no multi-GPU path (independent worlds would shard '
+              'across GPUs by world id, with no collective on the step path).', file=sys.stderr)
         sys.exit(2)
     from omi_physics_amd import _ffi
     if _ffi.load().omi_device_count() < 1:
