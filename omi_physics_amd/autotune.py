@@ -12,7 +12,10 @@ fastest.
 
 The result is a small JSON table per precision (``tuned.json`` beside this
 file holds the ones measured for the 4096-world workload on MI355X) that
-``Batch`` takes as ``sap_block`` / ``solve_block``.  FP64 solver blocks stop
+``Batch`` takes as ``sap_block`` / ``solve_block``.  Classes absent from the
+tuning workload keep their table entry: the class of worlds over 4096 bodies
+was timed on the 9101-body pyramids-and-debris world (FP32 solver 512
+threads, FP64 256).  FP64 solver blocks stop
 at 256 threads: the FP64 sweep needs more registers per thread than a larger
 block leaves.
 """
@@ -28,7 +31,7 @@ TUNED = os.path.join(HERE, 'tuned.json')
 CANDIDATES = (64, 128, 256, 512, 1024)
 FP64_SOLVE_BLOCK_MAX = 256
 CLASS_LIMITS = (256, 1024, 4096)
-DEFAULTS = {'fp32': {'sap_block': [64, 128, 1024, 1024], 'solve_block': [64, 128, 256, 256]},
+DEFAULTS = {'fp32': {'sap_block': [64, 128, 1024, 1024], 'solve_block': [64, 128, 256, 512]},
             'fp64': {'sap_block': [64, 128, 1024, 1024], 'solve_block': [64, 128, 256, 256]}}
 
 
