@@ -2295,57 +2295,55 @@ __global__ __launch_bounds__(BLOCK) void k_joints(Dev d, double dt, int32_t lds_
             for (int c = 0; c < 6; c++) BODY(d.lv, c, x) = v[c];
         }
     };
+    // Two lanes per joint: lane 2q + h takes end h (0: a, 1: b) of joint q of
+    // the pass — its velocity, its anchor's point velocity and its share of
+    // the impulse — and both lanes solve the joint's 3x3 system from the same
+    // right-hand side.  A joint whose two ends are one body keeps only end
+    // b's update, as the sequential loop's second write does.
+    const int32_t half = tid & 1, jq = tid >> 1;
+    constexpr int32_t JPB = BLOCK / 2;   // joints per pass of a level
     auto solve_item = [&](auto L, const JointItem<R>& it) {
-        const R* rA = it.v + JR_RA;
-        const R* rB = it.v + JR_RB;
-        double wa[6] = {0, 0, 0, 0, 0, 0}, wb[6] = {0, 0, 0, 0, 0, 0};
-        R va[6], vb[6], rhs[3], P[3];
-        if (it.ea >= 0) vel_get(L, it.ea, wa);
-        if (it.eb >= 0) vel_get(L, it.eb, wb);
-        for (int c = 0; c < 6; c++) { va[c] = (R)wa[c]; vb[c] = (R)wb[c]; }
-        R pva[3] = {R(0), R(0), R(0)}, pvb[3] = {R(0), R(0), R(0)};
-        if (it.ea >= 0) {
-            pva[0] = va[0] + (va[4] * rA[2] - va[5] * rA[1]);
-            pva[1] = va[1] + (va[5] * rA[0] - va[3] * rA[2]);
-            pva[2] = va[2] + (va[3] * rA[1] - va[4] * rA[0]);
+        const int32_t e = half ? it.eb : it.ea;
+        const R* rr = it.v + (half ? JR_RB : JR_RA);
+        double wv[6] = {0, 0, 0, 0, 0, 0};
+        R v[6], rhs[3], P[3], pv[3] = {R(0), R(0), R(0)}, po[3];
+        if (e >= 0) vel_get(L, e, wv);
+        for (int c = 0; c < 6; c++) v[c] = (R)wv[c];
+        if (e >= 0) {
+            pv[0] = v[0] + (v[4] * rr[2] - v[5] * rr[1]);
+            pv[1] = v[1] + (v[5] * rr[0] - v[3] * rr[2]);
+            pv[2] = v[2] + (v[3] * rr[1] - v[4] * rr[0]);
         }
-        if (it.eb >= 0) {
-            pvb[0] = vb[0] + (vb[4] * rB[2] - vb[5] * rB[1]);
-            pvb[1] = vb[1] + (vb[5] * rB[0] - vb[3] * rB[2]);
-            pvb[2] = vb[2] + (vb[3] * rB[1] - vb[4] * rB[0]);
-        }
+        for (int r = 0; r < 3; r++) po[r] = __shfl_xor(pv[r], 1, 64);
+        const R* pva = half ? po : pv;
+        const R* pvb = half ? pv : po;
         for (int r = 0; r < 3; r++) rhs[r] = -((pvb[r] - pva[r]) + it.v[JR_BDT + r]);
         lu3_solve(it.v + JR_LU, (int)it.v[JR_PIV], rhs, P);
-        for (int s2 = 0; s2 < 2; s2++) {
-            const int32_t e = s2 ? it.eb : it.ea;
-            if (e < 0 || !(e & 1)) continue;
-            const R* v = s2 ? vb : va;
-            double* wv = s2 ? wb : wa;
-            const R* rr = s2 ? rB : rA;
-            R I[10];   // world inverse inertia, inverse mass
-            if constexpr (decltype(L)::value) {
-                const R* q = (const R*)(s_rec + (size_t)(e >> 1) * JB::rec + JB::vel);
-                for (int r = 0; r < 10; r++) I[r] = q[r];
-            } else {
-                for (int r = 0; r < 9; r++) I[r] = IIW(iiw, e >> 1, r);
-                I[9] = (R)d.inv_mass[e >> 1];
-            }
-            const R Q[3] = {s2 ? P[0] : -P[0], s2 ? P[1] : -P[1], s2 ? P[2] : -P[2]};
-            const R im = I[9];
-            R nv[6];
-            for (int r = 0; r < 3; r++) nv[r] = v[r] + im * Q[r];
-            const R cx = rr[1] * Q[2] - rr[2] * Q[1], cy = rr[2] * Q[0] - rr[0] * Q[2], cz = rr[0] * Q[1] - rr[1] * Q[0];
-            for (int r = 0; r < 3; r++) nv[3 + r] = v[3 + r] + ((I[3 * r] * cx + I[3 * r + 1] * cy) + I[3 * r + 2] * cz);
-            for (int c = 0; c < 6; c++) wv[c] = fp64 ? (double)nv[c] : wv[c] + (double)(nv[c] - v[c]);
-            vel_put(L, e, wv);
+        const bool same = it.ea >= 0 && it.eb >= 0 && (it.ea >> 1) == (it.eb >> 1);
+        if (e < 0 || !(e & 1) || (same && !half)) return;
+        R I[10];   // world inverse inertia, inverse mass
+        if constexpr (decltype(L)::value) {
+            const R* q = (const R*)(s_rec + (size_t)(e >> 1) * JB::rec + JB::vel);
+            for (int r = 0; r < 10; r++) I[r] = q[r];
+        } else {
+            for (int r = 0; r < 9; r++) I[r] = IIW(iiw, e >> 1, r);
+            I[9] = (R)d.inv_mass[e >> 1];
         }
+        const R Q[3] = {half ? P[0] : -P[0], half ? P[1] : -P[1], half ? P[2] : -P[2]};
+        const R im = I[9];
+        R nv[6];
+        for (int r = 0; r < 3; r++) nv[r] = v[r] + im * Q[r];
+        const R cx = rr[1] * Q[2] - rr[2] * Q[1], cy = rr[2] * Q[0] - rr[0] * Q[2], cz = rr[0] * Q[1] - rr[1] * Q[0];
+        for (int r = 0; r < 3; r++) nv[3 + r] = v[3 + r] + ((I[3 * r] * cx + I[3 * r + 1] * cy) + I[3 * r + 2] * cz);
+        for (int c = 0; c < 6; c++) wv[c] = fp64 ? (double)nv[c] : wv[c] + (double)(nv[c] - v[c]);
+        vel_put(L, e, wv);
     };
     const int32_t iters = d.w_jiters[w];
     // end of each level in the order (an LDS copy in the LDS form)
     if (in_lds)
         for (int32_t l = tid; l < D; l += BLOCK) s_lend[l] = d.jcount[js + l];
     __syncthreads();
-    // One step per (sweep, level).  A step loads this thread's item of the
+    // One step per (sweep, level).  A step loads this lane pair's item of the
     // next step while it solves its item of this one; with the velocities in
     // LDS the barrier between steps orders LDS only, so those loads stay in
     // flight across it.  Unrolled by two so the item buffers swap roles
@@ -2360,10 +2358,10 @@ __global__ __launch_bounds__(BLOCK) void k_joints(Dev d, double dt, int32_t lds_
         if (more) {
             const int32_t l2 = l + 1 == D ? 0 : l + 1;
             const int32_t bg2 = l2 == 0 ? 0 : lend(l2 - 1), e2 = lend(l2);
-            if (bg2 + tid < e2) load_item(js + bg2 + tid, nxt);
+            if (bg2 + jq < e2) load_item(js + bg2 + jq, nxt);
         }
-        if (bg + tid < e) solve_item(L, cur);
-        for (int32_t t = bg + tid + BLOCK; t < e; t += BLOCK) {
+        if (bg + jq < e) solve_item(L, cur);
+        for (int32_t t = bg + jq + JPB; t < e; t += JPB) {
             JointItem<R> x;
             load_item(js + t, x);
             solve_item(L, x);
@@ -2378,7 +2376,7 @@ __global__ __launch_bounds__(BLOCK) void k_joints(Dev d, double dt, int32_t lds_
     };
     auto sweeps = [&](auto L) {
         JointItem<R> itA, itB;
-        if (G > 0 && tid < d.jcount[js]) load_item(js + tid, itA);
+        if (G > 0 && jq < d.jcount[js]) load_item(js + jq, itA);
         int32_t l = 0;
         for (int32_t g = 0; g < G; g += 2) {
             jstep(L, l, g + 1 < G, itA, itB);
