@@ -504,6 +504,35 @@ def test_joint_sweeps_in_lds_and_in_body_arrays(lib):
     assert np.abs(ows[0].state()['angular_velocity']).max() > 0.1
 
 
+def test_joint_schedule_rebuilt_on_reload(lib):
+    """The joint level schedule and joint-body LDS records are kept across steps
+    and rebuilt when a slot is reloaded: a small chain world steps, its slot is
+    refilled with a larger chain world (more joint bodies than the batch had
+    planned LDS for), and a second world joins; all stay FP64 bit-exact."""
+    first = helpers.world(scenes.hinge_chains(n_chains=2, links=5, seed=9, spacing=1.5, top=6.0)).export_arrays()
+    second = helpers.world(scenes.hinge_chains(n_chains=6, links=40, seed=10, spacing=1.5, top=30.0)).export_arrays()
+    third = helpers.world(scenes.hinge_chains(n_chains=3, links=8, seed=11, spacing=1.5, top=10.0)).export_arrays()
+    n = max(first['n'], second['n']) + third['n']
+    b = Batch(2, n, precision='fp64', max_joints=len(second['joint_a']) + len(third['joint_a']))
+    say('joint schedule: 10 links, then 240 links in the same slot, then a third world')
+    b.load(0, first)
+    ow = oracle.OracleWorld(first)
+    b.step(DT, 20); b.sync(); ow.step(DT, 20)
+    assert np.array_equal(b.read(0)['position'], ow.state()['position'])
+    b.remove(0)
+    b.load(0, second)
+    b.load(1, third)
+    ows = [oracle.OracleWorld(second), oracle.OracleWorld(third)]
+    for _ in range(3):
+        b.step(DT, 10); b.sync()
+        for slot, o in enumerate(ows):
+            o.step(DT, 10)
+            g, st = b.read(slot), o.state()
+            for key in ('position', 'orientation', 'linear_velocity', 'angular_velocity'):
+                assert np.array_equal(g[key], st[key]), (slot, key, np.abs(g[key] - st[key]).max())
+    b.close()
+
+
 def test_physicsworld_pendulum_on_hip(lib):
     """A ball joint to the world keeps a pendulum's bob at its rod length (joints.py:60-121)."""
     from omi_physics_amd.joints import PointConstraint
