@@ -255,6 +255,29 @@ def test_batch_equals_alone(lib):
             assert np.array_equal(x[key], y[key]), (i, key)
 
 
+def test_full_4096_world_batch_fp64_matches_oracle_sample(lib):
+    """The batched workload at its full size: 4096 box/sphere worlds (lognormal
+    sizes, 436778 bodies) stepped 300 times in FP64 in one graph-captured batch,
+    with no capacity overflow; the three largest worlds and five spread over the
+    slots equal the oracle bit for bit."""
+    sizes = scenes.world_sizes(4096, seed=3)
+    arrays = [scenes.arrays(scenes.boxes_on_ground(int(sizes[w]), seed=3 * 100003 + w, sphere_fraction=0.5))
+              for w in range(4096)]
+    say('4096 worlds, %d bodies, fp64 300 steps' % sum(a['n'] for a in arrays))
+    b = batch_from_arrays(arrays, precision='fp64')
+    b.step(DT, 300)
+    b.sync()
+    assert b.counters()['overflow'] == 0
+    largest = [int(i) for i in np.argsort([-a['n'] for a in arrays], kind='stable')[:3]]
+    for i in largest + [0, 5, 100, 2000, 4095]:
+        ow = oracle.OracleWorld(arrays[i])
+        ow.step(DT, 300)
+        g, o = b.read(i), ow.state()
+        for key in ('position', 'orientation', 'linear_velocity', 'angular_velocity', 'awake'):
+            assert np.array_equal(g[key], o[key]), (i, key)
+    b.close()
+
+
 @pytest.mark.parametrize('precision', ['fp64', 'fp32'])
 def test_group_lanes_equal_single_batch(lib, precision):
     """Worlds of three size classes split into lanes (one member batch per class,
