@@ -496,6 +496,30 @@ def test_convex_world_bit_exact(lib):
     assert contacts > 10
 
 
+def test_pyramids_and_debris_world_fp64_bit_exact(lib):
+    """The large single world at its full size (20 ten-level pyramids + 8000
+    box/sphere debris, 9101 bodies; its solver state lives in HBM): 40 FP64 steps
+    equal the oracle bit for bit."""
+    arrays = scenes.arrays(scenes.pyramids_and_debris())
+    say('pyramids and debris, 9101 bodies, fp64 40 steps')
+    for s, b, ow in run_both(arrays, 40, check_every=20):
+        assert_state_equal(b, ow, 'pyramids step %d' % s)
+
+
+def test_convex_and_chains_world_fp64_bit_exact(lib):
+    """Hulls and hinge chains in one world (the convex-and-joints workload at
+    300 hulls and 6 chains of 30 links): GJK/EPA contacts, joints in LDS and the
+    contact solve together, 200 FP64 steps equal the oracle bit for bit."""
+    arrays = scenes.build(scenes.convex_and_chains(n_hulls=300, n_chains=6, links=30, seed=5),
+                          PhysicsWorld, model).export_arrays()
+    say('300 hulls + 6 chains x 30, fp64 200 steps')
+    contacts = 0
+    for s, b, ow in run_both(arrays, 200, check_every=50):
+        assert_state_equal(b, ow, 'convex+chains step %d' % s)
+        contacts = max(contacts, b.counters()['contacts'])
+    assert contacts > 100
+
+
 def test_joint_chains_bit_exact(lib):
     """PointConstraint chains (joints.py:60-121) with contacts against the ground, 120 steps FP64."""
     arrays = helpers.world(scenes.hinge_chains(n_chains=3, links=8, seed=5, spacing=1.5,
