@@ -237,6 +237,9 @@ class Batch:
 
 
 def _single_batch(worlds: Sequence[dict], precision: str, **kw) -> Batch:
+    if 'sap_block' not in kw and 'solve_block' not in kw:
+        from .autotune import blocks_for
+        kw.update(blocks_for([int(w['n']) for w in worlds], precision))
     total = int(sum(int(w['n']) for w in worlds))
     hull = int(sum(len(np.asarray(w['hull_pts']).reshape(-1, 3)) for w in worlds))
     joints = int(sum(int(w.get('n_joints', 0)) for w in worlds))
