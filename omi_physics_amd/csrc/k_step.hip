@@ -567,7 +567,7 @@ __device__ __forceinline__ int narrow_pair(const Dev& d, int64_t a, int64_t b, R
 }
 
 template <typename R>
-__global__ __launch_bounds__(NARROW_BLOCK) void k_narrow(Dev d) {
+__global__ __launch_bounds__(NARROW_BLOCK, 2) void k_narrow(Dev d) {
     const uint64_t tot = d.cnt[C_PAIRS];
     const int64_t P = tot < (uint64_t)d.pcap ? (int64_t)tot : d.pcap;
     R* mn = (R*)d.mnrm;
