@@ -566,6 +566,9 @@ __device__ __forceinline__ int narrow_pair(const Dev& d, int64_t a, int64_t b, R
     return cnt;
 }
 
+// Compiled for at least two wavefronts per SIMD: the FP64 box-box path then
+// fits the 256 vector registers (no accumulator-register spills) and two
+// wavefronts hide each other's latency.
 template <typename R>
 __global__ __launch_bounds__(NARROW_BLOCK, 2) void k_narrow(Dev d) {
     const uint64_t tot = d.cnt[C_PAIRS];
