@@ -58,24 +58,31 @@ def _horizon(visible: List[Face]) -> List[Tuple[int, int]]:
 
 
 def convex_hull(points) -> Tuple[np.ndarray, np.ndarray]:
-    """``(vertices, triangles)`` of the hull; degenerate input returns the unique points."""
+    """``(vertices, triangles)`` of the hull; degenerate input returns the unique points.
+
+    Each face's unit normal and reference vertex are computed once, when the face
+    is created, and every point tests all current faces in one array operation.
+    """
     pts = np.unique(np.asarray(points, dtype='d'), axis=0)
     if len(pts) < 4:
         return pts, np.zeros((0, 3), dtype='i')
     faces = _seed(pts)
     if faces is None:
         return pts, np.zeros((0, 3), dtype='i')
+    planes = [_plane(pts, f) for f in faces]
+    nrm = np.array([pl[0] for pl in planes])
+    ref = np.array([pl[1] for pl in planes])
     for p in range(len(pts)):
-        q = pts[p]
-        visible = []
-        for f in faces:
-            n, ref = _plane(pts, f)
-            if np.dot(n, q - ref) > EPS:
-                visible.append(f)
-        if not visible:
+        sees = ((pts[p] - ref) * nrm).sum(axis=1) > EPS
+        if not sees.any():
             continue
-        edges = _horizon(visible)
-        faces = [f for f in faces if f not in visible] + [(i, j, p) for i, j in edges]
+        visible = [f for f, v in zip(faces, sees) if v]
+        new = [(i, j, p) for i, j in _horizon(visible)]
+        keep = ~sees
+        faces = [f for f, v in zip(faces, keep) if v] + new
+        planes = [_plane(pts, f) for f in new]
+        nrm = np.concatenate([nrm[keep], np.array([pl[0] for pl in planes]).reshape(-1, 3)])
+        ref = np.concatenate([ref[keep], np.array([pl[1] for pl in planes]).reshape(-1, 3)])
     used = np.unique(np.asarray(faces).ravel())
     remap = {int(o): k for k, o in enumerate(used)}
     tris = np.array([[remap[i] for i in f] for f in faces], dtype='i')

@@ -65,3 +65,50 @@ def trigger_scene(n=60, seed=4):
              linear_velocity=(0.0, -1.0, 0.0)),
     ]
     return spec
+
+
+def damped_kinematic_scene(n=90, seed=21, asleep_every=4):
+    """A pile whose bodies carry every integrate_forces term and which kinematic
+    pushers sweep through, with part of it starting asleep.
+
+    Per-body linear/angular damping, quadratic drag and gravity factors are
+    random; the world adds default damping; three kinematic boxes move at
+    constant velocity (one spinning) across the pile; every ``asleep_every``-th
+    dynamic body starts asleep, so contacts with an awake mover wake it
+    (solver.py:321-337) and the kinematic bodies count as awake movers in the
+    solid-pair filter (world.py:381-384).  Returns ``export_world_arrays`` output.
+    """
+    from omi_physics_amd import PhysicsWorld
+    rng = np.random.default_rng(seed)
+    w = PhysicsWorld(gravity=model.Gravity(gravity=9.81, direction=(0.0, -1.0, 0.0)),
+                     default_linear_damping=0.05, default_angular_damping=0.12)
+    mat = w.add_material(model.Material(staticFriction=0.5, dynamicFriction=0.5))
+    ground = w.add_shape(model.Shape.box((24.0, 1.0, 24.0)))
+    w.add_body(model.Motion(type=model.STATIC), collider=model.Collider(shape=ground, physicsMaterial=mat),
+               position=(0.0, -0.5, 0.0))
+    box = w.add_shape(model.Shape.box((1.0, 1.0, 1.0)))
+    ball = w.add_shape(model.Shape.sphere(0.5))
+    pusher = w.add_shape(model.Shape.box((1.0, 1.6, 5.0)))
+    for k in range(n):
+        q = mathutil.quat_from_axis_angle(tuple(rng.normal(size=3)), float(rng.uniform(0, 2 * np.pi)))
+        m = model.Motion(mass=float(rng.uniform(0.5, 2.0)),
+                         linearVelocity=tuple(rng.normal(scale=0.5, size=3)),
+                         angularVelocity=tuple(rng.normal(scale=1.0, size=3)),
+                         gravityFactor=float(rng.uniform(0.5, 1.5)),
+                         linearDamping=float(rng.uniform(0.0, 0.8)),
+                         angularDamping=float(rng.uniform(0.0, 0.8)),
+                         quadraticDrag=float(rng.uniform(0.0, 0.3)))
+        w.add_body(m, collider=model.Collider(shape=box if k % 2 else ball, physicsMaterial=mat),
+                   position=(float(rng.uniform(-3, 3)), float(rng.uniform(0.6, 5.0)), float(rng.uniform(-3, 3))),
+                   orientation=tuple(q))
+    for x0, v, spin in ((-8.0, (2.0, 0.0, 0.0), (0.0, 0.0, 0.0)), (8.0, (-1.5, 0.0, 0.3), (0.0, 0.8, 0.0)),
+                        (0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))):
+        w.add_body(model.Motion(type=model.KINEMATIC, linearVelocity=v, angularVelocity=spin),
+                   collider=model.Collider(shape=pusher, physicsMaterial=mat),
+                   position=(x0, 0.8, 0.0) if x0 else (0.0, 0.8, 9.0))
+    arrays = w.export_arrays()
+    aw = arrays['awake'].copy()
+    dyn = np.flatnonzero(arrays['motion_type'] == 2)
+    aw[dyn[::asleep_every]] = 0
+    arrays['awake'] = aw
+    return arrays

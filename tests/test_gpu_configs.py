@@ -1,0 +1,227 @@
+"""The workloads at their full sizes on the GPU against the oracle.
+
+FP64: bit-exact against the oracle for every 300-step workload (pyramids and
+debris, 9101 bodies; 2000 cooked hulls and 20 hinge chains of 50 links).
+
+FP32 (narrowphase and solver arithmetic in FP32, integration and contact points
+in FP64): one step from an identical state against the FP64 oracle, quantity by
+quantity.  The broadphase runs in FP64 in both forms, so the solid pair lists are
+compared bit for bit.  A manifold is "matched" when the FP32 step gives it the same
+contact count, a normal within 1e-3 and the same points (within 1e-3 m) as the
+oracle; the others "flip": a count flip (a pair that touches at depth ~0 in one
+form only, or a clip point at the keep threshold), an axis flip (two SAT axes
+whose overlaps differ by less than FP32 resolves) or a point flip (another choice
+of the four kept points).  The flip share and the errors on matched manifolds are
+bounded by the tolerances written in check_fp32.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from omi_physics_amd import PhysicsWorld, model, scenes
+from omi_physics_amd.batch import batch_from_arrays
+
+pytestmark = pytest.mark.gpu
+DT = 1.0 / 60.0
+STATE = ('position', 'orientation', 'linear_velocity', 'angular_velocity', 'awake', 'sleep_timer')
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def assert_state_equal(b, slot, ow, where):
+    g, o = b.read(slot), ow.state()
+    for key in ('position', 'orientation', 'linear_velocity', 'angular_velocity', 'aabb_min', 'aabb_max',
+                'sleep_timer'):
+        if not np.array_equal(g[key], o[key]):
+            raise AssertionError('%s: %s differs (max %.3g)' % (where, key, np.abs(g[key] - o[key]).max()))
+    assert np.array_equal(g['awake'], o['awake']), where
+
+
+def run_fp64_against_oracle(arrays, steps, every, name):
+    b = batch_from_arrays([arrays], precision='fp64')
+    ow = oracle.OracleWorld(arrays)
+    done, peak = 0, 0
+    while done < steps:
+        k = min(every, steps - done)
+        b.step(DT, k)
+        b.sync()
+        ow.step(DT, k)
+        done += k
+        assert_state_equal(b, 0, ow, '%s step %d' % (name, done))
+        c = b.counters()
+        assert c['overflow'] == 0
+        peak = max(peak, c['contacts'])
+        say('  %s step %d: contacts %d, bit-exact' % (name, done, c['contacts']))
+    b.close()
+    return peak
+
+
+def test_pyramids_and_debris_full_300_steps_fp64():
+    """20 ten-level pyramids + 8000 box/sphere debris (9101 bodies) for the whole
+    300-step workload: FP64 equals the oracle bit for bit every 50 steps."""
+    arrays = scenes.arrays(scenes.pyramids_and_debris())
+    say('pyramids and debris, %d bodies, fp64, 300 steps' % arrays['n'])
+    assert arrays['n'] == 9101
+    assert run_fp64_against_oracle(arrays, 300, 50, 'pyramids') > 10000
+
+
+@pytest.fixture(scope='module')
+def convex_chains_arrays():
+    return scenes.build(scenes.convex_and_chains(), PhysicsWorld, model).export_arrays()
+
+
+def test_convex_and_chains_full_300_steps_fp64(convex_chains_arrays):
+    """2000 cooked ellipsoid hulls (GJK/EPA carries their narrowphase) and 20 hinge
+    chains of 50 links from static anchors, one world of 3001 bodies and 1000
+    PointConstraint joints, for the whole 300-step workload: FP64 equals the oracle
+    bit for bit every 100 steps."""
+    a = convex_chains_arrays
+    assert int((a['kind'] == 3).sum()) == 2000 and a['n_joints'] == 1000
+    say('2000 hulls + 20 chains x 50 links, %d bodies, fp64, 300 steps' % a['n'])
+    assert run_fp64_against_oracle(a, 300, 100, 'convex+chains') > 1000
+
+
+def test_damped_kinematic_scene_fp64():
+    """Every integrate_forces term (per-body damping, world default damping,
+    quadratic drag, gravity factors), kinematic pushers moving through a pile, and
+    sleepers woken by contact: 240 FP64 steps equal the oracle bit for bit, with the
+    pair lists and contacts equal every 10 steps."""
+    from tests import helpers
+    arrays = helpers.damped_kinematic_scene()
+    asleep0 = arrays['awake'] == 0
+    say('damped + kinematic scene: %d bodies, %d asleep at start' % (arrays['n'], int(asleep0.sum())))
+    b = batch_from_arrays([arrays], precision='fp64', record_broadphase=True)
+    ow = oracle.OracleWorld(arrays)
+    woke = 0
+    for s in range(24):
+        b.step(DT, 10); b.sync(); ow.step(DT, 10)
+        assert_state_equal(b, 0, ow, 'damped step %d' % (10 * s + 10))
+        assert np.array_equal(b.pairs(0, solid=True), ow.pairs(True)), s
+        gc, oc = b.contacts(0), ow.contacts()
+        for key in ('a', 'b', 'point', 'normal', 'depth', 'normal_impulse', 'tangent_impulse'):
+            assert np.array_equal(gc[key], oc[key]), (s, key)
+        woke = max(woke, int(b.read(0)['awake'][asleep0].sum()))
+    assert woke >= asleep0.sum() // 2
+    kin = np.flatnonzero(arrays['motion_type'] == 1)
+    np.testing.assert_allclose(b.read(0)['position'][kin], arrays['position'][kin] + 240 * DT * arrays['linear_velocity'][kin],
+                               atol=1e-11)
+    b.close()
+
+
+# ------------------------------------------------------------------ FP32 --
+
+def with_state(arrays, st):
+    out = dict(arrays)
+    for key in STATE:
+        out[key] = np.ascontiguousarray(st[key])
+    out['prev_position'] = out['position'].copy()
+    out['prev_orientation'] = out['orientation'].copy()
+    return out
+
+
+def manifolds(c):
+    """Contacts grouped by pair, in order: {(a, b): index array}."""
+    out = {}
+    for k, key in enumerate(zip(c['a'].tolist(), c['b'].tolist())):
+        out.setdefault(key, []).append(k)
+    return {k: np.array(v) for k, v in out.items()}
+
+
+def fp32_one_step_errors(arrays, warm_steps):
+    """Advance ``arrays`` ``warm_steps`` FP64 steps, then one FP32 step and one oracle
+    step from that state; per-quantity comparison."""
+    b64 = batch_from_arrays([arrays], precision='fp64')
+    b64.step(DT, warm_steps); b64.sync()
+    start = with_state(arrays, b64.read(0))
+    b64.close()
+    b32 = batch_from_arrays([start], precision='fp32', record_broadphase=True)
+    ow = oracle.OracleWorld(start)
+    b32.step(DT, 1); b32.sync(); ow.step(DT, 1)
+    out = {'pairs_equal': bool(np.array_equal(b32.pairs(0, solid=True), ow.pairs(True)) and
+                               np.array_equal(b32.pairs(0, solid=False), ow.pairs(False)))}
+    gc, oc = b32.contacts(0), ow.contacts()
+    gm, om = manifolds(gc), manifolds(oc)
+    matched, flips = [], {'count': 0, 'axis': 0, 'points': 0}
+    for key, oi in om.items():
+        gi = gm.get(key)
+        if gi is None or len(gi) != len(oi):
+            flips['count'] += 1
+        elif np.abs(gc['normal'][gi[0]] - oc['normal'][oi[0]]).max() > 1e-3:
+            flips['axis'] += 1
+        elif np.abs(gc['point'][gi] - oc['point'][oi]).max() > 1e-3:
+            flips['points'] += 1           # another clip point kept, or another reduce choice
+        else:
+            matched.append((gi, oi))
+    flips['count'] += sum(1 for key in gm if key not in om)
+    nm = len(om)
+    err = lambda f: max([float(np.abs(gc[f][gi] - oc[f][oi]).max()) for gi, oi in matched] or [0.0])
+    out.update(manifolds=nm, matched=len(matched), flipped=sum(flips.values()), flips=flips,
+               contacts=len(oc['a']), normal=err('normal'), depth=err('depth'), point=err('point'))
+    imax = max(1e-12, float(np.abs(oc['normal_impulse']).max()))
+    out['normal_impulse_rel'] = err('normal_impulse') / imax
+    out['tangent_impulse_rel'] = err('tangent_impulse') / imax
+    g, o = b32.read(0), ow.state()
+    out['position'] = float(np.abs(g['position'] - o['position']).max())
+    out['velocity'] = float(np.abs(g['linear_velocity'] - o['linear_velocity']).max())
+    out['angular_velocity'] = float(np.abs(g['angular_velocity'] - o['angular_velocity']).max())
+    out['orientation'] = float(np.abs(g['orientation'] - o['orientation']).max())
+    # bodies touched by no unmatched manifold: the solver sees the same rows up to FP32 rounding
+    touched = set()
+    mk = {(int(oc['a'][oi[0]]), int(oc['b'][oi[0]])) for _, oi in matched}
+    for key in list(om) + list(gm):
+        if key not in mk:
+            touched.update(key)
+    clean = np.ones(len(o['position']), bool)
+    clean[list(touched)] = False
+    out['velocity_clean'] = float(np.abs(g['linear_velocity'][clean] - o['linear_velocity'][clean]).max())
+    out['position_clean'] = float(np.abs(g['position'][clean] - o['position'][clean]).max())
+    b32.close()
+    return out
+
+
+def check_fp32(out):
+    """Tolerances of the FP32 form for one step from an identical state.
+
+    FP32 rounds at 6e-8 relative.  On matched manifolds: normals within 5e-5 (a
+    normalised edge-edge cross product loses eps32 / |e_a x e_b| when the edges are
+    nearly parallel), depths within 1e-5 m and points within 2e-5 m (coordinates of
+    a few metres in the pair's frame, then seated in FP64); impulses within 1e-5 of
+    the world's largest normal impulse.  Whole state: positions within 5e-6 m,
+    linear velocities within 2e-4 m/s and angular within 5e-4 rad/s (one step of
+    ten Gauss-Seidel sweeps over FP32 rows, flips included).  At most 5 % of
+    manifolds flip (count, axis or point choice), every one a near-tie."""
+    say('  fp32 one step:', {k: (round(v, 12) if isinstance(v, float) else v) for k, v in out.items()})
+    assert out['pairs_equal']
+    assert out['flipped'] <= 0.05 * (out['manifolds'] + out['flips']['count'])
+    assert out['normal'] <= 5e-5
+    assert out['depth'] <= 1e-5
+    assert out['point'] <= 2e-5
+    assert out['normal_impulse_rel'] <= 1e-5 and out['tangent_impulse_rel'] <= 1e-5
+    assert out['position'] <= 5e-6
+    assert out['velocity'] <= 2e-4 and out['angular_velocity'] <= 5e-4
+
+
+def test_fp32_one_step_pyramids_and_debris():
+    arrays = scenes.arrays(scenes.pyramids_and_debris())
+    say('fp32 one step, pyramids and debris after 120 fp64 steps')
+    check_fp32(fp32_one_step_errors(arrays, 120))
+
+
+def test_fp32_one_step_batched_sample():
+    sizes = scenes.world_sizes(4096, seed=3)
+    order = np.argsort(-sizes, kind='stable')
+    pick = [int(order[0]), int(order[5]), int(order[50]), int(order[400]), 7, 1000, 3000]
+    for w in pick:
+        arrays = scenes.arrays(scenes.boxes_on_ground(int(sizes[w]), seed=3 * 100003 + w, sphere_fraction=0.5))
+        say('fp32 one step, batched world %d (%d bodies) after 150 fp64 steps' % (w, arrays['n']))
+        check_fp32(fp32_one_step_errors(arrays, 150))
+
+
+def test_fp32_one_step_convex_and_chains(convex_chains_arrays):
+    say('fp32 one step, convex and chains after 100 fp64 steps')
+    check_fp32(fp32_one_step_errors(convex_chains_arrays, 100))

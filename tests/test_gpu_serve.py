@@ -49,3 +49,45 @@ def test_plain_form_runs():
     reqs = make_requests(3, seed=2, steps=5)
     out = run_plain(reqs)
     assert out['body_steps_per_s'] > 0 and all(r.result is not None for r in reqs)
+
+
+def full_size_requests(k, rate, seed=5):
+    """The serving workload's requests: Poisson arrivals of box/sphere worlds with
+    log-uniform 8..4096 bodies, 60 steps each (scenes.serving_requests)."""
+    spec = scenes.serving_requests(k, rate, seed=seed)
+    return [Request(id=i, arrays=scenes.arrays(scenes.boxes_on_ground(r['n'], seed=r['seed'], sphere_fraction=0.5)),
+                    steps=r['steps'], arrival=r['arrival']) for i, r in enumerate(spec)]
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+def test_served_full_size_requests_equal_alone(precision):
+    """28 requests of 8..4096 bodies x 60 steps arriving at 400/s into a server of
+    12 slots and 16384 bodies (so requests wait for slots and memory and slots are
+    reused): every result equals the request stepped alone, bit for bit; in FP64 the
+    largest request and a mid-sized one equal the oracle."""
+    import oracle
+    reqs = full_size_requests(28, 400.0)
+    sizes = [r.bodies for r in reqs]
+    print('serving %d full-size requests %s: bodies %d..%d' % (len(reqs), precision, min(sizes), max(sizes)),
+          flush=True)
+    assert max(sizes) > 2000 and min(sizes) < 64
+    srv = RolloutServer(max_worlds=12, max_bodies=16384, precision=precision)
+    stats = srv.run(reqs)
+    srv.close()
+    assert len(srv.finished) == len(reqs)
+    assert stats.summary()['ticks'] < sum(r.steps for r in reqs)
+    for r in reqs:
+        alone = batch_from_arrays([r.arrays], precision=precision)
+        alone.step(1.0 / 60.0, r.steps)
+        alone.sync()
+        ref = alone.read(0)
+        alone.close()
+        for key in ('position', 'orientation', 'linear_velocity', 'angular_velocity', 'awake'):
+            assert np.array_equal(r.result[key], ref[key]), (r.id, key)
+    if precision == 'fp64':
+        order = np.argsort(sizes)
+        for i in (int(order[-1]), int(order[len(order) // 2])):
+            ow = oracle.OracleWorld(reqs[i].arrays)
+            ow.step(1.0 / 60.0, reqs[i].steps)
+            for key in ('position', 'orientation', 'linear_velocity', 'angular_velocity', 'awake'):
+                assert np.array_equal(reqs[i].result[key], ow.state()[key]), (i, key)

@@ -430,3 +430,127 @@ def test_capsule_triangles_against_brute_force():
             kinds['rim'] += 1
             assert abs(dep[i] - (cap.radius - _seg_tri_distance(cap.p0, cap.p1, tri))) < 2e-2, i
     assert min(kinds.values()) > 0, kinds
+
+
+# ---- integrate_forces and the kinematic branches (backend.py:26-49,
+#      world.py:381-384, solver.py:321-337), pinned by discrete closed forms ----
+
+def _one_body(**motion):
+    from omi_physics_amd import PhysicsWorld, model
+    w = PhysicsWorld(gravity=model.Gravity(gravity=9.81, direction=(0, -1, 0)),
+                     default_linear_damping=motion.pop('dlin', 0.0),
+                     default_angular_damping=motion.pop('dang', 0.0), sleep_enabled=False)
+    w.add_body(model.Motion(type=model.DYNAMIC, **motion), position=(0, 0, 0))
+    return oracle.OracleWorld(w.export_arrays())
+
+
+@pytest.mark.parametrize('c,c0', [(2.0, 0.0), (1.5, 0.5)])
+def test_linear_damping_discrete_fixed_point(c, c0):
+    """Reference test_physics_integrate.py:74-83 (|v| -> g/c) in its discrete form:
+    v <- (v + g dt)(1 - (c + c0) dt) converges to g dt (1 - C dt) / (C dt), C = c + c0
+    (per-body plus world default).  Damping applied before gravity would converge to
+    g / C instead, 1 - C dt = 0.8 % away."""
+    g, dt = 9.81, 1.0 / 240.0
+    ow = _one_body(linearDamping=c, dlin=c0)
+    ow.step(dt, 4000)
+    C = c + c0
+    expect = -g * dt * (1.0 - C * dt) / (C * dt)
+    assert ow.state()['linear_velocity'][0, 1] == pytest.approx(expect, rel=1e-12)
+    assert abs(ow.state()['linear_velocity'][0, 1]) == pytest.approx(g / C, rel=1e-2)
+
+
+def test_quadratic_drag_discrete_fixed_point():
+    """Reference test_physics_integrate.py:86-95 (|v| -> sqrt(g/k)) in its discrete form:
+    with u = v + g dt and v <- u - k |u| u dt, the fixed point has k |u| u = g, so
+    v* = -sqrt(g/k) + g dt along y; the drag uses the speed after gravity."""
+    g, k, dt = 9.81, 0.5, 1.0 / 240.0
+    ow = _one_body(quadraticDrag=k)
+    ow.step(dt, 6000)
+    expect = -np.sqrt(g / k) + g * dt
+    assert ow.state()['linear_velocity'][0, 1] == pytest.approx(expect, rel=1e-12)
+
+
+def test_damping_and_drag_together_fixed_point():
+    """Both terms in the reference's order: gravity, then linear damping, then the drag
+    on the damped speed.  Along y, falling, with f = 1 - C dt: u = v - g dt, w = f u,
+    v' = w + k w^2 dt.  At the fixed point v = w / f + g dt, so w solves
+    k dt w^2 + (1 - 1/f) w - g dt = 0 (its negative root)."""
+    g, k, C, dt = 9.81, 0.3, 0.7, 1.0 / 120.0
+    ow = _one_body(quadraticDrag=k, linearDamping=C)
+    ow.step(dt, 8000)
+    f = 1.0 - C * dt
+    b = 1.0 - 1.0 / f
+    w = (-b - np.sqrt(b * b + 4.0 * k * dt * g * dt)) / (2.0 * k * dt)
+    assert ow.state()['linear_velocity'][0, 1] == pytest.approx(w / f + g * dt, rel=1e-11)
+
+
+def test_angular_damping_geometric():
+    """w_n = w_0 (1 - (c + c0) dt)^n for a free body (angular damping, world default added)."""
+    dt, c, c0 = 1.0 / 60.0, 0.9, 0.3
+    ow = _one_body(angularDamping=c, dang=c0, angularVelocity=(1.0, -2.0, 0.5), gravityFactor=0.0)
+    ow.step(dt, 100)
+    f = 1.0 - (c + c0) * dt
+    np.testing.assert_allclose(ow.state()['angular_velocity'][0], np.array([1.0, -2.0, 0.5]) * f ** 100,
+                               rtol=1e-12)
+
+
+def test_damping_clamps_at_zero():
+    """max(1 - c dt, 0): a damping larger than 1/dt stops the body outright each step."""
+    ow = _one_body(linearDamping=100.0, angularDamping=100.0, angularVelocity=(3.0, 0, 0),
+                   linearVelocity=(5.0, 0, 0))
+    ow.step(1.0 / 60.0, 1)
+    assert np.all(ow.state()['linear_velocity'][0] == 0.0) and np.all(ow.state()['angular_velocity'][0] == 0.0)
+
+
+def test_kinematic_moves_at_constant_velocity_without_forces():
+    """A kinematic body ignores gravity and damping and integrates x = x0 + n v dt;
+    its orientation follows quat_integrate (a rotation about y by n w dt)."""
+    from omi_physics_amd import PhysicsWorld, model
+    w = PhysicsWorld(gravity=model.Gravity(gravity=9.81), default_linear_damping=0.5)
+    w.add_body(model.Motion(type=model.KINEMATIC, linearVelocity=(1.5, -0.25, 0.5),
+                            angularVelocity=(0.0, 0.6, 0.0), linearDamping=2.0), position=(1.0, 2.0, 3.0))
+    ow = oracle.OracleWorld(w.export_arrays())
+    dt, n = 1.0 / 60.0, 240
+    ow.step(dt, n)
+    S = ow.state()
+    np.testing.assert_allclose(S['position'][0], np.array([1.0, 2.0, 3.0]) + n * dt * np.array([1.5, -0.25, 0.5]),
+                               atol=1e-12, rtol=0)
+    assert np.array_equal(S['linear_velocity'][0], [1.5, -0.25, 0.5])
+    q = S['orientation'][0]
+    # first-order quaternion steps renormalised: the angle is n atan(w dt / 2) * 2 exactly
+    ang = 2.0 * n * np.arctan(0.5 * 0.6 * dt)
+    np.testing.assert_allclose(q, [0.0, np.sin(ang / 2), 0.0, np.cos(ang / 2)], atol=1e-12)
+    assert S['awake'][0] == 1
+
+
+def test_kinematic_wakes_and_pushes_a_sleeper():
+    """world.py:381-384 and solver.py:321-337: a sleeping box on the ground is no
+    solid pair with the ground (neither is an awake mover), but a kinematic pusher
+    sliding into it is an awake mover, wakes it on contact and pushes it along +x;
+    a sleeper that nothing touches stays asleep with zero velocity."""
+    from omi_physics_amd import PhysicsWorld, model
+    w = PhysicsWorld(gravity=model.Gravity(gravity=9.81))
+    g = w.add_shape(model.Shape.box((20, 1, 20)))
+    b = w.add_shape(model.Shape.box((1, 1, 1)))
+    w.add_body(model.Motion(type=model.STATIC), collider=model.Collider(shape=g), position=(0, -0.5, 0))
+    w.add_body(model.Motion(), collider=model.Collider(shape=b), position=(0.0, 0.5, 0.0))
+    w.add_body(model.Motion(), collider=model.Collider(shape=b), position=(6.0, 0.5, 6.0))
+    w.add_body(model.Motion(type=model.KINEMATIC, linearVelocity=(1.0, 0, 0)),
+               collider=model.Collider(shape=b), position=(-3.0, 0.5, 0.0))
+    arrays = w.export_arrays()
+    arrays['awake'] = np.array([1, 0, 0, 1], np.uint8)
+    ow = oracle.OracleWorld(arrays)
+    ow.step(1.0 / 60.0, 1)
+    assert ow.counters()['solid_pairs'] == 0          # sleepers on the ground: no solid pair
+    woke = None
+    for s in range(2, 200):
+        ow.step(1.0 / 60.0, 1)
+        S = ow.state()
+        if woke is None and S['awake'][1]:
+            woke = s
+            assert ow.counters()['contacts'] > 0      # woken by a contact with the pusher
+    assert woke is not None and 100 <= woke <= 130      # the gap of 2 m closes at 1 m/s
+    S = ow.state()
+    assert S['position'][1, 0] > 0.5                   # pushed along +x
+    assert S['awake'][2] == 0 and np.all(S['linear_velocity'][2] == 0)
+    np.testing.assert_allclose(S['position'][3, 0], -3.0 + 199 / 60.0, atol=1e-12)
