@@ -109,6 +109,28 @@ int32_t omi_collide_convex(int32_t npairs, int32_t precision,
                            const double *hull_pts, int64_t n_hull_pts,
                            int32_t *count, double *normal, double *point, double *depth);
 
+/* No counterpart in the reference: a variant of omi_collide_convex for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and results,
+ * but every buffer is a caller-owned DEVICE pointer, and the call is
+ * asynchronous on `stream` (a hipStream_t; NULL = the null stream): the caller
+ * synchronises before reading the outputs.  The GJK/EPA workspaces come from a
+ * static device array of OMI_NOALLOC_EPA_SLOTS workspaces (11.9 MB per device,
+ * held until the process ends); any number of pairs runs through them in turns,
+ * so no pair count is refused for lack of scratch.  The array is shared by every
+ * call on a device: calls that overlap on one device race.  Host checks: npairs
+ * >= 0 and non-NULL buffers (OMI_ERR_ARG); the kernel gives a pair whose kind is
+ * not sphere/box/convex or whose hull range leaves [0, n_hull_pts) count -1 and
+ * zero outputs. */
+#define OMI_NOALLOC_EPA_SLOTS 512
+int32_t omi_collide_convex_noalloc(int32_t npairs, const int32_t *kind_a, const double *ca,
+                                   const double *Ra, const double *ha, const int32_t *off_a,
+                                   const int32_t *cnt_a, const int32_t *kind_b, const double *cb,
+                                   const double *Rb, const double *hb, const int32_t *off_b,
+                                   const int32_t *cnt_b, const double *hull_pts, int64_t n_hull_pts,
+                                   int32_t *count, double *normal, double *point, double *depth,
+                                   void *stream);
+
 /* Replaces _solver_native.prepare_and_solve (_solver_native.pyx:179-312) with
  * the same arguments: K contacts over N bodies; pos/lv/av [N][3], nImp [K] and
  * tImp [K][2] updated in place.  The GPU keeps the reference's Gauss-Seidel

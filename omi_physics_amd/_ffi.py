@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, '_lib', 'libomi_hip.so')
 CSRC = os.path.join(HERE, 'csrc')
 
 OMI_FP64, OMI_FP32 = 64, 32
+OMI_OK, OMI_ERR_ARG, OMI_ERR_HIP, OMI_ERR_CAPACITY, OMI_ERR_STATE = 0, -1, -2, -3, -4
 
 _d = ctypes.POINTER(ctypes.c_double)
 _i32 = ctypes.POINTER(ctypes.c_int32)
@@ -65,7 +66,7 @@ class BatchConfig(ctypes.Structure):
 EXPORTS = (
     'omi_last_error', 'omi_abi_version', 'omi_device_count', 'omi_device_memory',
     'omi_refit', 'omi_sap_pairs', 'omi_box_box', 'omi_sphere_sphere', 'omi_sphere_box',
-    'omi_collide_convex', 'omi_capsule_triangles', 'omi_prepare_and_solve',
+    'omi_collide_convex', 'omi_collide_convex_noalloc', 'omi_capsule_triangles', 'omi_prepare_and_solve',
     'omi_batch_create', 'omi_batch_destroy', 'omi_batch_load_world', 'omi_batch_remove_world',
     'omi_batch_set_state', 'omi_batch_prepare', 'omi_batch_step', 'omi_batch_refit', 'omi_batch_sync',
     'omi_batch_read_world', 'omi_batch_snapshot', 'omi_batch_read_contacts', 'omi_batch_read_pairs',
@@ -109,6 +110,9 @@ def load() -> ctypes.CDLL:
         'omi_collide_convex': (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int32, _i32, _d, _d, _d, _i32, _i32,
                                                 _i32, _d, _d, _d, _i32, _i32, _d, ctypes.c_int64,
                                                 _i32, _d, _d, _d]),
+        'omi_collide_convex_noalloc': (ctypes.c_int32, [ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                        _vp, _vp, _vp, _vp, ctypes.c_int64, _vp, _vp, _vp, _vp,
+                                                        _vp]),
         'omi_capsule_triangles': (ctypes.c_int32, [ctypes.c_int64, _d, _d, ctypes.c_double, _d, _u8, _d, _d, _d]),
         'omi_prepare_and_solve': (ctypes.c_int32, [ctypes.c_int64, ctypes.c_int32, _i64, _i64, _d, _d, _d, _d, _d,
                                                    _d, _d, _d, _d, _d, _d, _d, ctypes.c_double, ctypes.c_int32,

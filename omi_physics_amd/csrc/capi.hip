@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -13,6 +14,7 @@
 #include "../../include/omi_hip.h"
 #include "omi_batch.hpp"
 #include "omi_math.hpp"
+#include "omi_gjk.hpp"
 
 namespace omi {
 void launch_begin(const Dev& d, hipStream_t s);
@@ -49,8 +51,8 @@ bool allow_joint_lds(int bytes);
 int32_t allow_convex_lds();
 void launch_stage_convex(int32_t np, const int32_t* ka, const double* ca, const double* Ra, const double* ha,
                          const int32_t* oa, const int32_t* na, const int32_t* kb, const double* cb, const double* Rb,
-                         const double* hb, const int32_t* ob, const int32_t* nb, const double* hull, void* work,
-                         int32_t slots, int32_t* cnt, double* nrm, double* pt, double* dep, hipStream_t s);
+                         const double* hb, const int32_t* ob, const int32_t* nb, const double* hull, int64_t nhull,
+                         void* work, int32_t slots, int32_t* cnt, double* nrm, double* pt, double* dep, hipStream_t s);
 }  // namespace omi
 
 using namespace omi;
@@ -1541,13 +1543,71 @@ extern "C" int32_t omi_collide_convex(int32_t np, int32_t precision, const int32
     int32_t* dc = m.up<int32_t>(nullptr, np);
     double *dn = m.up<double>(nullptr, 3 * np), *dp = m.up<double>(nullptr, 3 * np), *dd = m.up<double>(nullptr, np);
     if (!dd || !work) return fail(OMI_ERR_HIP, "device allocation failed");
-    launch_stage_convex(np, dka, dca, dRa, dha, doa, dna, dkb, dcb, dRb, dhb, dob, dnb, dh, work, slots, dc, dn, dp,
-                        dd, nullptr);
+    launch_stage_convex(np, dka, dca, dRa, dha, doa, dna, dkb, dcb, dRb, dhb, dob, dnb, dh, n_hull_pts, work, slots, dc,
+                        dn, dp, dd, nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(count, dc, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(normal, dn, sizeof(double) * 3 * np, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(point, dp, sizeof(double) * 3 * np, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(depth, dd, sizeof(double) * np, hipMemcpyDeviceToHost));
+    return OMI_OK;
+}
+
+// ------------------------------------------------ no-allocation entry point --
+//
+// omi_collide_convex_noalloc runs the convex stage kernel on caller-owned
+// device buffers with its GJK/EPA workspaces taken from `scratch`, a
+// file-scope device array of OMI_NOALLOC_EPA_SLOTS workspaces: it calls no
+// hipMalloc or hipFree.  Each device that runs it holds its own copy of the
+// array (11.9 MB) until the process ends.  In host code `scratch` names a
+// host-side stand-in, so its device address is looked up once per device with
+// hipGetSymbolAddress and kept in g_scratch_dev; only that address reaches a
+// kernel.  The workspaces carry nothing from one pair to the next, so the
+// array is never cleared; two calls that overlap on one device share it and
+// race.
+
+static_assert(OMI_NOALLOC_EPA_SLOTS % 64 == 0, "whole wavefronts of workspace slots");
+static __device__ __align__(256) unsigned char scratch[(size_t)OMI_NOALLOC_EPA_SLOTS * sizeof(EpaWork)];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up on its first use there.
+static int32_t scratch_address(void** out) {
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(OMI_ERR_ARG, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(OMI_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_collide_convex_noalloc(int32_t np, const int32_t* kind_a, const double* ca, const double* Ra,
+                                              const double* ha, const int32_t* off_a, const int32_t* cnt_a,
+                                              const int32_t* kind_b, const double* cb, const double* Rb,
+                                              const double* hb, const int32_t* off_b, const int32_t* cnt_b,
+                                              const double* hull_pts, int64_t n_hull_pts, int32_t* count,
+                                              double* normal, double* point, double* depth, void* stream) {
+    if (np < 0 || n_hull_pts < 0) return fail(OMI_ERR_ARG, "npairs and n_hull_pts must be >= 0");
+    if (np == 0) return OMI_OK;
+    if (!kind_a || !ca || !Ra || !ha || !off_a || !cnt_a || !kind_b || !cb || !Rb || !hb || !off_b || !cnt_b ||
+        !hull_pts || !count || !normal || !point || !depth)
+        return fail(OMI_ERR_ARG, "every buffer must be a device pointer (hull offsets and counts included)");
+    void* work = nullptr;
+    int32_t rc = scratch_address(&work);
+    if (rc) return rc;
+    const int32_t slots = std::min<int32_t>((np + 63) / 64 * 64, OMI_NOALLOC_EPA_SLOTS);
+    launch_stage_convex(np, kind_a, ca, Ra, ha, off_a, cnt_a, kind_b, cb, Rb, hb, off_b, cnt_b, hull_pts, n_hull_pts,
+                        work, slots, count, normal, point, depth, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
     return OMI_OK;
 }

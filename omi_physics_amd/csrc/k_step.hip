@@ -644,14 +644,29 @@ __global__ __launch_bounds__(64) void k_narrow_convex(Dev d, int32_t wp_lds) {
     }
 }
 
+// gjk.collide_convex over independent pairs, one thread per workspace slot;
+// pairs beyond `slots` take the slots in turn (a slot's workspace is rewritten
+// by every pair).  A pair whose kinds are not sphere/box/convex or whose hull
+// range leaves [0, nhull) gets count -1 and zero outputs, without reading its
+// hull.
 __global__ __launch_bounds__(64) void k_stage_convex(int32_t np, const int32_t* ka, const double* ca, const double* Ra, const double* ha,
                                const int32_t* oa, const int32_t* na, const int32_t* kb, const double* cb,
                                const double* Rb, const double* hb, const int32_t* ob, const int32_t* nb,
-                               const double* hull, EpaWork* work, int32_t slots, int32_t* cnt, double* nrm,
-                               double* pt, double* dep) {
+                               const double* hull, int64_t nhull, EpaWork* work, int32_t slots, int32_t* cnt,
+                               double* nrm, double* pt, double* dep) {
     const int64_t slot = gtid();
     if (slot >= slots) return;
+    auto valid = [&](int32_t k, const int32_t* off, const int32_t* num, int64_t i) {
+        if (k != K_SPHERE && k != K_BOX && k != K_CONVEX) return false;
+        return k != K_CONVEX || (off[i] >= 0 && num[i] >= 1 && (int64_t)off[i] + num[i] <= nhull);
+    };
     for (int64_t i = slot; i < np; i += slots) {
+        if (!valid(ka[i], oa, na, i) || !valid(kb[i], ob, nb, i)) {
+            cnt[i] = -1;
+            for (int r = 0; r < 3; r++) { nrm[3 * i + r] = 0.0; pt[3 * i + r] = 0.0; }
+            dep[i] = 0.0;
+            continue;
+        }
         Proxy A, B;
         A.kind = ka[i]; B.kind = kb[i];
         for (int r = 0; r < 3; r++) { A.c[r] = ca[3 * i + r]; A.h[r] = ha[3 * i + r]; B.c[r] = cb[3 * i + r]; B.h[r] = hb[3 * i + r]; }
@@ -2756,10 +2771,10 @@ size_t epa_work_bytes() { return sizeof(EpaWork); }
 
 void launch_stage_convex(int32_t np, const int32_t* ka, const double* ca, const double* Ra, const double* ha,
                          const int32_t* oa, const int32_t* na, const int32_t* kb, const double* cb, const double* Rb,
-                         const double* hb, const int32_t* ob, const int32_t* nb, const double* hull, void* work,
-                         int32_t slots, int32_t* cnt, double* nrm, double* pt, double* dep, hipStream_t s) {
+                         const double* hb, const int32_t* ob, const int32_t* nb, const double* hull, int64_t nhull,
+                         void* work, int32_t slots, int32_t* cnt, double* nrm, double* pt, double* dep, hipStream_t s) {
     hipLaunchKernelGGL(k_stage_convex, dim3((slots + 63) / 64), dim3(64), 0, s, np, ka, ca, Ra, ha, oa, na, kb, cb,
-                       Rb, hb, ob, nb, hull, (EpaWork*)work, slots, cnt, nrm, pt, dep);
+                       Rb, hb, ob, nb, hull, nhull, (EpaWork*)work, slots, cnt, nrm, pt, dep);
 }
 
 void launch_compact(const Dev& d, int precision, int grid, hipStream_t s) {
