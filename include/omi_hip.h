@@ -57,6 +57,39 @@ int32_t omi_refit(int32_t n, const double *pos, const double *quat, const int32_
                   const double *hull_pts, int64_t n_hull_pts, double margin,
                   double *aabb_min, double *aabb_max);
 
+/* Replaces NumpyBackend.integrate_forces (backend.py:26-40) on one world's
+ * columns: for awake dynamic bodies v += g·gravity_factor·dt, then
+ * v *= max(1 - (linear_damping + default_linear_damping)·dt, 0), then
+ * v -= quadratic_drag·|v|·v·dt, and w *= max(1 - (angular_damping +
+ * default_angular_damping)·dt, 0).  gravity is per body [n][3] (the reference's
+ * resolve_gravity, world.py:386-396); lv, av [n][3] are updated in place.
+ * FP64, bit-exact with the reference's NumPy. */
+int32_t omi_integrate_forces(int32_t n, const int32_t *motion_type, const uint8_t *awake,
+                             const double *gravity, const double *gravity_factor,
+                             const double *linear_damping, const double *angular_damping,
+                             const double *quadratic_drag, double default_linear_damping,
+                             double default_angular_damping, double dt, double *lv, double *av);
+
+/* Replaces NumpyBackend.integrate_positions (backend.py:42-49) with
+ * quat_integrate (mathutil.py:108-113): awake dynamic and kinematic bodies move
+ * x += v·dt and q <- normalize(q + dt/2·(w, 0)⊗q); pos [n][3] and quat [n][4]
+ * are updated in place.  FP64, bit-exact. */
+int32_t omi_integrate_positions(int32_t n, const int32_t *motion_type, const uint8_t *awake,
+                                const double *lv, const double *av, double dt, double *pos,
+                                double *quat);
+
+/* Replaces PhysicsWorld._update_sleep with _contact_islands (world.py:510-567)
+ * for n bodies and the step's K contacts (body indices ca[k], cb[k]): a dynamic
+ * body is slow when |v| < lin_thresh and |w| < ang_thresh; its timer grows by dt
+ * while slow and resets otherwise; an island of dynamic-dynamic contacts sleeps
+ * when every member has been slow for more than t_sleep, a contacted-by-nothing
+ * dynamic body when it alone has; moving dynamic bodies wake; sleeping dynamic
+ * bodies get zero velocity.  lv, av, awake and sleep_timer are updated in place.
+ * The step uses lin_thresh 0.08, ang_thresh 0.25, t_sleep 0.4. */
+int32_t omi_sleep(int32_t n, int64_t K, const int32_t *ca, const int32_t *cb, const int32_t *motion_type,
+                  double dt, double lin_thresh, double ang_thresh, double t_sleep, double *lv, double *av,
+                  uint8_t *awake, double *sleep_timer);
+
 /* Replaces BroadPhase.pairs (broadphase.py:243-294): stable sweep on lo.x,
  * inclusive y/z overlap, >= 1 dynamic body, collision-filter table lookup;
  * pairs (min, max) in the reference's (rank_i, rank_j) order.  Writes up to
@@ -176,6 +209,12 @@ typedef struct {
     int32_t n_joints, joint_iterations;
     const int32_t *joint_a, *joint_b;
     const double *joint_anchor;
+    /* body-frame anchors [J][3] given by the caller (PointConstraint local_a /
+     * local_b, joints.py:66-73): bit 0 / 1 of joint_fixed[j] marks local a / b
+     * as given; the others are taken from the pose at the joint's first step.
+     * All three may be NULL (none given). */
+    const double *joint_local_a, *joint_local_b;
+    const uint8_t *joint_fixed;
 } omi_world_desc;
 
 typedef struct {
@@ -217,11 +256,50 @@ int32_t omi_batch_remove_world(omi_batch *b, int32_t slot);
 int32_t omi_batch_set_state(omi_batch *b, int32_t slot, const double *pos, const double *quat,
                             const double *lv, const double *av, const uint8_t *awake,
                             const double *sleep_timer);
+/* Overwrite one world's AABBs ([n][3] each; either may be NULL). */
+int32_t omi_batch_set_aabbs(omi_batch *b, int32_t slot, const double *aabb_min, const double *aabb_max);
+/* Overwrite one world's per-body parameters and world settings that a host may
+ * edit between steps (the reference reads them every step, backend.py:26-49,
+ * solver.py:329): any pointer may be NULL to keep the value; gravity[3],
+ * default_damping[2] (linear, angular); sleep_enabled and velocity_iterations
+ * -1 to keep.  Motion types, shapes, materials and filters change the world's
+ * layout: reload it. */
+int32_t omi_batch_set_params(omi_batch *b, int32_t slot, const double *inv_mass,
+                             const double *inv_inertia, const double *gravity_factor,
+                             const double *linear_damping, const double *angular_damping,
+                             const double *quadratic_drag, const double *gravity,
+                             const double *default_damping, int32_t sleep_enabled,
+                             int32_t velocity_iterations);
+/* One world's warm-start cache (SequentialImpulseSolver._cache, solver.py:91,
+ * 271-278): the contacts its next step seeds from, world-local, with their
+ * normal and two tangent impulses.  Count first (pass NULLs), then copy up to
+ * `cap`.  Batch-layer state transfer so a host can keep the cache across a
+ * world reload, as the reference keeps it across edits. */
+int32_t omi_batch_read_cache(omi_batch *b, int32_t slot, int64_t cap, int64_t *count, int32_t *a,
+                             int32_t *bb, double *point, double *normal_impulse,
+                             double *tangent_impulse);
+/* Replace one world's warm-start cache with n entries (entries of one body pair
+ * consecutive, as omi_batch_read_cache returns them). */
+int32_t omi_batch_write_cache(omi_batch *b, int32_t slot, int64_t n, const int32_t *a,
+                              const int32_t *bb, const double *point, const double *normal_impulse,
+                              const double *tangent_impulse);
+/* The body-frame joint anchors of one world as the device holds them: for
+ * each joint, bit 0 / 1 of fixed[j] says whether local a / b is set (given at
+ * load, or taken from the pose at the joint's first step, joints.py:84-101);
+ * la, lb [J][3].  Any pointer may be NULL. */
+int32_t omi_batch_read_joint_anchors(omi_batch *b, int32_t slot, double *la, double *lb, uint8_t *fixed);
 /* Build the world lists and capture the step graph for dt without running it,
  * so the first omi_batch_step does no set-up work. */
 int32_t omi_batch_prepare(omi_batch *b, double dt);
 /* Enqueue nsteps fixed steps of dt for every loaded world (asynchronous). */
 int32_t omi_batch_step(omi_batch *b, double dt, int32_t nsteps);
+/* Enqueue the collision stage alone for every loaded world (asynchronous):
+ * _CollisionStages.run (world.py:657-676) — broadphase on the AABBs as they are,
+ * solid filter, narrowphase, wake, contact solve with warm start, trigger
+ * overlaps — for a host that integrates forces, refits, solves joints,
+ * integrates positions and updates sleep itself (the reference's
+ * PhysicsWorld.step, world.py:410-430). */
+int32_t omi_batch_collide(omi_batch *b, double dt);
 /* Recompute AABBs only (asynchronous). */
 int32_t omi_batch_refit(omi_batch *b, double margin);
 /* Wait for the batch's queued work; reports capacity overflow of any step. */

@@ -46,6 +46,7 @@ class WorldDesc(ctypes.Structure):
         ('has_collision', ctypes.c_int32),
         ('n_joints', ctypes.c_int32), ('joint_iterations', ctypes.c_int32),
         ('joint_a', _i32), ('joint_b', _i32), ('joint_anchor', _d),
+        ('joint_local_a', _d), ('joint_local_b', _d), ('joint_fixed', _u8),
     ]
 
 
@@ -68,7 +69,10 @@ EXPORTS = (
     'omi_refit', 'omi_sap_pairs', 'omi_box_box', 'omi_sphere_sphere', 'omi_sphere_box',
     'omi_collide_convex', 'omi_collide_convex_noalloc', 'omi_capsule_triangles', 'omi_prepare_and_solve',
     'omi_batch_create', 'omi_batch_destroy', 'omi_batch_load_world', 'omi_batch_remove_world',
-    'omi_batch_set_state', 'omi_batch_prepare', 'omi_batch_step', 'omi_batch_refit', 'omi_batch_sync',
+    'omi_integrate_forces', 'omi_integrate_positions', 'omi_sleep',
+    'omi_batch_set_state', 'omi_batch_set_aabbs', 'omi_batch_set_params', 'omi_batch_read_cache',
+    'omi_batch_write_cache', 'omi_batch_collide', 'omi_batch_read_joint_anchors',
+    'omi_batch_prepare', 'omi_batch_step', 'omi_batch_refit', 'omi_batch_sync',
     'omi_batch_read_world', 'omi_batch_snapshot', 'omi_batch_read_contacts', 'omi_batch_read_pairs',
     'omi_batch_read_trigger_overlaps',
     'omi_batch_counters', 'omi_batch_set_timing', 'omi_batch_kernel_times',
@@ -123,6 +127,19 @@ def load() -> ctypes.CDLL:
         'omi_batch_load_world': (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.POINTER(WorldDesc)]),
         'omi_batch_remove_world': (ctypes.c_int32, [_vp, ctypes.c_int32]),
         'omi_batch_set_state': (ctypes.c_int32, [_vp, ctypes.c_int32, _d, _d, _d, _d, _u8, _d]),
+        'omi_batch_set_aabbs': (ctypes.c_int32, [_vp, ctypes.c_int32, _d, _d]),
+        'omi_batch_read_joint_anchors': (ctypes.c_int32, [_vp, ctypes.c_int32, _d, _d, _u8]),
+        'omi_batch_set_params': (ctypes.c_int32, [_vp, ctypes.c_int32, _d, _d, _d, _d, _d, _d, _d, _d,
+                                                  ctypes.c_int32, ctypes.c_int32]),
+        'omi_batch_read_cache': (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_int64, _i64, _i32, _i32, _d, _d,
+                                                  _d]),
+        'omi_batch_write_cache': (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_int64, _i32, _i32, _d, _d, _d]),
+        'omi_batch_collide': (ctypes.c_int32, [_vp, ctypes.c_double]),
+        'omi_integrate_forces': (ctypes.c_int32, [ctypes.c_int32, _i32, _u8, _d, _d, _d, _d, _d, ctypes.c_double,
+                                                  ctypes.c_double, ctypes.c_double, _d, _d]),
+        'omi_integrate_positions': (ctypes.c_int32, [ctypes.c_int32, _i32, _u8, _d, _d, ctypes.c_double, _d, _d]),
+        'omi_sleep': (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int64, _i32, _i32, _i32, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, _d, _d, _u8, _d]),
         'omi_batch_prepare': (ctypes.c_int32, [_vp, ctypes.c_double]),
         'omi_batch_step': (ctypes.c_int32, [_vp, ctypes.c_double, ctypes.c_int32]),
         'omi_batch_refit': (ctypes.c_int32, [_vp, ctypes.c_double]),
@@ -228,4 +245,10 @@ def world_desc(arrays: dict):
     keep['janc'] = c64(np.asarray(arrays['joint_anchor']).reshape(-1, 3) if nj else np.zeros((1, 3)))
     d.joint_a = ptr(keep['ja'], _i32); d.joint_b = ptr(keep['jb'], _i32)
     d.joint_anchor = ptr(keep['janc'], _d)
+    if nj and arrays.get('joint_fixed') is not None:
+        keep['jla'] = c64(np.asarray(arrays['joint_local_a']).reshape(-1, 3))
+        keep['jlb'] = c64(np.asarray(arrays['joint_local_b']).reshape(-1, 3))
+        keep['jfx'] = np.ascontiguousarray(arrays['joint_fixed'], dtype=np.uint8)
+        d.joint_local_a = ptr(keep['jla'], _d); d.joint_local_b = ptr(keep['jlb'], _d)
+        d.joint_fixed = ptr(keep['jfx'], _u8)
     return d, keep

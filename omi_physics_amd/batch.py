@@ -108,7 +108,56 @@ class Batch:
         check(self.lib.omi_batch_set_state(self._h, int(slot), ptr(keep[0], _d), ptr(keep[1], _d),
                                            ptr(keep[2], _d), ptr(keep[3], _d), ptr(aw, _u8), ptr(tm, _d)))
 
+    def set_aabbs(self, slot: int, aabb_min=None, aabb_max=None) -> None:
+        lo = None if aabb_min is None else _ffi.c64(aabb_min)
+        hi = None if aabb_max is None else _ffi.c64(aabb_max)
+        check(self.lib.omi_batch_set_aabbs(self._h, int(slot), ptr(lo, _d), ptr(hi, _d)))
+
+    def set_params(self, slot: int, inv_mass=None, inv_inertia=None, gravity_factor=None, linear_damping=None,
+                   angular_damping=None, quadratic_drag=None, gravity=None, default_damping=None,
+                   sleep_enabled: Optional[bool] = None, velocity_iterations: Optional[int] = None) -> None:
+        """Overwrite per-body parameters and world settings of a loaded world (None keeps a value)."""
+        arrs = [None if x is None else _ffi.c64(x) for x in (inv_mass, inv_inertia, gravity_factor, linear_damping,
+                                                             angular_damping, quadratic_drag, gravity,
+                                                             default_damping)]
+        check(self.lib.omi_batch_set_params(
+            self._h, int(slot), *[ptr(a, _d) for a in arrs],
+            -1 if sleep_enabled is None else int(bool(sleep_enabled)),
+            -1 if velocity_iterations is None else int(velocity_iterations)))
+
+    def read_cache(self, slot: int) -> Dict[str, np.ndarray]:
+        """One world's warm-start cache (world-local pairs, points, impulses)."""
+        cnt = np.zeros(1, np.int64)
+        check(self.lib.omi_batch_read_cache(self._h, int(slot), 0, ptr(cnt, _i64), None, None, None, None, None))
+        K = int(cnt[0])
+        out = dict(a=np.zeros(K, np.int32), b=np.zeros(K, np.int32), point=np.zeros((K, 3)),
+                   normal_impulse=np.zeros(K), tangent_impulse=np.zeros((K, 2)))
+        if K:
+            check(self.lib.omi_batch_read_cache(self._h, int(slot), K, ptr(cnt, _i64), ptr(out['a'], _i32),
+                                                ptr(out['b'], _i32), ptr(out['point'], _d),
+                                                ptr(out['normal_impulse'], _d), ptr(out['tangent_impulse'], _d)))
+        return out
+
+    def write_cache(self, slot: int, cache: Dict[str, np.ndarray]) -> None:
+        """Replace one world's warm-start cache (the layout :meth:`read_cache` returns)."""
+        a = np.ascontiguousarray(cache['a'], np.int32)
+        b = np.ascontiguousarray(cache['b'], np.int32)
+        pt, ni, ti = (_ffi.c64(cache[k]) for k in ('point', 'normal_impulse', 'tangent_impulse'))
+        check(self.lib.omi_batch_write_cache(self._h, int(slot), len(a), ptr(a, _i32), ptr(b, _i32), ptr(pt, _d),
+                                             ptr(ni, _d), ptr(ti, _d)))
+
+    def read_joint_anchors(self, slot: int, n_joints: int):
+        """``(local_a, local_b, fixed)`` of one world's joints as the device holds them."""
+        la, lb, fx = np.zeros((n_joints, 3)), np.zeros((n_joints, 3)), np.zeros(n_joints, np.uint8)
+        if n_joints:
+            check(self.lib.omi_batch_read_joint_anchors(self._h, int(slot), ptr(la, _d), ptr(lb, _d), ptr(fx, _u8)))
+        return la, lb, fx
+
     # -- stepping -----------------------------------------------------------
+    def collide(self, dt: float) -> None:
+        """Enqueue the collision stage alone for every loaded world (asynchronous)."""
+        check(self.lib.omi_batch_collide(self._h, float(dt)))
+
     def step(self, dt: float, steps: int = 1) -> None:
         """Enqueue ``steps`` steps of ``dt`` for every loaded world (asynchronous)."""
         check(self.lib.omi_batch_step(self._h, float(dt), int(steps)))

@@ -18,7 +18,13 @@
 
 namespace omi {
 void launch_begin(const Dev& d, hipStream_t s);
-void launch_pre(const Dev& d, int precision, int64_t nhw, double dt, double margin, hipStream_t s);
+void launch_pre(const Dev& d, int precision, int64_t nhw, double dt, double margin, hipStream_t s, bool collide_only);
+void launch_stage_forces(int32_t n, const int32_t* mt, const uint8_t* awake, const double* g, const double* gf,
+                         const double* ld, const double* ad, const double* qd, double dlin, double dang, double dt,
+                         double* lv, double* av, hipStream_t s);
+void launch_stage_positions(int32_t n, const int32_t* mt, const uint8_t* awake, const double* lv, const double* av,
+                            double dt, double* pos, double* quat, hipStream_t s);
+void launch_cache_seed(const Dev& d, int64_t base, int32_t n, hipStream_t s);
 void launch_refit(const Dev& d, int64_t nhw, double margin, hipStream_t s);
 void launch_sap_sort(const Dev& d, int c, const int32_t* wl, const int32_t* wn, int32_t np, int32_t block,
                      int grid, hipStream_t s);
@@ -29,7 +35,7 @@ void launch_wake(const Dev& d, hipStream_t s);
 void launch_cache(const Dev& d, int precision, hipStream_t s);
 void launch_post(const Dev& d, int64_t nhw, double dt, hipStream_t s);
 void launch_joints(const Dev& d, int precision, double dt, int32_t lds_bytes, hipStream_t s);
-void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s);
+void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s, double lin, double ang, double tsleep);
 void launch_toggle(const Dev& d, hipStream_t s);
 void launch_trigger(const Dev& d, hipStream_t s);
 void launch_capsule_triangles(int64_t n, const double* c0, const double* c1, double radius, const double* verts,
@@ -172,7 +178,7 @@ struct omi_batch {
 };
 
 extern "C" const char* omi_last_error(void) { return g_err.c_str(); }
-extern "C" int32_t omi_abi_version(void) { return 1; }
+extern "C" int32_t omi_abi_version(void) { return 2; }
 extern "C" int32_t omi_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -543,14 +549,20 @@ extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_wo
     if (s.j_n > 0) {
         std::vector<int32_t> ja((size_t)s.j_n), jb((size_t)s.j_n);
         std::vector<uint8_t> jh((size_t)s.j_n, 0);
+        std::vector<double> zl((size_t)(3 * s.j_n), 0.0);
         for (int64_t j = 0; j < s.j_n; j++) {
             ja[j] = w->joint_a[j] >= 0 ? (int32_t)(s.off + w->joint_a[j]) : -1;
             jb[j] = w->joint_b[j] >= 0 ? (int32_t)(s.off + w->joint_b[j]) : -1;
+            if (w->joint_fixed) jh[j] = (uint8_t)(w->joint_fixed[j] & 3);
         }
         if ((rc = put(b, d.ja + s.j_off, ja.data(), s.j_n))) return rc;
         if ((rc = put(b, d.jb + s.j_off, jb.data(), s.j_n))) return rc;
         if ((rc = put(b, d.jhas + s.j_off, jh.data(), s.j_n))) return rc;
         if ((rc = upload_soa(b, d.janc, d.jcap, s.j_off, w->joint_anchor, s.j_n, 3))) return rc;
+        if ((rc = upload_soa(b, d.jla, d.jcap, s.j_off, w->joint_local_a ? w->joint_local_a : zl.data(), s.j_n, 3)))
+            return rc;
+        if ((rc = upload_soa(b, d.jlb, d.jcap, s.j_off, w->joint_local_b ? w->joint_local_b : zl.data(), s.j_n, 3)))
+            return rc;
         if (!b->any_joints) { b->any_joints = true; invalidate_graph(b); }
         // LDS for the records of this world's joint bodies during the sweeps
         std::vector<int32_t> ends;
@@ -612,6 +624,159 @@ extern "C" int32_t omi_batch_set_state(omi_batch* b, int32_t slot, const double*
     if (av && (rc = upload_soa(b, d.av, d.ncap, s.off, av, s.n, 3))) return rc;
     if (awake && (rc = put(b, d.awake + s.off, awake, s.n))) return rc;
     if (timer && (rc = put(b, d.timer + s.off, timer, s.n))) return rc;
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_read_joint_anchors(omi_batch* b, int32_t slot, double* la, double* lb, uint8_t* fixed) {
+    if (!b || slot < 0 || slot >= b->cfg.max_worlds || !b->slots[slot].live) return fail(OMI_ERR_ARG, "bad world slot");
+    const WorldSlot& s = b->slots[slot];
+    const Dev& d = b->d;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    int32_t rc;
+    if (s.j_n == 0) return OMI_OK;
+    if (la && (rc = download_soa(b, la, d.jla, d.jcap, s.j_off, s.j_n, 3))) return rc;
+    if (lb && (rc = download_soa(b, lb, d.jlb, d.jcap, s.j_off, s.j_n, 3))) return rc;
+    if (fixed && (rc = get(b, fixed, d.jhas + s.j_off, s.j_n))) return rc;
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_set_aabbs(omi_batch* b, int32_t slot, const double* aabb_min, const double* aabb_max) {
+    if (!b || slot < 0 || slot >= b->cfg.max_worlds || !b->slots[slot].live) return fail(OMI_ERR_ARG, "bad world slot");
+    const WorldSlot& s = b->slots[slot];
+    int32_t rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (aabb_min && (rc = upload_soa(b, b->d.lo, b->d.ncap, s.off, aabb_min, s.n, 3))) return rc;
+    if (aabb_max && (rc = upload_soa(b, b->d.hi, b->d.ncap, s.off, aabb_max, s.n, 3))) return rc;
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_set_params(omi_batch* b, int32_t slot, const double* inv_mass, const double* inv_inertia,
+                                        const double* gravity_factor, const double* linear_damping,
+                                        const double* angular_damping, const double* quadratic_drag,
+                                        const double* gravity, const double* default_damping, int32_t sleep_enabled,
+                                        int32_t velocity_iterations) {
+    if (!b || slot < 0 || slot >= b->cfg.max_worlds || !b->slots[slot].live) return fail(OMI_ERR_ARG, "bad world slot");
+    if (velocity_iterations > b->d.slmax - 1)
+        return fail(OMI_ERR_ARG, "velocity_iterations exceeds the batch's max_solver_iterations");
+    WorldSlot& s = b->slots[slot];
+    Dev& d = b->d;
+    const int32_t W = b->cfg.max_worlds;
+    int32_t rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (inv_mass && (rc = put(b, d.inv_mass + s.off, inv_mass, s.n))) return rc;
+    if (inv_inertia && (rc = upload_soa(b, d.inv_inertia, d.ncap, s.off, inv_inertia, s.n, 3))) return rc;
+    if (gravity_factor && (rc = put(b, d.gf + s.off, gravity_factor, s.n))) return rc;
+    if (linear_damping && (rc = put(b, d.ld + s.off, linear_damping, s.n))) return rc;
+    if (angular_damping && (rc = put(b, d.ad + s.off, angular_damping, s.n))) return rc;
+    if (quadratic_drag && (rc = put(b, d.qd + s.off, quadratic_drag, s.n))) return rc;
+    if (gravity)
+        for (int r = 0; r < 3; r++)
+            if ((rc = put(b, d.w_grav + (size_t)r * W + slot, &gravity[r], 1))) return rc;
+    if (default_damping) {
+        if ((rc = put(b, d.w_damp + slot, &default_damping[0], 1))) return rc;
+        if ((rc = put(b, d.w_damp + W + slot, &default_damping[1], 1))) return rc;
+    }
+    if (velocity_iterations >= 0 && (rc = put(b, d.w_iters + slot, &velocity_iterations, 1))) return rc;
+    if (sleep_enabled >= 0) {
+        s.flags = sleep_enabled ? (s.flags | WF_SLEEP) : (s.flags & ~WF_SLEEP);
+        if ((rc = put(b, d.w_flags + slot, &s.flags, 1))) return rc;
+    }
+    return OMI_OK;
+}
+
+// The warm-start cache of one world: the current buffer's entries of the slot,
+// world-local, impulses as doubles.
+extern "C" int32_t omi_batch_read_cache(omi_batch* b, int32_t slot, int64_t cap, int64_t* count, int32_t* a,
+                                        int32_t* bb, double* point, double* nimp, double* timp) {
+    if (!b || slot < 0 || slot >= b->cfg.max_worlds || !b->slots[slot].live || !count)
+        return fail(OMI_ERR_ARG, "bad arguments");
+    const WorldSlot& s = b->slots[slot];
+    const Dev& d = b->d;
+    const int32_t W = b->cfg.max_worlds;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    int32_t sel = 0, rs = 0, rn = 0, rc;
+    if ((rc = get(b, &sel, d.sel, 1))) return rc;
+    if ((rc = get(b, &rs, d.w_cache_s + (size_t)sel * W + slot, 1))) return rc;
+    if ((rc = get(b, &rn, d.w_cache_n + (size_t)sel * W + slot, 1))) return rc;
+    *count = rn;
+    const int64_t m = std::min<int64_t>(rn, cap);
+    if (m <= 0) return OMI_OK;
+    std::vector<int32_t> ia((size_t)m), ib((size_t)m);
+    if ((rc = get(b, ia.data(), d.ka[sel] + rs, m))) return rc;
+    if ((rc = get(b, ib.data(), d.kb[sel] + rs, m))) return rc;
+    for (int64_t k = 0; k < m; k++) {
+        if (a) a[k] = (int32_t)(ia[k] - s.off);
+        if (bb) bb[k] = (int32_t)(ib[k] - s.off);
+    }
+    const size_t R = b->cfg.precision == OMI_FP64 ? 8 : 4;
+    std::vector<double> tmp((size_t)m);
+    std::vector<unsigned char> raw((size_t)m * R);
+    for (int c = 0; c < 3; c++) {
+        if (point) {
+            if ((rc = get(b, tmp.data(), d.kpt[sel] + (size_t)c * d.kcap + rs, m))) return rc;
+            for (int64_t k = 0; k < m; k++) point[3 * k + c] = tmp[k];
+        }
+        HIPCHK(hipMemcpy(raw.data(), (const unsigned char*)d.kimp[sel] + ((size_t)c * d.kcap + rs) * R, m * R,
+                         hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < m; k++) {
+            const double v = R == 8 ? ((const double*)raw.data())[k] : (double)((const float*)raw.data())[k];
+            if (c == 0 && nimp) nimp[k] = v;
+            if (c > 0 && timp) timp[2 * k + c - 1] = v;
+        }
+    }
+    return OMI_OK;
+}
+
+// Replace one world's warm-start cache: the entries are appended to the
+// current buffer (after every world's entries of its last fill) and their
+// group heads hashed, so the next step seeds from them as from its own cache.
+// Entries of one (a, b) must be consecutive, as omi_batch_read_cache returns them.
+extern "C" int32_t omi_batch_write_cache(omi_batch* b, int32_t slot, int64_t n, const int32_t* a, const int32_t* bb,
+                                         const double* point, const double* nimp, const double* timp) {
+    if (!b || slot < 0 || slot >= b->cfg.max_worlds || !b->slots[slot].live || n < 0 || (n > 0 && (!a || !bb || !point || !nimp || !timp)))
+        return fail(OMI_ERR_ARG, "bad arguments");
+    const WorldSlot& s = b->slots[slot];
+    Dev& d = b->d;
+    const int32_t W = b->cfg.max_worlds;
+    std::vector<std::pair<int32_t, int32_t>> heads;
+    for (int64_t k = 0; k < n; k++) {
+        if (a[k] < 0 || a[k] >= s.n || bb[k] < 0 || bb[k] >= s.n) return fail(OMI_ERR_ARG, "cache body index out of range");
+        if (k == 0 || a[k] != a[k - 1] || bb[k] != bb[k - 1]) heads.push_back({a[k], bb[k]});
+    }
+    std::sort(heads.begin(), heads.end());
+    if (std::adjacent_find(heads.begin(), heads.end()) != heads.end())
+        return fail(OMI_ERR_ARG, "cache entries of one body pair must be consecutive");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    int32_t sel = 0, hn[2] = {0, 0}, rc;
+    if ((rc = get(b, &sel, d.sel, 1))) return rc;
+    if ((rc = get(b, hn, d.hslot_n, 2))) return rc;
+    const int64_t base = hn[sel];
+    if (base + n > d.kcap) return fail(OMI_ERR_CAPACITY, "warm-start cache capacity of the batch is full");
+    int32_t zero = 0, cnt = (int32_t)n, b32 = (int32_t)base;
+    if (n > 0) {
+        std::vector<int32_t> ga((size_t)n), gb((size_t)n);
+        for (int64_t k = 0; k < n; k++) { ga[k] = (int32_t)(s.off + a[k]); gb[k] = (int32_t)(s.off + bb[k]); }
+        if ((rc = put(b, d.ka[sel] + base, ga.data(), n))) return rc;
+        if ((rc = put(b, d.kb[sel] + base, gb.data(), n))) return rc;
+        if ((rc = upload_soa(b, d.kpt[sel], d.kcap, base, point, n, 3))) return rc;
+        const size_t R = b->cfg.precision == OMI_FP64 ? 8 : 4;
+        std::vector<unsigned char> raw((size_t)n * R);
+        for (int c = 0; c < 3; c++) {
+            for (int64_t k = 0; k < n; k++) {
+                const double v = c == 0 ? nimp[k] : timp[2 * k + c - 1];
+                if (R == 8) ((double*)raw.data())[k] = v; else ((float*)raw.data())[k] = (float)v;
+            }
+            HIPCHK(hipMemcpy((unsigned char*)d.kimp[sel] + ((size_t)c * d.kcap + base) * R, raw.data(), n * R,
+                             hipMemcpyHostToDevice));
+        }
+        hn[sel] = (int32_t)(base + n);
+        if ((rc = put(b, d.hslot_n, hn, 2))) return rc;
+    }
+    if ((rc = put(b, d.w_cache_s + (size_t)sel * W + slot, n > 0 ? &b32 : &zero, 1))) return rc;
+    if ((rc = put(b, d.w_cache_n + (size_t)sel * W + slot, &cnt, 1))) return rc;
+    launch_cache_seed(d, base, (int32_t)n, b->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(b->stream));
     return OMI_OK;
 }
 
@@ -720,7 +885,7 @@ static void enqueue_step(omi_batch* b, double dt) {
     auto mark = [&]() { if (tm) hipEventRecord(b->ev[e++], s); };
     mark();
     launch_begin(d, s); mark();
-    launch_pre(d, prec, b->nhw, dt, 0.05, s); mark();
+    launch_pre(d, prec, b->nhw, dt, 0.05, s, false); mark();
     enqueue_sap(b, b->cfg.record_broadphase);
     mark();
     launch_narrow(d, prec, s); mark();
@@ -734,8 +899,28 @@ static void enqueue_step(omi_batch* b, double dt) {
     if (b->any_joints) launch_joints(d, prec, dt, b->joint_lds, s);
     mark();
     launch_post(d, b->nhw, dt, s); mark();
-    launch_sleep(d, b->nhw, dt, s); mark();
+    launch_sleep(d, b->nhw, dt, s, 0.08, 0.25, 0.4); mark();
     launch_toggle(d, s); mark();
+}
+
+// The collision stage alone (_CollisionStages.run, world.py:657-676): the host
+// has integrated forces and refitted the AABBs and will solve joints,
+// integrate positions and update sleep itself.
+static void enqueue_collide(omi_batch* b, double dt) {
+    const Dev& d = b->d;
+    hipStream_t s = b->stream;
+    const int prec = b->cfg.precision;
+    launch_begin(d, s);
+    launch_pre(d, prec, b->nhw, dt, 0.05, s, true);
+    enqueue_sap(b, b->cfg.record_broadphase);
+    launch_narrow(d, prec, s);
+    launch_compact(d, prec, b->sap_grid, s);
+    launch_wake(d, s);
+    SolveIn in{nullptr, nullptr, 0.5, 0.8, 0.005, 1};
+    enqueue_solve(b, in, false, [] {});
+    launch_cache(d, prec, s);
+    if (b->any_triggers) launch_trigger(d, s);
+    launch_toggle(d, s);
 }
 
 extern "C" int32_t omi_batch_set_timing(omi_batch* b, int32_t on) {
@@ -857,6 +1042,16 @@ extern "C" int32_t omi_batch_step(omi_batch* b, double dt, int32_t nsteps) {
             HIPCHK(hipGetLastError());
         }
     }
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_collide(omi_batch* b, double dt) {
+    if (!b) return fail(OMI_ERR_ARG, "null batch");
+    if (!(dt > 0.0)) return fail(OMI_ERR_ARG, "dt must be positive");
+    int32_t rc;
+    if (b->lists_dirty && (rc = rebuild_lists(b))) return rc;
+    enqueue_collide(b, dt);
+    HIPCHK(hipGetLastError());
     return OMI_OK;
 }
 
@@ -1336,6 +1531,90 @@ extern "C" int32_t omi_refit(int32_t n, const double* pos, const double* quat, c
         if (hipGetLastError() != hipSuccess) rc = fail(OMI_ERR_HIP, "refit launch failed");
         if (!rc) rc = download_soa(b, lo, d.lo, d.ncap, 0, n, 3);
         if (!rc) rc = download_soa(b, hi, d.hi, d.ncap, 0, n, 3);
+    }
+    omi_batch_destroy(b);
+    return rc;
+}
+
+extern "C" int32_t omi_integrate_forces(int32_t n, const int32_t* motion_type, const uint8_t* awake,
+                                        const double* gravity, const double* gravity_factor,
+                                        const double* linear_damping, const double* angular_damping,
+                                        const double* quadratic_drag, double default_linear_damping,
+                                        double default_angular_damping, double dt, double* lv, double* av) {
+    if (n < 0) return fail(OMI_ERR_ARG, "negative body count");
+    if (n == 0) return OMI_OK;
+    if (!motion_type || !awake || !gravity || !gravity_factor || !linear_damping || !angular_damping ||
+        !quadratic_drag || !lv || !av)
+        return fail(OMI_ERR_ARG, "null argument");
+    int32_t rc = check_device();
+    if (rc) return rc;
+    DevBuf m;
+    int32_t* dmt = m.up(motion_type, n);
+    uint8_t* daw = m.up(awake, n);
+    double *dg = m.up(gravity, 3 * (int64_t)n), *dgf = m.up(gravity_factor, n), *dld = m.up(linear_damping, n);
+    double *dad = m.up(angular_damping, n), *dqd = m.up(quadratic_drag, n);
+    double *dlv = m.up(lv, 3 * (int64_t)n), *dav = m.up(av, 3 * (int64_t)n);
+    if (!dav) return fail(OMI_ERR_HIP, "device allocation failed");
+    launch_stage_forces(n, dmt, daw, dg, dgf, dld, dad, dqd, default_linear_damping, default_angular_damping, dt, dlv,
+                        dav, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(lv, dlv, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(av, dav, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_integrate_positions(int32_t n, const int32_t* motion_type, const uint8_t* awake,
+                                           const double* lv, const double* av, double dt, double* pos, double* quat) {
+    if (n < 0) return fail(OMI_ERR_ARG, "negative body count");
+    if (n == 0) return OMI_OK;
+    if (!motion_type || !awake || !lv || !av || !pos || !quat) return fail(OMI_ERR_ARG, "null argument");
+    int32_t rc = check_device();
+    if (rc) return rc;
+    DevBuf m;
+    int32_t* dmt = m.up(motion_type, n);
+    uint8_t* daw = m.up(awake, n);
+    double *dlv = m.up(lv, 3 * (int64_t)n), *dav = m.up(av, 3 * (int64_t)n);
+    double *dp = m.up(pos, 3 * (int64_t)n), *dq = m.up(quat, 4 * (int64_t)n);
+    if (!dq) return fail(OMI_ERR_HIP, "device allocation failed");
+    launch_stage_positions(n, dmt, daw, dlv, dav, dt, dp, dq, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(pos, dp, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(quat, dq, sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_sleep(int32_t n, int64_t K, const int32_t* ca, const int32_t* cb, const int32_t* motion_type,
+                             double dt, double lin_thresh, double ang_thresh, double t_sleep, double* lv, double* av,
+                             uint8_t* awake, double* sleep_timer) {
+    if (n < 0 || K < 0) return fail(OMI_ERR_ARG, "negative size");
+    if (n == 0) return OMI_OK;
+    if (!motion_type || !lv || !av || !awake || !sleep_timer || (K > 0 && (!ca || !cb)))
+        return fail(OMI_ERR_ARG, "null argument");
+    for (int64_t k = 0; k < K; k++)
+        if (ca[k] < 0 || ca[k] >= n || cb[k] < 0 || cb[k] >= n) return fail(OMI_ERR_ARG, "contact body index out of range");
+    omi_batch* b = nullptr;
+    int32_t rc = mini_batch(n, 1, std::max<int64_t>(K, 1), 64, 0, &b);
+    if (rc) return rc;
+    Dev& d = b->d;
+    std::vector<uint8_t> mt((size_t)n);
+    std::vector<int32_t> wz((size_t)n, 0), par((size_t)n);
+    for (int32_t i = 0; i < n; i++) { mt[i] = (uint8_t)motion_type[i]; par[i] = i; }
+    int32_t zero = 0, Ki = (int32_t)K, flags = WF_ACTIVE | WF_SLEEP;
+    uint64_t kc = (uint64_t)K;
+    if (!(rc = upload_soa(b, d.lv, d.ncap, 0, lv, n, 3)) && !(rc = upload_soa(b, d.av, d.ncap, 0, av, n, 3)) &&
+        !(rc = put(b, d.mt, mt.data(), n)) && !(rc = put(b, d.awake, awake, n)) &&
+        !(rc = put(b, d.timer, sleep_timer, n)) && !(rc = put(b, d.world, wz.data(), n)) &&
+        !(rc = put(b, d.parent, par.data(), n)) && !(rc = put(b, d.ca, ca, K)) && !(rc = put(b, d.cb, cb, K)) &&
+        !(rc = put(b, d.w_cs, &zero, 1)) && !(rc = put(b, d.w_cn, &Ki, 1)) && !(rc = put(b, d.w_flags, &flags, 1)) &&
+        !(rc = put(b, d.cnt + C_CONTACTS, &kc, 1))) {
+        launch_sleep(d, n, dt, b->stream, lin_thresh, ang_thresh, t_sleep);
+        if (hipGetLastError() != hipSuccess) rc = fail(OMI_ERR_HIP, "sleep launch failed");
+        if (!rc) rc = download_soa(b, lv, d.lv, d.ncap, 0, n, 3);
+        if (!rc) rc = download_soa(b, av, d.av, d.ncap, 0, n, 3);
+        if (!rc) rc = get(b, awake, d.awake, n);
+        if (!rc) rc = get(b, sleep_timer, d.timer, n);
     }
     omi_batch_destroy(b);
     return rc;

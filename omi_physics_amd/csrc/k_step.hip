@@ -140,19 +140,59 @@ __device__ __forceinline__ void refit_body(const Dev& d, int64_t i, const double
     for (int r = 0; r < 3; r++) { BODY(d.lo, r, i) = lo[r]; BODY(d.hi, r, i) = hi[r]; }
 }
 
+// NumpyBackend.integrate_forces (backend.py:26-40) for one awake dynamic body:
+// gravity, linear damping, quadratic drag on the damped speed, angular damping,
+// in the reference's operation order.
+__device__ __forceinline__ void forces_body(double* v, double* w, const double* g, double gf, double ld, double ad,
+                                            double qd, double dlin, double dang, double dt) {
+    for (int r = 0; r < 3; r++) v[r] = v[r] + (g[r] * gf) * dt;
+    const double fl = maxd(1.0 - (ld + dlin) * dt, 0.0);
+    for (int r = 0; r < 3; r++) v[r] = v[r] * fl;
+    const double sp = sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+    for (int r = 0; r < 3; r++) v[r] = v[r] - ((qd * sp) * v[r]) * dt;
+    const double fa = maxd(1.0 - (ad + dang) * dt, 0.0);
+    for (int r = 0; r < 3; r++) w[r] = w[r] * fa;
+}
+
+// NumpyBackend.integrate_positions (backend.py:42-49) with quat_integrate
+// (mathutil.py:108-113) for one moving body: x += v dt, then
+// q <- normalize(q + dt/2 (w, 0) q) with quat_mul's term order.
+__device__ __forceinline__ void positions_body(double* p, double* q, const double* v, const double* w, double dt) {
+    for (int r = 0; r < 3; r++) p[r] = p[r] + v[r] * dt;
+    const double wx = w[0], wy = w[1], wz = w[2];
+    const double bx = q[0], by = q[1], bz = q[2], bw = q[3];
+    const double mx = ((0.0 * bx + wx * bw) + wy * bz) - wz * by;
+    const double my = ((0.0 * by - wx * bz) + wy * bw) + wz * bx;
+    const double mz = ((0.0 * bz + wx * by) - wy * bx) + wz * bw;
+    const double mw = ((0.0 * bw - wx * bx) - wy * by) - wz * bz;
+    const double s0 = bx + (0.5 * mx) * dt, s1 = by + (0.5 * my) * dt;
+    const double s2 = bz + (0.5 * mz) * dt, s3 = bw + (0.5 * mw) * dt;
+    const double nn = sqrt(((s0 * s0 + s1 * s1) + s2 * s2) + s3 * s3);
+    const double dd = maxd(nn, 1e-12);
+    q[0] = s0 / dd; q[1] = s1 / dd; q[2] = s2 / dd; q[3] = s3 / dd;
+}
+
+// k_pre modes: the whole step's first kernel, or the first kernel of a
+// collision stage alone (the host has integrated forces and refitted already).
+enum : int { PRE_STEP = 0, PRE_COLLIDE = 1 };
+
 // Start of step for every live body: previous pose, sleep scratch reset,
-// integrate_forces (backend.py:26-40) for awake dynamic bodies, then the
-// collision-side per-body data: AABB, rotation, world inverse inertia
-// (solver.py:215-220, einsum order (p0 + p2) + p1).
-template <typename R>
+// integrate_forces for awake dynamic bodies, then the collision-side per-body
+// data: AABB, rotation, world inverse inertia (solver.py:215-220, einsum order
+// (p0 + p2) + p1).  PRE_COLLIDE keeps pose, velocity and AABB as they are and
+// only resets the scratch and writes rotation and world inverse inertia.
+template <typename R, int MODE>
 __global__ void k_pre(Dev d, int64_t nhw, double dt, double margin) {
     for (int64_t i = gtid(); i < nhw; i += gstride()) {
         const int32_t w = d.world[i];
         if (w < 0) continue;
         const int32_t wf = d.w_flags[w];
         double q[4];
-        for (int c = 0; c < 3; c++) BODY(d.ppos, c, i) = BODY(d.pos, c, i);
-        for (int c = 0; c < 4; c++) { q[c] = BODY(d.quat, c, i); BODY(d.pquat, c, i) = q[c]; }
+        for (int c = 0; c < 4; c++) q[c] = BODY(d.quat, c, i);
+        if (MODE == PRE_STEP) {
+            for (int c = 0; c < 3; c++) BODY(d.ppos, c, i) = BODY(d.pos, c, i);
+            for (int c = 0; c < 4; c++) BODY(d.pquat, c, i) = q[c];
+        }
         const int mt = d.mt[i];
         const uint8_t aw = d.awake[i];
         d.awake0[i] = aw;
@@ -161,16 +201,12 @@ __global__ void k_pre(Dev d, int64_t nhw, double dt, double margin) {
         d.notready[i] = 0;
         double v[3] = {BODY(d.lv, 0, i), BODY(d.lv, 1, i), BODY(d.lv, 2, i)};
         if (mt == M_DYNAMIC && aw) {
-            const double gf = d.gf[i], dt_ = dt;
-            for (int r = 0; r < 3; r++) v[r] = v[r] + (d.w_grav[(size_t)r * d.wcap + w] * gf) * dt_;
-            double fl = maxd(1.0 - (d.ld[i] + d.w_damp[w]) * dt_, 0.0);
-            for (int r = 0; r < 3; r++) v[r] = v[r] * fl;
-            double sp = sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
-            const double qd = d.qd[i];
-            for (int r = 0; r < 3; r++) v[r] = v[r] - ((qd * sp) * v[r]) * dt_;
-            double fa = maxd(1.0 - (d.ad[i] + d.w_damp[d.wcap + w]) * dt_, 0.0);
-            for (int r = 0; r < 3; r++) BODY(d.lv, r, i) = v[r];
-            for (int r = 0; r < 3; r++) BODY(d.av, r, i) = BODY(d.av, r, i) * fa;
+            if (MODE == PRE_STEP) {
+                double wv[3] = {BODY(d.av, 0, i), BODY(d.av, 1, i), BODY(d.av, 2, i)};
+                const double g[3] = {d.w_grav[w], d.w_grav[(size_t)d.wcap + w], d.w_grav[2 * (size_t)d.wcap + w]};
+                forces_body(v, wv, g, d.gf[i], d.ld[i], d.ad[i], d.qd[i], d.w_damp[w], d.w_damp[d.wcap + w], dt);
+                for (int r = 0; r < 3; r++) { BODY(d.lv, r, i) = v[r]; BODY(d.av, r, i) = wv[r]; }
+            }
         } else if (mt == M_DYNAMIC && (wf & WF_COLLISION)) {
             // A sleeping body with a wake-grade speed makes the wake order matter.
             double v2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
@@ -179,7 +215,7 @@ __global__ void k_pre(Dev d, int64_t nhw, double dt, double margin) {
         if (wf & (WF_COLLISION | WF_JOINTS)) {
             double Rm[9];
             quat_to_matrix(q, Rm);
-            if (wf & WF_COLLISION) refit_body(d, i, Rm, margin);
+            if (MODE == PRE_STEP && (wf & WF_COLLISION)) refit_body(d, i, Rm, margin);
             R* rot = (R*)d.rot;
             R* iiw = (R*)d.iiw;
             const double dg[3] = {BODY(d.inv_inertia, 0, i), BODY(d.inv_inertia, 1, i), BODY(d.inv_inertia, 2, i)};
@@ -1000,7 +1036,7 @@ __device__ __forceinline__ void seed_lookup(const Dev& d, int32_t w, int32_t old
     const int64_t hm = d.hcap - 1;
     int64_t h = (int64_t)(hash_mix(key) & (uint64_t)hm);
     const int32_t rs = d.w_cache_s[(size_t)old * d.wcap + w], rn = d.w_cache_n[(size_t)old * d.wcap + w];
-    for (int probe = 0; probe < 64; probe++) {
+    for (int probe = 0; probe < HASH_PROBES; probe++) {
         const uint64_t kk = d.hkey[old][h];
         if (kk == ~0ull) return;
         if (kk == key) {
@@ -2463,6 +2499,28 @@ __global__ __launch_bounds__(BLOCK) void k_cache_plan(Dev d) {
     if (threadIdx.x == 0) d.cnt[C_CACHE] = (uint64_t)tot;
 }
 
+// Insert the head of the group of cache entries at t of buffer `buf` (the
+// first entry of a run of one (a, b)) into that buffer's hash; returns the
+// hash slot it newly filled, or -1.  A key already present (a stale entry of
+// an earlier fill, which lookups reject by range) is repointed.  No slot within
+// HASH_PROBES probes is a cache overflow: the group then starts cold, as
+// lookups give up after as many probes.
+__device__ __forceinline__ int32_t cache_insert(const Dev& d, int buf, int64_t t, int32_t a, int32_t b) {
+    const uint64_t key = ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
+    const int64_t hm = d.hcap - 1;
+    int64_t h = (int64_t)(hash_mix(key) & (uint64_t)hm);
+    for (int probe = 0; probe < HASH_PROBES; probe++) {
+        unsigned long long prev = atomicCAS((unsigned long long*)&d.hkey[buf][h], ~0ull, (unsigned long long)key);
+        if (prev == ~0ull || prev == key) {
+            d.hval[buf][h] = (int32_t)t;
+            return prev == ~0ull ? (int32_t)h : -1;
+        }
+        h = (h + 1) & hm;
+    }
+    atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 4ull);
+    return -1;
+}
+
 template <typename R, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_cache(Dev d) {
     const int32_t w = blockIdx.x, tid = threadIdx.x;
@@ -2486,18 +2544,22 @@ __global__ __launch_bounds__(BLOCK) void k_cache(Dev d) {
         d.ka[nw][t] = a; d.kb[nw][t] = b;
         for (int r = 0; r < 3; r++) { CON(d.kpt[nw], r, t) = CON(sp, r, s); CON(di, r, t) = CON(si, r, s); }
         int32_t slot = -1;
-        if (k == 0 || sa[s - 1] != a || sb[s - 1] != b) {
-            const uint64_t key = ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
-            const int64_t hm = d.hcap - 1;
-            int64_t h = (int64_t)(hash_mix(key) & (uint64_t)hm);
-            for (int64_t probe = 0; probe < d.hcap; probe++) {
-                unsigned long long prev = atomicCAS((unsigned long long*)&d.hkey[nw][h], ~0ull, (unsigned long long)key);
-                if (prev == ~0ull) slot = (int32_t)h;
-                if (prev == ~0ull || prev == key) { d.hval[nw][h] = (int32_t)t; break; }
-                h = (h + 1) & hm;
-            }
-        }
+        if (k == 0 || sa[s - 1] != a || sb[s - 1] != b) slot = cache_insert(d, nw, t, a, b);
         d.hslot[nw][t] = slot;
+    }
+}
+
+// A world's warm-start cache written from the host (omi_batch_write_cache):
+// the entries [base, base + n) of the current buffer are in place; their group
+// heads go into its hash.
+__global__ void k_cache_seed(Dev d, int64_t base, int32_t n) {
+    const int32_t cur = *d.sel;
+    for (int64_t k = gtid(); k < n; k += gstride()) {
+        const int64_t t = base + k;
+        const int32_t a = d.ka[cur][t], b = d.kb[cur][t];
+        int32_t slot = -1;
+        if (k == 0 || d.ka[cur][t - 1] != a || d.kb[cur][t - 1] != b) slot = cache_insert(d, cur, t, a, b);
+        d.hslot[cur][t] = slot;
     }
 }
 
@@ -2521,20 +2583,38 @@ __global__ void k_post(Dev d, int64_t nhw, double dt) {
         if (d.world[i] < 0) continue;
         const int mt = d.mt[i];
         if (!((mt == M_DYNAMIC || mt == M_KINEMATIC) && d.awake[i])) continue;
-        for (int r = 0; r < 3; r++) BODY(d.pos, r, i) = BODY(d.pos, r, i) + BODY(d.lv, r, i) * dt;
-        const double wx = BODY(d.av, 0, i), wy = BODY(d.av, 1, i), wz = BODY(d.av, 2, i);
-        const double bx = BODY(d.quat, 0, i), by = BODY(d.quat, 1, i), bz = BODY(d.quat, 2, i),
-                     bw = BODY(d.quat, 3, i);
-        const double mx = ((0.0 * bx + wx * bw) + wy * bz) - wz * by;
-        const double my = ((0.0 * by - wx * bz) + wy * bw) + wz * bx;
-        const double mz = ((0.0 * bz + wx * by) - wy * bx) + wz * bw;
-        const double mw = ((0.0 * bw - wx * bx) - wy * by) - wz * bz;
-        const double s0 = bx + (0.5 * mx) * dt, s1 = by + (0.5 * my) * dt;
-        const double s2 = bz + (0.5 * mz) * dt, s3 = bw + (0.5 * mw) * dt;
-        const double nn = sqrt(((s0 * s0 + s1 * s1) + s2 * s2) + s3 * s3);
-        const double dd = maxd(nn, 1e-12);
-        BODY(d.quat, 0, i) = s0 / dd; BODY(d.quat, 1, i) = s1 / dd;
-        BODY(d.quat, 2, i) = s2 / dd; BODY(d.quat, 3, i) = s3 / dd;
+        double p[3], q[4], v[3], w[3];
+        for (int r = 0; r < 3; r++) { p[r] = BODY(d.pos, r, i); v[r] = BODY(d.lv, r, i); w[r] = BODY(d.av, r, i); }
+        for (int r = 0; r < 4; r++) q[r] = BODY(d.quat, r, i);
+        positions_body(p, q, v, w, dt);
+        for (int r = 0; r < 3; r++) BODY(d.pos, r, i) = p[r];
+        for (int r = 0; r < 4; r++) BODY(d.quat, r, i) = q[r];
+    }
+}
+
+// The integrate stages on one world's host-layout rows ([n][3] velocities,
+// positions, [n][4] quaternions; [n][3] per-body gravity), for the
+// stage-level entry points.
+__global__ void k_stage_forces(int32_t n, const int32_t* mt, const uint8_t* awake, const double* g, const double* gf,
+                               const double* ld, const double* ad, const double* qd, double dlin, double dang,
+                               double dt, double* lv, double* av) {
+    for (int64_t i = gtid(); i < n; i += gstride()) {
+        if (!(mt[i] == M_DYNAMIC && awake[i])) continue;
+        double v[3] = {lv[3 * i], lv[3 * i + 1], lv[3 * i + 2]}, w[3] = {av[3 * i], av[3 * i + 1], av[3 * i + 2]};
+        forces_body(v, w, g + 3 * i, gf[i], ld[i], ad[i], qd[i], dlin, dang, dt);
+        for (int r = 0; r < 3; r++) { lv[3 * i + r] = v[r]; av[3 * i + r] = w[r]; }
+    }
+}
+
+__global__ void k_stage_positions(int32_t n, const int32_t* mt, const uint8_t* awake, const double* lv,
+                                  const double* av, double dt, double* pos, double* quat) {
+    for (int64_t i = gtid(); i < n; i += gstride()) {
+        if (!((mt[i] == M_DYNAMIC || mt[i] == M_KINEMATIC) && awake[i])) continue;
+        double p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+        double q[4] = {quat[4 * i], quat[4 * i + 1], quat[4 * i + 2], quat[4 * i + 3]};
+        positions_body(p, q, lv + 3 * i, av + 3 * i, dt);
+        for (int r = 0; r < 3; r++) pos[3 * i + r] = p[r];
+        for (int r = 0; r < 4; r++) quat[4 * i + r] = q[r];
     }
 }
 
@@ -2611,7 +2691,7 @@ __global__ void k_sleep_union(Dev d) {
 
 // _update_sleep (world.py:510-538), first half: speed tests, timers, and an
 // island is marked not-ready if any member is not ready.
-__global__ void k_sleep_body(Dev d, int64_t nhw, double dt) {
+__global__ void k_sleep_body(Dev d, int64_t nhw, double dt, double lin2, double ang2, double tsleep) {
     for (int64_t i = gtid(); i < nhw; i += gstride()) {
         const int32_t w = d.world[i];
         if (w < 0 || !(d.w_flags[w] & WF_SLEEP)) continue;
@@ -2619,10 +2699,10 @@ __global__ void k_sleep_body(Dev d, int64_t nhw, double dt) {
         const double x = BODY(d.lv, 0, i), y = BODY(d.lv, 1, i), z = BODY(d.lv, 2, i);
         const double p = BODY(d.av, 0, i), q = BODY(d.av, 1, i), r = BODY(d.av, 2, i);
         const double s2 = (x * x + y * y) + z * z, w2 = (p * p + q * q) + r * r;
-        const bool slow = dyn && s2 < 0.08 * 0.08 && w2 < 0.25 * 0.25;
+        const bool slow = dyn && s2 < lin2 && w2 < ang2;
         double tm = slow ? d.timer[i] + dt : 0.0;
         d.timer[i] = tm;
-        const bool ready = slow && tm > 0.4;
+        const bool ready = slow && tm > tsleep;
         d.awake0[i] = (uint8_t)((slow ? 1 : 0) | (ready ? 2 : 0));   // reuse as per-body flags
         if (d.contacted[i] && !ready) d.notready[uf_find(d.parent, (int32_t)i)] = 1;
     }
@@ -2713,11 +2793,31 @@ struct Launch {
 namespace omi {
 void launch_begin(const Dev& d, hipStream_t s) { Launch::begin(d, s); }
 
-void launch_pre(const Dev& d, int precision, int64_t nhw, double dt, double margin, hipStream_t s) {
+void launch_pre(const Dev& d, int precision, int64_t nhw, double dt, double margin, hipStream_t s, bool collide_only) {
     if (nhw <= 0) return;
     int g = GRID_FOR(nhw, 256);
-    if (precision == 64) hipLaunchKernelGGL(k_pre<double>, dim3(g), dim3(256), 0, s, d, nhw, dt, margin);
-    else hipLaunchKernelGGL(k_pre<float>, dim3(g), dim3(256), 0, s, d, nhw, dt, margin);
+    if (collide_only) {
+        if (precision == 64) hipLaunchKernelGGL((k_pre<double, PRE_COLLIDE>), dim3(g), dim3(256), 0, s, d, nhw, dt, margin);
+        else hipLaunchKernelGGL((k_pre<float, PRE_COLLIDE>), dim3(g), dim3(256), 0, s, d, nhw, dt, margin);
+    } else {
+        if (precision == 64) hipLaunchKernelGGL((k_pre<double, PRE_STEP>), dim3(g), dim3(256), 0, s, d, nhw, dt, margin);
+        else hipLaunchKernelGGL((k_pre<float, PRE_STEP>), dim3(g), dim3(256), 0, s, d, nhw, dt, margin);
+    }
+}
+
+void launch_stage_forces(int32_t n, const int32_t* mt, const uint8_t* awake, const double* g, const double* gf,
+                         const double* ld, const double* ad, const double* qd, double dlin, double dang, double dt,
+                         double* lv, double* av, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_stage_forces, dim3(GRID_FOR(n, 256)), dim3(256), 0, s, n, mt, awake, g, gf, ld, ad, qd, dlin,
+                       dang, dt, lv, av);
+}
+
+void launch_stage_positions(int32_t n, const int32_t* mt, const uint8_t* awake, const double* lv, const double* av,
+                            double dt, double* pos, double* quat, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_stage_positions, dim3(GRID_FOR(n, 256)), dim3(256), 0, s, n, mt, awake, lv, av, dt, pos,
+                       quat);
 }
 
 void launch_refit(const Dev& d, int64_t nhw, double margin, hipStream_t s) {
@@ -2895,14 +2995,21 @@ void launch_post(const Dev& d, int64_t nhw, double dt, hipStream_t s) {
     hipLaunchKernelGGL(k_post, dim3(GRID_FOR(nhw, 256)), dim3(256), 0, s, d, nhw, dt);
 }
 
-void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s) {
+// _update_sleep's thresholds (world.py:510-511) as the step uses them.
+void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s, double lin, double ang, double tsleep) {
     if (nhw <= 0) return;
     hipLaunchKernelGGL(k_sleep_union, dim3(GRID_FOR(d.kcap, 256)), dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_sleep_body, dim3(GRID_FOR(nhw, 256)), dim3(256), 0, s, d, nhw, dt);
+    hipLaunchKernelGGL(k_sleep_body, dim3(GRID_FOR(nhw, 256)), dim3(256), 0, s, d, nhw, dt, lin * lin, ang * ang,
+                       tsleep);
     hipLaunchKernelGGL(k_sleep_final, dim3(GRID_FOR(nhw, 256)), dim3(256), 0, s, d, nhw);
 }
 
 void launch_toggle(const Dev& d, hipStream_t s) { hipLaunchKernelGGL(k_toggle, dim3(1), dim3(64), 0, s, d); }
+
+void launch_cache_seed(const Dev& d, int64_t base, int32_t n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cache_seed, dim3(GRID_FOR(n, 256)), dim3(256), 0, s, d, base, n);
+}
 
 // _collide_native.capsule_triangles for one capsule and n triangles
 // verts[n][3][3], one thread per triangle; non-contacts give zeros.

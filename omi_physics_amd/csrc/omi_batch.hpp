@@ -50,6 +50,9 @@ constexpr int32_t IIW_STRIDE = 12;
 constexpr int32_t FP64_SOLVE_BLOCK_MAX = 256;
 // bytes of one contact's solver row (FP64: 28 doubles; FP32: 56 floats)
 constexpr int32_t SOLVE_ROW_BYTES = 224;
+// linear probes of the warm-start hash, for both insertion and lookup; an
+// insertion that finds no slot within them is reported as a cache overflow
+constexpr int32_t HASH_PROBES = 64;
 
 struct Dev {
     int64_t ncap, pcap, kcap, hcap;   // bodies, pairs, contacts, hash slots (pow2)

@@ -328,4 +328,5 @@ def arrays(spec: Dict[str, Any], gravity=(0.0, -9.81, 0.0), sleep_enabled: bool 
         gravity=g, default_linear_damping=0.0, default_angular_damping=0.0,
         sleep_enabled=int(sleep_enabled), velocity_iterations=int(velocity_iterations), has_collision=1,
         n_joints=0, joint_iterations=10, joint_a=np.zeros(0, np.int32), joint_b=np.zeros(0, np.int32),
-        joint_anchor=np.zeros((0, 3)))
+        joint_anchor=np.zeros((0, 3)), joint_local_a=np.zeros((0, 3)), joint_local_b=np.zeros((0, 3)),
+        joint_fixed=np.zeros(0, np.uint8))

@@ -1,32 +1,33 @@
-"""``PhysicsWorld`` — structure-of-arrays world state and the fixed-step pipeline.
+"""``PhysicsWorld`` with its body state resident in HBM.
 
-The public surface mirrors the reference ``world.py``: the same constructor
-keywords (``world.py:30-35``), ``add_shape/add_material/add_filter``
-(``:175-193``), ``set_pair_friction`` (``:207-223``), ``add_body``
-(``:253-299``) with the same mass-property rules (``:301-316``, ``:614-639``),
-the same column views (``:145-167``) and the same ``step(dt)`` stage order
-(``:410-430``).
+The API is the reference ``PhysicsWorld``'s (``world.py:30-35`` constructor
+keywords, ``add_shape/add_material/add_filter``, ``set_pair_friction``,
+``add_body`` and its mass rules, the column views, ``step``/``advance``/
+``writeback``, trigger listeners and point joints), so scenes written for the
+reference build here unchanged.
 
-The difference is where the columns live while the world steps.  The compute
-backend (``backend.HipBackend``) keeps a device-resident copy of every column in
-HBM and runs each stage there; the host numpy columns are a mirror that is
-refreshed lazily when a column view is read, and re-uploaded before the next
-step when it may have been edited.  ``step()`` itself never copies state
-between host and device.
+What differs is where the columns live.  The device copy, held by the
+``HipBackend``, is the live state: ``step()`` enqueues one fused device step
+(every stage of the reference's ``step``, ``world.py:410-430``, in its order)
+and returns without waiting.  The host keeps a mirror, one numpy array per
+column in a :class:`BodyColumns` store: a view of a column the device writes
+first refreshes the mirror, and handing out a view marks its column group
+(state, parameters or layout) as possibly edited.  Before the next step the
+backend compares each marked group with the values it last synchronised and
+uploads what changed — or, for layout, reloads the world with its warm-start
+cache and joint anchors kept.
 """
-from typing import Any, Dict, List, Optional, Tuple
+from typing import Any, Callable, Dict, List, Optional, Sequence, Tuple
 import os
 
 import numpy as np
 
-from . import model
+from . import layout, model
 from . import mathutil
+from .layout import KIND_NONE, KIND_SPHERE, KIND_BOX, KIND_CONVEX  # noqa: F401  (re-exported codes)
 
-_TYPE_INT = {model.STATIC: 0, model.KINEMATIC: 1, model.DYNAMIC: 2}
 STATIC, KINEMATIC, DYNAMIC = 0, 1, 2
-
-#: shape-kind codes shared with the device and the C ABI
-KIND_NONE, KIND_SPHERE, KIND_BOX, KIND_CONVEX = 0, 1, 2, 3
+_MOTION_CODE = {model.STATIC: STATIC, model.KINEMATIC: KINEMATIC, model.DYNAMIC: DYNAMIC}
 
 # Constants of the contact pipeline (reference values).
 REFIT_MARGIN = 0.05                 # world.py:471
@@ -34,21 +35,112 @@ SLEEP_LIN, SLEEP_ANG, SLEEP_TIME = 0.08, 0.25, 0.4   # world.py:510-511
 WAKE_SPEED = 0.15                   # solver.py:322
 
 
-class PhysicsWorld:
-    """A rigid-body world: add shapes/materials/filters, then bodies, then step."""
+class BodyColumns:
+    """Per-body columns, row ``i`` = body ``i``, in arrays whose row capacity is a
+    power of two; new rows start at each column's rest value.
 
-    _VEC = ('position', 'prev_position', 'orientation', 'prev_orientation',
-            'linear_velocity', 'angular_velocity', 'inv_inertia',
-            'aabb_min', 'aabb_max')
-    _SCALAR_F = ('inv_mass', 'mass', 'gravity_factor', 'linear_damping',
-                 'angular_damping', 'quadratic_drag', 'sleep_timer',
-                 'restitution_pad')
-    _SCALAR_I = ('motion_type', 'collider_shape', 'collider_material',
-                 'collider_filter', 'trigger_shape', 'island')
-    #: columns the device pipeline writes; the host mirror refreshes these
-    _DEVICE_WRITES = ('position', 'prev_position', 'orientation', 'prev_orientation',
-                      'linear_velocity', 'angular_velocity', 'aabb_min', 'aabb_max',
-                      'sleep_timer', 'awake')
+    ``groups`` tells which kind of device upload an edit of a column needs:
+    ``'state'`` (pose, velocity, sleep), ``'params'`` (mass, inertia, forces)
+    or ``'layout'`` (what the world's device tables are built from).
+    """
+
+    #: name -> (row shape, dtype, rest value, group)
+    SPEC: Dict[str, Tuple[Tuple[int, ...], Any, Any, str]] = {
+        'position': ((3,), 'f8', 0.0, 'state'),
+        'prev_position': ((3,), 'f8', 0.0, 'state'),
+        'orientation': ((4,), 'f8', (0.0, 0.0, 0.0, 1.0), 'state'),
+        'prev_orientation': ((4,), 'f8', (0.0, 0.0, 0.0, 1.0), 'state'),
+        'linear_velocity': ((3,), 'f8', 0.0, 'state'),
+        'angular_velocity': ((3,), 'f8', 0.0, 'state'),
+        'awake': ((), bool, True, 'state'),
+        'sleep_timer': ((), 'f8', 0.0, 'state'),
+        'aabb_min': ((3,), 'f8', 0.0, 'state'),
+        'aabb_max': ((3,), 'f8', 0.0, 'state'),
+        'inv_mass': ((), 'f8', 0.0, 'params'),
+        'mass': ((), 'f8', 0.0, 'params'),
+        'inv_inertia': ((3,), 'f8', 0.0, 'params'),
+        'gravity_factor': ((), 'f8', 0.0, 'params'),
+        'linear_damping': ((), 'f8', 0.0, 'params'),
+        'angular_damping': ((), 'f8', 0.0, 'params'),
+        'quadratic_drag': ((), 'f8', 0.0, 'params'),
+        'restitution_pad': ((), 'f8', 0.0, 'params'),
+        'motion_type': ((), 'i4', -1, 'layout'),
+        'collider_shape': ((), 'i4', -1, 'layout'),
+        'collider_material': ((), 'i4', -1, 'layout'),
+        'collider_filter': ((), 'i4', -1, 'layout'),
+        'trigger_shape': ((), 'i4', -1, 'layout'),
+        'island': ((), 'i4', -1, 'params'),
+    }
+    #: columns the device step writes
+    DEVICE_WRITES = ('position', 'prev_position', 'orientation', 'prev_orientation', 'linear_velocity',
+                     'angular_velocity', 'awake', 'sleep_timer', 'aabb_min', 'aabb_max')
+
+    def __init__(self) -> None:
+        self.n = 0
+        self.rows = 0
+        self.arrays: Dict[str, np.ndarray] = {}
+        self._resize(8)
+
+    def _resize(self, rows: int) -> None:
+        for name, (shape, dtype, rest, _) in self.SPEC.items():
+            fresh = np.empty((rows,) + shape, dtype=dtype)
+            fresh[...] = rest
+            old = self.arrays.get(name)
+            if old is not None:
+                fresh[:self.n] = old[:self.n]
+            self.arrays[name] = fresh
+        self.rows = rows
+
+    def append(self) -> int:
+        """A new row at its rest values; returns its index."""
+        if self.n == self.rows:
+            self._resize(2 * self.rows)
+        self.n += 1
+        return self.n - 1
+
+    def live(self, name: str) -> np.ndarray:
+        return self.arrays[name][:self.n]
+
+
+def inertia_diagonal(shape: Optional[model.Shape], mass: float) -> np.ndarray:
+    """Solid-body inertia about the centre for a shape of ``mass`` (reference
+    ``world.py:614-639``): sphere 2/5 m r^2; box m/12 (y^2+z^2, x^2+z^2, x^2+y^2)
+    of its full extents; a convex shape takes the box of its points' bounds."""
+    def box(extent):
+        x, y, z = extent
+        return (mass / 12.0) * np.array([y * y + z * z, x * x + z * z, x * x + y * y])
+    if shape is None:
+        return np.zeros(3)
+    if shape.type == 'sphere':
+        return np.full(3, 0.4 * mass * shape.radius ** 2)
+    if shape.type == 'box':
+        return box(shape.size)
+    if shape.points is not None and len(shape.points):
+        return box(shape.points.max(axis=0) - shape.points.min(axis=0))
+    return np.zeros(3)
+
+
+def mass_properties(motion: model.Motion, shape: Optional[model.Shape]) -> Tuple[float, float, np.ndarray]:
+    """``(mass, inv_mass, inv_inertia)`` of a body (reference ``world.py:301-316``):
+    static and kinematic bodies get zeros; a dynamic body's mass is at least 1e-9,
+    its inertia diagonal the given one or the shape's, inverted where positive."""
+    if motion.type != model.DYNAMIC:
+        return 0.0, 0.0, np.zeros(3)
+    m = max(motion.mass, 1e-9)
+    diag = np.asarray(motion.inertiaDiagonal, dtype='d')
+    if not diag.any():
+        diag = inertia_diagonal(shape, m)
+    with np.errstate(divide='ignore'):
+        inv_i = np.where(diag > 0, 1.0 / np.maximum(diag, 1e-12), 0.0)
+    return m, 1.0 / m, inv_i
+
+
+def _column_view(name: str):
+    return property(lambda self: self._view(name), doc='Column %r over the live bodies.' % name)
+
+
+class PhysicsWorld:
+    """A rigid-body world on the GPU: add shapes/materials/filters, then bodies, then step."""
 
     def __init__(self, gravity: Optional[model.Gravity] = None,
                  fixed_dt: float = 1.0 / 60.0, max_frame: float = 0.25,
@@ -59,109 +151,71 @@ class PhysicsWorld:
                  velocity_iterations: int = 10,
                  precision: str = 'fp64') -> None:
         self.gravity = gravity if gravity is not None else model.Gravity()
-        self.fixed_dt = fixed_dt
-        self.max_frame = max_frame
+        self.fixed_dt, self.max_frame = fixed_dt, max_frame
         self.sleep_enabled = sleep_enabled
         self.default_linear_damping = default_linear_damping
         self.default_angular_damping = default_angular_damping
-        self.gpu_threshold = gpu_threshold
+        self.gpu_threshold = gpu_threshold          # accepted for the reference's signature
         self.velocity_iterations = int(velocity_iterations)
         self.precision = precision
-
         self.shapes: List[model.Shape] = []
         self.materials: List[model.Material] = []
         self.filters: List[model.CollisionFilter] = []
         self.pair_friction: Dict[Tuple[int, int], Tuple[float, float]] = {}
         self.bodies: List[Any] = []
-
-        self._n = 0
-        self._cap = 0
-        self._accumulator = 0.0
-        self.time = 0.0
-        self._alloc(8)
-        self._collision: Any = None
-        self.trigger_listeners: List[Any] = []
+        self.trigger_listeners: List[Callable[..., None]] = []
         self.joint_constraints: List[Any] = []
         self.joint_iterations = 10
-        self._layout_version = 0           # bumps when bodies/tables change
-        self._host_dirty = True            # host mirror may be newer than device
-        self._device_fresh = False         # device state newer than host mirror
-        self._contacts_cache: Any = None
-        self._init_backend(backend)
+        self.time = 0.0
+        self._accumulator = 0.0
+        self._cols = BodyColumns()
+        self._collision: Any = None
+        # device synchronisation: column groups edited on the host since the last
+        # upload, and whether the device has stepped past the host mirror
+        self._edited = {'state', 'params', 'layout'}
+        self._device_fresh = False
+        self._contacts_cache: Optional[List[Any]] = None
+        self.backend = self._make_backend(backend)
+        self._backend_mode = 'fixed'
 
-    # -- storage ---------------------------------------------------------
-    def _alloc(self, cap: int) -> None:
-        """Allocate zeroed columns for ``cap`` bodies (reference ``world.py:110-122``)."""
-        self._cap = cap
-        for name in self._VEC:
-            width = 4 if 'orientation' in name else 3
-            setattr(self, '_' + name, np.zeros((cap, width), dtype='d'))
-        for name in self._SCALAR_F:
-            setattr(self, '_' + name, np.zeros(cap, dtype='d'))
-        for name in self._SCALAR_I:
-            setattr(self, '_' + name, np.full(cap, -1, dtype='i4'))
-        self._awake = np.ones(cap, dtype=bool)
-        self._orientation[:, 3] = 1.0
-        self._prev_orientation[:, 3] = 1.0
+    @staticmethod
+    def _make_backend(backend: Any) -> Any:
+        """The compute backend: an object is used as is; a name (or
+        ``OPENGLCONTEXT_PHYSICS_BACKEND``, reference ``world.py:319-350``) goes to
+        :func:`backend.select_backend`, which fails loudly without the HIP library."""
+        if backend is not None and not isinstance(backend, str):
+            return backend
+        from .backend import select_backend
+        return select_backend(backend or os.environ.get('OPENGLCONTEXT_PHYSICS_BACKEND', 'auto'))
 
-    def _grow(self) -> None:
-        """Double the capacity, preserving state (reference ``world.py:124-138``)."""
-        old = self._cap
-        new = max(8, old * 2)
-        for name in self._VEC + self._SCALAR_F + self._SCALAR_I + ('awake',):
-            arr = getattr(self, '_' + name)
-            grown = np.zeros((new,) + arr.shape[1:], dtype=arr.dtype)
-            if name in self._SCALAR_I:
-                grown.fill(-1)
-            grown[:old] = arr
-            setattr(self, '_' + name, grown)
-        self._orientation[old:, 3] = 1.0
-        self._prev_orientation[old:, 3] = 1.0
-        self._awake[old:] = True
-        self._cap = new
-
-    def _pull(self) -> None:
-        """Refresh the host mirror from the device if the device is newer."""
-        if self._device_fresh:
-            self._device_fresh = False
-            self.backend.download(self)
-        # a view handed out may be written through: upload before the next step
-        self._host_dirty = True
-
+    # -- columns ---------------------------------------------------------
     def _view(self, name: str) -> np.ndarray:
-        """Column ``name`` over the live bodies (the first ``body_count`` rows)."""
-        if name in self._DEVICE_WRITES:
-            self._pull()
-        return getattr(self, '_' + name)[:self._n]
+        if name in BodyColumns.DEVICE_WRITES:
+            self._refresh_mirror()
+        self._edited.add(BodyColumns.SPEC[name][3])     # a view may be written through
+        return self._cols.live(name)
 
-    position = property(lambda s: s._view('position'))
-    prev_position = property(lambda s: s._view('prev_position'))
-    orientation = property(lambda s: s._view('orientation'))
-    prev_orientation = property(lambda s: s._view('prev_orientation'))
-    linear_velocity = property(lambda s: s._view('linear_velocity'))
-    angular_velocity = property(lambda s: s._view('angular_velocity'))
-    inv_mass = property(lambda s: s._view('inv_mass'))
-    mass = property(lambda s: s._view('mass'))
-    inv_inertia = property(lambda s: s._view('inv_inertia'))
-    gravity_factor = property(lambda s: s._view('gravity_factor'))
-    linear_damping = property(lambda s: s._view('linear_damping'))
-    angular_damping = property(lambda s: s._view('angular_damping'))
-    quadratic_drag = property(lambda s: s._view('quadratic_drag'))
-    motion_type = property(lambda s: s._view('motion_type'))
-    collider_shape = property(lambda s: s._view('collider_shape'))
-    collider_material = property(lambda s: s._view('collider_material'))
-    collider_filter = property(lambda s: s._view('collider_filter'))
-    trigger_shape = property(lambda s: s._view('trigger_shape'))
-    aabb_min = property(lambda s: s._view('aabb_min'))
-    aabb_max = property(lambda s: s._view('aabb_max'))
-    awake = property(lambda s: s._view('awake'))
-    island = property(lambda s: s._view('island'))
-    sleep_timer = property(lambda s: s._view('sleep_timer'))
+    def _refresh_mirror(self) -> None:
+        if self._device_fresh:
+            self.backend.download(self)                 # raises if the device state is invalid
+            self._device_fresh = False
+
+    def recover_device(self) -> None:
+        """After a capacity error: drop the device copy; the next step reloads the
+        world from the host mirror (the state last read back) into a batch with
+        doubled pair and contact capacities."""
+        self.backend.recover(self)
+        self._device_fresh = False
+        self._edited.update(('state', 'params', 'layout'))
+
+    def _raw(self, name: str) -> np.ndarray:
+        """The host mirror of a column without refreshing it or marking an edit."""
+        return self._cols.live(name)
 
     @property
     def body_count(self) -> int:
         """The number of live bodies."""
-        return self._n
+        return self._cols.n
 
     # -- document tables -------------------------------------------------
     def add_shape(self, shape: model.Shape) -> int:
@@ -175,39 +229,33 @@ class PhysicsWorld:
     def add_material(self, mat: model.Material) -> int:
         """Register a physics material; return its index."""
         self.materials.append(mat)
-        self._layout_version += 1
+        self._edited.add('layout')
         return len(self.materials) - 1
 
     def add_filter(self, filt: model.CollisionFilter) -> int:
         """Register a collision filter; return its index."""
         self.filters.append(filt)
-        self._layout_version += 1
+        self._edited.add('layout')
         return len(self.filters) - 1
 
     def material_for(self, index: int) -> model.Material:
-        """The material at ``index``, or the default material if out of range."""
-        if 0 <= index < len(self.materials):
-            return self.materials[index]
-        return model.DEFAULT_MATERIAL
+        """Material ``index``; an index outside the table means the default material."""
+        return self.materials[index] if 0 <= index < len(self.materials) else model.DEFAULT_MATERIAL
 
     def filter_for(self, index: int) -> model.CollisionFilter:
-        """The filter at ``index``, or the default filter if out of range."""
-        if 0 <= index < len(self.filters):
-            return self.filters[index]
-        return model.DEFAULT_FILTER
+        """Filter ``index``; an index outside the table means the default filter."""
+        return self.filters[index] if 0 <= index < len(self.filters) else model.DEFAULT_FILTER
 
     def set_pair_friction(self, material_a: int, material_b: int,
                           static_mu: float, dynamic_mu: float) -> None:
-        """Override contact friction for one unordered material pair (reference ``world.py:207-218``)."""
-        key = (material_a, material_b) if material_a <= material_b else (material_b, material_a)
-        self.pair_friction[key] = (float(static_mu), float(dynamic_mu))
-        self._layout_version += 1
+        """Friction for contacts between two materials, either order (reference ``world.py:207-218``)."""
+        self.pair_friction[(min(material_a, material_b), max(material_a, material_b))] = \
+            (float(static_mu), float(dynamic_mu))
+        self._edited.add('layout')
 
-    def pair_friction_for(self, material_a: int,
-                          material_b: int) -> Optional[Tuple[float, float]]:
-        """``(static_mu, dynamic_mu)`` for the pair, or ``None``."""
-        key = (material_a, material_b) if material_a <= material_b else (material_b, material_a)
-        return self.pair_friction.get(key)
+    def pair_friction_for(self, material_a: int, material_b: int) -> Optional[Tuple[float, float]]:
+        """``(static_mu, dynamic_mu)`` set for the material pair, or ``None``."""
+        return self.pair_friction.get((min(material_a, material_b), max(material_a, material_b)))
 
     # -- bodies ----------------------------------------------------------
     def add_body(self, motion: Optional[model.Motion] = None,
@@ -215,124 +263,101 @@ class PhysicsWorld:
                  trigger: Optional[model.Trigger] = None,
                  position=(0, 0, 0), orientation=(0, 0, 0, 1),
                  handle: Any = None) -> int:
-        """Create a body from its motion/collider/trigger; return its index."""
-        self._pull()
-        if motion is None:
-            motion = model.Motion()
-        if self._n >= self._cap:
-            self._grow()
-        i = self._n
-        self._n += 1
-        self._position[i] = position
-        self._prev_position[i] = position
-        self._orientation[i] = orientation
-        self._prev_orientation[i] = orientation
-        self._linear_velocity[i] = motion.linearVelocity
-        self._angular_velocity[i] = motion.angularVelocity
-        self._motion_type[i] = _TYPE_INT[motion.type]
-        self._gravity_factor[i] = motion.gravityFactor
-        self._linear_damping[i] = motion.linearDamping
-        self._angular_damping[i] = motion.angularDamping
-        self._quadratic_drag[i] = motion.quadraticDrag
-        self._awake[i] = True
-        self._sleep_timer[i] = 0.0
-
-        shape: Optional[model.Shape] = None
+        """Append a body built from its motion, collider and trigger; return its index."""
+        self._refresh_mirror()
+        motion = motion or model.Motion()
+        i = self._cols.append()
+        col = self._cols.arrays
+        for name in ('position', 'prev_position'):
+            col[name][i] = position
+        for name in ('orientation', 'prev_orientation'):
+            col[name][i] = orientation
+        col['linear_velocity'][i] = motion.linearVelocity
+        col['angular_velocity'][i] = motion.angularVelocity
+        col['motion_type'][i] = _MOTION_CODE[motion.type]
+        col['gravity_factor'][i] = motion.gravityFactor
+        col['linear_damping'][i] = motion.linearDamping
+        col['angular_damping'][i] = motion.angularDamping
+        col['quadratic_drag'][i] = motion.quadraticDrag
+        shape = None
         if collider is not None and 0 <= collider.shape < len(self.shapes):
             shape = self.shapes[collider.shape]
-            self._collider_shape[i] = collider.shape
-            self._collider_material[i] = collider.physicsMaterial
-            self._collider_filter[i] = collider.collisionFilter
-            self._collision = self._collision or self.backend.make_collision_stages(self)
+            col['collider_shape'][i] = collider.shape
+            col['collider_material'][i] = collider.physicsMaterial
+            col['collider_filter'][i] = collider.collisionFilter
         if trigger is not None and 0 <= trigger.shape < len(self.shapes):
-            self._trigger_shape[i] = trigger.shape
+            col['trigger_shape'][i] = trigger.shape
+        if col['collider_shape'][i] >= 0 or col['trigger_shape'][i] >= 0:
             self._collision = self._collision or self.backend.make_collision_stages(self)
-        self._set_mass_properties(i, motion, shape)
+        col['mass'][i], col['inv_mass'][i], col['inv_inertia'][i] = mass_properties(motion, shape)
         self.bodies.append(handle)
-        self._layout_version += 1
+        self._edited.update(('state', 'params', 'layout'))
         return i
 
-    def _set_mass_properties(self, i: int, motion: model.Motion,
-                             shape: Optional[model.Shape]) -> None:
-        """Mass and inverse inertia; zero for static/kinematic (reference ``world.py:301-316``)."""
-        if motion.type != model.DYNAMIC:
-            self._inv_mass[i] = 0.0
-            self._mass[i] = 0.0
-            self._inv_inertia[i] = 0.0
-            return
-        m = max(motion.mass, 1e-9)
-        self._mass[i] = m
-        self._inv_mass[i] = 1.0 / m
-        diag = np.asarray(motion.inertiaDiagonal, dtype='d')
-        if not diag.any():
-            diag = inertia_diagonal(shape, m) if shape is not None else np.zeros(3)
-        with np.errstate(divide='ignore'):
-            self._inv_inertia[i] = np.where(diag > 0, 1.0 / np.maximum(diag, 1e-12), 0.0)
+    def apply_impulse(self, i: int, impulse) -> None:
+        """``v += J / m`` on body ``i`` and wake it (reference ``world.py:231-238``)."""
+        lv = self.linear_velocity
+        lv[i] = lv[i] + self._raw('inv_mass')[i] * np.asarray(impulse, dtype='d')
+        self.wake(i)
 
-    # -- backend policy --------------------------------------------------
-    def _init_backend(self, backend: Any) -> None:
-        """Choose the compute backend (reference ``world.py:319-350``).
-
-        An explicit backend object is used as-is.  Otherwise the name (or the
-        ``OPENGLCONTEXT_PHYSICS_BACKEND`` variable) goes to
-        :func:`backend.select_backend`, which raises when the HIP library
-        cannot be loaded rather than substituting a CPU path.
-        """
-        from .backend import select_backend
-        if backend is not None and not isinstance(backend, str):
-            self.backend = backend
-        else:
-            mode = backend or os.environ.get('OPENGLCONTEXT_PHYSICS_BACKEND', 'auto')
-            self.backend = select_backend(mode)
-        self._backend_mode = 'fixed'
-
-    # -- masks & gravity -------------------------------------------------
-    def dynamic_mask(self) -> np.ndarray:
-        """Awake dynamic bodies."""
-        return (self.motion_type == DYNAMIC) & self.awake
-
-    def moving_mask(self) -> np.ndarray:
-        """Awake dynamic or kinematic bodies."""
-        return ((self.motion_type == DYNAMIC) | (self.motion_type == KINEMATIC)) & self.awake
-
-    def resolve_gravity(self) -> np.ndarray:
-        """The uniform gravity acceleration vector ``(3,)`` (reference ``world.py:386-396``)."""
-        base = np.asarray(self.gravity.direction, dtype='d')
-        norm = np.linalg.norm(base)
-        base = base / norm if norm else base
-        return base * self.gravity.gravity
+    def apply_angular_impulse(self, i: int, impulse) -> None:
+        """``w += I_world^-1 L`` on body ``i`` about world axes, and wake it
+        (reference ``world.py:240-251``)."""
+        R = mathutil.quat_to_matrix(self.orientation[i])
+        inv_world = (R * self._raw('inv_inertia')[i]) @ R.T
+        av = self.angular_velocity
+        av[i] = av[i] + inv_world @ np.asarray(impulse, dtype='d')
+        self.wake(i)
 
     def wake(self, i: int) -> None:
         """Wake body ``i`` and reset its sleep timer."""
-        self._pull()
-        self._awake[i] = True
-        self._sleep_timer[i] = 0.0
+        self.awake[i] = True
+        self.sleep_timer[i] = 0.0
 
-    # -- triggers --------------------------------------------------------
-    def add_trigger_listener(self, fn: Any) -> None:
-        """Register ``fn(event_type, trigger_index, other_index)`` (reference ``world.py:580-582``)."""
+    # -- masks & gravity -------------------------------------------------
+    def dynamic_mask(self) -> np.ndarray:
+        """The awake dynamic bodies."""
+        return (self.motion_type == DYNAMIC) & self.awake
+
+    def moving_mask(self) -> np.ndarray:
+        """The awake dynamic or kinematic bodies."""
+        return np.isin(self.motion_type, (DYNAMIC, KINEMATIC)) & self.awake
+
+    def is_awake_mover(self, i: int) -> bool:
+        """An awake dynamic body, or any kinematic body (reference ``world.py:381-384``)."""
+        t = self._raw('motion_type')[i]
+        return bool(t == KINEMATIC or (t == DYNAMIC and self.awake[i]))
+
+    def resolve_gravity(self) -> np.ndarray:
+        """The gravity acceleration ``(3,)`` every body feels (no gravity volumes here)."""
+        return layout.uniform_gravity(self)
+
+    # -- triggers and joints ---------------------------------------------
+    def add_trigger_listener(self, fn: Callable[..., None]) -> None:
+        """Register ``fn(event_type, trigger_index, other_index)``."""
         self.trigger_listeners.append(fn)
 
     def dispatch_trigger(self, event_type: str, trigger_index: int, other_index: int) -> None:
-        """Notify every registered trigger listener of one overlap event (reference ``world.py:584-588``)."""
+        """Call every trigger listener with one overlap event."""
         for fn in self.trigger_listeners:
             fn(event_type, trigger_index, other_index)
 
     def add_joint_constraint(self, constraint: Any) -> Any:
-        """Register a ``joints.PointConstraint``, waking its bodies (reference ``world.py:590-597``)."""
+        """Register a ``joints.PointConstraint`` and wake its bodies."""
         from .joints import PointConstraint
         if not isinstance(constraint, PointConstraint):
-            raise TypeError('only PointConstraint joints run on the device')
+            raise TypeError('the device solves PointConstraint joints')
         self.joint_constraints.append(constraint)
         for i in (constraint.a, constraint.b):
             if i >= 0:
                 self.wake(i)
-        self._layout_version += 1
+        self._edited.add('layout')
         return constraint
 
-    # -- the step --------------------------------------------------------
+    # -- stepping --------------------------------------------------------
     def advance(self, real_dt: float) -> float:
-        """Fixed-timestep accumulator (reference ``world.py:399-408``)."""
+        """Step as many fixed steps as the accumulated real time allows (clamped to
+        ``max_frame``); return the leftover fraction of a step for render blending."""
         self._accumulator = min(self._accumulator + real_dt, self.max_frame)
         while self._accumulator >= self.fixed_dt:
             self.step(self.fixed_dt)
@@ -340,167 +365,47 @@ class PhysicsWorld:
         return self._accumulator / self.fixed_dt
 
     def step(self, dt: float) -> None:
-        """Advance one fixed step: forces, collision, positions, sleep (reference ``world.py:410-430``).
-
-        Every stage is a device launch on the backend's stream; nothing here
-        waits for the GPU.
-        """
-        self.backend.begin_step(self, dt)
-        self.backend.integrate_forces(self, dt)
-        if self._collision is not None:
-            self.backend.refit_aabbs(self)
-            self._collision.run(self, dt)
-        self.backend.integrate_positions(self, dt)
-        if self.joint_constraints:
-            self.backend.solve_joints(self, dt)
-        if self.sleep_enabled:
-            self.backend.update_sleep(self, dt)
-        self.backend.end_step(self, dt)
+        """One fixed step of every stage, enqueued on the device without waiting."""
+        self.backend.step_resident(self, dt)
         self._device_fresh = True
         self._contacts_cache = None
         self.time += dt
 
     def refit_aabbs(self, margin: float = REFIT_MARGIN) -> None:
-        """Recompute world AABBs on the device (reference ``world.py:471-508``)."""
-        self.backend.refit_aabbs(self, margin)
+        """Recompute the world AABBs from the current poses."""
+        self.backend.refit_resident(self, margin)
         self._device_fresh = True
 
     @property
     def contacts(self) -> List[Any]:
-        """The last step's contacts, materialised on demand (reference ``world.py:672``)."""
+        """The last step's contacts in solver order, read from the device on first use."""
         if self._contacts_cache is None:
-            self._contacts_cache = self.backend.read_contacts(self) if self._collision else []
+            self._contacts_cache = self.backend.read_contacts(self) if self._collision is not None else []
         return self._contacts_cache
 
+    @contacts.setter
+    def contacts(self, value: List[Any]) -> None:
+        self._contacts_cache = list(value)
+
     def writeback(self, alpha: float = 1.0) -> Tuple[np.ndarray, np.ndarray]:
-        """Interpolated render poses (reference ``world.py:432-437``)."""
-        pos = self.prev_position * (1 - alpha) + self.position * alpha
-        quat = mathutil.quat_normalize(
-            self.prev_orientation * (1 - alpha) + self.orientation * alpha)
+        """Render poses blended from the previous to the current step by ``alpha``
+        (positions linearly, orientations normalised after blending)."""
+        a = float(alpha)
+        pos = self.prev_position * (1.0 - a) + self.position * a
+        quat = mathutil.quat_normalize(self.prev_orientation * (1.0 - a) + self.orientation * a)
         return pos, quat
 
-    # -- export for the device loader -------------------------------------
+    # -- export ----------------------------------------------------------
     def export_arrays(self) -> Dict[str, Any]:
-        """The world as flat contiguous arrays in the layout the C ABI loads.
-
-        Also used by the tests to hand the identical state to the oracle.
-        """
-        return export_world_arrays(self)
+        """The world in the C-ABI load layout (also what the oracle loads)."""
+        return layout.world_arrays(self)
 
 
-def inertia_diagonal(shape: Optional[model.Shape], mass: float) -> np.ndarray:
-    """Solid-body inertia diagonal about the centre (reference ``world.py:614-639``)."""
-    if shape is None:
-        return np.zeros(3)
-    t = shape.type
-    if t == 'sphere':
-        v = 0.4 * mass * shape.radius ** 2
-        return np.array([v, v, v])
-    if t == 'box':
-        x, y, z = shape.size
-        k = mass / 12.0
-        return k * np.array([y * y + z * z, x * x + z * z, x * x + y * y])
-    if shape.points is not None and len(shape.points):
-        lo = shape.points.min(axis=0)
-        hi = shape.points.max(axis=0)
-        x, y, z = hi - lo
-        k = mass / 12.0
-        return k * np.array([y * y + z * z, x * x + z * z, x * x + y * y])
-    return np.zeros(3)
+for _name in BodyColumns.SPEC:
+    setattr(PhysicsWorld, _name, _column_view(_name))
+del _name
 
 
-def _slot_table(raw: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
-    """Distinct raw index values -> dense slots; returns ``(values, slot_per_body)``."""
-    values, slots = np.unique(raw, return_inverse=True)
-    return values.astype(np.int64), slots.astype(np.int32)
-
-
-def export_world_arrays(w: PhysicsWorld) -> Dict[str, Any]:
-    """Flatten a world into the C-ABI layout (row-major ``[n][3]`` / ``[n][4]`` columns).
-
-    Material and filter indices are remapped to dense slots over the distinct
-    raw values the bodies use, and the per-slot-pair friction, restitution
-    (reference ``solver.py:252-259``) and filter (``broadphase.py:297-301``)
-    outcomes are tabulated, so device and oracle read one table entry per pair.
-    """
-    n = w._n
-    g = lambda name: np.ascontiguousarray(getattr(w, '_' + name)[:n])
-    kind = np.zeros(n, dtype=np.int32)
-    half = np.zeros((n, 3), dtype='d')
-    hull_off = np.full(n, -1, dtype=np.int32)
-    hull_cnt = np.zeros(n, dtype=np.int32)
-    hull_pts: List[np.ndarray] = []
-    hull_total = 0
-    cs = w._collider_shape[:n]
-    ts = w._trigger_shape[:n]
-    for i in range(n):
-        idx = int(cs[i]) if cs[i] >= 0 else int(ts[i])
-        if idx < 0:
-            continue
-        if cs[i] >= 0 and ts[i] >= 0 and cs[i] != ts[i]:
-            # one shape per body on the device: its collider's, which is also
-            # what the trigger overlap test uses
-            raise NotImplementedError('body %d has a collider and a different trigger shape' % i)
-        sh = w.shapes[idx]
-        if sh.type == 'sphere':
-            kind[i] = KIND_SPHERE
-            half[i] = sh.radius
-        elif sh.type == 'box':
-            kind[i] = KIND_BOX
-            half[i] = 0.5 * np.asarray(sh.size, dtype='d')
-        else:
-            kind[i] = KIND_CONVEX
-            pts = np.ascontiguousarray(sh.points, dtype='d')
-            hull_off[i] = hull_total
-            hull_cnt[i] = len(pts)
-            hull_pts.append(pts)
-            hull_total += len(pts)
-    mat_vals, mat_slot = _slot_table(w._collider_material[:n])
-    nm = len(mat_vals)
-    fric = np.zeros((nm, nm), dtype='d')
-    rest = np.zeros((nm, nm), dtype='d')
-    for i, ra in enumerate(mat_vals):
-        ma = w.material_for(int(ra))
-        for j, rb in enumerate(mat_vals):
-            mb = w.material_for(int(rb))
-            rest[i, j] = model.combine(ma.restitution, mb.restitution, ma.restitutionCombine)
-            override = w.pair_friction_for(int(ra), int(rb))
-            fric[i, j] = (override[1] if override is not None else
-                          model.combine(ma.dynamicFriction, mb.dynamicFriction,
-                                        ma.frictionCombine))
-    filt_vals, filt_slot = _slot_table(w._collider_filter[:n])
-    nf = len(filt_vals)
-    ftab = np.zeros((nf, nf), dtype=np.uint8)
-    for i, ra in enumerate(filt_vals):
-        for j, rb in enumerate(filt_vals):
-            ftab[i, j] = w.filter_for(int(ra)).collides_with(w.filter_for(int(rb)))
-    return dict(
-        n=n,
-        position=g('position'), orientation=g('orientation'),
-        prev_position=g('prev_position'), prev_orientation=g('prev_orientation'),
-        linear_velocity=g('linear_velocity'), angular_velocity=g('angular_velocity'),
-        inv_mass=g('inv_mass'), inv_inertia=g('inv_inertia'),
-        gravity_factor=g('gravity_factor'), linear_damping=g('linear_damping'),
-        angular_damping=g('angular_damping'), quadratic_drag=g('quadratic_drag'),
-        sleep_timer=g('sleep_timer'),
-        motion_type=g('motion_type').astype(np.int32),
-        collider_shape=g('collider_shape').astype(np.int32),
-        trigger_shape=g('trigger_shape').astype(np.int32),
-        awake=np.ascontiguousarray(w._awake[:n]).astype(np.uint8),
-        kind=kind, half=half,
-        hull_off=hull_off, hull_cnt=hull_cnt,
-        hull_pts=(np.concatenate(hull_pts) if hull_pts else np.zeros((0, 3))),
-        mat_slot=mat_slot, n_mat=nm,
-        friction_table=np.ascontiguousarray(fric), restitution_table=np.ascontiguousarray(rest),
-        filt_slot=filt_slot, n_filt=nf, filter_table=np.ascontiguousarray(ftab),
-        gravity=np.asarray(w.resolve_gravity(), dtype='d'),
-        default_linear_damping=float(w.default_linear_damping),
-        default_angular_damping=float(w.default_angular_damping),
-        sleep_enabled=int(bool(w.sleep_enabled)),
-        velocity_iterations=int(w.velocity_iterations),
-        has_collision=int(w._collision is not None),
-        n_joints=len(w.joint_constraints), joint_iterations=int(w.joint_iterations),
-        joint_a=np.array([c.a for c in w.joint_constraints], np.int32),
-        joint_b=np.array([c.b for c in w.joint_constraints], np.int32),
-        joint_anchor=np.array([c.anchor for c in w.joint_constraints], dtype='d').reshape(-1, 3),
-    )
+def export_world_arrays(world: Any) -> Dict[str, Any]:
+    """The C-ABI load layout of any world with the reference's public surface."""
+    return layout.world_arrays(world)

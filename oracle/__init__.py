@@ -53,6 +53,7 @@ class OWDesc(ctypes.Structure):
         ('has_collision', ctypes.c_int32),
         ('n_joints', ctypes.c_int32), ('joint_iterations', ctypes.c_int32),
         ('joint_a', _i32), ('joint_b', _i32), ('joint_anchor', _d),
+        ('joint_local_a', _d), ('joint_local_b', _d), ('joint_fixed', _u8),
     ]
 
 
@@ -76,6 +77,8 @@ def lib() -> ctypes.CDLL:
         L.ow_body_count.argtypes = [ctypes.c_void_p]
         L.ow_get_state.argtypes = [ctypes.c_void_p, _d, _d, _d, _d, _u8, _d, _d, _d]
         L.ow_set_velocity.argtypes = [ctypes.c_void_p, _d, _d]
+        L.ow_set_params.argtypes = [ctypes.c_void_p, _d, _d, _d, _d, _d, _d, _d, _d, ctypes.c_int32, ctypes.c_int32]
+        L.ow_set_state.argtypes = [ctypes.c_void_p, _d, _d, _d, _d, _u8, _d]
         L.ow_num_contacts.argtypes = [ctypes.c_void_p]
         L.ow_get_contacts.argtypes = [ctypes.c_void_p, _i32, _i32, _d, _d, _d, _d, _d]
         L.ow_num_pairs.argtypes = [ctypes.c_void_p, ctypes.c_int32]
@@ -104,8 +107,10 @@ def lib() -> ctypes.CDLL:
         L.ow_prepare_and_solve.argtypes = [ctypes.c_int64, _i64, _i64, _d, _d, _d, _d, _d, _d, _d, _d,
                                            _d, _d, _d, _d, ctypes.c_double, ctypes.c_int32,
                                            ctypes.c_int32, ctypes.c_double, ctypes.c_double]
-        L.ow_integrate_forces.argtypes = [ctypes.c_int32, _i32, _u8, _d, _d, _d, _d, _d,
+        L.ow_integrate_forces.argtypes = [ctypes.c_int32, _i32, _u8, _d, ctypes.c_int32, _d, _d, _d, _d,
                                           ctypes.c_double, ctypes.c_double, _d, _d, ctypes.c_double]
+        L.ow_update_sleep.argtypes = [ctypes.c_int32, ctypes.c_int64, _i32, _i32, _i32, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_double, _d, _d, _u8, _d]
         L.ow_integrate_positions.argtypes = [ctypes.c_int32, _i32, _u8, _d, _d, _d, _d, ctypes.c_double]
         L.ow_solve3.argtypes = [_d, _d, _d]
         L.ow_snapshot.argtypes = [ctypes.c_int32, _d, _d, _u8, _i32, _d, _d, _u8, _u8]
@@ -163,6 +168,12 @@ class OracleWorld:
         keep['jb'] = _c(arrays['joint_b'] if nj else np.zeros(1), np.int32)
         keep['janc'] = _c(np.asarray(arrays['joint_anchor']).reshape(-1, 3) if nj else np.zeros((1, 3)), np.float64)
         d.joint_a = _p(keep['ja'], _i32); d.joint_b = _p(keep['jb'], _i32); d.joint_anchor = _p(keep['janc'], _d)
+        if nj and arrays.get('joint_fixed') is not None:
+            keep['jla'] = _c(np.asarray(arrays['joint_local_a']).reshape(-1, 3), np.float64)
+            keep['jlb'] = _c(np.asarray(arrays['joint_local_b']).reshape(-1, 3), np.float64)
+            keep['jfx'] = _c(arrays['joint_fixed'], np.uint8)
+            d.joint_local_a = _p(keep['jla'], _d); d.joint_local_b = _p(keep['jlb'], _d)
+            d.joint_fixed = _p(keep['jfx'], _u8)
         self._keep = keep
         self._h = L.ow_create(ctypes.byref(d))
         self.n = d.n
@@ -193,6 +204,27 @@ class OracleWorld:
     def set_velocity(self, lv: np.ndarray, av: np.ndarray) -> None:
         lv = _c(lv, np.float64); av = _c(av, np.float64)
         lib().ow_set_velocity(self._h, _p(lv, _d), _p(av, _d))
+
+    def set_params(self, inv_mass=None, inv_inertia=None, gravity_factor=None, linear_damping=None,
+                   angular_damping=None, quadratic_drag=None, gravity=None, default_damping=None,
+                   sleep_enabled=None, velocity_iterations=None) -> None:
+        """Edit per-body parameters and world settings between steps (None keeps a value)."""
+        arrs = [None if x is None else _c(x, np.float64) for x in (inv_mass, inv_inertia, gravity_factor,
+                                                                   linear_damping, angular_damping,
+                                                                   quadratic_drag, gravity, default_damping)]
+        self._keep['params'] = arrs
+        lib().ow_set_params(self._h, *[_p(a, _d) for a in arrs],
+                            -1 if sleep_enabled is None else int(bool(sleep_enabled)),
+                            -1 if velocity_iterations is None else int(velocity_iterations))
+
+    def set_state(self, position=None, orientation=None, linear_velocity=None, angular_velocity=None,
+                  awake=None, sleep_timer=None) -> None:
+        """Edit the mutable state between steps (None keeps a column)."""
+        arrs = [None if x is None else _c(x, np.float64) for x in (position, orientation, linear_velocity,
+                                                                   angular_velocity)]
+        aw = None if awake is None else _c(awake, np.uint8)
+        tm = None if sleep_timer is None else _c(sleep_timer, np.float64)
+        lib().ow_set_state(self._h, *[_p(a, _d) for a in arrs], _p(aw, _u8), _p(tm, _d))
 
     def contacts(self) -> Dict[str, np.ndarray]:
         """The last step's contacts in solver order, with their final impulses."""
@@ -333,6 +365,38 @@ def solve3(K, rhs) -> np.ndarray:
     K = _c(K, np.float64); rhs = _c(rhs, np.float64); x = np.zeros(3)
     lib().ow_solve3(_p(K, _d), _p(rhs, _d), _p(x, _d))
     return x
+
+
+def integrate_forces(motion_type, awake, gravity, gravity_factor, linear_damping, angular_damping,
+                     quadratic_drag, default_linear_damping, default_angular_damping, lv, av, dt) -> None:
+    """In place on ``lv, av`` (``(n, 3)``); ``gravity`` is ``(3,)`` or per body ``(n, 3)``."""
+    mt = _c(motion_type, np.int32); aw = _c(awake, np.uint8); n = len(mt)
+    g = _c(gravity, np.float64)
+    cols = [_c(x, np.float64) for x in (gravity_factor, linear_damping, angular_damping, quadratic_drag)]
+    for x in (lv, av):
+        assert x.flags['C_CONTIGUOUS'] and x.dtype == np.float64
+    lib().ow_integrate_forces(n, _p(mt, _i32), _p(aw, _u8), _p(g, _d), 3 if g.ndim == 2 else 0,
+                              *[_p(x, _d) for x in cols], float(default_linear_damping),
+                              float(default_angular_damping), _p(lv, _d), _p(av, _d), float(dt))
+
+
+def integrate_positions(motion_type, awake, lv, av, pos, quat, dt) -> None:
+    """In place on ``pos`` ``(n, 3)`` and ``quat`` ``(n, 4)``."""
+    mt = _c(motion_type, np.int32); aw = _c(awake, np.uint8); n = len(mt)
+    lv = _c(lv, np.float64); av = _c(av, np.float64)
+    for x in (pos, quat):
+        assert x.flags['C_CONTIGUOUS'] and x.dtype == np.float64
+    lib().ow_integrate_positions(n, _p(mt, _i32), _p(aw, _u8), _p(lv, _d), _p(av, _d), _p(pos, _d),
+                                 _p(quat, _d), float(dt))
+
+
+def update_sleep(ca, cb, motion_type, lv, av, awake, sleep_timer, dt, lin=0.08, ang=0.25, t_sleep=0.4) -> None:
+    """In place on ``lv, av, awake (uint8), sleep_timer`` (the step's contacts ``ca, cb``)."""
+    ca = _c(ca, np.int32); cb = _c(cb, np.int32); mt = _c(motion_type, np.int32)
+    for x, t in ((lv, np.float64), (av, np.float64), (awake, np.uint8), (sleep_timer, np.float64)):
+        assert x.flags['C_CONTIGUOUS'] and x.dtype == t
+    lib().ow_update_sleep(len(mt), len(ca), _p(ca, _i32), _p(cb, _i32), _p(mt, _i32), float(dt), float(lin),
+                          float(ang), float(t_sleep), _p(lv, _d), _p(av, _d), _p(awake, _u8), _p(sleep_timer, _d))
 
 
 def prepare_and_solve(a_idx, b_idx, point, normal, depth, pos, lv, av, inv_mass, invIw,

@@ -1074,14 +1074,15 @@ void ow_prepare_and_solve(int64_t K, const int64_t *ai, const int64_t *bi,
 /* --------------------------------------------------------- integration -- */
 
 /* NumpyBackend.integrate_forces (backend.py:26-40) over dyn & awake bodies. */
-void ow_integrate_forces(int32_t n, const int32_t *mt, const uint8_t *awake, const double *g,
+void ow_integrate_forces(int32_t n, const int32_t *mt, const uint8_t *awake, const double *g, int32_t g_stride,
                          const double *gf, const double *ld, const double *ad, const double *qd,
                          double dlin, double dang, double *lv, double *av, double dt)
 {
     for (int32_t i = 0; i < n; i++) {
         if (!(mt[i] == M_DYNAMIC && awake[i])) continue;
         double *v = lv + 3 * i, *w = av + 3 * i;
-        for (int r = 0; r < 3; r++) v[r] = v[r] + (g[r] * gf[i]) * dt;
+        const double *gi = g + (size_t)g_stride * i;   /* stride 0: one vector for every body */
+        for (int r = 0; r < 3; r++) v[r] = v[r] + (gi[r] * gf[i]) * dt;
         double fl = dmax(1.0 - (ld[i] + dlin) * dt, 0.0);
         for (int r = 0; r < 3; r++) v[r] = v[r] * fl;
         double sp = sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
@@ -1196,6 +1197,17 @@ OWorld *ow_create(const ow_desc *d)
         w->joints[j].a = d->joint_a[j];
         w->joints[j].b = d->joint_b[j];
         for (int r = 0; r < 3; r++) w->joints[j].anchor[r] = d->joint_anchor[3 * j + r];
+        /* PointConstraint(local_a=..., local_b=...): a given anchor is used from
+         * the first prepare on (joints.py:93-96) */
+        const uint8_t fx = d->joint_fixed ? d->joint_fixed[j] : 0;
+        if ((fx & 1) && d->joint_local_a) {
+            w->joints[j].has_la = 1;
+            for (int r = 0; r < 3; r++) w->joints[j].la[r] = d->joint_local_a[3 * j + r];
+        }
+        if ((fx & 2) && d->joint_local_b) {
+            w->joints[j].has_lb = 1;
+            for (int r = 0; r < 3; r++) w->joints[j].lb[r] = d->joint_local_b[3 * j + r];
+        }
     }
     w->pair_cap = 1024;
     w->pairs = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)w->pair_cap);
@@ -1229,6 +1241,37 @@ void ow_get_state(const OWorld *w, double *pos, double *quat, double *lv, double
     if (timer) memcpy(timer, w->timer, sizeof(double) * n);
     if (lo) memcpy(lo, w->lo, sizeof(double) * 3 * n);
     if (hi) memcpy(hi, w->hi, sizeof(double) * 3 * n);
+}
+
+/* Host edits between steps: any pointer NULL keeps the value; sleep_enabled
+ * and iters -1 keep theirs. */
+void ow_set_params(OWorld *w, const double *inv_mass, const double *inv_inertia, const double *gf,
+                   const double *ld, const double *ad, const double *qd, const double *g3, const double *damp2,
+                   int32_t sleep_enabled, int32_t iters)
+{
+    const size_t n = (size_t)w->n;
+    if (inv_mass) memcpy(w->inv_mass, inv_mass, sizeof(double) * n);
+    if (inv_inertia) memcpy(w->inv_inertia, inv_inertia, sizeof(double) * 3 * n);
+    if (gf) memcpy(w->gf, gf, sizeof(double) * n);
+    if (ld) memcpy(w->ld, ld, sizeof(double) * n);
+    if (ad) memcpy(w->ad, ad, sizeof(double) * n);
+    if (qd) memcpy(w->qd, qd, sizeof(double) * n);
+    if (g3) for (int r = 0; r < 3; r++) w->g[r] = g3[r];
+    if (damp2) { w->dlin = damp2[0]; w->dang = damp2[1]; }
+    if (sleep_enabled >= 0) w->sleep_enabled = sleep_enabled;
+    if (iters >= 0) w->iters = iters;
+}
+
+void ow_set_state(OWorld *w, const double *pos, const double *quat, const double *lv, const double *av,
+                  const uint8_t *awake, const double *timer)
+{
+    const size_t n = (size_t)w->n;
+    if (pos) memcpy(w->pos, pos, sizeof(double) * 3 * n);
+    if (quat) memcpy(w->quat, quat, sizeof(double) * 4 * n);
+    if (lv) memcpy(w->lv, lv, sizeof(double) * 3 * n);
+    if (av) memcpy(w->av, av, sizeof(double) * 3 * n);
+    if (awake) memcpy(w->awake, awake, n);
+    if (timer) memcpy(w->timer, timer, sizeof(double) * n);
 }
 
 void ow_set_velocity(OWorld *w, const double *lv, const double *av)
@@ -1558,10 +1601,64 @@ static void solve_joints(OWorld *w, double dt)
 /* PhysicsWorld.step (world.py:410-430) with _CollisionStages.run (:658-676),
  * SequentialImpulseSolver.solve (solver.py:192-213) and _update_sleep
  * (world.py:510-567). */
+/* PhysicsWorld._update_sleep + _contact_islands (world.py:510-567) over n
+ * bodies and the step's K contacts: slow = dynamic with |v|^2 < lin^2 and
+ * |w|^2 < ang^2 (sums left to right); the timer grows by dt while slow and
+ * resets otherwise; ready = slow and timer > t_sleep; dynamic-dynamic contacts
+ * join islands; an island sleeps when every member is ready, a ready dynamic
+ * body in no contact sleeps alone; moving dynamic bodies wake; sleeping dynamic
+ * bodies get zero velocity. */
+void ow_update_sleep(int32_t n, int64_t K, const int32_t *ca, const int32_t *cb, const int32_t *mt, double dt,
+                     double lin, double ang, double t_sleep, double *lv, double *av, uint8_t *awake, double *timer)
+{
+    uint8_t *slow = (uint8_t *)calloc((size_t)n + 1, 1), *ready = (uint8_t *)calloc((size_t)n + 1, 1);
+    uint8_t *contacted = (uint8_t *)calloc((size_t)n + 1, 1), *notready = (uint8_t *)calloc((size_t)n + 1, 1);
+    int32_t *par = (int32_t *)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    for (int32_t i = 0; i < n; i++) {
+        int dyn = mt[i] == M_DYNAMIC;
+        double s2 = (lv[3 * i] * lv[3 * i] + lv[3 * i + 1] * lv[3 * i + 1]) + lv[3 * i + 2] * lv[3 * i + 2];
+        double w2 = (av[3 * i] * av[3 * i] + av[3 * i + 1] * av[3 * i + 1]) + av[3 * i + 2] * av[3 * i + 2];
+        slow[i] = dyn && s2 < lin * lin && w2 < ang * ang;
+        if (slow[i]) timer[i] += dt; else timer[i] = 0.0;
+        ready[i] = slow[i] && timer[i] > t_sleep;
+        par[i] = i;
+    }
+    for (int64_t k = 0; k < K; k++) {
+        int32_t a = ca[k], b = cb[k];
+        int da = mt[a] == M_DYNAMIC, db = mt[b] == M_DYNAMIC;
+        if (da) contacted[a] = 1;
+        if (db) contacted[b] = 1;
+        if (da && db) {
+            int32_t x = a, y = b;
+            while (par[x] != x) x = par[x];
+            while (par[y] != y) y = par[y];
+            if (x != y) par[x] = y;
+        }
+    }
+    for (int32_t i = 0; i < n; i++) {
+        if (!contacted[i]) continue;
+        int32_t r = i;
+        while (par[r] != r) r = par[r];
+        par[i] = r;
+    }
+    for (int32_t i = 0; i < n; i++) if (contacted[i] && !ready[i]) notready[par[i]] = 1;
+    for (int32_t i = 0; i < n; i++) {
+        int dyn = mt[i] == M_DYNAMIC;
+        int to_sleep = contacted[i] ? !notready[par[i]] : (dyn && ready[i]);
+        if (to_sleep) awake[i] = 0;
+        if (dyn && !slow[i]) awake[i] = 1;
+        if (!awake[i] && dyn) {
+            lv[3 * i] = lv[3 * i + 1] = lv[3 * i + 2] = 0.0;
+            av[3 * i] = av[3 * i + 1] = av[3 * i + 2] = 0.0;
+        }
+    }
+    free(slow); free(ready); free(contacted); free(notready); free(par);
+}
+
 void ow_step(OWorld *w, double dt)
 {
     int32_t n = w->n;
-    ow_integrate_forces(n, w->mt, w->awake, w->g, w->gf, w->ld, w->ad, w->qd, w->dlin, w->dang, w->lv, w->av, dt);
+    ow_integrate_forces(n, w->mt, w->awake, w->g, 0, w->gf, w->ld, w->ad, w->qd, w->dlin, w->dang, w->lv, w->av, dt);
     w->K = 0;
     w->npairs = w->nsolid = w->xcand = w->nmanifold = 0;
     if (w->has_collision) {
@@ -1631,49 +1728,6 @@ void ow_step(OWorld *w, double dt)
     }
     if (w->nj > 0) solve_joints(w, dt);
     ow_integrate_positions(n, w->mt, w->awake, w->lv, w->av, w->pos, w->quat, dt);
-    if (w->sleep_enabled) {
-        /* _update_sleep + _contact_islands (world.py:510-567) */
-        uint8_t *slow = (uint8_t *)calloc((size_t)n + 1, 1), *ready = (uint8_t *)calloc((size_t)n + 1, 1);
-        uint8_t *contacted = (uint8_t *)calloc((size_t)n + 1, 1), *notready = (uint8_t *)calloc((size_t)n + 1, 1);
-        int32_t *par = (int32_t *)malloc(sizeof(int32_t) * ((size_t)n + 1));
-        for (int32_t i = 0; i < n; i++) {
-            int dyn = w->mt[i] == M_DYNAMIC;
-            double s2 = (w->lv[3 * i] * w->lv[3 * i] + w->lv[3 * i + 1] * w->lv[3 * i + 1]) + w->lv[3 * i + 2] * w->lv[3 * i + 2];
-            double w2 = (w->av[3 * i] * w->av[3 * i] + w->av[3 * i + 1] * w->av[3 * i + 1]) + w->av[3 * i + 2] * w->av[3 * i + 2];
-            slow[i] = dyn && s2 < 0.08 * 0.08 && w2 < 0.25 * 0.25;
-            if (slow[i]) w->timer[i] += dt; else w->timer[i] = 0.0;
-            ready[i] = slow[i] && w->timer[i] > 0.4;
-            par[i] = i;
-        }
-        for (int64_t k = 0; k < w->K; k++) {
-            int32_t a = w->ca[k], b = w->cb[k];
-            int da = w->mt[a] == M_DYNAMIC, db = w->mt[b] == M_DYNAMIC;
-            if (da) contacted[a] = 1;
-            if (db) contacted[b] = 1;
-            if (da && db) {
-                int32_t x = a, y = b;
-                while (par[x] != x) x = par[x];
-                while (par[y] != y) y = par[y];
-                if (x != y) par[x] = y;
-            }
-        }
-        for (int32_t i = 0; i < n; i++) {
-            if (!contacted[i]) continue;
-            int32_t r = i;
-            while (par[r] != r) r = par[r];
-            par[i] = r;
-        }
-        for (int32_t i = 0; i < n; i++) if (contacted[i] && !ready[i]) notready[par[i]] = 1;
-        for (int32_t i = 0; i < n; i++) {
-            int dyn = w->mt[i] == M_DYNAMIC;
-            int to_sleep = contacted[i] ? !notready[par[i]] : (dyn && ready[i]);
-            if (to_sleep) w->awake[i] = 0;
-            if (dyn && !slow[i]) w->awake[i] = 1;
-            if (!w->awake[i] && dyn) {
-                w->lv[3 * i] = w->lv[3 * i + 1] = w->lv[3 * i + 2] = 0.0;
-                w->av[3 * i] = w->av[3 * i + 1] = w->av[3 * i + 2] = 0.0;
-            }
-        }
-        free(slow); free(ready); free(contacted); free(notready); free(par);
-    }
+    if (w->sleep_enabled)
+        ow_update_sleep(n, w->K, w->ca, w->cb, w->mt, dt, 0.08, 0.25, 0.4, w->lv, w->av, w->awake, w->timer);
 }

@@ -53,6 +53,10 @@ typedef struct {
     int32_t n_joints, joint_iterations;
     const int32_t *joint_a, *joint_b;
     const double *joint_anchor;
+    /* body-frame anchors given at construction (joints.py:66-73), [J][3]: bit
+     * 0 / 1 of joint_fixed[j] marks local a / b as given; NULL = none given */
+    const double *joint_local_a, *joint_local_b;
+    const uint8_t *joint_fixed;
 } ow_desc;
 
 typedef struct OWorld OWorld;
@@ -63,6 +67,11 @@ void ow_step(OWorld *w, double dt);
 int32_t ow_body_count(const OWorld *w);
 void ow_get_state(const OWorld *w, double *pos, double *quat, double *lv, double *av,
                   uint8_t *awake, double *sleep_timer, double *lo, double *hi);
+void ow_set_params(OWorld *w, const double *inv_mass, const double *inv_inertia, const double *gf,
+                   const double *ld, const double *ad, const double *qd, const double *g3, const double *damp2,
+                   int32_t sleep_enabled, int32_t iters);
+void ow_set_state(OWorld *w, const double *pos, const double *quat, const double *lv, const double *av,
+                  const uint8_t *awake, const double *timer);
 void ow_set_velocity(OWorld *w, const double *lv, const double *av);
 int32_t ow_num_contacts(const OWorld *w);
 void ow_get_contacts(const OWorld *w, int32_t *a, int32_t *b, double *point, double *normal,
@@ -109,8 +118,10 @@ void ow_prepare_and_solve(int64_t K, const int64_t *a_idx, const int64_t *b_idx,
                           const double *invIw, const double *restitution, const double *friction,
                           double *nImp, double *tImp, double restitution_threshold,
                           int32_t iters, int32_t warm_start, double correction, double slop);
+/* NumpyBackend.integrate_forces (backend.py:26-40); gravity_stride 0 applies
+ * gravity[3] to every body, 3 reads per-body rows [n][3]. */
 void ow_integrate_forces(int32_t n, const int32_t *motion_type, const uint8_t *awake,
-                         const double *gravity, const double *gravity_factor,
+                         const double *gravity, int32_t gravity_stride, const double *gravity_factor,
                          const double *linear_damping, const double *angular_damping,
                          const double *quadratic_drag, double dlin, double dang,
                          double *lv, double *av, double dt);
@@ -120,6 +131,9 @@ void ow_solve3(const double *K, const double *rhs, double *x);
 void ow_integrate_positions(int32_t n, const int32_t *motion_type, const uint8_t *awake,
                             const double *lv, const double *av, double *pos, double *quat,
                             double dt);
+void ow_update_sleep(int32_t n, int64_t K, const int32_t *ca, const int32_t *cb, const int32_t *motion_type,
+                     double dt, double lin_thresh, double ang_thresh, double t_sleep, double *lv, double *av,
+                     uint8_t *awake, double *sleep_timer);
 
 #ifdef __cplusplus
 }

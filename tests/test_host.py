@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_binding_loads_without_gpu():
     L = _ffi.load()
-    assert L.omi_abi_version() == 1
+    assert L.omi_abi_version() == 2
     from omi_physics_amd.batch import KERNELS
     assert tuple(L.omi_batch_kernel_names().decode().split(',')) == KERNELS
 
@@ -132,3 +132,52 @@ def test_hull_vertex_order_matches_reference(golden):
         for t in tris:
             n = np.cross(v[t[1]] - v[t[0]], v[t[2]] - v[t[0]])
             assert np.dot(n, c - v[t[0]]) < 0
+
+
+def test_layout_of_a_host_column_world_equals_the_resident_world():
+    """The C-ABI layout built from a reference-shaped host world (tests/seam_world.py)
+    equals the one from PhysicsWorld and the vectorised scenes.arrays, for every
+    column and table (trigger bodies and both kinds of shapes included)."""
+    from omi_physics_amd.layout import world_arrays
+    from tests import helpers
+    from tests.seam_world import SeamWorld
+    for spec in (helpers.trigger_scene(40, seed=2), helpers.pile(60, seed=4)):
+        a = scenes.build(spec, PhysicsWorld, model).export_arrays()
+        b = world_arrays(scenes.build(spec, SeamWorld, model), velocity_iterations=10)
+        assert set(a) == set(b)
+        for key, val in a.items():
+            if isinstance(val, np.ndarray):
+                assert np.array_equal(val, b[key]), key
+            else:
+                assert val == b[key], key
+
+
+def test_resident_world_capacity_error_poisons_until_recovered():
+    """A capacity error found when the mirror refreshes keeps the mirror's last good
+    state, keeps the device marked newer, and fails every later read and step with the
+    same message until recover_device(); then the next step reloads."""
+    from omi_physics_amd.backend import DeviceWorld
+
+    class FullBatch:
+        def sync(self):
+            raise RuntimeError('omi_hip error -3: batch capacity exceeded in a step since the last sync')
+
+        def close(self):
+            pass
+
+    w = PhysicsWorld()
+    w.add_body(model.Motion(), position=(1.0, 2.0, 3.0))
+    dev = DeviceWorld('fp64', with_joints=True)
+    dev.batch, dev.n, dev.key = FullBatch(), 1, ('layout',)
+    w._hip_device = dev
+    w._device_fresh = True
+    for _ in range(2):
+        with pytest.raises(RuntimeError, match='capacity'):
+            w.position
+        assert w._device_fresh and dev.error is not None
+    with pytest.raises(RuntimeError, match='recover_device'):
+        w.step(1.0 / 60.0)
+    assert np.array_equal(w._raw('position')[0], [1.0, 2.0, 3.0])
+    w.recover_device()
+    assert dev.error is None and dev.batch is None and dev.grow == 2 and not w._device_fresh
+    assert 'layout' in w._edited

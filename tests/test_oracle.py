@@ -554,3 +554,26 @@ def test_kinematic_wakes_and_pushes_a_sleeper():
     assert S['position'][1, 0] > 0.5                   # pushed along +x
     assert S['awake'][2] == 0 and np.all(S['linear_velocity'][2] == 0)
     np.testing.assert_allclose(S['position'][3, 0], -3.0 + 199 / 60.0, atol=1e-12)
+
+
+def test_update_sleep_rule():
+    """ow_update_sleep against the island rule on a hand-built case: bodies 0-2 form an
+    island with body 2 not yet ready, so none sleeps; bodies 3-4 form a ready island and
+    both sleep (velocities zeroed); ready loner 5 sleeps; slow-but-young loner 6 keeps
+    its timer running; fast body 7 wakes and resets; static 8 and kinematic 9 touch
+    the first island and never sleep or count; timers add dt exactly."""
+    n = 10
+    mt = np.array([2, 2, 2, 2, 2, 2, 2, 2, 0, 1], np.int32)
+    lv = np.zeros((n, 3)); av = np.zeros((n, 3))
+    lv[:, 0] = 0.01; lv[7] = (0.5, 0, 0); av[4] = (0, 0.2, 0)
+    awake = np.ones(n, np.uint8); awake[7] = 0
+    timer = np.array([0.5, 0.45, 0.1, 0.9, 0.41, 0.5, 0.2, 3.0, 0.0, 0.0])
+    ca = np.array([0, 1, 3, 0, 2], np.int32)
+    cb = np.array([1, 2, 4, 8, 9], np.int32)
+    dt = 1.0 / 60.0
+    t0 = timer.copy()
+    oracle.update_sleep(ca, cb, mt, lv, av, awake, timer, dt)
+    assert list(awake) == [1, 1, 1, 0, 0, 0, 1, 1, 1, 1]
+    slow = [0, 1, 2, 3, 4, 5, 6]
+    assert np.array_equal(timer[slow], t0[slow] + dt) and timer[7] == 0.0
+    assert not lv[3:6].any() and not av[3:6].any() and lv[0, 0] == 0.01
