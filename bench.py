@@ -1,24 +1,28 @@
 """Benchmark: body-steps/s of the batched contact pipeline on one MI355X.
 
-Workload (default): 4096 independent worlds, each a static ground box
+Workload (the headline): 4096 independent worlds, each a static ground box
 (half extents 10 x 0.5 x 10) plus n_w dynamic bodies, n_w =
-clip(round(lognormal(ln 64, 1.0)), 8, 2048) (seed 3, ~437k bodies), a 50/50
-mix of unit boxes and r = 0.5 spheres of mass 1 dropped from y in [2, 50]
-over x, z in [-10, 10]; mu = 0.5, e = 0, dt = 1/60, 10 solver iterations,
-sleeping on.  One "step" advances every world by dt; the timed region is steps
-1..K of the scene from its initial state (a separate batch takes the warm-up
-steps), inputs resident in HBM, one graph launch per step.
+clip(round(lognormal(ln 64, 1.0)), 8, 2048) (seed 3, 436778 bodies), a 50/50
+mix of unit boxes and r = 0.5 spheres of mass 1 dropped from y in [2, 50] over
+x, z in [-10, 10]; mu = 0.5, e = 0, dt = 1/60, 10 solver iterations, sleeping
+on; a 300-step rollout.  All worlds are one batch stepped by one graph launch
+per step, FP64 (bit-identical to the oracle).
 
-Printed: one JSON line.  ``value`` is the batched FP64 form, which is
-bit-identical to the FP64 oracle (the reference's algorithm) and so meets the
-300-step position-error bound exactly; the batched FP32 form (FP32
-narrowphase and solver) is reported beside it with its errors against FP64
-(after one step from the same state, and after 300 steps, next to the
-divergence that a 1-ulp change of the initial positions causes in FP64 alone
--- the piles are chaotic).  Also: the plain one-world-at-a-time GPU form,
-p50/p99 step latency, per-stage device time, the roofline of the dominant
-kernel, and the CPU baseline (the oracle, a C restatement of the reference
-algorithm, one world per thread on this host's cores).
+Timed window: the last ``--steps`` K steps of the rollout.  The batch is
+advanced untimed through steps 1..300-K-W, then W warm-up steps, then K steps
+are timed (a device synchronisation on both sides; each step is also synced to
+record its latency), and the counters of every step are read, so the line
+states the window's contacts per step next to the whole rollout's.  With the
+driver's K = 20, W = 5 the window is steps 281..300, after the piles have
+settled (contacts per step above the rollout's mean).
+
+Printed: one JSON line (details on stderr).  Beside ``value``: the batched FP32
+form on the same window, per-stage device time and the roofline of the
+dominant kernel and of the whole step from the same window, the plain
+one-world-at-a-time GPU form, the CPU baseline (the oracle on every world over
+the same window), parity of all 4096 worlds with the oracle at the window's
+end, FP32 error curves against FP64 beside the FP64 one-ulp control, a
+continuous-batching serving run, and the single-world workloads.
 """
 import argparse
 import json
@@ -32,9 +36,12 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
-METRIC = 'body-steps/s at 1 GPU (batched FP32 vs plain) + p99 step latency, err≤1e-4m'
+METRIC = 'body-steps/s at 1 GPU (batched vs plain) + p99 step latency'
 DT = 1.0 / 60.0
+ROLLOUT = 300
 HBM_PEAK_GBS = 8000.0
+# worker threads for the CPU legs: the host cores given to one GPU
+CPU_THREADS = min(len(os.sched_getaffinity(0)), 16)
 
 
 def log(*a):
@@ -51,19 +58,29 @@ def make_worlds(n_worlds, seed):
     return arrays
 
 
-def algorithmic_bytes(kernel, c, r):
-    """HBM bytes one launch of ``kernel`` must move, from the step's counters.
+# ----------------------------------------------------------- algorithmic bytes
 
-    Per-unit figures (r = bytes of the narrowphase/solver float):
-      pre     350 + 18r per body (pose, velocity and parameters in; previous pose,
+def step_bytes(c, bodies_in_contact, r):
+    """HBM bytes one whole step must move (batched form, solver float r bytes):
+    376 per body (refit 120, integrate 180, sleep 44, sort keys 32), 32 per
+    x-window candidate, 152 per solid pair, 16 per manifold, 80 per contact, 144
+    per body in a contacting island; the FP64 form adds 24 per body and 16 per
+    contact (FP64 velocities and impulses)."""
+    N, X, P, M, C = c['bodies'], c['x_candidates'], c['solid_pairs'], c['manifolds'], c['contacts']
+    extra = (24 * N + 16 * C) if r == 8 else 0
+    return 376 * N + 32 * X + 152 * P + 16 * M + 80 * C + 144 * bodies_in_contact + extra
+
+
+def stage_bytes(kernel, c, r):
+    """HBM bytes one launch of a stage must move, from the step's counters:
+      pre     350 + 18r per body (pose, velocity, parameters in; previous pose,
               velocity, AABB, rotation and world inverse inertia out)
       sap     16 per body (lo.x, hi.x) + 32 per x-window candidate (y/z bounds)
               + 8 per solid pair written
-      narrow  106 + 18r per solid pair (indices, kinds, poses, rotations,
-              half extents) + 24 + r per contact point written
+      narrow  106 + 18r per solid pair (indices, kinds, poses, rotations, half
+              extents) + 24 + r per contact point written
       solve   32 + 7r per contact (indices, point, normal, depth, impulses)
-              + 152 + 9r per body touched (velocity, pose, mass and inertia in
-              and out), bodies touched taken as min(bodies, 2 x manifolds)
+              + 152 + 9r per body touched, taken as min(bodies, 2 x manifolds)
     """
     N, X, P, M, C = c['bodies'], c['x_candidates'], c['solid_pairs'], c['manifolds'], c['contacts']
     if kernel == 'pre':
@@ -78,13 +95,11 @@ def algorithmic_bytes(kernel, c, r):
 
 
 def solve_top_bytes(stats, r):
-    """Algorithmic bytes of one solver launch of the largest size class present.
-
-    Per contact: its solver row (224 B, both precisions) in and its three
-    impulses (3r) out; per manifold its 16-byte record; per body in contact
-    (taken as min(bodies, 2 x manifolds) per world): FP64 velocity in and out
-    (96 B), FP64 position in and out (48 B) and world inverse inertia (12r).
-    """
+    """Algorithmic bytes of the solver launch of the largest size class present:
+    per contact its 224-byte row in and three impulses (3r) out, per manifold its
+    16-byte record, per body in contact (min(bodies, 2 x manifolds) per world) FP64
+    velocity in and out (96), FP64 position in and out (48) and world inverse
+    inertia (12r)."""
     top = int(stats['size_class'].max())
     sel = stats['size_class'] == top
     k = int(stats['contacts'][sel].sum())
@@ -93,386 +108,445 @@ def solve_top_bytes(stats, r):
     return k * (224 + 3 * r) + 16 * m + nd * (96 + 48 + 12 * r), top, int(sel.sum())
 
 
-def run_batched(arrays, precision, steps, warmup, timing=True, lanes=False):
-    """Warm up on one batch, then time ``steps`` steps of a fresh batch from the initial state.
+def bodies_in_contact(stats):
+    return int(np.minimum(stats['bodies'], 2 * stats['manifolds']).sum())
 
-    With ``timing`` the steps launch eagerly with a HIP event between kernels on
-    each lane's stream (events cannot be timed inside a captured graph); without
-    it every step is one graph launch.  Stage times, counters and algorithmic
-    bytes come back per lane (one lane for a single batch).
-    """
+
+# ------------------------------------------------------------------ rollouts
+
+def window_bounds(steps, warmup):
+    """(first step of the warm-up, first timed step, last timed step, rollout length)."""
+    first = max(1, ROLLOUT - steps - warmup + 1)
+    t0 = first + warmup
+    return first, t0, t0 + steps - 1, max(ROLLOUT, t0 + steps - 1)
+
+
+def rollout(arrays, precision, steps, warmup, timing=False, record=True):
+    """Step one batch of ``arrays`` through the rollout; time the window.
+
+    Returns the batch (state at the window's end) and a dict: window wall time,
+    per-step latencies of the window and of the rollout, counters of every step,
+    and with ``timing`` the per-stage device time and algorithmic bytes of the
+    window (its steps launched eagerly with HIP events between the stages; every
+    other step is one graph launch)."""
     from omi_physics_amd.batch import batch_from_arrays
-    if warmup > 0:
-        w = batch_from_arrays(arrays, precision=precision, lanes=lanes)
-        w.prepare(DT)
-        w.step(DT, warmup)
-        w.sync()
-        w.close()
-    t0 = time.perf_counter()
-    b = batch_from_arrays(arrays, precision=precision, lanes=lanes)
-    log('%s batch loaded: %.1f GB device, %.1f s' % (precision, b.device_bytes() / 1e9, time.perf_counter() - t0))
-    if timing:
-        b.set_timing(True)
+    first, t0, t1, total = window_bounds(steps, warmup)
+    tl = time.perf_counter()
+    b = batch_from_arrays(arrays, precision=precision)
+    log('%s batch loaded: %.1f GB device, %.1f s' % (precision, b.device_bytes() / 1e9, time.perf_counter() - tl))
     b.prepare(DT)
     b.sync()
-    lat = []
-    lanes = b.lanes if hasattr(b, 'lanes') else [b]
-    ktot = [{} for _ in lanes]
-    ctot = [{} for _ in lanes]
-    per_kernel_bytes = [{} for _ in lanes]
     r = 4 if precision == 'fp32' else 8
-    t_start = time.perf_counter()
-    for s in range(steps):
+    lat, cnt = [], []
+    kt, kb = {}, {}
+    wall = 0.0
+    for s in range(1, t1 + 1):
+        if s == t0:
+            if timing:
+                b.set_timing(True)
+            b.sync()
+            tw = time.perf_counter()
         t = time.perf_counter()
         b.step(DT, 1)
         b.sync()
         lat.append(time.perf_counter() - t)
-        if timing:
-            for li, lb in enumerate(lanes):
-                kt = lb.kernel_times()
-                c = lb.counters()
-                for k, v in kt.items():
-                    ktot[li][k] = ktot[li].get(k, 0.0) + v
-                for k, v in c.items():
-                    ctot[li][k] = ctot[li].get(k, 0) + v
-                kb = per_kernel_bytes[li]
-                for k in ('pre', 'sap', 'narrow', 'solve'):
-                    kb[k] = kb.get(k, 0) + algorithmic_bytes(k, c, r)
-                tb, top, ntop = solve_top_bytes(lb.world_stats(), r)
-                kb['solve_top'] = kb.get('solve_top', 0) + tb
-                kb['top_class'] = (top, ntop)
-        if (s + 1) % 50 == 0:
-            log('  %s step %d/%d  %.2f ms' % (precision, s + 1, steps, 1e3 * lat[-1]))
-    wall = time.perf_counter() - t_start
-    return b, np.array(lat), wall, ktot, ctot, per_kernel_bytes
+        if record:
+            cnt.append(b.counters())
+        if timing and s >= t0:
+            for k, v in b.kernel_times().items():
+                kt[k] = kt.get(k, 0.0) + v
+            c = cnt[-1]
+            st = b.world_stats()
+            for k in ('pre', 'sap', 'narrow', 'solve'):
+                kb[k] = kb.get(k, 0) + stage_bytes(k, c, r)
+            tb, top, ntop = solve_top_bytes(st, r)
+            kb['solve_top'] = kb.get('solve_top', 0) + tb
+            kb['top_class'] = (top, ntop)
+            kb['step'] = kb.get('step', 0) + step_bytes(c, bodies_in_contact(st), r)
+        if s == t1:
+            wall = time.perf_counter() - tw
+        if s % 50 == 0:
+            log('  %s step %d/%d  %.2f ms' % (precision, s, total, 1e3 * lat[-1]))
+    if timing:
+        b.set_timing(False)
+    return b, dict(wall=wall, lat=np.array(lat), window_lat=np.array(lat[t0 - 1:t1]), counters=cnt,
+                   kernel_ms=kt, kernel_bytes=kb, bounds=(first, t0, t1, total))
 
 
-def run_plain(arrays, steps, precision):
-    """The plain form: each sampled world alone on the GPU, one world at a time."""
+def finish_rollout(b, info, steps_done):
+    """Step the batch untimed to the end of the rollout, recording counters and latency."""
+    first, t0, t1, total = info['bounds']
+    for s in range(steps_done + 1, total + 1):
+        t = time.perf_counter()
+        b.step(DT, 1)
+        b.sync()
+        info['lat'] = np.append(info['lat'], time.perf_counter() - t)
+        info['counters'].append(b.counters())
+
+
+def mean_counter(cs, key):
+    return float(np.mean([c[key] for c in cs])) if cs else 0.0
+
+
+# ----------------------------------------------------------------- CPU legs
+
+def oracle_window(arrays, steps, warmup, threads):
+    """The oracle on every world through the same window: each world advanced
+    untimed to the window, then the window's steps timed over all worlds."""
+    import oracle
+    first, t0, t1, _ = window_bounds(steps, warmup)
+    worlds = [oracle.OracleWorld(a) for a in arrays]
+
+    def advance(ow):
+        ow.step(DT, t0 - 1)
+
+    def window(ow):
+        ow.step(DT, steps)
+    ta = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(advance, worlds))
+    tw = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(window, worlds))
+    wall = time.perf_counter() - tw
+    log('oracle: %d worlds advanced %d steps in %.1f s, window of %d steps in %.2f s on %d threads'
+        % (len(worlds), t0 - 1, tw - ta, steps, wall, threads))
+    return worlds, wall
+
+
+def error_curves(arrays, checkpoints, threads):
+    """Position error against the FP64 oracle at each checkpoint step (median,
+    90th percentile and largest over worlds of each world's largest error): the
+    batched FP32 form, and the FP64 oracle started from positions moved by one ulp
+    (how far the piles carry a last-bit change on their own)."""
+    import oracle
     from omi_physics_amd.batch import batch_from_arrays
-    body_steps = 0
-    total = 0.0
-    lat = []
+
+    def nudged(a):
+        a = dict(a)
+        p = a['position'].copy()
+        dyn = a['motion_type'] == 2
+        p[dyn] = np.nextafter(p[dyn], np.inf)
+        a['position'] = p
+        return a
+
+    def trace(a):
+        ow = oracle.OracleWorld(a)
+        out, s = [], 0
+        for c in checkpoints:
+            ow.step(DT, c - s)
+            s = c
+            out.append(ow.state()['position'])
+        return out
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        ref = list(ex.map(trace, arrays))
+        ulp = list(ex.map(trace, [nudged(a) for a in arrays]))
+    b = batch_from_arrays(arrays, precision='fp32')
+    f32, s = [], 0
+    for c in checkpoints:
+        b.step(DT, c - s)
+        b.sync()
+        s = c
+        f32.append([p for p, _, _, _ in b.snapshot()])
+    b.close()
+
+    def stats(errs):
+        e = np.asarray(errs)
+        return {'median_m': float(np.median(e)), 'p90_m': float(np.percentile(e, 90)), 'max_m': float(e.max()),
+                'worlds_within_1e-4': float(np.mean(e <= 1e-4))}
+    curve = {}
+    for k, c in enumerate(checkpoints):
+        e32 = [np.abs(f32[k][w] - ref[w][k]).max() for w in range(len(arrays))]
+        eul = [np.abs(ulp[w][k] - ref[w][k]).max() for w in range(len(arrays))]
+        curve[str(c)] = {'fp32': stats(e32), 'fp64_1ulp': stats(eul)}
+    return curve
+
+
+# -------------------------------------------------------------- other forms
+
+def run_plain(arrays, steps, warmup, precision):
+    """The plain form: each sampled world alone on the GPU, one world at a time,
+    over the same window of its rollout."""
+    from omi_physics_amd.batch import batch_from_arrays
+    first, t0, t1, _ = window_bounds(steps, warmup)
+    body_steps, total, lat = 0, 0.0, []
     for a in arrays:
         b = batch_from_arrays([a], precision=precision)
         b.prepare(DT)
+        b.step(DT, t0 - 1)
         b.sync()
-        t0 = time.perf_counter()
+        ts = time.perf_counter()
         for _ in range(steps):
             t = time.perf_counter()
             b.step(DT, 1)
             b.sync()
             lat.append(time.perf_counter() - t)
-        total += time.perf_counter() - t0
+        total += time.perf_counter() - ts
         body_steps += a['n'] * steps
         b.close()
     return body_steps / total, np.array(lat)
 
 
-def cpu_baseline(arrays, steps, threads):
-    """The oracle (C, FP64, one world per thread) on a sample of the worlds."""
-    import oracle
+def serving_run(rate, n_requests, precision, seed=5):
+    """Continuous batching of Poisson-arriving rollouts (8..4096 bodies, 60 steps)
+    against the same requests served one at a time."""
+    from omi_physics_amd import scenes
+    from omi_physics_amd.serve import Request, RolloutServer, run_plain as serve_plain
 
-    def one(a):
-        ow = oracle.OracleWorld(a)
-        ow.step(DT, steps)
-        return ow.state()['position']
-
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(max_workers=threads) as ex:
-        pos = list(ex.map(one, arrays))
-    wall = time.perf_counter() - t0
-    return sum(a['n'] for a in arrays) * steps / wall, wall, pos
-
-
-def single_world(name, arrays, precision, steps, warmup, cpu):
-    """One world stepped alone on the GPU (the plain form), vs the oracle on one core."""
-    from omi_physics_amd.batch import batch_from_arrays
-    n = int(arrays['n'])
-    if warmup:
-        w = batch_from_arrays([arrays], precision=precision)
-        w.step(DT, warmup); w.sync(); w.close()
-    b = batch_from_arrays([arrays], precision=precision)
-    b.prepare(DT); b.sync()
-    lat = []
-    cmax = 0
-    t0 = time.perf_counter()
-    for s in range(steps):
-        t = time.perf_counter()
-        b.step(DT, 1); b.sync()
-        lat.append(time.perf_counter() - t)
-        cmax = max(cmax, b.counters()['contacts'])
-    wall = time.perf_counter() - t0
-    pos = b.read(0)['position']
-    b.close()
-    out = {'metric': METRIC, 'value': n * steps / wall, 'unit': 'body-steps/s', 'n_gpus': 1, 'steps': steps,
-           'warmup': warmup, 'ms_per_step': 1e3 * wall / steps, 'higher_is_better': True, 'scaling': 'weak',
-           'vs_baseline': None, 'dtype': 'f64' if precision == 'fp64' else 'f32', 'data': 'synthetic',
-           'config': {'workload': name, 'bodies': n, 'dt': DT},
-           'p50_step_ms': 1e3 * float(np.percentile(lat, 50)), 'p99_step_ms': 1e3 * float(np.percentile(lat, 99)),
-           'max_contacts': cmax}
-    if cpu:
-        import oracle
-        ow = oracle.OracleWorld(arrays)
-        t0 = time.perf_counter()
-        ow.step(DT, steps)
-        cw = time.perf_counter() - t0
-        err = float(np.abs(ow.state()['position'] - pos).max())
-        out['cpu_baseline'] = {'value': n * steps / cw, 'unit': 'body-steps/s', 'cores': 1, 'kind': 'port',
-                               'sample': 'the whole world, %d steps, one core (%.1f s)' % (steps, cw)}
-        out['err'] = {'max_pos_m_vs_fp64_oracle': err}
+    def requests(spec):
+        return [Request(id=i, arrays=scenes.arrays(scenes.boxes_on_ground(r['n'], seed=r['seed'],
+                                                                          sphere_fraction=0.5)),
+                        steps=r['steps'], arrival=r['arrival']) for i, r in enumerate(spec)]
+    reqs = requests(scenes.serving_requests(n_requests, rate, seed=seed))
+    srv = RolloutServer(max_worlds=1024, max_bodies=1 << 19, precision=precision)
+    st = srv.run(reqs).summary()
+    srv.close()
+    wait = [r.admitted_at - r.arrival for r in reqs]
+    plain = serve_plain(requests(scenes.serving_requests(24, rate, seed=seed)), precision=precision)
+    out = {'arrival_rate_per_s': rate, 'requests': n_requests, 'precision': precision,
+           'body_steps_per_s': st['body_steps_per_s'], 'p50_step_ms': st['p50_step_ms'],
+           'p99_step_ms': st['p99_step_ms'], 'queue_wait_p99_ms': 1e3 * float(np.percentile(wait, 99)),
+           'mean_active_bodies': st['mean_active_bodies'], 'ticks': st['ticks'],
+           'plain': {'body_steps_per_s': plain['body_steps_per_s'], 'p50_step_ms': plain['p50_step_ms'],
+                     'p99_step_ms': plain['p99_step_ms'], 'requests': 24}}
+    out['over_plain'] = out['body_steps_per_s'] / max(plain['body_steps_per_s'], 1e-9)
     return out
 
 
-def one_step_error(worlds, states):
-    """FP32 against FP64 after one step from the same state (the worlds at step K):
-    the precision of the FP32 narrowphase + solver before any chaotic growth."""
+def single_world(name, arrays, precision, steps, warmup, cpu):
+    """One world stepped alone on the GPU (the plain form) over its 300-step
+    rollout, the last ``steps`` steps timed; the oracle on one core over the same
+    rollout, its state compared at the end."""
     import oracle
-    from omi_physics_amd.batch import batch_from_arrays
-    start = []
-    for a, st in zip(worlds, states):
-        a = dict(a)
-        for k_arr, k_st in (('position', 'position'), ('orientation', 'orientation'),
-                            ('linear_velocity', 'linear_velocity'), ('angular_velocity', 'angular_velocity'),
-                            ('sleep_timer', 'sleep_timer')):
-            a[k_arr] = np.ascontiguousarray(st[k_st], dtype=np.float64)
-        a['awake'] = np.ascontiguousarray(st['awake'], dtype=np.uint8)
-        start.append(a)
-    b = batch_from_arrays(start, precision='fp32')
-    b.step(DT, 1)
-    b.sync()
-    ep, ev = 0.0, 0.0
-    for i, a in enumerate(start):
-        ow = oracle.OracleWorld(a)
-        ow.step(DT, 1)
-        o, g = ow.state(), b.read(i)
-        ep = max(ep, float(np.abs(o['position'] - g['position']).max()))
-        ev = max(ev, float(np.abs(o['linear_velocity'] - g['linear_velocity']).max()))
+    n = int(arrays['n'])
+    b, info = rollout([arrays], precision, steps, warmup, record=True)
+    first, t0, t1, total = info['bounds']
+    finish_rollout(b, info, t1)
+    pos = b.read(0)['position']
     b.close()
-    return {'fp32_one_step_pos_max_m': ep, 'fp32_one_step_vel_max_mps': ev}
+    out = {'workload': name, 'bodies': n, 'precision': precision, 'window': [t0, t1],
+           'value': n * steps / info['wall'], 'ms_per_step': 1e3 * info['wall'] / steps,
+           'rollout_ms_per_step': 1e3 * float(info['lat'].mean()),
+           'p50_step_ms': 1e3 * float(np.percentile(info['window_lat'], 50)),
+           'p99_step_ms': 1e3 * float(np.percentile(info['window_lat'], 99)),
+           'contacts_per_step_window': mean_counter(info['counters'][t0 - 1:t1], 'contacts'),
+           'contacts_per_step_rollout': mean_counter(info['counters'], 'contacts')}
+    if cpu:
+        ow = oracle.OracleWorld(arrays)
+        tc = time.perf_counter()
+        ow.step(DT, total)
+        cw = time.perf_counter() - tc
+        out['cpu_baseline'] = {'value': n * total / cw, 'unit': 'body-steps/s', 'cores': 1, 'kind': 'port',
+                               'sample': 'the whole world, all %d steps, one core (%.1f s)' % (total, cw)}
+        d = np.abs(ow.state()['position'] - pos).max()
+        out['max_pos_err_vs_fp64_oracle_m'] = float(d)
+    log('single world %s: %s' % (name, {k: v for k, v in out.items() if k != 'workload'}))
+    return out
 
 
-def serving(rates, n_requests, precision, seed=5):
-    from omi_physics_amd import scenes
-    from omi_physics_amd.serve import Request, RolloutServer, run_plain
+def single_worlds(steps, warmup, cpu):
+    from omi_physics_amd import scenes, PhysicsWorld, model
     res = {}
-    for rate in rates:
-        reqs_spec = scenes.serving_requests(n_requests, rate, seed=seed)
-        reqs = [Request(id=i, arrays=scenes.arrays(scenes.boxes_on_ground(r['n'], seed=r['seed'],
-                                                                              sphere_fraction=0.5)),
-                        steps=r['steps'], arrival=r['arrival']) for i, r in enumerate(reqs_spec)]
-        srv = RolloutServer(max_worlds=1024, max_bodies=1 << 19, precision=precision)
-        st = srv.run(reqs).summary()
-        srv.close()
-        wait = [r.admitted_at - r.arrival for r in reqs]
-        st['queue_wait_p99_s'] = float(np.percentile(wait, 99))
-        res[str(rate)] = st
-        log('serving rate %s/s: %s' % (rate, st))
-    plain_reqs = [Request(id=i, arrays=scenes.arrays(scenes.boxes_on_ground(r['n'], seed=r['seed'],
-                                                                            sphere_fraction=0.5)),
-                          steps=r['steps']) for i, r in enumerate(scenes.serving_requests(24, 1.0, seed=seed))]
-    res['plain'] = run_plain(plain_reqs, precision=precision)
+    res['boxes1000_fp64'] = single_world('1000 unit boxes on a ground box (20 x 20 m), one world',
+                                         scenes.arrays(scenes.boxes_on_ground(1000, seed=1)), 'fp64', steps, warmup,
+                                         cpu)
+    pyr = scenes.arrays(scenes.pyramids_and_debris())
+    for prec in ('fp32', 'fp64'):
+        res['pyramids_%s' % prec] = single_world('20 ten-level box pyramids + 8000 box/sphere debris, one world',
+                                                 pyr, prec, steps, warmup, cpu and prec == 'fp64')
+    t = time.perf_counter()
+    cvx = scenes.build(scenes.convex_and_chains(), PhysicsWorld, model).export_arrays()
+    log('convex-and-chains world built in %.1f s' % (time.perf_counter() - t))
+    for prec in ('fp32', 'fp64'):
+        res['convex_chains_%s' % prec] = single_world(
+            '2000 cooked convex hulls (16-32 points) + 20 hinge chains of 50 links, one world', cvx, prec, steps,
+            warmup, cpu and prec == 'fp64')
     return res
+
+
+# ---------------------------------------------------------------------- main
+
+def copy_bench(mib):
+    """A plain device-to-device copy of known size through the HIP runtime: the
+    achievable HBM rate and the calibration of the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL(os.path.join(os.environ.get('ROCM_PATH', '/opt/rocm'), 'lib', 'libamdhip64.so'))
+    nbytes = ctypes.c_size_t(mib << 20)
+    x, y = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(x), nbytes) or hip.hipMalloc(ctypes.byref(y), nbytes):
+        raise RuntimeError('hipMalloc failed')
+    hip.hipMemset(x, 1, nbytes)
+    for _ in range(3):
+        hip.hipMemcpyAsync(y, x, nbytes, 3, None)
+    hip.hipDeviceSynchronize()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
+    reps = 20
+    hip.hipEventRecord(e0, None)
+    for _ in range(reps):
+        hip.hipMemcpyAsync(y, x, nbytes, 3, None)
+    hip.hipEventRecord(e1, None)
+    hip.hipEventSynchronize(e1)
+    ms = ctypes.c_float()
+    hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+    per = ms.value / reps
+    print(json.dumps({'workload': 'copy', 'bytes_read_per_copy': nbytes.value, 'bytes_written_per_copy':
+                      nbytes.value, 'ms_per_copy': per, 'GBps': 2 * nbytes.value / (per * 1e-3) / 1e9}), flush=True)
+    hip.hipFree(x)
+    hip.hipFree(y)
+
+
+def traffic_for(window, workload):
+    """Measured HBM bytes per launch of the dominant kernel from the committed
+    rocprof summary, if it was taken on this window of this workload."""
+    tf = os.path.join(HERE, 'profiles', 'traffic.json')
+    if not os.path.exists(tf):
+        return None, None
+    with open(tf) as f:
+        t = json.load(f)
+    if t.get('window') != list(window) or t.get('workload') != workload:
+        return None, None
+    return t.get('k_solve_top_fp64_bytes_per_launch'), t.get('command')
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument('--workload', default='batched4096',
-                    choices=['batched4096', 'boxes1000', 'pyramids', 'convex', 'serving', 'copy'])
-    ap.add_argument('--copy-mib', type=int, default=1024, help='buffer size of the copy workload')
-    ap.add_argument('--precision', default=None, help='fp32 or fp64 for the single-world workloads')
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=300)
-    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--steps', type=int, default=20, help='timed steps: the last steps of the 300-step rollout')
+    ap.add_argument('--warmup', type=int, default=5, help='untimed steps right before the timed window')
     ap.add_argument('--worlds', type=int, default=4096)
     ap.add_argument('--seed', type=int, default=3)
+    ap.add_argument('--workload', default='batched4096', choices=['batched4096', 'copy'])
+    ap.add_argument('--copy-mib', type=int, default=1024, help='buffer size of the copy workload')
     ap.add_argument('--plain-sample', type=int, default=48, help='worlds stepped alone for the plain form')
-    ap.add_argument('--cpu-sample', type=int, default=192, help='worlds the CPU baseline and error check step')
-    ap.add_argument('--no-plain', action='store_true')
+    ap.add_argument('--curve-sample', type=int, default=192, help='worlds of the FP32 error curves')
+    ap.add_argument('--only-headline', action='store_true', help='the FP64 window and nothing else (profiling)')
+    ap.add_argument('--no-timing', action='store_true', help='skip the event-timed pass of the window')
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--no-fp32', action='store_true', help='skip the batched FP32 form')
-    ap.add_argument('--no-timing', action='store_true')
-    ap.add_argument('--lanes', action='store_true', help='split the worlds by size class into lanes stepped side by side')
+    ap.add_argument('--no-extras', action='store_true', help='skip serving and the single-world workloads')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
-        print('bench.py measures one GPU only: there is no multi-GPU path (independent worlds would shard '
-              'across GPUs by world id, with no collective on the step path).', file=sys.stderr)
+        print('bench.py measures one GPU: every GPU box here has one.  The independent worlds would shard '
+              'across GPUs by world id (greedy bin-packing by body count, no collective on the step path).',
+              file=sys.stderr)
         sys.exit(2)
     from omi_physics_amd import _ffi
     if _ffi.load().omi_device_count() < 1:
         print('no HIP device visible', file=sys.stderr)
         sys.exit(1)
     if args.workload == 'copy':
-        # a plain device-to-device copy of known size through the HIP runtime (no
-        # framework): the achievable HBM rate and the calibration of the
-        # FETCH_SIZE / WRITE_SIZE counters
-        import ctypes
-        hip = ctypes.CDLL(os.path.join(os.environ.get('ROCM_PATH', '/opt/rocm'), 'lib', 'libamdhip64.so'))
-        nbytes = ctypes.c_size_t(args.copy_mib << 20)
-        x, y = ctypes.c_void_p(), ctypes.c_void_p()
-        if hip.hipMalloc(ctypes.byref(x), nbytes) or hip.hipMalloc(ctypes.byref(y), nbytes):
-            raise RuntimeError('hipMalloc failed')
-        hip.hipMemset(x, 1, nbytes)
-        d2d = 3  # hipMemcpyDeviceToDevice
-        for _ in range(3):
-            hip.hipMemcpyAsync(y, x, nbytes, d2d, None)
-        hip.hipDeviceSynchronize()
-        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
-        hip.hipEventCreate(ctypes.byref(e0))
-        hip.hipEventCreate(ctypes.byref(e1))
-        reps = 20
-        hip.hipEventRecord(e0, None)
-        for _ in range(reps):
-            hip.hipMemcpyAsync(y, x, nbytes, d2d, None)
-        hip.hipEventRecord(e1, None)
-        hip.hipEventSynchronize(e1)
-        ms = ctypes.c_float()
-        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
-        per = ms.value / reps
-        print(json.dumps({'workload': 'copy', 'bytes_read_per_copy': nbytes.value, 'bytes_written_per_copy':
-                          nbytes.value, 'ms_per_copy': per, 'GBps': 2 * nbytes.value / (per * 1e-3) / 1e9}),
-              flush=True)
-        hip.hipFree(x)
-        hip.hipFree(y)
-        return
-    if args.workload != 'batched4096':
-        from omi_physics_amd import scenes, PhysicsWorld, model
-        if args.workload == 'serving':
-            out = {'metric': METRIC, 'workload': 'serving', 'unit': 'body-steps/s',
-                   'rates': serving([100.0, 400.0, 1600.0], 160, args.precision or 'fp32')}
-            print(json.dumps(out), flush=True)
-            return
-        if args.workload == 'boxes1000':
-            arr = scenes.arrays(scenes.boxes_on_ground(1000, seed=1))
-            prec = args.precision or 'fp64'
-            name = '1000 unit boxes on a 20x20 m ground, one world, %s' % prec
-        elif args.workload == 'pyramids':
-            arr = scenes.arrays(scenes.pyramids_and_debris())
-            prec = args.precision or 'fp32'
-            name = '20 ten-level box pyramids + 8000 box/sphere debris, one world, %s' % prec
-        else:
-            arr = scenes.build(scenes.convex_and_chains(), PhysicsWorld, model).export_arrays()
-            prec = args.precision or 'fp32'
-            name = ('2000 cooked convex hulls (16-32 points) and 20 hinge chains of 50 links on a ground box, '
-                    'one world, %s' % prec)
-        out = single_world(name, arr, prec, args.steps, args.warmup, not args.no_cpu)
-        print(json.dumps(out), flush=True)
+        copy_bench(args.copy_mib)
         return
 
     arrays = make_worlds(args.worlds, args.seed)
     n_bodies = int(sum(a['n'] for a in arrays))
-    rng = np.random.default_rng(11)
-    sample = np.sort(rng.choice(len(arrays), size=min(args.cpu_sample, len(arrays)), replace=False))
+    first, t0, t1, total = window_bounds(args.steps, args.warmup)
+    workload = ('%d independent box/sphere worlds, lognormal(ln 64, 1.0) bodies clipped [8, 2048], batched in one '
+                'graph-captured step, FP64 (bit-identical to the oracle); timed window: steps %d..%d of the %d-step '
+                'rollout' % (args.worlds, t0, t1, total))
 
-    # headline: the batched FP64 form, graph-launched steps 1..K
-    b, lat, wall, _, _, _ = run_batched(arrays, 'fp64', args.steps, args.warmup, timing=False, lanes=args.lanes)
+    # headline: the batched FP64 form, graph-launched, the window timed
+    b, info = rollout(arrays, 'fp64', args.steps, args.warmup)
+    wall = info['wall']
     value = n_bodies * args.steps / wall
-    gpu64 = {int(i): b.read(int(i)) for i in sample}
+    snap64 = [p for p, _, _, _ in b.snapshot()]
+    finish_rollout(b, info, t1)
     b.close()
+    cw = info['counters'][t0 - 1:t1]
     out = {
         'metric': METRIC, 'value': value, 'unit': 'body-steps/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': 1e3 * wall / args.steps, 'higher_is_better': True,
         'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic',
-        'config': {'workload': '%d independent box/sphere worlds, lognormal(ln 64, 1.0) bodies clipped [8, 2048], '
-                               'batched in one graph-captured step%s, FP64 (bit-identical to the oracle); steps '
-                               '1..%d from rest' % (args.worlds, ' per size-class lane' if args.lanes else '', args.steps),
-                   'worlds': args.worlds, 'bodies': n_bodies, 'dt': DT, 'solver_iterations': 10,
-                   'sleep_enabled': True, 'seed': args.seed, 'lanes': bool(args.lanes)},
-        'p50_step_ms': 1e3 * float(np.percentile(lat, 50)),
-        'p99_step_ms': 1e3 * float(np.percentile(lat, 99)),
+        'config': {'workload': workload, 'worlds': args.worlds, 'bodies': n_bodies, 'dt': DT,
+                   'solver_iterations': 10, 'sleep_enabled': True, 'seed': args.seed, 'window': [t0, t1],
+                   'contacts_per_step_window': mean_counter(cw, 'contacts'),
+                   'contacts_per_step_rollout': mean_counter(info['counters'], 'contacts')},
+        'p50_step_ms': 1e3 * float(np.percentile(info['window_lat'], 50)),
+        'p99_step_ms': 1e3 * float(np.percentile(info['window_lat'], 99)),
+        'contacts_per_s': mean_counter(cw, 'contacts') * args.steps / wall,
+        'rollout': {'steps': total, 'ms_per_step': 1e3 * float(info['lat'].mean()),
+                    'p50_step_ms': 1e3 * float(np.percentile(info['lat'], 50)),
+                    'p99_step_ms': 1e3 * float(np.percentile(info['lat'], 99)),
+                    'body_steps_per_s': n_bodies / float(info['lat'].mean())},
     }
+    log('headline: %.3f ms/step over steps %d..%d, %.0f contacts/step (rollout mean %.0f)'
+        % (out['ms_per_step'], t0, t1, out['config']['contacts_per_step_window'],
+           out['config']['contacts_per_step_rollout']))
+    if args.only_headline:
+        print(json.dumps(out), flush=True)
+        return
+
     if not args.no_timing:
-        # the same K steps again from the initial state, launched eagerly with HIP
-        # events between the stages on the batch stream: device time per stage,
-        # counters and the roofline of the dominant kernel
-        bt, lat_t, wall_t, ktot, ctot, kbytes = run_batched(arrays, 'fp64', args.steps, 0, timing=True,
-                                                            lanes=args.lanes)
-        same = all(np.array_equal(bt.read(int(i))['position'], gpu64[int(i)]['position']) for i in sample[:8])
+        # the same window on a second batch, its steps launched eagerly with HIP
+        # events between the stages on the batch stream (events cannot be timed
+        # inside a captured graph)
+        bt, ti = rollout(arrays, 'fp64', args.steps, args.warmup, timing=True)
+        same = all(np.array_equal(p, q) for p, (q, _, _, _) in zip(snap64[:64], bt.snapshot(list(range(64)))))
         bt.close()
-        log('timed pass: %.2f ms/step eager (graph %.2f), identical results: %s'
-            % (1e3 * wall_t / args.steps, 1e3 * wall / args.steps, same))
-        out['eager_ms_per_step'] = 1e3 * wall_t / args.steps
-        tot = {}
-        for c in ctot:
-            for k, v in c.items():
-                if k not in ('overflow', 'max_levels'):
-                    tot[k] = tot.get(k, 0) + int(v)
-        out['contacts_per_s'] = tot.get('contacts', 0) / wall
-        out['totals'] = tot
-        # one lane per size class present (the worlds of each class are a member
-        # batch of one group; the lanes' steps run side by side)
-        names = []
-        for li, kt in enumerate(ktot):
-            kt['solve'] = kt['prepare'] + kt['solve_top'] + kt['solve_rest']
-            cls, nw = kbytes[li]['top_class']
-            names.append('lane %d: %d worlds of size class %d' % (li, nw, cls))
-        out['kernels_ms_per_step'] = {nm: {k: v / args.steps for k, v in kt.items()} for nm, kt in zip(names, ktot)}
-        crit = int(np.argmax([kt['solve_top'] for kt in ktot]))
-        top, ntop = kbytes[crit]['top_class']
-        avg_ms = ktot[crit]['solve_top'] / args.steps
-        per_launch = kbytes[crit]['solve_top'] / args.steps
+        kt, kb = ti['kernel_ms'], ti['kernel_bytes']
+        K = args.steps
+        kt['solve'] = kt['prepare'] + kt['solve_top'] + kt['solve_rest']
+        out['eager_ms_per_step'] = 1e3 * ti['wall'] / K
+        out['kernels_ms_per_step'] = {k: v / K for k, v in kt.items()}
+        top, ntop = kb['top_class']
+        avg_ms = kt['solve_top'] / K
+        per_launch = kb['solve_top'] / K
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
-        traffic = None
-        tf = os.path.join(HERE, 'profiles', 'traffic.json')
-        if os.path.exists(tf):
-            with open(tf) as f:
-                traffic = json.load(f).get('k_solve_top_fp64_bytes_per_launch')
-        out['roofline'] = {'kernel': 'k_solve<double> size class %d (%d worlds, the largest; alone on its '
-                                     'stream)' % (top, ntop),
-                           'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                           'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic, 'avg_ms': avg_ms,
-                           'bytes_per_launch': per_launch}
-        out['roofline_stages'] = {nm: {k: {'avg_ms': kt[k] / args.steps,
-                                           'GBps': kb[k] / max(kt[k] * 1e-3, 1e-12) / 1e9}
-                                       for k in ('pre', 'sap', 'narrow', 'solve')}
-                                  for nm, kt, kb in zip(names, ktot, kbytes)}
+        traffic, tcmd = traffic_for((t0, t1), 'batched4096')
+        frac = achieved / HBM_PEAK_GBS
+        out['roofline'] = {
+            'kernel': 'k_solve<double> size class %d (%d worlds, the largest; alone on its stream)' % (top, ntop),
+            'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': frac,
+            'traffic': traffic, 'traffic_command': tcmd, 'avg_ms': avg_ms, 'bytes_per_launch': per_launch,
+            'limiter': ('latency: %.2f %% of the HBM roof; the launch is one dependency chain of solver levels '
+                        '(occupancy and wave stalls: profiles/sq_stalls_solver_fp64.md)' % (100 * frac)
+                        if frac < 0.05 else 'hbm')}
+        step_ms = 1e3 * ti['wall'] / K
+        sb = kb['step'] / K
+        out['step_roofline'] = {'bytes_per_step': sb, 'ms_per_step': out['ms_per_step'],
+                                'achieved_GBps': sb / (out['ms_per_step'] * 1e-3) / 1e9,
+                                'frac': sb / (out['ms_per_step'] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        out['stage_roofline'] = {k: {'avg_ms': kt[k] / K, 'GBps': kb[k] / max(kt[k] * 1e-3, 1e-12) / 1e9}
+                                 for k in ('pre', 'sap', 'narrow', 'solve')}
+        log('timed pass: %.2f ms/step eager (graph %.2f), identical results: %s' % (step_ms, out['ms_per_step'],
+                                                                                   same))
 
-    if not args.no_fp32:
-        b32, lat32, wall32, *_ = run_batched(arrays, 'fp32', args.steps, 0, timing=False, lanes=args.lanes)
-        gpu32 = {int(i): b32.read(int(i))['position'] for i in sample}
-        b32.close()
-        out['fp32_batched'] = {'value': n_bodies * args.steps / wall32, 'ms_per_step': 1e3 * wall32 / args.steps,
-                               'p50_step_ms': 1e3 * float(np.percentile(lat32, 50)),
-                               'p99_step_ms': 1e3 * float(np.percentile(lat32, 99)), 'dtype': 'f32',
-                               'over_fp64': wall / wall32}
+    # the batched FP32 form on the same window
+    b32, i32 = rollout(arrays, 'fp32', args.steps, args.warmup, record=False)
+    b32.close()
+    out['fp32_batched'] = {'value': n_bodies * args.steps / i32['wall'], 'ms_per_step': 1e3 * i32['wall'] / args.steps,
+                           'p50_step_ms': 1e3 * float(np.percentile(i32['window_lat'], 50)),
+                           'p99_step_ms': 1e3 * float(np.percentile(i32['window_lat'], 99)), 'dtype': 'f32',
+                           'over_fp64': wall / i32['wall']}
 
-    if not args.no_plain:
-        ps = np.sort(rng.choice(len(arrays), size=min(args.plain_sample, len(arrays)), replace=False))
-        pv, plat = run_plain([arrays[i] for i in ps], args.steps, 'fp64')
-        out['plain'] = {'value': pv, 'unit': 'body-steps/s', 'dtype': 'f64', 'worlds_sampled': len(ps),
-                        'p50_step_ms': 1e3 * float(np.percentile(plat, 50)),
-                        'p99_step_ms': 1e3 * float(np.percentile(plat, 99)),
-                        'batched_over_plain': value / pv}
+    rng = np.random.default_rng(11)
+    ps = np.sort(rng.choice(len(arrays), size=min(args.plain_sample, len(arrays)), replace=False))
+    pv, plat = run_plain([arrays[i] for i in ps], args.steps, args.warmup, 'fp64')
+    out['plain'] = {'value': pv, 'unit': 'body-steps/s', 'dtype': 'f64', 'worlds_sampled': len(ps),
+                    'window': [t0, t1], 'p50_step_ms': 1e3 * float(np.percentile(plat, 50)),
+                    'p99_step_ms': 1e3 * float(np.percentile(plat, 99)), 'batched_over_plain': value / pv}
 
     if not args.no_cpu:
-        threads = min(len(os.sched_getaffinity(0)), 16)
-        sub = [arrays[i] for i in sample]
-        cv, cwall, cpos = cpu_baseline(sub, args.steps, threads)
-        out['cpu_baseline'] = {'value': cv, 'unit': 'body-steps/s', 'cores': threads, 'kind': 'port',
-                               'sample': '%d of the %d worlds (%d bodies), %d steps each, one world per '
-                                         'thread, %.1f s wall' % (len(sub), len(arrays),
-                                                                   sum(a['n'] for a in sub), args.steps, cwall)}
-        e64 = np.array([np.abs(gpu64[int(i)]['position'] - p).max() for i, p in zip(sample, cpos)])
-        err = {'worlds': len(sample), 'steps': args.steps, 'fp64_max_m': float(e64.max())}
-        if not args.no_fp32:
-            # FP64 alone: the same worlds with every dynamic body's initial position
-            # moved by one ulp -- how far the chaotic piles carry a last-bit change
-            def nudged(a):
-                a = dict(a)
-                p = a['position'].copy()
-                dyn = a['motion_type'] == 2
-                p[dyn] = np.nextafter(p[dyn], np.inf)
-                a['position'] = p
-                return a
-            _, _, upos = cpu_baseline([nudged(a) for a in sub], args.steps, threads)
-            e32 = np.array([np.abs(gpu32[int(i)] - p).max() for i, p in zip(sample, cpos)])
-            eu = np.array([np.abs(u - p).max() for u, p in zip(upos, cpos)])
-            err.update({'fp32_max_m': float(e32.max()), 'fp32_median_world_max_m': float(np.median(e32)),
-                        'fp32_worlds_within_1e-4': float(np.mean(e32 <= 1e-4)),
-                        'fp64_1ulp_max_m': float(eu.max()), 'fp64_1ulp_median_world_max_m': float(np.median(eu)),
-                        'fp64_1ulp_worlds_within_1e-4': float(np.mean(eu <= 1e-4))})
-            err.update(one_step_error(sub, [gpu64[int(i)] for i in sample]))
-        out['err'] = err
+        worlds, cwall = oracle_window(arrays, args.steps, args.warmup, CPU_THREADS)
+        exact = sum(int(np.array_equal(ow.state()['position'], p)) for ow, p in zip(worlds, snap64))
+        out['cpu_baseline'] = {'value': n_bodies * args.steps / cwall, 'unit': 'body-steps/s',
+                               'cores': CPU_THREADS, 'kind': 'port',
+                               'sample': 'every one of the %d worlds (%d bodies) over the same window (steps %d..%d, '
+                                         'advanced untimed to it), one world per task on %d threads, %.2f s wall; '
+                                         'an upper bound on the reference CPU path (C loops, no Python per-pair '
+                                         'work)' % (len(arrays), n_bodies, t0, t1, CPU_THREADS, cwall)}
+        del worlds
+        cs = np.sort(rng.choice(len(arrays), size=min(args.curve_sample, len(arrays)), replace=False))
+        out['err'] = {'fp64_worlds_bit_exact_at_window_end': exact, 'worlds': len(arrays),
+                      'curves_worlds': len(cs),
+                      'curves': error_curves([arrays[i] for i in cs], [1, 3, 10, 30, 100, 200, 300], CPU_THREADS)}
+        log('parity: %d of %d worlds bit-exact at step %d' % (exact, len(arrays), t1))
+
+    if not args.no_extras:
+        out['serving'] = serving_run(400.0, 160, 'fp32')
+        log('serving: %s' % out['serving'])
+        out['single_world'] = single_worlds(args.steps, args.warmup, not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 
