@@ -46,7 +46,7 @@ __device__ __forceinline__ uint64_t hash_mix(uint64_t x) {
 // Exclusive scan of a[0..n) in place (LDS or global), BLOCK threads; returns
 // the total to every thread.  scratch: BLOCK/64 + 2 ints of LDS.
 template <int BLOCK>
-__device__ int32_t block_exscan(int32_t* a, int32_t n, int32_t* scratch) {
+__device__ __forceinline__ int32_t block_exscan(int32_t* a, int32_t n, int32_t* scratch) {
     constexpr int NW = BLOCK / 64;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     int32_t running = 0;
@@ -970,6 +970,9 @@ template <> struct RowLayout<float> {
                          nrm = FR_DIR;
 };
 static_assert(SR_DEP < SR_HOT && SR_NX + 3 <= SR_HOT && SR_HOT % 2 == 0, "the FP64 sweep's fields lie in the hot prefix");
+static_assert(SR_NX + 3 <= SR_DEP && SR_IMA < SR_DEP && SR_IMB < SR_DEP && SR_FR < SR_DEP && SR_VB < SR_DEP &&
+                  FR_FR < FR_DEP && FR_IMA < FR_DEP && FR_IMB < FR_DEP && FR_VB < FR_DEP && FR_KB + 9 <= FR_DEP,
+              "the velocity passes read only the values before the depth");
 
 // n values of type R from 16-byte aligned memory as 16-byte loads
 // (a last partial chunk is loaded value by value)
@@ -1103,11 +1106,12 @@ struct LevelTable {
         if (G) __hip_atomic_store(&mask[i], (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         else mask[i] = 0;
     }
-    // orders this wavefront's table accesses (LDS: in-order, so a compiler
-    // barrier and an LDS wait suffice)
+    // orders this wavefront's table accesses.  LDS executes one wavefront's
+    // instructions in issue order, so there a compiler barrier suffices and no
+    // wait is spent; global memory takes a real fence.
     __device__ __forceinline__ void fence() const {
         if (G) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
-        else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
+        else asm volatile("" ::: "memory");
     }
 };
 
@@ -1118,7 +1122,7 @@ struct LevelTable {
 // pass-0 contact.  Called by the 64 threads of one wavefront; the table starts
 // at zero.
 template <bool G>
-__device__ int32_t assign_item_levels(const LevelTable<G>& T, int32_t M, int32_t NI, const int4* mrec,
+__device__ __forceinline__ int32_t assign_item_levels(const LevelTable<G>& T, int32_t M, int32_t NI, const int4* mrec,
                                       int32_t* level, int32_t* lcnt, int32_t cap, bool& over, int32_t& d0) {
     const int lane = threadIdx.x & 63;
     int32_t D = 0, Dp = 0;
@@ -1340,7 +1344,7 @@ struct SolveState {
 // Bucket the contact items by level: bound[l] is the start of level l on
 // entry (after the scan) and its end on exit.
 template <int BLOCK>
-__device__ void scatter_items(int32_t* bound, int32_t SL, int32_t M, const int4* mrec, const int32_t* level,
+__device__ __forceinline__ void scatter_items(int32_t* bound, int32_t SL, int32_t M, const int4* mrec, const int32_t* level,
                               int32_t cs, int64_t off, int32_t* corder, int32_t* cord_a, int32_t* cord_b) {
     for (int32_t s = 0; s < SL; s++)
         for (int32_t m = threadIdx.x; m < M; m += BLOCK) {
@@ -1356,18 +1360,22 @@ __device__ void scatter_items(int32_t* bound, int32_t SL, int32_t M, const int4*
         }
 }
 
-// The level loop of one world.  Each thread's first item of level l + 2 (code
-// and bodies) and of level l + 1 (row and inverse inertia) load while level l
-// runs.  POS = false: the velocity passes (warm start and sweeps) over all D
+// The level loop of one world, in steps of at most BLOCK items of one level.
+// Each thread's item of step k + 2 (code and bodies) and of step k + 1 (row
+// and inverse inertia) load while step k runs.  POS = false: the velocity passes (warm start and sweeps) over all D
 // levels; POS = true: the split-impulse position pass, which reads only the
 // contacts' depth, masses and normal and so runs on its own after the
 // velocity passes, over the levels holding pass-0 items (D is then that
 // count).  LIGHT: the state the mode updates (velocities and impulses, or
 // positions) is in LDS and a level ends on an LDS barrier; otherwise on a
-// full barrier.
-template <typename R, int BLOCK, bool LIGHT, bool POS>
-__device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>& st, size_t bound_off, int32_t cs,
-                           int32_t K, int32_t D, int32_t NC, int32_t S, bool lds_bounds, const int32_t* gbound,
+// full barrier.  LB: the level bounds are in LDS, else in global memory.  LB
+// is a template argument, not a flag, so that no global load feeds the bound
+// registers the loop carries: the compiler's wait for such a load, placed at
+// every level, would also wait for the next level's item loads and expose
+// their latency.
+template <typename R, int BLOCK, bool LIGHT, bool POS, bool LB>
+__device__ __forceinline__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>& st, size_t bound_off, int32_t cs,
+                           int32_t K, int32_t D, int32_t NC, int32_t S, const int32_t* gbound,
                            const int32_t* corder, const int32_t* cord_a, const int32_t* cord_b) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     const int tid = threadIdx.x;
@@ -1395,7 +1403,10 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>&
         it.ci = cs + (r.code >> 7);
         it.a = r.a; it.b = r.b;
         if (LIGHT) { it.cida = lmap[r.a - st.off]; it.cidb = lmap[r.b - st.off]; }
-        load_vec<R, RowLayout<R>::hot>(&ROW(S_, it.ci, 0), it.row);
+        // exactly the values the mode reads (the velocity passes stop short of
+        // the depth): a loaded value nothing reads lets the compiler reuse its
+        // register, which then waits for the load in flight
+        load_vec<R, POS ? RowLayout<R>::dep + 1 : RowLayout<R>::dep>(&ROW(S_, it.ci, 0), it.row);
         if constexpr (sizeof(R) == 8 && !POS) {
             load_vec<R, 9>(&IIW(iiw, it.a, 0), it.Ia);
             load_vec<R, 9>(&IIW(iiw, it.b, 0), it.Ib);
@@ -1591,33 +1602,34 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>&
             if (it.db) vel_store(b, it.cidb, vb2);
         }
     };
-    auto bound_at = [&](int32_t l) -> int32_t {
-        if (l >= D) return NC;
-        if (lds_bounds) return lbound[l];
+    // end of level l (l < D)
+    auto bound_raw = [&](int32_t l) -> int32_t {
+        if constexpr (LB) return lbound[l];
         // past L1: written by this block's atomics
         return __hip_atomic_load(&gbound[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     };
+    // the items of levels [0, D): all NC for the velocity passes, those up
+    // to the last pass-0 level for the position pass
+    int32_t NE = D > 0 ? bound_raw(D - 1) : 0;
+    NE = NE < NC ? NE : NC;
+    // Steps: each level's items in chunks of at most BLOCK, so every thread
+    // runs at most one item per step (the items of a level share no body, so
+    // any split of a level is a valid order).  sl/sp: the level and start of
+    // the next step to define; all uniform.
+    int32_t sl = 0, sp = 0;
+    auto step_end = [&]() -> int32_t {
+        if (sp >= NE) return NE;
+        const int32_t le = sl < D ? bound_raw(sl) : NE;
+        const int32_t e = sp + BLOCK < le ? sp + BLOCK : le;
+        sp = e;
+        if (e == le) sl++;
+        return e;
+    };
     // The loop is unrolled by two so the two item buffers swap roles instead
-    // of being copied (a copy would wait for the loads in flight).
-    int32_t e_prev = 0, e_cur = bound_at(0), e_next = bound_at(1);
-    IdRec idA{0, 0, 0}, idB{0, 0, 0};
-    SolveItem<R> itA, itB;
-    fetch_id(tid, idA);
-    if (tid < e_cur) fetch_item(idA, itA);
-    fetch_id(e_cur + tid, idB);     // first items of level 1
-    auto level_step = [&](int32_t l, SolveItem<R>& cur, SolveItem<R>& nxt, const IdRec& idn,
-                          IdRec& id2) {
-        fetch_id(e_next + tid, id2);                                    // level l + 2
-        const int32_t e_next2 = bound_at(l + 2);
-        if (l + 1 < D && e_cur + tid < e_next) fetch_item(idn, nxt);   // level l + 1
-        if (e_prev + tid < e_cur) process(cur);
-        for (int32_t t = e_prev + tid + BLOCK; t < e_cur; t += BLOCK) {
-            IdRec r{0, 0, 0};
-            SolveItem<R> x;
-            fetch_id(t, r);
-            fetch_item(r, x);
-            process(x);
-        }
+    // of being copied (a copy would wait for the loads in flight).  Each step
+    // loads the ids of step k + 2 and the items of step k + 1 while step k
+    // runs.
+    auto level_barrier = [&]() {
         if (LIGHT) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
             if (BLOCK > 64) __builtin_amdgcn_s_barrier();
@@ -1625,13 +1637,45 @@ __device__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>&
         } else {
             __syncthreads();
         }
+    };
+    // One step of prefetch: an item's loads are bound by the vector memory
+    // path (each lane's 16-byte piece of its own row is a separate cache line),
+    // not by latency, so a deeper prefetch gains nothing.
+    int32_t e_prev = 0;
+    int32_t e_cur = step_end();
+    int32_t e_next = step_end();
+    IdRec idA{0, 0, 0}, idB{0, 0, 0};
+    SolveItem<R> itA, itB;
+    fetch_id(tid, idA);
+    if (tid < e_cur) fetch_item(idA, itA);
+    fetch_id(e_cur + tid, idB);     // step 1
+    // Every step starts with nothing in flight: the loads a step issues
+    // for the next one are waited for at its end, after its own items
+    // ran.  The compiler then has no pending load to place a wait for
+    // inside an item (where, as it cannot tell which loads a lane
+    // skipped, it would wait for the newest ones).
+    __builtin_amdgcn_s_waitcnt(0);
+    auto do_step = [&](SolveItem<R>& cur, SolveItem<R>& nxt, const IdRec& idn, IdRec& id2) {
+        const int32_t e_next2 = step_end();
+        fetch_id(e_next + tid, id2);                               // step k + 2
+        if (e_cur + tid < e_next) fetch_item(idn, nxt);           // step k + 1
+        if (e_prev + tid < e_cur) process(cur);
+        __builtin_amdgcn_s_waitcnt(0);
+        level_barrier();
         e_prev = e_cur; e_cur = e_next; e_next = e_next2;
     };
-    for (int32_t l = 0; l < D; l += 2) {
-        level_step(l, itA, itB, idB, idA);
-        if (l + 1 < D) level_step(l + 1, itB, itA, idA, idB);
+    while (e_prev < NE) {
+        do_step(itA, itB, idB, idA);
+        if (e_prev < NE) do_step(itB, itA, idA, idB);
     }
 #undef IMP
+}
+
+// run_levels with the bounds' place chosen at run time
+template <typename R, int BLOCK, bool LIGHT, bool POS, typename... A>
+__device__ __forceinline__ void run_levels_lb(bool lds_bounds, A&&... a) {
+    if (lds_bounds) run_levels<R, BLOCK, LIGHT, POS, true>(a...);
+    else run_levels<R, BLOCK, LIGHT, POS, false>(a...);
 }
 
 // LDS of the per-body position pass for a world of n bodies, K contacts and M
@@ -1659,7 +1703,7 @@ __host__ __device__ __forceinline__ size_t split_pos_bytes(int32_t K, int32_t M,
 // is written directly, idempotently.  FP32 accumulates the corrections from
 // zero and adds them to the FP64 position.
 template <typename R, int BLOCK>
-__device__ void split_positions(const Dev& d, const SolveIn& in, int32_t cs, int32_t K, int32_t M, int32_t n,
+__device__ __forceinline__ void split_positions(const Dev& d, const SolveIn& in, int32_t cs, int32_t K, int32_t M, int32_t n,
                                 int64_t off, const int4* mrec, unsigned char* lds, bool staged, int32_t* s_scan) {
     constexpr int NW = BLOCK / 64;
     constexpr bool fp64 = sizeof(R) == sizeof(double);
@@ -1871,11 +1915,20 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     int32_t* g_bound = d.lcount + ib;
 
     // ---- level assignment: table (body masks 8 B, last levels 4 B per body)
+    // and, when they fit beside it, the manifold records, so the assignment
+    // loop issues no global loads (on this GPU a global load's wait also waits
+    // for the loop's earlier level stores)
     const bool table_lds = al16((size_t)n * 8) + (size_t)n * 4 <= avail;
+    const size_t mrec_at = al16((size_t)n * 8) + al16((size_t)n * 4);
+    const bool mrec_lds = table_lds && mrec_at + (size_t)M * 16 <= avail;
     if (table_lds) {
         uint64_t* sm = (uint64_t*)s_dyn;
         int32_t* sl = (int32_t*)(s_dyn + al16((size_t)n * 8));
         for (int32_t i = tid; i < n; i += BLOCK) { sl[i] = 0; sm[i] = 0; }
+        if (mrec_lds) {
+            int4* lm = (int4*)(s_dyn + mrec_at);
+            for (int32_t m = tid; m < M; m += BLOCK) lm[m] = mrec[m];
+        }
     } else {
         for (int32_t i = tid; i < n; i += BLOCK) { d.last[off + i] = 0; d.lmask[off + i] = 0; }
     }
@@ -1887,8 +1940,13 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
         if (table_lds) {
             uint64_t* sm = (uint64_t*)s_dyn;
             int32_t* sl = (int32_t*)(s_dyn + al16((size_t)n * 8));
-            D = assign_item_levels<false>(LevelTable<false>{sl, sm}, M, NI, mrec, level, s_bound, SOLVE_DCAP, over,
-                                          D0);
+            // two calls, so each knows its records' address space
+            if (mrec_lds)
+                D = assign_item_levels<false>(LevelTable<false>{sl, sm}, M, NI, (const int4*)(s_dyn + mrec_at), level,
+                                              s_bound, SOLVE_DCAP, over, D0);
+            else
+                D = assign_item_levels<false>(LevelTable<false>{sl, sm}, M, NI, mrec, level, s_bound, SOLVE_DCAP,
+                                              over, D0);
         } else {
             D = assign_item_levels<true>(LevelTable<true>{d.last + off, d.lmask + off}, M, NI, mrec, level, s_bound,
                                          SOLVE_DCAP, over, D0);
@@ -1966,11 +2024,11 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     // bound[l] is now the end of level l
     if (S > 0) {
         if (state_lds)
-            run_levels<R, BLOCK, true, false>(d, in, st, avail, cs, K, D, NC, S, lds_bounds, g_bound,
-                                                  corder, cord_a, cord_b);
+            run_levels_lb<R, BLOCK, true, false>(lds_bounds, d, in, st, avail, cs, K, D, NC, S, g_bound, corder,
+                                                 cord_a, cord_b);
         else
-            run_levels<R, BLOCK, false, false>(d, in, st, avail, cs, K, D, NC, S, lds_bounds, g_bound,
-                                                   corder, cord_a, cord_b);
+            run_levels_lb<R, BLOCK, false, false>(lds_bounds, d, in, st, avail, cs, K, D, NC, S, g_bound, corder,
+                                                  cord_a, cord_b);
     }
     __syncthreads();
 
@@ -2026,8 +2084,8 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
                     for (int q = 0; q < 3; q++) P[(size_t)q * nd + c] = fp64 ? (R)BODY(d.pos, q, off + i) : R(0);
                 }
                 __syncthreads();
-                run_levels<R, BLOCK, true, true>(d, in, st, avail, cs, K, D0, NC, S, lds_bounds, g_bound,
-                                                     corder, cord_a, cord_b);
+                run_levels_lb<R, BLOCK, true, true>(lds_bounds, d, in, st, avail, cs, K, D0, NC, S, g_bound, corder,
+                                                    cord_a, cord_b);
                 __syncthreads();
                 for (int32_t i = tid; i < n; i += BLOCK) {
                     if (!touched(i)) continue;
@@ -2039,8 +2097,8 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
                     }
                 }
             } else {
-                run_levels<R, BLOCK, false, true>(d, in, st, avail, cs, K, D0, NC, S, lds_bounds, g_bound,
-                                                      corder, cord_a, cord_b);
+                run_levels_lb<R, BLOCK, false, true>(lds_bounds, d, in, st, avail, cs, K, D0, NC, S, g_bound, corder,
+                                                     cord_a, cord_b);
             }
         }
     }
