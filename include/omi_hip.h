@@ -242,6 +242,11 @@ typedef struct {
     /* broadphase pairs with a trigger body per step, all worlds
      * (0 = max(1024, max_pairs / 16)). */
     int64_t max_trigger_pairs;
+    /* 0: a size class holding too few worlds to fill the device (fewer than
+     * one per 32 compute units), each fitting the solver's LDS, is solved
+     * island by island (solver.py:412-445), the largest such class only;
+     * -1: every world in one solver block.  The results are the same. */
+    int32_t island_solve;
 } omi_batch_config;
 
 typedef struct omi_batch omi_batch;
@@ -350,6 +355,10 @@ int32_t omi_batch_use_graph(omi_batch *b, int32_t on);
  * NULL.  Batch-layer diagnostics with no counterpart in the reference. */
 int32_t omi_batch_world_stats(omi_batch *b, int32_t n, int32_t *bodies, int32_t *contacts,
                               int32_t *manifolds, int32_t *levels, int32_t *size_class);
+/* The size class solved island by island (-1: none) and the islands of its
+ * worlds in the last step.  Batch-layer diagnostics with no counterpart in
+ * the reference. */
+int32_t omi_batch_islands(omi_batch *b, int32_t *size_class, int64_t *islands);
 /* Threads per world of the broadphase sort and of the solver for each size
  * class (as in omi_batch_config; 0 keeps the current value).  Batch-layer
  * tuning knob with no counterpart in the reference; the autotuner uses it. */

@@ -59,7 +59,7 @@ class BatchConfig(ctypes.Structure):
         ('record_broadphase', ctypes.c_int32), ('solver_block', ctypes.c_int32),
         ('sap_block', ctypes.c_int32 * 4), ('solve_block', ctypes.c_int32 * 4),
         ('max_candidates', ctypes.c_int64), ('max_solver_iterations', ctypes.c_int32),
-        ('max_trigger_pairs', ctypes.c_int64),
+        ('max_trigger_pairs', ctypes.c_int64), ('island_solve', ctypes.c_int32),
     ]
 
 
@@ -77,6 +77,7 @@ EXPORTS = (
     'omi_batch_read_trigger_overlaps',
     'omi_batch_counters', 'omi_batch_set_timing', 'omi_batch_kernel_times',
     'omi_batch_kernel_names', 'omi_batch_use_graph', 'omi_batch_set_blocks', 'omi_batch_world_stats',
+    'omi_batch_islands',
     'omi_batch_device_bytes',
     'omi_group_create', 'omi_group_destroy', 'omi_group_prepare', 'omi_group_step', 'omi_group_sync',
 )
@@ -152,6 +153,7 @@ def load() -> ctypes.CDLL:
         'omi_batch_read_pairs': (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, _i64,
                                                   _i32]),
         'omi_batch_counters': (ctypes.c_int32, [_vp, _i64]),
+        'omi_batch_islands': (ctypes.c_int32, [_vp, _i32, _i64]),
         'omi_batch_set_timing': (ctypes.c_int32, [_vp, ctypes.c_int32]),
         'omi_batch_kernel_times': (ctypes.c_int32, [_vp, ctypes.c_int32, _d]),
         'omi_batch_kernel_names': (ctypes.c_char_p, []),

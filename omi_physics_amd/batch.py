@@ -50,7 +50,7 @@ class Batch:
                  max_hull_pts: int = 1, max_joints: int = 1, record_broadphase: bool = False,
                  solver_block: int = 0, use_graph: bool = True,
                  sap_block: Optional[Sequence[int]] = None,
-                 solve_block: Optional[Sequence[int]] = None) -> None:
+                 solve_block: Optional[Sequence[int]] = None, island_solve: bool = True) -> None:
         self.lib = _ffi.load()
         caps = capacities_for(max_bodies, max_worlds)
         cfg = _ffi.BatchConfig()
@@ -63,6 +63,7 @@ class Batch:
         cfg.max_joints = int(max(1, max_joints))
         cfg.record_broadphase = int(bool(record_broadphase))
         cfg.solver_block = int(solver_block)
+        cfg.island_solve = 0 if island_solve else -1
         from .autotune import load_tuned
         tuned = load_tuned(precision)
         for c in range(4):
@@ -280,6 +281,13 @@ class Batch:
                                                             ('bodies', 'contacts', 'manifolds', 'levels',
                                                              'size_class'))))
         return out
+
+    def islands(self) -> Dict[str, int]:
+        """The size class solved island by island (-1: none) and its islands in the last step."""
+        c = np.zeros(1, np.int32)
+        k = np.zeros(1, np.int64)
+        check(self.lib.omi_batch_islands(self._h, ptr(c, _i32), ptr(k, _i64)))
+        return {'size_class': int(c[0]), 'islands': int(k[0])}
 
     def device_bytes(self) -> int:
         return int(self.lib.omi_batch_device_bytes(self._h))

@@ -50,7 +50,9 @@ void launch_stage_spheres(int precision, int32_t np, int32_t mode, const double*
                           double* nrm, double* pt, double* dep, hipStream_t s);
 void launch_prepare(const Dev& d, int precision, SolveIn in, bool stage, hipStream_t s);
 void launch_solve_class(const Dev& d, int precision, int block, SolveIn in, const int32_t* wl, const int32_t* wn,
-                        int32_t lds_bytes, int grid, hipStream_t s);
+                        int32_t lds_bytes, int grid, hipStream_t s, int32_t isl);
+void launch_islands(const Dev& d, const int32_t* wl, const int32_t* wn, int32_t lds_bytes, int grid, hipStream_t s);
+bool islands_fit(int32_t n, int32_t lds_bytes);
 size_t epa_work_bytes();
 bool allow_dynamic_lds(int bytes);
 bool allow_joint_lds(int bytes);
@@ -144,6 +146,10 @@ struct omi_batch {
     int32_t live[4] = {0, 0, 0, 0};        // grid of each class's per-world launches (0: none)
     int32_t sap_block[4] = {64, 128, 1024, 1024};
     int32_t solve_block[4] = {64, 128, 256, 256};
+    // size classes solved island by island (few worlds, each fitting the
+    // island solve's LDS): their worlds' islands spread over the chip
+    int32_t isl[4] = {0, 0, 0, 0};
+    int32_t cus = 256;                     // compute units of the device
     int32_t np_lds_max = 4096;
     int32_t sap_grid = 2048;               // blocks of the chip-wide broadphase pair passes
     bool lists_dirty = true;
@@ -178,7 +184,7 @@ struct omi_batch {
 };
 
 extern "C" const char* omi_last_error(void) { return g_err.c_str(); }
-extern "C" int32_t omi_abi_version(void) { return 2; }
+extern "C" int32_t omi_abi_version(void) { return 3; }
 extern "C" int32_t omi_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -255,6 +261,7 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     int cus = 256;
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     b->sap_grid = std::max(8, cus) * 8;
+    b->cus = std::max(1, cus);
     // dynamic LDS per launch: the device's LDS less 1 KB for the kernels'
     // static LDS, or 64 KB less that if the runtime refuses the larger size
     int dyn_lds = smem - 1024;
@@ -340,7 +347,8 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     d.cnrm = cn; d.cdep = cd; d.cimp = ci; d.csolv = cs;
     d.slmax = 1 + (cfg->max_solver_iterations > 0 ? cfg->max_solver_iterations : 16);
     A(d.clevel, K * d.slmax); A(d.corder, K * d.slmax); A(d.lcount, K * d.slmax);
-    A(d.cord_a, K * d.slmax); A(d.cord_b, K * d.slmax);
+    A(d.cord_a, K * d.slmax); A(d.cord_b, K * d.slmax); A(d.cord_c, K * d.slmax);
+    A(d.isl_lab, K); A(d.isl_desc, K * ISL_DESC); A(d.isl_mrec, K); A(d.isl_cidx, K);
     for (int q = 0; q < 2; q++) {
         unsigned char* ki;
         A(d.ka[q], K); A(d.kb[q], K); A(d.kpt[q], 3 * K); A(ki, 3 * K * (int64_t)rs); d.kimp[q] = ki;
@@ -801,6 +809,20 @@ static int32_t rebuild_lists(omi_batch* b) {
                          [&](int32_t x, int32_t y) { return b->slots[x].n > b->slots[y].n; });
     int32_t rc, counts[4];
     bool grew = false;
+    // A class is solved island by island when its worlds are too few to fill
+    // the chip (fewer than one per 32 CUs) and every one of them fits the
+    // island solve's LDS; else one block per world.  One class at most (the
+    // largest that qualifies): the island table and its count are shared.
+    int32_t isl_class = -1;
+    for (int c = 3; c >= 0 && isl_class < 0 && b->cfg.island_solve >= 0; c--) {
+        bool ok = !lists[c].empty() && (int64_t)lists[c].size() * 32 <= b->cus;
+        for (int32_t w : lists[c]) ok = ok && islands_fit(b->slots[w].n, b->lds_bytes[c]);
+        if (ok) isl_class = c;
+    }
+    for (int c = 0; c < 4; c++) {
+        const int32_t isl = c == isl_class;
+        if (isl != b->isl[c]) { b->isl[c] = isl; grew = true; }
+    }
     for (int c = 0; c < 4; c++) {
         counts[c] = (int32_t)lists[c].size();
         if ((rc = put(b, b->d_wl + (size_t)c * b->cfg.max_worlds, lists[c].data(), counts[c]))) return rc;
@@ -871,8 +893,15 @@ static void enqueue_solve(omi_batch* b, SolveIn in, bool stage, Mark mark) {
     launch_prepare(d, prec, in, stage, s);
     mark();
     fork_classes(b, [&](int c, hipStream_t sc) {
-        launch_solve_class(d, prec, b->solve_block[c], in, b->d_wl + (size_t)c * d.wcap, b->d_wn + c,
-                           b->lds_bytes[c], b->live[c], sc);
+        const int32_t* wl = b->d_wl + (size_t)c * d.wcap;
+        if (b->isl[c]) {
+            // the class's islands, then two blocks per CU taking them in turn
+            launch_islands(d, wl, b->d_wn + c, b->lds_bytes[c], b->live[c], sc);
+            launch_solve_class(d, prec, b->solve_block[c], in, wl, b->d_wn + c, b->lds_bytes[c],
+                               std::max(b->live[c], 2 * b->cus), sc, 1);
+        } else {
+            launch_solve_class(d, prec, b->solve_block[c], in, wl, b->d_wn + c, b->lds_bytes[c], b->live[c], sc, 0);
+        }
     }, mark);
 }
 
@@ -1186,11 +1215,23 @@ extern "C" int32_t omi_batch_sync(omi_batch* b) {
         char msg[320];
         snprintf(msg, sizeof msg,
                  "batch capacity exceeded in a step since the last sync (flags 0x%llx: 1 pairs, 2 contacts, "
-                 "4 cache, 8 broadphase record, 16 broadphase candidates, 32 trigger pairs); results of that step are invalid "
-                 "— recreate the batch with larger max_pairs/max_contacts/max_candidates",
+                 "4 cache, 8 broadphase record, 16 broadphase candidates, 32 trigger pairs, 64 island LDS); results "
+                 "of that step are invalid — recreate the batch with larger max_pairs/max_contacts/max_candidates",
                  (unsigned long long)ov);
         return fail(OMI_ERR_CAPACITY, msg);
     }
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_islands(omi_batch* b, int32_t* size_class, int64_t* islands) {
+    if (!b || !size_class || !islands) return fail(OMI_ERR_ARG, "null argument");
+    uint64_t c = 0;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(&c, b->d.cnt + C_ISLANDS, sizeof c, hipMemcpyDeviceToHost));
+    *size_class = -1;
+    for (int k = 0; k < 4; k++)
+        if (b->isl[k]) *size_class = k;
+    *islands = *size_class >= 0 ? (int64_t)c : 0;
     return OMI_OK;
 }
 

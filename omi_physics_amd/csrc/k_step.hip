@@ -1199,6 +1199,7 @@ __device__ __forceinline__ int32_t item_code(int32_t k, bool da, bool db, int32_
 template <typename R>
 struct SolveItem {
     int32_t ci, a, b;               // contact, bodies (batch indices)
+    int32_t k;                      // contact within the world or island (its impulse state)
     int32_t s, cida, cidb;          // pass; compact ids of the sides (LDS state)
     bool da, db;
     R row[RowLayout<R>::hot];
@@ -1345,7 +1346,8 @@ struct SolveState {
 // entry (after the scan) and its end on exit.
 template <int BLOCK>
 __device__ __forceinline__ void scatter_items(int32_t* bound, int32_t SL, int32_t M, const int4* mrec, const int32_t* level,
-                              int32_t cs, int64_t off, int32_t* corder, int32_t* cord_a, int32_t* cord_b) {
+                              int32_t cl, const int32_t* gmap, int64_t off, int32_t* corder, int32_t* cord_a,
+                              int32_t* cord_b, int32_t* cord_c) {
     for (int32_t s = 0; s < SL; s++)
         for (int32_t m = threadIdx.x; m < M; m += BLOCK) {
             const int4 r = mrec[m];
@@ -1353,7 +1355,9 @@ __device__ __forceinline__ void scatter_items(int32_t* bound, int32_t SL, int32_
             const bool da = r.y & MR_DYN_A, db = r.y & MR_DYN_B;
             for (int32_t j = 0; j < c; j++) {
                 const int32_t slot = atomicAdd(&bound[L - 1 + j], 1);
-                corder[slot] = item_code(r.x - cs + j, da, db, s);
+                const int32_t k = r.x - cl + j;
+                corder[slot] = item_code(k, da, db, s);
+                cord_c[slot] = gmap ? gmap[k] : r.x + j;
                 cord_a[slot] = (int32_t)off + r.z;
                 cord_b[slot] = (int32_t)off + r.w;
             }
@@ -1374,9 +1378,10 @@ __device__ __forceinline__ void scatter_items(int32_t* bound, int32_t SL, int32_
 // every level, would also wait for the next level's item loads and expose
 // their latency.
 template <typename R, int BLOCK, bool LIGHT, bool POS, bool LB>
-__device__ __forceinline__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>& st, size_t bound_off, int32_t cs,
+__device__ __forceinline__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>& st, size_t bound_off,
                            int32_t K, int32_t D, int32_t NC, int32_t S, const int32_t* gbound,
-                           const int32_t* corder, const int32_t* cord_a, const int32_t* cord_b) {
+                           const int32_t* corder, const int32_t* cord_a, const int32_t* cord_b,
+                           const int32_t* cord_c) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     const int tid = threadIdx.x;
     const bool fp64 = sizeof(R) == sizeof(double);
@@ -1390,17 +1395,18 @@ __device__ __forceinline__ void run_levels(const Dev& d, const SolveIn& in, cons
     const R* S_ = (const R*)d.csolv;
     const R* iiw = (const R*)d.iiw;
     // velocity c of side body i (global index) with compact id ci_
-#define IMP(c, ci) (LIGHT ? lP[(size_t)((ci) - cs) * 3 + (size_t)(c)] \
+#define IMP(c, ci) (LIGHT ? lP[(size_t)it.k * 3 + (size_t)(c)] \
                           : st.P[(size_t)((ci) - st.pb) * st.ps + (size_t)(c) * st.pc])
-    struct IdRec { int32_t code, a, b; };
+    struct IdRec { int32_t code, a, b, c; };
     auto fetch_id = [&](int32_t slot, IdRec& r) {
-        if (slot < NC) { r.code = corder[slot]; r.a = cord_a[slot]; r.b = cord_b[slot]; }
+        if (slot < NC) { r.code = corder[slot]; r.a = cord_a[slot]; r.b = cord_b[slot]; r.c = cord_c[slot]; }
     };
     auto fetch_item = [&](const IdRec& r, SolveItem<R>& it) {
         it.s = r.code & 31;
         it.da = (r.code >> 6) & 1;
         it.db = (r.code >> 5) & 1;
-        it.ci = cs + (r.code >> 7);
+        it.k = r.code >> 7;
+        it.ci = r.c;
         it.a = r.a; it.b = r.b;
         if (LIGHT) { it.cida = lmap[r.a - st.off]; it.cidb = lmap[r.b - st.off]; }
         // exactly the values the mode reads (the velocity passes stop short of
@@ -1644,7 +1650,7 @@ __device__ __forceinline__ void run_levels(const Dev& d, const SolveIn& in, cons
     int32_t e_prev = 0;
     int32_t e_cur = step_end();
     int32_t e_next = step_end();
-    IdRec idA{0, 0, 0}, idB{0, 0, 0};
+    IdRec idA{0, 0, 0, 0}, idB{0, 0, 0, 0};
     SolveItem<R> itA, itB;
     fetch_id(tid, idA);
     if (tid < e_cur) fetch_item(idA, itA);
@@ -1703,8 +1709,9 @@ __host__ __device__ __forceinline__ size_t split_pos_bytes(int32_t K, int32_t M,
 // is written directly, idempotently.  FP32 accumulates the corrections from
 // zero and adds them to the FP64 position.
 template <typename R, int BLOCK>
-__device__ __forceinline__ void split_positions(const Dev& d, const SolveIn& in, int32_t cs, int32_t K, int32_t M, int32_t n,
-                                int64_t off, const int4* mrec, unsigned char* lds, bool staged, int32_t* s_scan) {
+__device__ __forceinline__ void split_positions(const Dev& d, const SolveIn& in, int32_t cl, const int32_t* gmap, int32_t K,
+                                int32_t M, int32_t n, int64_t off, const int4* mrec, unsigned char* lds, bool staged,
+                                int32_t* s_scan) {
     constexpr int NW = BLOCK / 64;
     constexpr bool fp64 = sizeof(R) == sizeof(double);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1723,7 +1730,7 @@ __device__ __forceinline__ void split_positions(const Dev& d, const SolveIn& in,
     __syncthreads();
     // contact k's terms (FP64: computed by k_prepare) and normal
     auto terms = [&](int32_t k, R& ta, R& tb, R* nv) -> bool {
-        const int64_t ci = cs + k;
+        const int64_t ci = gmap ? (int64_t)gmap[k] : (int64_t)cl + k;
         for (int r = 0; r < 3; r++) nv[r] = ROW(S_, ci, RowLayout<R>::nrm + r);
         if constexpr (fp64) {
             ta = ROW(S_, ci, SR_TA);
@@ -1751,7 +1758,7 @@ __device__ __forceinline__ void split_positions(const Dev& d, const SolveIn& in,
     // on the other sides
     for (int32_t m = tid; m < M; m += BLOCK) {
         const int4 r4 = mrec[m];
-        const int32_t c = r4.y & MR_COUNT, k0 = r4.x - cs;
+        const int32_t c = r4.y & MR_COUNT, k0 = r4.x - cl;
         int32_t na = 0;
         for (int32_t j = 0; j < c; j++) {
             const int32_t k = k0 + j;
@@ -1796,15 +1803,15 @@ __device__ __forceinline__ void split_positions(const Dev& d, const SolveIn& in,
     int32_t nbits = 1;
     while ((1 << nbits) <= n) nbits++;
     const uint64_t lt = (1ull << lane) - 1ull;
-    int4 nxt = tid < M ? mrec[tid] : make_int4(cs, 0, 0, 0);
+    int4 nxt = tid < M ? mrec[tid] : make_int4(cl, 0, 0, 0);
     for (int32_t base = 0; base < M; base += BLOCK) {
         const int32_t m = base + tid;
         const int4 r4 = nxt;
-        nxt = m + BLOCK < M ? mrec[m + BLOCK] : make_int4(cs, 0, 0, 0);   // the next block's record
+        nxt = m + BLOCK < M ? mrec[m + BLOCK] : make_int4(cl, 0, 0, 0);   // the next block's record
         int32_t A = n, B = n, na = 0, k0 = 0, c = 0;
         if (m < M) {
             na = nact[m];
-            k0 = r4.x - cs;
+            k0 = r4.x - cl;
             c = r4.y & MR_COUNT;
             if (na && (r4.y & MR_DYN_A)) A = r4.z;
             if (na && (r4.y & MR_DYN_B)) B = r4.w;
@@ -1879,20 +1886,161 @@ __device__ __forceinline__ void split_positions(const Dev& d, const SolveIn& in,
     }
 }
 
-template <typename R, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_t* wlist, const int32_t* wcount,
-                                                 int32_t lds_bytes) {
+// ---------------------------------------------------------------- islands --
+
+// The islands of one world (SequentialImpulseSolver.build_islands,
+// solver.py:412-445): bodies joined by manifolds whose two sides are dynamic,
+// each manifold in the island of its dynamic side.  Islands share no body
+// whose velocity the solver changes, so sweeping each island's contacts in
+// their order in the world gives the world's Gauss-Seidel result exactly.
+// One block per world: union-find over the manifolds (the root is the
+// smallest body index), manifolds and contacts per root, then one record per
+// island in the class's island table and the island's manifolds in world
+// order with island-local first contacts, and the batch index of each island
+// contact.  Parent, manifold and contact counts per body are in LDS (a world
+// solved island by island fits them; else overflow bit 6 is set).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_islands(Dev d, const int32_t* wlist, const int32_t* wcount,
+                                                   int32_t lds_bytes) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     __shared__ int32_t s_scan[BLOCK / 64 + 2];
-    __shared__ int32_t s_D, s_D0, s_nd;
-    __shared__ bool s_over;
+    __shared__ int32_t s_base;
     if ((int32_t)blockIdx.x >= *wcount) return;
     const int32_t w = wlist[blockIdx.x];
     const int tid = threadIdx.x;
     if (w >= d.wcap || !(d.w_flags[w] & WF_ACTIVE)) return;
-    const int32_t K = d.w_cn[w], cs = d.w_cs[w], n = d.w_cnt[w], M = d.w_mn[w];
+    if (tid == 0) d.w_levels[w] = 0;
+    const int32_t K = d.w_cn[w], cs = d.w_cs[w], n = d.w_cnt[w], M = d.w_mn[w], ms = d.w_ms[w];
+    if (K == 0 || M == 0) return;
+    const size_t arr = al16((size_t)(n + 1) * 4);
+    if (3 * arr > (size_t)lds_bytes) {
+        if (tid == 0) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 64ull);
+        return;
+    }
+    const int4* mrec = d.mrec + ms;
+    int32_t* P = (int32_t*)s_dyn;                 // parent, then island index of each root
+    int32_t* MC = (int32_t*)(s_dyn + arr);        // manifolds per root, then cursor
+    int32_t* KC = (int32_t*)(s_dyn + 2 * arr);    // contacts per root, then cursor
+    for (int32_t i = tid; i <= n; i += BLOCK) { P[i] = i; MC[i] = 0; KC[i] = 0; }
+    __syncthreads();
+    auto ld = [&](int32_t i) { return __hip_atomic_load(&P[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); };
+    // root with path halving (halving writes only move a body closer to its root)
+    auto find = [&](int32_t x) {
+        for (;;) {
+            const int32_t p = ld(x);
+            if (p == x) return x;
+            const int32_t g = ld(p);
+            if (g == p) return p;
+            __hip_atomic_store(&P[x], g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            x = g;
+        }
+    };
+    for (int32_t m = tid; m < M; m += BLOCK) {
+        const int4 r = mrec[m];
+        if ((r.y & (MR_DYN_A | MR_DYN_B)) != (MR_DYN_A | MR_DYN_B)) continue;
+        int32_t x = r.z, y = r.w;
+        for (;;) {
+            x = find(x);
+            y = find(y);
+            if (x == y) break;
+            if (x < y) { const int32_t t = x; x = y; y = t; }
+            if (atomicCAS(&P[x], x, y) == x) break;
+        }
+    }
+    __syncthreads();
+    // each manifold's root: that of its dynamic side (of side a when neither
+    // is dynamic: such a manifold moves no body and joins no island)
+    for (int32_t m = tid; m < M; m += BLOCK) {
+        const int4 r = mrec[m];
+        const int32_t rt = find((r.y & MR_DYN_A) || !(r.y & MR_DYN_B) ? r.z : r.w);
+        d.isl_lab[ms + m] = rt;
+        atomicAdd(&MC[rt], 1);
+        atomicAdd(&KC[rt], r.y & MR_COUNT);
+    }
+    __syncthreads();
+    for (int32_t i = tid; i < n; i += BLOCK) P[i] = MC[i] > 0;
+    __syncthreads();
+    const int32_t NI = block_exscan<BLOCK>(P, n, s_scan);
+    if (tid == 0) {
+        P[n] = NI;
+        s_base = (int32_t)atomicAdd((unsigned long long*)&d.cnt[C_ISLANDS], (unsigned long long)NI);
+    }
+    __syncthreads();
+    const int32_t base = s_base;
+    auto is_root = [&](int32_t i) { return P[i + 1] > P[i]; };
+    for (int32_t i = tid; i < n; i += BLOCK) {
+        if (!is_root(i)) continue;
+        int32_t* ds = d.isl_desc + (size_t)(base + P[i]) * ISL_DESC;
+        ds[0] = w; ds[1] = MC[i]; ds[2] = KC[i];
+    }
+    __syncthreads();
+    block_exscan<BLOCK>(MC, n, s_scan);
+    block_exscan<BLOCK>(KC, n, s_scan);
+    for (int32_t i = tid; i < n; i += BLOCK) {
+        if (!is_root(i)) continue;
+        int32_t* ds = d.isl_desc + (size_t)(base + P[i]) * ISL_DESC;
+        ds[3] = ms + MC[i]; ds[4] = cs + KC[i];
+    }
+    __syncthreads();
+    // from here MC / KC are each island's cursors (manifold and contact
+    // positions in the world, from its bases) and P its contact base
+    for (int32_t i = tid; i < n; i += BLOCK) P[i] = KC[i];
+    __syncthreads();
+    // the island records of each root's manifolds, in world order: one
+    // wavefront, 64 manifolds at a time (the next group's records load
+    // meanwhile); a lane's peers (lanes with its root) from a ballot per bit
+    // of the root, its rank and contact offset from the peers before it, the
+    // cursors moved by the last lane of each root
+    if (tid < 64) {
+        const int lane = tid;
+        const uint64_t lt = (1ull << lane) - 1ull;
+        int32_t nbits = 1;
+        while ((1 << nbits) <= n) nbits++;
+        int4 nr = lane < M ? mrec[lane] : make_int4(0, 0, 0, 0);
+        int32_t nl = lane < M ? d.isl_lab[ms + lane] : 0;
+        for (int32_t g = 0; g < M; g += 64) {
+            const int32_t m = g + lane;
+            const bool valid = m < M;
+            const int4 r = nr;
+            const int32_t rt = nl;
+            if (m + 64 < M) { nr = mrec[m + 64]; nl = d.isl_lab[ms + m + 64]; }
+            const int32_t c = valid ? (r.y & MR_COUNT) : 0;
+            uint64_t peers = __ballot(valid);
+            for (int b = 0; b < nbits; b++) {
+                const bool x = (rt >> b) & 1;
+                const uint64_t bx = __ballot(x);
+                peers &= x ? bx : ~bx;
+            }
+            const int32_t rank = __popcll(peers & lt);
+            // contacts per manifold: MR_COUNT bits, three unless a count needs more
+            const int cb = __ballot(c > 7) ? 8 : 3;
+            int32_t krank = 0;
+            for (int b = 0; b < cb; b++) krank += __popcll(peers & lt & __ballot((c >> b) & 1)) << b;
+            if (valid) {
+                const int32_t pos = MC[rt] + rank, kpos = KC[rt] + krank;
+                const int32_t kl = kpos - P[rt];
+                d.isl_mrec[ms + pos] = make_int4(kl, r.y, r.z, r.w);
+                for (int32_t j = 0; j < c; j++) d.isl_cidx[cs + kpos + j] = r.x + j;
+                if (!(peers >> lane >> 1)) { MC[rt] = pos + 1; KC[rt] = kpos + c; }
+            }
+            // LDS runs this wavefront's accesses in issue order
+            asm volatile("" ::: "memory");
+        }
+    }
+}
+
+template <typename R, int BLOCK>
+__device__ __forceinline__ void solve_unit(const Dev& d, const SolveIn& in, int32_t lds_bytes, int32_t w, int32_t K,
+                                           int32_t M, const int4* mrec, int32_t cl, const int32_t* gmap, int64_t kb,
+                                           bool isl) {
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    __shared__ int32_t s_scan[BLOCK / 64 + 2];
+    __shared__ int32_t s_D, s_D0, s_nd;
+    __shared__ bool s_over;
+    const int tid = threadIdx.x;
+    const int32_t n = d.w_cnt[w];
     const int64_t off = d.w_off[w];
-    if (K == 0) { if (tid == 0) d.w_levels[w] = 0; return; }
+    auto gci = [&](int32_t k) -> int64_t { return gmap ? (int64_t)gmap[k] : (int64_t)cl + k; };
     const bool fp64 = sizeof(R) == sizeof(double);
     constexpr size_t RB = sizeof(R);
     // The launch's dynamic LDS: the last 4 * SOLVE_DCAP bytes hold the level
@@ -1901,17 +2049,17 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     const size_t avail = (size_t)lds_bytes - 4 * (size_t)SOLVE_DCAP;
     int32_t* s_bound = (int32_t*)(s_dyn + avail);
     R* S_ = (R*)d.csolv;
-    const int4* mrec = d.mrec + d.w_ms[w];
     const int32_t iters = d.w_iters[w];
     const int32_t S = in.warm + iters;          // velocity passes
     const int32_t SL = S > 0 ? S : 1;           // passes in the schedule (the position pass rides on pass 0)
     const int32_t NI = SL * M;                  // manifold items
     const int32_t NC = SL * K;                  // contact items
-    const int64_t ib = (int64_t)cs * d.slmax;   // this world's item slice
+    const int64_t ib = kb * d.slmax;           // this unit's item slice
     int32_t* level = d.clevel + ib;
     int32_t* corder = d.corder + ib;
     int32_t* cord_a = d.cord_a + ib;
     int32_t* cord_b = d.cord_b + ib;
+    int32_t* cord_c = d.cord_c + ib;
     int32_t* g_bound = d.lcount + ib;
 
     // ---- level assignment: table (body masks 8 B, last levels 4 B per body)
@@ -1921,6 +2069,12 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     const bool table_lds = al16((size_t)n * 8) + (size_t)n * 4 <= avail;
     const size_t mrec_at = al16((size_t)n * 8) + al16((size_t)n * 4);
     const bool mrec_lds = table_lds && mrec_at + (size_t)M * 16 <= avail;
+    // an island shares the world's body range with the world's other islands,
+    // so its tables must be its block's own
+    if (isl && !(table_lds && al16((size_t)n * 4) <= avail)) {
+        if (tid == 0) atomicOr((unsigned long long*)&d.cnt[C_OVERFLOW], 64ull);
+        return;
+    }
     if (table_lds) {
         uint64_t* sm = (uint64_t*)s_dyn;
         int32_t* sl = (int32_t*)(s_dyn + al16((size_t)n * 8));
@@ -1953,7 +2107,8 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
         }
         if (tid == 0) {
             s_D = D; s_D0 = D0; s_over = over;
-            d.w_levels[w] = D;
+            if (isl) atomicMax(&d.w_levels[w], D);
+            else d.w_levels[w] = D;
             atomicMax((unsigned long long*)&d.cnt[C_MAXLEVEL], (unsigned long long)D);
         }
     }
@@ -1962,18 +2117,35 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     const bool lds_bounds = !s_over;
 
     // ---- compact ids of the bodies in contact (the table is no longer needed)
+    // (-1: not in contact).  The first lane to name a body claims it and
+    // takes the next id, one counter update per wavefront; which body gets
+    // which id does not change any result.
     const bool map_lds = al16((size_t)n * 4) <= avail;
     int32_t* cmap = map_lds ? (int32_t*)s_dyn : d.last + off;
-    for (int32_t i = tid; i < n; i += BLOCK) cmap[i] = 0;
+    for (int32_t i = tid; i < n; i += BLOCK) cmap[i] = -1;
+    if (tid == 0) s_nd = 0;
     __syncthreads();
-    for (int32_t m = tid; m < M; m += BLOCK) {
-        const int4 r = mrec[m];
-        cmap[r.z] = 1;
-        cmap[r.w] = 1;
+    {
+        const int lane = tid & 63;
+        const uint64_t lt = (1ull << lane) - 1ull;
+        for (int32_t base = 0; base < M; base += BLOCK) {
+            const int32_t m = base + tid;
+            const bool v = m < M;
+            const int4 r = v ? mrec[m] : make_int4(0, 0, 0, 0);
+            for (int side = 0; side < 2; side++) {
+                const int32_t x = side ? r.w : r.z;
+                const bool own = v && atomicCAS(&cmap[x], -1, -2) == -1;
+                const uint64_t bal = __ballot(own);
+                int32_t wb = 0;
+                if (lane == 0 && bal) wb = atomicAdd(&s_nd, (int32_t)__popcll(bal));
+                wb = __shfl(wb, 0, 64);
+                if (own) cmap[x] = wb + (int32_t)__popcll(bal & lt);
+            }
+        }
     }
     __syncthreads();
-    const int32_t nd = block_exscan<BLOCK>(cmap, n, s_scan);
-    auto touched = [&](int32_t i) { return (i + 1 < n ? cmap[i + 1] : nd) > cmap[i]; };
+    const int32_t nd = s_nd;
+    auto touched = [&](int32_t i) { return cmap[i] >= 0; };
     SolveState<R> st;
     st.map = 0;
     st.vel = al16((size_t)n * 4);
@@ -1997,17 +2169,19 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
         }
         R* I = (R*)(s_dyn + st.imp);
         for (int32_t k = tid; k < K; k += BLOCK)
-            for (int q = 0; q < 3; q++) I[(size_t)k * 3 + q] = ROW(S_, cs + k, RowLayout<R>::imp + q);
+            for (int q = 0; q < 3; q++) I[(size_t)k * 3 + q] = ROW(S_, gci(k), RowLayout<R>::imp + q);
     } else if (!fp64) {
         R* V = (R*)d.svel;
-        for (int32_t i = tid; i < n; i += BLOCK)
+        for (int32_t i = tid; i < n; i += BLOCK) {
+            if (!touched(i)) continue;
             for (int q = 0; q < 6; q++) V[(size_t)q * d.ncap + off + i] = (R)BODY(d.lv, q, off + i);
+        }
     }
 
     // ---- contacts per level -> level starts -> items scattered by level
     if (lds_bounds) {
         block_exscan<BLOCK>(s_bound, D, s_scan);
-        scatter_items<BLOCK>(s_bound, SL, M, mrec, level, cs, off, corder, cord_a, cord_b);
+        scatter_items<BLOCK>(s_bound, SL, M, mrec, level, cl, gmap, off, corder, cord_a, cord_b, cord_c);
     } else {
         for (int32_t l = tid; l < D; l += BLOCK) g_bound[l] = 0;
         __syncthreads();
@@ -2018,17 +2192,17 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
             }
         __syncthreads();
         block_exscan<BLOCK>(g_bound, D, s_scan);
-        scatter_items<BLOCK>(g_bound, SL, M, mrec, level, cs, off, corder, cord_a, cord_b);
+        scatter_items<BLOCK>(g_bound, SL, M, mrec, level, cl, gmap, off, corder, cord_a, cord_b, cord_c);
     }
     __syncthreads();
     // bound[l] is now the end of level l
     if (S > 0) {
         if (state_lds)
-            run_levels_lb<R, BLOCK, true, false>(lds_bounds, d, in, st, avail, cs, K, D, NC, S, g_bound, corder,
-                                                 cord_a, cord_b);
+            run_levels_lb<R, BLOCK, true, false>(lds_bounds, d, in, st, avail, K, D, NC, S, g_bound, corder,
+                                                 cord_a, cord_b, cord_c);
         else
-            run_levels_lb<R, BLOCK, false, false>(lds_bounds, d, in, st, avail, cs, K, D, NC, S, g_bound, corder,
-                                                  cord_a, cord_b);
+            run_levels_lb<R, BLOCK, false, false>(lds_bounds, d, in, st, avail, K, D, NC, S, g_bound, corder,
+                                                  cord_a, cord_b, cord_c);
     }
     __syncthreads();
 
@@ -2037,7 +2211,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     // solver's change folded in for FP32)
     R* imp = (R*)d.cimp;
     for (int32_t k = tid; k < K; k += BLOCK) {
-        const int64_t ci = cs + k;
+        const int64_t ci = gci(k);
         for (int r = 0; r < 3; r++)
             CON(imp, r, ci) = state_lds ? ((R*)(s_dyn + st.imp))[(size_t)k * 3 + r] : ROW(S_, ci, RowLayout<R>::imp + r);
     }
@@ -2057,6 +2231,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
                 }
             }
         } else if (!fp64) {
+            if (!touched(i)) continue;
             for (int q = 0; q < 6; q++) {
                 const double v0 = BODY(d.lv, q, off + i);
                 const R dv = ((const R*)d.svel)[(size_t)q * d.ncap + off + i] - (R)v0;
@@ -2071,8 +2246,8 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
     if (D0 > 0) {
         __syncthreads();
         if (split_pos_bytes(K, M, n, 0) <= avail) {
-            split_positions<R, BLOCK>(d, in, cs, K, M, n, off, mrec, s_dyn, split_pos_bytes(K, M, n, RB) <= avail,
-                                      s_scan);
+            split_positions<R, BLOCK>(d, in, cl, gmap, K, M, n, off, mrec, s_dyn,
+                                      split_pos_bytes(K, M, n, RB) <= avail, s_scan);
         } else {
             st.pos = st.vel;
             st.pos_lds = map_lds && st.pos + al16((size_t)nd * 3 * RB) <= avail;
@@ -2084,23 +2259,55 @@ __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_
                     for (int q = 0; q < 3; q++) P[(size_t)q * nd + c] = fp64 ? (R)BODY(d.pos, q, off + i) : R(0);
                 }
                 __syncthreads();
-                run_levels_lb<R, BLOCK, true, true>(lds_bounds, d, in, st, avail, cs, K, D0, NC, S, g_bound, corder,
-                                                    cord_a, cord_b);
+                run_levels_lb<R, BLOCK, true, true>(lds_bounds, d, in, st, avail, K, D0, NC, S, g_bound, corder,
+                                                    cord_a, cord_b, cord_c);
                 __syncthreads();
                 for (int32_t i = tid; i < n; i += BLOCK) {
                     if (!touched(i)) continue;
                     const int32_t c = cmap[i];
                     for (int q = 0; q < 3; q++) {
-                        if (fp64) BODY(d.pos, q, off + i) = (double)P[(size_t)q * nd + c];
-                        else if (d.mt[off + i] == M_DYNAMIC)
+                        if (fp64 && d.mt[off + i] == M_DYNAMIC) {
+                            BODY(d.pos, q, off + i) = (double)P[(size_t)q * nd + c];
+                        } else if (fp64) {
+                            // a static or kinematic side: only -0.0 -> +0.0 can
+                            // change, written only if it did (another island
+                            // may share the body)
+                            const double p0 = BODY(d.pos, q, off + i), p1 = (double)P[(size_t)q * nd + c];
+                            if (__double_as_longlong(p1) != __double_as_longlong(p0)) BODY(d.pos, q, off + i) = p1;
+                        } else if (d.mt[off + i] == M_DYNAMIC)
                             BODY(d.pos, q, off + i) = BODY(d.pos, q, off + i) + (double)P[(size_t)q * nd + c];
                     }
                 }
             } else {
-                run_levels_lb<R, BLOCK, false, true>(lds_bounds, d, in, st, avail, cs, K, D0, NC, S, g_bound, corder,
-                                                     cord_a, cord_b);
+                run_levels_lb<R, BLOCK, false, true>(lds_bounds, d, in, st, avail, K, D0, NC, S, g_bound, corder,
+                                                     cord_a, cord_b, cord_c);
             }
         }
+    }
+}
+
+
+// The solver of a size class: one world per block, or (isl) the class's
+// islands, each block taking islands in turn from the island table.
+template <typename R, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_t* wlist, const int32_t* wcount,
+                                                 int32_t lds_bytes, int32_t isl) {
+    if (!isl) {
+        if ((int32_t)blockIdx.x >= *wcount) return;
+        const int32_t w = wlist[blockIdx.x];
+        if (w >= d.wcap || !(d.w_flags[w] & WF_ACTIVE)) return;
+        const int32_t K = d.w_cn[w], cs = d.w_cs[w];
+        if (K == 0) { if (threadIdx.x == 0) d.w_levels[w] = 0; return; }
+        solve_unit<R, BLOCK>(d, in, lds_bytes, w, K, d.w_mn[w], d.mrec + d.w_ms[w], cs, nullptr, cs, false);
+        return;
+    }
+    const uint64_t ni = d.cnt[C_ISLANDS];
+    const int32_t NIs = (int32_t)(ni < (uint64_t)d.kcap ? ni : (uint64_t)d.kcap);
+    for (int32_t ii = blockIdx.x; ii < NIs; ii += gridDim.x) {
+        const int32_t* ds = d.isl_desc + (size_t)ii * ISL_DESC;
+        const int32_t w = ds[0], M = ds[1], K = ds[2], mb = ds[3], kbase = ds[4];
+        solve_unit<R, BLOCK>(d, in, lds_bytes, w, K, M, d.isl_mrec + mb, 0, d.isl_cidx + kbase, kbase, true);
+        __syncthreads();
     }
 }
 
@@ -2951,25 +3158,25 @@ void launch_wake(const Dev& d, hipStream_t s) {
 
 template <typename R, int B>
 static void solve_one(const Dev& d, SolveIn in, const int32_t* wl, const int32_t* wn, int32_t lds_bytes,
-                      int grid, hipStream_t s) {
-    hipLaunchKernelGGL((k_solve<R, B>), dim3(grid), dim3(B), (size_t)lds_bytes, s, d, in, wl, wn, lds_bytes);
+                      int grid, hipStream_t s, int32_t isl) {
+    hipLaunchKernelGGL((k_solve<R, B>), dim3(grid), dim3(B), (size_t)lds_bytes, s, d, in, wl, wn, lds_bytes, isl);
 }
 
 template <typename R>
 static void solve_block(const Dev& d, int block, SolveIn in, const int32_t* wl, const int32_t* wn,
-                        int32_t lds_bytes, int grid, hipStream_t s) {
+                        int32_t lds_bytes, int grid, hipStream_t s, int32_t isl) {
     switch (block) {
-        case 64: solve_one<R, 64>(d, in, wl, wn, lds_bytes, grid, s); break;
-        case 128: solve_one<R, 128>(d, in, wl, wn, lds_bytes, grid, s); break;
-        default: solve_one<R, 256>(d, in, wl, wn, lds_bytes, grid, s); break;
+        case 64: solve_one<R, 64>(d, in, wl, wn, lds_bytes, grid, s, isl); break;
+        case 128: solve_one<R, 128>(d, in, wl, wn, lds_bytes, grid, s, isl); break;
+        default: solve_one<R, 256>(d, in, wl, wn, lds_bytes, grid, s, isl); break;
         // the FP64 sweep holds two prefetched items in ~300 VGPRs per thread,
         // more than a 512-thread block leaves it: FP64 runs up to 256 threads
         // (the C-ABI rejects larger FP64 blocks)
         case 512:
-            if constexpr (sizeof(R) == 4) solve_one<R, 512>(d, in, wl, wn, lds_bytes, grid, s);
+            if constexpr (sizeof(R) == 4) solve_one<R, 512>(d, in, wl, wn, lds_bytes, grid, s, isl);
             break;
         case 1024:
-            if constexpr (sizeof(R) == 4) solve_one<R, 1024>(d, in, wl, wn, lds_bytes, grid, s);
+            if constexpr (sizeof(R) == 4) solve_one<R, 1024>(d, in, wl, wn, lds_bytes, grid, s, isl);
             break;
     }
 }
@@ -2997,7 +3204,23 @@ bool allow_dynamic_lds(int bytes) {
     ok = allow_lds_fn(k_sap_sort<64, false>, bytes) && ok; ok = allow_lds_fn(k_sap_sort<128, false>, bytes) && ok;
     ok = allow_lds_fn(k_sap_sort<256, false>, bytes) && ok; ok = allow_lds_fn(k_sap_sort<512, false>, bytes) && ok;
     ok = allow_lds_fn(k_sap_sort<1024, false>, bytes) && ok;
+    ok = allow_lds_fn(k_islands<256>, bytes) && ok;
     return ok;
+}
+
+// Whether a world of n bodies can be solved island by island with lds_bytes
+// of dynamic LDS: the level table and the body map of an island's block, and
+// the island pass's three per-body arrays, all in LDS.
+bool islands_fit(int32_t n, int32_t lds_bytes) {
+    const size_t avail = (size_t)lds_bytes - 4 * (size_t)SOLVE_DCAP;
+    return lds_bytes > 4 * SOLVE_DCAP && al16((size_t)n * 8) + (size_t)n * 4 <= avail &&
+           3 * al16((size_t)(n + 1) * 4) <= (size_t)lds_bytes;
+}
+
+// The islands of size class c's worlds (one block per world) into the island
+// table the class's island-by-island solve launch reads.
+void launch_islands(const Dev& d, const int32_t* wl, const int32_t* wn, int32_t lds_bytes, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((k_islands<256>), dim3(grid), dim3(256), (size_t)lds_bytes, s, d, wl, wn, lds_bytes);
 }
 
 // The convex narrowphase's staged hull points: returns the dynamic LDS a
@@ -3024,12 +3247,13 @@ void launch_prepare(const Dev& d, int precision, SolveIn in, bool stage, hipStre
     }
 }
 
-// One block per world of size class c (world list wl, count *wn) with
-// lds_bytes of dynamic LDS (at least 4 * SOLVE_DCAP + 16, a multiple of 16).
+// One block per world of size class c (world list wl, count *wn), or with
+// isl the class's islands over `grid` blocks, with lds_bytes of dynamic LDS
+// (at least 4 * SOLVE_DCAP + 16, a multiple of 16).
 void launch_solve_class(const Dev& d, int precision, int block, SolveIn in, const int32_t* wl, const int32_t* wn,
-                        int32_t lds_bytes, int grid, hipStream_t s) {
-    if (precision == 64) solve_block<double>(d, block, in, wl, wn, lds_bytes, grid, s);
-    else solve_block<float>(d, block, in, wl, wn, lds_bytes, grid, s);
+                        int32_t lds_bytes, int grid, hipStream_t s, int32_t isl) {
+    if (precision == 64) solve_block<double>(d, block, in, wl, wn, lds_bytes, grid, s, isl);
+    else solve_block<float>(d, block, in, wl, wn, lds_bytes, grid, s, isl);
 }
 
 void launch_cache(const Dev& d, int precision, hipStream_t s) {

@@ -26,14 +26,19 @@ enum : int32_t {
     C_BPTOTAL = 5,     // broadphase pairs (all worlds)
     C_MANIFOLDS = 6,
     C_MAXLEVEL = 7,
-    C_OVERFLOW = 8,    // bit 0 pairs, 1 contacts, 2 cache, 3 broadphase record, 4 candidate tiles, 5 trigger pairs
+    C_OVERFLOW = 8,    // bit 0 pairs, 1 contacts, 2 cache, 3 broadphase record, 4 candidate tiles, 5 trigger pairs,
+                       // 6 a world solved island by island outgrew the launch's LDS
     C_CONVEX = 9,      // convex pairs queued for GJK/EPA
     C_TILES = 10,      // broadphase candidate tiles reserved
     C_PVALID = 11,     // solid pairs of the worlds whose pair lists fit (a prefix of the pair list)
     C_TPAIRS = 12,     // broadphase pairs with a trigger body (in no particular order)
     C_TOVER = 13,      // trigger overlaps found
+    C_ISLANDS = 14,    // islands of the step in the size classes solved island by island
     C_COUNT = 16
 };
+
+// ints per island record: world, manifolds, contacts, first manifold, first contact
+constexpr int32_t ISL_DESC = 8;
 
 // x-window candidates per broadphase tile (one wavefront task)
 constexpr int32_t SAP_TILE = 512;
@@ -118,6 +123,15 @@ struct Dev {
     int32_t slmax;                        // passes per solve the schedule holds (iterations + warm start)
     int32_t *clevel, *corder, *lcount;    // [slmax * kcap] level schedule of the solver passes
     int32_t *cord_a, *cord_b;             // [slmax * kcap] bodies of each scheduled item
+    int32_t *cord_c;                      // [slmax * kcap] contact (batch index) of each scheduled item
+    // islands (size classes solved island by island): per manifold its root
+    // body; per island a record (world, manifolds, contacts, first island
+    // manifold, first island contact); the island's manifold records in order
+    // with island-local first contacts; the batch index of each island contact
+    int32_t *isl_lab;                     // [kcap]
+    int32_t *isl_desc;                    // [kcap][ISL_DESC]
+    int4 *isl_mrec;                       // [kcap]
+    int32_t *isl_cidx;                    // [kcap]
     // warm-start cache, two buffers
     int32_t *ka[2], *kb[2];
     double *kpt[2];                       // [3][kcap]

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_binding_loads_without_gpu():
     L = _ffi.load()
-    assert L.omi_abi_version() == 2
+    assert L.omi_abi_version() == 3
     from omi_physics_amd.batch import KERNELS
     assert tuple(L.omi_batch_kernel_names().decode().split(',')) == KERNELS
 
