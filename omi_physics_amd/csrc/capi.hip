@@ -151,6 +151,10 @@ struct omi_batch {
     int32_t isl[4] = {0, 0, 0, 0};
     int32_t cus = 256;                     // compute units of the device
     int32_t np_lds_max = 4096;
+    int32_t dyn_lds = 0;                   // dynamic LDS a launch may take
+    // entries of each class's broadphase sort in LDS (0: global scratch);
+    // classes 0-2 their size bound, class 3 its largest world when it fits
+    int32_t sort_np[4] = {256, 1024, 4096, 0};
     int32_t sap_grid = 2048;               // blocks of the chip-wide broadphase pair passes
     bool lists_dirty = true;
     int64_t nhw = 0;
@@ -268,6 +272,8 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     if (dyn_lds > 65536 - 1024 && !allow_dynamic_lds(dyn_lds)) dyn_lds = 65536 - 1024;
     b->np_lds_max = dyn_lds >= 4096 * 16 ? 4096 : 2048;
     b->np_class[2] = b->np_lds_max;
+    b->dyn_lds = dyn_lds;
+    for (int c = 0; c < 3; c++) b->sort_np[c] = b->np_class[c];
     // the solver's LDS per world of a size class: small worlds take less so
     // that more of them share a CU (a world that does not fit keeps its
     // solver state in global memory)
@@ -809,6 +815,14 @@ static int32_t rebuild_lists(omi_batch* b) {
                          [&](int32_t x, int32_t y) { return b->slots[x].n > b->slots[y].n; });
     int32_t rc, counts[4];
     bool grew = false;
+    // the largest world of class 3 sorts in LDS when its 16 bytes per body fit
+    {
+        int32_t big = 0;
+        for (int32_t w : lists[3]) big = std::max(big, (int32_t)b->slots[w].n);
+        big = (big + 63) & ~63;
+        const int32_t np3 = big > 0 && (int64_t)big * 16 <= b->dyn_lds ? big : 0;
+        if (np3 != b->sort_np[3]) { b->sort_np[3] = np3; grew = true; }
+    }
     // A class is solved island by island when its worlds are too few to fill
     // the chip (fewer than one per 32 CUs) and every one of them fits the
     // island solve's LDS; else one block per world.  One class at most (the
@@ -876,7 +890,7 @@ static void fork_classes(omi_batch* b, F launch_class, G after_top) {
 static void enqueue_sap(omi_batch* b, int32_t record) {
     const Dev& d = b->d;
     fork_classes(b, [&](int c, hipStream_t sc) {
-        launch_sap_sort(d, c, b->d_wl + (size_t)c * d.wcap, b->d_wn + c, b->np_class[c], b->sap_block[c],
+        launch_sap_sort(d, c, b->d_wl + (size_t)c * d.wcap, b->d_wn + c, b->sort_np[c], b->sap_block[c],
                         b->live[c], sc);
     }, [] {});
     launch_sap_pairs(d, record, b->sap_grid, b->stream);

@@ -288,24 +288,34 @@ __global__ __launch_bounds__(BLOCK) void k_sap_sort(Dev d, const int32_t* wlist,
     } else {
         key = (uint64_t*)smem; idx = (int32_t*)(key + np_max); coff = idx + np_max;
     }
-    for (int32_t t = tid; t < NP; t += BLOCK) {
-        if (t < n) { key[t] = order_key(BODY(d.lo, 0, off + t)); idx[t] = t; }
-        else { key[t] = ~0ull; idx[t] = 0x7fffffff; }
-    }
+    for (int32_t t = tid; t < n; t += BLOCK) { key[t] = order_key(BODY(d.lo, 0, off + t)); idx[t] = t; }
     __syncthreads();
-    for (int32_t k = 2; k <= NP; k <<= 1)
-        for (int32_t j = k >> 1; j > 0; j >>= 1) {
+    // Bitonic sort of (key, index) with every comparator ascending: each
+    // merge of size k opens with a mirror stage (i against its mirror in the
+    // block) and continues with half-cleaners.  Positions past n stand for
+    // +inf keys, which no ascending comparator moves, so the comparators that
+    // reach them are skipped and only n entries are held.
+    auto cmpx = [&](int32_t i, int32_t l) {
+        if (l >= n) return;
+        const uint64_t ki = key[i], kl = key[l];
+        const int32_t ii = idx[i], il = idx[l];
+        if (ki > kl || (ki == kl && ii > il)) { key[i] = kl; key[l] = ki; idx[i] = il; idx[l] = ii; }
+    };
+    for (int32_t k = 2; k <= NP; k <<= 1) {
+        const int32_t h = k >> 1;
+        for (int32_t t = tid; t < (NP >> 1); t += BLOCK) {
+            const int32_t b0 = (t / h) * k, o = t % h;
+            cmpx(b0 + o, b0 + k - 1 - o);
+        }
+        __syncthreads();
+        for (int32_t j = k >> 2; j > 0; j >>= 1) {
             for (int32_t t = tid; t < (NP >> 1); t += BLOCK) {
-                int32_t i = 2 * t - (t & (j - 1));
-                int32_t l = i + j;
-                uint64_t ki = key[i], kl = key[l];
-                int32_t ii = idx[i], il = idx[l];
-                bool gt = ki > kl || (ki == kl && ii > il);
-                bool up = (i & k) == 0;
-                if (gt == up) { key[i] = kl; key[l] = ki; idx[i] = il; idx[l] = ii; }
+                const int32_t i = 2 * t - (t & (j - 1));
+                cmpx(i, i + j);
             }
             __syncthreads();
         }
+    }
     int64_t part = 0;
     for (int32_t k = tid; k < n; k += BLOCK) {
         const int64_t g = off + idx[k];
@@ -3101,10 +3111,12 @@ static void sap_sort_one(const Dev& d, const int32_t* wl, const int32_t* wn, int
     else hipLaunchKernelGGL((k_sap_sort<B, false>), dim3(grid), dim3(B), (size_t)np * 16, s, d, wl, wn, np);
 }
 
-// The broadphase sort of size class c (LDS for c < 3, global scratch above).
+// The broadphase sort of size class c, in LDS or (np = 0) in global scratch.
 void launch_sap_sort(const Dev& d, int c, const int32_t* wl, const int32_t* wn, int32_t np, int32_t block,
                      int grid, hipStream_t s) {
-    const bool global = c == 3;
+    // np: entries each world's sort holds in LDS (16 bytes each), 0: the
+    // worlds' sorts run in global memory
+    const bool global = np <= 0;
     switch (block) {
         case 64: sap_sort_one<64>(d, wl, wn, np, global, grid, s); break;
         case 128: sap_sort_one<128>(d, wl, wn, np, global, grid, s); break;
