@@ -121,6 +121,49 @@ def window_bounds(steps, warmup):
     return first, t0, t0 + steps - 1, max(ROLLOUT, t0 + steps - 1)
 
 
+def profile_summary(path, last):
+    """Per-kernel figures of a rocprofv3 CSV output directory over each
+    kernel's last ``last`` dispatches (with ``--only-headline`` and a kernel
+    launched once per step: the timed window): from a kernel trace the mean
+    and total duration, from a counter collection the mean of each counter per
+    dispatch.  Returns markdown table lines."""
+    import csv
+    import glob
+    import re
+    lines = []
+    for f in sorted(glob.glob(os.path.join(path, '**', '*.csv'), recursive=True)):
+        with open(f) as fh:
+            rows = list(csv.DictReader(fh))
+        if not rows or 'Kernel_Name' not in rows[0]:
+            continue
+        short = lambda n: re.sub(r'\(.*', '', n).strip()
+        if 'Counter_Name' in rows[0]:
+            per = {}
+            for r in rows:
+                key = (short(r['Kernel_Name']), r['Counter_Name'])
+                d = per.setdefault(key, {})
+                d[int(r['Dispatch_Id'])] = d.get(int(r['Dispatch_Id']), 0.0) + float(r['Counter_Value'])
+            lines += ['', '%s: mean per dispatch over the last %d dispatches per kernel' % (os.path.basename(f), last),
+                      '', '| kernel | counter | dispatches | mean |', '|---|---|---|---|']
+            tab = []
+            for (k, c), d in per.items():
+                v = [d[i] for i in sorted(d)][-last:]
+                tab.append((k, c, len(v), sum(v) / len(v)))
+            for k, c, nd, m in sorted(tab, key=lambda t: -t[3]):
+                lines.append('| `%s` | %s | %d | %.1f |' % (k, c, nd, m))
+        elif 'Start_Timestamp' in rows[0]:
+            per = {}
+            for r in sorted(rows, key=lambda r: int(r['Start_Timestamp'])):
+                per.setdefault(short(r['Kernel_Name']), []).append(
+                    (int(r['End_Timestamp']) - int(r['Start_Timestamp'])) * 1e-3)
+            lines += ['', '%s: last %d dispatches per kernel' % (os.path.basename(f), last), '',
+                      '| kernel | dispatches | mean us | total us |', '|---|---|---|---|']
+            tab = sorted(((k, v[-last:]) for k, v in per.items()), key=lambda kv: -sum(kv[1]))
+            for k, v in tab:
+                lines.append('| `%s` | %d | %.1f | %.1f |' % (k, len(v), sum(v) / len(v), sum(v)))
+    return lines
+
+
 def rollout(arrays, precision, steps, warmup, timing=False, record=True):
     """Step one batch of ``arrays`` through the rollout; time the window.
 
@@ -344,23 +387,32 @@ def single_world(name, arrays, precision, steps, warmup, cpu):
     return out
 
 
-def single_worlds(steps, warmup, cpu):
+SINGLE = ('boxes1000', 'pyramids', 'convex_chains')
+
+
+def single_arrays(which):
+    """(description, arrays) of a single-world workload."""
     from omi_physics_amd import scenes, PhysicsWorld, model
-    res = {}
-    res['boxes1000_fp64'] = single_world('1000 unit boxes on a ground box (20 x 20 m), one world',
-                                         scenes.arrays(scenes.boxes_on_ground(1000, seed=1)), 'fp64', steps, warmup,
-                                         cpu)
-    pyr = scenes.arrays(scenes.pyramids_and_debris())
-    for prec in ('fp32', 'fp64'):
-        res['pyramids_%s' % prec] = single_world('20 ten-level box pyramids + 8000 box/sphere debris, one world',
-                                                 pyr, prec, steps, warmup, cpu and prec == 'fp64')
+    if which == 'boxes1000':
+        return ('1000 unit boxes on a ground box (20 x 20 m), one world',
+                scenes.arrays(scenes.boxes_on_ground(1000, seed=1)))
+    if which == 'pyramids':
+        return ('20 ten-level box pyramids + 8000 box/sphere debris, one world',
+                scenes.arrays(scenes.pyramids_and_debris()))
     t = time.perf_counter()
     cvx = scenes.build(scenes.convex_and_chains(), PhysicsWorld, model).export_arrays()
     log('convex-and-chains world built in %.1f s' % (time.perf_counter() - t))
-    for prec in ('fp32', 'fp64'):
-        res['convex_chains_%s' % prec] = single_world(
-            '2000 cooked convex hulls (16-32 points) + 20 hinge chains of 50 links, one world', cvx, prec, steps,
-            warmup, cpu and prec == 'fp64')
+    return '2000 cooked convex hulls (16-32 points) + 20 hinge chains of 50 links, one world', cvx
+
+
+def single_worlds(steps, warmup, cpu):
+    res = {}
+    name, arrays = single_arrays('boxes1000')
+    res['boxes1000_fp64'] = single_world(name, arrays, 'fp64', steps, warmup, cpu)
+    for which in ('pyramids', 'convex_chains'):
+        name, arrays = single_arrays(which)
+        for prec in ('fp32', 'fp64'):
+            res['%s_%s' % (which, prec)] = single_world(name, arrays, prec, steps, warmup, cpu and prec == 'fp64')
     return res
 
 
@@ -417,7 +469,10 @@ def main():
     ap.add_argument('--warmup', type=int, default=5, help='untimed steps right before the timed window')
     ap.add_argument('--worlds', type=int, default=4096)
     ap.add_argument('--seed', type=int, default=3)
-    ap.add_argument('--workload', default='batched4096', choices=['batched4096', 'copy'])
+    ap.add_argument('--workload', default='batched4096',
+                    choices=['batched4096', 'copy', 'boxes1000', 'pyramids', 'convex_chains'],
+                    help='the headline batch, a plain copy (counter calibration), or one single-world workload '
+                         '(FP64, its 300-step rollout, the last --steps steps timed)')
     ap.add_argument('--copy-mib', type=int, default=1024, help='buffer size of the copy workload')
     ap.add_argument('--plain-sample', type=int, default=48, help='worlds stepped alone for the plain form')
     ap.add_argument('--curve-sample', type=int, default=192, help='worlds of the FP32 error curves')
@@ -425,7 +480,12 @@ def main():
     ap.add_argument('--no-timing', action='store_true', help='skip the event-timed pass of the window')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-extras', action='store_true', help='skip serving and the single-world workloads')
+    ap.add_argument('--profile-summary', metavar='DIR',
+                    help='print per-kernel figures of a rocprofv3 CSV directory over the last --steps dispatches')
     args = ap.parse_args()
+    if args.profile_summary:
+        print('\n'.join(profile_summary(args.profile_summary, args.steps)))
+        return
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: every GPU box here has one.  The independent worlds would shard '
               'across GPUs by world id (greedy bin-packing by body count, no collective on the step path).',
@@ -437,6 +497,10 @@ def main():
         sys.exit(1)
     if args.workload == 'copy':
         copy_bench(args.copy_mib)
+        return
+    if args.workload in SINGLE:
+        name, arrays = single_arrays(args.workload)
+        print(json.dumps(single_world(name, arrays, 'fp64', args.steps, args.warmup, not args.no_cpu)), flush=True)
         return
 
     arrays = make_worlds(args.worlds, args.seed)
