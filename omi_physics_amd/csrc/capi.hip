@@ -442,8 +442,15 @@ static int32_t get(omi_batch* b, T* dst, const T* src, int64_t n) {
     return OMI_OK;
 }
 
+// The executable graph is destroyed only after its launches on the batch
+// stream have finished (a group steps its members without synchronising, and
+// rebuilding the lists between group steps re-instantiates the graph).
 static void invalidate_graph(omi_batch* b) {
-    if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+    if (b->graph) {
+        hipStreamSynchronize(b->stream);
+        hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
 }
 
 extern "C" int32_t omi_batch_remove_world(omi_batch* b, int32_t slot) {

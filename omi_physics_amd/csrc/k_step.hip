@@ -1387,11 +1387,11 @@ __device__ __forceinline__ void scatter_items(int32_t* bound, int32_t SL, int32_
 // registers the loop carries: the compiler's wait for such a load, placed at
 // every level, would also wait for the next level's item loads and expose
 // their latency.
-template <typename R, int BLOCK, bool LIGHT, bool POS, bool LB>
+template <typename R, int BLOCK, bool LIGHT, bool POS, bool LB, bool ISL>
 __device__ __forceinline__ void run_levels(const Dev& d, const SolveIn& in, const SolveState<R>& st, size_t bound_off,
                            int32_t K, int32_t D, int32_t NC, int32_t S, const int32_t* gbound,
                            const int32_t* corder, const int32_t* cord_a, const int32_t* cord_b,
-                           const int32_t* cord_c) {
+                           const int32_t* cord_c, int32_t cs) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     const int tid = threadIdx.x;
     const bool fp64 = sizeof(R) == sizeof(double);
@@ -1407,16 +1407,21 @@ __device__ __forceinline__ void run_levels(const Dev& d, const SolveIn& in, cons
     // velocity c of side body i (global index) with compact id ci_
 #define IMP(c, ci) (LIGHT ? lP[(size_t)it.k * 3 + (size_t)(c)] \
                           : st.P[(size_t)((ci) - st.pb) * st.ps + (size_t)(c) * st.pc])
+    // an island's item carries its batch contact; a world's is cs + its contact
     struct IdRec { int32_t code, a, b, c; };
     auto fetch_id = [&](int32_t slot, IdRec& r) {
-        if (slot < NC) { r.code = corder[slot]; r.a = cord_a[slot]; r.b = cord_b[slot]; r.c = cord_c[slot]; }
+        if (slot < NC) {
+            r.code = corder[slot]; r.a = cord_a[slot]; r.b = cord_b[slot];
+            if constexpr (ISL) r.c = cord_c[slot];
+        }
     };
     auto fetch_item = [&](const IdRec& r, SolveItem<R>& it) {
         it.s = r.code & 31;
         it.da = (r.code >> 6) & 1;
         it.db = (r.code >> 5) & 1;
         it.k = r.code >> 7;
-        it.ci = r.c;
+        if constexpr (ISL) it.ci = r.c;
+        else it.ci = cs + it.k;
         it.a = r.a; it.b = r.b;
         if (LIGHT) { it.cida = lmap[r.a - st.off]; it.cidb = lmap[r.b - st.off]; }
         // exactly the values the mode reads (the velocity passes stop short of
@@ -1688,10 +1693,10 @@ __device__ __forceinline__ void run_levels(const Dev& d, const SolveIn& in, cons
 }
 
 // run_levels with the bounds' place chosen at run time
-template <typename R, int BLOCK, bool LIGHT, bool POS, typename... A>
+template <typename R, int BLOCK, bool LIGHT, bool POS, bool ISL, typename... A>
 __device__ __forceinline__ void run_levels_lb(bool lds_bounds, A&&... a) {
-    if (lds_bounds) run_levels<R, BLOCK, LIGHT, POS, true>(a...);
-    else run_levels<R, BLOCK, LIGHT, POS, false>(a...);
+    if (lds_bounds) run_levels<R, BLOCK, LIGHT, POS, true, ISL>(a...);
+    else run_levels<R, BLOCK, LIGHT, POS, false, ISL>(a...);
 }
 
 // LDS of the per-body position pass for a world of n bodies, K contacts and M
@@ -2039,10 +2044,10 @@ __global__ __launch_bounds__(BLOCK) void k_islands(Dev d, const int32_t* wlist, 
     }
 }
 
-template <typename R, int BLOCK>
+template <typename R, int BLOCK, bool ISL>
 __device__ __forceinline__ void solve_unit(const Dev& d, const SolveIn& in, int32_t lds_bytes, int32_t w, int32_t K,
-                                           int32_t M, const int4* mrec, int32_t cl, const int32_t* gmap, int64_t kb,
-                                           bool isl) {
+                                           int32_t M, const int4* mrec, int32_t cl, const int32_t* gmap, int64_t kb) {
+    constexpr bool isl = ISL;
     extern __shared__ __align__(16) unsigned char s_dyn[];
     __shared__ int32_t s_scan[BLOCK / 64 + 2];
     __shared__ int32_t s_D, s_D0, s_nd;
@@ -2208,11 +2213,11 @@ __device__ __forceinline__ void solve_unit(const Dev& d, const SolveIn& in, int3
     // bound[l] is now the end of level l
     if (S > 0) {
         if (state_lds)
-            run_levels_lb<R, BLOCK, true, false>(lds_bounds, d, in, st, avail, K, D, NC, S, g_bound, corder,
-                                                 cord_a, cord_b, cord_c);
+            run_levels_lb<R, BLOCK, true, false, ISL>(lds_bounds, d, in, st, avail, K, D, NC, S, g_bound, corder,
+                                                      cord_a, cord_b, cord_c, cl);
         else
-            run_levels_lb<R, BLOCK, false, false>(lds_bounds, d, in, st, avail, K, D, NC, S, g_bound, corder,
-                                                  cord_a, cord_b, cord_c);
+            run_levels_lb<R, BLOCK, false, false, ISL>(lds_bounds, d, in, st, avail, K, D, NC, S, g_bound, corder,
+                                                       cord_a, cord_b, cord_c, cl);
     }
     __syncthreads();
 
@@ -2269,8 +2274,8 @@ __device__ __forceinline__ void solve_unit(const Dev& d, const SolveIn& in, int3
                     for (int q = 0; q < 3; q++) P[(size_t)q * nd + c] = fp64 ? (R)BODY(d.pos, q, off + i) : R(0);
                 }
                 __syncthreads();
-                run_levels_lb<R, BLOCK, true, true>(lds_bounds, d, in, st, avail, K, D0, NC, S, g_bound, corder,
-                                                    cord_a, cord_b, cord_c);
+                run_levels_lb<R, BLOCK, true, true, ISL>(lds_bounds, d, in, st, avail, K, D0, NC, S, g_bound, corder,
+                                                         cord_a, cord_b, cord_c, cl);
                 __syncthreads();
                 for (int32_t i = tid; i < n; i += BLOCK) {
                     if (!touched(i)) continue;
@@ -2289,35 +2294,36 @@ __device__ __forceinline__ void solve_unit(const Dev& d, const SolveIn& in, int3
                     }
                 }
             } else {
-                run_levels_lb<R, BLOCK, false, true>(lds_bounds, d, in, st, avail, K, D0, NC, S, g_bound, corder,
-                                                     cord_a, cord_b, cord_c);
+                run_levels_lb<R, BLOCK, false, true, ISL>(lds_bounds, d, in, st, avail, K, D0, NC, S, g_bound, corder,
+                                                          cord_a, cord_b, cord_c, cl);
             }
         }
     }
 }
 
 
-// The solver of a size class: one world per block, or (isl) the class's
-// islands, each block taking islands in turn from the island table.
-template <typename R, int BLOCK>
+// The solver of a size class: one world per block, or (ISL) the class's
+// islands, each block taking islands in turn from the island table.  Two
+// instantiations, so the per-world kernel holds no register for the islands.
+template <typename R, int BLOCK, bool ISL>
 __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_t* wlist, const int32_t* wcount,
-                                                 int32_t lds_bytes, int32_t isl) {
-    if (!isl) {
+                                                 int32_t lds_bytes) {
+    if constexpr (!ISL) {
         if ((int32_t)blockIdx.x >= *wcount) return;
         const int32_t w = wlist[blockIdx.x];
         if (w >= d.wcap || !(d.w_flags[w] & WF_ACTIVE)) return;
         const int32_t K = d.w_cn[w], cs = d.w_cs[w];
         if (K == 0) { if (threadIdx.x == 0) d.w_levels[w] = 0; return; }
-        solve_unit<R, BLOCK>(d, in, lds_bytes, w, K, d.w_mn[w], d.mrec + d.w_ms[w], cs, nullptr, cs, false);
-        return;
-    }
+        solve_unit<R, BLOCK, false>(d, in, lds_bytes, w, K, d.w_mn[w], d.mrec + d.w_ms[w], cs, nullptr, cs);
+    } else {
     const uint64_t ni = d.cnt[C_ISLANDS];
     const int32_t NIs = (int32_t)(ni < (uint64_t)d.kcap ? ni : (uint64_t)d.kcap);
     for (int32_t ii = blockIdx.x; ii < NIs; ii += gridDim.x) {
         const int32_t* ds = d.isl_desc + (size_t)ii * ISL_DESC;
         const int32_t w = ds[0], M = ds[1], K = ds[2], mb = ds[3], kbase = ds[4];
-        solve_unit<R, BLOCK>(d, in, lds_bytes, w, K, M, d.isl_mrec + mb, 0, d.isl_cidx + kbase, kbase, true);
+        solve_unit<R, BLOCK, true>(d, in, lds_bytes, w, K, M, d.isl_mrec + mb, 0, d.isl_cidx + kbase, kbase);
         __syncthreads();
+    }
     }
 }
 
@@ -3171,7 +3177,8 @@ void launch_wake(const Dev& d, hipStream_t s) {
 template <typename R, int B>
 static void solve_one(const Dev& d, SolveIn in, const int32_t* wl, const int32_t* wn, int32_t lds_bytes,
                       int grid, hipStream_t s, int32_t isl) {
-    hipLaunchKernelGGL((k_solve<R, B>), dim3(grid), dim3(B), (size_t)lds_bytes, s, d, in, wl, wn, lds_bytes, isl);
+    if (isl) hipLaunchKernelGGL((k_solve<R, B, true>), dim3(grid), dim3(B), (size_t)lds_bytes, s, d, in, wl, wn, lds_bytes);
+    else hipLaunchKernelGGL((k_solve<R, B, false>), dim3(grid), dim3(B), (size_t)lds_bytes, s, d, in, wl, wn, lds_bytes);
 }
 
 template <typename R>
@@ -3204,7 +3211,9 @@ static bool allow_lds_fn(F f, int bytes) {
 }
 
 template <typename R, int B>
-static bool allow_lds_one(int bytes) { return allow_lds_fn(k_solve<R, B>, bytes); }
+static bool allow_lds_one(int bytes) {
+    return allow_lds_fn(k_solve<R, B, false>, bytes) && allow_lds_fn(k_solve<R, B, true>, bytes);
+}
 
 bool allow_dynamic_lds(int bytes) {
     bool ok = true;
