@@ -253,9 +253,14 @@ typedef struct omi_batch omi_batch;
 
 int32_t omi_batch_create(const omi_batch_config *cfg, omi_batch **out);
 int32_t omi_batch_destroy(omi_batch *b);
-/* Place (or replace) a world in `slot`; resets its warm-start cache. */
+/* Place (or replace) a world in `slot`; resets its warm-start cache.  The
+ * device writes are ordered on the batch stream after the queued steps (one
+ * transfer on a side stream overlapping them, one copy launch): the call does
+ * not wait for the device, the descriptor's arrays may be freed on return,
+ * and steps queued afterwards include the world. */
 int32_t omi_batch_load_world(omi_batch *b, int32_t slot, const omi_world_desc *d);
-/* Free `slot`; its bodies stop stepping and their memory is reusable. */
+/* Free `slot` (ordered the same way): steps queued afterwards skip its bodies,
+ * and its memory is reusable at once by a later load. */
 int32_t omi_batch_remove_world(omi_batch *b, int32_t slot);
 /* Overwrite the mutable state of a loaded world (same body count). */
 int32_t omi_batch_set_state(omi_batch *b, int32_t slot, const double *pos, const double *quat,
@@ -311,6 +316,26 @@ int32_t omi_batch_refit(omi_batch *b, double margin);
 int32_t omi_batch_sync(omi_batch *b);
 /* Copy one world's state out (any pointer may be NULL). */
 int32_t omi_batch_read_world(omi_batch *b, int32_t slot, double *pos, double *quat, double *lv,
+                             double *av, uint8_t *awake, double *sleep_timer, double *aabb_min,
+                             double *aabb_max, double *prev_pos, double *prev_quat);
+/* Asynchronous readback (batch layer, no counterpart in the reference): gather
+ * one world's state, as omi_batch_read_world returns it, after the queued
+ * work, into pinned host memory of the batch's own; returns at once with a
+ * ticket.  A server retiring a world fetches it, removes it and admits the
+ * next world without waiting for the device. */
+int32_t omi_batch_fetch_world(omi_batch *b, int32_t slot, int64_t *ticket);
+/* A marker after the queued work (asynchronous); with omi_batch_mark_wait a
+ * host keeps one step queued while it waits for the one before. */
+int32_t omi_batch_mark(omi_batch *b, int64_t *ticket);
+/* Wait until the work queued before the marker has run; reports a capacity
+ * overflow of the steps up to it as omi_batch_sync does (the flags stay set
+ * until omi_batch_sync clears them); the ticket is spent. */
+int32_t omi_batch_mark_wait(omi_batch *b, int64_t ticket);
+/* 1 when the fetch has arrived, 0 while it is in flight (negative: error). */
+int32_t omi_batch_fetch_ready(omi_batch *b, int64_t ticket);
+/* Wait for the fetch and copy it out (any pointer may be NULL; the layouts of
+ * omi_batch_read_world); the ticket is spent. */
+int32_t omi_batch_fetch_wait(omi_batch *b, int64_t ticket, double *pos, double *quat, double *lv,
                              double *av, uint8_t *awake, double *sleep_timer, double *aabb_min,
                              double *aabb_max, double *prev_pos, double *prev_quat);
 /* Pose snapshot of the worlds slots[0..nslots), packed world after world:

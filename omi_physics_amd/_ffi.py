@@ -73,7 +73,9 @@ EXPORTS = (
     'omi_batch_set_state', 'omi_batch_set_aabbs', 'omi_batch_set_params', 'omi_batch_read_cache',
     'omi_batch_write_cache', 'omi_batch_collide', 'omi_batch_read_joint_anchors',
     'omi_batch_prepare', 'omi_batch_step', 'omi_batch_refit', 'omi_batch_sync',
-    'omi_batch_read_world', 'omi_batch_snapshot', 'omi_batch_read_contacts', 'omi_batch_read_pairs',
+    'omi_batch_read_world', 'omi_batch_fetch_world', 'omi_batch_fetch_ready', 'omi_batch_fetch_wait',
+    'omi_batch_mark', 'omi_batch_mark_wait',
+    'omi_batch_snapshot', 'omi_batch_read_contacts', 'omi_batch_read_pairs',
     'omi_batch_read_trigger_overlaps',
     'omi_batch_counters', 'omi_batch_set_timing', 'omi_batch_kernel_times',
     'omi_batch_kernel_names', 'omi_batch_use_graph', 'omi_batch_set_blocks', 'omi_batch_world_stats',
@@ -146,6 +148,11 @@ def load() -> ctypes.CDLL:
         'omi_batch_refit': (ctypes.c_int32, [_vp, ctypes.c_double]),
         'omi_batch_sync': (ctypes.c_int32, [_vp]),
         'omi_batch_read_world': (ctypes.c_int32, [_vp, ctypes.c_int32, _d, _d, _d, _d, _u8, _d, _d, _d, _d, _d]),
+        'omi_batch_fetch_world': (ctypes.c_int32, [_vp, ctypes.c_int32, _i64]),
+        'omi_batch_fetch_ready': (ctypes.c_int32, [_vp, ctypes.c_int64]),
+        'omi_batch_mark': (ctypes.c_int32, [_vp, _i64]),
+        'omi_batch_mark_wait': (ctypes.c_int32, [_vp, ctypes.c_int64]),
+        'omi_batch_fetch_wait': (ctypes.c_int32, [_vp, ctypes.c_int64, _d, _d, _d, _d, _u8, _d, _d, _d, _d, _d]),
         'omi_batch_snapshot': (ctypes.c_int32, [_vp, ctypes.c_int32, _i32, _i64, _d, _d, _u8, _u8]),
         'omi_batch_read_trigger_overlaps': (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_int64, _i64, _i32, _i32]),
         'omi_batch_read_contacts': (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_int64, _i64, _i32, _i32,

@@ -208,6 +208,45 @@ class Batch:
             ptr(out['prev_position'], _d), ptr(out['prev_orientation'], _d)))
         return out
 
+    def mark(self) -> int:
+        """A marker after the queued work (asynchronous); see :meth:`mark_wait`."""
+        t = np.zeros(1, np.int64)
+        check(self.lib.omi_batch_mark(self._h, ptr(t, _i64)))
+        return int(t[0])
+
+    def mark_wait(self, ticket: int) -> None:
+        """Wait for the work queued before the marker; raises on a capacity overflow
+        of those steps (the flags stay set until :meth:`sync`)."""
+        check(self.lib.omi_batch_mark_wait(self._h, int(ticket)))
+
+    def fetch(self, slot: int):
+        """Queue a copy of one world's state after the queued steps; returns a ticket
+        for :meth:`fetched` (the device is not waited for)."""
+        t = np.zeros(1, np.int64)
+        check(self.lib.omi_batch_fetch_world(self._h, int(slot), ptr(t, _i64)))
+        return int(t[0]), self.sizes[int(slot)]
+
+    def fetch_ready(self, ticket) -> bool:
+        r = int(self.lib.omi_batch_fetch_ready(self._h, int(ticket[0])))
+        if r < 0:
+            check(r)
+        return r == 1
+
+    def fetched(self, ticket) -> Dict[str, np.ndarray]:
+        """The state a :meth:`fetch` copied (waits for it; the layout of :meth:`read`)."""
+        t, n = ticket
+        out = dict(position=np.empty((n, 3)), orientation=np.empty((n, 4)),
+                   linear_velocity=np.empty((n, 3)), angular_velocity=np.empty((n, 3)),
+                   awake=np.empty(n, np.uint8), sleep_timer=np.empty(n),
+                   aabb_min=np.empty((n, 3)), aabb_max=np.empty((n, 3)),
+                   prev_position=np.empty((n, 3)), prev_orientation=np.empty((n, 4)))
+        check(self.lib.omi_batch_fetch_wait(
+            self._h, int(t), ptr(out['position'], _d), ptr(out['orientation'], _d),
+            ptr(out['linear_velocity'], _d), ptr(out['angular_velocity'], _d), ptr(out['awake'], _u8),
+            ptr(out['sleep_timer'], _d), ptr(out['aabb_min'], _d), ptr(out['aabb_max'], _d),
+            ptr(out['prev_position'], _d), ptr(out['prev_orientation'], _d)))
+        return out
+
     def snapshot(self, slots: Optional[Sequence[int]] = None) -> List[tuple]:
         """Pose snapshots ``(position (n,3), axis_angle (n,4), awake (n,), dynamic (n,))``
         of ``slots`` (default: every loaded world), in the reference's snapshot format

@@ -37,6 +37,7 @@ void launch_post(const Dev& d, int64_t nhw, double dt, hipStream_t s);
 void launch_joints(const Dev& d, int precision, double dt, int32_t lds_bytes, hipStream_t s);
 void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s, double lin, double ang, double tsleep);
 void launch_toggle(const Dev& d, hipStream_t s);
+void launch_copy_segments(const CopySeg* segs, int32_t nseg, hipStream_t s);
 void launch_trigger(const Dev& d, hipStream_t s);
 void launch_capsule_triangles(int64_t n, const double* c0, const double* c1, double radius, const double* verts,
                               uint8_t* hit, double* points, double* normals, double* depths, hipStream_t s);
@@ -71,6 +72,8 @@ static int32_t fail(int32_t code, const std::string& msg) {
     g_err = msg;
     return code;
 }
+
+static int32_t capacity_error(uint64_t ov);
 
 #define HIPCHK(x)                                                                                    \
     do {                                                                                             \
@@ -173,6 +176,35 @@ struct omi_batch {
     int32_t* snap_wl = nullptr;
     int64_t* snap_woff = nullptr;
     int32_t snap_wcap = 0;
+    // Staged device writes (world loads and removals, world lists): pending
+    // segments and their data on the host, then at a flush one transfer from
+    // a pinned arena on the upload stream (it overlaps work queued on the
+    // batch stream) into device staging, and one k_copy_segments launch
+    // ordered on the batch stream.  Arena space is reused once the last
+    // launch that read it has run.
+    std::vector<CopySeg> seg;              // pending
+    std::vector<uint8_t> seg_rel;          // 1: src is an offset into seg_data
+    size_t seg_group = 0;                  // first segment of the open group
+    std::vector<int32_t> seg_cuts;         // group ends: one launch per group, in order
+    std::vector<unsigned char> seg_data;
+    unsigned char* stage_host = nullptr;   // pinned
+    unsigned char* stage_dev = nullptr;
+    int64_t stage_cap = 0, stage_used = 0;
+    hipStream_t up_stream = nullptr;
+    hipEvent_t ev_up = nullptr, ev_staged = nullptr;
+    // Readback of worlds (omi_batch_fetch_world): gathered on the device into
+    // staging, one transfer into a pinned arena; tickets until collected.
+    // Each fetch holds a buffer pair (pinned host, device) from a pool until
+    // its ticket is collected.
+    struct FetchBuf { int64_t cap = 0; unsigned char* host = nullptr; unsigned char* dev = nullptr; };
+    struct Fetch { int64_t n = 0; int32_t buf = -1; hipEvent_t ev = nullptr; bool live = false; };
+    std::vector<FetchBuf> fbufs;
+    std::vector<uint8_t> fbuf_busy;
+    std::vector<Fetch> fetches;
+    // step markers (omi_batch_mark): an event and the overflow flags copied
+    // at that point into a pinned slot
+    struct Mark { hipEvent_t ev = nullptr; uint64_t* flags = nullptr; bool live = false; };
+    std::vector<Mark> marks;
 
     template <typename T>
     int32_t alloc(T** p, int64_t count) {
@@ -213,6 +245,27 @@ static int32_t check_device() {
 
 static void free_all(omi_batch* b) {
     if (b->graph) hipGraphExecDestroy(b->graph);
+    if (b->up_stream) hipStreamSynchronize(b->up_stream);
+    if (b->stage_host) hipHostFree(b->stage_host);
+    if (b->stage_dev) hipFree(b->stage_dev);
+    b->stage_host = b->stage_dev = nullptr;
+    for (auto& f : b->fbufs) {
+        if (f.host) hipHostFree(f.host);
+        if (f.dev) hipFree(f.dev);
+    }
+    b->fbufs.clear();
+    for (auto& f : b->fetches) if (f.ev) hipEventDestroy(f.ev);
+    b->fetches.clear();
+    for (auto& m : b->marks) {
+        if (m.ev) hipEventDestroy(m.ev);
+        if (m.flags) hipHostFree(m.flags);
+    }
+    b->marks.clear();
+    if (b->ev_up) hipEventDestroy(b->ev_up);
+    if (b->ev_staged) hipEventDestroy(b->ev_staged);
+    if (b->up_stream) hipStreamDestroy(b->up_stream);
+    b->ev_up = b->ev_staged = nullptr;
+    b->up_stream = nullptr;
     if (b->snap_dev) hipFree(b->snap_dev);
     if (b->snap_host) hipHostFree(b->snap_host);
     if (b->snap_wl) hipFree(b->snap_wl);
@@ -378,6 +431,9 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
         HIPCHK(hipMemcpy(d.world, m1.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
     }
     HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&b->up_stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&b->ev_up, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&b->ev_staged, hipEventDisableTiming));
     // the size classes' branch streams are created once a second class holds
     // worlds: a batch of one class (a member of a group, say) then holds a
     // single stream, and so a single hardware queue
@@ -442,6 +498,121 @@ static int32_t get(omi_batch* b, T* dst, const T* src, int64_t n) {
     return OMI_OK;
 }
 
+// ---- staged device writes --------------------------------------------------
+
+static int64_t al16h(int64_t x) { return (x + 15) & ~(int64_t)15; }
+
+// The segments of one launch are copied by concurrent workgroups in no order,
+// so a write that overlaps an earlier pending write of the open group closes
+// the group: groups are launched one after another.
+static void st_order(omi_batch* b, uint64_t dst, int64_t bytes) {
+    for (size_t i = b->seg_group; i < b->seg.size(); i++) {
+        const CopySeg& g = b->seg[i];
+        const uint64_t e = g.dst + (uint64_t)(g.n * g.es);
+        if (dst < e && g.dst < dst + (uint64_t)bytes) {
+            b->seg_cuts.push_back((int32_t)b->seg.size());
+            b->seg_group = b->seg.size();
+            return;
+        }
+    }
+}
+
+// n elements of src to dst (a device address inside the batch)
+template <typename T>
+static void st_copy(omi_batch* b, T* dst, const T* src, int64_t n) {
+    if (n <= 0) return;
+    st_order(b, (uint64_t)(uintptr_t)dst, (int64_t)sizeof(T) * n);
+    const int64_t off = al16h((int64_t)b->seg_data.size());
+    b->seg_data.resize((size_t)(off + (int64_t)sizeof(T) * n));
+    std::memcpy(b->seg_data.data() + off, src, sizeof(T) * (size_t)n);
+    b->seg.push_back(CopySeg{(uint64_t)(uintptr_t)dst, (uint64_t)off, n, (int32_t)sizeof(T), 0});
+    b->seg_rel.push_back(1);
+}
+
+// n elements from one device address of the batch to another
+template <typename T>
+static void st_dcopy(omi_batch* b, T* dst, const T* src, int64_t n) {
+    if (n <= 0) return;
+    st_order(b, (uint64_t)(uintptr_t)dst, (int64_t)sizeof(T) * n);
+    b->seg.push_back(CopySeg{(uint64_t)(uintptr_t)dst, (uint64_t)(uintptr_t)src, n, (int32_t)sizeof(T), 0});
+    b->seg_rel.push_back(0);
+}
+
+// host [n][k] rows -> device component-major rows at body offset `off`
+template <typename T>
+static void st_soa(omi_batch* b, T* dst, int64_t ldd, int64_t off, const T* src, int64_t n, int k) {
+    if (n <= 0) return;
+    std::vector<T> tmp((size_t)n);
+    for (int c = 0; c < k; c++) {
+        for (int64_t i = 0; i < n; i++) tmp[(size_t)i] = src[i * k + c];
+        st_copy(b, dst + (size_t)c * ldd + off, tmp.data(), n);
+    }
+}
+
+template <typename T>
+static void st_fill(omi_batch* b, T* dst, T v, int64_t n) {
+    if (n <= 0) return;
+    st_order(b, (uint64_t)(uintptr_t)dst, (int64_t)sizeof(T) * n);
+    uint64_t bits = 0;
+    std::memcpy(&bits, &v, sizeof(T));
+    b->seg.push_back(CopySeg{(uint64_t)(uintptr_t)dst, bits, n, (int32_t)sizeof(T), 1});
+    b->seg_rel.push_back(0);
+}
+
+// Enqueue the pending segments: one host-to-device transfer on the upload
+// stream, then one copy launch on the batch stream after it (so after every
+// step queued before and before every step queued after).  Returns without
+// waiting.
+static int32_t st_flush(omi_batch* b) {
+    if (b->seg.empty()) return OMI_OK;
+    const int64_t nseg = (int64_t)b->seg.size();
+    const int64_t tab = al16h(nseg * (int64_t)sizeof(CopySeg));
+    const int64_t need = tab + al16h((int64_t)b->seg_data.size());
+    // arena space is free again once the last copy launch has run
+    if (b->stage_used > 0 && hipEventQuery(b->ev_staged) == hipSuccess) b->stage_used = 0;
+    if (b->stage_used + need > b->stage_cap) {
+        HIPCHK(hipEventSynchronize(b->ev_staged));
+        b->stage_used = 0;
+    }
+    if (need > b->stage_cap) {
+        HIPCHK(hipStreamSynchronize(b->up_stream));
+        int64_t cap = std::max<int64_t>(1 << 20, b->stage_cap);
+        while (cap < need) cap <<= 1;
+        if (b->stage_host) HIPCHK(hipHostFree(b->stage_host));
+        if (b->stage_dev) HIPCHK(hipFree(b->stage_dev));
+        b->stage_host = b->stage_dev = nullptr;
+        b->stage_cap = 0;
+        HIPCHK(hipHostMalloc((void**)&b->stage_host, (size_t)cap, hipHostMallocDefault));
+        HIPCHK(hipMalloc((void**)&b->stage_dev, (size_t)cap));
+        b->stage_cap = cap;
+    }
+    unsigned char* h = b->stage_host + b->stage_used;
+    unsigned char* dv = b->stage_dev + b->stage_used;
+    for (size_t i = 0; i < b->seg.size(); i++)
+        if (b->seg_rel[i]) b->seg[i].src = (uint64_t)(uintptr_t)(dv + tab + (int64_t)b->seg[i].src);
+    std::memcpy(h, b->seg.data(), sizeof(CopySeg) * (size_t)nseg);
+    if (!b->seg_data.empty()) std::memcpy(h + tab, b->seg_data.data(), b->seg_data.size());
+    std::vector<int32_t> cuts;
+    cuts.swap(b->seg_cuts);
+    cuts.push_back((int32_t)nseg);
+    b->seg.clear();
+    b->seg_rel.clear();
+    b->seg_data.clear();
+    b->seg_group = 0;
+    HIPCHK(hipMemcpyAsync(dv, h, (size_t)need, hipMemcpyHostToDevice, b->up_stream));
+    HIPCHK(hipEventRecord(b->ev_up, b->up_stream));
+    HIPCHK(hipStreamWaitEvent(b->stream, b->ev_up, 0));
+    int32_t first = 0;
+    for (int32_t cut : cuts) {
+        launch_copy_segments((const CopySeg*)dv + first, cut - first, b->stream);
+        first = cut;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(b->ev_staged, b->stream));
+    b->stage_used += need;
+    return OMI_OK;
+}
+
 // The executable graph is destroyed only after its launches on the batch
 // stream have finished (a group steps its members without synchronising, and
 // rebuilding the lists between group steps re-instantiates the graph).
@@ -453,18 +624,16 @@ static void invalidate_graph(omi_batch* b) {
     }
 }
 
-extern "C" int32_t omi_batch_remove_world(omi_batch* b, int32_t slot) {
-    if (!b || slot < 0 || slot >= b->cfg.max_worlds) return fail(OMI_ERR_ARG, "bad world slot");
+// A world's removal as staged writes (its bodies' world ids to -1, its flags
+// and cache counts to zero) with its host ranges released; the caller flushes.
+static void remove_staged(omi_batch* b, int32_t slot) {
     WorldSlot& s = b->slots[slot];
-    if (!s.live) return OMI_OK;
-    HIPCHK(hipStreamSynchronize(b->stream));
-    std::vector<int32_t> m1((size_t)s.n, -1);
-    int32_t rc;
-    if ((rc = put(b, b->d.world + s.off, m1.data(), s.n))) return rc;
-    int32_t zero = 0;
-    if ((rc = put(b, b->d.w_flags + slot, &zero, 1))) return rc;
-    if ((rc = put(b, b->d.w_cache_n + slot, &zero, 1))) return rc;
-    if ((rc = put(b, b->d.w_cache_n + b->cfg.max_worlds + slot, &zero, 1))) return rc;
+    if (!s.live) return;
+    const int32_t W = b->cfg.max_worlds;
+    st_fill<int32_t>(b, b->d.world + s.off, -1, s.n);
+    st_fill<int32_t>(b, b->d.w_flags + slot, 0, 1);
+    st_fill<int32_t>(b, b->d.w_cache_n + slot, 0, 1);
+    st_fill<int32_t>(b, b->d.w_cache_n + W + slot, 0, 1);
     b->bodies.release(s.off, s.n);
     b->mats.release(s.mat_off, s.mat_n);
     b->filts.release(s.filt_off, s.filt_n);
@@ -472,7 +641,16 @@ extern "C" int32_t omi_batch_remove_world(omi_batch* b, int32_t slot) {
     b->joints.release(s.j_off, s.j_n);
     s = WorldSlot();
     b->lists_dirty = true;
-    return OMI_OK;
+}
+
+// Removal and load are ordered on the batch stream (staged writes, see
+// st_flush): neither waits for queued steps, and a step queued after either
+// call sees the change.
+extern "C" int32_t omi_batch_remove_world(omi_batch* b, int32_t slot) {
+    if (!b || slot < 0 || slot >= b->cfg.max_worlds) return fail(OMI_ERR_ARG, "bad world slot");
+    if (!b->slots[slot].live) return OMI_OK;
+    remove_staged(b, slot);
+    return st_flush(b);
 }
 
 extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_world_desc* w) {
@@ -497,11 +675,13 @@ extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_wo
         if (a < -1 || a >= w->n || c < -1 || c >= w->n || (a < 0 && c < 0) || a == c)
             return fail(OMI_ERR_ARG, "joint body index out of range");
     }
-    int32_t rc = omi_batch_remove_world(b, slot);
-    if (rc) return rc;
+    remove_staged(b, slot);
     WorldSlot s;
     s.off = b->bodies.alloc(n);
-    if (s.off < 0) return fail(OMI_ERR_CAPACITY, "not enough free body capacity in the batch");
+    if (s.off < 0) {
+        st_flush(b);                          // the previous world's removal
+        return fail(OMI_ERR_CAPACITY, "not enough free body capacity in the batch");
+    }
     s.n = n;
     s.mat_n = (int64_t)w->n_mat * w->n_mat;
     s.filt_n = (int64_t)w->n_filt * w->n_filt;
@@ -516,6 +696,7 @@ extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_wo
         if (s.mat_off >= 0) b->mats.release(s.mat_off, s.mat_n);
         if (s.filt_off >= 0) b->filts.release(s.filt_off, s.filt_n);
         if (s.hull_off >= 0) b->hulls.release(s.hull_off, s.hull_n);
+        st_flush(b);
         return fail(OMI_ERR_CAPACITY, "joint capacity of the batch is full");
     }
     if (s.mat_off < 0 || s.filt_off < 0 || s.hull_off < 0) {
@@ -524,26 +705,26 @@ extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_wo
         if (s.mat_off >= 0) b->mats.release(s.mat_off, s.mat_n);
         if (s.filt_off >= 0) b->filts.release(s.filt_off, s.filt_n);
         if (s.hull_off >= 0) b->hulls.release(s.hull_off, s.hull_n);
+        st_flush(b);
         return fail(OMI_ERR_CAPACITY, "table or hull pool of the batch is full");
     }
-    HIPCHK(hipStreamSynchronize(b->stream));
     Dev& d = b->d;
     const int64_t N = d.ncap;
-    if ((rc = upload_soa(b, d.pos, N, s.off, w->position, n, 3))) return rc;
-    if ((rc = upload_soa(b, d.quat, N, s.off, w->orientation, n, 4))) return rc;
-    if ((rc = upload_soa(b, d.ppos, N, s.off, w->prev_position ? w->prev_position : w->position, n, 3))) return rc;
-    if ((rc = upload_soa(b, d.pquat, N, s.off, w->prev_orientation ? w->prev_orientation : w->orientation, n, 4))) return rc;
-    if ((rc = upload_soa(b, d.lv, N, s.off, w->linear_velocity, n, 3))) return rc;
-    if ((rc = upload_soa(b, d.av, N, s.off, w->angular_velocity, n, 3))) return rc;
-    if ((rc = upload_soa(b, d.half, N, s.off, w->half, n, 3))) return rc;
-    if ((rc = upload_soa(b, d.inv_inertia, N, s.off, w->inv_inertia, n, 3))) return rc;
-    if ((rc = put(b, d.inv_mass + s.off, w->inv_mass, n))) return rc;
-    if ((rc = put(b, d.gf + s.off, w->gravity_factor, n))) return rc;
-    if ((rc = put(b, d.ld + s.off, w->linear_damping, n))) return rc;
-    if ((rc = put(b, d.ad + s.off, w->angular_damping, n))) return rc;
-    if ((rc = put(b, d.qd + s.off, w->quadratic_drag, n))) return rc;
-    if ((rc = put(b, d.timer + s.off, w->sleep_timer, n))) return rc;
-    if ((rc = put(b, d.awake + s.off, w->awake, n))) return rc;
+    st_soa(b, d.pos, N, s.off, w->position, n, 3);
+    st_soa(b, d.quat, N, s.off, w->orientation, n, 4);
+    st_soa(b, d.ppos, N, s.off, w->prev_position ? w->prev_position : w->position, n, 3);
+    st_soa(b, d.pquat, N, s.off, w->prev_orientation ? w->prev_orientation : w->orientation, n, 4);
+    st_soa(b, d.lv, N, s.off, w->linear_velocity, n, 3);
+    st_soa(b, d.av, N, s.off, w->angular_velocity, n, 3);
+    st_soa(b, d.half, N, s.off, w->half, n, 3);
+    st_soa(b, d.inv_inertia, N, s.off, w->inv_inertia, n, 3);
+    st_copy(b, d.inv_mass + s.off, w->inv_mass, n);
+    st_copy(b, d.gf + s.off, w->gravity_factor, n);
+    st_copy(b, d.ld + s.off, w->linear_damping, n);
+    st_copy(b, d.ad + s.off, w->angular_damping, n);
+    st_copy(b, d.qd + s.off, w->quadratic_drag, n);
+    st_copy(b, d.timer + s.off, w->sleep_timer, n);
+    st_copy(b, d.awake + s.off, w->awake, n);
     std::vector<uint8_t> mt((size_t)n), kind((size_t)n), bf((size_t)n);
     std::vector<int32_t> wid((size_t)n, slot), hoff((size_t)n), hcnt((size_t)n);
     for (int64_t i = 0; i < n; i++) {
@@ -554,17 +735,17 @@ extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_wo
         hoff[i] = w->kind[i] == 3 ? (int32_t)(s.hull_off + w->hull_off[i]) : 0;
         hcnt[i] = w->kind[i] == 3 ? w->hull_cnt[i] : 0;
     }
-    if ((rc = put(b, d.mt + s.off, mt.data(), n))) return rc;
-    if ((rc = put(b, d.kind + s.off, kind.data(), n))) return rc;
-    if ((rc = put(b, d.bflags + s.off, bf.data(), n))) return rc;
-    if ((rc = put(b, d.mat + s.off, w->mat_slot, n))) return rc;
-    if ((rc = put(b, d.filt + s.off, w->filt_slot, n))) return rc;
-    if ((rc = put(b, d.hoff + s.off, hoff.data(), n))) return rc;
-    if ((rc = put(b, d.hcnt + s.off, hcnt.data(), n))) return rc;
-    if ((rc = put(b, d.mat_fric + s.mat_off, w->friction_table, s.mat_n))) return rc;
-    if ((rc = put(b, d.mat_rest + s.mat_off, w->restitution_table, s.mat_n))) return rc;
-    if ((rc = put(b, d.filt_tab + s.filt_off, w->filter_table, s.filt_n))) return rc;
-    if (s.hull_n && (rc = put(b, d.hull + 3 * s.hull_off, w->hull_pts, 3 * s.hull_n))) return rc;
+    st_copy(b, d.mt + s.off, mt.data(), n);
+    st_copy(b, d.kind + s.off, kind.data(), n);
+    st_copy(b, d.bflags + s.off, bf.data(), n);
+    st_copy(b, d.mat + s.off, w->mat_slot, n);
+    st_copy(b, d.filt + s.off, w->filt_slot, n);
+    st_copy(b, d.hoff + s.off, hoff.data(), n);
+    st_copy(b, d.hcnt + s.off, hcnt.data(), n);
+    st_copy(b, d.mat_fric + s.mat_off, w->friction_table, s.mat_n);
+    st_copy(b, d.mat_rest + s.mat_off, w->restitution_table, s.mat_n);
+    st_copy(b, d.filt_tab + s.filt_off, w->filter_table, s.filt_n);
+    if (s.hull_n) st_copy(b, d.hull + 3 * s.hull_off, w->hull_pts, 3 * s.hull_n);
     const int32_t W = b->cfg.max_worlds;
     int32_t i32;
     if (s.j_n > 0) {
@@ -576,14 +757,12 @@ extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_wo
             jb[j] = w->joint_b[j] >= 0 ? (int32_t)(s.off + w->joint_b[j]) : -1;
             if (w->joint_fixed) jh[j] = (uint8_t)(w->joint_fixed[j] & 3);
         }
-        if ((rc = put(b, d.ja + s.j_off, ja.data(), s.j_n))) return rc;
-        if ((rc = put(b, d.jb + s.j_off, jb.data(), s.j_n))) return rc;
-        if ((rc = put(b, d.jhas + s.j_off, jh.data(), s.j_n))) return rc;
-        if ((rc = upload_soa(b, d.janc, d.jcap, s.j_off, w->joint_anchor, s.j_n, 3))) return rc;
-        if ((rc = upload_soa(b, d.jla, d.jcap, s.j_off, w->joint_local_a ? w->joint_local_a : zl.data(), s.j_n, 3)))
-            return rc;
-        if ((rc = upload_soa(b, d.jlb, d.jcap, s.j_off, w->joint_local_b ? w->joint_local_b : zl.data(), s.j_n, 3)))
-            return rc;
+        st_copy(b, d.ja + s.j_off, ja.data(), s.j_n);
+        st_copy(b, d.jb + s.j_off, jb.data(), s.j_n);
+        st_copy(b, d.jhas + s.j_off, jh.data(), s.j_n);
+        st_soa(b, d.janc, d.jcap, s.j_off, w->joint_anchor, s.j_n, 3);
+        st_soa(b, d.jla, d.jcap, s.j_off, w->joint_local_a ? w->joint_local_a : zl.data(), s.j_n, 3);
+        st_soa(b, d.jlb, d.jcap, s.j_off, w->joint_local_b ? w->joint_local_b : zl.data(), s.j_n, 3);
         if (!b->any_joints) { b->any_joints = true; invalidate_graph(b); }
         // LDS for the records of this world's joint bodies during the sweeps
         std::vector<int32_t> ends;
@@ -598,37 +777,37 @@ extern "C" int32_t omi_batch_load_world(omi_batch* b, int32_t slot, const omi_wo
         const int64_t want = std::min<int64_t>((nb * per_body + 15) & ~15, (int64_t)b->joint_lds_max);
         if (want > b->joint_lds) { b->joint_lds = (int32_t)want; invalidate_graph(b); }
     }
-    i32 = (int32_t)s.j_off; if ((rc = put(b, d.w_js + slot, &i32, 1))) return rc;
-    i32 = (int32_t)s.j_n; if ((rc = put(b, d.w_jn + slot, &i32, 1))) return rc;
-    i32 = w->joint_iterations; if ((rc = put(b, d.w_jiters + slot, &i32, 1))) return rc;
-    i32 = 0; if ((rc = put(b, d.w_jsched + slot, &i32, 1))) return rc;   // joint schedule rebuilt
+    i32 = (int32_t)s.j_off; st_copy(b, d.w_js + slot, &i32, 1);
+    i32 = (int32_t)s.j_n; st_copy(b, d.w_jn + slot, &i32, 1);
+    i32 = w->joint_iterations; st_copy(b, d.w_jiters + slot, &i32, 1);
+    i32 = 0; st_copy(b, d.w_jsched + slot, &i32, 1);   // joint schedule rebuilt
     s.flags = WF_ACTIVE | (w->has_collision ? WF_COLLISION : 0) | (w->sleep_enabled ? WF_SLEEP : 0) |
               (s.j_n > 0 ? WF_JOINTS : 0);
-    i32 = (int32_t)s.off; if ((rc = put(b, d.w_off + slot, &i32, 1))) return rc;
-    i32 = (int32_t)n; if ((rc = put(b, d.w_cnt + slot, &i32, 1))) return rc;
-    i32 = w->velocity_iterations; if ((rc = put(b, d.w_iters + slot, &i32, 1))) return rc;
-    i32 = (int32_t)s.mat_off; if ((rc = put(b, d.w_mat_off + slot, &i32, 1))) return rc;
-    i32 = w->n_mat; if ((rc = put(b, d.w_nmat + slot, &i32, 1))) return rc;
-    i32 = (int32_t)s.filt_off; if ((rc = put(b, d.w_filt_off + slot, &i32, 1))) return rc;
-    i32 = w->n_filt; if ((rc = put(b, d.w_nfilt + slot, &i32, 1))) return rc;
+    i32 = (int32_t)s.off; st_copy(b, d.w_off + slot, &i32, 1);
+    i32 = (int32_t)n; st_copy(b, d.w_cnt + slot, &i32, 1);
+    i32 = w->velocity_iterations; st_copy(b, d.w_iters + slot, &i32, 1);
+    i32 = (int32_t)s.mat_off; st_copy(b, d.w_mat_off + slot, &i32, 1);
+    i32 = w->n_mat; st_copy(b, d.w_nmat + slot, &i32, 1);
+    i32 = (int32_t)s.filt_off; st_copy(b, d.w_filt_off + slot, &i32, 1);
+    i32 = w->n_filt; st_copy(b, d.w_nfilt + slot, &i32, 1);
     i32 = 0;
-    if ((rc = put(b, d.w_mn + slot, &i32, 1))) return rc;
-    if ((rc = put(b, d.w_ms + slot, &i32, 1))) return rc;
-    if ((rc = put(b, d.w_cache_n + slot, &i32, 1))) return rc;
-    if ((rc = put(b, d.w_cache_n + W + slot, &i32, 1))) return rc;
-    if ((rc = put(b, d.w_pn + slot, &i32, 1))) return rc;
-    if ((rc = put(b, d.w_bpn + slot, &i32, 1))) return rc;
-    if ((rc = put(b, d.w_cn + slot, &i32, 1))) return rc;
+    st_copy(b, d.w_mn + slot, &i32, 1);
+    st_copy(b, d.w_ms + slot, &i32, 1);
+    st_copy(b, d.w_cache_n + slot, &i32, 1);
+    st_copy(b, d.w_cache_n + W + slot, &i32, 1);
+    st_copy(b, d.w_pn + slot, &i32, 1);
+    st_copy(b, d.w_bpn + slot, &i32, 1);
+    st_copy(b, d.w_cn + slot, &i32, 1);
     for (int r = 0; r < 3; r++)
-        if ((rc = put(b, d.w_grav + (size_t)r * W + slot, &w->gravity[r], 1))) return rc;
-    if ((rc = put(b, d.w_damp + slot, &w->default_linear_damping, 1))) return rc;
-    if ((rc = put(b, d.w_damp + W + slot, &w->default_angular_damping, 1))) return rc;
-    if ((rc = put(b, d.w_flags + slot, &s.flags, 1))) return rc;
-    if ((rc = put(b, d.world + s.off, wid.data(), n))) return rc;
+        st_copy(b, d.w_grav + (size_t)r * W + slot, &w->gravity[r], 1);
+    st_copy(b, d.w_damp + slot, &w->default_linear_damping, 1);
+    st_copy(b, d.w_damp + W + slot, &w->default_angular_damping, 1);
+    st_copy(b, d.w_flags + slot, &s.flags, 1);
+    st_copy(b, d.world + s.off, wid.data(), n);
     s.live = true;
     b->slots[slot] = s;
     b->lists_dirty = true;
-    return OMI_OK;
+    return st_flush(b);
 }
 
 extern "C" int32_t omi_batch_set_state(omi_batch* b, int32_t slot, const double* pos, const double* quat,
@@ -846,7 +1025,7 @@ static int32_t rebuild_lists(omi_batch* b) {
     }
     for (int c = 0; c < 4; c++) {
         counts[c] = (int32_t)lists[c].size();
-        if ((rc = put(b, b->d_wl + (size_t)c * b->cfg.max_worlds, lists[c].data(), counts[c]))) return rc;
+        st_copy(b, b->d_wl + (size_t)c * b->cfg.max_worlds, lists[c].data(), counts[c]);
         // a class's launches take a grid of its worlds rounded up to a power of
         // two, kept as a high-water mark so the captured graph changes rarely;
         // blocks past the class's count exit at once, but each still needs a
@@ -855,7 +1034,8 @@ static int32_t rebuild_lists(omi_batch* b) {
         while (g < counts[c]) g <<= 1;
         if (counts[c] > 0 && g > b->live[c]) { b->live[c] = g; grew = true; }
     }
-    if ((rc = put(b, b->d_wn, counts, 4))) return rc;
+    st_copy(b, b->d_wn, counts, 4);
+    if ((rc = st_flush(b))) return rc;
     int32_t nlive = 0;
     for (int c = 0; c < 4; c++) nlive += b->live[c] > 0;
     for (int q = 0; q + 1 < nlive; q++)
@@ -1056,6 +1236,8 @@ extern "C" int32_t omi_batch_use_graph(omi_batch* b, int32_t on) {
 static int32_t ensure_graph(omi_batch* b, double dt) {
     if (b->graph && b->graph_dt == dt) return OMI_OK;
     invalidate_graph(b);
+    // capture starts on an idle stream (staged writes may be queued on it)
+    HIPCHK(hipStreamSynchronize(b->stream));
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     enqueue_step(b, dt);
@@ -1233,13 +1415,7 @@ extern "C" int32_t omi_batch_sync(omi_batch* b) {
     if (ov) {
         // the flags are sticky over the steps since the last sync; report and clear them
         HIPCHK(hipMemset(b->d.cnt + C_OVERFLOW, 0, sizeof(uint64_t)));
-        char msg[320];
-        snprintf(msg, sizeof msg,
-                 "batch capacity exceeded in a step since the last sync (flags 0x%llx: 1 pairs, 2 contacts, "
-                 "4 cache, 8 broadphase record, 16 broadphase candidates, 32 trigger pairs, 64 island LDS); results "
-                 "of that step are invalid — recreate the batch with larger max_pairs/max_contacts/max_candidates",
-                 (unsigned long long)ov);
-        return fail(OMI_ERR_CAPACITY, msg);
+        return capacity_error(ov);
     }
     return OMI_OK;
 }
@@ -1288,6 +1464,150 @@ extern "C" int32_t omi_batch_read_world(omi_batch* b, int32_t slot, double* pos,
     if (awake && (rc = get(b, awake, d.awake + s.off, s.n))) return rc;
     if (timer && (rc = get(b, timer, d.timer + s.off, s.n))) return rc;
     return OMI_OK;
+}
+
+// ---- asynchronous readback -------------------------------------------------
+//
+// A fetch gathers one world's state on the device (staged copy segments) into
+// a buffer of its own and copies it to pinned host memory, all ordered on the
+// batch stream after the queued steps; the host collects it later.  Buffer
+// layout: 27 doubles per body, component-major (pos 3, quat 4, lv 3, av 3,
+// timer, aabb lo 3, hi 3, prev pos 3, prev quat 4), then the awake bytes.
+
+static const int kFetchDoubles = 27;
+
+extern "C" int32_t omi_batch_fetch_world(omi_batch* b, int32_t slot, int64_t* ticket) {
+    if (!b || !ticket) return fail(OMI_ERR_ARG, "null argument");
+    if (slot < 0 || slot >= b->cfg.max_worlds || !b->slots[slot].live) return fail(OMI_ERR_ARG, "bad world slot");
+    const WorldSlot& s = b->slots[slot];
+    const Dev& d = b->d;
+    const int64_t n = s.n;
+    const int64_t need = al16h(8 * kFetchDoubles * std::max<int64_t>(n, 1)) + al16h(std::max<int64_t>(n, 1));
+    // a free pooled buffer that fits, else a new one
+    int32_t bi = -1;
+    for (size_t i = 0; i < b->fbufs.size(); i++)
+        if (!b->fbuf_busy[i] && b->fbufs[i].cap >= need && (bi < 0 || b->fbufs[i].cap < b->fbufs[bi].cap))
+            bi = (int32_t)i;
+    if (bi < 0) {
+        {
+            omi_batch::FetchBuf f;
+            f.cap = std::max<int64_t>(need, 1 << 16);
+            HIPCHK(hipHostMalloc((void**)&f.host, (size_t)f.cap, hipHostMallocDefault));
+            if (hipMalloc((void**)&f.dev, (size_t)f.cap) != hipSuccess) {
+                hipHostFree(f.host);
+                return fail(OMI_ERR_HIP, "fetch buffer: device allocation failed");
+            }
+            b->fbufs.push_back(f);
+            b->fbuf_busy.push_back(0);
+            bi = (int32_t)b->fbufs.size() - 1;
+        }
+    }
+    int32_t fi = -1;
+    for (size_t i = 0; i < b->fetches.size(); i++)
+        if (!b->fetches[i].live) { fi = (int32_t)i; break; }
+    if (fi < 0) {
+        omi_batch::Fetch f;
+        HIPCHK(hipEventCreateWithFlags(&f.ev, hipEventDisableTiming));
+        b->fetches.push_back(f);
+        fi = (int32_t)b->fetches.size() - 1;
+    }
+    double* out = (double*)b->fbufs[bi].dev;
+    const double* srcs[9] = {d.pos, d.quat, d.lv, d.av, d.timer, d.lo, d.hi, d.ppos, d.pquat};
+    const int comps[9] = {3, 4, 3, 3, 1, 3, 3, 3, 4};
+    int64_t q = 0;
+    for (int a = 0; a < 9; a++)
+        for (int c = 0; c < comps[a]; c++, q++) st_dcopy(b, out + q * n, srcs[a] + (size_t)c * d.ncap + s.off, n);
+    st_dcopy(b, (uint8_t*)(out + kFetchDoubles * n), d.awake + s.off, n);
+    int32_t rc;
+    if ((rc = st_flush(b))) return rc;
+    HIPCHK(hipMemcpyAsync(b->fbufs[bi].host, b->fbufs[bi].dev, (size_t)need, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipEventRecord(b->fetches[fi].ev, b->stream));
+    b->fetches[fi].n = n;
+    b->fetches[fi].buf = bi;
+    b->fetches[fi].live = true;
+    b->fbuf_busy[bi] = 1;
+    *ticket = fi;
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_fetch_ready(omi_batch* b, int64_t ticket) {
+    if (!b || ticket < 0 || ticket >= (int64_t)b->fetches.size() || !b->fetches[ticket].live)
+        return fail(OMI_ERR_ARG, "bad fetch ticket");
+    const hipError_t e = hipEventQuery(b->fetches[ticket].ev);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return fail(OMI_ERR_HIP, std::string("fetch: ") + hipGetErrorString(e));
+}
+
+extern "C" int32_t omi_batch_fetch_wait(omi_batch* b, int64_t ticket, double* pos, double* quat, double* lv,
+                                        double* av, uint8_t* awake, double* timer, double* lo, double* hi,
+                                        double* ppos, double* pquat) {
+    if (!b || ticket < 0 || ticket >= (int64_t)b->fetches.size() || !b->fetches[ticket].live)
+        return fail(OMI_ERR_ARG, "bad fetch ticket");
+    omi_batch::Fetch& f = b->fetches[ticket];
+    HIPCHK(hipEventSynchronize(f.ev));
+    const int64_t n = f.n;
+    const double* src = (const double*)b->fbufs[f.buf].host;
+    double* dsts[9] = {pos, quat, lv, av, timer, lo, hi, ppos, pquat};
+    const int comps[9] = {3, 4, 3, 3, 1, 3, 3, 3, 4};
+    int64_t q = 0;
+    for (int a = 0; a < 9; a++) {
+        for (int c = 0; c < comps[a]; c++, q++) {
+            if (!dsts[a]) continue;
+            const double* col = src + q * n;
+            for (int64_t i = 0; i < n; i++) dsts[a][i * comps[a] + c] = col[i];
+        }
+    }
+    if (awake) std::memcpy(awake, src + kFetchDoubles * n, (size_t)n);
+    b->fbuf_busy[f.buf] = 0;
+    f.live = false;
+    f.buf = -1;
+    return OMI_OK;
+}
+
+// ---- markers -----------------------------------------------------------------
+
+static int32_t capacity_error(uint64_t ov) {
+    char msg[320];
+    snprintf(msg, sizeof msg,
+             "batch capacity exceeded in a step since the last sync (flags 0x%llx: 1 pairs, 2 contacts, "
+             "4 cache, 8 broadphase record, 16 broadphase candidates, 32 trigger pairs, 64 island LDS); results "
+             "of that step are invalid — recreate the batch with larger max_pairs/max_contacts/max_candidates",
+             (unsigned long long)ov);
+    return fail(OMI_ERR_CAPACITY, msg);
+}
+
+extern "C" int32_t omi_batch_mark(omi_batch* b, int64_t* ticket) {
+    if (!b || !ticket) return fail(OMI_ERR_ARG, "null argument");
+    int32_t mi = -1;
+    for (size_t i = 0; i < b->marks.size(); i++)
+        if (!b->marks[i].live) { mi = (int32_t)i; break; }
+    if (mi < 0) {
+        omi_batch::Mark m;
+        HIPCHK(hipEventCreateWithFlags(&m.ev, hipEventDisableTiming));
+        if (hipHostMalloc((void**)&m.flags, sizeof(uint64_t), hipHostMallocDefault) != hipSuccess) {
+            hipEventDestroy(m.ev);
+            return fail(OMI_ERR_HIP, "marker: pinned allocation failed");
+        }
+        b->marks.push_back(m);
+        mi = (int32_t)b->marks.size() - 1;
+    }
+    omi_batch::Mark& m = b->marks[mi];
+    HIPCHK(hipMemcpyAsync(m.flags, b->d.cnt + C_OVERFLOW, sizeof(uint64_t), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipEventRecord(m.ev, b->stream));
+    m.live = true;
+    *ticket = mi;
+    return OMI_OK;
+}
+
+extern "C" int32_t omi_batch_mark_wait(omi_batch* b, int64_t ticket) {
+    if (!b || ticket < 0 || ticket >= (int64_t)b->marks.size() || !b->marks[ticket].live)
+        return fail(OMI_ERR_ARG, "bad marker ticket");
+    omi_batch::Mark& m = b->marks[ticket];
+    HIPCHK(hipEventSynchronize(m.ev));
+    m.live = false;
+    const uint64_t ov = *m.flags;
+    return ov ? capacity_error(ov) : OMI_OK;
 }
 
 extern "C" int32_t omi_batch_read_trigger_overlaps(omi_batch* b, int32_t slot, int64_t cap, int64_t* count,

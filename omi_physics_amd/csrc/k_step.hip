@@ -2855,6 +2855,21 @@ __global__ void k_toggle(Dev d) {
     }
 }
 
+// Staged device writes: block b copies segment b (elements strided over the
+// block's threads).  The grid is exactly the segment count.
+__global__ __launch_bounds__(256) void k_copy_segments(const CopySeg* segs) {
+    const CopySeg sg = segs[blockIdx.x];
+    for (int64_t i = threadIdx.x; i < sg.n; i += blockDim.x) {
+        if (sg.es == 8) {
+            ((uint64_t*)sg.dst)[i] = sg.fill ? sg.src : ((const uint64_t*)sg.src)[i];
+        } else if (sg.es == 4) {
+            ((uint32_t*)sg.dst)[i] = sg.fill ? (uint32_t)sg.src : ((const uint32_t*)sg.src)[i];
+        } else {
+            ((uint8_t*)sg.dst)[i] = sg.fill ? (uint8_t)sg.src : ((const uint8_t*)sg.src)[i];
+        }
+    }
+}
+
 // ------------------------------------------------------------- integrate --
 
 // NumpyBackend.integrate_positions (backend.py:42-49) with quat_integrate
@@ -3308,6 +3323,10 @@ void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s, double li
 }
 
 void launch_toggle(const Dev& d, hipStream_t s) { hipLaunchKernelGGL(k_toggle, dim3(1), dim3(64), 0, s, d); }
+
+void launch_copy_segments(const CopySeg* segs, int32_t nseg, hipStream_t s) {
+    if (nseg > 0) hipLaunchKernelGGL(k_copy_segments, dim3(nseg), dim3(256), 0, s, segs);
+}
 
 void launch_cache_seed(const Dev& d, int64_t base, int32_t n, hipStream_t s) {
     if (n <= 0) return;

@@ -37,6 +37,18 @@ enum : int32_t {
     C_COUNT = 16
 };
 
+// One staged device write (host world loads and removals, world lists,
+// readback of retiring worlds): n elements of es bytes (1, 4 or 8) to dst,
+// from src, or (fill) src's low es bytes repeated.
+struct CopySeg {
+    uint64_t dst;
+    uint64_t src;
+    int64_t n;
+    int32_t es;
+    int32_t fill;
+};
+static_assert(sizeof(CopySeg) == 32, "32-byte segment records");
+
 // ints per island record: world, manifolds, contacts, first manifold, first contact
 constexpr int32_t ISL_DESC = 8;
 

@@ -45,6 +45,46 @@ def test_served_results_equal_alone(precision):
             assert np.array_equal(r.result[key], ref[key]), (r.id, key)
 
 
+@pytest.mark.parametrize('precision', ['fp64'])
+def test_unwaited_load_remove_fetch_keep_stream_order(precision):
+    """Loads, removals, fetches and markers queued behind steps without waiting:
+    a world fetched after its steps, its slot reloaded at once with another world,
+    and a world that stays resident all equal their alone runs bit for bit."""
+    def world(n, seed):
+        return scenes.arrays(scenes.boxes_on_ground(n, seed=seed, spread=3.0, y_range=(0.6, 5.0),
+                                                    sphere_fraction=0.5))
+    w0, w1, w2 = world(40, 11), world(90, 12), world(120, 13)
+    print('unwaited load/remove/fetch', precision, flush=True)
+
+    def alone(w, steps):
+        b = batch_from_arrays([w], precision=precision)
+        b.step(1.0 / 60.0, steps)
+        b.sync()
+        r = b.read(0)
+        b.close()
+        return r
+    from omi_physics_amd.batch import Batch
+    b = Batch(max_worlds=2, max_bodies=400, precision=precision)
+    b.load(0, w0)
+    b.load(1, w1)
+    b.step(1.0 / 60.0, 5)
+    m = b.mark()
+    t0 = b.fetch(0)
+    b.remove(0)
+    b.load(0, w2)                   # the same slot and body range, queued behind the fetch
+    b.step(1.0 / 60.0, 3)
+    t1 = b.fetch(0)
+    b.mark_wait(m)
+    r0, r2 = b.fetched(t0), b.fetched(t1)
+    r1 = b.read(1)
+    b.sync()
+    b.close()
+    for got, ref in ((r0, alone(w0, 5)), (r2, alone(w2, 3)), (r1, alone(w1, 8))):
+        for key in ('position', 'orientation', 'linear_velocity', 'angular_velocity', 'awake', 'sleep_timer',
+                    'aabb_min', 'aabb_max', 'prev_position', 'prev_orientation'):
+            assert np.array_equal(got[key], ref[key]), key
+
+
 def test_plain_form_runs():
     reqs = make_requests(3, seed=2, steps=5)
     out = run_plain(reqs)
