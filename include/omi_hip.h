@@ -247,6 +247,10 @@ typedef struct {
      * island by island (solver.py:412-445), the largest such class only;
      * -1: every world in one solver block.  The results are the same. */
     int32_t island_solve;
+    /* 1: every box/sphere pair takes the narrowphase's 16-vertex clip pass
+     * (normally only a box pair whose clip outgrows 8 vertices does); for
+     * testing that pass, the results are the same. */
+    int32_t clip16_all;
 } omi_batch_config;
 
 typedef struct omi_batch omi_batch;

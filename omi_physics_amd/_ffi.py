@@ -59,7 +59,7 @@ class BatchConfig(ctypes.Structure):
         ('record_broadphase', ctypes.c_int32), ('solver_block', ctypes.c_int32),
         ('sap_block', ctypes.c_int32 * 4), ('solve_block', ctypes.c_int32 * 4),
         ('max_candidates', ctypes.c_int64), ('max_solver_iterations', ctypes.c_int32),
-        ('max_trigger_pairs', ctypes.c_int64), ('island_solve', ctypes.c_int32),
+        ('max_trigger_pairs', ctypes.c_int64), ('island_solve', ctypes.c_int32), ('clip16_all', ctypes.c_int32),
     ]
 
 

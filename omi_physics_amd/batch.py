@@ -50,7 +50,8 @@ class Batch:
                  max_hull_pts: int = 1, max_joints: int = 1, record_broadphase: bool = False,
                  solver_block: int = 0, use_graph: bool = True,
                  sap_block: Optional[Sequence[int]] = None,
-                 solve_block: Optional[Sequence[int]] = None, island_solve: bool = True) -> None:
+                 solve_block: Optional[Sequence[int]] = None, island_solve: bool = True,
+                 clip16_all: bool = False) -> None:
         self.lib = _ffi.load()
         caps = capacities_for(max_bodies, max_worlds)
         cfg = _ffi.BatchConfig()
@@ -64,6 +65,7 @@ class Batch:
         cfg.record_broadphase = int(bool(record_broadphase))
         cfg.solver_block = int(solver_block)
         cfg.island_solve = 0 if island_solve else -1
+        cfg.clip16_all = int(bool(clip16_all))
         from .autotune import load_tuned
         tuned = load_tuned(precision)
         for c in range(4):

@@ -310,6 +310,7 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     int64_t H = 16;
     while (H < 2 * K) H <<= 1;
     d.ncap = N; d.pcap = P; d.kcap = K; d.hcap = H; d.wcap = W;
+    d.clip16_all = cfg->clip16_all ? 1 : 0;
     const size_t rs = cfg->precision == OMI_FP64 ? 8 : 4;
     int dev = 0;
     hipGetDevice(&dev);

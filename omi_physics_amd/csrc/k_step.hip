@@ -587,8 +587,9 @@ __global__ __launch_bounds__(BLOCK) void k_sap_scan(Dev d, int32_t record) {
 
 // NarrowPhase.generate dispatch (narrowphase.py:78-100) for one solid pair.
 // FP64 runs in world coordinates exactly like the reference; FP32 first moves
-// the pair into a frame at body a's centre (offset taken in FP64).
-template <typename R>
+// the pair into a frame at body a's centre (offset taken in FP64).  Without
+// GENERAL a box pair whose clip outgrows 8 vertices returns -1.
+template <typename R, bool GENERAL>
 __device__ __forceinline__ int narrow_pair(const Dev& d, int64_t a, int64_t b, R* nrm, R* pts, R* dep,
                                            double* org) {
     const R* rot = (const R*)d.rot;
@@ -602,7 +603,7 @@ __device__ __forceinline__ int narrow_pair(const Dev& d, int64_t a, int64_t b, R
         else { ca[r] = R(0); cb[r] = (R)(pb - pa); org[r] = pa; }
     }
     int cnt = 0;
-    if (ka == K_BOX && kb == K_BOX) cnt = box_box(ca, Ra, ha, cb, Rb, hb, nrm, pts, dep);
+    if (ka == K_BOX && kb == K_BOX) cnt = box_box<R, GENERAL>(ca, Ra, ha, cb, Rb, hb, nrm, pts, dep);
     else if (ka == K_SPHERE && kb == K_SPHERE) cnt = sphere_sphere(ca, ha[0], cb, hb[0], nrm, pts, dep);
     else if (ka == K_SPHERE && kb == K_BOX) cnt = sphere_box(ca, ha[0], cb, Rb, hb, nrm, pts, dep);
     else if (ka == K_BOX && kb == K_SPHERE) {
@@ -612,15 +613,31 @@ __device__ __forceinline__ int narrow_pair(const Dev& d, int64_t a, int64_t b, R
     return cnt;
 }
 
+// One pair's manifold into its slot (count, normal, points, depths).
+template <typename R>
+__device__ __forceinline__ void put_manifold(const Dev& d, int64_t p, int cnt, const R* nrm, const R* pts,
+                                             const R* dep, const double* org) {
+    R* mn = (R*)d.mnrm;
+    R* md = (R*)d.mdep;
+    d.mcnt[p] = (int8_t)cnt;
+    for (int r = 0; r < 3; r++) mn[(size_t)r * d.pcap + p] = nrm[r];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        if (m >= cnt) break;
+        for (int r = 0; r < 3; r++) d.mpts[(size_t)r * 4 * d.pcap + 4 * p + m] = org[r] + (double)pts[3 * m + r];
+        md[4 * p + m] = dep[m];
+    }
+}
+
 // Compiled for at least two wavefronts per SIMD: the FP64 box-box path then
-// fits the 256 vector registers (no accumulator-register spills) and two
-// wavefronts hide each other's latency.
+// fits the 256 vector registers and two wavefronts hide each other's latency.
+// The kernel holds no private memory: a box pair whose clip polygon outgrows
+// the 8 register slots (near-degenerate input, rare) is queued from the top
+// of the convex queue's array and clipped by k_narrow_clip16.
 template <typename R>
 __global__ __launch_bounds__(NARROW_BLOCK, 2) void k_narrow(Dev d) {
     const uint64_t tot = d.cnt[C_PAIRS];
     const int64_t P = tot < (uint64_t)d.pcap ? (int64_t)tot : d.pcap;
-    R* mn = (R*)d.mnrm;
-    R* md = (R*)d.mdep;
     for (int64_t p = gtid(); p < P; p += gstride()) {
         const int64_t a = d.pa[p], b = d.pb[p];
         if (d.kind[a] == K_CONVEX || d.kind[b] == K_CONVEX) {
@@ -632,15 +649,33 @@ __global__ __launch_bounds__(NARROW_BLOCK, 2) void k_narrow(Dev d) {
         }
         R nrm[3] = {R(0), R(0), R(0)}, pts[12] = {}, dep[4] = {};
         double org[3];
-        int cnt = narrow_pair<R>(d, a, b, nrm, pts, dep, org);
-        d.mcnt[p] = (int8_t)cnt;
-        for (int r = 0; r < 3; r++) mn[(size_t)r * d.pcap + p] = nrm[r];
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-            if (m >= cnt) break;
-            for (int r = 0; r < 3; r++) d.mpts[(size_t)r * 4 * d.pcap + 4 * p + m] = org[r] + (double)pts[3 * m + r];
-            md[4 * p + m] = dep[m];
+        const int cnt = narrow_pair<R, false>(d, a, b, nrm, pts, dep, org);
+        if (cnt < 0 || d.clip16_all) {
+            // the convex queue fills cvx from 0 and this one from P - 1: a pair
+            // is in at most one of them, so they never meet
+            d.mcnt[p] = 0;
+            const uint64_t q = atomicAdd((unsigned long long*)&d.cnt[C_CLIP16], 1ull);
+            d.cvx[P - 1 - (int64_t)q] = (int32_t)p;
+            continue;
         }
+        put_manifold<R>(d, p, cnt, nrm, pts, dep, org);
+    }
+}
+
+// The queued box pairs of k_narrow, clipped on 16-vertex polygons in private
+// memory (_collide_native._clip in full).
+template <typename R>
+__global__ __launch_bounds__(64) void k_narrow_clip16(Dev d) {
+    const uint64_t tot = d.cnt[C_PAIRS];
+    const int64_t P = tot < (uint64_t)d.pcap ? (int64_t)tot : d.pcap;
+    const uint64_t nq = d.cnt[C_CLIP16];
+    const int64_t n = nq < (uint64_t)P ? (int64_t)nq : P;
+    for (int64_t q = gtid(); q < n; q += gstride()) {
+        const int64_t p = d.cvx[P - 1 - q];
+        R nrm[3] = {R(0), R(0), R(0)}, pts[12] = {}, dep[4] = {};
+        double org[3];
+        const int cnt = narrow_pair<R, true>(d, d.pa[p], d.pb[p], nrm, pts, dep, org);
+        put_manifold<R>(d, p, cnt, nrm, pts, dep, org);
     }
 }
 
@@ -1236,7 +1271,7 @@ __device__ __forceinline__ bool pos_terms(R ima, R imb, R dep, const SolveIn& in
 // restitution bias, friction, seeds (warm-start cache, solver._seed_pair,
 // or the stage call's arrays), from the pre-solve velocities.
 template <typename R, bool STAGE>
-__global__ void k_prepare(Dev d, SolveIn in) {
+__global__ __launch_bounds__(256) void k_prepare(Dev d, SolveIn in) {
     const uint64_t tc = d.cnt[C_CONTACTS];
     const int64_t T = tc < (uint64_t)d.kcap ? (int64_t)tc : d.kcap;
     R* S_ = (R*)d.csolv;
@@ -3158,6 +3193,8 @@ void launch_narrow(const Dev& d, int precision, hipStream_t s) {
     int g = GRID_FOR(d.pcap, NARROW_BLOCK);
     if (precision == 64) hipLaunchKernelGGL(k_narrow<double>, dim3(g), dim3(NARROW_BLOCK), 0, s, d);
     else hipLaunchKernelGGL(k_narrow<float>, dim3(g), dim3(NARROW_BLOCK), 0, s, d);
+    if (precision == 64) hipLaunchKernelGGL(k_narrow_clip16<double>, dim3(4), dim3(64), 0, s, d);
+    else hipLaunchKernelGGL(k_narrow_clip16<float>, dim3(4), dim3(64), 0, s, d);
     if (d.epa_slots > 0) {
         int ge = (d.epa_slots + 63) / 64;
         if (precision == 64) hipLaunchKernelGGL(k_narrow_convex<double>, dim3(ge), dim3(64), d.cvx_lds, s, d, d.cvx_lds);

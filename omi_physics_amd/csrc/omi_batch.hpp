@@ -34,6 +34,7 @@ enum : int32_t {
     C_TPAIRS = 12,     // broadphase pairs with a trigger body (in no particular order)
     C_TOVER = 13,      // trigger overlaps found
     C_ISLANDS = 14,    // islands of the step in the size classes solved island by island
+    C_CLIP16 = 15,     // box pairs whose clip outgrew 8 vertices (queued from the top of cvx)
     C_COUNT = 16
 };
 
@@ -124,6 +125,7 @@ struct Dev {
     void *epa_work;                       // EpaWork per GJK/EPA thread
     int32_t epa_slots;
     int32_t cvx_lds;                      // dynamic LDS of the convex narrowphase (0: hull points from HBM)
+    int32_t clip16_all;                   // every box/sphere pair through k_narrow_clip16 (testing)
     double *mpts;                         // [3][4*pcap] absolute points
     void *mnrm, *mdep;                    // R [3][pcap], R [4*pcap]
     // contacts

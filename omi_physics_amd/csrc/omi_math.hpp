@@ -387,7 +387,9 @@ __device__ __forceinline__ int reduce8(const Poly8<R>& P, const R (&D)[8], int n
 // box face most aligned with the normal, incident face = the other box's face
 // most anti-parallel to it, clipped against the reference face's side planes;
 // points below the reference face are seated on it and reduced to 4.
-template <typename R>
+// Without GENERAL a polygon that outgrows the 8 slots returns -1 (the caller
+// runs the pair again with it), so the kernel holds no private arrays.
+template <typename R, bool GENERAL = true>
 __device__ __forceinline__ int clip_manifold(const R* ca, const R* Ra, const R* ha, const R* cb, const R* Rb,
                                              const R* hb, const R* nrm, R depth, R* op, R* od) {
     R a_align = R(-1), b_align = R(-1);
@@ -439,6 +441,7 @@ __device__ __forceinline__ int clip_manifold(const R* ca, const R* Ra, const R* 
         else np = clip_side8(A, np, p0, p1, p2, offset, B, over);
     }
     if (over) {
+        if constexpr (!GENERAL) return -1;
         // copies that only this rare branch hands to the out-of-line path, so
         // the caller's arrays stay in registers
         R gca[3], gRa[9], gha[3], gcb[3], gRb[9], ghb[3], gn[3], gp[12], gd[4];
@@ -484,8 +487,9 @@ __device__ __forceinline__ int clip_manifold(const R* ca, const R* Ra, const R* 
 
 // _collide_native._box_box (_collide_native.pyx:20-76): 15 SAT axes generated
 // on the fly (A's faces, B's faces, the non-degenerate edge crosses), first
-// minimum overlap wins, sign from the centre offset.
-template <typename R>
+// minimum overlap wins, sign from the centre offset.  -1 without GENERAL
+// when the clip outgrows its register polygon (see clip_manifold).
+template <typename R, bool GENERAL = true>
 __device__ __forceinline__ int box_box(const R* ca, const R* Ra, const R* ha, const R* cb, const R* Rb,
                                        const R* hb, R* normal, R* pts, R* deps) {
     const R d0 = cb[0] - ca[0], d1 = cb[1] - ca[1], d2 = cb[2] - ca[2];
@@ -526,7 +530,7 @@ __device__ __forceinline__ int box_box(const R* ca, const R* Ra, const R* ha, co
             }
         }
     normal[0] = b0; normal[1] = b1; normal[2] = b2;
-    return clip_manifold(ca, Ra, ha, cb, Rb, hb, normal, best, pts, deps);
+    return clip_manifold<R, GENERAL>(ca, Ra, ha, cb, Rb, hb, normal, best, pts, deps);
 }
 
 // _collide_native.sphere_sphere (_collide_native.pyx:285-299)
