@@ -1388,25 +1388,61 @@ struct SolveState {
 };
 
 // Bucket the contact items by level: bound[l] is the start of level l on
-// entry (after the scan) and its end on exit.
+// entry (after the scan) and its end on exit.  A thread takes a manifold and
+// loads its levels of PB passes at once before it places their items (the
+// item stores would otherwise order every level load after the previous
+// pass's stores).  The order of the items inside a level is free: they share
+// no dynamic body.  cord_c (the item's batch contact) is written for island
+// solves only (gmap).
 template <int BLOCK>
 __device__ __forceinline__ void scatter_items(int32_t* bound, int32_t SL, int32_t M, const int4* mrec, const int32_t* level,
                               int32_t cl, const int32_t* gmap, int64_t off, int32_t* corder, int32_t* cord_a,
                               int32_t* cord_b, int32_t* cord_c) {
-    for (int32_t s = 0; s < SL; s++)
-        for (int32_t m = threadIdx.x; m < M; m += BLOCK) {
-            const int4 r = mrec[m];
-            const int32_t L = level[s * M + m], c = r.y & MR_COUNT;
-            const bool da = r.y & MR_DYN_A, db = r.y & MR_DYN_B;
-            for (int32_t j = 0; j < c; j++) {
-                const int32_t slot = atomicAdd(&bound[L - 1 + j], 1);
-                const int32_t k = r.x - cl + j;
-                corder[slot] = item_code(k, da, db, s);
-                cord_c[slot] = gmap ? gmap[k] : r.x + j;
-                cord_a[slot] = (int32_t)off + r.z;
-                cord_b[slot] = (int32_t)off + r.w;
+    constexpr int PB = 8;
+    for (int32_t m = threadIdx.x; m < M; m += BLOCK) {
+        const int4 r = mrec[m];
+        const int32_t c = r.y & MR_COUNT;
+        const bool da = r.y & MR_DYN_A, db = r.y & MR_DYN_B;
+        const int32_t ka = (int32_t)off + r.z, kb = (int32_t)off + r.w;
+        for (int32_t s0 = 0; s0 < SL; s0 += PB) {
+            int32_t Ls[PB];
+#pragma unroll
+            for (int q = 0; q < PB; q++) Ls[q] = s0 + q < SL ? level[(size_t)(s0 + q) * M + m] : 0;
+#pragma unroll
+            for (int q = 0; q < PB; q++) {
+                if (s0 + q >= SL) break;
+                for (int32_t j = 0; j < c; j++) {
+                    const int32_t slot = atomicAdd(&bound[Ls[q] - 1 + j], 1);
+                    const int32_t k = r.x - cl + j;
+                    corder[slot] = item_code(k, da, db, s0 + q);
+                    if (gmap) cord_c[slot] = gmap[k];
+                    cord_a[slot] = ka;
+                    cord_b[slot] = kb;
+                }
             }
         }
+    }
+}
+
+// Contacts per level into bound[] (zeroed, global), PB passes' levels loaded
+// at once as in scatter_items.
+template <int BLOCK>
+__device__ __forceinline__ void count_items(int32_t* bound, int32_t SL, int32_t M, const int4* mrec,
+                                            const int32_t* level) {
+    constexpr int PB = 8;
+    for (int32_t m = threadIdx.x; m < M; m += BLOCK) {
+        const int32_t c = mrec[m].y & MR_COUNT;
+        for (int32_t s0 = 0; s0 < SL; s0 += PB) {
+            int32_t Ls[PB];
+#pragma unroll
+            for (int q = 0; q < PB; q++) Ls[q] = s0 + q < SL ? level[(size_t)(s0 + q) * M + m] : 0;
+#pragma unroll
+            for (int q = 0; q < PB; q++) {
+                if (s0 + q >= SL) break;
+                for (int32_t j = 0; j < c; j++) atomicAdd(&bound[Ls[q] - 1 + j], 1);
+            }
+        }
+    }
 }
 
 // The level loop of one world, in steps of at most BLOCK items of one level.
@@ -2209,6 +2245,23 @@ __device__ __forceinline__ void solve_unit(const Dev& d, const SolveIn& in, int3
     st.V = fp64 ? (R*)d.lv : (R*)d.svel; st.vs = d.ncap;
     st.P = S_ + RowLayout<R>::imp; st.ps = RowLayout<R>::stride; st.pc = 1; st.pb = 0;
 
+    // ---- contacts per level -> level starts -> items scattered by level
+    // (before the velocities are staged: the manifold records' LDS copy made
+    // for the assignment is still intact)
+    const int4* mrec_b = mrec_lds ? (const int4*)(s_dyn + mrec_at) : mrec;
+    if (lds_bounds) {
+        block_exscan<BLOCK>(s_bound, D, s_scan);
+        scatter_items<BLOCK>(s_bound, SL, M, mrec_b, level, cl, gmap, off, corder, cord_a, cord_b, cord_c);
+    } else {
+        for (int32_t l = tid; l < D; l += BLOCK) g_bound[l] = 0;
+        __syncthreads();
+        count_items<BLOCK>(g_bound, SL, M, mrec_b, level);
+        __syncthreads();
+        block_exscan<BLOCK>(g_bound, D, s_scan);
+        scatter_items<BLOCK>(g_bound, SL, M, mrec_b, level, cl, gmap, off, corder, cord_a, cord_b, cord_c);
+    }
+    __syncthreads();
+
     // ---- stage velocities, impulses
     if (state_lds) {
         R* V = (R*)(s_dyn + st.vel);
@@ -2226,23 +2279,6 @@ __device__ __forceinline__ void solve_unit(const Dev& d, const SolveIn& in, int3
             if (!touched(i)) continue;
             for (int q = 0; q < 6; q++) V[(size_t)q * d.ncap + off + i] = (R)BODY(d.lv, q, off + i);
         }
-    }
-
-    // ---- contacts per level -> level starts -> items scattered by level
-    if (lds_bounds) {
-        block_exscan<BLOCK>(s_bound, D, s_scan);
-        scatter_items<BLOCK>(s_bound, SL, M, mrec, level, cl, gmap, off, corder, cord_a, cord_b, cord_c);
-    } else {
-        for (int32_t l = tid; l < D; l += BLOCK) g_bound[l] = 0;
-        __syncthreads();
-        for (int32_t s = 0; s < SL; s++)
-            for (int32_t m = tid; m < M; m += BLOCK) {
-                const int32_t L = level[s * M + m], c = mrec[m].y & MR_COUNT;
-                for (int32_t j = 0; j < c; j++) atomicAdd(&g_bound[L - 1 + j], 1);
-            }
-        __syncthreads();
-        block_exscan<BLOCK>(g_bound, D, s_scan);
-        scatter_items<BLOCK>(g_bound, SL, M, mrec, level, cl, gmap, off, corder, cord_a, cord_b, cord_c);
     }
     __syncthreads();
     // bound[l] is now the end of level l
