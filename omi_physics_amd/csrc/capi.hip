@@ -355,7 +355,7 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     A(d.gf, N); A(d.ld, N); A(d.ad, N); A(d.qd, N); A(d.timer, N);
     unsigned char *rot, *iiw, *svel;
     A(rot, 9 * N * (int64_t)rs); A(iiw, IIW_STRIDE * N * (int64_t)rs);
-    A(svel, cfg->precision == OMI_FP32 ? 6 * N * 4 : 1);
+    A(svel, 6 * N * (int64_t)rs);
     d.rot = rot; d.iiw = iiw; d.svel = svel;
     A(d.world, N); A(d.mat, N); A(d.filt, N); A(d.hoff, N); A(d.hcnt, N);
     A(d.mt, N); A(d.kind, N); A(d.bflags, N); A(d.awake, N); A(d.awake0, N); A(d.contacted, N); A(d.notready, N);

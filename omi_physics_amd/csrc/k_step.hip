@@ -1382,7 +1382,7 @@ struct SolveState {
     size_t map, vel, imp, pos;     // LDS offsets
     int32_t nd;                    // bodies in contact (compact ids)
     bool pos_lds;
-    R* V; int64_t vs;              // global velocities: V[c * vs + i]
+    R* V; int64_t vs, vi;          // global velocities: V[c * vs + i * vi]
     R* P; int64_t ps, pc, pb;      // global impulses: P[(ci - pb) * ps + c * pc]
     int64_t off;
 };
@@ -1517,7 +1517,7 @@ __device__ __forceinline__ void run_levels(const Dev& d, const SolveIn& in, cons
                 v[0] = t0.x; v[1] = t0.y; v[2] = t0.z; v[3] = t0.w; v[4] = t1.x; v[5] = t1.y;
             }
         } else {
-            for (int c = 0; c < 6; c++) v[c] = st.V[(size_t)c * st.vs + (size_t)i];
+            for (int c = 0; c < 6; c++) v[c] = st.V[(size_t)c * st.vs + (size_t)i * st.vi];
         }
     };
     auto vel_store = [&](int64_t i, int32_t cid, const R* v) {
@@ -1531,7 +1531,7 @@ __device__ __forceinline__ void run_levels(const Dev& d, const SolveIn& in, cons
                 ((float2*)p)[2] = make_float2(v[4], v[5]);
             }
         } else {
-            for (int c = 0; c < 6; c++) st.V[(size_t)c * st.vs + (size_t)i] = v[c];
+            for (int c = 0; c < 6; c++) st.V[(size_t)c * st.vs + (size_t)i * st.vi] = v[c];
         }
     };
     auto pos_add = [&](int64_t i, int32_t cid, int r, R term, bool neg, bool dyn) {
@@ -2242,7 +2242,12 @@ __device__ __forceinline__ void solve_unit(const Dev& d, const SolveIn& in, int3
     // velocities and impulses (positions come later, in the same space)
     const bool state_lds = map_lds && st.pos <= avail;
     st.pos_lds = false;
-    st.V = fp64 ? (R*)d.lv : (R*)d.svel; st.vs = d.ncap;
+    // global velocity state (a world whose state does not fit the LDS): FP64
+    // one 48-byte record per body (a body's six components in one or two
+    // lines rather than six), FP32 component-major
+    st.V = (R*)d.svel;
+    st.vs = fp64 ? 1 : d.ncap;
+    st.vi = fp64 ? 6 : 1;
     st.P = S_ + RowLayout<R>::imp; st.ps = RowLayout<R>::stride; st.pc = 1; st.pb = 0;
 
     // ---- contacts per level -> level starts -> items scattered by level
@@ -2278,6 +2283,12 @@ __device__ __forceinline__ void solve_unit(const Dev& d, const SolveIn& in, int3
         for (int32_t i = tid; i < n; i += BLOCK) {
             if (!touched(i)) continue;
             for (int q = 0; q < 6; q++) V[(size_t)q * d.ncap + off + i] = (R)BODY(d.lv, q, off + i);
+        }
+    } else {
+        R* V = (R*)d.svel;
+        for (int32_t i = tid; i < n; i += BLOCK) {
+            if (!touched(i)) continue;
+            for (int q = 0; q < 6; q++) V[(size_t)(off + i) * 6 + q] = BODY(d.lv, q, off + i);
         }
     }
     __syncthreads();
@@ -2323,6 +2334,9 @@ __device__ __forceinline__ void solve_unit(const Dev& d, const SolveIn& in, int3
                 const R dv = ((const R*)d.svel)[(size_t)q * d.ncap + off + i] - (R)v0;
                 BODY(d.lv, q, off + i) = v0 + (double)dv;
             }
+        } else {
+            if (!touched(i)) continue;
+            for (int q = 0; q < 6; q++) BODY(d.lv, q, off + i) = ((const R*)d.svel)[(size_t)(off + i) * 6 + q];
         }
     }
     // ---- the split-impulse position pass: per body, in the LDS the velocity

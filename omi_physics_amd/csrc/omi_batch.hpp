@@ -83,7 +83,7 @@ struct Dev {
     double *gf, *ld, *ad, *qd, *timer;
     void *rot;                            // R[9][ncap] narrowphase rotation
     void *iiw;                            // R[ncap][IIW_STRIDE] world inverse inertia, row-major 3x3
-    void *svel;                           // R[6][ncap] FP32 solver velocity staging
+    void *svel;                           // FP32: R[6][ncap] solver velocity staging; FP64: R[ncap][6] (body records)
     int32_t *world;                       // world slot or -1
     int32_t *mat, *filt, *hoff, *hcnt;
     uint8_t *mt, *kind, *bflags, *awake, *awake0, *contacted, *notready;
