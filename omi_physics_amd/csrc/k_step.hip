@@ -7,7 +7,7 @@
 //   contact list in pair-then-manifold order            -> k_pair_*     (scan over all pairs)
 //   SequentialImpulseSolver._wake_sleepers              -> k_wake       (1 block / world)
 //   _solve_native_full + prepare_and_solve              -> k_solve      (1 block / world)
-//   warm-start cache rebuild                            -> k_cache      (1 block / world)
+//   warm-start cache rebuild                            -> k_cache_all  (1 thread / entry)
 //   integrate_positions                                 -> k_post       (1 thread / body)
 //   _update_sleep + _contact_islands                    -> k_sleep_*    (contacts, bodies)
 #include <type_traits>
@@ -2832,9 +2832,6 @@ __global__ void k_hash_clear(Dev d) {
     }
 }
 
-// The cache after this step (solver.py:271-278): this step's contacts with
-// their impulses, or — when the world had no contacts — last step's cache
-// unchanged (solver.py:197-198).  Group heads go into the hash.
 // Where each world's cache entries go this step: its contacts, or — when it
 // has none — last step's entries, placed by an exclusive scan over the worlds
 // in slot order; a world whose entries pass the capacity keeps none (and the
@@ -2887,26 +2884,41 @@ __device__ __forceinline__ int32_t cache_insert(const Dev& d, int buf, int64_t t
     return -1;
 }
 
-template <typename R, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_cache(Dev d) {
-    const int32_t w = blockIdx.x, tid = threadIdx.x;
-    if (w >= d.wcap) return;
+// The cache after this step (solver.py:271-278): this step's contacts with
+// their impulses, or — when the world had no contacts — last step's cache
+// unchanged (solver.py:197-198).  One thread per entry over the whole batch:
+// this step's contacts of every world that has contacts, then last step's
+// entries of every world that has none, each copied to the place k_cache_plan
+// gave its world (an entry is its world's when it lies in the world's source
+// range, which also rejects entries left by removed worlds).  Group heads go
+// into the hash; the order of the inserts only decides which probe slot a key
+// takes, and lookups probe until they find the key.
+template <typename R>
+__global__ __launch_bounds__(256) void k_cache_all(Dev d) {
     const int32_t old = *d.sel, nw = old ^ 1;
-    const size_t os = (size_t)old * d.wcap + w, ns = (size_t)nw * d.wcap + w;
-    if (!(d.w_flags[w] & WF_ACTIVE)) return;
-    const int32_t K = d.w_cn[w];
-    const int32_t cnt = K > 0 ? K : d.w_cache_n[os];
-    if (cnt == 0 || d.w_cache_n[ns] != cnt) return;   // nothing to keep, or past the capacity
-    const int32_t base = d.w_cache_s[ns];
-    const int32_t src = K > 0 ? d.w_cs[w] : d.w_cache_s[os];
-    const int32_t* sa = K > 0 ? d.ca : d.ka[old];
-    const int32_t* sb = K > 0 ? d.cb : d.kb[old];
-    const double* sp = K > 0 ? d.cpt : d.kpt[old];
-    const R* si = (const R*)(K > 0 ? d.cimp : d.kimp[old]);
+    const uint64_t tc = d.cnt[C_CONTACTS];
+    const int64_t T = tc < (uint64_t)d.kcap ? (int64_t)tc : d.kcap;
+    const int64_t To = d.hslot_n[old];
     R* di = (R*)d.kimp[nw];
-    for (int32_t k = tid; k < cnt; k += BLOCK) {
-        const int64_t s = src + k, t = base + k;
-        const int32_t a = sa[s], b = sb[s];
+    for (int64_t e = gtid(); e < T + To; e += gstride()) {
+        const bool cur = e < T;
+        const int64_t s = cur ? e : e - T;
+        const int32_t* sa = cur ? d.ca : d.ka[old];
+        const int32_t* sb = cur ? d.cb : d.kb[old];
+        const int32_t a = sa[s];
+        if (a < 0 || a >= d.ncap) continue;
+        const int32_t w = d.world[a];
+        if (w < 0 || !(d.w_flags[w] & WF_ACTIVE)) continue;
+        const size_t os = (size_t)old * d.wcap + w, ns = (size_t)nw * d.wcap + w;
+        const int32_t K = d.w_cn[w];
+        if (cur != (K > 0)) continue;
+        const int32_t cnt = cur ? K : d.w_cache_n[os];
+        const int64_t src = cur ? d.w_cs[w] : d.w_cache_s[os];
+        if (s < src || s >= src + cnt || d.w_cache_n[ns] != cnt) continue;
+        const int64_t k = s - src, t = d.w_cache_s[ns] + k;
+        const int32_t b = sb[s];
+        const double* sp = cur ? d.cpt : d.kpt[old];
+        const R* si = (const R*)(cur ? d.cimp : d.kimp[old]);
         d.ka[nw][t] = a; d.kb[nw][t] = b;
         for (int r = 0; r < 3; r++) { CON(d.kpt[nw], r, t) = CON(sp, r, s); CON(di, r, t) = CON(si, r, s); }
         int32_t slot = -1;
@@ -3382,8 +3394,9 @@ void launch_solve_class(const Dev& d, int precision, int block, SolveIn in, cons
 void launch_cache(const Dev& d, int precision, hipStream_t s) {
     hipLaunchKernelGGL(k_hash_clear, dim3(GRID_FOR(d.kcap, 256)), dim3(256), 0, s, d);
     hipLaunchKernelGGL((k_cache_plan<1024>), dim3(1), dim3(1024), 0, s, d);
-    if (precision == 64) hipLaunchKernelGGL((k_cache<double, 256>), dim3(d.wcap), dim3(256), 0, s, d);
-    else hipLaunchKernelGGL((k_cache<float, 256>), dim3(d.wcap), dim3(256), 0, s, d);
+    const int g = GRID_FOR(d.kcap, 256);
+    if (precision == 64) hipLaunchKernelGGL((k_cache_all<double>), dim3(g), dim3(256), 0, s, d);
+    else hipLaunchKernelGGL((k_cache_all<float>), dim3(g), dim3(256), 0, s, d);
 }
 
 // One block per world; lds_bytes (a multiple of 16) of dynamic LDS holds
