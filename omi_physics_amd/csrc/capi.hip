@@ -49,7 +49,9 @@ void launch_stage_boxbox(int precision, int32_t np, const double* ca, const doub
 void launch_stage_spheres(int precision, int32_t np, int32_t mode, const double* ca, const double* ra,
                           const double* cb, const double* Rb, const double* hb, const double* rb, int32_t* cnt,
                           double* nrm, double* pt, double* dep, hipStream_t s);
-void launch_prepare(const Dev& d, int precision, SolveIn in, bool stage, hipStream_t s);
+void launch_prepare(const Dev& d, int precision, SolveIn in, bool stage, hipStream_t s, int32_t skip_above);
+void launch_prepare_worlds(const Dev& d, int precision, SolveIn in, bool stage, const int32_t* wl,
+                           const int32_t* wn, int grid, hipStream_t s);
 void launch_solve_class(const Dev& d, int precision, int block, SolveIn in, const int32_t* wl, const int32_t* wn,
                         int32_t lds_bytes, int grid, hipStream_t s, int32_t isl);
 void launch_islands(const Dev& d, const int32_t* wl, const int32_t* wn, int32_t lds_bytes, int grid, hipStream_t s);
@@ -134,6 +136,8 @@ struct omi_batch {
     hipStream_t stream = nullptr;
     hipStream_t aux[3] = {nullptr, nullptr, nullptr};   // size-class branches of the solve (default priority)
     hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t rows = nullptr;                     // the smaller size classes' solver rows done
+    hipEvent_t solved = nullptr, cached = nullptr;  // the solve done; the warm-start cache rebuilt
     std::vector<void*> allocs;
     int64_t bytes = 0;
     Ranges bodies, mats, filts, hulls, joints;
@@ -281,6 +285,8 @@ static void free_all(omi_batch* b) {
         if (b->join[q]) hipEventDestroy(b->join[q]);
     }
     if (b->fork) hipEventDestroy(b->fork);
+    for (hipEvent_t e : {b->rows, b->solved, b->cached})
+        if (e) hipEventDestroy(e);
     b->allocs.clear();
 }
 
@@ -440,6 +446,7 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     // single stream, and so a single hardware queue
     for (int q = 0; q < 3; q++) HIPCHK(hipEventCreateWithFlags(&b->join[q], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    for (hipEvent_t* e : {&b->rows, &b->solved, &b->cached}) HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
     b->bodies.init(N);
     b->mats.init(TAB);
     b->filts.init(TAB);
@@ -1084,27 +1091,99 @@ static void enqueue_sap(omi_batch* b, int32_t record) {
     launch_sap_pairs(d, record, b->sap_grid, b->stream);
 }
 
-// Rows for every contact, then the size classes' solve launches: the largest
-// class on the batch stream, the others on branch streams forked and joined
-// with events, so a captured graph runs them side by side.
-template <typename Mark>
-static void enqueue_solve(omi_batch* b, SolveIn in, bool stage, Mark mark) {
+// The wake pass (`before`, given the stream it runs on), the solver rows and
+// the size classes' solve launches, largest class first.  The largest class's
+// worlds set the step's critical path, so its solve is launched first, on the
+// batch stream, and takes its CUs before the smaller classes' blocks; its
+// blocks write their own worlds' rows while their first wavefront assigns
+// the levels (a class of 64-thread blocks, or solved island by island, gets
+// its rows from a launch of its own first).  The wake pass and the other rows
+// run beside it on a branch stream, which then solves the next class, and
+// further classes take further branches after those rows.  With stage timing
+// on (or one class) everything runs in stage order, each stage timed alone:
+// wake, rows, the largest class's solve (marked), the others.
+template <typename Before, typename Mark>
+static void enqueue_solve(omi_batch* b, SolveIn in, bool stage, Before before, Mark mark) {
     const Dev& d = b->d;
     hipStream_t s = b->stream;
     const int prec = b->cfg.precision;
-    launch_prepare(d, prec, in, stage, s);
-    mark();
-    fork_classes(b, [&](int c, hipStream_t sc) {
+    auto solve_class = [&](int c, hipStream_t sc, bool rows) {
         const int32_t* wl = b->d_wl + (size_t)c * d.wcap;
         if (b->isl[c]) {
             // the class's islands, then two blocks per CU taking them in turn
+            if (rows) launch_prepare_worlds(d, prec, in, stage, wl, b->d_wn + c, b->live[c], sc);
             launch_islands(d, wl, b->d_wn + c, b->lds_bytes[c], b->live[c], sc);
             launch_solve_class(d, prec, b->solve_block[c], in, wl, b->d_wn + c, b->lds_bytes[c],
                                std::max(b->live[c], 2 * b->cus), sc, 1);
-        } else {
+        } else if (rows && b->solve_block[c] == 64) {
+            launch_prepare_worlds(d, prec, in, stage, wl, b->d_wn + c, b->live[c], sc);
             launch_solve_class(d, prec, b->solve_block[c], in, wl, b->d_wn + c, b->lds_bytes[c], b->live[c], sc, 0);
+        } else {
+            launch_solve_class(d, prec, b->solve_block[c], in, wl, b->d_wn + c, b->lds_bytes[c], b->live[c], sc,
+                               rows ? 2 : 0);
         }
-    }, mark);
+    };
+    int top = -1, nclass = 0;
+    for (int c = 3; c >= 0; c--)
+        if (b->live[c]) { if (top < 0) top = c; nclass++; }
+    if (b->timing || nclass < 2) {
+        before(s);
+        launch_prepare(d, prec, in, stage, s, 0);
+        mark();
+        fork_classes(b, [&](int c, hipStream_t sc) { solve_class(c, sc, false); }, mark);
+        return;
+    }
+    // (the batch stream's launches are captured first: a graph runs the first
+    // branch of a fork on its launch stream and the later ones on streams of
+    // their own)
+    hipStream_t r = b->aux[0];
+    hipEventRecord(b->fork, s);
+    solve_class(top, s, true);
+    mark();
+    hipStreamWaitEvent(r, b->fork, 0);
+    before(r);
+    // the largest class's worlds are those of more bodies than the next
+    // class's power-of-two bound
+    launch_prepare(d, prec, in, stage, r, b->np_class[top - 1]);
+    hipEventRecord(b->rows, r);
+    int q = 0;
+    for (int c = top - 1; c >= 0; c--) {
+        if (!b->live[c]) continue;
+        hipStream_t sc = b->aux[q];
+        if (q > 0) hipStreamWaitEvent(sc, b->rows, 0);
+        solve_class(c, sc, false);
+        hipEventRecord(b->join[q], sc);
+        q++;
+    }
+    for (int k = 0; k < q; k++) hipStreamWaitEvent(s, b->join[k], 0);
+}
+
+// The warm-start cache rebuild, then joints, positions and sleep; without
+// stage timing the cache rebuild runs on a branch stream beside the others
+// (it reads the contacts and their impulses, they the velocities) and is
+// joined before the cache buffers toggle.
+template <typename Mark>
+static void enqueue_after_solve(omi_batch* b, double dt, Mark mark) {
+    const Dev& d = b->d;
+    hipStream_t s = b->stream;
+    const int prec = b->cfg.precision;
+    const bool side = !b->timing && b->aux[0];
+    if (side) {
+        hipEventRecord(b->solved, s);
+        hipStreamWaitEvent(b->aux[0], b->solved, 0);
+        launch_cache(d, prec, b->aux[0]);
+        hipEventRecord(b->cached, b->aux[0]);
+    } else {
+        launch_cache(d, prec, s);
+    }
+    mark();
+    if (b->any_triggers) launch_trigger(d, s);
+    if (b->any_joints) launch_joints(d, prec, dt, b->joint_lds, s);
+    mark();
+    launch_post(d, b->nhw, dt, s); mark();
+    launch_sleep(d, b->nhw, dt, s, 0.08, 0.25, 0.4); mark();
+    if (side) hipStreamWaitEvent(s, b->cached, 0);
+    launch_toggle(d, s); mark();
 }
 
 static void enqueue_step(omi_batch* b, double dt) {
@@ -1121,17 +1200,10 @@ static void enqueue_step(omi_batch* b, double dt) {
     mark();
     launch_narrow(d, prec, s); mark();
     launch_compact(d, prec, b->sap_grid, s); mark();
-    launch_wake(d, s); mark();
     SolveIn in{nullptr, nullptr, 0.5, 0.8, 0.005, 1};
-    enqueue_solve(b, in, false, mark);
+    enqueue_solve(b, in, false, [&](hipStream_t ws) { launch_wake(d, ws); mark(); }, mark);
     mark();
-    launch_cache(d, prec, s); mark();
-    if (b->any_triggers) launch_trigger(d, s);
-    if (b->any_joints) launch_joints(d, prec, dt, b->joint_lds, s);
-    mark();
-    launch_post(d, b->nhw, dt, s); mark();
-    launch_sleep(d, b->nhw, dt, s, 0.08, 0.25, 0.4); mark();
-    launch_toggle(d, s); mark();
+    enqueue_after_solve(b, dt, mark);
 }
 
 // The collision stage alone (_CollisionStages.run, world.py:657-676): the host
@@ -1146,9 +1218,8 @@ static void enqueue_collide(omi_batch* b, double dt) {
     enqueue_sap(b, b->cfg.record_broadphase);
     launch_narrow(d, prec, s);
     launch_compact(d, prec, b->sap_grid, s);
-    launch_wake(d, s);
     SolveIn in{nullptr, nullptr, 0.5, 0.8, 0.005, 1};
-    enqueue_solve(b, in, false, [] {});
+    enqueue_solve(b, in, false, [&](hipStream_t ws) { launch_wake(d, ws); }, [] {});
     launch_cache(d, prec, s);
     if (b->any_triggers) launch_trigger(d, s);
     launch_toggle(d, s);
@@ -2149,7 +2220,7 @@ extern "C" int32_t omi_prepare_and_solve(int64_t K, int32_t N, const int64_t* a_
         b->slots[0].live = true;
         b->slots[0].n = N;
         if (!(rc = rebuild_lists(b)))
-            enqueue_solve(b, in, true, [] {});
+            enqueue_solve(b, in, true, [](hipStream_t) {}, [] {});
         if (!rc && hipGetLastError() != hipSuccess) rc = fail(OMI_ERR_HIP, "solver launch failed");
         if (!rc) rc = download_soa(b, pos, d.pos, d.ncap, 0, N, 3);
         if (!rc) rc = download_soa(b, lv, d.lv, d.ncap, 0, N, 3);

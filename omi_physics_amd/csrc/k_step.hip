@@ -1266,26 +1266,20 @@ __device__ __forceinline__ bool pos_terms(R ima, R imb, R dep, const SolveIn& in
     return true;
 }
 
-// Solver rows for every contact of the step, one thread per contact
-// (_solver_native.pyx:179-256): lever arms, tangent basis, effective masses,
-// restitution bias, friction, seeds (warm-start cache, solver._seed_pair,
-// or the stage call's arrays), from the pre-solve velocities.
+// The solver row of contact ci (bodies a, b) of world w, whose contacts start
+// at cs (_solver_native.pyx:179-256): lever arms, tangent basis, effective
+// masses, restitution bias, friction, seeds (warm-start cache,
+// solver._seed_pair, or the stage call's arrays), from the pre-solve
+// velocities.
 template <typename R, bool STAGE>
-__global__ __launch_bounds__(256) void k_prepare(Dev d, SolveIn in) {
-    const uint64_t tc = d.cnt[C_CONTACTS];
-    const int64_t T = tc < (uint64_t)d.kcap ? (int64_t)tc : d.kcap;
+__device__ __forceinline__ void prepare_contact(const Dev& d, const SolveIn& in, int64_t ci, int64_t a, int64_t b,
+                                                int32_t w, int32_t cs, int32_t old) {
     R* S_ = (R*)d.csolv;
     const R* imp = (const R*)d.cimp;
     const R* cn = (const R*)d.cnrm;
     const R* cdep = (const R*)d.cdep;
     const R* iiw = (const R*)d.iiw;
-    const int32_t old = *d.sel;
-    for (int64_t ci = gtid(); ci < T; ci += gstride()) {
-        const int64_t a = d.ca[ci], b = d.cb[ci];
-        const int32_t w = d.world[a];
-        if (w < 0) continue;
-        const int32_t cs = d.w_cs[w];
-        if (ci < cs || ci >= (int64_t)cs + d.w_cn[w]) continue;   // a world that overflowed
+    {
         R rest, fric, n0 = R(0), u0 = R(0), w0 = R(0);
         if (STAGE) {
             rest = (R)in.st_rest[ci - cs]; fric = (R)in.st_fric[ci - cs];
@@ -1360,6 +1354,42 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d, SolveIn in) {
             __builtin_memcpy(&t, row + q * P, 16);
             ((uint4*)dst)[q] = t;
         }
+    }
+}
+
+// Rows of the step's contacts, one thread per contact; with skip_above > 0
+// only those of worlds of at most skip_above bodies (the larger worlds'
+// rows come from k_prepare_worlds).
+template <typename R, bool STAGE>
+__global__ __launch_bounds__(256) void k_prepare(Dev d, SolveIn in, int32_t skip_above) {
+    const uint64_t tc = d.cnt[C_CONTACTS];
+    const int64_t T = tc < (uint64_t)d.kcap ? (int64_t)tc : d.kcap;
+    const int32_t old = *d.sel;
+    for (int64_t ci = gtid(); ci < T; ci += gstride()) {
+        const int64_t a = d.ca[ci], b = d.cb[ci];
+        const int32_t w = d.world[a];
+        if (w < 0) continue;
+        if (skip_above > 0 && d.w_cnt[w] > skip_above) continue;
+        const int32_t cs = d.w_cs[w];
+        if (ci < cs || ci >= (int64_t)cs + d.w_cn[w]) continue;   // a world that overflowed
+        prepare_contact<R, STAGE>(d, in, ci, a, b, w, cs, old);
+    }
+}
+
+// Rows of the contacts of the listed worlds (list wl, count *wn): G blocks
+// per world, their threads striding over its contacts.
+template <typename R, bool STAGE>
+__global__ __launch_bounds__(256) void k_prepare_worlds(Dev d, SolveIn in, const int32_t* wl, const int32_t* wn,
+                                                        int32_t G) {
+    const int32_t j = blockIdx.x / G, part = blockIdx.x % G;
+    if (j >= *wn) return;
+    const int32_t w = wl[j];
+    if (w >= d.wcap || !(d.w_flags[w] & WF_ACTIVE)) return;
+    const int32_t cs = d.w_cs[w], K = d.w_cn[w];
+    const int32_t old = *d.sel;
+    for (int32_t k = part * 256 + (int32_t)threadIdx.x; k < K; k += G * 256) {
+        const int64_t ci = (int64_t)cs + k;
+        prepare_contact<R, STAGE>(d, in, ci, d.ca[ci], d.cb[ci], w, cs, old);
     }
 }
 
@@ -2115,9 +2145,13 @@ __global__ __launch_bounds__(BLOCK) void k_islands(Dev d, const int32_t* wlist, 
     }
 }
 
+// One world's (or island's) solve.  With self_rows (a world, BLOCK > 64)
+// the block also writes its contacts' solver rows: the threads past the first
+// wavefront do it while that wavefront assigns the levels.
 template <typename R, int BLOCK, bool ISL>
 __device__ __forceinline__ void solve_unit(const Dev& d, const SolveIn& in, int32_t lds_bytes, int32_t w, int32_t K,
-                                           int32_t M, const int4* mrec, int32_t cl, const int32_t* gmap, int64_t kb) {
+                                           int32_t M, const int4* mrec, int32_t cl, const int32_t* gmap, int64_t kb,
+                                           bool self_rows = false) {
     constexpr bool isl = ISL;
     extern __shared__ __align__(16) unsigned char s_dyn[];
     __shared__ int32_t s_scan[BLOCK / 64 + 2];
@@ -2196,6 +2230,12 @@ __device__ __forceinline__ void solve_unit(const Dev& d, const SolveIn& in, int3
             if (isl) atomicMax(&d.w_levels[w], D);
             else d.w_levels[w] = D;
             atomicMax((unsigned long long*)&d.cnt[C_MAXLEVEL], (unsigned long long)D);
+        }
+    } else if (!ISL && BLOCK > 64 && self_rows) {
+        const int32_t old = *d.sel;
+        for (int32_t k = tid - 64; k < K; k += BLOCK - 64) {
+            const int64_t ci = (int64_t)cl + k;
+            prepare_contact<R, false>(d, in, ci, d.ca[ci], d.cb[ci], w, cl, old);
         }
     }
     __syncthreads();
@@ -2392,14 +2432,15 @@ __device__ __forceinline__ void solve_unit(const Dev& d, const SolveIn& in, int3
 // instantiations, so the per-world kernel holds no register for the islands.
 template <typename R, int BLOCK, bool ISL>
 __global__ __launch_bounds__(BLOCK) void k_solve(Dev d, SolveIn in, const int32_t* wlist, const int32_t* wcount,
-                                                 int32_t lds_bytes) {
+                                                 int32_t lds_bytes, int32_t self_rows) {
     if constexpr (!ISL) {
         if ((int32_t)blockIdx.x >= *wcount) return;
         const int32_t w = wlist[blockIdx.x];
         if (w >= d.wcap || !(d.w_flags[w] & WF_ACTIVE)) return;
         const int32_t K = d.w_cn[w], cs = d.w_cs[w];
         if (K == 0) { if (threadIdx.x == 0) d.w_levels[w] = 0; return; }
-        solve_unit<R, BLOCK, false>(d, in, lds_bytes, w, K, d.w_mn[w], d.mrec + d.w_ms[w], cs, nullptr, cs);
+        solve_unit<R, BLOCK, false>(d, in, lds_bytes, w, K, d.w_mn[w], d.mrec + d.w_ms[w], cs, nullptr, cs,
+                                    self_rows != 0);
     } else {
     const uint64_t ni = d.cnt[C_ISLANDS];
     const int32_t NIs = (int32_t)(ni < (uint64_t)d.kcap ? ni : (uint64_t)d.kcap);
@@ -3291,8 +3332,11 @@ void launch_wake(const Dev& d, hipStream_t s) {
 template <typename R, int B>
 static void solve_one(const Dev& d, SolveIn in, const int32_t* wl, const int32_t* wn, int32_t lds_bytes,
                       int grid, hipStream_t s, int32_t isl) {
-    if (isl) hipLaunchKernelGGL((k_solve<R, B, true>), dim3(grid), dim3(B), (size_t)lds_bytes, s, d, in, wl, wn, lds_bytes);
-    else hipLaunchKernelGGL((k_solve<R, B, false>), dim3(grid), dim3(B), (size_t)lds_bytes, s, d, in, wl, wn, lds_bytes);
+    if (isl == 1)
+        hipLaunchKernelGGL((k_solve<R, B, true>), dim3(grid), dim3(B), (size_t)lds_bytes, s, d, in, wl, wn, lds_bytes, 0);
+    else
+        hipLaunchKernelGGL((k_solve<R, B, false>), dim3(grid), dim3(B), (size_t)lds_bytes, s, d, in, wl, wn, lds_bytes,
+                           (int32_t)(isl == 2));
 }
 
 template <typename R>
@@ -3371,20 +3415,38 @@ bool allow_joint_lds(int bytes) {
 }
 
 // Solver rows for every contact (one thread per contact).
-void launch_prepare(const Dev& d, int precision, SolveIn in, bool stage, hipStream_t s) {
+// Rows of the step's contacts; with skip_above > 0 only those of worlds of at
+// most skip_above bodies.
+void launch_prepare(const Dev& d, int precision, SolveIn in, bool stage, hipStream_t s, int32_t skip_above) {
     const int g = GRID_FOR(d.kcap, 256);
     if (precision == 64) {
-        if (stage) hipLaunchKernelGGL((k_prepare<double, true>), dim3(g), dim3(256), 0, s, d, in);
-        else hipLaunchKernelGGL((k_prepare<double, false>), dim3(g), dim3(256), 0, s, d, in);
+        if (stage) hipLaunchKernelGGL((k_prepare<double, true>), dim3(g), dim3(256), 0, s, d, in, skip_above);
+        else hipLaunchKernelGGL((k_prepare<double, false>), dim3(g), dim3(256), 0, s, d, in, skip_above);
     } else {
-        if (stage) hipLaunchKernelGGL((k_prepare<float, true>), dim3(g), dim3(256), 0, s, d, in);
-        else hipLaunchKernelGGL((k_prepare<float, false>), dim3(g), dim3(256), 0, s, d, in);
+        if (stage) hipLaunchKernelGGL((k_prepare<float, true>), dim3(g), dim3(256), 0, s, d, in, skip_above);
+        else hipLaunchKernelGGL((k_prepare<float, false>), dim3(g), dim3(256), 0, s, d, in, skip_above);
+    }
+}
+
+// Rows of the contacts of the listed worlds (list wl, count *wn, at most
+// `grid` worlds), eight blocks of 256 threads per world.
+void launch_prepare_worlds(const Dev& d, int precision, SolveIn in, bool stage, const int32_t* wl,
+                           const int32_t* wn, int grid, hipStream_t s) {
+    constexpr int32_t G = 8;
+    const dim3 g((unsigned)grid * G);
+    if (precision == 64) {
+        if (stage) hipLaunchKernelGGL((k_prepare_worlds<double, true>), g, dim3(256), 0, s, d, in, wl, wn, G);
+        else hipLaunchKernelGGL((k_prepare_worlds<double, false>), g, dim3(256), 0, s, d, in, wl, wn, G);
+    } else {
+        if (stage) hipLaunchKernelGGL((k_prepare_worlds<float, true>), g, dim3(256), 0, s, d, in, wl, wn, G);
+        else hipLaunchKernelGGL((k_prepare_worlds<float, false>), g, dim3(256), 0, s, d, in, wl, wn, G);
     }
 }
 
 // One block per world of size class c (world list wl, count *wn), or with
-// isl the class's islands over `grid` blocks, with lds_bytes of dynamic LDS
-// (at least 4 * SOLVE_DCAP + 16, a multiple of 16).
+// isl = 1 the class's islands over `grid` blocks, with lds_bytes of dynamic
+// LDS (at least 4 * SOLVE_DCAP + 16, a multiple of 16); isl = 2: one block per
+// world that also writes its contacts' solver rows (blocks above 64 threads).
 void launch_solve_class(const Dev& d, int precision, int block, SolveIn in, const int32_t* wl, const int32_t* wn,
                         int32_t lds_bytes, int grid, hipStream_t s, int32_t isl) {
     if (precision == 64) solve_block<double>(d, block, in, wl, wn, lds_bytes, grid, s, isl);
