@@ -9,12 +9,15 @@ on; a 300-step rollout.  All worlds are one batch stepped by one graph launch
 per step, FP64 (bit-identical to the oracle).
 
 Timed window: the last ``--steps`` K steps of the rollout.  The batch is
-advanced untimed through steps 1..300-K-W, then W warm-up steps, then K steps
-are timed (a device synchronisation on both sides; each step is also synced to
-record its latency), and the counters of every step are read, so the line
-states the window's contacts per step next to the whole rollout's.  With the
-driver's K = 20, W = 5 the window is steps 281..300, after the piles have
-settled (contacts per step above the rollout's mean).
+advanced untimed through steps 1..300-K-W, then W warm-up steps, then the K
+steps are launched back to back and timed as one job (a device
+synchronisation on both sides): ``value`` and ``ms_per_step``.  A second
+batch runs the same rollout with every step synchronised, for the per-step
+latency (``p50_step_ms``, ``p99_step_ms``, ``ms_per_step_synced``) and the
+counters of every step, so the line states the window's contacts per step
+next to the whole rollout's (its results equal the first batch's bit for
+bit).  With the driver's K = 20, W = 5 the window is steps 281..300, after the
+piles have settled (contacts per step above the rollout's mean).
 
 Printed: one JSON line (details on stderr).  Beside ``value``: the batched FP32
 form on the same window, per-stage device time and the roofline of the
@@ -164,14 +167,16 @@ def profile_summary(path, last):
     return lines
 
 
-def rollout(arrays, precision, steps, warmup, timing=False, record=True):
+def rollout(arrays, precision, steps, warmup, timing=False, record=True, chained=False):
     """Step one batch of ``arrays`` through the rollout; time the window.
 
     Returns the batch (state at the window's end) and a dict: window wall time,
     per-step latencies of the window and of the rollout, counters of every step,
     and with ``timing`` the per-stage device time and algorithmic bytes of the
     window (its steps launched eagerly with HIP events between the stages; every
-    other step is one graph launch)."""
+    other step is one graph launch).  With ``chained`` the window's steps are
+    launched back to back, one device synchronisation before and one after
+    (a job's throughput): no per-step latency or counters inside the window."""
     from omi_physics_amd.batch import batch_from_arrays
     first, t0, t1, total = window_bounds(steps, warmup)
     tl = time.perf_counter()
@@ -189,6 +194,12 @@ def rollout(arrays, precision, steps, warmup, timing=False, record=True):
                 b.set_timing(True)
             b.sync()
             tw = time.perf_counter()
+            if chained:
+                for _ in range(t0, t1 + 1):
+                    b.step(DT, 1)
+                b.sync()
+                wall = time.perf_counter() - tw
+                break
         t = time.perf_counter()
         b.step(DT, 1)
         b.sync()
@@ -523,11 +534,17 @@ def main():
                 'graph-captured step, FP64 (bit-identical to the oracle); timed window: steps %d..%d of the %d-step '
                 'rollout' % (args.worlds, t0, t1, total))
 
-    # headline: the batched FP64 form, graph-launched, the window timed
-    b, info = rollout(arrays, 'fp64', args.steps, args.warmup)
-    wall = info['wall']
+    # headline: the batched FP64 form, graph-launched, the window's steps
+    # launched back to back and timed as one job
+    b, hinfo = rollout(arrays, 'fp64', args.steps, args.warmup, chained=True)
+    wall = hinfo['wall']
     value = n_bodies * args.steps / wall
     snap64 = [p for p, _, _, _ in b.snapshot()]
+    b.close()
+    # the same window again with every step synchronised: per-step latency and
+    # the step counters
+    b, info = rollout(arrays, 'fp64', args.steps, args.warmup)
+    same_synced = all(np.array_equal(p, q) for p, (q, _, _, _) in zip(snap64, b.snapshot()))
     finish_rollout(b, info, t1)
     b.close()
     cw = info['counters'][t0 - 1:t1]
@@ -541,6 +558,7 @@ def main():
                    'contacts_per_step_rollout': mean_counter(info['counters'], 'contacts')},
         'p50_step_ms': 1e3 * float(np.percentile(info['window_lat'], 50)),
         'p99_step_ms': 1e3 * float(np.percentile(info['window_lat'], 99)),
+        'ms_per_step_synced': 1e3 * info['wall'] / args.steps, 'synced_equals_chained': same_synced,
         'contacts_per_s': mean_counter(cw, 'contacts') * args.steps / wall,
         'rollout': {'steps': total, 'ms_per_step': 1e3 * float(info['lat'].mean()),
                     'p50_step_ms': 1e3 * float(np.percentile(info['lat'], 50)),
@@ -590,11 +608,13 @@ def main():
                                                                                    same))
 
     # the batched FP32 form on the same window
-    b32, i32 = rollout(arrays, 'fp32', args.steps, args.warmup, record=False)
+    b32, i32 = rollout(arrays, 'fp32', args.steps, args.warmup, record=False, chained=True)
+    b32.close()
+    b32, l32 = rollout(arrays, 'fp32', args.steps, args.warmup, record=False)
     b32.close()
     out['fp32_batched'] = {'value': n_bodies * args.steps / i32['wall'], 'ms_per_step': 1e3 * i32['wall'] / args.steps,
-                           'p50_step_ms': 1e3 * float(np.percentile(i32['window_lat'], 50)),
-                           'p99_step_ms': 1e3 * float(np.percentile(i32['window_lat'], 99)), 'dtype': 'f32',
+                           'p50_step_ms': 1e3 * float(np.percentile(l32['window_lat'], 50)),
+                           'p99_step_ms': 1e3 * float(np.percentile(l32['window_lat'], 99)), 'dtype': 'f32',
                            'over_fp64': wall / i32['wall']}
 
     rng = np.random.default_rng(11)

@@ -33,9 +33,13 @@ void launch_narrow(const Dev& d, int precision, hipStream_t s);
 void launch_compact(const Dev& d, int precision, int grid, hipStream_t s);
 void launch_wake(const Dev& d, hipStream_t s);
 void launch_cache(const Dev& d, int precision, hipStream_t s);
+void launch_cache_plan(const Dev& d, hipStream_t s);
+void launch_cache_write(const Dev& d, int precision, hipStream_t s);
 void launch_post(const Dev& d, int64_t nhw, double dt, hipStream_t s);
 void launch_joints(const Dev& d, int precision, double dt, int32_t lds_bytes, hipStream_t s);
 void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s, double lin, double ang, double tsleep);
+void launch_sleep_union(const Dev& d, int64_t nhw, hipStream_t s);
+void launch_sleep_decide(const Dev& d, int64_t nhw, double dt, hipStream_t s, double lin, double ang, double tsleep);
 void launch_toggle(const Dev& d, hipStream_t s);
 void launch_copy_segments(const CopySeg* segs, int32_t nseg, hipStream_t s);
 void launch_trigger(const Dev& d, hipStream_t s);
@@ -1099,11 +1103,13 @@ static void enqueue_sap(omi_batch* b, int32_t record) {
 // the levels (a class of 64-thread blocks, or solved island by island, gets
 // its rows from a launch of its own first).  The wake pass and the other rows
 // run beside it on a branch stream, which then solves the next class, and
-// further classes take further branches after those rows.  With stage timing
+// further classes take further branches after those rows.  `early` (given
+// the stream it runs on) is launched on the third branch, when there is one,
+// before it waits for the rows; returns whether it was.  With stage timing
 // on (or one class) everything runs in stage order, each stage timed alone:
 // wake, rows, the largest class's solve (marked), the others.
-template <typename Before, typename Mark>
-static void enqueue_solve(omi_batch* b, SolveIn in, bool stage, Before before, Mark mark) {
+template <typename Before, typename Early, typename Mark>
+static bool enqueue_solve(omi_batch* b, SolveIn in, bool stage, Before before, Early early, Mark mark) {
     const Dev& d = b->d;
     hipStream_t s = b->stream;
     const int prec = b->cfg.precision;
@@ -1131,7 +1137,7 @@ static void enqueue_solve(omi_batch* b, SolveIn in, bool stage, Before before, M
         launch_prepare(d, prec, in, stage, s, 0);
         mark();
         fork_classes(b, [&](int c, hipStream_t sc) { solve_class(c, sc, false); }, mark);
-        return;
+        return false;
     }
     // (the batch stream's launches are captured first: a graph runs the first
     // branch of a fork on its launch stream and the later ones on streams of
@@ -1146,6 +1152,11 @@ static void enqueue_solve(omi_batch* b, SolveIn in, bool stage, Before before, M
     // class's power-of-two bound
     launch_prepare(d, prec, in, stage, r, b->np_class[top - 1]);
     hipEventRecord(b->rows, r);
+    const bool has_early = nclass >= 3;
+    if (has_early) {
+        hipStreamWaitEvent(b->aux[1], b->fork, 0);
+        early(b->aux[1]);
+    }
     int q = 0;
     for (int c = top - 1; c >= 0; c--) {
         if (!b->live[c]) continue;
@@ -1156,32 +1167,35 @@ static void enqueue_solve(omi_batch* b, SolveIn in, bool stage, Before before, M
         q++;
     }
     for (int k = 0; k < q; k++) hipStreamWaitEvent(s, b->join[k], 0);
+    return has_early;
 }
 
 // The warm-start cache rebuild, then joints, positions and sleep; without
 // stage timing the cache rebuild runs on a branch stream beside the others
 // (it reads the contacts and their impulses, they the velocities) and is
-// joined before the cache buffers toggle.
+// joined before the cache buffers toggle.  `early`: the cache plan and the
+// sleep pass's island union already ran beside the solve.
 template <typename Mark>
-static void enqueue_after_solve(omi_batch* b, double dt, Mark mark) {
+static void enqueue_after_solve(omi_batch* b, double dt, bool early, Mark mark) {
     const Dev& d = b->d;
     hipStream_t s = b->stream;
     const int prec = b->cfg.precision;
     const bool side = !b->timing && b->aux[0];
+    hipStream_t cs = side ? b->aux[0] : s;
     if (side) {
         hipEventRecord(b->solved, s);
-        hipStreamWaitEvent(b->aux[0], b->solved, 0);
-        launch_cache(d, prec, b->aux[0]);
-        hipEventRecord(b->cached, b->aux[0]);
-    } else {
-        launch_cache(d, prec, s);
+        hipStreamWaitEvent(cs, b->solved, 0);
     }
+    if (!early) launch_cache_plan(d, cs);
+    launch_cache_write(d, prec, cs);
+    if (side) hipEventRecord(b->cached, cs);
     mark();
     if (b->any_triggers) launch_trigger(d, s);
     if (b->any_joints) launch_joints(d, prec, dt, b->joint_lds, s);
     mark();
     launch_post(d, b->nhw, dt, s); mark();
-    launch_sleep(d, b->nhw, dt, s, 0.08, 0.25, 0.4); mark();
+    if (!early) launch_sleep_union(d, b->nhw, s);
+    launch_sleep_decide(d, b->nhw, dt, s, 0.08, 0.25, 0.4); mark();
     if (side) hipStreamWaitEvent(s, b->cached, 0);
     launch_toggle(d, s); mark();
 }
@@ -1201,9 +1215,15 @@ static void enqueue_step(omi_batch* b, double dt) {
     launch_narrow(d, prec, s); mark();
     launch_compact(d, prec, b->sap_grid, s); mark();
     SolveIn in{nullptr, nullptr, 0.5, 0.8, 0.005, 1};
-    enqueue_solve(b, in, false, [&](hipStream_t ws) { launch_wake(d, ws); mark(); }, mark);
+    // the cache plan and the sleep pass's island union need only the step's
+    // contacts: they run beside the solve when a branch is free
+    const bool early = enqueue_solve(b, in, false, [&](hipStream_t ws) { launch_wake(d, ws); mark(); },
+                                     [&](hipStream_t es) {
+                                         launch_sleep_union(d, b->nhw, es);
+                                         launch_cache_plan(d, es);
+                                     }, mark);
     mark();
-    enqueue_after_solve(b, dt, mark);
+    enqueue_after_solve(b, dt, early, mark);
 }
 
 // The collision stage alone (_CollisionStages.run, world.py:657-676): the host
@@ -1219,7 +1239,7 @@ static void enqueue_collide(omi_batch* b, double dt) {
     launch_narrow(d, prec, s);
     launch_compact(d, prec, b->sap_grid, s);
     SolveIn in{nullptr, nullptr, 0.5, 0.8, 0.005, 1};
-    enqueue_solve(b, in, false, [&](hipStream_t ws) { launch_wake(d, ws); }, [] {});
+    enqueue_solve(b, in, false, [&](hipStream_t ws) { launch_wake(d, ws); }, [](hipStream_t) {}, [] {});
     launch_cache(d, prec, s);
     if (b->any_triggers) launch_trigger(d, s);
     launch_toggle(d, s);
@@ -2220,7 +2240,7 @@ extern "C" int32_t omi_prepare_and_solve(int64_t K, int32_t N, const int64_t* a_
         b->slots[0].live = true;
         b->slots[0].n = N;
         if (!(rc = rebuild_lists(b)))
-            enqueue_solve(b, in, true, [](hipStream_t) {}, [] {});
+            enqueue_solve(b, in, true, [](hipStream_t) {}, [](hipStream_t) {}, [] {});
         if (!rc && hipGetLastError() != hipSuccess) rc = fail(OMI_ERR_HIP, "solver launch failed");
         if (!rc) rc = download_soa(b, pos, d.pos, d.ncap, 0, N, 3);
         if (!rc) rc = download_soa(b, lv, d.lv, d.ncap, 0, N, 3);

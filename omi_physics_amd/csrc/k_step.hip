@@ -3453,12 +3453,24 @@ void launch_solve_class(const Dev& d, int precision, int block, SolveIn in, cons
     else solve_block<float>(d, block, in, wl, wn, lds_bytes, grid, s, isl);
 }
 
-void launch_cache(const Dev& d, int precision, hipStream_t s) {
+// The warm-start cache rebuild in two parts: the hash slots of the buffer
+// the step writes emptied and each world's place planned (both need only the
+// step's contact counts), then the entries copied (they need the solve's
+// impulses).
+void launch_cache_plan(const Dev& d, hipStream_t s) {
     hipLaunchKernelGGL(k_hash_clear, dim3(GRID_FOR(d.kcap, 256)), dim3(256), 0, s, d);
     hipLaunchKernelGGL((k_cache_plan<1024>), dim3(1), dim3(1024), 0, s, d);
+}
+
+void launch_cache_write(const Dev& d, int precision, hipStream_t s) {
     const int g = GRID_FOR(d.kcap, 256);
     if (precision == 64) hipLaunchKernelGGL((k_cache_all<double>), dim3(g), dim3(256), 0, s, d);
     else hipLaunchKernelGGL((k_cache_all<float>), dim3(g), dim3(256), 0, s, d);
+}
+
+void launch_cache(const Dev& d, int precision, hipStream_t s) {
+    launch_cache_plan(d, s);
+    launch_cache_write(d, precision, s);
 }
 
 // One block per world; lds_bytes (a multiple of 16) of dynamic LDS holds
@@ -3476,12 +3488,23 @@ void launch_post(const Dev& d, int64_t nhw, double dt, hipStream_t s) {
 }
 
 // _update_sleep's thresholds (world.py:510-511) as the step uses them.
-void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s, double lin, double ang, double tsleep) {
+// The sleep pass in two parts: the islands of the step's contacts (they need
+// only the contact list), then the speed tests and the sleep decisions (they
+// need the solved velocities).
+void launch_sleep_union(const Dev& d, int64_t nhw, hipStream_t s) {
+    if (nhw > 0) hipLaunchKernelGGL(k_sleep_union, dim3(GRID_FOR(d.kcap, 256)), dim3(256), 0, s, d);
+}
+
+void launch_sleep_decide(const Dev& d, int64_t nhw, double dt, hipStream_t s, double lin, double ang, double tsleep) {
     if (nhw <= 0) return;
-    hipLaunchKernelGGL(k_sleep_union, dim3(GRID_FOR(d.kcap, 256)), dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_sleep_body, dim3(GRID_FOR(nhw, 256)), dim3(256), 0, s, d, nhw, dt, lin * lin, ang * ang,
                        tsleep);
     hipLaunchKernelGGL(k_sleep_final, dim3(GRID_FOR(nhw, 256)), dim3(256), 0, s, d, nhw);
+}
+
+void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s, double lin, double ang, double tsleep) {
+    launch_sleep_union(d, nhw, s);
+    launch_sleep_decide(d, nhw, dt, s, lin, ang, tsleep);
 }
 
 void launch_toggle(const Dev& d, hipStream_t s) { hipLaunchKernelGGL(k_toggle, dim3(1), dim3(64), 0, s, d); }
