@@ -102,13 +102,16 @@ def solve_top_bytes(stats, r):
     per contact its 224-byte row in and three impulses (3r) out, per manifold its
     16-byte record, per body in contact (min(bodies, 2 x manifolds) per world) FP64
     velocity in and out (96), FP64 position in and out (48) and world inverse
-    inertia (12r)."""
+    inertia (12r); with more than one class present the launch also writes its
+    contacts' rows (224 out and 60 of contact data in per contact, as k_prepare)."""
     top = int(stats['size_class'].max())
     sel = stats['size_class'] == top
     k = int(stats['contacts'][sel].sum())
     m = int(stats['manifolds'][sel].sum())
     nd = int(np.minimum(stats['bodies'][sel], 2 * stats['manifolds'][sel]).sum())
-    return k * (224 + 3 * r) + 16 * m + nd * (96 + 48 + 12 * r), top, int(sel.sum())
+    own_rows = len(set(stats['size_class'][stats['size_class'] >= 0].tolist())) > 1
+    return (k * (224 + 3 * r) + 16 * m + nd * (96 + 48 + 12 * r) + (k * (224 + 60) if own_rows else 0),
+            top, int(sel.sum()))
 
 
 def bodies_in_contact(stats):
@@ -486,6 +489,24 @@ def traffic_for(window, workload):
     return t.get('k_solve_top_fp64_bytes_per_launch'), t.get('command')
 
 
+def solve_roofline(ti, K, window):
+    """The roofline of the largest size class's solver launch from an event-timed pass."""
+    kt, kb = ti['kernel_ms'], ti['kernel_bytes']
+    top, ntop = kb['top_class']
+    avg_ms = kt['solve_top'] / K
+    per_launch = kb['solve_top'] / K
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    traffic, tcmd = traffic_for(window, 'batched4096')
+    frac = achieved / HBM_PEAK_GBS
+    return {
+        'kernel': 'k_solve<double> size class %d (%d worlds, the largest; alone on its stream)' % (top, ntop),
+        'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': frac,
+        'traffic': traffic, 'traffic_command': tcmd, 'avg_ms': avg_ms, 'bytes_per_launch': per_launch,
+        'limiter': ('latency: %.2f %% of the HBM roof; the launch is one dependency chain of solver levels '
+                    '(occupancy and wave stalls: profiles/sq_stalls_solver_fp64.md)' % (100 * frac)
+                    if frac < 0.05 else 'hbm')}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -501,6 +522,8 @@ def main():
     ap.add_argument('--plain-sample', type=int, default=48, help='worlds stepped alone for the plain form')
     ap.add_argument('--curve-sample', type=int, default=192, help='worlds of the FP32 error curves')
     ap.add_argument('--only-headline', action='store_true', help='the FP64 window and nothing else (profiling)')
+    ap.add_argument('--only-timing', action='store_true',
+                    help='the event-timed pass of the FP64 window and its roofline, nothing else (profiling)')
     ap.add_argument('--no-timing', action='store_true', help='skip the event-timed pass of the window')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-extras', action='store_true', help='skip serving and the single-world workloads')
@@ -530,6 +553,12 @@ def main():
     arrays = make_worlds(args.worlds, args.seed)
     n_bodies = int(sum(a['n'] for a in arrays))
     first, t0, t1, total = window_bounds(args.steps, args.warmup)
+    if args.only_timing:
+        bt, ti = rollout(arrays, 'fp64', args.steps, args.warmup, timing=True)
+        bt.close()
+        print(json.dumps({'roofline': solve_roofline(ti, args.steps, (t0, t1)),
+                          'kernels_ms_per_step': {k: v / args.steps for k, v in ti['kernel_ms'].items()}}), flush=True)
+        return
     workload = ('%d independent box/sphere worlds, lognormal(ln 64, 1.0) bodies clipped [8, 2048], batched in one '
                 'graph-captured step, FP64 (bit-identical to the oracle); timed window: steps %d..%d of the %d-step '
                 'rollout' % (args.worlds, t0, t1, total))
@@ -584,19 +613,7 @@ def main():
         kt['solve'] = kt['prepare'] + kt['solve_top'] + kt['solve_rest']
         out['eager_ms_per_step'] = 1e3 * ti['wall'] / K
         out['kernels_ms_per_step'] = {k: v / K for k, v in kt.items()}
-        top, ntop = kb['top_class']
-        avg_ms = kt['solve_top'] / K
-        per_launch = kb['solve_top'] / K
-        achieved = per_launch / (avg_ms * 1e-3) / 1e9
-        traffic, tcmd = traffic_for((t0, t1), 'batched4096')
-        frac = achieved / HBM_PEAK_GBS
-        out['roofline'] = {
-            'kernel': 'k_solve<double> size class %d (%d worlds, the largest; alone on its stream)' % (top, ntop),
-            'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': frac,
-            'traffic': traffic, 'traffic_command': tcmd, 'avg_ms': avg_ms, 'bytes_per_launch': per_launch,
-            'limiter': ('latency: %.2f %% of the HBM roof; the launch is one dependency chain of solver levels '
-                        '(occupancy and wave stalls: profiles/sq_stalls_solver_fp64.md)' % (100 * frac)
-                        if frac < 0.05 else 'hbm')}
+        out['roofline'] = solve_roofline(ti, K, (t0, t1))
         step_ms = 1e3 * ti['wall'] / K
         sb = kb['step'] / K
         out['step_roofline'] = {'bytes_per_step': sb, 'ms_per_step': out['ms_per_step'],

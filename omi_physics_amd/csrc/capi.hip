@@ -1107,7 +1107,8 @@ static void enqueue_sap(omi_batch* b, int32_t record) {
 // the stream it runs on) is launched on the third branch, when there is one,
 // before it waits for the rows; returns whether it was.  With stage timing
 // on (or one class) everything runs in stage order, each stage timed alone:
-// wake, rows, the largest class's solve (marked), the others.
+// wake, the other classes' rows, the largest class's solve with its rows
+// (marked), the others.
 template <typename Before, typename Early, typename Mark>
 static bool enqueue_solve(omi_batch* b, SolveIn in, bool stage, Before before, Early early, Mark mark) {
     const Dev& d = b->d;
@@ -1133,10 +1134,13 @@ static bool enqueue_solve(omi_batch* b, SolveIn in, bool stage, Before before, E
     for (int c = 3; c >= 0; c--)
         if (b->live[c]) { if (top < 0) top = c; nclass++; }
     if (b->timing || nclass < 2) {
+        // (the largest class's blocks still write their own rows, so its
+        // solve launch is the one the step graph runs)
+        const bool own = nclass >= 2;
         before(s);
-        launch_prepare(d, prec, in, stage, s, 0);
+        launch_prepare(d, prec, in, stage, s, own ? b->np_class[top - 1] : 0);
         mark();
-        fork_classes(b, [&](int c, hipStream_t sc) { solve_class(c, sc, false); }, mark);
+        fork_classes(b, [&](int c, hipStream_t sc) { solve_class(c, sc, own && c == top); }, mark);
         return false;
     }
     // (the batch stream's launches are captured first: a graph runs the first
