@@ -631,34 +631,69 @@ __device__ __forceinline__ void put_manifold(const Dev& d, int64_t p, int cnt, c
 
 // Compiled for at least two wavefronts per SIMD: the FP64 box-box path then
 // fits the 256 vector registers and two wavefronts hide each other's latency.
-// The kernel holds no private memory: a box pair whose clip polygon outgrows
-// the 8 register slots (near-degenerate input, rare) is queued from the top
-// of the convex queue's array and clipped by k_narrow_clip16.
+// A block takes NARROW_BLOCK pairs at a time and hands them to its threads box
+// pairs first, so the costly box-box path runs in as few wavefronts as the
+// box pairs fill (a wavefront runs every path one of its pairs takes); each
+// pair's manifold goes to its own slot, so the order changes nothing.  The
+// kernel holds no private memory: a box pair whose clip polygon outgrows the 8
+// register slots (near-degenerate input, rare) is queued from the top of the
+// convex queue's array and clipped by k_narrow_clip16.
 template <typename R>
 __global__ __launch_bounds__(NARROW_BLOCK, 2) void k_narrow(Dev d) {
+    constexpr int NW = NARROW_BLOCK / 64;
+    __shared__ int32_t s_cnt[NW];
+    __shared__ int32_t s_pair[NARROW_BLOCK];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint64_t tot = d.cnt[C_PAIRS];
     const int64_t P = tot < (uint64_t)d.pcap ? (int64_t)tot : d.pcap;
-    for (int64_t p = gtid(); p < P; p += gstride()) {
-        const int64_t a = d.pa[p], b = d.pb[p];
-        if (d.kind[a] == K_CONVEX || d.kind[b] == K_CONVEX) {
-            // queued for the GJK/EPA kernel, which writes this pair's manifold
-            d.mcnt[p] = 0;
-            uint64_t slot = atomicAdd((unsigned long long*)&d.cnt[C_CONVEX], 1ull);
-            d.cvx[slot] = (int32_t)p;
-            continue;
+    for (int64_t base = (int64_t)blockIdx.x * NARROW_BLOCK; base < P; base += (int64_t)gridDim.x * NARROW_BLOCK) {
+        const int64_t p = base + tid;
+        bool boxes = false, other = false;
+        if (p < P) {
+            const int64_t a = d.pa[p], b = d.pb[p];
+            const int ka = d.kind[a], kb = d.kind[b];
+            if (ka == K_CONVEX || kb == K_CONVEX) {
+                // queued for the GJK/EPA kernel, which writes this pair's manifold
+                d.mcnt[p] = 0;
+                uint64_t slot = atomicAdd((unsigned long long*)&d.cnt[C_CONVEX], 1ull);
+                d.cvx[slot] = (int32_t)p;
+            } else if (ka == K_BOX && kb == K_BOX) {
+                boxes = true;
+            } else {
+                other = true;
+            }
         }
-        R nrm[3] = {R(0), R(0), R(0)}, pts[12] = {}, dep[4] = {};
-        double org[3];
-        const int cnt = narrow_pair<R, false>(d, a, b, nrm, pts, dep, org);
-        if (cnt < 0 || d.clip16_all) {
-            // the convex queue fills cvx from 0 and this one from P - 1: a pair
-            // is in at most one of them, so they never meet
-            d.mcnt[p] = 0;
-            const uint64_t q = atomicAdd((unsigned long long*)&d.cnt[C_CLIP16], 1ull);
-            d.cvx[P - 1 - (int64_t)q] = (int32_t)p;
-            continue;
+        const uint64_t mb = __ballot(boxes), mo = __ballot(other), lt = (1ull << lane) - 1ull;
+        if (lane == 0) s_cnt[wv] = __popcll(mb) | (__popcll(mo) << 16);
+        __syncthreads();
+        int32_t nb = 0, nbb = 0, nob = 0, no = 0;
+#pragma unroll
+        for (int q = 0; q < NW; q++) {
+            const int32_t c = s_cnt[q];
+            nb += c & 0xffff;
+            no += c >> 16;
+            if (q < wv) { nbb += c & 0xffff; nob += c >> 16; }
         }
-        put_manifold<R>(d, p, cnt, nrm, pts, dep, org);
+        if (boxes) s_pair[nbb + __popcll(mb & lt)] = (int32_t)p;
+        if (other) s_pair[nb + nob + __popcll(mo & lt)] = (int32_t)p;
+        __syncthreads();
+        if (tid < nb + no) {
+            const int64_t q = s_pair[tid];
+            const int64_t a = d.pa[q], b = d.pb[q];
+            R nrm[3] = {R(0), R(0), R(0)}, pts[12] = {}, dep[4] = {};
+            double org[3];
+            const int cnt = narrow_pair<R, false>(d, a, b, nrm, pts, dep, org);
+            if (cnt < 0 || d.clip16_all) {
+                // the convex queue fills cvx from 0 and this one from P - 1: a
+                // pair is in at most one of them, so they never meet
+                d.mcnt[q] = 0;
+                const uint64_t k = atomicAdd((unsigned long long*)&d.cnt[C_CLIP16], 1ull);
+                d.cvx[P - 1 - (int64_t)k] = (int32_t)q;
+            } else {
+                put_manifold<R>(d, q, cnt, nrm, pts, dep, org);
+            }
+        }
+        __syncthreads();
     }
 }
 
