@@ -1201,22 +1201,58 @@ struct LevelTable {
 // level reaches cap).  Returns the depth; d0 gets the deepest level holding a
 // pass-0 contact.  Called by the 64 threads of one wavefront; the table starts
 // at zero.
+//
+// With `prev` (M + cap ints of LDS) the groups keep to one pass each, and after
+// each pass s >= 1 the largest shift of a manifold's level from pass s - 1,
+// Delta = max_m (L_s(m) - L_{s-1}(m)), is known.  Every later pass may then be
+// pass s shifted by Delta, L_{s+j}(m) = L_s(m) + j Delta: a shift keeps every
+// dependency inside a pass, and a pass-(s+1) item's dependency on the last
+// pass-s item q of one of its bodies holds because it held one pass earlier
+// (L_s(f) >= L_{s-1}(q) + c_q) and q moved by at most Delta.  Such a schedule
+// is as valid as the longest-path one (every item still runs after the items
+// it depends on), only deeper when Delta exceeds the longest path's growth per
+// pass, so the rest of the passes are shifted once that excess, times the
+// passes left, costs fewer level steps than assigning them would (about half
+// a level step per group of 64 manifolds).
 template <bool G>
 __device__ __forceinline__ int32_t assign_item_levels(const LevelTable<G>& T, int32_t M, int32_t NI, const int4* mrec,
-                                      int32_t* level, int32_t* lcnt, int32_t cap, bool& over, int32_t& d0) {
+                                      int32_t* level, int32_t* lcnt, int32_t cap, bool& over, int32_t& d0,
+                                      int32_t* prev = nullptr) {
     const int lane = threadIdx.x & 63;
     int32_t D = 0, Dp = 0;
-    int32_t m = lane;
-    while (m >= M) m -= M;
+    const bool aligned = prev != nullptr;
+    const int32_t SLp = (NI + M - 1) / M;                  // passes
+    const int32_t GP = (M + 63) / 64;                      // groups per pass when aligned
+    const int32_t NG = aligned ? SLp * GP : (NI + 63) / 64;
+    // this lane's item of the next group: pass sn, group gn within it (aligned),
+    // item in, manifold mn
+    int32_t sn = 0, gn = 0, in_ = lane, mn = lane;
+    if (!aligned) while (mn >= M) mn -= M;
+    auto next_valid = [&]() { return aligned ? gn * 64 + lane < M : in_ < NI; };
+    bool v_n = next_valid();
     int4 rec = make_int4(0, 1, 0, 0);
-    if (lane < NI) rec = mrec[m];
-    for (int32_t base = 0; base < NI; base += 64) {
-        const bool valid = base + lane < NI;
+    if (v_n) rec = mrec[mn];
+    int32_t dmax = 0, dpass = 0, dprev = 0;   // this pass: largest shift, depth; the last pass's depth
+    for (int32_t g = 0; g < NG; g++) {
+        const bool valid = v_n;
+        const int32_t i = in_, m = mn, s = sn;
+        const bool pass_end = aligned && gn + 1 == GP;
         const int4 cur = rec;
         // the next group's manifold records load while this group resolves
-        m += 64;
-        while (m >= M) m -= M;
-        if (base + 64 + lane < NI) rec = mrec[m];
+        if (aligned) {
+            if (++gn == GP) { gn = 0; sn++; }
+            mn = gn * 64 + lane;
+            in_ = sn * M + mn;
+        } else {
+            in_ += 64;
+            mn += 64;
+            while (mn >= M) mn -= M;
+        }
+        v_n = g + 1 < NG && next_valid();
+        if (v_n) rec = mrec[mn];
+        // the manifold's level one pass earlier (read early: it does not
+        // depend on this group)
+        const int32_t pl = (aligned && valid && i >= M) ? prev[m] : 0;
         const int32_t al = (valid && (cur.y & MR_DYN_A)) ? cur.z : -1;
         const int32_t bl = (valid && (cur.y & MR_DYN_B)) ? cur.w : -1;
         const int32_t c = valid ? (cur.y & MR_COUNT) : 1;
@@ -1225,18 +1261,24 @@ __device__ __forceinline__ int32_t assign_item_levels(const LevelTable<G>& T, in
         if (bl >= 0) T.or_mask(bl, bit);
         T.fence();
         const uint64_t below = bit - 1ull, above = ~((bit << 1) - 1ull);
-        uint64_t ma = 0, mb = 0;
+        // both sides' entries read at once (a missing side reads entry 0 and
+        // drops it), so the wavefront waits for the table once
+        const uint64_t ma0 = T.get_mask(al >= 0 ? al : 0), mb0 = T.get_mask(bl >= 0 ? bl : 0);
+        const int32_t la0 = T.get_last(al >= 0 ? al : 0), lb0 = T.get_last(bl >= 0 ? bl : 0);
+        const uint64_t ma = al >= 0 ? ma0 : 0ull, mb = bl >= 0 ? mb0 : 0ull;
         int32_t L = 1;
-        if (al >= 0) { ma = T.get_mask(al); const int32_t x = T.get_last(al) + 1; L = x > L ? x : L; }
-        if (bl >= 0) { mb = T.get_mask(bl); const int32_t x = T.get_last(bl) + 1; L = x > L ? x : L; }
+        if (al >= 0 && la0 + 1 > L) L = la0 + 1;
+        if (bl >= 0 && lb0 + 1 > L) L = lb0 + 1;
         const int32_t pa = (ma & below) ? 63 - __clzll(ma & below) : lane;
         const int32_t pb = (mb & below) ? 63 - __clzll(mb & below) : lane;
-        const int32_t ca = __shfl(c, pa, 64), cb = __shfl(c, pb, 64);
+        // relaxation: a predecessor's manifold ends on level L + c - 1 of its
+        // lane, so each round exchanges L + c
         for (;;) {
-            const int32_t La = __shfl(L, pa, 64), Lb = __shfl(L, pb, 64);
+            const int32_t Lc = L + c;
+            const int32_t Ea = __shfl(Lc, pa, 64), Eb = __shfl(Lc, pb, 64);
             int32_t nL = L;
-            if (pa != lane && La + ca > nL) nL = La + ca;
-            if (pb != lane && Lb + cb > nL) nL = Lb + cb;
+            if (pa != lane && Ea > nL) nL = Ea;
+            if (pb != lane && Eb > nL) nL = Eb;
             const bool ch = nL != L;
             L = nL;
             if (!__any(ch)) break;
@@ -1249,13 +1291,61 @@ __device__ __forceinline__ int32_t assign_item_levels(const LevelTable<G>& T, in
         if (bl >= 0) T.clear_mask(bl);
         T.fence();
         if (valid) {
-            level[base + lane] = L;
+            level[i] = L;
             D = Le > D ? Le : D;
-            if (base + lane < M) Dp = Le > Dp ? Le : Dp;
+            if (i < M) Dp = Le > Dp ? Le : Dp;
             if (lcnt) {
                 if (Le <= cap) for (int32_t j = 0; j < c; j++) atomicAdd(&lcnt[L - 1 + j], 1);
                 else over = true;
             }
+            if (aligned) {
+                if (i >= M) { const int32_t dl = L - pl; dmax = dl > dmax ? dl : dmax; }
+                prev[m] = L;
+                dpass = Le > dpass ? Le : dpass;
+            }
+        }
+        if (pass_end) {
+            // pass s done
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                int32_t x = __shfl_xor(dmax, o, 64); dmax = x > dmax ? x : dmax;
+                x = __shfl_xor(dpass, o, 64); dpass = x > dpass ? x : dpass;
+            }
+            // (hs, cap ints after `prev`: the contacts per level after the
+            // previous pass, on levels below dprev)
+            int32_t* hs = prev + M;
+            if (s >= 1 && s + 1 < SLp && 2 * (dmax - (dpass - dprev)) <= GP) {
+                // the remaining passes: pass s shifted by dmax per pass; their
+                // contacts per level are pass s's, shifted
+                const int32_t last = dpass + (SLp - 1 - s) * dmax;
+                D = last > D ? last : D;
+                for (int32_t mm = lane; mm < M; mm += 64) {
+                    const int32_t L0 = prev[mm];
+                    for (int32_t j = s + 1; j < SLp; j++) level[j * M + mm] = L0 + (j - s) * dmax;
+                }
+                if (lcnt && !over && last <= cap) {
+                    // pass s's contacts per level: the counts now less those
+                    // after the previous pass (all read before any is added to)
+                    for (int32_t l = lane; l < dpass; l += 64) hs[l] = lcnt[l] - (l < dprev ? hs[l] : 0);
+                    T.fence();
+                    for (int32_t l = lane; l < dpass; l += 64) {
+                        const int32_t h = hs[l];
+                        if (h)
+                            for (int32_t j = 1; j < SLp - s; j++) atomicAdd(&lcnt[l + j * dmax], h);
+                    }
+                } else if (lcnt) {
+                    over = true;
+                }
+                break;
+            }
+            if (lcnt && dpass <= cap) {
+                T.fence();
+                for (int32_t l = lane; l < dpass; l += 64) hs[l] = lcnt[l];
+                T.fence();
+            }
+            dprev = dpass;
+            dmax = 0;
+            dpass = 0;
         }
     }
 #pragma unroll
@@ -2224,6 +2314,10 @@ __device__ __forceinline__ void solve_unit(const Dev& d, const SolveIn& in, int3
     const bool table_lds = al16((size_t)n * 8) + (size_t)n * 4 <= avail;
     const size_t mrec_at = al16((size_t)n * 8) + al16((size_t)n * 4);
     const bool mrec_lds = table_lds && mrec_at + (size_t)M * 16 <= avail;
+    // the previous pass's level per manifold, for shifting the last passes
+    // (worlds of at least four groups of manifolds per pass)
+    const size_t prev_at = al16(mrec_at + (size_t)M * 16);
+    const bool prev_lds = mrec_lds && M >= 256 && prev_at + (size_t)M * 4 + 4 * (size_t)SOLVE_DCAP <= avail;
     // an island shares the world's body range with the world's other islands,
     // so its tables must be its block's own
     if (isl && !(table_lds && al16((size_t)n * 4) <= avail)) {
@@ -2252,7 +2346,8 @@ __device__ __forceinline__ void solve_unit(const Dev& d, const SolveIn& in, int3
             // two calls, so each knows its records' address space
             if (mrec_lds)
                 D = assign_item_levels<false>(LevelTable<false>{sl, sm}, M, NI, (const int4*)(s_dyn + mrec_at), level,
-                                              s_bound, SOLVE_DCAP, over, D0);
+                                              s_bound, SOLVE_DCAP, over, D0,
+                                              prev_lds ? (int32_t*)(s_dyn + prev_at) : nullptr);
             else
                 D = assign_item_levels<false>(LevelTable<false>{sl, sm}, M, NI, mrec, level, s_bound, SOLVE_DCAP,
                                               over, D0);

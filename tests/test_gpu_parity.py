@@ -282,8 +282,8 @@ def test_full_4096_world_batch_fp64_matches_oracle_sample(lib):
 def test_group_lanes_equal_single_batch(lib, precision):
     """Worlds of three size classes split into lanes (one member batch per class,
     stepped side by side) end bit-identical to one batch holding them all: states,
-    contacts, per-world statistics and counters, in graph mode, in eager timed mode
-    and in graph mode again after it."""
+    contacts, per-world statistics and counters (schedule depths aside), in graph
+    mode, in eager timed mode and in graph mode again after it."""
     sizes = ((40, 1), (600, 2), (1500, 3), (130, 4))
     arrays = [scenes.arrays(scenes.boxes_on_ground(n, seed=s, spread=10.0, y_range=(0.6, 8.0),
                                                    sphere_fraction=0.5)) for n, s in sizes]
@@ -304,12 +304,15 @@ def test_group_lanes_equal_single_batch(lib, precision):
             cx, cy = grp.contacts(i), one.contacts(i)
             for key in ('a', 'b', 'normal', 'depth', 'normal_impulse'):
                 assert np.array_equal(cx[key], cy[key]), (phase, i, key)
+        # (schedule depths may differ: a class alone in its lane can be solved
+        # island by island, and a world's long schedule may shift its last
+        # passes; every schedule gives the same results)
         sg, so = grp.world_stats(), one.world_stats()
-        for key in ('bodies', 'contacts', 'manifolds', 'levels', 'size_class'):
+        for key in ('bodies', 'contacts', 'manifolds', 'size_class'):
             assert np.array_equal(sg[key], so[key]), (phase, key)
+        assert (sg['levels'] > 0).sum() == (so['levels'] > 0).sum(), phase
         cg, co = grp.counters(), one.counters()
-        for key in ('bodies', 'x_candidates', 'broadphase_pairs', 'solid_pairs', 'manifolds', 'contacts',
-                    'max_levels'):
+        for key in ('bodies', 'x_candidates', 'broadphase_pairs', 'solid_pairs', 'manifolds', 'contacts'):
             assert cg[key] == co[key], (phase, key)
         if timed:
             lt = grp.lane_kernel_times()
