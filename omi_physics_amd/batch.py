@@ -18,7 +18,7 @@ from . import _ffi
 from ._ffi import _d, _i32, _i64, _u8, check, ptr
 from .autotune import size_class
 
-KERNELS = ('begin', 'pre', 'sap', 'narrow', 'compact', 'wake', 'prepare', 'solve_top', 'solve_rest', 'cache',
+KERNELS = ('begin', 'pre', 'sap', 'narrow', 'compact', 'prepare', 'solve_top', 'solve_rest', 'cache',
            'joints', 'post', 'sleep', 'toggle')
 
 

@@ -31,8 +31,6 @@ void launch_sap_sort(const Dev& d, int c, const int32_t* wl, const int32_t* wn, 
 void launch_sap_pairs(const Dev& d, int32_t record, int32_t grid, hipStream_t s);
 void launch_narrow(const Dev& d, int precision, hipStream_t s);
 void launch_compact(const Dev& d, int precision, int grid, hipStream_t s);
-void launch_wake(const Dev& d, hipStream_t s);
-void launch_cache(const Dev& d, int precision, hipStream_t s);
 void launch_cache_plan(const Dev& d, hipStream_t s);
 void launch_cache_write(const Dev& d, int precision, hipStream_t s);
 void launch_post(const Dev& d, int64_t nhw, double dt, hipStream_t s);
@@ -131,8 +129,8 @@ struct WorldSlot {
 // Timed stages of a step; solve_top is the solver launch of the largest size
 // class alone on the batch stream, solve_rest the wait for the other classes.
 static const char* kKernelNames =
-    "begin,pre,sap,narrow,compact,wake,prepare,solve_top,solve_rest,cache,joints,post,sleep,toggle";
-static const int kNumKernels = 14;
+    "begin,pre,sap,narrow,compact,prepare,solve_top,solve_rest,cache,joints,post,sleep,toggle";
+static const int kNumKernels = 13;
 
 struct omi_batch {
     omi_batch_config cfg{};
@@ -141,6 +139,7 @@ struct omi_batch {
     hipStream_t aux[3] = {nullptr, nullptr, nullptr};   // size-class branches of the solve (default priority)
     hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
     hipEvent_t rows = nullptr;                     // the smaller size classes' solver rows done
+    hipEvent_t next = nullptr;                     // the second largest class's solver rows done
     hipEvent_t solved = nullptr, cached = nullptr;  // the solve done; the warm-start cache rebuilt
     std::vector<void*> allocs;
     int64_t bytes = 0;
@@ -289,7 +288,7 @@ static void free_all(omi_batch* b) {
         if (b->join[q]) hipEventDestroy(b->join[q]);
     }
     if (b->fork) hipEventDestroy(b->fork);
-    for (hipEvent_t e : {b->rows, b->solved, b->cached})
+    for (hipEvent_t e : {b->rows, b->next, b->solved, b->cached})
         if (e) hipEventDestroy(e);
     b->allocs.clear();
 }
@@ -450,7 +449,7 @@ extern "C" int32_t omi_batch_create(const omi_batch_config* cfg, omi_batch** out
     // single stream, and so a single hardware queue
     for (int q = 0; q < 3; q++) HIPCHK(hipEventCreateWithFlags(&b->join[q], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
-    for (hipEvent_t* e : {&b->rows, &b->solved, &b->cached}) HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    for (hipEvent_t* e : {&b->rows, &b->next, &b->solved, &b->cached}) HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
     b->bodies.init(N);
     b->mats.init(TAB);
     b->filts.init(TAB);
@@ -1050,7 +1049,7 @@ static int32_t rebuild_lists(omi_batch* b) {
     if ((rc = st_flush(b))) return rc;
     int32_t nlive = 0;
     for (int c = 0; c < 4; c++) nlive += b->live[c] > 0;
-    for (int q = 0; q + 1 < nlive; q++)
+    for (int q = 0; q < 3 && nlive >= 2; q++)
         if (!b->aux[q]) HIPCHK(hipStreamCreateWithFlags(&b->aux[q], hipStreamNonBlocking));
     b->nhw = b->d.ncap;
     b->lists_dirty = false;
@@ -1095,22 +1094,24 @@ static void enqueue_sap(omi_batch* b, int32_t record) {
     launch_sap_pairs(d, record, b->sap_grid, b->stream);
 }
 
-// The wake pass (`before`, given the stream it runs on), the solver rows and
-// the size classes' solve launches, largest class first.  The largest class's
-// worlds set the step's critical path, so its solve is launched first, on the
-// batch stream, and takes its CUs before the smaller classes' blocks; its
-// blocks write their own worlds' rows while their first wavefront assigns
-// the levels (a class of 64-thread blocks, or solved island by island, gets
-// its rows from a launch of its own first).  The wake pass and the other rows
-// run beside it on a branch stream, which then solves the next class, and
-// further classes take further branches after those rows.  `early` (given
-// the stream it runs on) is launched on the third branch, when there is one,
-// before it waits for the rows; returns whether it was.  With stage timing
-// on (or one class) everything runs in stage order, each stage timed alone:
-// wake, the other classes' rows, the largest class's solve with its rows
-// (marked), the others.
-template <typename Before, typename Early, typename Mark>
-static bool enqueue_solve(omi_batch* b, SolveIn in, bool stage, Before before, Early early, Mark mark) {
+// The solver rows (with in.wake, the wake pass rides on them) and the size
+// classes' solve launches, largest class first.  The largest class's worlds
+// set the step's critical path, so its solve is launched first, on the batch
+// stream, and takes its CUs before the smaller classes' blocks; its blocks
+// write their own worlds' rows while their first wavefront assigns the levels
+// (a class of 64-thread blocks, or solved island by island, gets its rows from
+// a launch of its own first).  The next class runs beside it on a first
+// branch stream, after a launch that writes that class's rows (a few tens of
+// microseconds: the largest class's blocks, each taking a whole CU, are all
+// placed before the next class's blocks spread over the CUs).  After that
+// launch, the remaining classes' rows, then their solves, run on a second and
+// a third branch (so each class's blocks are placed before a smaller class's),
+// and `early` (given the stream it runs on) on the third, before it waits for
+// those rows; returns whether `early` was launched.  With stage timing on (or
+// one class) everything runs in stage order, each stage timed alone: the other
+// classes' rows, the largest class's solve with its rows (marked), the others.
+template <typename Early, typename Mark>
+static bool enqueue_solve(omi_batch* b, SolveIn in, bool stage, Early early, Mark mark) {
     const Dev& d = b->d;
     hipStream_t s = b->stream;
     const int prec = b->cfg.precision;
@@ -1130,15 +1131,20 @@ static bool enqueue_solve(omi_batch* b, SolveIn in, bool stage, Before before, E
                                rows ? 2 : 0);
         }
     };
-    int top = -1, nclass = 0;
-    for (int c = 3; c >= 0; c--)
-        if (b->live[c]) { if (top < 0) top = c; nclass++; }
+    int top = -1, nclass = 0, rest[4], nrest = 0;
+    for (int c = 3; c >= 0; c--) {
+        if (!b->live[c]) continue;
+        if (top < 0) top = c;
+        else rest[nrest++] = c;
+        nclass++;
+    }
     if (b->timing || nclass < 2) {
         // (the largest class's blocks still write their own rows, so its
         // solve launch is the one the step graph runs)
         const bool own = nclass >= 2;
-        before(s);
-        launch_prepare(d, prec, in, stage, s, own ? b->np_class[top - 1] : 0);
+        // class c's worlds are those of more bodies than the next smaller
+        // class's power-of-two bound
+        launch_prepare(d, prec, in, stage, s, own ? b->np_class[rest[0]] : 0);
         mark();
         fork_classes(b, [&](int c, hipStream_t sc) { solve_class(c, sc, own && c == top); }, mark);
         return false;
@@ -1146,32 +1152,35 @@ static bool enqueue_solve(omi_batch* b, SolveIn in, bool stage, Before before, E
     // (the batch stream's launches are captured first: a graph runs the first
     // branch of a fork on its launch stream and the later ones on streams of
     // their own)
-    hipStream_t r = b->aux[0];
     hipEventRecord(b->fork, s);
     solve_class(top, s, true);
     mark();
-    hipStreamWaitEvent(r, b->fork, 0);
-    before(r);
-    // the largest class's worlds are those of more bodies than the next
-    // class's power-of-two bound
-    launch_prepare(d, prec, in, stage, r, b->np_class[top - 1]);
-    hipEventRecord(b->rows, r);
-    const bool has_early = nclass >= 3;
-    if (has_early) {
-        hipStreamWaitEvent(b->aux[1], b->fork, 0);
-        early(b->aux[1]);
+    hipStreamWaitEvent(b->aux[0], b->fork, 0);
+    launch_prepare_worlds(d, prec, in, stage, b->d_wl + (size_t)rest[0] * d.wcap, b->d_wn + rest[0],
+                          b->live[rest[0]], b->aux[0]);
+    hipEventRecord(b->next, b->aux[0]);
+    solve_class(rest[0], b->aux[0], false);
+    hipEventRecord(b->join[0], b->aux[0]);
+    bool used[3] = {true, false, false};
+    if (nrest >= 2) {
+        hipStreamWaitEvent(b->aux[1], b->next, 0);
+        launch_prepare(d, prec, in, stage, b->aux[1], b->np_class[rest[1]]);
+        hipEventRecord(b->rows, b->aux[1]);
+        solve_class(rest[1], b->aux[1], false);
+        hipEventRecord(b->join[1], b->aux[1]);
+        used[1] = true;
     }
-    int q = 0;
-    for (int c = top - 1; c >= 0; c--) {
-        if (!b->live[c]) continue;
-        hipStream_t sc = b->aux[q];
-        if (q > 0) hipStreamWaitEvent(sc, b->rows, 0);
-        solve_class(c, sc, false);
-        hipEventRecord(b->join[q], sc);
-        q++;
+    hipStreamWaitEvent(b->aux[2], b->next, 0);
+    early(b->aux[2]);
+    if (nrest >= 3) {
+        hipStreamWaitEvent(b->aux[2], b->rows, 0);
+        solve_class(rest[2], b->aux[2], false);
     }
-    for (int k = 0; k < q; k++) hipStreamWaitEvent(s, b->join[k], 0);
-    return has_early;
+    hipEventRecord(b->join[2], b->aux[2]);
+    used[2] = true;
+    for (int k = 0; k < 3; k++)
+        if (used[k]) hipStreamWaitEvent(s, b->join[k], 0);
+    return true;
 }
 
 // The warm-start cache rebuild, then joints, positions and sleep; without
@@ -1218,10 +1227,10 @@ static void enqueue_step(omi_batch* b, double dt) {
     mark();
     launch_narrow(d, prec, s); mark();
     launch_compact(d, prec, b->sap_grid, s); mark();
-    SolveIn in{nullptr, nullptr, 0.5, 0.8, 0.005, 1};
+    SolveIn in{nullptr, nullptr, 0.5, 0.8, 0.005, 1, 1};
     // the cache plan and the sleep pass's island union need only the step's
     // contacts: they run beside the solve when a branch is free
-    const bool early = enqueue_solve(b, in, false, [&](hipStream_t ws) { launch_wake(d, ws); mark(); },
+    const bool early = enqueue_solve(b, in, false,
                                      [&](hipStream_t es) {
                                          launch_sleep_union(d, b->nhw, es);
                                          launch_cache_plan(d, es);
@@ -1242,9 +1251,11 @@ static void enqueue_collide(omi_batch* b, double dt) {
     enqueue_sap(b, b->cfg.record_broadphase);
     launch_narrow(d, prec, s);
     launch_compact(d, prec, b->sap_grid, s);
-    SolveIn in{nullptr, nullptr, 0.5, 0.8, 0.005, 1};
-    enqueue_solve(b, in, false, [&](hipStream_t ws) { launch_wake(d, ws); }, [](hipStream_t) {}, [] {});
-    launch_cache(d, prec, s);
+    SolveIn in{nullptr, nullptr, 0.5, 0.8, 0.005, 1, 1};
+    // (the cache plan needs only the step's contacts)
+    const bool early = enqueue_solve(b, in, false, [&](hipStream_t es) { launch_cache_plan(d, es); }, [] {});
+    if (!early) launch_cache_plan(d, s);
+    launch_cache_write(d, prec, s);
     if (b->any_triggers) launch_trigger(d, s);
     launch_toggle(d, s);
 }
@@ -2244,7 +2255,7 @@ extern "C" int32_t omi_prepare_and_solve(int64_t K, int32_t N, const int64_t* a_
         b->slots[0].live = true;
         b->slots[0].n = N;
         if (!(rc = rebuild_lists(b)))
-            enqueue_solve(b, in, true, [](hipStream_t) {}, [](hipStream_t) {}, [] {});
+            enqueue_solve(b, in, true, [](hipStream_t) {}, [] {});
         if (!rc && hipGetLastError() != hipSuccess) rc = fail(OMI_ERR_HIP, "solver launch failed");
         if (!rc) rc = download_soa(b, pos, d.pos, d.ncap, 0, N, 3);
         if (!rc) rc = download_soa(b, lv, d.lv, d.ncap, 0, N, 3);

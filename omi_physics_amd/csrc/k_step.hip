@@ -5,7 +5,7 @@
 //   BroadPhase.pairs + solid filter                     -> k_sap        (1 block / world)
 //   NarrowPhase.generate (box/sphere)                   -> k_narrow     (1 thread / pair)
 //   contact list in pair-then-manifold order            -> k_pair_*     (scan over all pairs)
-//   SequentialImpulseSolver._wake_sleepers              -> k_wake       (1 block / world)
+//   SequentialImpulseSolver._wake_sleepers              -> with the rows (wake_pair / wake_seq)
 //   _solve_native_full + prepare_and_solve              -> k_solve      (1 block / world)
 //   warm-start cache rebuild                            -> k_cache_all  (1 thread / entry)
 //   integrate_positions                                 -> k_post       (1 thread / body)
@@ -984,41 +984,40 @@ __global__ void k_world_slices(Dev d) {
 
 // ------------------------------------------------------------------ wake --
 
-// SequentialImpulseSolver._wake_sleepers (solver.py:321-337).  Waking is
-// idempotent and, unless a sleeping dynamic body carries a wake-grade speed
-// (flag WF_SEQWAKE, set by k_pre), independent of contact order, so the
-// contacts are processed in parallel; otherwise one thread replays the
-// reference's sequential loop.
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_wake(Dev d) {
-    const int32_t w = blockIdx.x;
-    if (w >= d.wcap || !(d.w_flags[w] & WF_ACTIVE)) return;
-    const int32_t K = d.w_cn[w], cs = d.w_cs[w];
-    if (K == 0) return;
+// SequentialImpulseSolver._wake_sleepers (solver.py:321-337), run by the
+// kernels that write a world's solver rows, for the same contacts (it reads
+// the velocities before the solve changes them).  Waking is idempotent and,
+// unless a sleeping dynamic body carries a wake-grade speed (flag WF_SEQWAKE,
+// set by k_pre), independent of contact order, so each contact's thread wakes
+// its pair (wake_pair); otherwise one thread replays the reference's
+// sequential loop over the world's contacts (wake_seq) and wake_pair does
+// nothing.
+__device__ __forceinline__ double wake_speed2(const Dev& d, int64_t i) {
+    const double x = BODY(d.lv, 0, i), y = BODY(d.lv, 1, i), z = BODY(d.lv, 2, i);
+    return (x * x + y * y) + z * z;
+}
+
+__device__ __forceinline__ void wake_pair(const Dev& d, int32_t w, int64_t a, int64_t b) {
+    if ((d.w_flags[w] & (WF_ACTIVE | WF_SEQWAKE)) != WF_ACTIVE) return;
     const double th = 0.15 * 0.15;
-    auto v2 = [&](int64_t i) {
-        double x = BODY(d.lv, 0, i), y = BODY(d.lv, 1, i), z = BODY(d.lv, 2, i);
-        return (x * x + y * y) + z * z;
-    };
-    if (d.w_flags[w] & WF_SEQWAKE) {
-        if (threadIdx.x != 0) return;
-        for (int32_t k = 0; k < K; k++) {
-            const int64_t a = d.ca[cs + k], b = d.cb[cs + k];
-            const int ma = d.mt[a], mb = d.mt[b];
-            bool wa = ma == M_KINEMATIC || (ma == M_DYNAMIC && d.awake[a] && v2(a) > th);
-            if (!d.awake[b] && wa) { d.awake[b] = 1; d.timer[b] = 0.0; }
-            bool wb = mb == M_KINEMATIC || (mb == M_DYNAMIC && d.awake[b] && v2(b) > th);
-            if (!d.awake[a] && wb) { d.awake[a] = 1; d.timer[a] = 0.0; }
-        }
-        return;
-    }
-    for (int32_t k = threadIdx.x; k < K; k += BLOCK) {
+    const int ma = d.mt[a], mb = d.mt[b];
+    const bool wa = ma == M_KINEMATIC || (ma == M_DYNAMIC && d.awake0[a] && wake_speed2(d, a) > th);
+    const bool wb = mb == M_KINEMATIC || (mb == M_DYNAMIC && d.awake0[b] && wake_speed2(d, b) > th);
+    if (!d.awake0[b] && wa) { d.awake[b] = 1; d.timer[b] = 0.0; }
+    if (!d.awake0[a] && wb) { d.awake[a] = 1; d.timer[a] = 0.0; }
+}
+
+__device__ __forceinline__ void wake_seq(const Dev& d, int32_t w) {
+    if ((d.w_flags[w] & (WF_ACTIVE | WF_SEQWAKE)) != (WF_ACTIVE | WF_SEQWAKE)) return;
+    const double th = 0.15 * 0.15;
+    const int32_t K = d.w_cn[w], cs = d.w_cs[w];
+    for (int32_t k = 0; k < K; k++) {
         const int64_t a = d.ca[cs + k], b = d.cb[cs + k];
         const int ma = d.mt[a], mb = d.mt[b];
-        const bool wa = ma == M_KINEMATIC || (ma == M_DYNAMIC && d.awake0[a] && v2(a) > th);
-        const bool wb = mb == M_KINEMATIC || (mb == M_DYNAMIC && d.awake0[b] && v2(b) > th);
-        if (!d.awake0[b] && wa) { d.awake[b] = 1; d.timer[b] = 0.0; }
-        if (!d.awake0[a] && wb) { d.awake[a] = 1; d.timer[a] = 0.0; }
+        const bool wa = ma == M_KINEMATIC || (ma == M_DYNAMIC && d.awake[a] && wake_speed2(d, a) > th);
+        if (!d.awake[b] && wa) { d.awake[b] = 1; d.timer[b] = 0.0; }
+        const bool wb = mb == M_KINEMATIC || (mb == M_DYNAMIC && d.awake[b] && wake_speed2(d, b) > th);
+        if (!d.awake[a] && wb) { d.awake[a] = 1; d.timer[a] = 0.0; }
     }
 }
 
@@ -1497,6 +1496,10 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d, SolveIn in, int32_t skip
         if (skip_above > 0 && d.w_cnt[w] > skip_above) continue;
         const int32_t cs = d.w_cs[w];
         if (ci < cs || ci >= (int64_t)cs + d.w_cn[w]) continue;   // a world that overflowed
+        if (in.wake) {
+            if (ci == cs) wake_seq(d, w);
+            wake_pair(d, w, a, b);
+        }
         prepare_contact<R, STAGE>(d, in, ci, a, b, w, cs, old);
     }
 }
@@ -1512,9 +1515,12 @@ __global__ __launch_bounds__(256) void k_prepare_worlds(Dev d, SolveIn in, const
     if (w >= d.wcap || !(d.w_flags[w] & WF_ACTIVE)) return;
     const int32_t cs = d.w_cs[w], K = d.w_cn[w];
     const int32_t old = *d.sel;
+    if (in.wake && part == 0 && threadIdx.x == 0) wake_seq(d, w);
     for (int32_t k = part * 256 + (int32_t)threadIdx.x; k < K; k += G * 256) {
         const int64_t ci = (int64_t)cs + k;
-        prepare_contact<R, STAGE>(d, in, ci, d.ca[ci], d.cb[ci], w, cs, old);
+        const int64_t a = d.ca[ci], b = d.cb[ci];
+        if (in.wake) wake_pair(d, w, a, b);
+        prepare_contact<R, STAGE>(d, in, ci, a, b, w, cs, old);
     }
 }
 
@@ -2363,9 +2369,12 @@ __device__ __forceinline__ void solve_unit(const Dev& d, const SolveIn& in, int3
         }
     } else if (!ISL && BLOCK > 64 && self_rows) {
         const int32_t old = *d.sel;
+        if (in.wake && tid == 64) wake_seq(d, w);
         for (int32_t k = tid - 64; k < K; k += BLOCK - 64) {
             const int64_t ci = (int64_t)cl + k;
-            prepare_contact<R, false>(d, in, ci, d.ca[ci], d.cb[ci], w, cl, old);
+            const int64_t a = d.ca[ci], b = d.cb[ci];
+            if (in.wake) wake_pair(d, w, a, b);
+            prepare_contact<R, false>(d, in, ci, a, b, w, cl, old);
         }
     }
     __syncthreads();
@@ -3455,10 +3464,6 @@ void launch_compact(const Dev& d, int precision, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_world_slices, dim3(GRID_FOR(d.wcap, 256)), dim3(256), 0, s, d);
 }
 
-void launch_wake(const Dev& d, hipStream_t s) {
-    hipLaunchKernelGGL((k_wake<256>), dim3(d.wcap), dim3(256), 0, s, d);
-}
-
 template <typename R, int B>
 static void solve_one(const Dev& d, SolveIn in, const int32_t* wl, const int32_t* wn, int32_t lds_bytes,
                       int grid, hipStream_t s, int32_t isl) {
@@ -3596,11 +3601,6 @@ void launch_cache_write(const Dev& d, int precision, hipStream_t s) {
     const int g = GRID_FOR(d.kcap, 256);
     if (precision == 64) hipLaunchKernelGGL((k_cache_all<double>), dim3(g), dim3(256), 0, s, d);
     else hipLaunchKernelGGL((k_cache_all<float>), dim3(g), dim3(256), 0, s, d);
-}
-
-void launch_cache(const Dev& d, int precision, hipStream_t s) {
-    launch_cache_plan(d, s);
-    launch_cache_write(d, precision, s);
 }
 
 // One block per world; lds_bytes (a multiple of 16) of dynamic LDS holds

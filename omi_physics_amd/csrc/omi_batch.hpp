@@ -177,6 +177,7 @@ struct SolveIn {
     const double* st_fric;
     double rthr, correction, slop;
     int32_t warm;
+    int32_t wake;            // 1: the kernels writing a world's rows also wake its sleepers
 };
 
 }  // namespace omi
