@@ -67,7 +67,7 @@ def trigger_scene(n=60, seed=4):
     return spec
 
 
-def damped_kinematic_scene(n=90, seed=21, asleep_every=4):
+def damped_kinematic_scene(n=90, seed=21, asleep_every=4, spread=3.0):
     """A pile whose bodies carry every integrate_forces term and which kinematic
     pushers sweep through, with part of it starting asleep.
 
@@ -76,7 +76,8 @@ def damped_kinematic_scene(n=90, seed=21, asleep_every=4):
     constant velocity (one spinning) across the pile; every ``asleep_every``-th
     dynamic body starts asleep, so contacts with an awake mover wake it
     (solver.py:321-337) and the kinematic bodies count as awake movers in the
-    solid-pair filter (world.py:381-384).  Returns ``export_world_arrays`` output.
+    solid-pair filter (world.py:381-384).  The dynamic bodies start within
+    ``spread`` of the centre in x and z.  Returns ``export_world_arrays`` output.
     """
     from omi_physics_amd import PhysicsWorld
     rng = np.random.default_rng(seed)
@@ -99,7 +100,8 @@ def damped_kinematic_scene(n=90, seed=21, asleep_every=4):
                          angularDamping=float(rng.uniform(0.0, 0.8)),
                          quadraticDrag=float(rng.uniform(0.0, 0.3)))
         w.add_body(m, collider=model.Collider(shape=box if k % 2 else ball, physicsMaterial=mat),
-                   position=(float(rng.uniform(-3, 3)), float(rng.uniform(0.6, 5.0)), float(rng.uniform(-3, 3))),
+                   position=(float(rng.uniform(-spread, spread)), float(rng.uniform(0.6, 5.0)),
+                             float(rng.uniform(-spread, spread))),
                    orientation=tuple(q))
     for x0, v, spin in ((-8.0, (2.0, 0.0, 0.0), (0.0, 0.0, 0.0)), (8.0, (-1.5, 0.0, 0.3), (0.0, 0.8, 0.0)),
                         (0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))):

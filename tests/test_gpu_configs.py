@@ -206,6 +206,45 @@ def check_fp32(out):
     assert out['velocity'] <= 2e-4 and out['angular_velocity'] <= 5e-4
 
 
+@pytest.mark.parametrize('island_solve', [False, True])
+def test_wake_with_rows_in_every_size_class_fp64(island_solve):
+    """Sleepers woken by contact (solver.py:321-337) in worlds of three size
+    classes stepped together, so each class's wake rides on the launch that
+    writes its rows: the largest class's solve blocks (or, solved island by
+    island, its own rows launch), the next class's rows launch, and the
+    chip-wide rows launch of the smallest.  In each class one world's sleepers
+    keep their initial speeds (the sequential wake replay) and one's are still
+    (the parallel wake).  100 FP64 steps equal the oracle bit for bit."""
+    from tests import helpers
+    worlds = []
+    for n, spread in ((60, 3.0), (400, 6.0), (1200, 10.0)):
+        for still in (False, True):
+            a = helpers.damped_kinematic_scene(n=n, seed=31 + n + still, spread=spread)
+            if still:
+                asleep = a['awake'] == 0
+                for key in ('linear_velocity', 'angular_velocity'):
+                    v = a[key].copy()
+                    v[asleep] = 0.0
+                    a[key] = v
+            worlds.append(a)
+    say('wake by size class: %s bodies, island_solve=%s' % ([w['n'] for w in worlds], island_solve))
+    b = batch_from_arrays(worlds, precision='fp64', island_solve=island_solve)
+    ows = [oracle.OracleWorld(a) for a in worlds]
+    asleep0 = [a['awake'] == 0 for a in worlds]
+    woke = [0] * len(worlds)
+    for s in range(5):
+        b.step(DT, 20)
+        b.sync()
+        assert b.counters()['overflow'] == 0
+        for k, ow in enumerate(ows):
+            ow.step(DT, 20)
+            assert_state_equal(b, k, ow, 'world %d (%d bodies) step %d' % (k, worlds[k]['n'], 20 * s + 20))
+            woke[k] = max(woke[k], int(b.read(k)['awake'][asleep0[k]].sum()))
+    say('  woken per world: %s of %s' % (woke, [int(x.sum()) for x in asleep0]))
+    assert all(w > 0 for w in woke)
+    b.close()
+
+
 def test_fp32_one_step_pyramids_and_debris():
     arrays = scenes.arrays(scenes.pyramids_and_debris())
     say('fp32 one step, pyramids and debris after 120 fp64 steps')
