@@ -38,6 +38,7 @@ void launch_joints(const Dev& d, int precision, double dt, int32_t lds_bytes, hi
 void launch_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s, double lin, double ang, double tsleep);
 void launch_sleep_union(const Dev& d, int64_t nhw, hipStream_t s);
 void launch_sleep_decide(const Dev& d, int64_t nhw, double dt, hipStream_t s, double lin, double ang, double tsleep);
+void launch_post_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s, double lin, double ang, double tsleep);
 void launch_toggle(const Dev& d, hipStream_t s);
 void launch_copy_segments(const CopySeg* segs, int32_t nseg, hipStream_t s);
 void launch_trigger(const Dev& d, hipStream_t s);
@@ -1206,9 +1207,16 @@ static void enqueue_after_solve(omi_batch* b, double dt, bool early, Mark mark) 
     if (b->any_triggers) launch_trigger(d, s);
     if (b->any_joints) launch_joints(d, prec, dt, b->joint_lds, s);
     mark();
-    launch_post(d, b->nhw, dt, s); mark();
-    if (!early) launch_sleep_union(d, b->nhw, s);
-    launch_sleep_decide(d, b->nhw, dt, s, 0.08, 0.25, 0.4); mark();
+    if (b->timing) {
+        launch_post(d, b->nhw, dt, s); mark();
+        if (!early) launch_sleep_union(d, b->nhw, s);
+        launch_sleep_decide(d, b->nhw, dt, s, 0.08, 0.25, 0.4); mark();
+    } else {
+        // one launch for positions and the speed tests (the stage times
+        // keep them apart)
+        if (!early) launch_sleep_union(d, b->nhw, s);
+        launch_post_sleep(d, b->nhw, dt, s, 0.08, 0.25, 0.4);
+    }
     if (side) hipStreamWaitEvent(s, b->cached, 0);
     launch_toggle(d, s); mark();
 }

@@ -3151,18 +3151,20 @@ __global__ __launch_bounds__(256) void k_copy_segments(const CopySeg* segs) {
 
 // NumpyBackend.integrate_positions (backend.py:42-49) with quat_integrate
 // (mathutil.py:108-113) for awake dynamic and kinematic bodies.
+__device__ __forceinline__ void post_body(const Dev& d, int64_t i, double dt) {
+    if (d.world[i] < 0) return;
+    const int mt = d.mt[i];
+    if (!((mt == M_DYNAMIC || mt == M_KINEMATIC) && d.awake[i])) return;
+    double p[3], q[4], v[3], w[3];
+    for (int r = 0; r < 3; r++) { p[r] = BODY(d.pos, r, i); v[r] = BODY(d.lv, r, i); w[r] = BODY(d.av, r, i); }
+    for (int r = 0; r < 4; r++) q[r] = BODY(d.quat, r, i);
+    positions_body(p, q, v, w, dt);
+    for (int r = 0; r < 3; r++) BODY(d.pos, r, i) = p[r];
+    for (int r = 0; r < 4; r++) BODY(d.quat, r, i) = q[r];
+}
+
 __global__ void k_post(Dev d, int64_t nhw, double dt) {
-    for (int64_t i = gtid(); i < nhw; i += gstride()) {
-        if (d.world[i] < 0) continue;
-        const int mt = d.mt[i];
-        if (!((mt == M_DYNAMIC || mt == M_KINEMATIC) && d.awake[i])) continue;
-        double p[3], q[4], v[3], w[3];
-        for (int r = 0; r < 3; r++) { p[r] = BODY(d.pos, r, i); v[r] = BODY(d.lv, r, i); w[r] = BODY(d.av, r, i); }
-        for (int r = 0; r < 4; r++) q[r] = BODY(d.quat, r, i);
-        positions_body(p, q, v, w, dt);
-        for (int r = 0; r < 3; r++) BODY(d.pos, r, i) = p[r];
-        for (int r = 0; r < 4; r++) BODY(d.quat, r, i) = q[r];
-    }
+    for (int64_t i = gtid(); i < nhw; i += gstride()) post_body(d, i, dt);
 }
 
 // The integrate stages on one world's host-layout rows ([n][3] velocities,
@@ -3264,20 +3266,32 @@ __global__ void k_sleep_union(Dev d) {
 
 // _update_sleep (world.py:510-538), first half: speed tests, timers, and an
 // island is marked not-ready if any member is not ready.
+__device__ __forceinline__ void sleep_body(const Dev& d, int64_t i, double dt, double lin2, double ang2,
+                                           double tsleep) {
+    const int32_t w = d.world[i];
+    if (w < 0 || !(d.w_flags[w] & WF_SLEEP)) return;
+    const bool dyn = d.mt[i] == M_DYNAMIC;
+    const double x = BODY(d.lv, 0, i), y = BODY(d.lv, 1, i), z = BODY(d.lv, 2, i);
+    const double p = BODY(d.av, 0, i), q = BODY(d.av, 1, i), r = BODY(d.av, 2, i);
+    const double s2 = (x * x + y * y) + z * z, w2 = (p * p + q * q) + r * r;
+    const bool slow = dyn && s2 < lin2 && w2 < ang2;
+    double tm = slow ? d.timer[i] + dt : 0.0;
+    d.timer[i] = tm;
+    const bool ready = slow && tm > tsleep;
+    d.awake0[i] = (uint8_t)((slow ? 1 : 0) | (ready ? 2 : 0));   // reuse as per-body flags
+    if (d.contacted[i] && !ready) d.notready[uf_find(d.parent, (int32_t)i)] = 1;
+}
+
 __global__ void k_sleep_body(Dev d, int64_t nhw, double dt, double lin2, double ang2, double tsleep) {
+    for (int64_t i = gtid(); i < nhw; i += gstride()) sleep_body(d, i, dt, lin2, ang2, tsleep);
+}
+
+// Positions, then the sleep pass's speed tests, one thread per body: the
+// positions write only poses, the speed tests read velocities and timers.
+__global__ void k_post_sleep(Dev d, int64_t nhw, double dt, double lin2, double ang2, double tsleep) {
     for (int64_t i = gtid(); i < nhw; i += gstride()) {
-        const int32_t w = d.world[i];
-        if (w < 0 || !(d.w_flags[w] & WF_SLEEP)) continue;
-        const bool dyn = d.mt[i] == M_DYNAMIC;
-        const double x = BODY(d.lv, 0, i), y = BODY(d.lv, 1, i), z = BODY(d.lv, 2, i);
-        const double p = BODY(d.av, 0, i), q = BODY(d.av, 1, i), r = BODY(d.av, 2, i);
-        const double s2 = (x * x + y * y) + z * z, w2 = (p * p + q * q) + r * r;
-        const bool slow = dyn && s2 < lin2 && w2 < ang2;
-        double tm = slow ? d.timer[i] + dt : 0.0;
-        d.timer[i] = tm;
-        const bool ready = slow && tm > tsleep;
-        d.awake0[i] = (uint8_t)((slow ? 1 : 0) | (ready ? 2 : 0));   // reuse as per-body flags
-        if (d.contacted[i] && !ready) d.notready[uf_find(d.parent, (int32_t)i)] = 1;
+        post_body(d, i, dt);
+        sleep_body(d, i, dt, lin2, ang2, tsleep);
     }
 }
 
@@ -3628,6 +3642,14 @@ void launch_sleep_union(const Dev& d, int64_t nhw, hipStream_t s) {
 void launch_sleep_decide(const Dev& d, int64_t nhw, double dt, hipStream_t s, double lin, double ang, double tsleep) {
     if (nhw <= 0) return;
     hipLaunchKernelGGL(k_sleep_body, dim3(GRID_FOR(nhw, 256)), dim3(256), 0, s, d, nhw, dt, lin * lin, ang * ang,
+                       tsleep);
+    hipLaunchKernelGGL(k_sleep_final, dim3(GRID_FOR(nhw, 256)), dim3(256), 0, s, d, nhw);
+}
+
+// Positions and the sleep decisions (after the islands of the step's contacts).
+void launch_post_sleep(const Dev& d, int64_t nhw, double dt, hipStream_t s, double lin, double ang, double tsleep) {
+    if (nhw <= 0) return;
+    hipLaunchKernelGGL(k_post_sleep, dim3(GRID_FOR(nhw, 256)), dim3(256), 0, s, d, nhw, dt, lin * lin, ang * ang,
                        tsleep);
     hipLaunchKernelGGL(k_sleep_final, dim3(GRID_FOR(nhw, 256)), dim3(256), 0, s, d, nhw);
 }
